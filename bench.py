@@ -1,0 +1,196 @@
+"""Benchmark: fused D3Q19 collide+stream MLUPS and %HBM roofline at 256^3, FP64 vs FP32-shifted.
+
+Workload (one "step" = one fused collide+stream pass over the lattice): the porous channel --
+256^3 periodic box, Guo body force g = 1e-6 along x, random solid spheres (radius 4..12, seeded)
+until porosity 0.8, bounce-back on the spheres, tau = 0.6.  Populations are resident in HBM when
+the timed region starts.  MLUPS counts fluid-cell updates.  Algorithmic bytes per fluid update:
+2 * 19 * sizeof(T) (each population read once and written once).
+
+Prints ONE JSON line.  `value` is the FP64 rate; the FP32-shifted rate, the adjoint/forward cost
+and the CPU baseline (the FP64 oracle, OpenMP on the host cores, bounded sample) ride along.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+METRIC = "D3Q19 MLUPS + %HBM roofline at 256^3, FP64 vs FP32-shifted; adjoint/fwd cost"
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "dense_step_traffic.md")
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def read_traffic(key):
+    """HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (committed summary)."""
+    if not os.path.exists(TRAFFIC_FILE):
+        return None
+    for line in open(TRAFFIC_FILE):
+        if line.startswith("|") and key in line:
+            cells = [c.strip() for c in line.strip().strip("|").split("|")]
+            try:
+                return float(cells[-1])
+            except ValueError:
+                return None
+    return None
+
+
+def run_dense(lbm, solid, n, tau, prec, steps, warmup, g):
+    lat = lbm.Lattice(n, n, n, tau, precision=prec, periodic=(True, True, True), force=(g, 0.0, 0.0))
+    lat.set_flags(~solid)
+    lat.init_equilibrium()
+    nfluid = lat.fluid_cells
+    lat.step(warmup)
+    lat.synchronize()
+    t0 = time.perf_counter()
+    ms = lat.step_timed(steps)  # HIP events on the context stream around exactly `steps` launches
+    lat.synchronize()
+    wall = time.perf_counter() - t0
+    m = lat.moments()
+    lat.close()
+    esize = 8 if prec == lbm.FP64 else 4
+    per_launch_s = ms * 1e-3 / steps
+    bytes_per_launch = nfluid * 2 * 19 * esize
+    achieved = bytes_per_launch / per_launch_s / 1e9
+    key = f"{'fp64' if prec == lbm.FP64 else 'fp32s'} {n}^3"
+    traffic = read_traffic(key)
+    return {
+        "mlups": nfluid * steps / (ms * 1e-3) / 1e6,
+        "wall_mlups": nfluid * steps / wall / 1e6,
+        "ms_per_step": ms / steps,
+        "wall_s": wall,
+        "fluid_cells": nfluid,
+        "mass": m[0],
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "algorithmic_bytes_per_launch": bytes_per_launch},
+    }
+
+
+def run_adjoint(lbm, wl, n=128, steps=500, snapshots=23):
+    solid = wl.channel_walls(n, n, n)
+    alpha = wl.fourier_porosity(n, n, n).reshape(-1)
+    out = {}
+    lat = lbm.Lattice(n, n, n, 0.6, precision=lbm.FP32_SHIFTED, periodic=(True, False, False), force=(1e-5, 0, 0))
+    lat.set_flags(~solid)
+    lat.set_porosity(alpha)
+    lat.init_equilibrium()
+    lat.step(10)
+    lat.synchronize()
+    fwd_ms = lat.step_timed(steps) / steps
+    lat.init_equilibrium()
+    J, grad, st = lat.adjoint_pressure_drop(steps, snapshots, n // 4, 3 * n // 4)
+    lat.close()
+    rev_total = st.reverse_ms
+    adj_step_ms = (rev_total - st.recompute_steps * fwd_ms) / steps
+    out.update({
+        "workload": f"{n}^3 duct, Brinkman porosity (8 Fourier modes), FP32-shifted, {steps} steps, "
+                    f"revolve {snapshots} snapshots, J = FP64 pressure drop",
+        "J": J, "forward_ms_per_step": fwd_ms, "adjoint_ms_per_step": adj_step_ms,
+        "adjoint_over_forward_step": adj_step_ms / fwd_ms,
+        "recompute_steps": int(st.recompute_steps),
+        "gradient_over_forward_run": (st.forward_ms + st.reverse_ms) / (fwd_ms * steps),
+        "grad_norm": float(np.linalg.norm(grad)),
+    })
+    return out
+
+
+def cpu_baseline(wl, n=128, seed=None, tau=0.6, g=1e-6, budget_s=12.0):
+    import oracle
+
+    cores = len(os.sched_getaffinity(0))
+    solid = wl.porous_spheres(n) if seed is None else wl.porous_spheres(n, seed=seed)
+    lat = oracle.Lattice(n, n, n, periodic=(True, True, True))
+    lat.set_solid(solid)
+    lat.set_force((g, 0.0, 0.0))
+    lat.params.tau = tau
+    f = lat.equilibrium()
+    nfluid = int(lat.fluid.sum())
+    t0 = time.perf_counter()
+    f = lat.step(f, 2, nthreads=cores)
+    t1 = time.perf_counter()
+    per = (t1 - t0) / 2
+    steps = max(2, int(budget_s / max(per, 1e-6)))
+    t0 = time.perf_counter()
+    f = lat.step(f, steps, nthreads=cores)
+    dt = time.perf_counter() - t0
+    model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"value": round(nfluid * steps / dt / 1e6, 2), "unit": "MLUPS", "cores": cores, "kind": "port",
+            "sample": f"FP64 oracle (C, OpenMP) on the porous channel at {n}^3 (porosity 0.8, same generator), "
+                      f"{steps} steps in {dt:.1f} s; CPU: {model}"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=300)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--size", type=int, default=256)
+    ap.add_argument("--tau", type=float, default=0.6)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path (the domain would shard "
+              "into slabs with xGMI halo exchange; see the sharding notes).  Run with --gpus 1.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    from madpac_amd import lbm, workloads as wl
+
+    n = args.size
+    t0 = time.perf_counter()
+    solid = wl.porous_spheres(n)
+    log(f"[bench] porous geometry {n}^3: solid fraction {solid.mean():.4f} ({time.perf_counter() - t0:.1f} s)")
+    res = {}
+    for name, prec in (("fp64", lbm.FP64), ("fp32s", lbm.FP32_SHIFTED)):
+        res[name] = run_dense(lbm, solid, n, args.tau, prec, args.steps, args.warmup, 1e-6)
+        log(f"[bench] {name}: {res[name]['mlups']:.0f} MLUPS, {res[name]['ms_per_step']:.3f} ms/step, "
+            f"{res[name]['roofline']['achieved']:.0f} GB/s ({100 * res[name]['roofline']['frac']:.1f} % of peak)")
+    adj = None
+    if not args.no_adjoint:
+        adj = run_adjoint(lbm, wl)
+        log(f"[bench] adjoint: {adj['adjoint_over_forward_step']:.2f}x forward step, "
+            f"gradient {adj['gradient_over_forward_run']:.2f}x forward run")
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(wl)
+        log(f"[bench] cpu baseline: {cpu['value']} MLUPS on {cpu['cores']} cores")
+    r64, r32 = res["fp64"], res["fp32s"]
+    line = {
+        "metric": METRIC, "value": round(r64["mlups"], 1), "unit": "MLUPS", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(r64["ms_per_step"], 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"porous channel {n}^3 periodic, Guo g=1e-6 along x, random spheres r=4..12 "
+                               f"(seed {wl.DEFAULT_SEED}) to porosity 0.8, tau={args.tau}, pull-form fused "
+                               f"collide+stream", "cells": n ** 3, "fluid_cells": r64["fluid_cells"],
+                   "mlups_counts": "fluid cells"},
+        "roofline": r64["roofline"],
+        "fp32_shifted": {"value": round(r32["mlups"], 1), "unit": "MLUPS", "dtype": "f32",
+                         "ms_per_step": round(r32["ms_per_step"], 4), "roofline": r32["roofline"]},
+        "wall_mlups": {"fp64": round(r64["wall_mlups"], 1), "fp32s": round(r32["wall_mlups"], 1)},
+        "adjoint": adj,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

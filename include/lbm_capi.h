@@ -1,0 +1,156 @@
+/*
+ * lbm_capi.h -- C ABI of the MI355X-native D3Q19 block-LB solver.
+ *
+ * Drop-in boundary for the Peano block lattice-Boltzmann hot path of MaDPaC.  Every
+ * entry point is plain C (extern "C", plain pointers and sizes, int status) so that the
+ * reference's C++ operator API (madpac_amd/csrc/peano_blocklb.h) and any FFI (ctypes,
+ * cgo, JNI) can bind it.  Citations are relative to
+ * madpac/peano/applications/latticeboltzmann/ (LB/) and LB/blocklatticeboltzmann/ (BLB/).
+ *
+ * Conventions
+ *   - D3Q19 ordering and weights as LB/LatticeBoltzmannDefinitions.cpp:113-151
+ *     (z-major, rest population q = 9, inverse direction 18 - q).
+ *   - Cell index i = x + nx*(y + ny*z); populations are SoA f[q*ncells + i].
+ *   - Populations exchanged through this ABI are always FP64 host arrays holding the
+ *     reference's state (post-streaming distributions, as GridManagementService holds them
+ *     between time steps); FP32-shifted contexts convert on the device.
+ *   - Return value 0 = success, negative = error; lbm_last_error() gives the text.
+ *     A context is thread-compatible (one host thread at a time), not thread-safe.
+ *   - All device work of a context is ordered on one HIP stream owned by the context.
+ */
+#ifndef LBM_CAPI_H
+#define LBM_CAPI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LBM_Q 19
+
+enum lbm_status {
+  LBM_OK = 0,
+  LBM_ERR_ARG = -1,      /* invalid argument (null pointer, size, range) */
+  LBM_ERR_STATE = -2,    /* call not valid in the current state (e.g. no flags set) */
+  LBM_ERR_HIP = -3,      /* a HIP runtime call failed */
+  LBM_ERR_NOMEM = -4,    /* device allocation failed */
+  LBM_ERR_NUMERIC = -5   /* non-finite value detected by a reduction guard */
+};
+
+enum lbm_precision {
+  LBM_FP64 = 0,          /* FP64 populations, FP64 collision (reference arithmetic order) */
+  LBM_FP32_SHIFTED = 1   /* FP32 populations stored as f - w_q, FP32 collision */
+};
+
+typedef struct lbm_ctx lbm_ctx;
+
+typedef struct {
+  int nx, ny, nz;        /* lattice extent in cells */
+  int periodic[3];       /* 1 = periodic along x / y / z */
+  int precision;         /* enum lbm_precision */
+  double tau;            /* configured relaxation time; tau_eff follows LB/MultiLevelSimData.h:160 */
+  double force[3];       /* body acceleration for Guo forcing; all zero disables it */
+  int device;            /* HIP device ordinal */
+} lbm_config;
+
+/* ---- lifetime ---------------------------------------------------------------------- */
+/* Replaces the regular-grid runner's set-up (BLB/runners/...ForRegularGrid.cpp:49-121) and
+ * GridManagementService::init (BLB/services/GridManagementService.cpp:26-54). */
+int lbm_create(lbm_ctx** ctx, const lbm_config* cfg);
+int lbm_destroy(lbm_ctx* ctx);
+const char* lbm_last_error(const lbm_ctx* ctx);
+/* cells with the fluid ("inner") flag */
+int64_t lbm_fluid_cells(const lbm_ctx* ctx);
+
+/* ---- boundary flags ------------------------------------------------------------------ */
+/* inner[i] != 0 marks a fluid cell: the reference's per-block `inner` bitset
+ * (GridManagementService.cpp:91-111).  Links to non-inner cells are halfway bounce-back
+ * (BLB/boundaryconditions/BlockHalfwayBounceBack.cpp:20-22, implicit in the swap streaming
+ * of BLB/nonlocaltreatment/AABlockStreaming.cpp:19-53). */
+int lbm_set_flags(lbm_ctx* ctx, const uint8_t* inner);
+/* Moving-wall class k (1..7): wall velocity uw and the mask of outgoing directions q whose
+ * bounce-back links are moving walls (BLB/boundaryconditions/BlockMovingWall.cpp:13-33).
+ * cls[i] selects the class of each cell (0 = none); NULL clears all classes. */
+int lbm_set_moving_wall_class(lbm_ctx* ctx, int k, const double uw[3], uint32_t dir_mask);
+int lbm_set_wall_classes(lbm_ctx* ctx, const uint8_t* cls);
+/* The reference's lid-driven-cavity classification (SLB/BoundaryData4LidDrivenCavity.cpp:17-55
+ * with SLB/services/ReceiveBoundaryDataService.cpp:107-185): links of the top fluid layer
+ * with c_z > 0 that leave the fluid are moving walls with u_w = (ulid, 0, 0). */
+int lbm_set_cavity_lid(lbm_ctx* ctx, double ulid);
+/* Per-cell Brinkman porosity alpha in [0,1] (NULL disables).  Equilibrium velocity is
+ * scaled by (1 - alpha).  No reference counterpart (design variable of the adjoint). */
+int lbm_set_porosity(lbm_ctx* ctx, const double* alpha);
+
+/* ---- state ----------------------------------------------------------------------------- */
+/* f = feq(rho, u) in fluid cells, 0 elsewhere; NULL rho means 1, NULL u means 0
+ * (GridManagementService::createVertex, BLB/services/GridManagementService.cpp:121-148). */
+int lbm_init_equilibrium(lbm_ctx* ctx, const double* rho, const double* u);
+int lbm_set_populations(lbm_ctx* ctx, const double* f);
+int lbm_get_populations(lbm_ctx* ctx, double* f);
+
+/* ---- time stepping ---------------------------------------------------------------------- */
+/* nsteps fused collide+stream steps: phases A, B and C of one reference iteration
+ * (BLB/mappings/RegularGrid2RegularBlockSolver.cpp:83-248) per step. */
+int lbm_step(lbm_ctx* ctx, int nsteps);
+/* Same, timed with HIP events on the context stream; *ms = elapsed device time. */
+int lbm_step_timed(lbm_ctx* ctx, int nsteps, float* ms);
+int lbm_synchronize(lbm_ctx* ctx);
+
+/* ---- observables ------------------------------------------------------------------------- */
+/* rho[i] and u[3*i+d] per cell (BLB/BlockComputeCellValues.cpp:26-68); 0 on non-fluid cells. */
+int lbm_get_fields(lbm_ctx* ctx, double* rho, double* u);
+/* FP64 global mass and momentum over fluid cells: out = {sum f, sum f c_x, sum f c_y, sum f c_z}.
+ * Deterministic two-pass reduction; returns LBM_ERR_NUMERIC if a non-finite value is found. */
+int lbm_reduce_moments(lbm_ctx* ctx, double out[4]);
+/* J = cs^2 (<rho>_{x=xin} - <rho>_{x=xout}) over fluid cells of two planes, FP64. */
+int lbm_pressure_drop(lbm_ctx* ctx, int xin, int xout, double* J);
+
+/* ---- adjoint ---------------------------------------------------------------------------- */
+typedef struct {
+  int snapshots_used;      /* revolve checkpoints held (besides the initial state) */
+  int64_t forward_steps;   /* steps of the first forward sweep (= nsteps) */
+  int64_t recompute_steps; /* forward steps re-run from checkpoints during the reverse sweep */
+  int64_t reverse_steps;   /* adjoint steps run (= nsteps) */
+  float forward_ms;        /* device time of the first forward sweep */
+  float reverse_ms;        /* device time of the reverse sweep, recomputation included */
+} lbm_adjoint_stats;
+/* Runs nsteps forward steps from the current state, evaluates J = pressure drop (FP64) and
+ * returns dJ/dalpha for every cell (0 on non-fluid cells), by a hand-written reverse sweep of
+ * the collide+stream kernel with revolve (binomial) checkpointing on `snapshots` device
+ * checkpoints.  The state afterwards equals the state after nsteps forward steps. */
+int lbm_adjoint_pressure_drop(lbm_ctx* ctx, int nsteps, int snapshots, int xin, int xout,
+                              double* J, double* grad_alpha, lbm_adjoint_stats* stats);
+
+/* ---- batched uneven patches -------------------------------------------------------------- */
+typedef struct lbm_patchset lbm_patchset;
+typedef struct {
+  int n;                   /* cubic patch edge in cells (each patch holds (n+2)^3 slots with ghosts) */
+  double tau;              /* per-patch relaxation time */
+  int nbr[LBM_Q];          /* neighbour patch in direction c_q (q != 9); must share edge n */
+} lbm_patch;
+typedef struct {
+  int tile_x, tile_y, tile_z;  /* tile extents in cells (one workgroup per tile) */
+  int blocks_per_cu;           /* persistent grid = CUs * blocks_per_cu */
+} lbm_tiling;
+int lbm_patches_create(lbm_patchset** ps, const lbm_patch* table, int npatches, int precision,
+                       int device);
+int lbm_patches_destroy(lbm_patchset* ps);
+const char* lbm_patches_last_error(const lbm_patchset* ps);
+int64_t lbm_patches_cells(const lbm_patchset* ps);
+/* initial state: f = feq(rho, u) with per-cell rho/u, interior cells of all patches in
+ * patch order, x fastest (sum n^3 entries; u has 3 per cell); NULL = rest. */
+int lbm_patches_init(lbm_patchset* ps, const double* rho, const double* u);
+int lbm_patches_get_populations(lbm_patchset* ps, double* f);   /* 19 * sum n^3, per patch SoA */
+/* Pick a tiling by timing candidates on the device (the autotuner); *chosen may be NULL. */
+int lbm_patches_autotune(lbm_patchset* ps, lbm_tiling* chosen);
+int lbm_patches_set_tiling(lbm_patchset* ps, const lbm_tiling* t);
+/* nsteps steps; each step is one persistent launch.  With use_graph != 0 the steps are
+ * captured once into a hipGraph and replayed. */
+int lbm_patches_step(lbm_patchset* ps, int nsteps, int use_graph, float* ms);
+int lbm_patches_reduce_moments(lbm_patchset* ps, double out[4]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,428 @@
+// Adjoint (reverse-mode) collide+stream and revolve checkpointing for dJ/d(porosity).
+//
+// No reference counterpart: the reference has no adjoint.  The forward step being reversed is
+// k_dense_step (pull form): f_k = P(s_k) gathers the reference state from the stored state,
+// s_{k+1} = C(f_k) collides per cell (BGK, Guo forcing, Brinkman scaling of the equilibrium
+// velocity by (1 - alpha), moving-wall term).  The reverse sweep keeps lambda_k = dJ/df_k (the
+// adjoint of the reference state) and runs, per cell,
+//     mu_q      = dJ/ds_{k+1}(q, x) = lambda_{k+1,q}(x + c_q)  (or lambda_{k+1,18-q}(x) when
+//                 x + c_q is not fluid)                       -- the transpose of the pull, a gather
+//     lambda_k  = (dC/df)^T mu ,   dJ/dalpha(x) += (dC/dalpha)^T mu
+// so one adjoint step reads the stored state (19 gathers, to rebuild rho and u), lambda_{k+1}
+// (19 gathers) and writes lambda_k (19 aligned stores): 57 values per cell, plus alpha and the
+// gradient.  The per-cell algebra is the same as the oracle's CPU tape (lbo_step_adjoint).
+//
+// Checkpointing: the states s_k needed in reverse order are recomputed from checkpoints placed by
+// an optimal binomial ("revolve") schedule, found by dynamic programming over
+//     F(l, c) = min_m  m + F(l - m, c - 1) + F(m, c),   F(1, c) = 0,   F(l, 0) = l(l-1)/2,
+// the fewest forward steps that reverse l steps with c free checkpoint slots (Griewank & Walther,
+// ACM TOMS 26(1), 2000).
+#include <algorithm>
+#include <climits>
+#include <cstdint>
+#include <cmath>
+#include <cstring>
+#include <functional>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "lbm_ctx.h"
+#include "lbm_geom.h"
+
+namespace lbm {
+
+struct AdjointState {
+  std::vector<void*> slots;  // checkpoint slots (stored state), slot 0 = initial state
+  void* final_state = nullptr;
+  void* lam[2] = {nullptr, nullptr};
+  double* galpha = nullptr;
+  int64_t N = 0;
+  size_t esize = 0;
+};
+
+// ---------------------------------------------------------------------------------------------
+// dJ/df of J = cs^2 (<rho>_xin - <rho>_xout): uniform per population on the two planes.
+template <typename L>
+__global__ __launch_bounds__(BX * BY) void k_adj_terminal(DenseGeom g, const uint32_t* __restrict__ words, int xin,
+                                                          int xout, double vin, double vout, L* __restrict__ lam) {
+  const CellPos c = map_cell(g);
+  if (!c.valid) return;
+  const bool fl = words[c.i] & FLUID;
+  double v = 0.0;
+  if (fl && c.x == xin) v += vin;
+  if (fl && c.x == xout) v += vout;
+#pragma unroll
+  for (int q = 0; q < Q; q++) lam[(int64_t)q * g.N + c.i] = (L)v;
+}
+
+// The adjoint step.  T: stored-state type; L: adjoint type (double or float); gradient in double.
+template <typename T, bool FORCE, bool ALPHA, bool WALLS>
+__global__ __launch_bounds__(BX * BY) void k_adj_step(DenseGeom g, const T* __restrict__ s,
+                                                      const T* __restrict__ lam_out, T* __restrict__ lam_in,
+                                                      const uint32_t* __restrict__ words, const T* __restrict__ alpha,
+                                                      double* __restrict__ galpha, CollideParams p, WallTable wt) {
+  const CellPos c = map_cell(g);
+  if (!c.valid) return;
+  const uint32_t w = words[c.i];
+  const int64_t N = g.N;
+  if (!(w & FLUID)) {
+#pragma unroll
+    for (int q = 0; q < Q; q++) lam_in[(int64_t)q * N + c.i] = T(0);
+    return;
+  }
+  const NbrOff o = nbr_offsets(g, c);
+  T f[Q], mu[Q];
+#pragma unroll
+  for (int q = 0; q < Q; q++) {
+    f[q] = s[pull_index(q, w, c.i, N, o)];
+    const bool bounce = (w >> q) & 1u;  // x + c_q is not fluid
+    mu[q] = bounce ? lam_out[(int64_t)(Q - 1 - q) * N + c.i] : lam_out[(int64_t)q * N + c.i + dst_off(q, o)];
+  }
+  constexpr bool D = sizeof(T) == 8;
+  using R = T;  // arithmetic type
+  const R one = R(1), half = R(0.5), three = R(3), nine = R(9), four5 = R(4.5), one5 = R(1.5);
+  R drho_wall = R(0);
+  if (WALLS) {
+    const uint32_t k = (w >> CLASS_SHIFT) & (MAX_CLASSES - 1);
+    if (k) {
+      const uint32_t m = w & wt.mask[k];
+#pragma unroll
+      for (int q = 0; q < Q; q++)
+        if ((m >> q) & 1u) drho_wall -= mu[q] * (R)moving_wall_f64(q, wt.uw[k], 1.0);
+    }
+  }
+  // rho, j (the FP32 state holds f - w; the rest part adds to rho only)
+  R rho = R(0), j0 = R(0), j1 = R(0), j2 = R(0);
+#pragma unroll
+  for (int q = 0; q < Q; q++) {
+    rho += f[q];
+    j0 += (R)CX(q) * f[q];
+    j1 += (R)CY(q) * f[q];
+    j2 += (R)CZ(q) * f[q];
+  }
+  if (!D) rho += one;
+  const R om = (R)p.omega;
+  const R g0 = FORCE ? (R)p.g[0] : R(0), g1 = FORCE ? (R)p.g[1] : R(0), g2 = FORCE ? (R)p.g[2] : R(0);
+  const R a = ALPHA ? alpha[c.i] : R(0);
+  const R sc = ALPHA ? (one - a) : one;
+  const R u0 = j0 / rho, u1 = j1 / rho, u2 = j2 / rho;
+  const R v0 = sc * (u0 + half * g0), v1 = sc * (u1 + half * g1), v2 = sc * (u2 + half * g2);
+  const R vv = v0 * v0 + v1 * v1 + v2 * v2;
+  const R fpref = FORCE ? (one - half * om) : R(0);
+  R dPr = R(0), dv0 = R(0), dv1 = R(0), dv2 = R(0);
+#pragma unroll
+  for (int q = 0; q < Q; q++) {
+    const R cx = (R)CX(q), cy = (R)CY(q), cz = (R)CZ(q);
+    const R cv = cx * v0 + cy * v1 + cz * v2;
+    const R cg = cx * g0 + cy * g1 + cz * g2;
+    const R vg = v0 * g0 + v1 * g1 + v2 * g2;
+    const R wq = (R)W64(q);
+    const R feq_over_rho = wq * (one + three * cv + four5 * cv * cv - one5 * vv);
+    const R S_over_rho = fpref * wq * (three * (cg - vg) + nine * cv * cg);
+    dPr += mu[q] * (om * feq_over_rho + S_over_rho);
+    const R d0 = wq * rho * (three * cx + nine * cv * cx - three * v0);
+    const R d1 = wq * rho * (three * cy + nine * cv * cy - three * v1);
+    const R d2 = wq * rho * (three * cz + nine * cv * cz - three * v2);
+    const R e0 = fpref * wq * rho * (-three * g0 + nine * cx * cg);
+    const R e1 = fpref * wq * rho * (-three * g1 + nine * cy * cg);
+    const R e2 = fpref * wq * rho * (-three * g2 + nine * cz * cg);
+    dv0 += mu[q] * (om * d0 + e0);
+    dv1 += mu[q] * (om * d1 + e1);
+    dv2 += mu[q] * (om * d2 + e2);
+  }
+#pragma unroll
+  for (int q = 0; q < Q; q++) {
+    R l = (one - om) * mu[q] + dPr + drho_wall;
+    l += dv0 * sc * ((R)CX(q) - u0) / rho;
+    l += dv1 * sc * ((R)CY(q) - u1) / rho;
+    l += dv2 * sc * ((R)CZ(q) - u2) / rho;
+    lam_in[(int64_t)q * N + c.i] = l;
+  }
+  if (ALPHA) {
+    R ga = R(0);
+    ga -= dv0 * (u0 + half * g0);
+    ga -= dv1 * (u1 + half * g1);
+    ga -= dv2 * (u2 + half * g2);
+    galpha[c.i] += (double)ga;
+  }
+}
+
+template <typename T>
+static hipError_t adj_step_t(const DenseGeom& g, const void* s, const void* lo, void* li, const uint32_t* words,
+                             const void* alpha, double* galpha, bool force, bool walls, const CollideParams& p,
+                             const WallTable& wt, hipStream_t st) {
+  const dim3 grid((unsigned)g.nblocks_padded), block(BX, BY);
+  const T* sp = (const T*)s;
+  const T* lop = (const T*)lo;
+  T* lip = (T*)li;
+  const T* ap = (const T*)alpha;
+#define LBM_ADJ(F, A, W) \
+  hipLaunchKernelGGL((k_adj_step<T, F, A, W>), grid, block, 0, st, g, sp, lop, lip, words, ap, galpha, p, wt)
+  const int key = (force ? 4 : 0) | (alpha ? 2 : 0) | (walls ? 1 : 0);
+  switch (key) {
+    case 0: LBM_ADJ(false, false, false); break;
+    case 1: LBM_ADJ(false, false, true); break;
+    case 2: LBM_ADJ(false, true, false); break;
+    case 3: LBM_ADJ(false, true, true); break;
+    case 4: LBM_ADJ(true, false, false); break;
+    case 5: LBM_ADJ(true, false, true); break;
+    case 6: LBM_ADJ(true, true, false); break;
+    default: LBM_ADJ(true, true, true); break;
+  }
+#undef LBM_ADJ
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------------------------
+// Optimal binomial checkpoint schedule (dynamic programming, memoised).
+struct Revolve {
+  int cmax;
+  std::vector<std::vector<int64_t>> F;  // F[l][c]
+  std::vector<std::vector<int>> M;      // best first split
+  Revolve(int lmax, int c) : cmax(c), F(lmax + 1, std::vector<int64_t>(c + 1, 0)), M(lmax + 1, std::vector<int>(c + 1, 0)) {
+    for (int l = 1; l <= lmax; l++) {
+      F[l][0] = (int64_t)l * (l - 1) / 2;
+      for (int cc = 1; cc <= c; cc++) {
+        if (l == 1) {
+          F[l][cc] = 0;
+          continue;
+        }
+        int64_t best = INT64_MAX;
+        int bm = 1;
+        for (int m = 1; m < l; m++) {
+          const int64_t v = m + F[l - m][cc - 1] + F[m][cc];
+          if (v < best) {
+            best = v;
+            bm = m;
+          }
+        }
+        F[l][cc] = best;
+        M[l][cc] = bm;
+      }
+    }
+  }
+};
+
+}  // namespace lbm
+
+using namespace lbm;
+
+void lbm_adjoint_release(lbm_ctx* c) {
+  if (!c || !c->adj) return;
+  AdjointState* a = c->adj;
+  for (void* p : a->slots)
+    if (p) (void)hipFree(p);
+  if (a->final_state) (void)hipFree(a->final_state);
+  if (a->lam[0]) (void)hipFree(a->lam[0]);
+  if (a->lam[1]) (void)hipFree(a->lam[1]);
+  if (a->galpha) (void)hipFree(a->galpha);
+  delete a;
+  c->adj = nullptr;
+}
+
+namespace {
+int adj_fail(lbm_ctx* c, int code, const std::string& m) {
+  c->err = m;
+  return code;
+}
+}  // namespace
+
+extern "C" int lbm_adjoint_pressure_drop(lbm_ctx* c, int nsteps, int snapshots, int xin, int xout, double* J,
+                                         double* grad_alpha, lbm_adjoint_stats* stats) {
+  if (!c) return LBM_ERR_ARG;
+  if (!J || !grad_alpha) return adj_fail(c, LBM_ERR_ARG, "output is null");
+  // the schedule search is O(nsteps^2 * snapshots) on the host
+  if (nsteps < 1 || nsteps > 2048) return adj_fail(c, LBM_ERR_ARG, "nsteps must be in 1..2048");
+  if (snapshots < 0 || snapshots > 64) return adj_fail(c, LBM_ERR_ARG, "snapshots must be in 0..64");
+  if (xin < 0 || xin >= c->g.nx || xout < 0 || xout >= c->g.nx) return adj_fail(c, LBM_ERR_ARG, "plane out of range");
+  if (!c->flags_set || !c->state_set) return adj_fail(c, LBM_ERR_STATE, "flags and state must be set first");
+  if (hipSetDevice(c->cfg.device) != hipSuccess) return adj_fail(c, LBM_ERR_HIP, "hipSetDevice failed");
+
+  const int64_t N = c->g.N;
+  const size_t sbytes = (size_t)Q * N * c->esize;
+  const int nslots = std::min(snapshots, nsteps) + 1;
+  // (re)allocate workspace
+  if (c->adj && ((int)c->adj->slots.size() != nslots || c->adj->N != N || c->adj->esize != c->esize))
+    lbm_adjoint_release(c);
+  if (!c->adj) {
+    AdjointState* a = new AdjointState();
+    a->N = N;
+    a->esize = c->esize;
+    c->adj = a;
+    a->slots.assign(nslots, nullptr);
+    bool ok = true;
+    for (int k = 0; k < nslots && ok; k++) ok = hipMalloc(&a->slots[k], sbytes) == hipSuccess;
+    ok = ok && hipMalloc(&a->final_state, sbytes) == hipSuccess;
+    ok = ok && hipMalloc(&a->lam[0], sbytes) == hipSuccess && hipMalloc(&a->lam[1], sbytes) == hipSuccess;
+    ok = ok && hipMalloc(&a->galpha, (size_t)N * sizeof(double)) == hipSuccess;
+    if (!ok) {
+      lbm_adjoint_release(c);
+      return adj_fail(c, LBM_ERR_NOMEM, "adjoint workspace allocation failed");
+    }
+  }
+  AdjointState* a = c->adj;
+  hipStream_t st = c->stream;
+#define ACHK(call)                                                                                 \
+  do {                                                                                             \
+    hipError_t e_ = (call);                                                                        \
+    if (e_ != hipSuccess) return adj_fail(c, LBM_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+  Revolve rv(nsteps, nslots - 1);
+  // First forward sweep from s_0, storing the checkpoints of the schedule's first descent.
+  std::map<int, int> held;  // time step -> slot holding s_t
+  std::vector<bool> busy(nslots, false);
+  ACHK(hipMemcpyAsync(a->slots[0], c->buf[c->cur], sbytes, hipMemcpyDeviceToDevice, st));
+  held[0] = 0;
+  busy[0] = true;
+  {
+    int t = 0, l = nsteps, cc = nslots - 1;
+    std::vector<int> descent;
+    while (l > 1 && cc > 0) {
+      const int m = rv.M[l][cc];
+      t += m;
+      l -= m;
+      cc--;
+      descent.push_back(t);
+    }
+    ACHK(hipEventRecord(c->ev0, st));
+    int now = 0;
+    for (int pos : descent) {
+      int rc = lbm_enqueue_steps(c, pos - now);
+      if (rc) return rc;
+      now = pos;
+      int slot = -1;
+      for (int k = 1; k < nslots; k++)
+        if (!busy[k]) {
+          slot = k;
+          break;
+        }
+      ACHK(hipMemcpyAsync(a->slots[slot], c->buf[c->cur], sbytes, hipMemcpyDeviceToDevice, st));
+      busy[slot] = true;
+      held[pos] = slot;
+    }
+    int rc = lbm_enqueue_steps(c, nsteps - now);
+    if (rc) return rc;
+    ACHK(hipEventRecord(c->ev1, st));
+    ACHK(hipMemcpyAsync(a->final_state, c->buf[c->cur], sbytes, hipMemcpyDeviceToDevice, st));
+  }
+  float fwd_ms = 0.f;
+  ACHK(hipEventSynchronize(c->ev1));
+  ACHK(hipEventElapsedTime(&fwd_ms, c->ev0, c->ev1));
+
+  // objective and its gradient w.r.t. the final reference state
+  double Jv = 0.0;
+  {
+    int rc = lbm_pressure_drop(c, xin, xout, &Jv);
+    if (rc) return rc;
+  }
+  double hsum[4];
+  ACHK(launch_dense_plane_rho(c->g, c->fp32, c->buf[c->cur], c->words, xin, xout, c->partials, c->small, st));
+  ACHK(hipMemcpyAsync(hsum, c->small, sizeof(hsum), hipMemcpyDeviceToHost, st));
+  ACHK(hipStreamSynchronize(st));
+  const double vin = (1.0 / 3.0) / hsum[1], vout = -(1.0 / 3.0) / hsum[3];
+  int lc = 0;
+  const dim3 grid((unsigned)c->g.nblocks_padded), block(BX, BY);
+  if (c->fp32)
+    hipLaunchKernelGGL(k_adj_terminal<float>, grid, block, 0, st, c->g, c->words, xin, xout, vin, vout,
+                       (float*)a->lam[lc]);
+  else
+    hipLaunchKernelGGL(k_adj_terminal<double>, grid, block, 0, st, c->g, c->words, xin, xout, vin, vout,
+                       (double*)a->lam[lc]);
+  ACHK(hipGetLastError());
+  ACHK(hipMemsetAsync(a->galpha, 0, (size_t)N * sizeof(double), st));
+
+  // reverse sweep
+  int64_t recompute = 0, reversed = 0;
+  int rc_err = 0;
+  ACHK(hipEventRecord(c->ev0, st));
+  auto load = [&](int t) -> int {
+    auto it = held.find(t);
+    if (it == held.end()) return adj_fail(c, LBM_ERR_STATE, "checkpoint schedule lost a state");
+    hipError_t e = hipMemcpyAsync(c->buf[c->cur], a->slots[it->second], sbytes, hipMemcpyDeviceToDevice, st);
+    return e == hipSuccess ? 0 : adj_fail(c, LBM_ERR_HIP, "checkpoint restore failed");
+  };
+  auto adj_step = [&]() -> int {
+    hipError_t e = c->fp32 ? adj_step_t<float>(c->g, c->buf[c->cur], a->lam[lc], a->lam[lc ^ 1], c->words, c->alpha,
+                                               a->galpha, c->force, c->walls, c->p64, c->wt, st)
+                           : adj_step_t<double>(c->g, c->buf[c->cur], a->lam[lc], a->lam[lc ^ 1], c->words, c->alpha,
+                                                a->galpha, c->force, c->walls, c->p64, c->wt, st);
+    if (e != hipSuccess) return adj_fail(c, LBM_ERR_HIP, std::string("adjoint step: ") + hipGetErrorString(e));
+    lc ^= 1;
+    reversed++;
+    return 0;
+  };
+  // reverse steps t0 .. t0+l-1 with s_t0 held and cc free slots
+  std::function<int(int, int, int)> rev = [&](int t0, int l, int cc) -> int {
+    if (rc_err) return rc_err;
+    if (l == 1) {
+      int r = load(t0);
+      return r ? r : adj_step();
+    }
+    if (cc == 0) {
+      for (int j = l - 1; j >= 0; j--) {
+        int r = load(t0);
+        if (r) return r;
+        r = lbm_enqueue_steps(c, j);
+        if (r) return r;
+        recompute += j;
+        r = adj_step();
+        if (r) return r;
+      }
+      return 0;
+    }
+    const int m = rv.M[l][cc];
+    const int tm = t0 + m;
+    bool placed_here = false;
+    if (!held.count(tm)) {
+      int r = load(t0);
+      if (r) return r;
+      r = lbm_enqueue_steps(c, m);
+      if (r) return r;
+      recompute += m;
+      int slot = -1;
+      for (int k = 1; k < nslots; k++)
+        if (!busy[k]) {
+          slot = k;
+          break;
+        }
+      if (slot < 0) return adj_fail(c, LBM_ERR_STATE, "no free checkpoint slot");
+      hipError_t e = hipMemcpyAsync(a->slots[slot], c->buf[c->cur], sbytes, hipMemcpyDeviceToDevice, st);
+      if (e != hipSuccess) return adj_fail(c, LBM_ERR_HIP, "checkpoint store failed");
+      busy[slot] = true;
+      held[tm] = slot;
+      placed_here = true;
+    }
+    (void)placed_here;
+    int r = rev(tm, l - m, cc - 1);
+    if (r) return r;
+    busy[held[tm]] = false;
+    held.erase(tm);
+    return rev(t0, m, cc);
+  };
+  rc_err = rev(0, nsteps, nslots - 1);
+  if (rc_err) return rc_err;
+  ACHK(hipEventRecord(c->ev1, st));
+  float rev_ms = 0.f;
+  ACHK(hipEventSynchronize(c->ev1));
+  ACHK(hipEventElapsedTime(&rev_ms, c->ev0, c->ev1));
+
+  ACHK(hipMemcpyAsync(grad_alpha, a->galpha, (size_t)N * sizeof(double), hipMemcpyDeviceToHost, st));
+  // leave the context at the state after nsteps
+  ACHK(hipMemcpyAsync(c->buf[c->cur], a->final_state, sbytes, hipMemcpyDeviceToDevice, st));
+  ACHK(hipStreamSynchronize(st));
+  *J = Jv;
+  if (stats) {
+    stats->snapshots_used = nslots - 1;
+    stats->forward_steps = nsteps;
+    stats->recompute_steps = recompute;
+    stats->reverse_steps = reversed;
+    stats->forward_ms = fwd_ms;
+    stats->reverse_ms = rev_ms;
+  }
+  for (int64_t i = 0; i < N; i++)
+    if (!std::isfinite(grad_alpha[i])) return adj_fail(c, LBM_ERR_NUMERIC, "non-finite gradient");
+#undef ACHK
+  return LBM_OK;
+}

@@ -1,0 +1,373 @@
+// C-ABI of the dense D3Q19 solver context (include/lbm_capi.h).  Host code only: it owns the
+// device buffers, validates every argument and enqueues the kernels of lbm_dense.hip on the
+// context's stream.  Citations: LB/ = madpac/peano/applications/latticeboltzmann/,
+// BLB/ = LB/blocklatticeboltzmann/.
+#include "lbm_capi.h"
+
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "lbm_ctx.h"
+#include "lbm_internal.h"
+
+using lbm::Q;
+
+namespace {
+thread_local std::string g_create_error;
+
+int fail(lbm_ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  else g_create_error = msg;
+  return code;
+}
+
+int hipfail(lbm_ctx* c, hipError_t e, const char* what) {
+  return fail(c, LBM_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define HIPCHK(ctx, call)                              \
+  do {                                                 \
+    hipError_t e_ = (call);                            \
+    if (e_ != hipSuccess) return hipfail(ctx, e_, #call); \
+  } while (0)
+
+int ensure_scratch(lbm_ctx* c, size_t bytes) {
+  if (c->scratch_bytes >= bytes) return LBM_OK;
+  if (c->scratch) (void)hipFree(c->scratch);
+  c->scratch = nullptr;
+  c->scratch_bytes = 0;
+  if (hipMalloc(&c->scratch, bytes) != hipSuccess) return fail(c, LBM_ERR_NOMEM, "scratch allocation failed");
+  c->scratch_bytes = bytes;
+  return LBM_OK;
+}
+
+int rebuild_words(lbm_ctx* c) {
+  HIPCHK(c, lbm::launch_build_words(c->g, c->d_inner, c->has_cls ? c->d_cls : nullptr, c->words, c->stream));
+  HIPCHK(c, lbm::launch_count_fluid(c->g, c->words, c->partials, c->small, c->stream));
+  double n = 0.0;
+  HIPCHK(c, hipMemcpyAsync(&n, c->small, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->nfluid = (int64_t)n;
+  // the stored state is a permutation that depends on the flags: it must be set again
+  const size_t bytes = (size_t)Q * c->g.N * c->esize;
+  HIPCHK(c, hipMemsetAsync(c->buf[0], 0, bytes, c->stream));
+  HIPCHK(c, hipMemsetAsync(c->buf[1], 0, bytes, c->stream));
+  c->state_set = false;
+  return LBM_OK;
+}
+
+int need_state(lbm_ctx* c) {
+  if (!c->flags_set) return fail(c, LBM_ERR_STATE, "boundary flags not set (lbm_set_flags)");
+  if (!c->state_set) return fail(c, LBM_ERR_STATE, "state not initialised (lbm_init_equilibrium / lbm_set_populations)");
+  return LBM_OK;
+}
+}  // namespace
+
+// LB/configurations/LatticeBoltzmannConfiguration.cpp:269 and LB/MultiLevelSimData.h:160
+double lbm_tau_effective(double tau) {
+  double nu = 1.0 / 3.0 * (tau - 0.5);
+  const double fac = 1.0;
+  nu = nu * fac;
+  return 3.0 * nu + 0.5;
+}
+
+extern "C" {
+
+int lbm_create(lbm_ctx** out, const lbm_config* cfg) {
+  if (!out || !cfg) return fail(nullptr, LBM_ERR_ARG, "null argument");
+  *out = nullptr;
+  if (cfg->nx < 1 || cfg->ny < 1 || cfg->nz < 1) return fail(nullptr, LBM_ERR_ARG, "lattice extent must be >= 1");
+  if ((int64_t)cfg->nx * cfg->ny * cfg->nz > (int64_t)1 << 33)
+    return fail(nullptr, LBM_ERR_ARG, "lattice too large");
+  if (cfg->precision != LBM_FP64 && cfg->precision != LBM_FP32_SHIFTED)
+    return fail(nullptr, LBM_ERR_ARG, "unknown precision");
+  if (!(cfg->tau > 0.5) || !(cfg->tau < 1e6)) return fail(nullptr, LBM_ERR_ARG, "tau must be > 0.5");
+  for (int k = 0; k < 3; k++)
+    if (!std::isfinite(cfg->force[k])) return fail(nullptr, LBM_ERR_ARG, "force must be finite");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(nullptr, LBM_ERR_HIP, "no HIP device");
+  if (cfg->device < 0 || cfg->device >= ndev) return fail(nullptr, LBM_ERR_ARG, "device ordinal out of range");
+
+  lbm_ctx* c = new lbm_ctx();
+  c->cfg = *cfg;
+  c->fp32 = cfg->precision == LBM_FP32_SHIFTED;
+  c->esize = c->fp32 ? sizeof(float) : sizeof(double);
+  c->g = lbm::make_geom(cfg->nx, cfg->ny, cfg->nz, cfg->periodic);
+  c->force = cfg->force[0] != 0.0 || cfg->force[1] != 0.0 || cfg->force[2] != 0.0;
+  c->p64.tau_eff = lbm_tau_effective(cfg->tau);
+  c->p64.omega = 1.0 / c->p64.tau_eff;
+  for (int k = 0; k < 3; k++) c->p64.g[k] = cfg->force[k];
+  c->p32.omega = (float)c->p64.omega;
+  c->p32.fpref = (float)(1.0 - 0.5 * c->p64.omega);
+  for (int k = 0; k < 3; k++) c->p32.g[k] = (float)cfg->force[k];
+  std::memset(&c->wt, 0, sizeof(c->wt));
+
+  auto bail = [&](int code, const char* msg) {
+    g_create_error = msg;
+    lbm_destroy(c);
+    return code;
+  };
+  if (hipSetDevice(cfg->device) != hipSuccess) return bail(LBM_ERR_HIP, "hipSetDevice failed");
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
+    return bail(LBM_ERR_HIP, "stream creation failed");
+  if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess)
+    return bail(LBM_ERR_HIP, "event creation failed");
+  const size_t pbytes = (size_t)Q * c->g.N * c->esize;
+  c->npartials = lbm::dense_moment_partials(c->g);
+  if (hipMalloc(&c->buf[0], pbytes) != hipSuccess || hipMalloc(&c->buf[1], pbytes) != hipSuccess ||
+      hipMalloc(&c->words, (size_t)c->g.N * sizeof(uint32_t)) != hipSuccess ||
+      hipMalloc(&c->d_inner, (size_t)c->g.N) != hipSuccess || hipMalloc(&c->d_cls, (size_t)c->g.N) != hipSuccess ||
+      hipMalloc(&c->partials, (size_t)(c->npartials + 1024) * 4 * sizeof(double)) != hipSuccess ||
+      hipMalloc(&c->small, 8 * sizeof(double)) != hipSuccess)
+    return bail(LBM_ERR_NOMEM, "device allocation failed");
+  if (hipMemsetAsync(c->buf[0], 0, pbytes, c->stream) != hipSuccess ||
+      hipMemsetAsync(c->buf[1], 0, pbytes, c->stream) != hipSuccess ||
+      hipMemsetAsync(c->words, 0, (size_t)c->g.N * sizeof(uint32_t), c->stream) != hipSuccess ||
+      hipMemsetAsync(c->d_cls, 0, (size_t)c->g.N, c->stream) != hipSuccess ||
+      hipStreamSynchronize(c->stream) != hipSuccess)
+    return bail(LBM_ERR_HIP, "initial clear failed");
+  *out = c;
+  return LBM_OK;
+}
+
+int lbm_destroy(lbm_ctx* c) {
+  if (!c) return LBM_OK;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  lbm_adjoint_release(c);
+  void* ptrs[] = {c->buf[0], c->buf[1], c->words, c->d_inner, c->d_cls, c->alpha, c->partials, c->small, c->scratch};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  if (c->ev0) (void)hipEventDestroy(c->ev0);
+  if (c->ev1) (void)hipEventDestroy(c->ev1);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return LBM_OK;
+}
+
+const char* lbm_last_error(const lbm_ctx* c) { return c ? c->err.c_str() : g_create_error.c_str(); }
+
+int64_t lbm_fluid_cells(const lbm_ctx* c) { return c ? c->nfluid : -1; }
+
+int lbm_set_flags(lbm_ctx* c, const uint8_t* inner) {
+  if (!c) return LBM_ERR_ARG;
+  if (!inner) return fail(c, LBM_ERR_ARG, "inner flags are null");
+  HIPCHK(c, hipSetDevice(c->cfg.device));
+  HIPCHK(c, hipMemcpyAsync(c->d_inner, inner, (size_t)c->g.N, hipMemcpyHostToDevice, c->stream));
+  c->flags_set = true;
+  return rebuild_words(c);
+}
+
+int lbm_set_moving_wall_class(lbm_ctx* c, int k, const double uw[3], uint32_t dir_mask) {
+  if (!c) return LBM_ERR_ARG;
+  if (k < 1 || k >= lbm::MAX_CLASSES) return fail(c, LBM_ERR_ARG, "moving-wall class must be in 1..7");
+  if (!uw) return fail(c, LBM_ERR_ARG, "wall velocity is null");
+  for (int d = 0; d < 3; d++)
+    if (!std::isfinite(uw[d])) return fail(c, LBM_ERR_ARG, "wall velocity must be finite");
+  for (int d = 0; d < 3; d++) c->wt.uw[k][d] = uw[d];
+  c->wt.mask[k] = dir_mask & ((1u << Q) - 1u) & ~(1u << lbm::REST);
+  c->walls = false;
+  for (int j = 1; j < lbm::MAX_CLASSES; j++) c->walls = c->walls || c->wt.mask[j] != 0u;
+  return LBM_OK;
+}
+
+int lbm_set_wall_classes(lbm_ctx* c, const uint8_t* cls) {
+  if (!c) return LBM_ERR_ARG;
+  HIPCHK(c, hipSetDevice(c->cfg.device));
+  if (cls) {
+    for (int64_t i = 0; i < c->g.N; i++)
+      if (cls[i] >= lbm::MAX_CLASSES) return fail(c, LBM_ERR_ARG, "wall class id must be < 8");
+    HIPCHK(c, hipMemcpyAsync(c->d_cls, cls, (size_t)c->g.N, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->has_cls = true;
+  } else {
+    c->has_cls = false;
+  }
+  return c->flags_set ? rebuild_words(c) : LBM_OK;
+}
+
+int lbm_set_cavity_lid(lbm_ctx* c, double ulid) {
+  if (!c) return LBM_ERR_ARG;
+  const double uw[3] = {ulid, 0.0, 0.0};
+  uint32_t mask = 0;
+  for (int q = 0; q < Q; q++)
+    if (lbm::CZ(q) > 0) mask |= 1u << q;
+  int rc = lbm_set_moving_wall_class(c, 1, uw, mask);
+  if (rc) return rc;
+  std::vector<uint8_t> cls((size_t)c->g.N, 0);
+  const int64_t plane = (int64_t)c->g.nx * c->g.ny;
+  for (int64_t i = (int64_t)(c->g.nz - 1) * plane; i < c->g.N; i++) cls[i] = 1;
+  return lbm_set_wall_classes(c, cls.data());
+}
+
+int lbm_set_porosity(lbm_ctx* c, const double* alpha) {
+  if (!c) return LBM_ERR_ARG;
+  HIPCHK(c, hipSetDevice(c->cfg.device));
+  if (!alpha) {
+    if (c->alpha) (void)hipFree(c->alpha);
+    c->alpha = nullptr;
+    return LBM_OK;
+  }
+  for (int64_t i = 0; i < c->g.N; i++)
+    if (!(alpha[i] >= 0.0 && alpha[i] <= 1.0)) return fail(c, LBM_ERR_ARG, "porosity must lie in [0,1]");
+  if (!c->alpha && hipMalloc(&c->alpha, (size_t)c->g.N * c->esize) != hipSuccess)
+    return fail(c, LBM_ERR_NOMEM, "porosity allocation failed");
+  if (c->fp32) {
+    std::vector<float> a32((size_t)c->g.N);
+    for (int64_t i = 0; i < c->g.N; i++) a32[i] = (float)alpha[i];
+    HIPCHK(c, hipMemcpy(c->alpha, a32.data(), a32.size() * sizeof(float), hipMemcpyHostToDevice));
+  } else {
+    HIPCHK(c, hipMemcpy(c->alpha, alpha, (size_t)c->g.N * sizeof(double), hipMemcpyHostToDevice));
+  }
+  return LBM_OK;
+}
+
+int lbm_init_equilibrium(lbm_ctx* c, const double* rho, const double* u) {
+  if (!c) return LBM_ERR_ARG;
+  if (!c->flags_set) return fail(c, LBM_ERR_STATE, "boundary flags not set (lbm_set_flags)");
+  HIPCHK(c, hipSetDevice(c->cfg.device));
+  const size_t n = (size_t)c->g.N;
+  int rc = ensure_scratch(c, (rho ? n : 0) * sizeof(double) + (u ? 3 * n : 0) * sizeof(double) + 8);
+  if (rc) return rc;
+  double* d_rho = rho ? (double*)c->scratch : nullptr;
+  double* d_u = u ? (double*)c->scratch + (rho ? n : 0) : nullptr;
+  if (rho) HIPCHK(c, hipMemcpyAsync(d_rho, rho, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  if (u) HIPCHK(c, hipMemcpyAsync(d_u, u, 3 * n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, lbm::launch_dense_init_eq(c->g, c->fp32, d_rho, d_u, c->buf[c->cur], c->words, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->state_set = true;
+  return LBM_OK;
+}
+
+int lbm_set_populations(lbm_ctx* c, const double* f) {
+  if (!c) return LBM_ERR_ARG;
+  if (!f) return fail(c, LBM_ERR_ARG, "populations are null");
+  if (!c->flags_set) return fail(c, LBM_ERR_STATE, "boundary flags not set (lbm_set_flags)");
+  HIPCHK(c, hipSetDevice(c->cfg.device));
+  const size_t bytes = (size_t)Q * c->g.N * sizeof(double);
+  int rc = ensure_scratch(c, bytes);
+  if (rc) return rc;
+  HIPCHK(c, hipMemcpyAsync(c->scratch, f, bytes, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, lbm::launch_dense_unpull(c->g, c->fp32, (const double*)c->scratch, c->buf[c->cur], c->words, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->state_set = true;
+  return LBM_OK;
+}
+
+int lbm_get_populations(lbm_ctx* c, double* f) {
+  if (!c) return LBM_ERR_ARG;
+  if (!f) return fail(c, LBM_ERR_ARG, "output is null");
+  int rc = need_state(c);
+  if (rc) return rc;
+  HIPCHK(c, hipSetDevice(c->cfg.device));
+  const size_t bytes = (size_t)Q * c->g.N * sizeof(double);
+  rc = ensure_scratch(c, bytes);
+  if (rc) return rc;
+  HIPCHK(c, lbm::launch_dense_pull(c->g, c->fp32, c->buf[c->cur], (double*)c->scratch, c->words, c->stream));
+  HIPCHK(c, hipMemcpyAsync(f, c->scratch, bytes, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return LBM_OK;
+}
+
+}  // extern "C"
+
+int lbm_enqueue_steps(lbm_ctx* c, int nsteps) {
+  for (int s = 0; s < nsteps; s++) {
+    HIPCHK(c, lbm::launch_dense_step(c->g, c->fp32, c->buf[c->cur], c->buf[c->cur ^ 1], c->words, c->alpha, c->force,
+                                     c->walls, c->p64, c->p32, c->wt, c->stream));
+    c->cur ^= 1;
+  }
+  return LBM_OK;
+}
+
+extern "C" {
+
+int lbm_step(lbm_ctx* c, int nsteps) {
+  if (!c) return LBM_ERR_ARG;
+  if (nsteps < 0) return fail(c, LBM_ERR_ARG, "nsteps must be >= 0");
+  int rc = need_state(c);
+  if (rc) return rc;
+  HIPCHK(c, hipSetDevice(c->cfg.device));
+  return lbm_enqueue_steps(c, nsteps);
+}
+
+int lbm_step_timed(lbm_ctx* c, int nsteps, float* ms) {
+  if (!c) return LBM_ERR_ARG;
+  if (!ms) return fail(c, LBM_ERR_ARG, "ms is null");
+  if (nsteps < 0) return fail(c, LBM_ERR_ARG, "nsteps must be >= 0");
+  int rc = need_state(c);
+  if (rc) return rc;
+  HIPCHK(c, hipSetDevice(c->cfg.device));
+  HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+  rc = lbm_enqueue_steps(c, nsteps);
+  if (rc) return rc;
+  HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+  HIPCHK(c, hipEventSynchronize(c->ev1));
+  HIPCHK(c, hipEventElapsedTime(ms, c->ev0, c->ev1));
+  return LBM_OK;
+}
+
+int lbm_synchronize(lbm_ctx* c) {
+  if (!c) return LBM_ERR_ARG;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return LBM_OK;
+}
+
+int lbm_get_fields(lbm_ctx* c, double* rho, double* u) {
+  if (!c) return LBM_ERR_ARG;
+  if (!rho || !u) return fail(c, LBM_ERR_ARG, "output is null");
+  int rc = need_state(c);
+  if (rc) return rc;
+  HIPCHK(c, hipSetDevice(c->cfg.device));
+  const size_t n = (size_t)c->g.N;
+  rc = ensure_scratch(c, 4 * n * sizeof(double));
+  if (rc) return rc;
+  double* d_rho = (double*)c->scratch;
+  double* d_u = d_rho + n;
+  HIPCHK(c, lbm::launch_dense_fields(c->g, c->fp32, c->buf[c->cur], c->words, d_rho, d_u, c->stream));
+  HIPCHK(c, hipMemcpyAsync(rho, d_rho, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(u, d_u, 3 * n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return LBM_OK;
+}
+
+int lbm_reduce_moments(lbm_ctx* c, double out[4]) {
+  if (!c) return LBM_ERR_ARG;
+  if (!out) return fail(c, LBM_ERR_ARG, "output is null");
+  int rc = need_state(c);
+  if (rc) return rc;
+  HIPCHK(c, hipSetDevice(c->cfg.device));
+  double h[4];
+  HIPCHK(c, lbm::launch_dense_moments(c->g, c->fp32, c->buf[c->cur], c->words, c->partials, c->small, c->stream));
+  HIPCHK(c, hipMemcpyAsync(h, c->small, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  // the kernel sums deviations from rest (f - w); the rest mass is one per fluid cell
+  out[0] = (double)c->nfluid + h[0];
+  out[1] = h[1];
+  out[2] = h[2];
+  out[3] = h[3];
+  for (int k = 0; k < 4; k++)
+    if (!std::isfinite(out[k])) return fail(c, LBM_ERR_NUMERIC, "non-finite population detected");
+  return LBM_OK;
+}
+
+int lbm_pressure_drop(lbm_ctx* c, int xin, int xout, double* J) {
+  if (!c) return LBM_ERR_ARG;
+  if (!J) return fail(c, LBM_ERR_ARG, "output is null");
+  if (xin < 0 || xin >= c->g.nx || xout < 0 || xout >= c->g.nx) return fail(c, LBM_ERR_ARG, "plane out of range");
+  int rc = need_state(c);
+  if (rc) return rc;
+  HIPCHK(c, hipSetDevice(c->cfg.device));
+  double h[4];
+  HIPCHK(c, lbm::launch_dense_plane_rho(c->g, c->fp32, c->buf[c->cur], c->words, xin, xout, c->partials, c->small,
+                                         c->stream));
+  HIPCHK(c, hipMemcpyAsync(h, c->small, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (h[1] < 1.0 || h[3] < 1.0) return fail(c, LBM_ERR_ARG, "a plane holds no fluid cell");
+  *J = (1.0 / 3.0) * (h[0] / h[1] - h[2] / h[3]);
+  if (!std::isfinite(*J)) return fail(c, LBM_ERR_NUMERIC, "non-finite pressure drop");
+  return LBM_OK;
+}
+
+}  // extern "C"

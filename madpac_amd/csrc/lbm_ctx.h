@@ -1,0 +1,47 @@
+// The dense solver context behind the opaque lbm_ctx handle of include/lbm_capi.h.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+
+#include "lbm_capi.h"
+#include "lbm_internal.h"
+
+namespace lbm {
+struct AdjointState;  // lbm_adjoint.hip
+}
+
+struct lbm_ctx {
+  lbm_config cfg{};
+  lbm::DenseGeom g{};
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  std::string err;
+  bool fp32 = false;
+  size_t esize = 8;
+  void* buf[2] = {nullptr, nullptr};  // populations (double or shifted float), ping-pong
+  int cur = 0;                         // buffer holding the current stored state
+  uint32_t* words = nullptr;           // per-cell fluid / bounce / wall-class word
+  uint8_t* d_inner = nullptr;
+  uint8_t* d_cls = nullptr;
+  bool has_cls = false;
+  void* alpha = nullptr;               // Brinkman porosity (same element type as the populations)
+  bool flags_set = false, state_set = false;
+  bool walls = false, force = false;
+  lbm::WallTable wt{};
+  lbm::CollideParams p64{};
+  lbm::CollideParams32 p32{};
+  double* partials = nullptr;
+  int64_t npartials = 0;
+  double* small = nullptr;             // 8 doubles of reduction results
+  void* scratch = nullptr;
+  size_t scratch_bytes = 0;
+  int64_t nfluid = 0;
+  lbm::AdjointState* adj = nullptr;
+};
+
+double lbm_tau_effective(double tau);
+int lbm_enqueue_steps(lbm_ctx* c, int nsteps);
+void lbm_adjoint_release(lbm_ctx* c);

@@ -1,0 +1,448 @@
+// Dense-lattice kernels of the D3Q19 block-LB solver for CDNA4 (gfx950).
+//
+// Hot path: k_dense_step, one fused collide+stream per cell in pull form.  A thread owns one
+// cell; it gathers the 19 incoming populations from the SoA state (population q of cell
+// x - c_q, or the reflected population 18-q of its own cell when x - c_q is not fluid), then
+// collides, applies the moving-wall term and writes its 19 post-collision populations back to
+// its own slots of the other buffer.  Reads of one population plane are contiguous along x, so
+// a wavefront of 64 lanes reads 64 consecutive values (512 B in FP64, 256 B in FP32) offset
+// by at most one element; writes are aligned.  Nothing is staged in LDS: each value is read
+// once and written once, and the neighbour rows a workgroup re-reads sit in L2.
+//
+// The reference's own update is the three-phase swap algorithm
+// (BLB/mappings/RegularGrid2RegularBlockSolver.cpp:83-248 with
+// BLB/nonlocaltreatment/AABlockStreaming.cpp:19-92); storing the post-collision state and
+// pulling gives the same numbers, because a moving-wall correction is applied to the outgoing
+// population of a wall link, which only that cell's own bounce-back ever reads.
+#include "lbm_geom.h"
+
+namespace lbm {
+
+DenseGeom make_geom(int nx, int ny, int nz, const int periodic[3]) {
+  DenseGeom g;
+  g.nx = nx; g.ny = ny; g.nz = nz;
+  g.px = periodic[0] ? 1 : 0; g.py = periodic[1] ? 1 : 0; g.pz = periodic[2] ? 1 : 0;
+  g.N = (int64_t)nx * ny * nz;
+  g.nbx = (nx + BX - 1) / BX;
+  g.nby = (ny + BY - 1) / BY;
+  g.nblocks = (int64_t)g.nbx * g.nby * nz;
+  g.nblocks_padded = (g.nblocks + NXCD - 1) / NXCD * NXCD;
+  return g;
+}
+
+// ------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(BX * BY) void k_build_words(DenseGeom g, const uint8_t* __restrict__ inner,
+                                                         const uint8_t* __restrict__ cls,
+                                                         uint32_t* __restrict__ words) {
+  const CellPos c = map_cell(g);
+  if (!c.valid) return;
+  if (!inner[c.i]) {
+    words[c.i] = 0u;
+    return;
+  }
+  // class ids index an 8-entry table; the host rejects larger ids, the mask keeps it in range
+  uint32_t w = FLUID | ((cls ? (uint32_t)cls[c.i] & (MAX_CLASSES - 1) : 0u) << CLASS_SHIFT);
+#pragma unroll
+  for (int q = 0; q < Q; q++) {
+    if (q == REST) continue;
+    int n[3] = {c.x + CX(q), c.y + CY(q), c.z + CZ(q)};
+    const int e[3] = {g.nx, g.ny, g.nz};
+    const int per[3] = {g.px, g.py, g.pz};
+    bool outside = false;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      if (n[k] < 0 || n[k] >= e[k]) {
+        if (per[k]) n[k] = (n[k] + e[k]) % e[k];
+        else outside = true;
+      }
+    }
+    if (outside || !inner[(int64_t)n[0] + (int64_t)g.nx * ((int64_t)n[1] + (int64_t)g.ny * n[2])]) w |= 1u << q;
+  }
+  words[c.i] = w;
+}
+
+// ------------------------------------------------------------------------------------------
+// The hot kernel.  T = double: reference arithmetic; T = float: shifted populations.
+template <typename T, bool FORCE, bool ALPHA, bool WALLS>
+__global__ __launch_bounds__(BX * BY) void k_dense_step(DenseGeom g, const T* __restrict__ src,
+                                                        T* __restrict__ dst, const uint32_t* __restrict__ words,
+                                                        const T* __restrict__ alpha, CollideParams p64,
+                                                        CollideParams32 p32, WallTable wt) {
+  const CellPos c = map_cell(g);
+  if (!c.valid) return;
+  const uint32_t w = words[c.i];
+  if (!(w & FLUID)) return;
+  const NbrOff o = nbr_offsets(g, c);
+  const int64_t N = g.N;
+  T f[Q];
+#pragma unroll
+  for (int q = 0; q < Q; q++) f[q] = src[pull_index(q, w, c.i, N, o)];
+  const T a = ALPHA ? alpha[c.i] : T(0);
+  T rho;
+  if constexpr (sizeof(T) == 8) {
+    rho = collide_f64<FORCE, ALPHA>(f, p64, a);
+  } else {
+    rho = collide_f32s<FORCE, ALPHA>(f, p32, a);
+  }
+  if (WALLS) {
+    const uint32_t k = (w >> CLASS_SHIFT) & (MAX_CLASSES - 1);
+    if (k) {
+      const uint32_t m = w & wt.mask[k];
+#pragma unroll
+      for (int q = 0; q < Q; q++) {
+        if ((m >> q) & 1u) {
+          if constexpr (sizeof(T) == 8) f[q] -= moving_wall_f64(q, wt.uw[k], rho);
+          else f[q] -= moving_wall_f32(q, wt.uw[k], rho);
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < Q; q++) __builtin_nontemporal_store(f[q], &dst[(int64_t)q * N + c.i]);
+}
+
+// ------------------------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(BX * BY) void k_dense_pull(DenseGeom g, const T* __restrict__ src,
+                                                        double* __restrict__ f, const uint32_t* __restrict__ words) {
+  const CellPos c = map_cell(g);
+  if (!c.valid) return;
+  const uint32_t w = words[c.i];
+  const int64_t N = g.N;
+  if (!(w & FLUID)) {
+#pragma unroll
+    for (int q = 0; q < Q; q++) f[(int64_t)q * N + c.i] = 0.0;
+    return;
+  }
+  const NbrOff o = nbr_offsets(g, c);
+#pragma unroll
+  for (int q = 0; q < Q; q++) {
+    const T v = src[pull_index(q, w, c.i, N, o)];
+    f[(int64_t)q * N + c.i] = (sizeof(T) == 8) ? (double)v : (double)v + W64(q);
+  }
+}
+
+// Scatter of a reference state into the stored representation (inverse of the pull).
+template <typename T>
+__global__ __launch_bounds__(BX * BY) void k_dense_unpull(DenseGeom g, const double* __restrict__ f,
+                                                          T* __restrict__ dst, const uint32_t* __restrict__ words) {
+  const CellPos c = map_cell(g);
+  if (!c.valid) return;
+  const uint32_t w = words[c.i];
+  if (!(w & FLUID)) return;
+  const NbrOff o = nbr_offsets(g, c);
+  const int64_t N = g.N;
+#pragma unroll
+  for (int q = 0; q < Q; q++) {
+    const double v = f[(int64_t)q * N + c.i];
+    dst[pull_index(q, w, c.i, N, o)] = (sizeof(T) == 8) ? (T)v : (T)(v - W64(q));
+  }
+}
+
+// feq(rho, u) exactly as BLB/BlockComputeCellValues.cpp:9-22 evaluates it, scattered like unpull.
+template <typename T>
+__global__ __launch_bounds__(BX * BY) void k_dense_init_eq(DenseGeom g, const double* __restrict__ rho,
+                                                           const double* __restrict__ u, T* __restrict__ dst,
+                                                           const uint32_t* __restrict__ words) {
+  const CellPos c = map_cell(g);
+  if (!c.valid) return;
+  const uint32_t w = words[c.i];
+  if (!(w & FLUID)) return;
+  const NbrOff o = nbr_offsets(g, c);
+  const int64_t N = g.N;
+  const double r = rho ? rho[c.i] : 1.0;
+  const double u0 = u ? u[3 * c.i] : 0.0, u1 = u ? u[3 * c.i + 1] : 0.0, u2 = u ? u[3 * c.i + 2] : 0.0;
+  double uu = u0 * u0;
+  uu += u1 * u1;
+  uu += u2 * u2;
+#pragma unroll
+  for (int q = 0; q < Q; q++) {
+    const double cu = cdot64(q, u0, u1, u2);
+    const double feq = W64(q) * r *
+        (1.0 + cu / (1.0 / 3.0) + (cu * cu) / (2.0 * (1.0 / 3.0) * (1.0 / 3.0)) - (uu / (2.0 * (1.0 / 3.0))));
+    dst[pull_index(q, w, c.i, N, o)] = (sizeof(T) == 8) ? (T)feq : (T)(feq - W64(q));
+  }
+}
+
+// rho and u of the reference state, in the reference's operation order for FP64.
+template <typename T>
+__global__ __launch_bounds__(BX * BY) void k_dense_fields(DenseGeom g, const T* __restrict__ src,
+                                                          const uint32_t* __restrict__ words,
+                                                          double* __restrict__ rho, double* __restrict__ u) {
+  const CellPos c = map_cell(g);
+  if (!c.valid) return;
+  const uint32_t w = words[c.i];
+  if (!(w & FLUID)) {
+    rho[c.i] = 0.0;
+    u[3 * c.i] = u[3 * c.i + 1] = u[3 * c.i + 2] = 0.0;
+    return;
+  }
+  const NbrOff o = nbr_offsets(g, c);
+  double r = 0.0, b0 = 0.0, b1 = 0.0, b2 = 0.0;
+#pragma unroll
+  for (int q = 0; q < Q; q++) {
+    const T v = src[pull_index(q, w, c.i, g.N, o)];
+    const double fv = (sizeof(T) == 8) ? (double)v : (double)v + W64(q);
+    r += fv;
+    if (CX(q) == 1) b0 += fv;
+    if (CX(q) == -1) b0 -= fv;
+    if (CY(q) == 1) b1 += fv;
+    if (CY(q) == -1) b1 -= fv;
+    if (CZ(q) == 1) b2 += fv;
+    if (CZ(q) == -1) b2 -= fv;
+  }
+  const double inv = 1.0 / r;
+  rho[c.i] = r;
+  u[3 * c.i] = b0 * inv;
+  u[3 * c.i + 1] = b1 * inv;
+  u[3 * c.i + 2] = b2 * inv;
+}
+
+// ------------------------------------------------------------------------------------------
+// Deterministic FP64 reductions: each block reduces its cells with 64-lane wavefront shuffles
+// then an LDS step over its 4 wavefronts, writes one partial, and a single block sums the
+// partials in a fixed order.
+template <int K>
+__device__ __forceinline__ void block_reduce_store(double (&v)[K], double* __restrict__ out) {
+  __shared__ double lds[K][BX * BY / 64];
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_xor(v[k], off, 64);
+  }
+  const int tid = threadIdx.x + threadIdx.y * BX;
+  const int wave = tid / 64, lane = tid % 64;
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < K; k++) lds[k][wave] = v[k];
+  }
+  __syncthreads();
+  if (tid == 0) {
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      double s = 0.0;
+      for (int j = 0; j < BX * BY / 64; j++) s += lds[k][j];
+      out[k] = s;
+    }
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(BX * BY) void k_dense_moments(DenseGeom g, const T* __restrict__ src,
+                                                           const uint32_t* __restrict__ words,
+                                                           double* __restrict__ partials) {
+  const CellPos c = map_cell(g);
+  double v[4] = {0.0, 0.0, 0.0, 0.0};
+  if (c.valid) {
+    const uint32_t w = words[c.i];
+    if (w & FLUID) {
+      const NbrOff o = nbr_offsets(g, c);
+#pragma unroll
+      for (int q = 0; q < Q; q++) {
+        const T s = src[pull_index(q, w, c.i, g.N, o)];
+        // deviation from the rest state: f - w (FP64) or the shifted value itself (FP32)
+        const double fv = (sizeof(T) == 8) ? (double)s : (double)s + W64(q);
+        const double dv = (sizeof(T) == 8) ? (double)s - W64(q) : (double)s;
+        v[0] += dv;
+        if (CX(q) == 1) v[1] += fv;
+        if (CX(q) == -1) v[1] -= fv;
+        if (CY(q) == 1) v[2] += fv;
+        if (CY(q) == -1) v[2] -= fv;
+        if (CZ(q) == 1) v[3] += fv;
+        if (CZ(q) == -1) v[3] -= fv;
+      }
+    }
+  }
+  block_reduce_store<4>(v, partials + 4 * (int64_t)blockIdx.x);
+}
+
+template <int K>
+__global__ __launch_bounds__(256) void k_sum_partials(const double* __restrict__ partials, int64_t n,
+                                                      double* __restrict__ out) {
+  __shared__ double lds[K][256];
+  double s[K];
+#pragma unroll
+  for (int k = 0; k < K; k++) s[k] = 0.0;
+  for (int64_t j = threadIdx.x; j < n; j += 256) {
+#pragma unroll
+    for (int k = 0; k < K; k++) s[k] += partials[K * j + k];
+  }
+#pragma unroll
+  for (int k = 0; k < K; k++) lds[k][threadIdx.x] = s[k];
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) {
+#pragma unroll
+      for (int k = 0; k < K; k++) lds[k][threadIdx.x] += lds[k][threadIdx.x + h];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < K; k++) out[k] = lds[k][0];
+  }
+}
+
+// density sums over the fluid cells of two x-planes; one thread per (y, z) row of each plane
+template <typename T>
+__global__ __launch_bounds__(256) void k_plane_rho(DenseGeom g, const T* __restrict__ src,
+                                                   const uint32_t* __restrict__ words, int xin, int xout,
+                                                   double* __restrict__ partials) {
+  const int64_t rows = (int64_t)g.ny * g.nz;
+  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  double v[4] = {0.0, 0.0, 0.0, 0.0};
+  if (t < rows) {
+    const int y = (int)(t % g.ny), z = (int)(t / g.ny);
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+      CellPos c;
+      c.x = k == 0 ? xin : xout;
+      c.y = y;
+      c.z = z;
+      c.i = (int64_t)c.x + (int64_t)g.nx * ((int64_t)y + (int64_t)g.ny * z);
+      const uint32_t w = words[c.i];
+      if (w & FLUID) {
+        const NbrOff o = nbr_offsets(g, c);
+        double r = 0.0;
+#pragma unroll
+        for (int q = 0; q < Q; q++) {
+          const T s = src[pull_index(q, w, c.i, g.N, o)];
+          r += (sizeof(T) == 8) ? (double)s : (double)s + W64(q);
+        }
+        v[2 * k] = r;
+        v[2 * k + 1] = 1.0;
+      }
+    }
+  }
+  // 256 threads = 4 wavefronts; reuse the 2-D block reducer layout
+  __shared__ double lds[4][4];
+#pragma unroll
+  for (int k = 0; k < 4; k++)
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_xor(v[k], off, 64);
+  if (threadIdx.x % 64 == 0)
+#pragma unroll
+    for (int k = 0; k < 4; k++) lds[k][threadIdx.x / 64] = v[k];
+  __syncthreads();
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int k = 0; k < 4; k++) partials[4 * (int64_t)blockIdx.x + k] = lds[k][0] + lds[k][1] + lds[k][2] + lds[k][3];
+}
+
+__global__ __launch_bounds__(BX * BY) void k_count_fluid(DenseGeom g, const uint32_t* __restrict__ words,
+                                                         double* __restrict__ partials) {
+  const CellPos c = map_cell(g);
+  double v[1] = {(c.valid && (words[c.i] & FLUID)) ? 1.0 : 0.0};
+  block_reduce_store<1>(v, partials + (int64_t)blockIdx.x);
+}
+
+// ------------------------------------------------------------------------------------------
+static dim3 dense_block() { return dim3(BX, BY, 1); }
+static dim3 dense_grid(const DenseGeom& g) { return dim3((unsigned)g.nblocks_padded, 1, 1); }
+
+hipError_t launch_build_words(const DenseGeom& g, const uint8_t* inner, const uint8_t* cls, uint32_t* words,
+                              hipStream_t s) {
+  hipLaunchKernelGGL(k_build_words, dense_grid(g), dense_block(), 0, s, g, inner, cls, words);
+  return hipGetLastError();
+}
+
+template <typename T>
+static hipError_t step_t(const DenseGeom& g, const void* src, void* dst, const uint32_t* words, const void* alpha,
+                         bool force, bool walls, const CollideParams& p64, const CollideParams32& p32,
+                         const WallTable& wt, hipStream_t s) {
+  const T* a = (const T*)alpha;
+  const T* sp = (const T*)src;
+  T* dp = (T*)dst;
+#define LBM_STEP(F, A, W)                                                                                 \
+  hipLaunchKernelGGL((k_dense_step<T, F, A, W>), dense_grid(g), dense_block(), 0, s, g, sp, dp, words, a, \
+                     p64, p32, wt)
+  const int key = (force ? 4 : 0) | (alpha ? 2 : 0) | (walls ? 1 : 0);
+  switch (key) {
+    case 0: LBM_STEP(false, false, false); break;
+    case 1: LBM_STEP(false, false, true); break;
+    case 2: LBM_STEP(false, true, false); break;
+    case 3: LBM_STEP(false, true, true); break;
+    case 4: LBM_STEP(true, false, false); break;
+    case 5: LBM_STEP(true, false, true); break;
+    case 6: LBM_STEP(true, true, false); break;
+    default: LBM_STEP(true, true, true); break;
+  }
+#undef LBM_STEP
+  return hipGetLastError();
+}
+
+hipError_t launch_dense_step(const DenseGeom& g, bool fp32, const void* src, void* dst, const uint32_t* words,
+                             const void* alpha, bool force, bool walls, const CollideParams& p64,
+                             const CollideParams32& p32, const WallTable& wt, hipStream_t s) {
+  return fp32 ? step_t<float>(g, src, dst, words, alpha, force, walls, p64, p32, wt, s)
+              : step_t<double>(g, src, dst, words, alpha, force, walls, p64, p32, wt, s);
+}
+
+hipError_t launch_dense_pull(const DenseGeom& g, bool fp32, const void* src, double* f, const uint32_t* words,
+                             hipStream_t s) {
+  if (fp32) hipLaunchKernelGGL(k_dense_pull<float>, dense_grid(g), dense_block(), 0, s, g, (const float*)src, f, words);
+  else hipLaunchKernelGGL(k_dense_pull<double>, dense_grid(g), dense_block(), 0, s, g, (const double*)src, f, words);
+  return hipGetLastError();
+}
+
+hipError_t launch_dense_unpull(const DenseGeom& g, bool fp32, const double* f, void* dst, const uint32_t* words,
+                               hipStream_t s) {
+  if (fp32) hipLaunchKernelGGL(k_dense_unpull<float>, dense_grid(g), dense_block(), 0, s, g, f, (float*)dst, words);
+  else hipLaunchKernelGGL(k_dense_unpull<double>, dense_grid(g), dense_block(), 0, s, g, f, (double*)dst, words);
+  return hipGetLastError();
+}
+
+hipError_t launch_dense_init_eq(const DenseGeom& g, bool fp32, const double* rho, const double* u, void* dst,
+                                const uint32_t* words, hipStream_t s) {
+  if (fp32)
+    hipLaunchKernelGGL(k_dense_init_eq<float>, dense_grid(g), dense_block(), 0, s, g, rho, u, (float*)dst, words);
+  else
+    hipLaunchKernelGGL(k_dense_init_eq<double>, dense_grid(g), dense_block(), 0, s, g, rho, u, (double*)dst, words);
+  return hipGetLastError();
+}
+
+hipError_t launch_dense_fields(const DenseGeom& g, bool fp32, const void* src, const uint32_t* words, double* rho,
+                               double* u, hipStream_t s) {
+  if (fp32)
+    hipLaunchKernelGGL(k_dense_fields<float>, dense_grid(g), dense_block(), 0, s, g, (const float*)src, words, rho, u);
+  else
+    hipLaunchKernelGGL(k_dense_fields<double>, dense_grid(g), dense_block(), 0, s, g, (const double*)src, words, rho,
+                       u);
+  return hipGetLastError();
+}
+
+int64_t dense_moment_partials(const DenseGeom& g) { return g.nblocks_padded; }
+
+hipError_t launch_dense_moments(const DenseGeom& g, bool fp32, const void* src, const uint32_t* words,
+                                double* partials, double* out4, hipStream_t s) {
+  if (fp32)
+    hipLaunchKernelGGL(k_dense_moments<float>, dense_grid(g), dense_block(), 0, s, g, (const float*)src, words,
+                       partials);
+  else
+    hipLaunchKernelGGL(k_dense_moments<double>, dense_grid(g), dense_block(), 0, s, g, (const double*)src, words,
+                       partials);
+  hipLaunchKernelGGL(k_sum_partials<4>, dim3(1), dim3(256), 0, s, (const double*)partials, g.nblocks_padded, out4);
+  return hipGetLastError();
+}
+
+hipError_t launch_dense_plane_rho(const DenseGeom& g, bool fp32, const void* src, const uint32_t* words, int xin,
+                                  int xout, double* partials, double* out4, hipStream_t s) {
+  const int64_t rows = (int64_t)g.ny * g.nz;
+  const unsigned nb = (unsigned)((rows + 255) / 256);
+  if (fp32)
+    hipLaunchKernelGGL(k_plane_rho<float>, dim3(nb), dim3(256), 0, s, g, (const float*)src, words, xin, xout, partials);
+  else
+    hipLaunchKernelGGL(k_plane_rho<double>, dim3(nb), dim3(256), 0, s, g, (const double*)src, words, xin, xout,
+                       partials);
+  hipLaunchKernelGGL(k_sum_partials<4>, dim3(1), dim3(256), 0, s, (const double*)partials, (int64_t)nb, out4);
+  return hipGetLastError();
+}
+
+hipError_t launch_count_fluid(const DenseGeom& g, const uint32_t* words, double* partials, double* out1,
+                              hipStream_t s) {
+  hipLaunchKernelGGL(k_count_fluid, dense_grid(g), dense_block(), 0, s, g, words, partials);
+  hipLaunchKernelGGL(k_sum_partials<1>, dim3(1), dim3(256), 0, s, (const double*)partials, g.nblocks_padded, out1);
+  return hipGetLastError();
+}
+
+}  // namespace lbm

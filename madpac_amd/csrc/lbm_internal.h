@@ -1,0 +1,61 @@
+// Internal declarations shared by the HIP translation units of liblbm_hip.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+
+#include "lbm_cell.h"
+
+namespace lbm {
+
+// One lattice stored as SoA populations s[q*N + i], i = x + nx*(y + ny*z).  The stored state is
+// the post-collision state of the last step; the reference's post-streaming state is its pull
+// (see k_dense_pull).  Blocks are BX x BY cells of one z-plane.
+constexpr int BX = 64;
+constexpr int BY = 4;
+constexpr int NXCD = 8;  // MI355X: 8 XCDs, each with its own L2
+
+struct DenseGeom {
+  int nx, ny, nz;
+  int px, py, pz;            // periodic flags
+  int64_t N;                 // cells
+  int nbx, nby;              // blocks along x and y
+  int64_t nblocks;           // real blocks
+  int64_t nblocks_padded;    // rounded up to a multiple of NXCD
+};
+
+DenseGeom make_geom(int nx, int ny, int nz, const int periodic[3]);
+
+// ---- launchers (all enqueue on `stream`; return hipError_t) ----
+hipError_t launch_build_words(const DenseGeom& g, const uint8_t* inner, const uint8_t* cls, uint32_t* words,
+                              hipStream_t stream);
+// one step src -> dst
+hipError_t launch_dense_step(const DenseGeom& g, bool fp32, const void* src, void* dst, const uint32_t* words,
+                             const void* alpha, bool force, bool walls, const CollideParams& p64,
+                             const CollideParams32& p32, const WallTable& wt, hipStream_t stream);
+// reference state f (FP64) from the stored state
+hipError_t launch_dense_pull(const DenseGeom& g, bool fp32, const void* src, double* f, const uint32_t* words,
+                             hipStream_t stream);
+// stored state from a reference state f (FP64)
+hipError_t launch_dense_unpull(const DenseGeom& g, bool fp32, const double* f, void* dst, const uint32_t* words,
+                               hipStream_t stream);
+// stored state from feq(rho, u); rho/u may be null (1 / 0)
+hipError_t launch_dense_init_eq(const DenseGeom& g, bool fp32, const double* rho, const double* u, void* dst,
+                                const uint32_t* words, hipStream_t stream);
+// per-cell rho/u (FP64) of the reference state
+hipError_t launch_dense_fields(const DenseGeom& g, bool fp32, const void* src, const uint32_t* words, double* rho,
+                               double* u, hipStream_t stream);
+// deterministic two-pass reduction: out4 = {sum(rho - base), jx, jy, jz}, base = 1 for FP32-shifted
+hipError_t launch_dense_moments(const DenseGeom& g, bool fp32, const void* src, const uint32_t* words,
+                                double* partials, double* out4, hipStream_t stream);
+int64_t dense_moment_partials(const DenseGeom& g);
+// plane density sums for the pressure drop: out4 = {sum rho(xin), n(xin), sum rho(xout), n(xout)}
+hipError_t launch_dense_plane_rho(const DenseGeom& g, bool fp32, const void* src, const uint32_t* words, int xin,
+                                  int xout, double* partials, double* out4, hipStream_t stream);
+// count fluid cells (exact: sums of 1.0 below 2^53); needs dense_moment_partials(g) partial slots
+hipError_t launch_count_fluid(const DenseGeom& g, const uint32_t* words, double* partials, double* out1,
+                              hipStream_t stream);
+
+}  // namespace lbm

@@ -1,0 +1,260 @@
+"""ctypes binding of the C-ABI in include/lbm_capi.h (liblbm_hip.so).
+
+This is the Python face of the solver used by the tests and the benchmark: every call goes
+through the C-ABI to the HIP kernels.  There is no CPU fallback -- if the shared library is
+missing or cannot be loaded this module raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liblbm_hip.so")
+
+Q = 19
+FP64 = 0
+FP32_SHIFTED = 1
+
+# D3Q19 ordering (madpac/peano/applications/latticeboltzmann/LatticeBoltzmannDefinitions.cpp:113-131)
+C = np.array([
+    [0, -1, -1], [-1, 0, -1], [0, 0, -1], [1, 0, -1], [0, 1, -1],
+    [-1, -1, 0], [0, -1, 0], [1, -1, 0], [-1, 0, 0], [0, 0, 0],
+    [1, 0, 0], [-1, 1, 0], [0, 1, 0], [1, 1, 0], [0, -1, 1],
+    [-1, 0, 1], [0, 0, 1], [1, 0, 1], [0, 1, 1]], dtype=np.int64)
+
+# every symbol include/lbm_capi.h declares
+EXPORTS = (
+    "lbm_create", "lbm_destroy", "lbm_last_error", "lbm_fluid_cells", "lbm_set_flags",
+    "lbm_set_moving_wall_class", "lbm_set_wall_classes", "lbm_set_cavity_lid", "lbm_set_porosity",
+    "lbm_init_equilibrium", "lbm_set_populations", "lbm_get_populations", "lbm_step", "lbm_step_timed",
+    "lbm_synchronize", "lbm_get_fields", "lbm_reduce_moments", "lbm_pressure_drop",
+    "lbm_adjoint_pressure_drop",
+    "lbm_patches_create", "lbm_patches_destroy", "lbm_patches_last_error", "lbm_patches_cells",
+    "lbm_patches_init", "lbm_patches_get_populations", "lbm_patches_autotune", "lbm_patches_set_tiling",
+    "lbm_patches_step", "lbm_patches_reduce_moments",
+)
+
+
+class LbmError(RuntimeError):
+    pass
+
+
+class Config(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int),
+                ("periodic", ctypes.c_int * 3), ("precision", ctypes.c_int), ("tau", ctypes.c_double),
+                ("force", ctypes.c_double * 3), ("device", ctypes.c_int)]
+
+
+class AdjointStats(ctypes.Structure):
+    _fields_ = [("snapshots_used", ctypes.c_int), ("forward_steps", ctypes.c_int64),
+                ("recompute_steps", ctypes.c_int64), ("reverse_steps", ctypes.c_int64),
+                ("forward_ms", ctypes.c_float), ("reverse_ms", ctypes.c_float)]
+
+
+class Patch(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int), ("tau", ctypes.c_double), ("nbr", ctypes.c_int * Q)]
+
+
+class Tiling(ctypes.Structure):
+    _fields_ = [("tile_x", ctypes.c_int), ("tile_y", ctypes.c_int), ("tile_z", ctypes.c_int),
+                ("blocks_per_cu", ctypes.c_int)]
+
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH):
+    """Load liblbm_hip.so (raises if it is absent: there is no fallback path)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise LbmError(f"{path} is missing: build it with `python -m madpac_amd.build` (hipcc, gfx950)")
+    L = ctypes.CDLL(path)
+    vp = ctypes.c_void_p
+    dp = ctypes.POINTER(ctypes.c_double)
+    u8p = ctypes.POINTER(ctypes.c_uint8)
+    L.lbm_create.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(Config)]
+    L.lbm_destroy.argtypes = [vp]
+    L.lbm_last_error.argtypes = [vp]
+    L.lbm_last_error.restype = ctypes.c_char_p
+    L.lbm_fluid_cells.argtypes = [vp]
+    L.lbm_fluid_cells.restype = ctypes.c_int64
+    L.lbm_set_flags.argtypes = [vp, u8p]
+    L.lbm_set_moving_wall_class.argtypes = [vp, ctypes.c_int, dp, ctypes.c_uint32]
+    L.lbm_set_wall_classes.argtypes = [vp, u8p]
+    L.lbm_set_cavity_lid.argtypes = [vp, ctypes.c_double]
+    L.lbm_set_porosity.argtypes = [vp, dp]
+    L.lbm_init_equilibrium.argtypes = [vp, dp, dp]
+    L.lbm_set_populations.argtypes = [vp, dp]
+    L.lbm_get_populations.argtypes = [vp, dp]
+    L.lbm_step.argtypes = [vp, ctypes.c_int]
+    L.lbm_step_timed.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_float)]
+    L.lbm_synchronize.argtypes = [vp]
+    L.lbm_get_fields.argtypes = [vp, dp, dp]
+    L.lbm_reduce_moments.argtypes = [vp, dp]
+    L.lbm_pressure_drop.argtypes = [vp, ctypes.c_int, ctypes.c_int, dp]
+    L.lbm_adjoint_pressure_drop.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, dp, dp,
+                                            ctypes.POINTER(AdjointStats)]
+    if hasattr(L, "lbm_patches_create"):
+        L.lbm_patches_create.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(Patch), ctypes.c_int, ctypes.c_int,
+                                         ctypes.c_int]
+        L.lbm_patches_destroy.argtypes = [vp]
+        L.lbm_patches_last_error.argtypes = [vp]
+        L.lbm_patches_last_error.restype = ctypes.c_char_p
+        L.lbm_patches_cells.argtypes = [vp]
+        L.lbm_patches_cells.restype = ctypes.c_int64
+        L.lbm_patches_init.argtypes = [vp, dp, dp]
+        L.lbm_patches_get_populations.argtypes = [vp, dp]
+        L.lbm_patches_autotune.argtypes = [vp, ctypes.POINTER(Tiling)]
+        L.lbm_patches_set_tiling.argtypes = [vp, ctypes.POINTER(Tiling)]
+        L.lbm_patches_step.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_float)]
+        L.lbm_patches_reduce_moments.argtypes = [vp, dp]
+    _lib = L
+    return L
+
+
+def _dp(a):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _u8(a):
+    assert a.dtype == np.uint8 and a.flags.c_contiguous
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))
+
+
+class Lattice:
+    """A dense D3Q19 lattice on one GPU (an lbm_ctx)."""
+
+    def __init__(self, nx, ny, nz, tau, precision=FP64, periodic=(False, False, False), force=(0.0, 0.0, 0.0),
+                 device=0):
+        self._L = load_library()
+        cfg = Config(nx, ny, nz, (ctypes.c_int * 3)(*[int(p) for p in periodic]), int(precision), float(tau),
+                     (ctypes.c_double * 3)(*[float(v) for v in force]), int(device))
+        h = ctypes.c_void_p()
+        rc = self._L.lbm_create(ctypes.byref(h), ctypes.byref(cfg))
+        if rc != 0:
+            raise LbmError(f"lbm_create failed ({rc}): {self._L.lbm_last_error(None).decode()}")
+        self._h = h
+        self.shape = (nz, ny, nx)
+        self.n = nx * ny * nz
+        self.precision = precision
+
+    def _check(self, rc, what):
+        if rc != 0:
+            raise LbmError(f"{what} failed ({rc}): {self._L.lbm_last_error(self._h).decode()}")
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.lbm_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # ---- geometry
+    def set_flags(self, inner):
+        a = np.ascontiguousarray(np.asarray(inner).reshape(-1) != 0, dtype=np.uint8)
+        assert a.size == self.n
+        self._check(self._L.lbm_set_flags(self._h, _u8(a)), "lbm_set_flags")
+
+    def set_cavity_lid(self, ulid):
+        self._check(self._L.lbm_set_cavity_lid(self._h, float(ulid)), "lbm_set_cavity_lid")
+
+    def set_moving_wall_class(self, k, uw, mask):
+        uw = np.ascontiguousarray(uw, dtype=np.float64)
+        self._check(self._L.lbm_set_moving_wall_class(self._h, int(k), _dp(uw), ctypes.c_uint32(mask)),
+                    "lbm_set_moving_wall_class")
+
+    def set_wall_classes(self, cls):
+        if cls is None:
+            self._check(self._L.lbm_set_wall_classes(self._h, None), "lbm_set_wall_classes")
+            return
+        a = np.ascontiguousarray(np.asarray(cls).reshape(-1), dtype=np.uint8)
+        self._check(self._L.lbm_set_wall_classes(self._h, _u8(a)), "lbm_set_wall_classes")
+
+    def set_porosity(self, alpha):
+        if alpha is None:
+            self._check(self._L.lbm_set_porosity(self._h, None), "lbm_set_porosity")
+            return
+        a = np.ascontiguousarray(np.asarray(alpha, dtype=np.float64).reshape(-1))
+        self._check(self._L.lbm_set_porosity(self._h, _dp(a)), "lbm_set_porosity")
+
+    @property
+    def fluid_cells(self):
+        return int(self._L.lbm_fluid_cells(self._h))
+
+    # ---- state
+    def init_equilibrium(self, rho=None, u=None):
+        r = None if rho is None else np.ascontiguousarray(np.asarray(rho, dtype=np.float64).reshape(-1))
+        v = None if u is None else np.ascontiguousarray(np.asarray(u, dtype=np.float64).reshape(-1))
+        self._check(self._L.lbm_init_equilibrium(self._h, None if r is None else _dp(r), None if v is None else _dp(v)),
+                    "lbm_init_equilibrium")
+
+    def set_populations(self, f):
+        a = np.ascontiguousarray(np.asarray(f, dtype=np.float64).reshape(-1))
+        assert a.size == Q * self.n
+        self._check(self._L.lbm_set_populations(self._h, _dp(a)), "lbm_set_populations")
+
+    def get_populations(self):
+        f = np.empty(Q * self.n)
+        self._check(self._L.lbm_get_populations(self._h, _dp(f)), "lbm_get_populations")
+        return f
+
+    def step(self, n=1):
+        self._check(self._L.lbm_step(self._h, int(n)), "lbm_step")
+
+    def step_timed(self, n):
+        ms = ctypes.c_float()
+        self._check(self._L.lbm_step_timed(self._h, int(n), ctypes.byref(ms)), "lbm_step_timed")
+        return float(ms.value)
+
+    def synchronize(self):
+        self._check(self._L.lbm_synchronize(self._h), "lbm_synchronize")
+
+    def fields(self):
+        rho = np.empty(self.n)
+        u = np.empty(3 * self.n)
+        self._check(self._L.lbm_get_fields(self._h, _dp(rho), _dp(u)), "lbm_get_fields")
+        return rho, u.reshape(-1, 3)
+
+    def moments(self):
+        out = np.empty(4)
+        self._check(self._L.lbm_reduce_moments(self._h, _dp(out)), "lbm_reduce_moments")
+        return out
+
+    def pressure_drop(self, xin, xout):
+        J = ctypes.c_double()
+        self._check(self._L.lbm_pressure_drop(self._h, int(xin), int(xout), ctypes.byref(J)), "lbm_pressure_drop")
+        return J.value
+
+    def adjoint_pressure_drop(self, nsteps, snapshots, xin, xout):
+        J = ctypes.c_double()
+        grad = np.empty(self.n)
+        st = AdjointStats()
+        self._check(self._L.lbm_adjoint_pressure_drop(self._h, int(nsteps), int(snapshots), int(xin), int(xout),
+                                                      ctypes.byref(J), _dp(grad), ctypes.byref(st)),
+                    "lbm_adjoint_pressure_drop")
+        return J.value, grad, st
+
+
+def cavity(n, tau, ulid, precision=FP64, device=0):
+    """The reference's lid-driven cavity: n^3 fluid cells, lid (ulid, 0, 0) on +z, f = w."""
+    lat = Lattice(n, n, n, tau, precision=precision, device=device)
+    lat.set_flags(np.ones(lat.n, np.uint8))
+    lat.set_cavity_lid(ulid)
+    lat.init_equilibrium()
+    return lat

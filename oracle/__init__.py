@@ -1,0 +1,204 @@
+"""CPU oracle for the D3Q19 block-LB hot path -- TEST INFRASTRUCTURE ONLY.
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
+``bench.py`` may import this package, and only as the checker.  The product
+package ``madpac_amd`` never imports it.
+
+The numerics live in ``lbm_oracle.c`` (a plain-C restatement of the reference's
+FP64 arithmetic, see the header for citations); this module is a thin ctypes
+binding over ``liblbm_oracle.so`` plus numpy helpers for layouts.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liblbm_oracle.so")
+
+Q = 19
+REST = 9
+FLUID = 0x80000000
+CLASS_SHIFT = 24
+MAX_CLASSES = 8
+CS2 = 1.0 / 3.0
+
+# LB/LatticeBoltzmannDefinitions.cpp:113-131 (z-major, then y, then x)
+C = np.array([
+    [0, -1, -1], [-1, 0, -1], [0, 0, -1], [1, 0, -1], [0, 1, -1],
+    [-1, -1, 0], [0, -1, 0], [1, -1, 0], [-1, 0, 0], [0, 0, 0],
+    [1, 0, 0], [-1, 1, 0], [0, 1, 0], [1, 1, 0], [0, -1, 1],
+    [-1, 0, 1], [0, 0, 1], [1, 0, 1], [0, 1, 1]], dtype=np.int64)
+# LB/LatticeBoltzmannDefinitions.cpp:149-151
+W = np.array([1 / 36, 1 / 36, 1 / 18, 1 / 36, 1 / 36, 1 / 36, 1 / 18, 1 / 36, 1 / 18, 1 / 3,
+              1 / 18, 1 / 36, 1 / 18, 1 / 36, 1 / 36, 1 / 36, 1 / 18, 1 / 36, 1 / 36])
+
+
+class Dims(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int),
+                ("periodic", ctypes.c_int * 3)]
+
+
+class Params(ctypes.Structure):
+    _fields_ = [("tau", ctypes.c_double),
+                ("uw", (ctypes.c_double * 3) * MAX_CLASSES),
+                ("moving_mask", ctypes.c_uint32 * MAX_CLASSES),
+                ("g", ctypes.c_double * 3),
+                ("use_force", ctypes.c_int),
+                ("alpha", ctypes.POINTER(ctypes.c_double))]
+
+
+_lib = None
+
+
+def build() -> str:
+    """Compile liblbm_oracle.so from oracle/lbm_oracle.c (gcc)."""
+    subprocess.run(["make", "-s", "-f", os.path.join(_HERE, "Makefile")], check=True)
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        dp = ctypes.POINTER(ctypes.c_double)
+        up = ctypes.POINTER(ctypes.c_uint32)
+        L.lbo_tau_eff.restype = ctypes.c_double
+        L.lbo_tau_eff.argtypes = [ctypes.c_double]
+        L.lbo_density.restype = ctypes.c_double
+        L.lbo_density.argtypes = [dp]
+        L.lbo_velocity.argtypes = [dp, ctypes.c_double, dp]
+        L.lbo_feq.argtypes = [ctypes.c_double, dp, dp]
+        L.lbo_bgk.argtypes = [dp, ctypes.c_double, dp]
+        L.lbo_moving_wall_term.restype = ctypes.c_double
+        L.lbo_moving_wall_term.argtypes = [dp, ctypes.c_int, ctypes.c_double]
+        L.lbo_build_cavity.argtypes = [ctypes.c_int, up, ctypes.POINTER(Params), ctypes.c_double]
+        L.lbo_build_words.argtypes = [ctypes.POINTER(Dims), ctypes.POINTER(ctypes.c_uint8), up]
+        L.lbo_step.argtypes = [ctypes.POINTER(Dims), up, ctypes.POINTER(Params), dp, dp, ctypes.c_int]
+        L.lbo_stream.argtypes = [ctypes.POINTER(Dims), up, dp, dp]
+        L.lbo_step_swap_aos.argtypes = [ctypes.POINTER(Dims), up, ctypes.POINTER(Params), dp]
+        L.lbo_fields.argtypes = [ctypes.POINTER(Dims), up, dp, dp, dp]
+        L.lbo_init_equilibrium.argtypes = [ctypes.POINTER(Dims), up, dp, dp, dp]
+        L.lbo_moments.argtypes = [ctypes.POINTER(Dims), up, dp, dp]
+        L.lbo_pressure_drop.restype = ctypes.c_double
+        L.lbo_pressure_drop.argtypes = [ctypes.POINTER(Dims), up, dp, ctypes.c_int, ctypes.c_int]
+        L.lbo_pressure_drop_grad.argtypes = [ctypes.POINTER(Dims), up, ctypes.c_int, ctypes.c_int, dp]
+        L.lbo_step_adjoint.argtypes = [ctypes.POINTER(Dims), up, ctypes.POINTER(Params), dp, dp, dp, dp,
+                                       ctypes.c_int]
+        _lib = L
+    return _lib
+
+
+def _dp(a):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _up(a):
+    assert a.dtype == np.uint32 and a.flags.c_contiguous
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))
+
+
+class Lattice:
+    """A dense D3Q19 lattice with SoA populations f[q, z, y, x] and per-cell words."""
+
+    def __init__(self, nx, ny, nz, periodic=(False, False, False)):
+        self.dims = Dims(nx, ny, nz, (ctypes.c_int * 3)(*[int(p) for p in periodic]))
+        self.shape = (nz, ny, nx)
+        self.n = nx * ny * nz
+        self.words = np.zeros(self.n, dtype=np.uint32)
+        self.params = Params()
+        self.params.tau = 1.0
+        self._alpha = None
+
+    # ----- geometry -----
+    @classmethod
+    def cavity(cls, n, tau, ulid):
+        lat = cls(n, n, n)
+        lib().lbo_build_cavity(n, _up(lat.words), ctypes.byref(lat.params), ulid)
+        lat.params.tau = tau
+        return lat
+
+    def set_solid(self, solid):
+        solid = np.ascontiguousarray(solid.reshape(-1), dtype=np.uint8)
+        lib().lbo_build_words(ctypes.byref(self.dims), solid.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
+                              _up(self.words))
+
+    def set_force(self, g):
+        self.params.use_force = 1
+        for k in range(3):
+            self.params.g[k] = float(g[k])
+
+    def set_alpha(self, alpha):
+        self._alpha = np.ascontiguousarray(alpha.reshape(-1), dtype=np.float64)
+        self.params.alpha = _dp(self._alpha)
+
+    @property
+    def fluid(self):
+        return (self.words & FLUID) != 0
+
+    # ----- state -----
+    def equilibrium(self, rho=None, u=None):
+        rho = np.ones(self.n) if rho is None else np.ascontiguousarray(rho.reshape(-1), dtype=np.float64)
+        u = np.zeros((self.n, 3)) if u is None else np.ascontiguousarray(u.reshape(-1, 3), dtype=np.float64)
+        f = np.empty(Q * self.n)
+        lib().lbo_init_equilibrium(ctypes.byref(self.dims), _up(self.words), _dp(rho), _dp(u), _dp(f))
+        return f
+
+    def step(self, f, steps=1, nthreads=1):
+        a = np.ascontiguousarray(f, dtype=np.float64).copy()
+        b = np.zeros_like(a)
+        b[:] = a  # non-fluid cells keep their values
+        for _ in range(steps):
+            lib().lbo_step(ctypes.byref(self.dims), _up(self.words), ctypes.byref(self.params), _dp(a), _dp(b),
+                           int(nthreads))
+            a, b = b, a
+        return a
+
+    def stream(self, f):
+        src = np.ascontiguousarray(f, dtype=np.float64)
+        dst = np.zeros_like(src)
+        lib().lbo_stream(ctypes.byref(self.dims), _up(self.words), _dp(src), _dp(dst))
+        return dst
+
+    def step_swap(self, f_soa, steps=1):
+        aos = np.ascontiguousarray(f_soa.reshape(Q, self.n).T)
+        for _ in range(steps):
+            lib().lbo_step_swap_aos(ctypes.byref(self.dims), _up(self.words), ctypes.byref(self.params), _dp(aos))
+        return np.ascontiguousarray(aos.T).reshape(-1)
+
+    def fields(self, f):
+        rho = np.empty(self.n)
+        u = np.empty((self.n, 3))
+        lib().lbo_fields(ctypes.byref(self.dims), _up(self.words), _dp(np.ascontiguousarray(f)), _dp(rho), _dp(u))
+        return rho, u
+
+    def moments(self, f):
+        out = np.empty(4)
+        lib().lbo_moments(ctypes.byref(self.dims), _up(self.words), _dp(np.ascontiguousarray(f)), _dp(out))
+        return out
+
+    def pressure_drop(self, f, xin, xout):
+        return lib().lbo_pressure_drop(ctypes.byref(self.dims), _up(self.words), _dp(np.ascontiguousarray(f)),
+                                       int(xin), int(xout))
+
+    def pressure_drop_grad(self, xin, xout):
+        lam = np.empty(Q * self.n)
+        lib().lbo_pressure_drop_grad(ctypes.byref(self.dims), _up(self.words), int(xin), int(xout), _dp(lam))
+        return lam
+
+    def step_adjoint(self, f_in, lam_out, galpha, nthreads=1):
+        lam_in = np.empty(Q * self.n)
+        lib().lbo_step_adjoint(ctypes.byref(self.dims), _up(self.words), ctypes.byref(self.params),
+                               _dp(np.ascontiguousarray(f_in)), _dp(np.ascontiguousarray(lam_out)), _dp(lam_in),
+                               _dp(galpha), int(nthreads))
+        return lam_in
+
+
+def tau_eff(tau):
+    return lib().lbo_tau_eff(float(tau))

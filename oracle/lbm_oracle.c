@@ -1,0 +1,449 @@
+/*
+ * CPU oracle for the D3Q19 BGK block-LB hot path (TEST INFRASTRUCTURE ONLY).
+ *
+ * Only tests/, __graft_entry__.smoke() and the cpu_baseline leg of bench.py use
+ * this file, and only as the checker.  See lbm_oracle.h for the pinning summary.
+ * Reference citations: LB/ = madpac/peano/applications/latticeboltzmann/,
+ * BLB/ = LB/blocklatticeboltzmann/, SLB/ = madpac/scenario/latticeboltzmann/blocklatticeboltzmann/.
+ *
+ * Build: gcc -O2 -ffp-contract=off -fopenmp -fPIC -shared (oracle/Makefile).
+ */
+#include "lbm_oracle.h"
+
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* D3Q19 velocities, z-major then y then x; rest population q = 9, inverse q' = 18 - q.
+ * LB/LatticeBoltzmannDefinitions.cpp:113-131. */
+const int LBO_C[LBO_Q][3] = {
+  { 0,-1,-1}, {-1, 0,-1}, { 0, 0,-1}, { 1, 0,-1}, { 0, 1,-1},
+  {-1,-1, 0}, { 0,-1, 0}, { 1,-1, 0}, {-1, 0, 0}, { 0, 0, 0},
+  { 1, 0, 0}, {-1, 1, 0}, { 0, 1, 0}, { 1, 1, 0}, { 0,-1, 1},
+  {-1, 0, 1}, { 0, 0, 1}, { 1, 0, 1}, { 0, 1, 1}};
+
+/* LB/LatticeBoltzmannDefinitions.cpp:149-151 */
+const double LBO_W[LBO_Q] = {
+  1.0/36.0, 1.0/36.0, 1.0/18.0, 1.0/36.0, 1.0/36.0,
+  1.0/36.0, 1.0/18.0, 1.0/36.0, 1.0/18.0, 1.0/3.0, 1.0/18.0, 1.0/36.0, 1.0/18.0, 1.0/36.0,
+  1.0/36.0, 1.0/36.0, 1.0/18.0, 1.0/36.0, 1.0/36.0};
+
+/* The reference keeps the velocities as doubles; products c*u are exact either way. */
+static double cd(int q, int d) { return (double)LBO_C[q][d]; }
+
+/* nu_L = 1.0/3.0*(tau-0.5) (LB/configurations/LatticeBoltzmannConfiguration.cpp:269);
+ * tau_eff = 3.0*nu_L + 0.5 (LB/MultiLevelSimData.h:160, level factor pow(3,0) = 1). */
+double lbo_tau_eff(double tau) {
+  double nu = 1.0 / 3.0 * (tau - 0.5);
+  double fac = 1.0;
+  nu = nu * fac;
+  return 3.0 * nu + 0.5;
+}
+
+/* BLB/BlockComputeCellValues.cpp:26-43: rho = sum_q f_q in q order, from 0.0. */
+double lbo_density(const double* f) {
+  double rho = 0.0;
+  for (int q = 0; q < LBO_Q; q++) rho += f[q];
+  return rho;
+}
+
+/* BLB/BlockComputeCellValues.cpp:47-68: buffer += f_q*c_q, then (1.0/rho)*buffer. */
+void lbo_velocity(const double* f, double rho, double* u) {
+  double b[3] = {0.0, 0.0, 0.0};
+  for (int q = 0; q < LBO_Q; q++)
+    for (int d = 0; d < 3; d++) b[d] += cd(q, d) * f[q];
+  double inv = 1.0 / rho;
+  for (int d = 0; d < 3; d++) u[d] = b[d] * inv;
+}
+
+/* tarch::la::dot (tarch/la/VectorVectorOperations.cpph:194-209): l0*r0 + l1*r1 + l2*r2 in order. */
+static double dot3(const double* a, const double* b) {
+  double r = a[0] * b[0];
+  r += a[1] * b[1];
+  r += a[2] * b[2];
+  return r;
+}
+
+/* BLB/BlockComputeCellValues.cpp:9-22 */
+void lbo_feq(double rho, const double* u, double* feq) {
+  for (int q = 0; q < LBO_Q; q++) {
+    double c[3] = {cd(q, 0), cd(q, 1), cd(q, 2)};
+    double cu = dot3(c, u);
+    feq[q] = LBO_W[q] * rho * (1.0 + cu / (LBO_CS2) + (cu * cu) / (2.0 * LBO_CS2 * LBO_CS2)
+                               - (dot3(u, u) / (2.0 * LBO_CS2)));
+  }
+}
+
+/* BLB/collisionmodels/BlockBGK.cpp:43: f -= 1.0/tau*(f - feq) */
+void lbo_bgk(double* f, double tau_eff, const double* feq) {
+  for (int q = 0; q < LBO_Q; q++) f[q] -= 1.0 / tau_eff * (f[q] - feq[q]);
+}
+
+/* BLB/boundaryconditions/BlockMovingWall.cpp:23-28, with q the outgoing direction:
+ * ciVel = u_w . c_q ; ciVel = ciVel*2.0*w_q*rho/cs2 ; f_q -= ciVel */
+double lbo_moving_wall_term(const double* uw, int q, double rho) {
+  double c[3] = {cd(q, 0), cd(q, 1), cd(q, 2)};
+  double civel = dot3(uw, c);
+  civel = civel * 2.0 * LBO_W[q] * rho / (LBO_CS2);
+  return civel;
+}
+
+/* Collision of one fluid cell, in place on f[19].  Without force and porosity this is
+ * the reference sequence computeDensity -> computeVelocity -> BGK (mapping
+ * BLB/mappings/RegularGrid2RegularBlockSolver.cpp:126-137).  With a body acceleration g
+ * it applies Guo forcing (equilibrium velocity u + g/2, source (1-1/(2 tau)) w rho
+ * [(c-u).g/cs2 + (c.u)(c.g)/cs2^2]); the reference's only forcing routine
+ * (madpac/peanocoupling/PeanoDataExchangeService.cpp:161-186) squares c.u in the second
+ * term, which is not Guo's scheme, so the published scheme is used here.  With a
+ * porosity alpha the equilibrium velocity is scaled by (1-alpha) (Brinkman penalisation). */
+static double collide_cell(double* f, double tau_eff, const lbo_params* p, double alpha) {
+  double rho = lbo_density(f);
+  double u[3];
+  lbo_velocity(f, rho, u);
+  if (p->use_force) {
+    for (int d = 0; d < 3; d++) u[d] = u[d] + 0.5 * p->g[d];
+  }
+  if (p->alpha) {
+    for (int d = 0; d < 3; d++) u[d] = (1.0 - alpha) * u[d];
+  }
+  double feq[LBO_Q];
+  lbo_feq(rho, u, feq);
+  lbo_bgk(f, tau_eff, feq);
+  if (p->use_force) {
+    double pref = (1.0 - 0.5 / tau_eff) * rho;
+    for (int q = 0; q < LBO_Q; q++) {
+      double c[3] = {cd(q, 0), cd(q, 1), cd(q, 2)};
+      double cmu[3] = {c[0] - u[0], c[1] - u[1], c[2] - u[2]};
+      double cu = dot3(c, u);
+      f[q] += pref * LBO_W[q] * (dot3(cmu, p->g) / LBO_CS2 + cu * dot3(c, p->g) / (LBO_CS2 * LBO_CS2));
+    }
+  }
+  return rho;
+}
+
+static int64_t ncells(const lbo_dims* d) { return (int64_t)d->nx * d->ny * d->nz; }
+
+/* neighbour of (x,y,z) along c_q, or -1 if it leaves a non-periodic domain */
+static int64_t neighbour(const lbo_dims* d, int x, int y, int z, int q) {
+  int n[3] = {x + LBO_C[q][0], y + LBO_C[q][1], z + LBO_C[q][2]};
+  int e[3] = {d->nx, d->ny, d->nz};
+  for (int k = 0; k < 3; k++) {
+    if (n[k] < 0 || n[k] >= e[k]) {
+      if (!d->periodic[k]) return -1;
+      n[k] = (n[k] + e[k]) % e[k];
+    }
+  }
+  return (int64_t)n[0] + (int64_t)d->nx * ((int64_t)n[1] + (int64_t)d->ny * n[2]);
+}
+
+void lbo_build_words(const lbo_dims* d, const uint8_t* solid, uint32_t* words) {
+  for (int z = 0; z < d->nz; z++)
+    for (int y = 0; y < d->ny; y++)
+      for (int x = 0; x < d->nx; x++) {
+        int64_t i = (int64_t)x + (int64_t)d->nx * ((int64_t)y + (int64_t)d->ny * z);
+        if (solid[i]) { words[i] = 0; continue; }
+        uint32_t w = LBO_FLUID;
+        for (int q = 0; q < LBO_Q; q++) {
+          if (q == LBO_REST) continue;
+          int64_t j = neighbour(d, x, y, z, q);
+          if (j < 0 || solid[j]) w |= (1u << q);
+        }
+        words[i] = w;
+      }
+}
+
+/* Lid-driven cavity (SLB/BoundaryData4LidDrivenCavity.cpp:17-55 with the link scan of
+ * SLB/services/ReceiveBoundaryDataService.cpp:107-185): every fluid cell of the n^3 box;
+ * a link leaving the box is a moving wall iff c_q,z > 0 and the cell is in the top layer
+ * (its neighbour centre lies above the lid), including edge and corner links; all other
+ * leaving links are halfway bounce-back.  Class 1 carries u_w = (ulid, 0, 0). */
+void lbo_build_cavity(int n, uint32_t* words, lbo_params* p, double ulid) {
+  lbo_dims d = {n, n, n, {0, 0, 0}};
+  uint8_t* solid = (uint8_t*)calloc((size_t)n * n * n, 1);
+  lbo_build_words(&d, solid, words);
+  free(solid);
+  uint32_t lidmask = 0;
+  for (int q = 0; q < LBO_Q; q++)
+    if (LBO_C[q][2] > 0) lidmask |= 1u << q;
+  for (int y = 0; y < n; y++)
+    for (int x = 0; x < n; x++) {
+      int64_t i = (int64_t)x + (int64_t)n * ((int64_t)y + (int64_t)n * (n - 1));
+      words[i] |= 1u << LBO_CLASS_SHIFT;
+    }
+  memset(p->uw, 0, sizeof(p->uw));
+  memset(p->moving_mask, 0, sizeof(p->moving_mask));
+  p->uw[1][0] = ulid;
+  p->moving_mask[1] = lidmask;
+}
+
+/* One step in push form.  Equivalent to the reference's swap streaming
+ * (BLB/nonlocaltreatment/AABlockStreaming.cpp:19-92): f*_q(x) goes to x+c_q when that cell
+ * is fluid, otherwise it is reflected into q' = 18-q of x (halfway bounce-back is implicit,
+ * BLB/boundaryconditions/BlockHalfwayBounceBack.cpp:20-22), after the moving-wall term
+ * (BLB/boundaryconditions/BlockMovingWall.cpp:23-28) for moving links.  Non-fluid cells are
+ * neither collided nor written (BLB/collisionmodels/CollisionModelWrapper.cpp:36). */
+void lbo_step(const lbo_dims* d, const uint32_t* words, const lbo_params* p,
+              const double* src, double* dst, int nthreads) {
+  const int64_t N = ncells(d);
+  const double tau_eff = lbo_tau_eff(p->tau);
+#ifdef _OPENMP
+  if (nthreads < 1) nthreads = 1;
+  #pragma omp parallel for num_threads(nthreads) schedule(static) collapse(2)
+#endif
+  for (int z = 0; z < d->nz; z++)
+    for (int y = 0; y < d->ny; y++) {
+      for (int x = 0; x < d->nx; x++) {
+        int64_t i = (int64_t)x + (int64_t)d->nx * ((int64_t)y + (int64_t)d->ny * z);
+        uint32_t w = words[i];
+        if (!(w & LBO_FLUID)) continue;
+        double f[LBO_Q];
+        for (int q = 0; q < LBO_Q; q++) f[q] = src[(int64_t)q * N + i];
+        double alpha = p->alpha ? p->alpha[i] : 0.0;
+        double rho = collide_cell(f, tau_eff, p, alpha);
+        unsigned cls = (w >> LBO_CLASS_SHIFT) & LBO_CLASS_MASK;
+        for (int q = 0; q < LBO_Q; q++) {
+          if (w & (1u << q)) {
+            double v = f[q];
+            if (cls && ((p->moving_mask[cls] >> q) & 1u)) v -= lbo_moving_wall_term(p->uw[cls], q, rho);
+            dst[(int64_t)(LBO_Q - 1 - q) * N + i] = v;
+          } else {
+            int64_t j = neighbour(d, x, y, z, q);
+            dst[(int64_t)q * N + j] = f[q];
+          }
+        }
+      }
+    }
+}
+
+void lbo_stream(const lbo_dims* d, const uint32_t* words, const double* src, double* dst) {
+  const int64_t N = ncells(d);
+  for (int z = 0; z < d->nz; z++)
+    for (int y = 0; y < d->ny; y++)
+      for (int x = 0; x < d->nx; x++) {
+        int64_t i = (int64_t)x + (int64_t)d->nx * ((int64_t)y + (int64_t)d->ny * z);
+        uint32_t w = words[i];
+        if (!(w & LBO_FLUID)) continue;
+        for (int q = 0; q < LBO_Q; q++) {
+          if (w & (1u << q)) dst[(int64_t)(LBO_Q - 1 - q) * N + i] = src[(int64_t)q * N + i];
+          else dst[(int64_t)q * N + neighbour(d, x, y, z, q)] = src[(int64_t)q * N + i];
+        }
+      }
+}
+
+/* The reference's streaming exactly as its three phases do it, on a dense AoS lattice:
+ * collide every fluid cell (phase A, mapping :126-147), swap f[x][q] <-> f[x+c_q][q'] for
+ * each fluid-fluid link once (phases A/B, AABlockStreaming.cpp:19-53, link enumerated
+ * from the cell whose partner has not claimed it, :111-186), then swap q <-> q' inside
+ * every cell (phase C, :80-92). */
+void lbo_step_swap_aos(const lbo_dims* d, const uint32_t* words, const lbo_params* p, double* f) {
+  const int64_t N = ncells(d);
+  const double tau_eff = lbo_tau_eff(p->tau);
+  for (int64_t i = 0; i < N; i++) {
+    uint32_t w = words[i];
+    if (!(w & LBO_FLUID)) continue;
+    double* fi = f + i * LBO_Q;
+    double alpha = p->alpha ? p->alpha[i] : 0.0;
+    double rho = collide_cell(fi, tau_eff, p, alpha);
+    unsigned cls = (w >> LBO_CLASS_SHIFT) & LBO_CLASS_MASK;
+    if (cls)
+      for (int q = 0; q < LBO_Q; q++)
+        if ((w & (1u << q)) && ((p->moving_mask[cls] >> q) & 1u))
+          fi[q] -= lbo_moving_wall_term(p->uw[cls], q, rho);
+  }
+  for (int z = 0; z < d->nz; z++)
+    for (int y = 0; y < d->ny; y++)
+      for (int x = 0; x < d->nx; x++) {
+        int64_t i = (int64_t)x + (int64_t)d->nx * ((int64_t)y + (int64_t)d->ny * z);
+        if (!(words[i] & LBO_FLUID)) continue;
+        /* directions q > 9 point "forward"; each link is swapped once from its lower end */
+        for (int q = LBO_REST + 1; q < LBO_Q; q++) {
+          if (words[i] & (1u << q)) continue;
+          int64_t j = neighbour(d, x, y, z, q);
+          double t = f[i * LBO_Q + q];
+          f[i * LBO_Q + q] = f[j * LBO_Q + (LBO_Q - 1 - q)];
+          f[j * LBO_Q + (LBO_Q - 1 - q)] = t;
+        }
+      }
+  for (int64_t i = 0; i < N; i++)
+    for (int q = 0; q < LBO_Q / 2; q++) {
+      double t = f[i * LBO_Q + q];
+      f[i * LBO_Q + q] = f[i * LBO_Q + (LBO_Q - 1 - q)];
+      f[i * LBO_Q + (LBO_Q - 1 - q)] = t;
+    }
+}
+
+void lbo_fields(const lbo_dims* d, const uint32_t* words, const double* f, double* rho, double* u) {
+  const int64_t N = ncells(d);
+  for (int64_t i = 0; i < N; i++) {
+    if (!(words[i] & LBO_FLUID)) {
+      rho[i] = 0.0;
+      u[3 * i] = u[3 * i + 1] = u[3 * i + 2] = 0.0;
+      continue;
+    }
+    double fi[LBO_Q];
+    for (int q = 0; q < LBO_Q; q++) fi[q] = f[(int64_t)q * N + i];
+    rho[i] = lbo_density(fi);
+    lbo_velocity(fi, rho[i], u + 3 * i);
+  }
+}
+
+/* BLB/services/GridManagementService.cpp:121-148 initialises f = w (rho = 1, u = 0) in
+ * fluid cells and 0 elsewhere; this generalises it to f = feq(rho, u). */
+void lbo_init_equilibrium(const lbo_dims* d, const uint32_t* words, const double* rho,
+                          const double* u, double* f) {
+  const int64_t N = ncells(d);
+  for (int64_t i = 0; i < N; i++) {
+    double feq[LBO_Q];
+    if (words[i] & LBO_FLUID) {
+      lbo_feq(rho[i], u + 3 * i, feq);
+    } else {
+      for (int q = 0; q < LBO_Q; q++) feq[q] = 0.0;
+    }
+    for (int q = 0; q < LBO_Q; q++) f[(int64_t)q * N + i] = feq[q];
+  }
+}
+
+void lbo_moments(const lbo_dims* d, const uint32_t* words, const double* f, double out[4]) {
+  const int64_t N = ncells(d);
+  double m[4] = {0, 0, 0, 0};
+  for (int64_t i = 0; i < N; i++) {
+    if (!(words[i] & LBO_FLUID)) continue;
+    for (int q = 0; q < LBO_Q; q++) {
+      double v = f[(int64_t)q * N + i];
+      m[0] += v;
+      m[1] += cd(q, 0) * v;
+      m[2] += cd(q, 1) * v;
+      m[3] += cd(q, 2) * v;
+    }
+  }
+  for (int k = 0; k < 4; k++) out[k] = m[k];
+}
+
+double lbo_pressure_drop(const lbo_dims* d, const uint32_t* words, const double* f, int xin, int xout) {
+  const int64_t N = ncells(d);
+  double s[2] = {0, 0};
+  int64_t n[2] = {0, 0};
+  int xs[2] = {xin, xout};
+  for (int k = 0; k < 2; k++)
+    for (int z = 0; z < d->nz; z++)
+      for (int y = 0; y < d->ny; y++) {
+        int64_t i = (int64_t)xs[k] + (int64_t)d->nx * ((int64_t)y + (int64_t)d->ny * z);
+        if (!(words[i] & LBO_FLUID)) continue;
+        double r = 0.0;
+        for (int q = 0; q < LBO_Q; q++) r += f[(int64_t)q * N + i];
+        s[k] += r;
+        n[k]++;
+      }
+  if (n[0] == 0 || n[1] == 0) return 0.0;
+  return LBO_CS2 * (s[0] / (double)n[0] - s[1] / (double)n[1]);
+}
+
+void lbo_pressure_drop_grad(const lbo_dims* d, const uint32_t* words, int xin, int xout, double* lam) {
+  const int64_t N = ncells(d);
+  memset(lam, 0, sizeof(double) * (size_t)N * LBO_Q);
+  int xs[2] = {xin, xout};
+  for (int k = 0; k < 2; k++) {
+    int64_t n = 0;
+    for (int z = 0; z < d->nz; z++)
+      for (int y = 0; y < d->ny; y++) {
+        int64_t i = (int64_t)xs[k] + (int64_t)d->nx * ((int64_t)y + (int64_t)d->ny * z);
+        if (words[i] & LBO_FLUID) n++;
+      }
+    if (n == 0) continue;
+    double v = (k == 0 ? LBO_CS2 : -LBO_CS2) / (double)n;
+    for (int z = 0; z < d->nz; z++)
+      for (int y = 0; y < d->ny; y++) {
+        int64_t i = (int64_t)xs[k] + (int64_t)d->nx * ((int64_t)y + (int64_t)d->ny * z);
+        if (!(words[i] & LBO_FLUID)) continue;
+        for (int q = 0; q < LBO_Q; q++) lam[(int64_t)q * N + i] += v;
+      }
+  }
+}
+
+/* Reverse mode of lbo_step (a hand-derived tape of one step; no reference counterpart).
+ * Forward per fluid cell: rho = sum f, j = sum c f, u = j/rho, v = (1-a)(u + g/2),
+ * post_q = f_q - w(f_q - feq_q(rho,v)) + S_q(rho,v), then push/bounce with the
+ * moving-wall term -2 w_q rho (c_q.uw)/cs2 on moving links.
+ * Reverse: mu_q = adjoint of post_q (gathered from where post_q was pushed),
+ * Phi(rho,v) = sum_k mu_k (w feq_k + S_k), lam_q = (1-w) mu_q + dPhi/drho
+ * + sum_d dPhi/dv_d (1-a)(c_qd - u_d)/rho - sum_moving mu_wall * 2 w rho' ...,
+ * dJ/da = -sum_d dPhi/dv_d (u_d + g_d/2). */
+void lbo_step_adjoint(const lbo_dims* d, const uint32_t* words, const lbo_params* p,
+                      const double* src, const double* lam_out, double* lam_in, double* galpha,
+                      int nthreads) {
+  const int64_t N = ncells(d);
+  const double tau_eff = lbo_tau_eff(p->tau);
+  const double om = 1.0 / tau_eff;
+#ifdef _OPENMP
+  if (nthreads < 1) nthreads = 1;
+  #pragma omp parallel for num_threads(nthreads) schedule(static) collapse(2)
+#endif
+  for (int z = 0; z < d->nz; z++)
+    for (int y = 0; y < d->ny; y++) {
+      for (int x = 0; x < d->nx; x++) {
+        int64_t i = (int64_t)x + (int64_t)d->nx * ((int64_t)y + (int64_t)d->ny * z);
+        uint32_t w = words[i];
+        if (!(w & LBO_FLUID)) {
+          for (int q = 0; q < LBO_Q; q++) lam_in[(int64_t)q * N + i] = 0.0;
+          continue;
+        }
+        double f[LBO_Q], mu[LBO_Q];
+        for (int q = 0; q < LBO_Q; q++) f[q] = src[(int64_t)q * N + i];
+        unsigned cls = (w >> LBO_CLASS_SHIFT) & LBO_CLASS_MASK;
+        double drho_wall = 0.0;
+        for (int q = 0; q < LBO_Q; q++) {
+          if (w & (1u << q)) {
+            mu[q] = lam_out[(int64_t)(LBO_Q - 1 - q) * N + i];
+            if (cls && ((p->moving_mask[cls] >> q) & 1u))
+              drho_wall -= mu[q] * lbo_moving_wall_term(p->uw[cls], q, 1.0);
+          } else {
+            int64_t j = neighbour(d, x, y, z, q);
+            mu[q] = lam_out[(int64_t)q * N + j];
+          }
+        }
+        double rho = 0.0, j3[3] = {0, 0, 0};
+        for (int q = 0; q < LBO_Q; q++) {
+          rho += f[q];
+          for (int k = 0; k < 3; k++) j3[k] += cd(q, k) * f[q];
+        }
+        double u[3], g[3] = {0, 0, 0};
+        if (p->use_force) for (int k = 0; k < 3; k++) g[k] = p->g[k];
+        double a = p->alpha ? p->alpha[i] : 0.0;
+        double s = p->alpha ? (1.0 - a) : 1.0;
+        double v[3];
+        for (int k = 0; k < 3; k++) {
+          u[k] = j3[k] / rho;
+          v[k] = s * (u[k] + 0.5 * g[k]);
+        }
+        double vv = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+        double fpref = p->use_force ? (1.0 - 0.5 * om) : 0.0;
+        double dPhi_drho = 0.0, dPhi_dv[3] = {0, 0, 0};
+        for (int q = 0; q < LBO_Q; q++) {
+          double c[3] = {cd(q, 0), cd(q, 1), cd(q, 2)};
+          double cv = c[0] * v[0] + c[1] * v[1] + c[2] * v[2];
+          double cg = c[0] * g[0] + c[1] * g[1] + c[2] * g[2];
+          double vg = v[0] * g[0] + v[1] * g[1] + v[2] * g[2];
+          double feq_over_rho = LBO_W[q] * (1.0 + 3.0 * cv + 4.5 * cv * cv - 1.5 * vv);
+          double S_over_rho = fpref * LBO_W[q] * (3.0 * (cg - vg) + 9.0 * cv * cg);
+          dPhi_drho += mu[q] * (om * feq_over_rho + S_over_rho);
+          for (int k = 0; k < 3; k++) {
+            double dfeq = LBO_W[q] * rho * (3.0 * c[k] + 9.0 * cv * c[k] - 3.0 * v[k]);
+            double dS = fpref * LBO_W[q] * rho * (-3.0 * g[k] + 9.0 * c[k] * cg);
+            dPhi_dv[k] += mu[q] * (om * dfeq + dS);
+          }
+        }
+        for (int q = 0; q < LBO_Q; q++) {
+          double l = (1.0 - om) * mu[q] + dPhi_drho + drho_wall;
+          for (int k = 0; k < 3; k++) l += dPhi_dv[k] * s * (cd(q, k) - u[k]) / rho;
+          lam_in[(int64_t)q * N + i] = l;
+        }
+        if (p->alpha && galpha) {
+          double ga = 0.0;
+          for (int k = 0; k < 3; k++) ga -= dPhi_dv[k] * (u[k] + 0.5 * g[k]);
+          galpha[i] += ga;
+        }
+      }
+    }
+}

@@ -1,0 +1,106 @@
+/*
+ * CPU oracle for the D3Q19 BGK block-LB hot path (TEST INFRASTRUCTURE ONLY).
+ *
+ * This is a plain-C restatement of the reference Peano block-LB arithmetic, used
+ * only as the checker by tests/, __graft_entry__.smoke() and the cpu_baseline leg
+ * of bench.py.  The product path (madpac_amd/) never links, loads or calls it.
+ *
+ * Every function reproduces the reference operation order so that the FP64 result
+ * is bit-identical to the reference on x86-64 (no FMA contraction: build with
+ * -ffp-contract=off).  Citations are relative to
+ * madpac/peano/applications/latticeboltzmann/ (LB/) and its blocklatticeboltzmann/
+ * subfolder (BLB/).
+ *
+ * Pinning (see tests/test_oracle.py): the lattice tables, moments, equilibrium and
+ * BGK relaxation are pinned by closed-form identities; the full step (collide,
+ * swap streaming with implicit halfway bounce-back, moving wall) is pinned
+ * bit-for-bit against fixtures produced by the compiled reference
+ * (tests/golden/); Guo forcing, Brinkman porosity, periodic wrap and the adjoint
+ * step go beyond the reference and are pinned by analytic solutions (Poiseuille
+ * flow, Taylor-Green decay), conservation laws and central finite differences.
+ */
+#ifndef LBM_ORACLE_H
+#define LBM_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LBO_Q 19
+#define LBO_REST 9
+/* cs^2 exactly as the reference spells it (LB/LatticeBoltzmannDefinitions.h:89). */
+#define LBO_CS2 (1.0 / 3.0)
+
+/* Cell word: bit 31 fluid, bit q (0..18) = neighbour x+c_q is not fluid (bounce-back),
+ * bits 24..30 = moving-wall class (0 = none). */
+#define LBO_FLUID 0x80000000u
+#define LBO_CLASS_SHIFT 24
+#define LBO_CLASS_MASK 0x7Fu
+#define LBO_MAX_CLASSES 8
+
+extern const int LBO_C[LBO_Q][3];
+extern const double LBO_W[LBO_Q];
+
+typedef struct {
+  int nx, ny, nz;
+  int periodic[3];
+} lbo_dims;
+
+typedef struct {
+  double tau;                              /* configured tau (LB/configurations/...:269) */
+  double uw[LBO_MAX_CLASSES][3];           /* wall velocity per moving-wall class */
+  uint32_t moving_mask[LBO_MAX_CLASSES];   /* bit q: outgoing link q of a class-k cell is a moving wall */
+  double g[3];                             /* body acceleration (Guo forcing); 0 = off */
+  int use_force;
+  const double* alpha;                     /* per-cell Brinkman porosity or NULL */
+} lbo_params;
+
+/* a1/a8 */
+double lbo_tau_eff(double tau);
+/* a4-a7 (per cell, AoS f[19]) */
+double lbo_density(const double* f);
+void lbo_velocity(const double* f, double rho, double* u);
+void lbo_feq(double rho, const double* u, double* feq);
+void lbo_bgk(double* f, double tau_eff, const double* feq);
+/* a13: the amount subtracted from outgoing population q of a moving-wall link */
+double lbo_moving_wall_term(const double* uw, int q, double rho);
+
+/* geometry: lid-driven cavity of n^3 fluid cells, lid on +z moving with (ulid,0,0). */
+void lbo_build_cavity(int n, uint32_t* words, lbo_params* p, double ulid);
+/* generic words from a solid mask (1 = solid); moving-wall classes left at 0. */
+void lbo_build_words(const lbo_dims* d, const uint8_t* solid, uint32_t* words);
+
+/* one time step, SoA layout f[q*N + cell], push form of the reference swap streaming. */
+void lbo_step(const lbo_dims* d, const uint32_t* words, const lbo_params* p,
+              const double* src, double* dst, int nthreads);
+/* the reference's own streaming algorithm on a dense lattice (collide, pair swaps
+ * along fluid-fluid links, then in-cell swap q <-> 18-q), in place on AoS f[cell*19+q]. */
+void lbo_step_swap_aos(const lbo_dims* d, const uint32_t* words, const lbo_params* p, double* f);
+
+/* pure streaming (no collision) in push form with implicit halfway bounce-back */
+void lbo_stream(const lbo_dims* d, const uint32_t* words, const double* src, double* dst);
+
+/* fields: rho, u (u = 0 and rho untouched = 0 on non-fluid cells) */
+void lbo_fields(const lbo_dims* d, const uint32_t* words, const double* f, double* rho, double* u);
+/* equilibrium initialisation from rho/u fields (non-fluid cells: 0) */
+void lbo_init_equilibrium(const lbo_dims* d, const uint32_t* words, const double* rho,
+                          const double* u, double* f);
+/* global mass and momentum over fluid cells (Kahan-free plain double sums, cell order) */
+void lbo_moments(const lbo_dims* d, const uint32_t* words, const double* f, double out[4]);
+
+/* pressure drop J = cs^2 (<rho>_{x=xin} - <rho>_{x=xout}) over fluid cells of two planes */
+double lbo_pressure_drop(const lbo_dims* d, const uint32_t* words, const double* f, int xin, int xout);
+/* reverse-mode (adjoint) of lbo_step: given src (the step's input state) and lam_out =
+ * dJ/d(dst), returns lam_in = dJ/d(src) and accumulates dJ/dalpha into galpha. */
+void lbo_step_adjoint(const lbo_dims* d, const uint32_t* words, const lbo_params* p,
+                      const double* src, const double* lam_out, double* lam_in, double* galpha,
+                      int nthreads);
+/* dJ/df of the pressure drop */
+void lbo_pressure_drop_grad(const lbo_dims* d, const uint32_t* words, int xin, int xout, double* lam);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,185 @@
+"""Generate the golden fixtures from the compiled reference (run in the build container only).
+
+The reference Peano block-LB is compiled from its own sources by ``oracle/ref/Makefile``
+into ``oracle/_ref/`` (never committed, never shipped).  This script runs it and stores
+its outputs as small ``.npz`` fixtures next to this file:
+
+* ``cellkernels_s{seed}.npz`` -- one 6^3 block of seeded non-equilibrium populations
+  through the reference's computeDensity / computeVelocity / computeFeq / BGK collide /
+  moving wall / local swap streaming (``oracle/ref/cellkernels.cpp``).
+* ``cavity_n{N}_t{steps}.npz`` -- the full FP64 population field of the lid-driven cavity
+  (N^3 fluid cells) after ``steps`` time steps of the reference time-step driver
+  (``oracle/ref/harness.cpp``), as SoA f[q, z, y, x] over the fluid box.
+* ``cavity_n64_t1000.npz`` -- the 64^3 cavity (tau 0.6, lid 0.05) after 1000 steps:
+  SHA-256 of the full SoA population field, the rho/u mid-planes and centre lines, and
+  global mass and momentum.
+
+Usage:  python tests/golden/make_fixtures.py [--skip-64]
+"""
+from __future__ import annotations
+
+import argparse
+import hashlib
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.abspath(os.path.join(HERE, "..", ".."))
+REF_DIR = os.path.join(ROOT, "oracle", "_ref")
+HARNESS = os.path.join(REF_DIR, "peano_lb_harness")
+CELLK = os.path.join(REF_DIR, "peano_lb_cellkernels")
+
+# Cavity XML for the reference's regular-grid block-LB batch job.  Tags and attributes are the
+# ones its configuration parsers read (blocklatticeboltzmann/configurations/
+# BlockLatticeBoltzmannBatchJobConfigurationForRegularGrid.cpp:41-91).
+XML = """<?xml version="1.0"?>
+<peano-configuration>
+  <run-blocklatticeboltzmann-on-regular-grid>
+    <log-output column-separator=" " log-time-stamp="no" log-time-stamp-human-readable="no" log-machine-name="no" log-message-type="yes" log-trace="yes" />
+    <regular-grid number-of-vertices="{n};{n};{n}" domain-size="1.0;1.0;1.0" domain-offset="0.0;0.0;0.0" />
+    <lattice-boltzmann-configuration tau="{tau!r}" dt="1.0" density="1.0" number-of-timesteps="{steps}" measure-performance="no" />
+    <lattice-boltzmann-scenario scenario="lid-driven-cavity" characteristic-velocity_L="{ulid!r}" dynamic-geometry="no" compute-forces="no"></lattice-boltzmann-scenario>
+    <collision-model name="BGK" hMin="0.00001" hMax="10.0" />
+    <builtin-geometry-configuration>
+      <hexahedron geometry-id="0" invert="false" width="1.0;1.0;1.0" offset="0.0;0.0;0.0" />
+    </builtin-geometry-configuration>
+  </run-blocklatticeboltzmann-on-regular-grid>
+</peano-configuration>
+"""
+
+
+def blocks_per_dim(n: int) -> int:
+    # BLB/runners/BlockLatticeBoltzmannBatchJobRunnerForRegularGrid.cpp:78-92
+    b = n // 6 + 1
+    if n % 6:
+        b += 1
+    return b
+
+
+def read_dump(path):
+    raw = open(path, "rb").read()
+    hdr = np.frombuffer(raw[:16], dtype="<i4")
+    assert hdr[0] == 0x4C42524D and hdr[2] == 216 and hdr[3] == 19
+    nb = int(hdr[1])
+    rec = 216 * 2 + 8 * (216 + 648 + 4104)
+    out = []
+    for k in range(nb):
+        o = 16 + k * rec
+        inner = np.frombuffer(raw[o:o + 216], dtype=np.uint8)
+        f = np.frombuffer(raw[o + 432 + 8 * 864:o + rec], dtype="<f8")
+        out.append((inner, f))
+    return out
+
+
+def run_cavity(n, tau, ulid, steps):
+    """Run the reference and return the SoA populations over the n^3 fluid box."""
+    nb = blocks_per_dim(n)
+    with tempfile.TemporaryDirectory() as td:
+        xml = os.path.join(td, "cavity.xml")
+        with open(xml, "w") as fh:
+            fh.write(XML.format(n=n, tau=tau, ulid=ulid, steps=steps))
+        out = os.path.join(td, "dump.bin")
+        subprocess.run([HARNESS, xml, str(nb ** 3), out], check=True, cwd=td,
+                       stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+        blocks = read_dump(out)
+    s = 6 * nb
+    dense = np.zeros((s, s, s, 19))
+    inner = np.zeros((s, s, s), np.uint8)
+    # blocks are created in lexicographic vertex order, x fastest
+    for k, (inn, f) in enumerate(blocks):
+        bx, by, bz = k % nb, (k // nb) % nb, k // (nb * nb)
+        sl = (slice(6 * bz, 6 * bz + 6), slice(6 * by, 6 * by + 6), slice(6 * bx, 6 * bx + 6))
+        dense[sl] = f.reshape(6, 6, 6, 19)
+        inner[sl] = inn.reshape(6, 6, 6)
+    # fluid cell centres (s - 2.5) dx lie in [0, 1] for slots s = 3 .. n+2
+    box = (slice(3, 3 + n),) * 3
+    assert inner[box].all() and int(inner.sum()) == n ** 3, "unexpected cavity flag layout"
+    outside = np.where(inner[..., None] == 0, dense, 0.0)
+    assert not outside.any(), "non-fluid cells must keep zero populations"
+    return np.ascontiguousarray(np.moveaxis(dense[box], -1, 0))  # [q, z, y, x]
+
+
+def cellkernel_fixture(seed, tau):
+    with tempfile.TemporaryDirectory() as td:
+        out = os.path.join(td, "ck.bin")
+        subprocess.run([CELLK, str(seed), repr(tau), out], check=True)
+        raw = open(out, "rb").read()
+    o = 0
+
+    def take(dt, n):
+        nonlocal o
+        a = np.frombuffer(raw[o:o + np.dtype(dt).itemsize * n], dtype=dt)
+        o += np.dtype(dt).itemsize * n
+        return a.copy()
+
+    magic, sd = take("<i4", 2)
+    assert magic == 0x4C42434B and sd == seed
+    d = {"seed": np.int64(seed), "tau": np.float64(tau)}
+    d["tau_eff"] = take("<f8", 1)[0]
+    d["uw"] = take("<f8", 3)
+    d["inner"] = take(np.uint8, 216)
+    d["pdf_in"] = take("<f8", 216 * 19)
+    d["density"] = take("<f8", 216)
+    d["velocity"] = take("<f8", 648)
+    d["feq"] = take("<f8", 216 * 19)
+    d["pdf_collided"] = take("<f8", 216 * 19)
+    nl = int(take("<i4", 1)[0])
+    d["wall_links"] = take("<i4", 2 * nl).reshape(nl, 2)
+    d["pdf_after_wall"] = take("<f8", 216 * 19)
+    d["pdf_after_local_stream"] = take("<f8", 216 * 19)
+    assert o == len(raw)
+    return d
+
+
+def field_digest(f_soa):
+    return hashlib.sha256(np.ascontiguousarray(f_soa, dtype="<f8").tobytes()).hexdigest()
+
+
+def moments(f):
+    from_c = np.array([[0, -1, -1], [-1, 0, -1], [0, 0, -1], [1, 0, -1], [0, 1, -1], [-1, -1, 0], [0, -1, 0],
+                       [1, -1, 0], [-1, 0, 0], [0, 0, 0], [1, 0, 0], [-1, 1, 0], [0, 1, 0], [1, 1, 0],
+                       [0, -1, 1], [-1, 0, 1], [0, 0, 1], [1, 0, 1], [0, 1, 1]], dtype=np.float64)
+    rho = f.sum(axis=0)
+    j = np.tensordot(from_c.T, f, axes=([1], [0]))
+    return rho, j / rho
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--skip-64", action="store_true")
+    args = ap.parse_args()
+    subprocess.run(["make", "-s", "-f", os.path.join(ROOT, "oracle", "ref", "Makefile"), "-j8"], check=True)
+
+    for seed, tau in ((1, 0.6), (2, 0.93)):
+        d = cellkernel_fixture(seed, tau)
+        np.savez_compressed(os.path.join(HERE, f"cellkernels_s{seed}.npz"), **d)
+        print("cellkernels", seed, file=sys.stderr)
+
+    for n, tau, ulid, steps in ((8, 0.6, 0.05, 1), (8, 0.6, 0.05, 10), (8, 0.6, 0.05, 100),
+                                (10, 0.8, 0.1, 37), (13, 0.55, 0.08, 5)):
+        f = run_cavity(n, tau, ulid, steps)
+        np.savez_compressed(os.path.join(HERE, f"cavity_n{n}_t{steps}.npz"), f=f, n=np.int64(n),
+                            tau=np.float64(tau), ulid=np.float64(ulid), steps=np.int64(steps))
+        print("cavity", n, steps, file=sys.stderr)
+
+    if not args.skip_64:
+        n, tau, ulid, steps = 64, 0.6, 0.05, 1000
+        f = run_cavity(n, tau, ulid, steps)
+        rho, u = moments(f)
+        h = n // 2
+        np.savez_compressed(
+            os.path.join(HERE, f"cavity_n{n}_t{steps}.npz"),
+            n=np.int64(n), tau=np.float64(tau), ulid=np.float64(ulid), steps=np.int64(steps),
+            sha256=np.array(field_digest(f)),
+            rho_mid_y=rho[:, h, :], u_mid_y=u[:, :, h, :],
+            rho_line_z=rho[:, h, h], u_line_z=u[:, :, h, h], u_line_x=u[:, h, h, :],
+            mass=np.float64(f.sum()))
+        print("cavity", n, steps, file=sys.stderr)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,107 @@
+"""GPU adjoint (reverse collide+stream with revolve checkpointing) against the FP64 CPU tape and
+central finite differences."""
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+lbm = pytest.importorskip("madpac_amd.lbm")
+
+
+def _channel(nx, ny, nz, seed):
+    solid = np.zeros((nz, ny, nx), bool)
+    solid[0], solid[-1], solid[:, 0], solid[:, -1] = True, True, True, True
+    rng = np.random.default_rng(seed)
+    alpha = np.clip(rng.random(nx * ny * nz) * 0.6, 0, 1)
+    return solid, alpha
+
+
+def _oracle_gradient(nx, ny, nz, solid, alpha, tau, g, steps, xin, xout, f0):
+    lat = O.Lattice(nx, ny, nz, periodic=(True, False, False))
+    lat.set_solid(solid)
+    lat.set_force(g)
+    lat.params.tau = tau
+    lat.set_alpha(alpha)
+    states = [f0]
+    for _ in range(steps):
+        states.append(lat.step(states[-1], 1))
+    J = lat.pressure_drop(states[-1], xin, xout)
+    lam = lat.pressure_drop_grad(xin, xout)
+    ga = np.zeros(lat.n)
+    for t in range(steps - 1, -1, -1):
+        lam = lat.step_adjoint(states[t], lam, ga)
+    return J, ga, states[-1]
+
+
+@pytest.mark.parametrize("snapshots", [0, 2, 5])
+def test_fp64_adjoint_matches_cpu_tape(snapshots):
+    nx, ny, nz, steps, tau, g = 16, 8, 8, 12, 0.7, (1e-4, 0.0, 0.0)
+    solid, alpha = _channel(nx, ny, nz, 2)
+    rng = np.random.default_rng(3)
+    ora = O.Lattice(nx, ny, nz, periodic=(True, False, False))
+    ora.set_solid(solid)
+    f0 = ora.equilibrium(u=0.01 * rng.standard_normal((nx * ny * nz, 3)))
+    Jo, go, fo = _oracle_gradient(nx, ny, nz, solid, alpha, tau, g, steps, 3, 11, f0)
+    print(f"[gpu] adjoint FP64 vs tape, snapshots={snapshots}", flush=True)
+    with lbm.Lattice(nx, ny, nz, tau, periodic=(True, False, False), force=g) as lat:
+        lat.set_flags(~solid)
+        lat.set_porosity(alpha)
+        lat.set_populations(f0)
+        J, grad, st = lat.adjoint_pressure_drop(steps, snapshots, 3, 11)
+        f_after = lat.get_populations()
+    assert np.array_equal(f_after, fo)  # the context is left at the final state
+    assert abs(J - Jo) <= 1e-15
+    fluid = ~solid.reshape(-1)
+    scale = np.abs(go[fluid]).max()
+    assert np.max(np.abs(grad - go)) <= 1e-12 * scale
+    assert st.reverse_steps == steps and st.forward_steps == steps
+    if snapshots == 0:
+        assert st.recompute_steps == steps * (steps - 1) // 2
+
+
+def test_fp64_adjoint_matches_central_differences():
+    nx, ny, nz, steps, tau, g = 16, 8, 8, 10, 0.65, (2e-4, 0.0, 0.0)
+    solid, alpha = _channel(nx, ny, nz, 5)
+    xin, xout = 2, 10
+
+    def J_of(a):
+        with lbm.Lattice(nx, ny, nz, tau, periodic=(True, False, False), force=g) as lat:
+            lat.set_flags(~solid)
+            lat.set_porosity(a)
+            lat.init_equilibrium()
+            lat.step(steps)
+            return lat.pressure_drop(xin, xout)
+
+    with lbm.Lattice(nx, ny, nz, tau, periodic=(True, False, False), force=g) as lat:
+        lat.set_flags(~solid)
+        lat.set_porosity(alpha)
+        lat.init_equilibrium()
+        J, grad, _ = lat.adjoint_pressure_drop(steps, 3, xin, xout)
+    assert abs(J - J_of(alpha)) <= 1e-15
+    rng = np.random.default_rng(0)
+    cells = rng.choice(np.flatnonzero(~solid.reshape(-1)), 8, replace=False)
+    h = 1e-4
+    scale = np.abs(grad).max()
+    for i in cells:
+        ap, am = alpha.copy(), alpha.copy()
+        ap[i] += h
+        am[i] -= h
+        fd = (J_of(ap) - J_of(am)) / (2 * h)
+        assert abs(fd - grad[i]) <= 1e-6 * max(abs(fd), 1e-3 * scale), (i, fd, grad[i])
+
+
+def test_fp32_adjoint_tracks_fp64():
+    nx, ny, nz, steps, tau, g = 24, 12, 12, 40, 0.7, (1e-4, 0.0, 0.0)
+    solid, alpha = _channel(nx, ny, nz, 7)
+    out = {}
+    for prec in (lbm.FP64, lbm.FP32_SHIFTED):
+        with lbm.Lattice(nx, ny, nz, tau, precision=prec, periodic=(True, False, False), force=g) as lat:
+            lat.set_flags(~solid)
+            lat.set_porosity(alpha)
+            lat.init_equilibrium()
+            out[prec] = lat.adjoint_pressure_drop(steps, 4, 4, 16)
+    g64, g32 = out[lbm.FP64][1], out[lbm.FP32_SHIFTED][1]
+    rel = np.linalg.norm(g32 - g64) / np.linalg.norm(g64)
+    print(f"[gpu] FP32 adjoint vs FP64: relative L2 {rel:.3e}", flush=True)
+    assert rel <= 1e-2
