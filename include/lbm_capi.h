@@ -95,6 +95,13 @@ int lbm_get_populations(lbm_ctx* ctx, double* f);
 int lbm_step(lbm_ctx* ctx, int nsteps);
 /* Same, timed with HIP events on the context stream; *ms = elapsed device time. */
 int lbm_step_timed(lbm_ctx* ctx, int nsteps, float* ms);
+/* Times the step kernel's two load schedules for nsteps steps each on a copy of the current state
+ * (the state is restored) and keeps the faster one; *variant (may be NULL) receives it: 0 = flag
+ * word first, then the 19 pulled loads (solid cells load nothing); 1 = the 19 neighbour loads issue
+ * alongside the flag word and bounce-back populations are re-loaded afterwards.  With
+ * nsteps == 0, *variant is an input and selects the schedule directly.  Both give bit-identical
+ * results.  No reference counterpart (the reference's CPU traversal has no such choice). */
+int lbm_tune(lbm_ctx* ctx, int nsteps, int* variant);
 int lbm_synchronize(lbm_ctx* ctx);
 
 /* ---- observables ------------------------------------------------------------------------- */

@@ -118,14 +118,14 @@ int lbm_create(lbm_ctx** out, const lbm_config* cfg) {
   c->npartials = lbm::dense_moment_partials(c->g);
   if (hipMalloc(&c->buf[0], pbytes) != hipSuccess || hipMalloc(&c->buf[1], pbytes) != hipSuccess ||
       hipMalloc(&c->words, (size_t)c->g.N * sizeof(uint32_t)) != hipSuccess ||
-      hipMalloc(&c->d_inner, (size_t)c->g.N) != hipSuccess || hipMalloc(&c->d_cls, (size_t)c->g.N) != hipSuccess ||
+      hipMalloc(&c->d_inner, (size_t)c->g.Nu) != hipSuccess || hipMalloc(&c->d_cls, (size_t)c->g.Nu) != hipSuccess ||
       hipMalloc(&c->partials, (size_t)(c->npartials + 1024) * 4 * sizeof(double)) != hipSuccess ||
       hipMalloc(&c->small, 8 * sizeof(double)) != hipSuccess)
     return bail(LBM_ERR_NOMEM, "device allocation failed");
   if (hipMemsetAsync(c->buf[0], 0, pbytes, c->stream) != hipSuccess ||
       hipMemsetAsync(c->buf[1], 0, pbytes, c->stream) != hipSuccess ||
       hipMemsetAsync(c->words, 0, (size_t)c->g.N * sizeof(uint32_t), c->stream) != hipSuccess ||
-      hipMemsetAsync(c->d_cls, 0, (size_t)c->g.N, c->stream) != hipSuccess ||
+      hipMemsetAsync(c->d_cls, 0, (size_t)c->g.Nu, c->stream) != hipSuccess ||
       hipStreamSynchronize(c->stream) != hipSuccess)
     return bail(LBM_ERR_HIP, "initial clear failed");
   *out = c;
@@ -154,7 +154,7 @@ int lbm_set_flags(lbm_ctx* c, const uint8_t* inner) {
   if (!c) return LBM_ERR_ARG;
   if (!inner) return fail(c, LBM_ERR_ARG, "inner flags are null");
   HIPCHK(c, hipSetDevice(c->cfg.device));
-  HIPCHK(c, hipMemcpyAsync(c->d_inner, inner, (size_t)c->g.N, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->d_inner, inner, (size_t)c->g.Nu, hipMemcpyHostToDevice, c->stream));
   c->flags_set = true;
   return rebuild_words(c);
 }
@@ -176,9 +176,9 @@ int lbm_set_wall_classes(lbm_ctx* c, const uint8_t* cls) {
   if (!c) return LBM_ERR_ARG;
   HIPCHK(c, hipSetDevice(c->cfg.device));
   if (cls) {
-    for (int64_t i = 0; i < c->g.N; i++)
+    for (int64_t i = 0; i < c->g.Nu; i++)
       if (cls[i] >= lbm::MAX_CLASSES) return fail(c, LBM_ERR_ARG, "wall class id must be < 8");
-    HIPCHK(c, hipMemcpyAsync(c->d_cls, cls, (size_t)c->g.N, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->d_cls, cls, (size_t)c->g.Nu, hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->has_cls = true;
   } else {
@@ -195,9 +195,9 @@ int lbm_set_cavity_lid(lbm_ctx* c, double ulid) {
     if (lbm::CZ(q) > 0) mask |= 1u << q;
   int rc = lbm_set_moving_wall_class(c, 1, uw, mask);
   if (rc) return rc;
-  std::vector<uint8_t> cls((size_t)c->g.N, 0);
+  std::vector<uint8_t> cls((size_t)c->g.Nu, 0);
   const int64_t plane = (int64_t)c->g.nx * c->g.ny;
-  for (int64_t i = (int64_t)(c->g.nz - 1) * plane; i < c->g.N; i++) cls[i] = 1;
+  for (int64_t i = (int64_t)(c->g.nz - 1) * plane; i < c->g.Nu; i++) cls[i] = 1;
   return lbm_set_wall_classes(c, cls.data());
 }
 
@@ -209,17 +209,26 @@ int lbm_set_porosity(lbm_ctx* c, const double* alpha) {
     c->alpha = nullptr;
     return LBM_OK;
   }
-  for (int64_t i = 0; i < c->g.N; i++)
+  for (int64_t i = 0; i < c->g.Nu; i++)
     if (!(alpha[i] >= 0.0 && alpha[i] <= 1.0)) return fail(c, LBM_ERR_ARG, "porosity must lie in [0,1]");
   if (!c->alpha && hipMalloc(&c->alpha, (size_t)c->g.N * c->esize) != hipSuccess)
     return fail(c, LBM_ERR_NOMEM, "porosity allocation failed");
-  if (c->fp32) {
-    std::vector<float> a32((size_t)c->g.N);
-    for (int64_t i = 0; i < c->g.N; i++) a32[i] = (float)alpha[i];
-    HIPCHK(c, hipMemcpy(c->alpha, a32.data(), a32.size() * sizeof(float), hipMemcpyHostToDevice));
-  } else {
-    HIPCHK(c, hipMemcpy(c->alpha, alpha, (size_t)c->g.N * sizeof(double), hipMemcpyHostToDevice));
-  }
+  // the kernels index the porosity by padded cell; halo cells get 0
+  const lbm::DenseGeom& g = c->g;
+  std::vector<double> a64;
+  std::vector<float> a32;
+  if (c->fp32) a32.assign((size_t)g.N, 0.f);
+  else a64.assign((size_t)g.N, 0.0);
+  for (int z = 0; z < g.nz; z++)
+    for (int y = 0; y < g.ny; y++)
+      for (int x = 0; x < g.nx; x++) {
+        const int64_t u = (int64_t)x + (int64_t)g.nx * ((int64_t)y + (int64_t)g.ny * z);
+        const int64_t p = (int64_t)(x + g.hx) + (int64_t)g.NX * ((int64_t)(y + g.hy) + (int64_t)g.NY * (z + g.hz));
+        if (c->fp32) a32[p] = (float)alpha[u];
+        else a64[p] = alpha[u];
+      }
+  if (c->fp32) HIPCHK(c, hipMemcpy(c->alpha, a32.data(), a32.size() * sizeof(float), hipMemcpyHostToDevice));
+  else HIPCHK(c, hipMemcpy(c->alpha, a64.data(), a64.size() * sizeof(double), hipMemcpyHostToDevice));
   return LBM_OK;
 }
 
@@ -227,7 +236,7 @@ int lbm_init_equilibrium(lbm_ctx* c, const double* rho, const double* u) {
   if (!c) return LBM_ERR_ARG;
   if (!c->flags_set) return fail(c, LBM_ERR_STATE, "boundary flags not set (lbm_set_flags)");
   HIPCHK(c, hipSetDevice(c->cfg.device));
-  const size_t n = (size_t)c->g.N;
+  const size_t n = (size_t)c->g.Nu;
   int rc = ensure_scratch(c, (rho ? n : 0) * sizeof(double) + (u ? 3 * n : 0) * sizeof(double) + 8);
   if (rc) return rc;
   double* d_rho = rho ? (double*)c->scratch : nullptr;
@@ -245,7 +254,7 @@ int lbm_set_populations(lbm_ctx* c, const double* f) {
   if (!f) return fail(c, LBM_ERR_ARG, "populations are null");
   if (!c->flags_set) return fail(c, LBM_ERR_STATE, "boundary flags not set (lbm_set_flags)");
   HIPCHK(c, hipSetDevice(c->cfg.device));
-  const size_t bytes = (size_t)Q * c->g.N * sizeof(double);
+  const size_t bytes = (size_t)Q * c->g.Nu * sizeof(double);
   int rc = ensure_scratch(c, bytes);
   if (rc) return rc;
   HIPCHK(c, hipMemcpyAsync(c->scratch, f, bytes, hipMemcpyHostToDevice, c->stream));
@@ -261,7 +270,7 @@ int lbm_get_populations(lbm_ctx* c, double* f) {
   int rc = need_state(c);
   if (rc) return rc;
   HIPCHK(c, hipSetDevice(c->cfg.device));
-  const size_t bytes = (size_t)Q * c->g.N * sizeof(double);
+  const size_t bytes = (size_t)Q * c->g.Nu * sizeof(double);
   rc = ensure_scratch(c, bytes);
   if (rc) return rc;
   HIPCHK(c, lbm::launch_dense_pull(c->g, c->fp32, c->buf[c->cur], (double*)c->scratch, c->words, c->stream));
@@ -275,7 +284,7 @@ int lbm_get_populations(lbm_ctx* c, double* f) {
 int lbm_enqueue_steps(lbm_ctx* c, int nsteps) {
   for (int s = 0; s < nsteps; s++) {
     HIPCHK(c, lbm::launch_dense_step(c->g, c->fp32, c->buf[c->cur], c->buf[c->cur ^ 1], c->words, c->alpha, c->force,
-                                     c->walls, c->p64, c->p32, c->wt, c->stream));
+                                     c->walls, c->p64, c->p32, c->wt, c->spec, c->stream));
     c->cur ^= 1;
   }
   return LBM_OK;
@@ -308,6 +317,50 @@ int lbm_step_timed(lbm_ctx* c, int nsteps, float* ms) {
   return LBM_OK;
 }
 
+int lbm_tune(lbm_ctx* c, int nsteps, int* variant) {
+  if (!c) return LBM_ERR_ARG;
+  if (nsteps < 0) return fail(c, LBM_ERR_ARG, "nsteps must be >= 0");
+  if (nsteps == 0) {  // explicit choice
+    if (!variant || (*variant != 0 && *variant != 1)) return fail(c, LBM_ERR_ARG, "variant must be 0 or 1");
+    c->spec = *variant == 1;
+    return LBM_OK;
+  }
+  int rc = need_state(c);
+  if (rc) return rc;
+  HIPCHK(c, hipSetDevice(c->cfg.device));
+  const size_t bytes = (size_t)Q * c->g.N * c->esize;
+  rc = ensure_scratch(c, bytes);
+  if (rc) return rc;
+  const int cur0 = c->cur;
+  HIPCHK(c, hipMemcpyAsync(c->scratch, c->buf[cur0], bytes, hipMemcpyDeviceToDevice, c->stream));
+  float best = 0.f;
+  bool best_spec = false;
+  for (int v = 0; v < 2; v++) {
+    c->spec = v == 1;
+    HIPCHK(c, hipMemcpyAsync(c->buf[cur0], c->scratch, bytes, hipMemcpyDeviceToDevice, c->stream));
+    c->cur = cur0;
+    rc = lbm_enqueue_steps(c, 2);  // warm-up
+    if (rc) return rc;
+    float ms = 0.f;
+    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    rc = lbm_enqueue_steps(c, nsteps);
+    if (rc) return rc;
+    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    HIPCHK(c, hipEventSynchronize(c->ev1));
+    HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    if (v == 0 || ms < best) {
+      best = ms;
+      best_spec = c->spec;
+    }
+  }
+  c->spec = best_spec;
+  HIPCHK(c, hipMemcpyAsync(c->buf[cur0], c->scratch, bytes, hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->cur = cur0;
+  if (variant) *variant = c->spec ? 1 : 0;
+  return LBM_OK;
+}
+
 int lbm_synchronize(lbm_ctx* c) {
   if (!c) return LBM_ERR_ARG;
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -320,7 +373,7 @@ int lbm_get_fields(lbm_ctx* c, double* rho, double* u) {
   int rc = need_state(c);
   if (rc) return rc;
   HIPCHK(c, hipSetDevice(c->cfg.device));
-  const size_t n = (size_t)c->g.N;
+  const size_t n = (size_t)c->g.Nu;
   rc = ensure_scratch(c, 4 * n * sizeof(double));
   if (rc) return rc;
   double* d_rho = (double*)c->scratch;

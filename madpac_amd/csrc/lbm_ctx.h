@@ -30,6 +30,7 @@ struct lbm_ctx {
   void* alpha = nullptr;               // Brinkman porosity (same element type as the populations)
   bool flags_set = false, state_set = false;
   bool walls = false, force = false;
+  bool spec = false;                   // step-kernel load schedule (lbm_tune)
   lbm::WallTable wt{};
   lbm::CollideParams p64{};
   lbm::CollideParams32 p32{};

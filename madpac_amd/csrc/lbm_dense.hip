@@ -27,6 +27,9 @@ DenseGeom make_geom(int nx, int ny, int nz, const int periodic[3]) {
   g.nby = (ny + BY - 1) / BY;
   g.nblocks = (int64_t)g.nbx * g.nby * nz;
   g.nblocks_padded = (g.nblocks + NXCD - 1) / NXCD * NXCD;
+  g.hx = g.hy = g.hz = 0;
+  g.NX = nx; g.NY = ny; g.NZ = nz;
+  g.Nu = g.N;
   return g;
 }
 
@@ -63,20 +66,37 @@ __global__ __launch_bounds__(BX * BY) void k_build_words(DenseGeom g, const uint
 
 // ------------------------------------------------------------------------------------------
 // The hot kernel.  T = double: reference arithmetic; T = float: shifted populations.
-template <typename T, bool FORCE, bool ALPHA, bool WALLS>
+// SPEC = false: the flag word is read first, then the 19 loads at the pulled addresses (two
+// dependent memory round trips; solid lanes load nothing).  SPEC = true: the 19 neighbour loads
+// issue together with the word, and the few bounce-back populations are re-loaded from the cell's
+// own reflected slots once the word has arrived (one round trip on most lanes; solid lanes load).
+template <typename T, bool FORCE, bool ALPHA, bool WALLS, bool SPEC>
 __global__ __launch_bounds__(BX * BY) void k_dense_step(DenseGeom g, const T* __restrict__ src,
                                                         T* __restrict__ dst, const uint32_t* __restrict__ words,
                                                         const T* __restrict__ alpha, CollideParams p64,
                                                         CollideParams32 p32, WallTable wt) {
   const CellPos c = map_cell(g);
   if (!c.valid) return;
-  const uint32_t w = words[c.i];
-  if (!(w & FLUID)) return;
   const NbrOff o = nbr_offsets(g, c);
   const int64_t N = g.N;
   T f[Q];
+  uint32_t w;
+  if (SPEC) {
 #pragma unroll
-  for (int q = 0; q < Q; q++) f[q] = src[pull_index(q, w, c.i, N, o)];
+    for (int q = 0; q < Q; q++) f[q] = src[(int64_t)q * N + c.i + src_off(q, o)];
+    w = words[c.i];
+    if (!(w & FLUID)) return;
+    if (w & 0x7FFFFu) {
+#pragma unroll
+      for (int q = 0; q < Q; q++)
+        if ((w >> (Q - 1 - q)) & 1u) f[q] = src[(int64_t)(Q - 1 - q) * N + c.i];
+    }
+  } else {
+    w = words[c.i];
+    if (!(w & FLUID)) return;
+#pragma unroll
+    for (int q = 0; q < Q; q++) f[q] = src[pull_index(q, w, c.i, N, o)];
+  }
   const T a = ALPHA ? alpha[c.i] : T(0);
   T rho;
   if constexpr (sizeof(T) == 8) {
@@ -346,15 +366,15 @@ hipError_t launch_build_words(const DenseGeom& g, const uint8_t* inner, const ui
   return hipGetLastError();
 }
 
-template <typename T>
+template <typename T, bool SPEC>
 static hipError_t step_t(const DenseGeom& g, const void* src, void* dst, const uint32_t* words, const void* alpha,
                          bool force, bool walls, const CollideParams& p64, const CollideParams32& p32,
                          const WallTable& wt, hipStream_t s) {
   const T* a = (const T*)alpha;
   const T* sp = (const T*)src;
   T* dp = (T*)dst;
-#define LBM_STEP(F, A, W)                                                                                 \
-  hipLaunchKernelGGL((k_dense_step<T, F, A, W>), dense_grid(g), dense_block(), 0, s, g, sp, dp, words, a, \
+#define LBM_STEP(F, A, W)                                                                                        \
+  hipLaunchKernelGGL((k_dense_step<T, F, A, W, SPEC>), dense_grid(g), dense_block(), 0, s, g, sp, dp, words, a, \
                      p64, p32, wt)
   const int key = (force ? 4 : 0) | (alpha ? 2 : 0) | (walls ? 1 : 0);
   switch (key) {
@@ -373,9 +393,12 @@ static hipError_t step_t(const DenseGeom& g, const void* src, void* dst, const u
 
 hipError_t launch_dense_step(const DenseGeom& g, bool fp32, const void* src, void* dst, const uint32_t* words,
                              const void* alpha, bool force, bool walls, const CollideParams& p64,
-                             const CollideParams32& p32, const WallTable& wt, hipStream_t s) {
-  return fp32 ? step_t<float>(g, src, dst, words, alpha, force, walls, p64, p32, wt, s)
-              : step_t<double>(g, src, dst, words, alpha, force, walls, p64, p32, wt, s);
+                             const CollideParams32& p32, const WallTable& wt, bool spec, hipStream_t s) {
+  if (fp32)
+    return spec ? step_t<float, true>(g, src, dst, words, alpha, force, walls, p64, p32, wt, s)
+                : step_t<float, false>(g, src, dst, words, alpha, force, walls, p64, p32, wt, s);
+  return spec ? step_t<double, true>(g, src, dst, words, alpha, force, walls, p64, p32, wt, s)
+              : step_t<double, false>(g, src, dst, words, alpha, force, walls, p64, p32, wt, s);
 }
 
 hipError_t launch_dense_pull(const DenseGeom& g, bool fp32, const void* src, double* f, const uint32_t* words,

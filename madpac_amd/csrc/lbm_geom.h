@@ -31,8 +31,10 @@ __device__ __forceinline__ CellPos map_cell(const DenseGeom& g) {
   return c;
 }
 
-// Linear offsets of the x-1/x+1, y-1/y+1, z-1/z+1 neighbours (periodic wrap where enabled;
-// a non-periodic out-of-range offset is never dereferenced because the bounce bit is set).
+// Linear offsets of the x-1/x+1, y-1/y+1, z-1/z+1 neighbours.  Periodic axes wrap; on a
+// non-periodic axis a neighbour outside the lattice is clamped to the cell itself (offset 0), so
+// every address stays inside the lattice even for loads issued before the bounce bits are known
+// (such a link always carries the bounce bit, and the value loaded there is replaced).
 struct NbrOff {
   int64_t xm, xp, ym, yp, zm, zp;
 };
@@ -40,12 +42,12 @@ struct NbrOff {
 __device__ __forceinline__ NbrOff nbr_offsets(const DenseGeom& g, const CellPos& c) {
   NbrOff o;
   const int64_t sy = g.nx, sz = (int64_t)g.nx * g.ny;
-  o.xm = (c.x == 0 && g.px) ? (int64_t)(g.nx - 1) : -1;
-  o.xp = (c.x == g.nx - 1 && g.px) ? -(int64_t)(g.nx - 1) : 1;
-  o.ym = (c.y == 0 && g.py) ? (int64_t)(g.ny - 1) * sy : -sy;
-  o.yp = (c.y == g.ny - 1 && g.py) ? -(int64_t)(g.ny - 1) * sy : sy;
-  o.zm = (c.z == 0 && g.pz) ? (int64_t)(g.nz - 1) * sz : -sz;
-  o.zp = (c.z == g.nz - 1 && g.pz) ? -(int64_t)(g.nz - 1) * sz : sz;
+  o.xm = c.x == 0 ? (g.px ? (int64_t)(g.nx - 1) : 0) : -1;
+  o.xp = c.x == g.nx - 1 ? (g.px ? -(int64_t)(g.nx - 1) : 0) : 1;
+  o.ym = c.y == 0 ? (g.py ? (int64_t)(g.ny - 1) * sy : 0) : -sy;
+  o.yp = c.y == g.ny - 1 ? (g.py ? -(int64_t)(g.ny - 1) * sy : 0) : sy;
+  o.zm = c.z == 0 ? (g.pz ? (int64_t)(g.nz - 1) * sz : 0) : -sz;
+  o.zp = c.z == g.nz - 1 ? (g.pz ? -(int64_t)(g.nz - 1) * sz : 0) : sz;
   return o;
 }
 

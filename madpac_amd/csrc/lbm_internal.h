@@ -24,6 +24,12 @@ struct DenseGeom {
   int nbx, nby;              // blocks along x and y
   int64_t nblocks;           // real blocks
   int64_t nblocks_padded;    // rounded up to a multiple of NXCD
+  // The stored lattice has no halo, so the user layout (nx, ny, nz; Nu cells) and the stored
+  // layout (NX, NY, NZ; N cells, offset hx, hy, hz) coincide; host code addresses both through
+  // these fields.
+  int hx, hy, hz;
+  int NX, NY, NZ;
+  int64_t Nu;
 };
 
 DenseGeom make_geom(int nx, int ny, int nz, const int periodic[3]);
@@ -32,9 +38,11 @@ DenseGeom make_geom(int nx, int ny, int nz, const int periodic[3]);
 hipError_t launch_build_words(const DenseGeom& g, const uint8_t* inner, const uint8_t* cls, uint32_t* words,
                               hipStream_t stream);
 // one step src -> dst
+// spec: issue the 19 neighbour loads alongside the flag word and re-load the bounce-back
+// populations afterwards, instead of reading the word first (same results either way)
 hipError_t launch_dense_step(const DenseGeom& g, bool fp32, const void* src, void* dst, const uint32_t* words,
                              const void* alpha, bool force, bool walls, const CollideParams& p64,
-                             const CollideParams32& p32, const WallTable& wt, hipStream_t stream);
+                             const CollideParams32& p32, const WallTable& wt, bool spec, hipStream_t stream);
 // reference state f (FP64) from the stored state
 hipError_t launch_dense_pull(const DenseGeom& g, bool fp32, const void* src, double* f, const uint32_t* words,
                              hipStream_t stream);

@@ -30,7 +30,7 @@ EXPORTS = (
     "lbm_create", "lbm_destroy", "lbm_last_error", "lbm_fluid_cells", "lbm_set_flags",
     "lbm_set_moving_wall_class", "lbm_set_wall_classes", "lbm_set_cavity_lid", "lbm_set_porosity",
     "lbm_init_equilibrium", "lbm_set_populations", "lbm_get_populations", "lbm_step", "lbm_step_timed",
-    "lbm_synchronize", "lbm_get_fields", "lbm_reduce_moments", "lbm_pressure_drop",
+    "lbm_tune", "lbm_synchronize", "lbm_get_fields", "lbm_reduce_moments", "lbm_pressure_drop",
     "lbm_adjoint_pressure_drop",
     "lbm_patches_create", "lbm_patches_destroy", "lbm_patches_last_error", "lbm_patches_cells",
     "lbm_patches_init", "lbm_patches_get_populations", "lbm_patches_autotune", "lbm_patches_set_tiling",
@@ -93,6 +93,7 @@ def load_library(path: str = LIB_PATH):
     L.lbm_get_populations.argtypes = [vp, dp]
     L.lbm_step.argtypes = [vp, ctypes.c_int]
     L.lbm_step_timed.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_float)]
+    L.lbm_tune.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
     L.lbm_synchronize.argtypes = [vp]
     L.lbm_get_fields.argtypes = [vp, dp, dp]
     L.lbm_reduce_moments.argtypes = [vp, dp]
@@ -221,6 +222,14 @@ class Lattice:
         ms = ctypes.c_float()
         self._check(self._L.lbm_step_timed(self._h, int(n), ctypes.byref(ms)), "lbm_step_timed")
         return float(ms.value)
+
+    def tune(self, nsteps=10, variant=None):
+        """Pick the faster step-kernel load schedule on the current state (returns 0 or 1), or with
+        `variant` given, select that schedule directly."""
+        v = ctypes.c_int(0 if variant is None else int(variant))
+        n = 0 if variant is not None else int(nsteps)
+        self._check(self._L.lbm_tune(self._h, n, ctypes.byref(v)), "lbm_tune")
+        return int(v.value)
 
     def synchronize(self):
         self._check(self._L.lbm_synchronize(self._h), "lbm_synchronize")

@@ -81,14 +81,20 @@ def test_fp64_adjoint_matches_central_differences():
     assert abs(J - J_of(alpha)) <= 1e-15
     rng = np.random.default_rng(0)
     cells = rng.choice(np.flatnonzero(~solid.reshape(-1)), 8, replace=False)
+    # Bar: agreement to 1e-6 relative to the gradient's largest component (J is a difference of
+    # mean densities near 1/3, so the FD quotient carries ~1e-16/h absolute round-off), and to 1e-6
+    # relative per component for components within 100x of the largest.
     h = 1e-4
     scale = np.abs(grad).max()
+    cells = np.concatenate([cells, [int(np.argmax(np.abs(grad)))]])
     for i in cells:
         ap, am = alpha.copy(), alpha.copy()
         ap[i] += h
         am[i] -= h
         fd = (J_of(ap) - J_of(am)) / (2 * h)
-        assert abs(fd - grad[i]) <= 1e-6 * max(abs(fd), 1e-3 * scale), (i, fd, grad[i])
+        assert abs(fd - grad[i]) <= 1e-6 * scale, (i, fd, grad[i])
+        if abs(grad[i]) >= 1e-2 * scale:
+            assert abs(fd - grad[i]) <= 1e-6 * abs(fd), (i, fd, grad[i])
 
 
 def test_fp32_adjoint_tracks_fp64():
