@@ -110,6 +110,9 @@ int lbm_get_fields(lbm_ctx* ctx, double* rho, double* u);
 /* FP64 global mass and momentum over fluid cells: out = {sum f, sum f c_x, sum f c_y, sum f c_z}.
  * Deterministic two-pass reduction; returns LBM_ERR_NUMERIC if a non-finite value is found. */
 int lbm_reduce_moments(lbm_ctx* ctx, double out[4]);
+/* FP64 kinetic energy sum over fluid cells of |j|^2 / (2 rho) (deterministic two-pass reduction).
+ * No reference counterpart (observable for the Taylor-Green validation). */
+int lbm_kinetic_energy(lbm_ctx* ctx, double* ke);
 /* J = cs^2 (<rho>_{x=xin} - <rho>_{x=xout}) over fluid cells of two planes, FP64. */
 int lbm_pressure_drop(lbm_ctx* ctx, int xin, int xout, double* J);
 

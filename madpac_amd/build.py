@@ -59,5 +59,20 @@ def build(verbose: bool = False) -> str:
     return LIB
 
 
+def build_cpp_test(verbose: bool = False) -> str:
+    """Compile tests/cpp/test_peano_api (a C++ program on the reference-shaped API) against the library."""
+    src = os.path.join(ROOT, "tests", "cpp", "test_peano_api.cpp")
+    exe = os.path.join(ROOT, "tests", "cpp", "test_peano_api")
+    cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-I" + os.path.join(ROOT, "include"), "-I" + CSRC, src,
+           "-L" + HERE, "-llbm_hip", "-Wl,-rpath,$ORIGIN/../../madpac_amd", "-o", exe]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"C++ API test build failed:\n{r.stderr}")
+    if verbose:
+        print("built", exe, file=sys.stderr)
+    return exe
+
+
 if __name__ == "__main__":
     build(verbose=True)
+    build_cpp_test(verbose=True)

@@ -405,6 +405,19 @@ int lbm_reduce_moments(lbm_ctx* c, double out[4]) {
   return LBM_OK;
 }
 
+int lbm_kinetic_energy(lbm_ctx* c, double* ke) {
+  if (!c) return LBM_ERR_ARG;
+  if (!ke) return fail(c, LBM_ERR_ARG, "output is null");
+  int rc = need_state(c);
+  if (rc) return rc;
+  HIPCHK(c, hipSetDevice(c->cfg.device));
+  HIPCHK(c, lbm::launch_dense_energy(c->g, c->fp32, c->buf[c->cur], c->words, c->partials, c->small, c->stream));
+  HIPCHK(c, hipMemcpyAsync(ke, c->small, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (!std::isfinite(*ke)) return fail(c, LBM_ERR_NUMERIC, "non-finite kinetic energy");
+  return LBM_OK;
+}
+
 int lbm_pressure_drop(lbm_ctx* c, int xin, int xout, double* J) {
   if (!c) return LBM_ERR_ARG;
   if (!J) return fail(c, LBM_ERR_ARG, "output is null");

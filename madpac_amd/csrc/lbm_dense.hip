@@ -276,6 +276,36 @@ __global__ __launch_bounds__(BX * BY) void k_dense_moments(DenseGeom g, const T*
   block_reduce_store<4>(v, partials + 4 * (int64_t)blockIdx.x);
 }
 
+// kinetic energy sum over fluid cells of (j . j) / (2 rho), FP64
+template <typename T>
+__global__ __launch_bounds__(BX * BY) void k_dense_energy(DenseGeom g, const T* __restrict__ src,
+                                                          const uint32_t* __restrict__ words,
+                                                          double* __restrict__ partials) {
+  const CellPos c = map_cell(g);
+  double v[1] = {0.0};
+  if (c.valid) {
+    const uint32_t w = words[c.i];
+    if (w & FLUID) {
+      const NbrOff o = nbr_offsets(g, c);
+      double r = 0.0, j0 = 0.0, j1 = 0.0, j2 = 0.0;
+#pragma unroll
+      for (int q = 0; q < Q; q++) {
+        const T s = src[pull_index(q, w, c.i, g.N, o)];
+        const double fv = (sizeof(T) == 8) ? (double)s : (double)s + W64(q);
+        r += fv;
+        if (CX(q) == 1) j0 += fv;
+        if (CX(q) == -1) j0 -= fv;
+        if (CY(q) == 1) j1 += fv;
+        if (CY(q) == -1) j1 -= fv;
+        if (CZ(q) == 1) j2 += fv;
+        if (CZ(q) == -1) j2 -= fv;
+      }
+      v[0] = (j0 * j0 + j1 * j1 + j2 * j2) / (2.0 * r);
+    }
+  }
+  block_reduce_store<1>(v, partials + (int64_t)blockIdx.x);
+}
+
 template <int K>
 __global__ __launch_bounds__(256) void k_sum_partials(const double* __restrict__ partials, int64_t n,
                                                       double* __restrict__ out) {
@@ -445,6 +475,18 @@ hipError_t launch_dense_moments(const DenseGeom& g, bool fp32, const void* src, 
     hipLaunchKernelGGL(k_dense_moments<double>, dense_grid(g), dense_block(), 0, s, g, (const double*)src, words,
                        partials);
   hipLaunchKernelGGL(k_sum_partials<4>, dim3(1), dim3(256), 0, s, (const double*)partials, g.nblocks_padded, out4);
+  return hipGetLastError();
+}
+
+hipError_t launch_dense_energy(const DenseGeom& g, bool fp32, const void* src, const uint32_t* words,
+                               double* partials, double* out1, hipStream_t s) {
+  if (fp32)
+    hipLaunchKernelGGL(k_dense_energy<float>, dense_grid(g), dense_block(), 0, s, g, (const float*)src, words,
+                       partials);
+  else
+    hipLaunchKernelGGL(k_dense_energy<double>, dense_grid(g), dense_block(), 0, s, g, (const double*)src, words,
+                       partials);
+  hipLaunchKernelGGL(k_sum_partials<1>, dim3(1), dim3(256), 0, s, (const double*)partials, g.nblocks_padded, out1);
   return hipGetLastError();
 }
 

@@ -59,6 +59,9 @@ hipError_t launch_dense_fields(const DenseGeom& g, bool fp32, const void* src, c
 hipError_t launch_dense_moments(const DenseGeom& g, bool fp32, const void* src, const uint32_t* words,
                                 double* partials, double* out4, hipStream_t stream);
 int64_t dense_moment_partials(const DenseGeom& g);
+// kinetic energy sum over fluid cells of |j|^2 / (2 rho)
+hipError_t launch_dense_energy(const DenseGeom& g, bool fp32, const void* src, const uint32_t* words,
+                               double* partials, double* out1, hipStream_t stream);
 // plane density sums for the pressure drop: out4 = {sum rho(xin), n(xin), sum rho(xout), n(xout)}
 hipError_t launch_dense_plane_rho(const DenseGeom& g, bool fp32, const void* src, const uint32_t* words, int xin,
                                   int xout, double* partials, double* out4, hipStream_t stream);

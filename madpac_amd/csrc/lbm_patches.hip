@@ -1,0 +1,601 @@
+// Batched uneven patches: thousands of cubic patches (edge n, one-cell ghost layer) advanced by
+// one persistent launch per step, with the ghost-layer exchange fused into the producer.
+//
+// Reference counterpart: each Peano vertex owns one regular block, and one iteration runs the
+// per-block collision (touchVertexFirstTime), the streaming between the blocks that meet in a
+// Peano cell (handleCell -> AABlockStreaming::streamingNonLocal, the reference's "ghost-layer
+// exchange") and the in-cell swap (touchVertexLastTime); BLB/mappings/RegularGrid2RegularBlockSolver.cpp
+// :83-248, BLB/nonlocaltreatment/AABlockStreaming.cpp:37-53.  Here a patch is stored with a
+// ghost layer: the state is post-collision, f_q(x) = s_q(x - c_q) for every interior cell, and a
+// cell on a patch boundary, after colliding, also writes each population that leaves the patch
+// into the ghost layer of the neighbouring patch (face or edge neighbour, a 19-entry table per
+// patch), where the next step's pull finds it.  No grid-wide barrier is needed inside a step: the
+// kernel boundary orders the steps.
+//
+// Work decomposition: patches are cut into tiles of tile_x * tile_y * tile_z cells; a persistent
+// grid of (CUs * blocks_per_cu) workgroups walks the tile list (largest patches first) with a
+// fixed stride, so every workgroup exits after its last tile.  lbm_patches_autotune times
+// candidate tilings on the device and keeps the fastest.  Steps can be captured once in a
+// hipGraph (two launches: A -> B, B -> A) and replayed.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "lbm_capi.h"
+#include "lbm_cell.h"
+
+namespace lbm {
+
+struct PatchDev {
+  int64_t base;    // element offset of the patch's SoA block (19 planes of np^3)
+  int n;           // interior edge
+  int np;          // n + 2
+  double omega;    // 1 / tau_eff of the patch
+  int nbr[Q];      // neighbour patch in direction c_q
+};
+
+struct TileDev {
+  int patch;
+  int x0, y0, z0;  // tile origin, interior coordinates
+};
+
+// D3Q19 index of a direction vector e in {-1,0,1}^3 (-1 where not a D3Q19 direction)
+__host__ __device__ constexpr int dir_of(int ex, int ey, int ez) {
+  for (int q = 0; q < Q; q++)
+    if (CX(q) == ex && CY(q) == ey && CZ(q) == ez) return q;
+  return -1;
+}
+
+template <typename T>
+__device__ __forceinline__ void collide_patch(T (&f)[Q], double omega) {
+  if constexpr (sizeof(T) == 8) {
+    CollideParams p{1.0 / omega, omega, {0.0, 0.0, 0.0}};
+    collide_f64<false, false>(f, p, 0.0);
+  } else {
+    CollideParams32 p{(float)omega, 0.f, {0.f, 0.f, 0.f}};
+    collide_f32s<false, false>(f, p, 0.f);
+  }
+}
+
+template <typename T, int TX, int TY, int TZ>
+__global__ __launch_bounds__(256) void k_patch_step(const PatchDev* __restrict__ patches,
+                                                    const TileDev* __restrict__ tiles, int ntiles,
+                                                    const T* __restrict__ src, T* __restrict__ dst) {
+  constexpr int TC = TX * TY * TZ;
+  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const TileDev tl = tiles[t];
+    const PatchDev pd = patches[tl.patch];
+    const int n = pd.n, np = pd.np;
+    const int64_t P = (int64_t)np * np * np;
+    const T* __restrict__ s = src + pd.base;
+    T* __restrict__ d = dst + pd.base;
+    for (int c = threadIdx.x; c < TC; c += 256) {
+      const int x = tl.x0 + c % TX, y = tl.y0 + (c / TX) % TY, z = tl.z0 + c / (TX * TY);
+      if (x >= n || y >= n || z >= n) continue;
+      const int64_t i = (int64_t)(x + 1) + (int64_t)np * ((y + 1) + (int64_t)np * (z + 1));
+      T f[Q];
+#pragma unroll
+      for (int q = 0; q < Q; q++) {
+        const int64_t off = (int64_t)CX(q) + (int64_t)np * (CY(q) + (int64_t)np * CZ(q));
+        f[q] = s[(int64_t)q * P + i - off];
+      }
+      collide_patch<T>(f, pd.omega);
+#pragma unroll
+      for (int q = 0; q < Q; q++) __builtin_nontemporal_store(f[q], &d[(int64_t)q * P + i]);
+      // ghost-layer exchange: populations leaving the patch go to the neighbour's ghost layer
+      if (x == 0 || y == 0 || z == 0 || x == n - 1 || y == n - 1 || z == n - 1) {
+#pragma unroll
+        for (int q = 0; q < Q; q++) {
+          if (q == REST) continue;
+          const int tx = x + CX(q), ty = y + CY(q), tz = z + CZ(q);
+          const int ex = tx < 0 ? -1 : (tx >= n ? 1 : 0);
+          const int ey = ty < 0 ? -1 : (ty >= n ? 1 : 0);
+          const int ez = tz < 0 ? -1 : (tz >= n ? 1 : 0);
+          if ((ex | ey | ez) == 0) continue;
+          const int k = dir_of(ex, ey, ez);  // a face or edge direction: always a D3Q19 direction
+          const PatchDev& nb = patches[pd.nbr[k]];
+          const int gx = x - n * ex, gy = y - n * ey, gz = z - n * ez;  // ghost position in the neighbour
+          const int64_t gi = (int64_t)(gx + 1) + (int64_t)np * ((gy + 1) + (int64_t)np * (gz + 1));
+          dst[nb.base + (int64_t)q * P + gi] = f[q];
+        }
+      }
+    }
+  }
+}
+
+// initial state: f = feq(rho, u) per interior cell, stored so that f_q(x) = s_q(x - c_q); a slot
+// in the ghost layer also has its counterpart in the neighbour's interior, which is written too
+template <typename T>
+__global__ __launch_bounds__(256) void k_patch_init(const PatchDev* __restrict__ patches, int npatch,
+                                                    const int64_t* __restrict__ cell_base, const double* __restrict__ rho,
+                                                    const double* __restrict__ u, T* __restrict__ s) {
+  const int p = blockIdx.y;
+  if (p >= npatch) return;
+  const PatchDev pd = patches[p];
+  const int n = pd.n, np = pd.np;
+  const int64_t P = (int64_t)np * np * np;
+  const int64_t cells = (int64_t)n * n * n;
+  for (int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x; c < cells; c += (int64_t)gridDim.x * 256) {
+    const int x = (int)(c % n), y = (int)((c / n) % n), z = (int)(c / ((int64_t)n * n));
+    const int64_t ui = cell_base[p] + c;
+    const double r = rho ? rho[ui] : 1.0;
+    const double u0 = u ? u[3 * ui] : 0.0, u1 = u ? u[3 * ui + 1] : 0.0, u2 = u ? u[3 * ui + 2] : 0.0;
+    double uu = u0 * u0;
+    uu += u1 * u1;
+    uu += u2 * u2;
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+      const double cu = cdot64(q, u0, u1, u2);
+      const double feq = W64(q) * r *
+          (1.0 + cu / (1.0 / 3.0) + (cu * cu) / (2.0 * (1.0 / 3.0) * (1.0 / 3.0)) - (uu / (2.0 * (1.0 / 3.0))));
+      const T v = (sizeof(T) == 8) ? (T)feq : (T)(feq - W64(q));
+      const int sx = x - CX(q), sy = y - CY(q), sz = z - CZ(q);  // the slot x - c_q
+      s[pd.base + (int64_t)q * P + (int64_t)(sx + 1) + (int64_t)np * ((sy + 1) + (int64_t)np * (sz + 1))] = v;
+      const int ex = sx < 0 ? -1 : (sx >= n ? 1 : 0);
+      const int ey = sy < 0 ? -1 : (sy >= n ? 1 : 0);
+      const int ez = sz < 0 ? -1 : (sz >= n ? 1 : 0);
+      if ((ex | ey | ez) != 0) {
+        const PatchDev& nb = patches[pd.nbr[dir_of(ex, ey, ez)]];
+        const int gx = sx - n * ex, gy = sy - n * ey, gz = sz - n * ez;  // interior of the neighbour
+        s[nb.base + (int64_t)q * P + (int64_t)(gx + 1) + (int64_t)np * ((gy + 1) + (int64_t)np * (gz + 1))] = v;
+      }
+    }
+  }
+}
+
+// reference state of every interior cell (FP64), patch-major SoA blocks of 19 * n^3
+template <typename T>
+__global__ __launch_bounds__(256) void k_patch_pull(const PatchDev* __restrict__ patches, int npatch,
+                                                    const int64_t* __restrict__ cell_base, const T* __restrict__ s,
+                                                    double* __restrict__ f) {
+  const int p = blockIdx.y;
+  if (p >= npatch) return;
+  const PatchDev pd = patches[p];
+  const int n = pd.n, np = pd.np;
+  const int64_t P = (int64_t)np * np * np;
+  const int64_t cells = (int64_t)n * n * n;
+  for (int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x; c < cells; c += (int64_t)gridDim.x * 256) {
+    const int x = (int)(c % n), y = (int)((c / n) % n), z = (int)(c / ((int64_t)n * n));
+    const int64_t i = (int64_t)(x + 1) + (int64_t)np * ((y + 1) + (int64_t)np * (z + 1));
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+      const int64_t off = (int64_t)CX(q) + (int64_t)np * (CY(q) + (int64_t)np * CZ(q));
+      const T v = s[pd.base + (int64_t)q * P + i - off];
+      f[Q * cell_base[p] + (int64_t)q * cells + c] = (sizeof(T) == 8) ? (double)v : (double)v + W64(q);
+    }
+  }
+}
+
+// per-block partial mass / momentum deviations over interior cells, summed in fixed order later
+template <typename T>
+__global__ __launch_bounds__(256) void k_patch_moments(const PatchDev* __restrict__ patches, int npatch,
+                                                       const T* __restrict__ s, double* __restrict__ partials) {
+  const int p = blockIdx.y;
+  double v[4] = {0.0, 0.0, 0.0, 0.0};
+  if (p < npatch) {
+    const PatchDev pd = patches[p];
+    const int n = pd.n, np = pd.np;
+    const int64_t P = (int64_t)np * np * np;
+    const int64_t cells = (int64_t)n * n * n;
+    for (int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x; c < cells; c += (int64_t)gridDim.x * 256) {
+      const int x = (int)(c % n), y = (int)((c / n) % n), z = (int)(c / ((int64_t)n * n));
+      const int64_t i = (int64_t)(x + 1) + (int64_t)np * ((y + 1) + (int64_t)np * (z + 1));
+#pragma unroll
+      for (int q = 0; q < Q; q++) {
+        const int64_t off = (int64_t)CX(q) + (int64_t)np * (CY(q) + (int64_t)np * CZ(q));
+        const T sv = s[pd.base + (int64_t)q * P + i - off];
+        const double fv = (sizeof(T) == 8) ? (double)sv : (double)sv + W64(q);
+        v[0] += (sizeof(T) == 8) ? (double)sv - W64(q) : (double)sv;
+        if (CX(q) == 1) v[1] += fv;
+        if (CX(q) == -1) v[1] -= fv;
+        if (CY(q) == 1) v[2] += fv;
+        if (CY(q) == -1) v[2] -= fv;
+        if (CZ(q) == 1) v[3] += fv;
+        if (CZ(q) == -1) v[3] -= fv;
+      }
+    }
+  }
+  __shared__ double lds[4][4];
+#pragma unroll
+  for (int k = 0; k < 4; k++)
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_xor(v[k], off, 64);
+  if (threadIdx.x % 64 == 0)
+#pragma unroll
+    for (int k = 0; k < 4; k++) lds[k][threadIdx.x / 64] = v[k];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int64_t slot = (int64_t)blockIdx.y * gridDim.x + blockIdx.x;
+#pragma unroll
+    for (int k = 0; k < 4; k++) partials[4 * slot + k] = lds[k][0] + lds[k][1] + lds[k][2] + lds[k][3];
+  }
+}
+
+}  // namespace lbm
+
+using namespace lbm;
+
+namespace {
+// Candidate tilings for the autotuner: 256 threads per workgroup, 256 or 512 cells per tile.
+struct Shape {
+  int x, y, z;
+};
+const Shape kShapes[] = {{8, 8, 4}, {8, 8, 8}, {16, 8, 2}, {16, 16, 1}, {16, 16, 2}, {32, 8, 1}, {32, 8, 2}, {16, 4, 4}};
+
+double tau_eff_of(double tau) {
+  double nu = 1.0 / 3.0 * (tau - 0.5);
+  nu = nu * 1.0;
+  return 3.0 * nu + 0.5;
+}
+}  // namespace
+
+struct lbm_patchset {
+  int device = 0;
+  bool fp32 = false;
+  size_t esize = 8;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  std::string err;
+  std::vector<PatchDev> host;
+  std::vector<int64_t> cell_base;  // interior-cell offset of each patch in user arrays
+  int64_t cells = 0;               // interior cells
+  int64_t elems = 0;               // stored elements per buffer
+  PatchDev* d_patches = nullptr;
+  int64_t* d_cell_base = nullptr;
+  void* buf[2] = {nullptr, nullptr};
+  int cur = 0;
+  TileDev* d_tiles = nullptr;
+  int ntiles = 0;
+  lbm_tiling tiling{16, 16, 1, 4};
+  int cus = 256;
+  double* partials = nullptr;
+  int64_t npartials = 0;
+  double* small = nullptr;
+  bool state_set = false;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  int graph_cur = -1;  // buffer parity the graph was captured for
+};
+
+namespace {
+thread_local std::string g_patch_create_error;
+
+int pfail(lbm_patchset* ps, int code, const std::string& m) {
+  if (ps) ps->err = m;
+  return code;
+}
+#define PCHK(ps, call)                                                                                           \
+  do {                                                                                                           \
+    hipError_t e_ = (call);                                                                                      \
+    if (e_ != hipSuccess) return pfail(ps, LBM_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_));    \
+  } while (0)
+
+void drop_graph(lbm_patchset* ps) {
+  if (ps->gexec) (void)hipGraphExecDestroy(ps->gexec);
+  if (ps->graph) (void)hipGraphDestroy(ps->graph);
+  ps->gexec = nullptr;
+  ps->graph = nullptr;
+  ps->graph_cur = -1;
+}
+
+// tile list for a shape: largest patches first, then patch order, then z, y, x
+int build_tiles(lbm_patchset* ps, const lbm_tiling& t) {
+  std::vector<int> order(ps->host.size());
+  for (size_t p = 0; p < order.size(); p++) order[p] = (int)p;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return ps->host[a].n > ps->host[b].n; });
+  std::vector<TileDev> tiles;
+  for (int p : order) {
+    const int n = ps->host[p].n;
+    for (int z = 0; z < n; z += t.tile_z)
+      for (int y = 0; y < n; y += t.tile_y)
+        for (int x = 0; x < n; x += t.tile_x) tiles.push_back(TileDev{p, x, y, z});
+  }
+  if (ps->d_tiles) (void)hipFree(ps->d_tiles);
+  ps->d_tiles = nullptr;
+  if (hipMalloc(&ps->d_tiles, tiles.size() * sizeof(TileDev)) != hipSuccess)
+    return pfail(ps, LBM_ERR_NOMEM, "tile table allocation failed");
+  PCHK(ps, hipMemcpy(ps->d_tiles, tiles.data(), tiles.size() * sizeof(TileDev), hipMemcpyHostToDevice));
+  ps->ntiles = (int)tiles.size();
+  ps->tiling = t;
+  drop_graph(ps);
+  return LBM_OK;
+}
+
+template <typename T>
+hipError_t launch_step_t(lbm_patchset* ps, const void* src, void* dst) {
+  const lbm_tiling& t = ps->tiling;
+  const dim3 grid((unsigned)std::max(1, std::min(ps->ntiles, ps->cus * t.blocks_per_cu))), block(256);
+  const T* s = (const T*)src;
+  T* d = (T*)dst;
+#define LBM_PATCH(X, Y, Z)                                                                                     \
+  if (t.tile_x == X && t.tile_y == Y && t.tile_z == Z) {                                                       \
+    hipLaunchKernelGGL((k_patch_step<T, X, Y, Z>), grid, block, 0, ps->stream, ps->d_patches, ps->d_tiles,    \
+                       ps->ntiles, s, d);                                                                      \
+    return hipGetLastError();                                                                                  \
+  }
+  LBM_PATCH(8, 8, 4)
+  LBM_PATCH(8, 8, 8)
+  LBM_PATCH(16, 8, 2)
+  LBM_PATCH(16, 16, 1)
+  LBM_PATCH(16, 16, 2)
+  LBM_PATCH(32, 8, 1)
+  LBM_PATCH(32, 8, 2)
+  LBM_PATCH(16, 4, 4)
+#undef LBM_PATCH
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_step(lbm_patchset* ps, int from) {
+  return ps->fp32 ? launch_step_t<float>(ps, ps->buf[from], ps->buf[from ^ 1])
+                  : launch_step_t<double>(ps, ps->buf[from], ps->buf[from ^ 1]);
+}
+
+bool shape_supported(const lbm_tiling* t) {
+  for (const Shape& s : kShapes)
+    if (s.x == t->tile_x && s.y == t->tile_y && s.z == t->tile_z) return true;
+  return false;
+}
+}  // namespace
+
+extern "C" {
+
+int lbm_patches_create(lbm_patchset** out, const lbm_patch* table, int npatches, int precision, int device) {
+  if (!out) return LBM_ERR_ARG;
+  *out = nullptr;
+  lbm_patchset* ps = new lbm_patchset();
+  auto bail = [&](int code, const std::string& m) {
+    g_patch_create_error = m;  // lbm_patches_last_error(NULL) reports it
+    lbm_patches_destroy(ps);
+    return code;
+  };
+  if (!table || npatches < 1) return bail(LBM_ERR_ARG, "empty patch table");
+  if (precision != LBM_FP64 && precision != LBM_FP32_SHIFTED) return bail(LBM_ERR_ARG, "unknown precision");
+  for (int p = 0; p < npatches; p++) {
+    const lbm_patch& P = table[p];
+    if (P.n < 2 || P.n > 256) return bail(LBM_ERR_ARG, "patch edge must be in 2..256");
+    if (!(P.tau > 0.5)) return bail(LBM_ERR_ARG, "patch tau must be > 0.5");
+    for (int q = 0; q < Q; q++) {
+      if (q == REST) continue;
+      if (P.nbr[q] < 0 || P.nbr[q] >= npatches) return bail(LBM_ERR_ARG, "neighbour index out of range");
+      if (table[P.nbr[q]].n != P.n) return bail(LBM_ERR_ARG, "neighbouring patches must share their edge");
+    }
+  }
+  ps->device = device;
+  ps->fp32 = precision == LBM_FP32_SHIFTED;
+  ps->esize = ps->fp32 ? 4 : 8;
+  int64_t base = 0, cb = 0;
+  for (int p = 0; p < npatches; p++) {
+    PatchDev d{};
+    d.n = table[p].n;
+    d.np = d.n + 2;
+    d.base = base;
+    d.omega = 1.0 / tau_eff_of(table[p].tau);
+    for (int q = 0; q < Q; q++) d.nbr[q] = q == REST ? p : table[p].nbr[q];
+    ps->host.push_back(d);
+    ps->cell_base.push_back(cb);
+    base += ((int64_t)Q * d.np * d.np * d.np + 63) / 64 * 64;  // 64-element aligned blocks
+    cb += (int64_t)d.n * d.n * d.n;
+  }
+  ps->elems = base;
+  ps->cells = cb;
+  hipDeviceProp_t prop;
+  if (hipSetDevice(device) != hipSuccess || hipGetDeviceProperties(&prop, device) != hipSuccess)
+    return bail(LBM_ERR_HIP, "device query failed");
+  ps->cus = prop.multiProcessorCount;
+  if (hipStreamCreateWithFlags(&ps->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&ps->ev0) != hipSuccess || hipEventCreate(&ps->ev1) != hipSuccess)
+    return bail(LBM_ERR_HIP, "stream/event creation failed");
+  const size_t bytes = (size_t)ps->elems * ps->esize;
+  int maxn = 0;
+  for (auto& d : ps->host) maxn = std::max(maxn, d.n);
+  ps->npartials = (int64_t)npatches * 64;
+  if (hipMalloc(&ps->buf[0], bytes) != hipSuccess || hipMalloc(&ps->buf[1], bytes) != hipSuccess ||
+      hipMalloc(&ps->d_patches, ps->host.size() * sizeof(PatchDev)) != hipSuccess ||
+      hipMalloc(&ps->d_cell_base, ps->cell_base.size() * sizeof(int64_t)) != hipSuccess ||
+      hipMalloc(&ps->partials, (size_t)ps->npartials * 4 * sizeof(double)) != hipSuccess ||
+      hipMalloc(&ps->small, 8 * sizeof(double)) != hipSuccess)
+    return bail(LBM_ERR_NOMEM, "device allocation failed");
+  if (hipMemcpy(ps->d_patches, ps->host.data(), ps->host.size() * sizeof(PatchDev), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(ps->d_cell_base, ps->cell_base.data(), ps->cell_base.size() * sizeof(int64_t), hipMemcpyHostToDevice) !=
+          hipSuccess ||
+      hipMemset(ps->buf[0], 0, bytes) != hipSuccess || hipMemset(ps->buf[1], 0, bytes) != hipSuccess)
+    return bail(LBM_ERR_HIP, "initial upload failed");
+  if (build_tiles(ps, ps->tiling) != LBM_OK) return bail(LBM_ERR_HIP, ps->err);
+  *out = ps;
+  return LBM_OK;
+}
+
+int lbm_patches_destroy(lbm_patchset* ps) {
+  if (!ps) return LBM_OK;
+  if (ps->stream) (void)hipStreamSynchronize(ps->stream);
+  drop_graph(ps);
+  void* ptrs[] = {ps->buf[0], ps->buf[1], ps->d_patches, ps->d_cell_base, ps->d_tiles, ps->partials, ps->small};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  if (ps->ev0) (void)hipEventDestroy(ps->ev0);
+  if (ps->ev1) (void)hipEventDestroy(ps->ev1);
+  if (ps->stream) (void)hipStreamDestroy(ps->stream);
+  delete ps;
+  return LBM_OK;
+}
+
+const char* lbm_patches_last_error(const lbm_patchset* ps) {
+  return ps ? ps->err.c_str() : g_patch_create_error.c_str();
+}
+
+int64_t lbm_patches_cells(const lbm_patchset* ps) { return ps ? ps->cells : -1; }
+
+int lbm_patches_init(lbm_patchset* ps, const double* rho, const double* u) {
+  if (!ps) return LBM_ERR_ARG;
+  PCHK(ps, hipSetDevice(ps->device));
+  double *d_rho = nullptr, *d_u = nullptr;
+  if (rho && hipMalloc(&d_rho, (size_t)ps->cells * sizeof(double)) != hipSuccess)
+    return pfail(ps, LBM_ERR_NOMEM, "staging allocation failed");
+  if (u && hipMalloc(&d_u, (size_t)ps->cells * 3 * sizeof(double)) != hipSuccess) {
+    if (d_rho) (void)hipFree(d_rho);
+    return pfail(ps, LBM_ERR_NOMEM, "staging allocation failed");
+  }
+  hipError_t e = hipSuccess;
+  if (rho) e = hipMemcpy(d_rho, rho, (size_t)ps->cells * sizeof(double), hipMemcpyHostToDevice);
+  if (e == hipSuccess && u) e = hipMemcpy(d_u, u, (size_t)ps->cells * 3 * sizeof(double), hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    int maxn = 0;
+    for (auto& d : ps->host) maxn = std::max(maxn, d.n);
+    const int64_t maxc = (int64_t)maxn * maxn * maxn;
+    const dim3 grid((unsigned)std::min<int64_t>((maxc + 255) / 256, 64), (unsigned)ps->host.size());
+    if (ps->fp32)
+      hipLaunchKernelGGL(k_patch_init<float>, grid, dim3(256), 0, ps->stream, ps->d_patches, (int)ps->host.size(),
+                         ps->d_cell_base, d_rho, d_u, (float*)ps->buf[ps->cur]);
+    else
+      hipLaunchKernelGGL(k_patch_init<double>, grid, dim3(256), 0, ps->stream, ps->d_patches, (int)ps->host.size(),
+                         ps->d_cell_base, d_rho, d_u, (double*)ps->buf[ps->cur]);
+    e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(ps->stream);
+  }
+  if (d_rho) (void)hipFree(d_rho);
+  if (d_u) (void)hipFree(d_u);
+  if (e != hipSuccess) return pfail(ps, LBM_ERR_HIP, std::string("patch init: ") + hipGetErrorString(e));
+  ps->state_set = true;
+  return LBM_OK;
+}
+
+int lbm_patches_get_populations(lbm_patchset* ps, double* f) {
+  if (!ps) return LBM_ERR_ARG;
+  if (!f) return pfail(ps, LBM_ERR_ARG, "output is null");
+  if (!ps->state_set) return pfail(ps, LBM_ERR_STATE, "state not initialised (lbm_patches_init)");
+  PCHK(ps, hipSetDevice(ps->device));
+  double* d_f = nullptr;
+  const size_t bytes = (size_t)ps->cells * Q * sizeof(double);
+  if (hipMalloc(&d_f, bytes) != hipSuccess) return pfail(ps, LBM_ERR_NOMEM, "staging allocation failed");
+  int maxn = 0;
+  for (auto& d : ps->host) maxn = std::max(maxn, d.n);
+  const int64_t maxc = (int64_t)maxn * maxn * maxn;
+  const dim3 grid((unsigned)std::min<int64_t>((maxc + 255) / 256, 64), (unsigned)ps->host.size());
+  if (ps->fp32)
+    hipLaunchKernelGGL(k_patch_pull<float>, grid, dim3(256), 0, ps->stream, ps->d_patches, (int)ps->host.size(),
+                       ps->d_cell_base, (const float*)ps->buf[ps->cur], d_f);
+  else
+    hipLaunchKernelGGL(k_patch_pull<double>, grid, dim3(256), 0, ps->stream, ps->d_patches, (int)ps->host.size(),
+                       ps->d_cell_base, (const double*)ps->buf[ps->cur], d_f);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(f, d_f, bytes, hipMemcpyDeviceToHost, ps->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ps->stream);
+  (void)hipFree(d_f);
+  if (e != hipSuccess) return pfail(ps, LBM_ERR_HIP, std::string("patch readout: ") + hipGetErrorString(e));
+  return LBM_OK;
+}
+
+int lbm_patches_set_tiling(lbm_patchset* ps, const lbm_tiling* t) {
+  if (!ps) return LBM_ERR_ARG;
+  if (!t || !shape_supported(t)) return pfail(ps, LBM_ERR_ARG, "unsupported tile shape");
+  if (t->blocks_per_cu < 1 || t->blocks_per_cu > 32) return pfail(ps, LBM_ERR_ARG, "blocks_per_cu must be in 1..32");
+  PCHK(ps, hipSetDevice(ps->device));
+  return build_tiles(ps, *t);
+}
+
+int lbm_patches_step(lbm_patchset* ps, int nsteps, int use_graph, float* ms) {
+  if (!ps) return LBM_ERR_ARG;
+  if (nsteps < 0) return pfail(ps, LBM_ERR_ARG, "nsteps must be >= 0");
+  if (!ps->state_set) return pfail(ps, LBM_ERR_STATE, "state not initialised (lbm_patches_init)");
+  PCHK(ps, hipSetDevice(ps->device));
+  PCHK(ps, hipEventRecord(ps->ev0, ps->stream));
+  int s = 0;
+  if (use_graph && nsteps >= 2) {
+    if (!ps->gexec || ps->graph_cur != ps->cur) {
+      drop_graph(ps);
+      PCHK(ps, hipStreamBeginCapture(ps->stream, hipStreamCaptureModeThreadLocal));
+      hipError_t e1 = launch_step(ps, ps->cur);
+      hipError_t e2 = launch_step(ps, ps->cur ^ 1);
+      hipError_t e3 = hipStreamEndCapture(ps->stream, &ps->graph);
+      if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess) {
+        drop_graph(ps);
+        return pfail(ps, LBM_ERR_HIP, "graph capture failed");
+      }
+      PCHK(ps, hipGraphInstantiate(&ps->gexec, ps->graph, nullptr, nullptr, 0));
+      ps->graph_cur = ps->cur;
+    }
+    for (; s + 2 <= nsteps; s += 2) PCHK(ps, hipGraphLaunch(ps->gexec, ps->stream));
+  }
+  for (; s < nsteps; s++) {
+    PCHK(ps, launch_step(ps, ps->cur));
+    ps->cur ^= 1;
+  }
+  PCHK(ps, hipEventRecord(ps->ev1, ps->stream));
+  if (ms) {
+    PCHK(ps, hipEventSynchronize(ps->ev1));
+    PCHK(ps, hipEventElapsedTime(ms, ps->ev0, ps->ev1));
+  }
+  return LBM_OK;
+}
+
+int lbm_patches_autotune(lbm_patchset* ps, lbm_tiling* chosen) {
+  if (!ps) return LBM_ERR_ARG;
+  if (!ps->state_set) return pfail(ps, LBM_ERR_STATE, "state not initialised (lbm_patches_init)");
+  PCHK(ps, hipSetDevice(ps->device));
+  // time on a copy of the state, restore afterwards
+  const size_t bytes = (size_t)ps->elems * ps->esize;
+  void* save = nullptr;
+  if (hipMalloc(&save, bytes) != hipSuccess) return pfail(ps, LBM_ERR_NOMEM, "autotune staging allocation failed");
+  const int cur0 = ps->cur;
+  PCHK(ps, hipMemcpy(save, ps->buf[cur0], bytes, hipMemcpyDeviceToDevice));
+  lbm_tiling best{};
+  float best_ms = 1e30f;
+  const int bpcs[] = {2, 4, 8};
+  int rc = LBM_OK;
+  for (const Shape& sh : kShapes) {
+    for (int bpc : bpcs) {
+      lbm_tiling t{sh.x, sh.y, sh.z, bpc};
+      rc = build_tiles(ps, t);
+      if (rc) break;
+      float ms = 0.f;
+      rc = lbm_patches_step(ps, 2, 0, nullptr);
+      if (!rc) rc = lbm_patches_step(ps, 6, 0, &ms);
+      if (rc) break;
+      if (ms < best_ms) {
+        best_ms = ms;
+        best = t;
+      }
+    }
+    if (rc) break;
+  }
+  hipError_t e = hipMemcpy(ps->buf[cur0], save, bytes, hipMemcpyDeviceToDevice);
+  (void)hipFree(save);
+  ps->cur = cur0;
+  if (rc) return rc;
+  if (e != hipSuccess) return pfail(ps, LBM_ERR_HIP, "autotune restore failed");
+  rc = build_tiles(ps, best);
+  if (rc) return rc;
+  if (chosen) *chosen = best;
+  return LBM_OK;
+}
+
+int lbm_patches_reduce_moments(lbm_patchset* ps, double out[4]) {
+  if (!ps) return LBM_ERR_ARG;
+  if (!out) return pfail(ps, LBM_ERR_ARG, "output is null");
+  if (!ps->state_set) return pfail(ps, LBM_ERR_STATE, "state not initialised (lbm_patches_init)");
+  PCHK(ps, hipSetDevice(ps->device));
+  const int np = (int)ps->host.size();
+  const dim3 grid(64, (unsigned)np);
+  if (ps->fp32)
+    hipLaunchKernelGGL(k_patch_moments<float>, grid, dim3(256), 0, ps->stream, ps->d_patches, np,
+                       (const float*)ps->buf[ps->cur], ps->partials);
+  else
+    hipLaunchKernelGGL(k_patch_moments<double>, grid, dim3(256), 0, ps->stream, ps->d_patches, np,
+                       (const double*)ps->buf[ps->cur], ps->partials);
+  PCHK(ps, hipGetLastError());
+  std::vector<double> h((size_t)np * 64 * 4);
+  PCHK(ps, hipMemcpyAsync(h.data(), ps->partials, h.size() * sizeof(double), hipMemcpyDeviceToHost, ps->stream));
+  PCHK(ps, hipStreamSynchronize(ps->stream));
+  double s[4] = {0, 0, 0, 0};
+  for (size_t j = 0; j < h.size() / 4; j++)
+    for (int k = 0; k < 4; k++) s[k] += h[4 * j + k];
+  out[0] = (double)ps->cells + s[0];
+  for (int k = 1; k < 4; k++) out[k] = s[k];
+  for (int k = 0; k < 4; k++)
+    if (!std::isfinite(out[k])) return pfail(ps, LBM_ERR_NUMERIC, "non-finite population detected");
+  return LBM_OK;
+}
+
+}  // extern "C"

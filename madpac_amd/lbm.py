@@ -30,7 +30,8 @@ EXPORTS = (
     "lbm_create", "lbm_destroy", "lbm_last_error", "lbm_fluid_cells", "lbm_set_flags",
     "lbm_set_moving_wall_class", "lbm_set_wall_classes", "lbm_set_cavity_lid", "lbm_set_porosity",
     "lbm_init_equilibrium", "lbm_set_populations", "lbm_get_populations", "lbm_step", "lbm_step_timed",
-    "lbm_tune", "lbm_synchronize", "lbm_get_fields", "lbm_reduce_moments", "lbm_pressure_drop",
+    "lbm_tune", "lbm_synchronize", "lbm_get_fields", "lbm_reduce_moments", "lbm_kinetic_energy",
+    "lbm_pressure_drop",
     "lbm_adjoint_pressure_drop",
     "lbm_patches_create", "lbm_patches_destroy", "lbm_patches_last_error", "lbm_patches_cells",
     "lbm_patches_init", "lbm_patches_get_populations", "lbm_patches_autotune", "lbm_patches_set_tiling",
@@ -97,6 +98,7 @@ def load_library(path: str = LIB_PATH):
     L.lbm_synchronize.argtypes = [vp]
     L.lbm_get_fields.argtypes = [vp, dp, dp]
     L.lbm_reduce_moments.argtypes = [vp, dp]
+    L.lbm_kinetic_energy.argtypes = [vp, dp]
     L.lbm_pressure_drop.argtypes = [vp, ctypes.c_int, ctypes.c_int, dp]
     L.lbm_adjoint_pressure_drop.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, dp, dp,
                                             ctypes.POINTER(AdjointStats)]
@@ -245,6 +247,11 @@ class Lattice:
         self._check(self._L.lbm_reduce_moments(self._h, _dp(out)), "lbm_reduce_moments")
         return out
 
+    def kinetic_energy(self):
+        ke = ctypes.c_double()
+        self._check(self._L.lbm_kinetic_energy(self._h, ctypes.byref(ke)), "lbm_kinetic_energy")
+        return ke.value
+
     def pressure_drop(self, xin, xout):
         J = ctypes.c_double()
         self._check(self._L.lbm_pressure_drop(self._h, int(xin), int(xout), ctypes.byref(J)), "lbm_pressure_drop")
@@ -258,6 +265,87 @@ class Lattice:
                                                       ctypes.byref(J), _dp(grad), ctypes.byref(st)),
                     "lbm_adjoint_pressure_drop")
         return J.value, grad, st
+
+
+class PatchSet:
+    """Thousands of cubic patches with ghost layers, one persistent launch per step (an lbm_patchset).
+
+    `table` is a sequence of (n, tau, nbr) with nbr a length-19 sequence of neighbour patch ids per
+    D3Q19 direction (entry 9 is ignored)."""
+
+    def __init__(self, table, precision=FP64, device=0):
+        self._L = load_library()
+        arr = (Patch * len(table))()
+        for k, (n, tau, nbr) in enumerate(table):
+            arr[k].n = int(n)
+            arr[k].tau = float(tau)
+            for q in range(Q):
+                arr[k].nbr[q] = int(nbr[q])
+        h = ctypes.c_void_p()
+        rc = self._L.lbm_patches_create(ctypes.byref(h), arr, len(table), int(precision), int(device))
+        if rc != 0:
+            raise LbmError(f"lbm_patches_create failed ({rc}): {self._L.lbm_patches_last_error(None).decode()}")
+        self._h = h
+        self.edges = [int(t[0]) for t in table]
+        self.cells = int(self._L.lbm_patches_cells(h))
+        self.precision = precision
+
+    def _check(self, rc, what):
+        if rc != 0:
+            raise LbmError(f"{what} failed ({rc}): {self._L.lbm_patches_last_error(self._h).decode()}")
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.lbm_patches_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def init(self, rho=None, u=None):
+        r = None if rho is None else np.ascontiguousarray(np.asarray(rho, dtype=np.float64).reshape(-1))
+        v = None if u is None else np.ascontiguousarray(np.asarray(u, dtype=np.float64).reshape(-1))
+        self._check(self._L.lbm_patches_init(self._h, None if r is None else _dp(r), None if v is None else _dp(v)),
+                    "lbm_patches_init")
+
+    def get_populations(self):
+        """Reference state per patch: a list of arrays shaped (19, n, n, n) (x fastest)."""
+        f = np.empty(Q * self.cells)
+        self._check(self._L.lbm_patches_get_populations(self._h, _dp(f)), "lbm_patches_get_populations")
+        out, o = [], 0
+        for n in self.edges:
+            out.append(f[o:o + Q * n ** 3].reshape(Q, n, n, n))
+            o += Q * n ** 3
+        return out
+
+    def autotune(self):
+        t = Tiling()
+        self._check(self._L.lbm_patches_autotune(self._h, ctypes.byref(t)), "lbm_patches_autotune")
+        return (t.tile_x, t.tile_y, t.tile_z, t.blocks_per_cu)
+
+    def set_tiling(self, tx, ty, tz, blocks_per_cu):
+        t = Tiling(int(tx), int(ty), int(tz), int(blocks_per_cu))
+        self._check(self._L.lbm_patches_set_tiling(self._h, ctypes.byref(t)), "lbm_patches_set_tiling")
+
+    def step(self, n=1, graph=False, timed=False):
+        ms = ctypes.c_float()
+        self._check(self._L.lbm_patches_step(self._h, int(n), 1 if graph else 0, ctypes.byref(ms) if timed else None),
+                    "lbm_patches_step")
+        return float(ms.value) if timed else None
+
+    def moments(self):
+        out = np.empty(4)
+        self._check(self._L.lbm_patches_reduce_moments(self._h, _dp(out)), "lbm_patches_reduce_moments")
+        return out
 
 
 def cavity(n, tau, ulid, precision=FP64, device=0):

@@ -62,6 +62,50 @@ def fourier_porosity(nx: int, ny: int, nz: int, modes: int = 8, seed: int = DEFA
     return np.clip(a, 0.0, 1.0)
 
 
+# D3Q19 directions (z-major), as in the solver
+_C = np.array([[0, -1, -1], [-1, 0, -1], [0, 0, -1], [1, 0, -1], [0, 1, -1], [-1, -1, 0], [0, -1, 0], [1, -1, 0],
+               [-1, 0, 0], [0, 0, 0], [1, 0, 0], [-1, 1, 0], [0, 1, 0], [1, 1, 0], [0, -1, 1], [-1, 0, 1],
+               [0, 0, 1], [1, 0, 1], [0, 1, 1]])
+
+
+def torus_table(n: int, dims, taus, first: int = 0):
+    """Patch table entries for an (a, b, c) torus of edge-n patches: patch (i, j, k) -> id
+    first + i + a*(j + b*k); its neighbour along c_q is the patch at (i, j, k) + c_q (periodic)."""
+    a, b, c = dims
+    out = []
+    for k in range(c):
+        for j in range(b):
+            for i in range(a):
+                nbr = []
+                for q in range(19):
+                    ii, jj, kk = (i + _C[q, 0]) % a, (j + _C[q, 1]) % b, (k + _C[q, 2]) % c
+                    nbr.append(first + ii + a * (jj + b * kk))
+                out.append((n, float(taus[len(out)]), nbr))
+    return out
+
+
+def uneven_patches(count: int = 4096, edges=(8, 16, 24, 32), tau_range=(0.55, 1.5), seed: int = DEFAULT_SEED):
+    """The batched-patch workload: `count` cubic patches with edges drawn uniformly from `edges`
+    and tau uniform in tau_range.  Patches of equal edge are joined into 2x2x2 tori (the remainder
+    of each edge class into one ring), so every face and edge exchange crosses between patches.
+    Returns (table, tori) where tori lists (first_id, dims, n) per torus."""
+    rng = np.random.default_rng(seed)
+    n_of = rng.choice(np.asarray(edges), size=count)
+    taus = rng.uniform(tau_range[0], tau_range[1], size=count)
+    table, tori = [], []
+    for n in edges:
+        members = int((n_of == n).sum())
+        k = 0
+        while k < members:
+            size = 8 if members - k >= 8 else members - k
+            dims = (2, 2, 2) if size == 8 else (size, 1, 1)
+            first = len(table)
+            table += torus_table(int(n), dims, taus[first:first + size], first)
+            tori.append((first, dims, int(n)))
+            k += size
+    return table, tori
+
+
 def taylor_green(n: int, u0: float):
     """Taylor-Green vortex on a periodic n^3 box (k = 2 pi / n): rho and u per cell."""
     k = 2 * np.pi / n

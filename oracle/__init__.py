@@ -48,7 +48,8 @@ class Params(ctypes.Structure):
                 ("moving_mask", ctypes.c_uint32 * MAX_CLASSES),
                 ("g", ctypes.c_double * 3),
                 ("use_force", ctypes.c_int),
-                ("alpha", ctypes.POINTER(ctypes.c_double))]
+                ("alpha", ctypes.POINTER(ctypes.c_double)),
+                ("tau_cell", ctypes.POINTER(ctypes.c_double))]
 
 
 _lib = None
@@ -137,6 +138,12 @@ class Lattice:
     def set_alpha(self, alpha):
         self._alpha = np.ascontiguousarray(alpha.reshape(-1), dtype=np.float64)
         self.params.alpha = _dp(self._alpha)
+
+    def set_tau_cells(self, tau):
+        """Per-cell configured relaxation time (the batched-patch case: tau per patch)."""
+        self._tau_cell = np.ascontiguousarray(np.asarray(tau, dtype=np.float64).reshape(-1))
+        assert self._tau_cell.size == self.n
+        self.params.tau_cell = _dp(self._tau_cell)
 
     @property
     def fluid(self):

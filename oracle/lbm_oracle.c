@@ -201,7 +201,7 @@ void lbo_step(const lbo_dims* d, const uint32_t* words, const lbo_params* p,
         double f[LBO_Q];
         for (int q = 0; q < LBO_Q; q++) f[q] = src[(int64_t)q * N + i];
         double alpha = p->alpha ? p->alpha[i] : 0.0;
-        double rho = collide_cell(f, tau_eff, p, alpha);
+        double rho = collide_cell(f, p->tau_cell ? lbo_tau_eff(p->tau_cell[i]) : tau_eff, p, alpha);
         unsigned cls = (w >> LBO_CLASS_SHIFT) & LBO_CLASS_MASK;
         for (int q = 0; q < LBO_Q; q++) {
           if (w & (1u << q)) {
@@ -245,7 +245,7 @@ void lbo_step_swap_aos(const lbo_dims* d, const uint32_t* words, const lbo_param
     if (!(w & LBO_FLUID)) continue;
     double* fi = f + i * LBO_Q;
     double alpha = p->alpha ? p->alpha[i] : 0.0;
-    double rho = collide_cell(fi, tau_eff, p, alpha);
+    double rho = collide_cell(fi, p->tau_cell ? lbo_tau_eff(p->tau_cell[i]) : tau_eff, p, alpha);
     unsigned cls = (w >> LBO_CLASS_SHIFT) & LBO_CLASS_MASK;
     if (cls)
       for (int q = 0; q < LBO_Q; q++)

@@ -55,6 +55,7 @@ typedef struct {
   double g[3];                             /* body acceleration (Guo forcing); 0 = off */
   int use_force;
   const double* alpha;                     /* per-cell Brinkman porosity or NULL */
+  const double* tau_cell;                  /* per-cell configured tau (overrides tau) or NULL */
 } lbo_params;
 
 /* a1/a8 */
