@@ -1,0 +1,141 @@
+"""Device validation runs (one GPU): precision of the FP32-shifted path against FP64, and the
+large-scale Taylor-Green vortex.
+
+  python -m madpac_amd.validate precision [--size 256 --steps 2000]   porous-channel tau sweep
+  python -m madpac_amd.validate tgv [--size 512 --steps 5000]          Taylor-Green energy decay
+  python -m madpac_amd.validate tgv --size 384 --precisions fp64
+
+The FP64 device path is bit-identical to the CPU oracle (tests/test_gpu_dense.py), so it serves as
+the FP64 reference at sizes the CPU oracle cannot finish in time.  Output: one JSON object.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import sys
+import time
+
+import numpy as np
+
+from . import lbm
+from . import workloads as wl
+
+
+def _log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def _errors(ref, val, fluid):
+    """Relative L2 and Linf of val against ref over fluid cells (per component and vector norm)."""
+    out = {}
+    r, v = ref[fluid], val[fluid]
+    if r.ndim == 1:
+        d = v - r
+        out["L2"] = float(np.linalg.norm(d) / np.linalg.norm(r))
+        out["Linf"] = float(np.max(np.abs(d)) / np.max(np.abs(r)))
+        return out
+    d = v - r
+    out["L2"] = float(np.linalg.norm(d) / np.linalg.norm(r))
+    out["Linf"] = float(np.max(np.abs(d)) / np.max(np.abs(r)))
+    for k, name in enumerate("xyz"):
+        nr = np.linalg.norm(r[:, k])
+        out[f"u{name}_L2"] = float(np.linalg.norm(d[:, k]) / nr) if nr > 0 else 0.0
+    return out
+
+
+def porous_precision_sweep(n=256, steps=2000, taus=(0.55, 0.6, 0.7, 0.8, 1.0), g=1e-6):
+    solid = wl.porous_spheres(n)
+    fluid = ~solid.reshape(-1)
+    rows = []
+    for tau in taus:
+        res = {}
+        for name, prec in (("fp64", lbm.FP64), ("fp32s", lbm.FP32_SHIFTED)):
+            with lbm.Lattice(n, n, n, tau, precision=prec, periodic=(True, True, True), force=(g, 0.0, 0.0)) as lat:
+                lat.set_flags(~solid)
+                lat.init_equilibrium()
+                lat.tune(5)
+                t0 = time.perf_counter()
+                ms = lat.step_timed(steps)
+                res[name] = {"ms_per_step": ms / steps, "fields": lat.fields(), "moments": lat.moments(),
+                             "wall_s": time.perf_counter() - t0}
+        rho64, u64 = res["fp64"]["fields"]
+        rho32, u32 = res["fp32s"]["fields"]
+        eu = _errors(u64, u32, fluid)
+        erho = {"L2": float(np.linalg.norm((rho32 - rho64)[fluid]) / np.linalg.norm((rho64 - 1.0)[fluid])),
+                "Linf_abs": float(np.max(np.abs(rho32 - rho64)[fluid]))}
+        m64, m32 = res["fp64"]["moments"], res["fp32s"]["moments"]
+        row = {"tau": tau, "u": eu, "rho_deviation": erho,
+               "u_max_fp64": float(np.max(np.abs(u64[fluid]))),
+               "mass_fp64": float(m64[0]), "mass_fp32s": float(m32[0]),
+               "momentum_x_fp64": float(m64[1]), "momentum_x_fp32s": float(m32[1]),
+               "ms_per_step": {"fp64": res["fp64"]["ms_per_step"], "fp32s": res["fp32s"]["ms_per_step"]}}
+        _log(f"[validate] tau {tau}: u L2 {eu['L2']:.3e} Linf {eu['Linf']:.3e}, "
+             f"|u|max {row['u_max_fp64']:.3e}")
+        rows.append(row)
+    return {"workload": f"porous channel {n}^3 periodic, Guo g={g} along x, spheres r 4..12 to porosity 0.8, "
+                        f"{steps} steps", "fluid_cells": int(fluid.sum()), "sweep": rows}
+
+
+def taylor_green(n=512, steps=5000, u0=0.04, tau=0.52, precisions=("fp32s", "fp64"), every=250):
+    nu = (tau - 0.5) / 3.0
+    k = 2 * np.pi / n
+    t0 = time.perf_counter()
+    rho, u = wl.taylor_green(n, u0)
+    rho = rho.reshape(-1)
+    u = u.reshape(-1, 3)
+    _log(f"[validate] TGV {n}^3 initial fields ({time.perf_counter() - t0:.1f} s)")
+    out = {"workload": f"Taylor-Green vortex {n}^3 periodic, u0={u0}, tau={tau} (Re = u0 n / (2 pi nu) = "
+                       f"{u0 * n / (2 * np.pi * nu):.0f}), {steps} steps", "runs": {}}
+    curves = {}
+    for name in precisions:
+        prec = lbm.FP64 if name == "fp64" else lbm.FP32_SHIFTED
+        with lbm.Lattice(n, n, n, tau, precision=prec, periodic=(True, True, True)) as lat:
+            lat.set_flags(np.ones(n ** 3, np.uint8))
+            lat.init_equilibrium(rho=rho, u=u)
+            lat.tune(3)
+            lat.init_equilibrium(rho=rho, u=u)
+            E, ts, ms_total = [lat.kinetic_energy()], [0], 0.0
+            for s in range(every, steps + 1, every):
+                ms_total += lat.step_timed(every)
+                E.append(lat.kinetic_energy())
+                ts.append(s)
+            m = lat.moments()
+        esize = 8 if prec == lbm.FP64 else 4
+        per = ms_total * 1e-3 / steps
+        gbs = n ** 3 * 38 * esize / per / 1e9
+        curves[name] = np.array(E)
+        out["runs"][name] = {"ms_per_step": per * 1e3, "mlups": n ** 3 / per / 1e6, "achieved_GBs": gbs,
+                             "hbm_frac": gbs / 8000.0, "energy": E, "t": ts, "mass_drift": float(m[0] - n ** 3)}
+        _log(f"[validate] TGV {name}: {n ** 3 / per / 1e6:.0f} MLUPS, {gbs:.0f} GB/s, E(T)/E0 {E[-1] / E[0]:.5f}")
+    # early-time viscous decay E(t) = E0 exp(-2 nu k_eff^2 t) with k_eff^2 = 3 k^2 for this vortex
+    first = next(iter(curves.values()))
+    t1 = out["runs"][next(iter(curves))]["t"][1]
+    out["early_decay"] = {"measured_rate": float(-np.log(first[1] / first[0]) / t1),
+                          "viscous_rate": float(2 * nu * 3 * k * k)}
+    if "fp64" in curves and "fp32s" in curves:
+        rel = np.abs(curves["fp32s"] - curves["fp64"]) / curves["fp64"]
+        out["energy_rel_diff_fp32s_vs_fp64"] = {"max": float(rel.max()), "final": float(rel[-1])}
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("what", choices=["precision", "tgv"])
+    ap.add_argument("--size", type=int, default=None)
+    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--precisions", default="fp32s,fp64")
+    ap.add_argument("--out", default=None)
+    a = ap.parse_args()
+    if a.what == "precision":
+        r = porous_precision_sweep(n=a.size or 256, steps=a.steps or 2000)
+    else:
+        r = taylor_green(n=a.size or 512, steps=a.steps or 5000, precisions=tuple(a.precisions.split(",")))
+    s = json.dumps(r)
+    if a.out:
+        with open(a.out, "w") as fh:
+            fh.write(s + "\n")
+    print(s, flush=True)
+
+
+if __name__ == "__main__":
+    main()

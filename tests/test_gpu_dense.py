@@ -171,6 +171,25 @@ def test_moments_and_pressure_drop_match_oracle():
         assert abs(J - Jo) <= (1e-13 if prec == lbm.FP64 else 1e-7)
 
 
+def test_kinetic_energy_matches_oracle_taylor_green():
+    wl = pytest.importorskip("madpac_amd.workloads")
+    n, steps, tau, u0 = 16, 40, 0.6, 0.03
+    rho0, u0f = wl.taylor_green(n, u0)
+    ora = O.Lattice(n, n, n, periodic=(True, True, True))
+    ora.set_solid(np.zeros((n, n, n), np.uint8))
+    ora.params.tau = tau
+    fo = ora.step(ora.equilibrium(rho=rho0, u=u0f), steps)
+    rho, u = ora.fields(fo)
+    ke_o = float(np.sum(0.5 * rho * np.sum(u * u, axis=1)))
+    for prec, tol in ((lbm.FP64, 1e-12), (lbm.FP32_SHIFTED, 1e-5)):
+        with lbm.Lattice(n, n, n, tau, precision=prec, periodic=(True, True, True)) as lat:
+            lat.set_flags(np.ones(n ** 3, np.uint8))
+            lat.init_equilibrium(rho=rho0, u=u0f)
+            lat.step(steps)
+            ke = lat.kinetic_energy()
+        assert abs(ke - ke_o) <= tol * ke_o, (prec, ke, ke_o)
+
+
 def test_fp32_shifted_cavity_error_vs_fp64_oracle():
     # FP32 shifted populations against the FP64 oracle: velocity L2 / Linf relative errors
     n, steps = 24, 600
