@@ -69,15 +69,44 @@ int64_t lbm_fluid_cells(const lbm_ctx* ctx);
  * (BLB/boundaryconditions/BlockHalfwayBounceBack.cpp:20-22, implicit in the swap streaming
  * of BLB/nonlocaltreatment/AABlockStreaming.cpp:19-53). */
 int lbm_set_flags(lbm_ctx* ctx, const uint8_t* inner);
-/* Moving-wall class k (1..7): wall velocity uw and the mask of outgoing directions q whose
- * bounce-back links are moving walls (BLB/boundaryconditions/BlockMovingWall.cpp:13-33).
- * cls[i] selects the class of each cell (0 = none); NULL clears all classes. */
+/* Boundary-condition types of a class: the reference's BoundaryData::BoundaryType
+ * (LB/BoundaryData.h:44-61) with the handlers of
+ * BLB/boundaryconditions/BlockBoundaryConditionManager.cpp:15-20.  Halfway bounce-back is the
+ * default of every link to a non-fluid cell and needs no class. */
+enum {
+  LBM_BC_MOVING_WALL = 0,     /* BlockMovingWall.cpp:13-33: f_q -= 2 w_q rho (u_w . c_q) / cs2 */
+  LBM_BC_PRESSURE_OUTLET = 1, /* BlockPressureOutletCondition.h: rho_L, the cell's velocity */
+  LBM_BC_PRESSURE_INLET = 2,  /* BlockPressureInletCondition.h: u_L, the cell's density */
+  LBM_BC_VELOCITY_INLET = 3   /* BlockVelocityInletEquilibrium.cpp:12-24: feq(rho_L, u_L) */
+};
+/* or-ed into an inlet type: u_L per cell from lbm_set_boundary_velocity */
+#define LBM_BC_CELL_VELOCITY 16
+/* Boundary class k (1..31): condition `type` with target density rho (rho_L) and velocity u
+ * (u_w for a moving wall, u_L for an inlet) on the outgoing directions q in dir_mask whose
+ * links leave the fluid; the cell's other links stay halfway bounce-back.  This replaces the
+ * per-link BoundaryData list of the reference (SLB/services/ReceiveBoundaryDataService.cpp:
+ * 107-185) and its dispatch in BlockBoundaryConditionManager::treatBoundaryAfterCollision
+ * (:51-62).  The pressure conditions apply -f_q + feq_{18-q} + feq_q
+ * (BLB/boundaryconditions/BlockAbstractPressureCondition.cpp:24-37). */
+int lbm_set_boundary_class(lbm_ctx* ctx, int k, int type, double rho, const double u[3], uint32_t dir_mask);
+/* Moving-wall class k (1..31): lbm_set_boundary_class with LBM_BC_MOVING_WALL and rho = 1. */
 int lbm_set_moving_wall_class(lbm_ctx* ctx, int k, const double uw[3], uint32_t dir_mask);
+/* cls[i] selects the class of each cell (0 = none); NULL clears all classes. */
 int lbm_set_wall_classes(lbm_ctx* ctx, const uint8_t* cls);
+/* Per-cell target velocities u[3*i + d] read by classes typed with LBM_BC_CELL_VELOCITY (the
+ * reference's position-dependent BoundaryData, e.g. a parabolic inlet); NULL frees them. */
+int lbm_set_boundary_velocity(lbm_ctx* ctx, const double* u);
 /* The reference's lid-driven-cavity classification (SLB/BoundaryData4LidDrivenCavity.cpp:17-55
  * with SLB/services/ReceiveBoundaryDataService.cpp:107-185): links of the top fluid layer
  * with c_z > 0 that leave the fluid are moving walls with u_w = (ulid, 0, 0). */
 int lbm_set_cavity_lid(lbm_ctx* ctx, double ulid);
+/* The reference's default channel (SLB/AbstractBoundaryData4Channel.cpp:17-101,
+ * SLB/BoundaryData4ChannelWithNoSlipBoundaries.cpp:17-27) on a non-periodic box: links leaving
+ * through a y or z face are no-slip walls; other links leaving through x- are a velocity inlet
+ * with u_L = (u_mean, 0, 0), or with parabolic != 0 the profile
+ * u_mean * prod_{d=y,z} 6 a_d (L_d - a_d) / L_d^2 at the cell centre (lattice units), rho_L = 1;
+ * links leaving through x+ are a pressure outlet with rho_L = 1.  Uses classes 1..18. */
+int lbm_set_channel(lbm_ctx* ctx, double u_mean, int parabolic);
 /* Per-cell Brinkman porosity alpha in [0,1] (NULL disables).  Equilibrium velocity is
  * scaled by (1 - alpha).  No reference counterpart (design variable of the adjoint). */
 int lbm_set_porosity(lbm_ctx* ctx, const double* alpha);

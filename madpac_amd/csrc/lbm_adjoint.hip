@@ -61,7 +61,8 @@ template <typename T, bool FORCE, bool ALPHA, bool WALLS>
 __global__ __launch_bounds__(BX * BY) void k_adj_step(DenseGeom g, const T* __restrict__ s,
                                                       const T* __restrict__ lam_out, T* __restrict__ lam_in,
                                                       const uint32_t* __restrict__ words, const T* __restrict__ alpha,
-                                                      double* __restrict__ galpha, CollideParams p, WallTable wt) {
+                                                      double* __restrict__ galpha, CollideParams p,
+                                                      const BoundaryClass* __restrict__ bct) {
   const CellPos c = map_cell(g);
   if (!c.valid) return;
   const uint32_t w = words[c.i];
@@ -84,12 +85,14 @@ __global__ __launch_bounds__(BX * BY) void k_adj_step(DenseGeom g, const T* __re
   const R one = R(1), half = R(0.5), three = R(3), nine = R(9), four5 = R(4.5), one5 = R(1.5);
   R drho_wall = R(0);
   if (WALLS) {
-    const uint32_t k = (w >> CLASS_SHIFT) & (MAX_CLASSES - 1);
+    // moving walls only (the host rejects other boundary types)
+    const uint32_t k = (w >> CLASS_SHIFT) & CLASS_BITS;
     if (k) {
-      const uint32_t m = w & wt.mask[k];
+      const uint32_t m = w & bct[k].mask;
+      const double uw[3] = {bct[k].u[0], bct[k].u[1], bct[k].u[2]};
 #pragma unroll
       for (int q = 0; q < Q; q++)
-        if ((m >> q) & 1u) drho_wall -= mu[q] * (R)moving_wall_f64(q, wt.uw[k], 1.0);
+        if ((m >> q) & 1u) drho_wall -= mu[q] * (R)moving_wall_f64(q, uw, 1.0);
     }
   }
   // rho, j (the FP32 state holds f - w; the rest part adds to rho only)
@@ -110,33 +113,63 @@ __global__ __launch_bounds__(BX * BY) void k_adj_step(DenseGeom g, const T* __re
   const R v0 = sc * (u0 + half * g0), v1 = sc * (u1 + half * g1), v2 = sc * (u2 + half * g2);
   const R vv = v0 * v0 + v1 * v1 + v2 * v2;
   const R fpref = FORCE ? (one - half * om) : R(0);
-  R dPr = R(0), dv0 = R(0), dv1 = R(0), dv2 = R(0);
+  // Moment form.  With M0 = sum mu w, M1 = sum mu w c, M2 = sum mu w c c^T (w_q the lattice
+  // weights), the per-population sums of the tape collapse to
+  //   dPhi/drho = om [M0 (1 - 1.5 v.v) + 3 v.M1 + 4.5 v^T M2 v] + fpref [3 (M1.g - M0 v.g) + 9 v^T M2 g]
+  //   dPhi/dv   = rho { om [3 M1 + 9 M2 v - 3 M0 v] + fpref [9 M2 g - 3 M0 g] }
+  // and lambda_q = (1 - om) mu_q + A + B.c_q is affine in c_q.
+  R M0 = R(0), m1x = R(0), m1y = R(0), m1z = R(0);
+  R mxx = R(0), myy = R(0), mzz = R(0), mxy = R(0), mxz = R(0), myz = R(0);
 #pragma unroll
   for (int q = 0; q < Q; q++) {
-    const R cx = (R)CX(q), cy = (R)CY(q), cz = (R)CZ(q);
-    const R cv = cx * v0 + cy * v1 + cz * v2;
-    const R cg = cx * g0 + cy * g1 + cz * g2;
-    const R vg = v0 * g0 + v1 * g1 + v2 * g2;
-    const R wq = (R)W64(q);
-    const R feq_over_rho = wq * (one + three * cv + four5 * cv * cv - one5 * vv);
-    const R S_over_rho = fpref * wq * (three * (cg - vg) + nine * cv * cg);
-    dPr += mu[q] * (om * feq_over_rho + S_over_rho);
-    const R d0 = wq * rho * (three * cx + nine * cv * cx - three * v0);
-    const R d1 = wq * rho * (three * cy + nine * cv * cy - three * v1);
-    const R d2 = wq * rho * (three * cz + nine * cv * cz - three * v2);
-    const R e0 = fpref * wq * rho * (-three * g0 + nine * cx * cg);
-    const R e1 = fpref * wq * rho * (-three * g1 + nine * cy * cg);
-    const R e2 = fpref * wq * rho * (-three * g2 + nine * cz * cg);
-    dv0 += mu[q] * (om * d0 + e0);
-    dv1 += mu[q] * (om * d1 + e1);
-    dv2 += mu[q] * (om * d2 + e2);
+    const R wm = (R)W64(q) * mu[q];
+    M0 += wm;
+    if (CX(q) == 1) m1x += wm;
+    if (CX(q) == -1) m1x -= wm;
+    if (CY(q) == 1) m1y += wm;
+    if (CY(q) == -1) m1y -= wm;
+    if (CZ(q) == 1) m1z += wm;
+    if (CZ(q) == -1) m1z -= wm;
+    if (CX(q) != 0) mxx += wm;
+    if (CY(q) != 0) myy += wm;
+    if (CZ(q) != 0) mzz += wm;
+    if (CX(q) * CY(q) == 1) mxy += wm;
+    if (CX(q) * CY(q) == -1) mxy -= wm;
+    if (CX(q) * CZ(q) == 1) mxz += wm;
+    if (CX(q) * CZ(q) == -1) mxz -= wm;
+    if (CY(q) * CZ(q) == 1) myz += wm;
+    if (CY(q) * CZ(q) == -1) myz -= wm;
   }
+  // M2 v and M2 g
+  const R m2v0 = mxx * v0 + mxy * v1 + mxz * v2;
+  const R m2v1 = mxy * v0 + myy * v1 + myz * v2;
+  const R m2v2 = mxz * v0 + myz * v1 + mzz * v2;
+  const R m2g0 = mxx * g0 + mxy * g1 + mxz * g2;
+  const R m2g1 = mxy * g0 + myy * g1 + myz * g2;
+  const R m2g2 = mxz * g0 + myz * g1 + mzz * g2;
+  const R vm1 = v0 * m1x + v1 * m1y + v2 * m1z;
+  const R vm2v = v0 * m2v0 + v1 * m2v1 + v2 * m2v2;
+  const R gm1 = g0 * m1x + g1 * m1y + g2 * m1z;
+  const R vg = v0 * g0 + v1 * g1 + v2 * g2;
+  const R vm2g = v0 * m2g0 + v1 * m2g1 + v2 * m2g2;
+  const R dPr = om * (M0 * (one - one5 * vv) + three * vm1 + four5 * vm2v) +
+                fpref * (three * (gm1 - M0 * vg) + nine * vm2g);
+  const R dv0 = rho * (om * (three * m1x + nine * m2v0 - three * M0 * v0) + fpref * (nine * m2g0 - three * M0 * g0));
+  const R dv1 = rho * (om * (three * m1y + nine * m2v1 - three * M0 * v1) + fpref * (nine * m2g1 - three * M0 * g1));
+  const R dv2 = rho * (om * (three * m1z + nine * m2v2 - three * M0 * v2) + fpref * (nine * m2g2 - three * M0 * g2));
+  const R sr = sc / rho;
+  const R A = dPr + drho_wall - sr * (dv0 * u0 + dv1 * u1 + dv2 * u2);
+  const R B0 = sr * dv0, B1 = sr * dv1, B2 = sr * dv2;
+  const R omc = one - om;
 #pragma unroll
   for (int q = 0; q < Q; q++) {
-    R l = (one - om) * mu[q] + dPr + drho_wall;
-    l += dv0 * sc * ((R)CX(q) - u0) / rho;
-    l += dv1 * sc * ((R)CY(q) - u1) / rho;
-    l += dv2 * sc * ((R)CZ(q) - u2) / rho;
+    R l = omc * mu[q] + A;
+    if (CX(q) == 1) l += B0;
+    if (CX(q) == -1) l -= B0;
+    if (CY(q) == 1) l += B1;
+    if (CY(q) == -1) l -= B1;
+    if (CZ(q) == 1) l += B2;
+    if (CZ(q) == -1) l -= B2;
     lam_in[(int64_t)q * N + c.i] = l;
   }
   if (ALPHA) {
@@ -151,14 +184,14 @@ __global__ __launch_bounds__(BX * BY) void k_adj_step(DenseGeom g, const T* __re
 template <typename T>
 static hipError_t adj_step_t(const DenseGeom& g, const void* s, const void* lo, void* li, const uint32_t* words,
                              const void* alpha, double* galpha, bool force, bool walls, const CollideParams& p,
-                             const WallTable& wt, hipStream_t st) {
+                             const BoundaryClass* bct, hipStream_t st) {
   const dim3 grid((unsigned)g.nblocks_padded), block(BX, BY);
   const T* sp = (const T*)s;
   const T* lop = (const T*)lo;
   T* lip = (T*)li;
   const T* ap = (const T*)alpha;
 #define LBM_ADJ(F, A, W) \
-  hipLaunchKernelGGL((k_adj_step<T, F, A, W>), grid, block, 0, st, g, sp, lop, lip, words, ap, galpha, p, wt)
+  hipLaunchKernelGGL((k_adj_step<T, F, A, W>), grid, block, 0, st, g, sp, lop, lip, words, ap, galpha, p, bct)
   const int key = (force ? 4 : 0) | (alpha ? 2 : 0) | (walls ? 1 : 0);
   switch (key) {
     case 0: LBM_ADJ(false, false, false); break;
@@ -237,6 +270,9 @@ extern "C" int lbm_adjoint_pressure_drop(lbm_ctx* c, int nsteps, int snapshots, 
   if (snapshots < 0 || snapshots > 64) return adj_fail(c, LBM_ERR_ARG, "snapshots must be in 0..64");
   if (xin < 0 || xin >= c->g.nx || xout < 0 || xout >= c->g.nx) return adj_fail(c, LBM_ERR_ARG, "plane out of range");
   if (!c->flags_set || !c->state_set) return adj_fail(c, LBM_ERR_STATE, "flags and state must be set first");
+  for (int k = 1; k < MAX_CLASSES; k++)
+    if (c->bct[k].mask && (c->bct[k].type & BC_TYPE_MASK) != BC_MOVING_WALL)
+      return adj_fail(c, LBM_ERR_STATE, "the adjoint supports halfway bounce-back and moving walls only");
   if (hipSetDevice(c->cfg.device) != hipSuccess) return adj_fail(c, LBM_ERR_HIP, "hipSetDevice failed");
 
   const int64_t N = c->g.N;
@@ -345,9 +381,9 @@ extern "C" int lbm_adjoint_pressure_drop(lbm_ctx* c, int nsteps, int snapshots, 
   };
   auto adj_step = [&]() -> int {
     hipError_t e = c->fp32 ? adj_step_t<float>(c->g, c->buf[c->cur], a->lam[lc], a->lam[lc ^ 1], c->words, c->alpha,
-                                               a->galpha, c->force, c->walls, c->p64, c->wt, st)
+                                               a->galpha, c->force, c->walls, c->p64, c->d_bct, st)
                            : adj_step_t<double>(c->g, c->buf[c->cur], a->lam[lc], a->lam[lc ^ 1], c->words, c->alpha,
-                                                a->galpha, c->force, c->walls, c->p64, c->wt, st);
+                                                a->galpha, c->force, c->walls, c->p64, c->d_bct, st);
     if (e != hipSuccess) return adj_fail(c, LBM_ERR_HIP, std::string("adjoint step: ") + hipGetErrorString(e));
     lc ^= 1;
     reversed++;

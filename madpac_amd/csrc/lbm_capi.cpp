@@ -58,6 +58,14 @@ int rebuild_words(lbm_ctx* c) {
   return LBM_OK;
 }
 
+int upload_boundary_table(lbm_ctx* c) {
+  HIPCHK(c, hipMemcpyAsync(c->d_bct, c->bct, sizeof(c->bct), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->walls = false;
+  for (int j = 1; j < lbm::MAX_CLASSES; j++) c->walls = c->walls || c->bct[j].mask != 0u;
+  return LBM_OK;
+}
+
 int need_state(lbm_ctx* c) {
   if (!c->flags_set) return fail(c, LBM_ERR_STATE, "boundary flags not set (lbm_set_flags)");
   if (!c->state_set) return fail(c, LBM_ERR_STATE, "state not initialised (lbm_init_equilibrium / lbm_set_populations)");
@@ -102,7 +110,7 @@ int lbm_create(lbm_ctx** out, const lbm_config* cfg) {
   c->p32.omega = (float)c->p64.omega;
   c->p32.fpref = (float)(1.0 - 0.5 * c->p64.omega);
   for (int k = 0; k < 3; k++) c->p32.g[k] = (float)cfg->force[k];
-  std::memset(&c->wt, 0, sizeof(c->wt));
+  for (int k = 0; k < lbm::MAX_CLASSES; k++) c->bct[k] = lbm::BoundaryClass{{0.0, 0.0, 0.0}, 1.0, 0u, 0u};
 
   auto bail = [&](int code, const char* msg) {
     g_create_error = msg;
@@ -120,12 +128,14 @@ int lbm_create(lbm_ctx** out, const lbm_config* cfg) {
       hipMalloc(&c->words, (size_t)c->g.N * sizeof(uint32_t)) != hipSuccess ||
       hipMalloc(&c->d_inner, (size_t)c->g.Nu) != hipSuccess || hipMalloc(&c->d_cls, (size_t)c->g.Nu) != hipSuccess ||
       hipMalloc(&c->partials, (size_t)(c->npartials + 1024) * 4 * sizeof(double)) != hipSuccess ||
-      hipMalloc(&c->small, 8 * sizeof(double)) != hipSuccess)
+      hipMalloc(&c->small, 8 * sizeof(double)) != hipSuccess ||
+      hipMalloc(&c->d_bct, sizeof(c->bct)) != hipSuccess)
     return bail(LBM_ERR_NOMEM, "device allocation failed");
   if (hipMemsetAsync(c->buf[0], 0, pbytes, c->stream) != hipSuccess ||
       hipMemsetAsync(c->buf[1], 0, pbytes, c->stream) != hipSuccess ||
       hipMemsetAsync(c->words, 0, (size_t)c->g.N * sizeof(uint32_t), c->stream) != hipSuccess ||
       hipMemsetAsync(c->d_cls, 0, (size_t)c->g.Nu, c->stream) != hipSuccess ||
+      hipMemcpyAsync(c->d_bct, c->bct, sizeof(c->bct), hipMemcpyHostToDevice, c->stream) != hipSuccess ||
       hipStreamSynchronize(c->stream) != hipSuccess)
     return bail(LBM_ERR_HIP, "initial clear failed");
   *out = c;
@@ -136,7 +146,8 @@ int lbm_destroy(lbm_ctx* c) {
   if (!c) return LBM_OK;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   lbm_adjoint_release(c);
-  void* ptrs[] = {c->buf[0], c->buf[1], c->words, c->d_inner, c->d_cls, c->alpha, c->partials, c->small, c->scratch};
+  void* ptrs[] = {c->buf[0], c->buf[1], c->words,   c->d_inner, c->d_cls, c->alpha,
+                  c->partials, c->small,  c->scratch, c->d_bct,   c->d_bcu};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -159,17 +170,112 @@ int lbm_set_flags(lbm_ctx* c, const uint8_t* inner) {
   return rebuild_words(c);
 }
 
-int lbm_set_moving_wall_class(lbm_ctx* c, int k, const double uw[3], uint32_t dir_mask) {
+int lbm_set_boundary_class(lbm_ctx* c, int k, int type, double rho, const double u[3], uint32_t dir_mask) {
   if (!c) return LBM_ERR_ARG;
-  if (k < 1 || k >= lbm::MAX_CLASSES) return fail(c, LBM_ERR_ARG, "moving-wall class must be in 1..7");
-  if (!uw) return fail(c, LBM_ERR_ARG, "wall velocity is null");
+  if (k < 1 || k >= lbm::MAX_CLASSES) return fail(c, LBM_ERR_ARG, "boundary class must be in 1..31");
+  const int ty = type & (int)lbm::BC_TYPE_MASK;
+  if ((type & ~(int)(lbm::BC_TYPE_MASK | lbm::BC_CELL_VELOCITY)) || ty > (int)lbm::BC_VELOCITY_INLET)
+    return fail(c, LBM_ERR_ARG, "unknown boundary type");
+  if ((type & LBM_BC_CELL_VELOCITY) && ty != LBM_BC_PRESSURE_INLET && ty != LBM_BC_VELOCITY_INLET)
+    return fail(c, LBM_ERR_ARG, "per-cell velocities apply to inlet classes only");
+  if (!u) return fail(c, LBM_ERR_ARG, "boundary velocity is null");
   for (int d = 0; d < 3; d++)
-    if (!std::isfinite(uw[d])) return fail(c, LBM_ERR_ARG, "wall velocity must be finite");
-  for (int d = 0; d < 3; d++) c->wt.uw[k][d] = uw[d];
-  c->wt.mask[k] = dir_mask & ((1u << Q) - 1u) & ~(1u << lbm::REST);
-  c->walls = false;
-  for (int j = 1; j < lbm::MAX_CLASSES; j++) c->walls = c->walls || c->wt.mask[j] != 0u;
+    if (!std::isfinite(u[d])) return fail(c, LBM_ERR_ARG, "boundary velocity must be finite");
+  if (!(rho > 0.0) || !std::isfinite(rho)) return fail(c, LBM_ERR_ARG, "boundary density must be positive");
+  HIPCHK(c, hipSetDevice(c->cfg.device));
+  lbm::BoundaryClass& b = c->bct[k];
+  for (int d = 0; d < 3; d++) b.u[d] = u[d];
+  b.rho = rho;
+  b.type = (uint32_t)type;
+  b.mask = dir_mask & ((1u << Q) - 1u) & ~(1u << lbm::REST);
+  return upload_boundary_table(c);
+}
+
+int lbm_set_moving_wall_class(lbm_ctx* c, int k, const double uw[3], uint32_t dir_mask) {
+  return lbm_set_boundary_class(c, k, LBM_BC_MOVING_WALL, 1.0, uw, dir_mask);
+}
+
+int lbm_set_boundary_velocity(lbm_ctx* c, const double* u) {
+  if (!c) return LBM_ERR_ARG;
+  HIPCHK(c, hipSetDevice(c->cfg.device));
+  if (!u) {
+    if (c->d_bcu) {
+      HIPCHK(c, hipStreamSynchronize(c->stream));
+      (void)hipFree(c->d_bcu);
+    }
+    c->d_bcu = nullptr;
+    return LBM_OK;
+  }
+  const size_t n = 3 * (size_t)c->g.Nu;
+  for (size_t i = 0; i < n; i++)
+    if (!std::isfinite(u[i])) return fail(c, LBM_ERR_ARG, "boundary velocities must be finite");
+  if (!c->d_bcu && hipMalloc(&c->d_bcu, n * sizeof(double)) != hipSuccess)
+    return fail(c, LBM_ERR_NOMEM, "boundary velocity allocation failed");
+  HIPCHK(c, hipMemcpyAsync(c->d_bcu, u, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
   return LBM_OK;
+}
+
+// The reference's default channel (SLB/AbstractBoundaryData4Channel.cpp:17-50 with
+// SLB/BoundaryData4ChannelWithNoSlipBoundaries.cpp:17-27), classified link by link: a link
+// whose target leaves the box through a y or z face is a no-slip wall; otherwise a link
+// leaving through x- is a velocity inlet (rho_L = 1, LB/BoundaryData.cpp:5; u_L = (u_mean,0,0)
+// or the parabolic profile of :72-77 at the cell centre, in lattice units), one leaving through
+// x+ a pressure outlet with rho_L = 1 (:85-92); links into solid cells stay walls.  The inlet
+// and outlet links of one cell form a class per distinct link set (at most 9 per face).
+int lbm_set_channel(lbm_ctx* c, double u_mean, int parabolic) {
+  if (!c) return LBM_ERR_ARG;
+  const lbm::DenseGeom& g = c->g;
+  if (c->cfg.periodic[0] || c->cfg.periodic[1] || c->cfg.periodic[2])
+    return fail(c, LBM_ERR_ARG, "a channel needs a non-periodic box");
+  if (g.nx < 2) return fail(c, LBM_ERR_ARG, "a channel needs nx >= 2");
+  if (!std::isfinite(u_mean)) return fail(c, LBM_ERR_ARG, "inlet velocity must be finite");
+  std::vector<uint8_t> cls((size_t)g.Nu, 0);
+  std::vector<double> prof;
+  if (parabolic) prof.assign(3 * (size_t)g.Nu, 0.0);
+  lbm::BoundaryClass table[lbm::MAX_CLASSES];
+  for (int k = 0; k < lbm::MAX_CLASSES; k++) table[k] = lbm::BoundaryClass{{0.0, 0.0, 0.0}, 1.0, 0u, 0u};
+  int ncls = 1;
+  for (int z = 0; z < g.nz; z++)
+    for (int y = 0; y < g.ny; y++)
+      for (int x : {0, g.nx - 1}) {
+        const int64_t i = (int64_t)x + (int64_t)g.nx * ((int64_t)y + (int64_t)g.ny * z);
+        uint32_t m = 0;
+        for (int q = 0; q < Q; q++) {
+          const int tx = x + lbm::CX(q), ty = y + lbm::CY(q), tz = z + lbm::CZ(q);
+          if (ty < 0 || ty >= g.ny || tz < 0 || tz >= g.nz) continue;  // wall first
+          if ((x == 0 && tx < 0) || (x == g.nx - 1 && tx >= g.nx)) m |= 1u << q;
+        }
+        if (!m) continue;
+        const bool in = x == 0;
+        const uint32_t type = in ? (lbm::BC_VELOCITY_INLET | (parabolic ? lbm::BC_CELL_VELOCITY : 0u))
+                                 : lbm::BC_PRESSURE_OUTLET;
+        int k = 1;
+        while (k < ncls && !(table[k].mask == m && table[k].type == type)) k++;
+        if (k == ncls) {
+          if (ncls == lbm::MAX_CLASSES) return fail(c, LBM_ERR_ARG, "too many boundary classes");
+          ncls++;
+          table[k].mask = m;
+          table[k].type = type;
+          if (in) table[k].u[0] = u_mean;
+        }
+        cls[i] = (uint8_t)k;
+        if (in && parabolic) {
+          // vel(0) = vel(0)*6/L^2 * a (L - a) for d = y, z at the cell centre a = index + 1/2
+          double v = u_mean;
+          const double L[2] = {(double)g.ny, (double)g.nz};
+          const double a[2] = {y + 0.5, z + 0.5};
+          for (int e = 0; e < 2; e++) v = v * 6.0 / (L[e] * L[e]) * a[e] * (L[e] - a[e]);
+          prof[3 * i] = v;
+        }
+      }
+  HIPCHK(c, hipSetDevice(c->cfg.device));
+  int rc = parabolic ? lbm_set_boundary_velocity(c, prof.data()) : LBM_OK;
+  if (rc) return rc;
+  std::memcpy(c->bct, table, sizeof(table));
+  rc = upload_boundary_table(c);
+  if (rc) return rc;
+  return lbm_set_wall_classes(c, cls.data());
 }
 
 int lbm_set_wall_classes(lbm_ctx* c, const uint8_t* cls) {
@@ -177,7 +283,7 @@ int lbm_set_wall_classes(lbm_ctx* c, const uint8_t* cls) {
   HIPCHK(c, hipSetDevice(c->cfg.device));
   if (cls) {
     for (int64_t i = 0; i < c->g.Nu; i++)
-      if (cls[i] >= lbm::MAX_CLASSES) return fail(c, LBM_ERR_ARG, "wall class id must be < 8");
+      if (cls[i] >= lbm::MAX_CLASSES) return fail(c, LBM_ERR_ARG, "boundary class id must be < 32");
     HIPCHK(c, hipMemcpyAsync(c->d_cls, cls, (size_t)c->g.Nu, hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->has_cls = true;
@@ -284,7 +390,8 @@ int lbm_get_populations(lbm_ctx* c, double* f) {
 int lbm_enqueue_steps(lbm_ctx* c, int nsteps) {
   for (int s = 0; s < nsteps; s++) {
     HIPCHK(c, lbm::launch_dense_step(c->g, c->fp32, c->buf[c->cur], c->buf[c->cur ^ 1], c->words, c->alpha, c->force,
-                                     c->walls, c->p64, c->p32, c->wt, c->spec, c->stream));
+                                     c->walls, c->p64, c->p32, lbm::BoundaryArgs{c->d_bct, c->d_bcu},
+                                     c->spec, c->stream));
     c->cur ^= 1;
   }
   return LBM_OK;

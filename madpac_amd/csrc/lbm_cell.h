@@ -18,9 +18,9 @@ namespace lbm {
 constexpr int Q = 19;
 constexpr int REST = 9;
 constexpr uint32_t FLUID = 0x80000000u;
-constexpr int CLASS_SHIFT = 24;
-constexpr uint32_t CLASS_MASK = 0x7Fu;
-constexpr int MAX_CLASSES = 8;
+constexpr int CLASS_SHIFT = 24;       // bits 24..28 of the cell word: boundary class
+constexpr int MAX_CLASSES = 32;       // class 0 = plain halfway bounce-back
+constexpr uint32_t CLASS_BITS = MAX_CLASSES - 1;
 
 // LB/LatticeBoltzmannDefinitions.cpp:113-131 (z-major; q' = 18 - q)
 __host__ __device__ constexpr int CX(int q) {
@@ -42,11 +42,21 @@ __host__ __device__ constexpr float W32(int q) {
        : (q == 2 || q == 6 || q == 8 || q == 10 || q == 12 || q == 16) ? 1.0f / 18.0f : 1.0f / 36.0f;
 }
 
-// Moving-wall classes and body force, uploaded per context into constant memory slots that
-// the kernels receive by value.
-struct WallTable {
-  double uw[MAX_CLASSES][3];
-  uint32_t mask[MAX_CLASSES];
+// Boundary classes.  A fluid cell's class selects one condition and its target state; the
+// class mask says which of the cell's bounce-back links take it (the others stay halfway
+// bounce-back).  Types follow LB/BoundaryData.h:44-61 and the handlers registered in
+// BLB/boundaryconditions/BlockBoundaryConditionManager.cpp:15-20.  BC_CELL_VELOCITY or-ed into
+// the type takes the target velocity per cell from a separate [3*N] array.  The table lives in
+// device memory (MAX_CLASSES entries) and is read only by cells that carry a class.
+enum : uint32_t { BC_MOVING_WALL = 0, BC_PRESSURE_OUTLET = 1, BC_PRESSURE_INLET = 2, BC_VELOCITY_INLET = 3 };
+constexpr uint32_t BC_TYPE_MASK = 15u;
+constexpr uint32_t BC_CELL_VELOCITY = 16u;
+
+struct BoundaryClass {
+  double u[3];     // wall velocity u_w, or target velocity u_L
+  double rho;      // target density rho_L
+  uint32_t mask;   // bit q: outgoing link q (a bounce-back link of the cell) takes this condition
+  uint32_t type;   // BC_* | BC_CELL_VELOCITY
 };
 
 struct CollideParams {
@@ -74,8 +84,11 @@ __device__ __forceinline__ double cdot64(int q, double v0, double v1, double v2)
 // FP64, reference operation order.  f[] in, post-collision f*[] out (in place).  Returns rho.
 // FORCE: Guo forcing with equilibrium velocity u + g/2.  ALPHA: equilibrium velocity scaled by
 // (1 - alpha).  Both go beyond the reference and match the oracle's collide_cell.
+// uo receives the cell velocity as the boundary conditions read it: (sum f c) / rho, plus g/2
+// with a body force, before any porosity scaling.
 template <bool FORCE, bool ALPHA>
-__device__ __forceinline__ double collide_f64(double (&f)[Q], const CollideParams& p, double alpha) {
+__device__ __forceinline__ double collide_f64(double (&f)[Q], const CollideParams& p, double alpha,
+                                              double (&uo)[3]) {
   // computeDensity (BLB/BlockComputeCellValues.cpp:36): sequential sum from 0.0
   double rho = 0.0;
 #pragma unroll
@@ -99,6 +112,9 @@ __device__ __forceinline__ double collide_f64(double (&f)[Q], const CollideParam
     u1 = u1 + 0.5 * p.g[1];
     u2 = u2 + 0.5 * p.g[2];
   }
+  uo[0] = u0;
+  uo[1] = u1;
+  uo[2] = u2;
   if (ALPHA) {
     u0 = (1.0 - alpha) * u0;
     u1 = (1.0 - alpha) * u1;
@@ -148,8 +164,11 @@ struct CollideParams32 {
   float g[3];
 };
 
+// uo receives the velocity the boundary conditions read (as in collide_f64) and, in uo[3],
+// the density deviation rho - 1 as summed from the shifted populations.
 template <bool FORCE, bool ALPHA>
-__device__ __forceinline__ float collide_f32s(float (&g)[Q], const CollideParams32& p, float alpha) {
+__device__ __forceinline__ float collide_f32s(float (&g)[Q], const CollideParams32& p, float alpha,
+                                              float (&uo)[4]) {
   float drho = 0.f, j0 = 0.f, j1 = 0.f, j2 = 0.f;
 #pragma unroll
   for (int q = 0; q < Q; q++) {
@@ -169,6 +188,10 @@ __device__ __forceinline__ float collide_f32s(float (&g)[Q], const CollideParams
     u1 += 0.5f * p.g[1];
     u2 += 0.5f * p.g[2];
   }
+  uo[0] = u0;
+  uo[1] = u1;
+  uo[2] = u2;
+  uo[3] = drho;
   if (ALPHA) {
     const float s = 1.0f - alpha;
     u0 *= s; u1 *= s; u2 *= s;
@@ -193,6 +216,93 @@ __device__ __forceinline__ float collide_f32s(float (&g)[Q], const CollideParams
 __device__ __forceinline__ float moving_wall_f32(int q, const double* uw, float rho) {
   const float cu = (float)CX(q) * (float)uw[0] + (float)CY(q) * (float)uw[1] + (float)CZ(q) * (float)uw[2];
   return 6.0f * W32(q) * rho * cu;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Boundary links of a classed cell, applied to the post-collision populations of the outgoing
+// slots q in m (the swap streaming later moves each into the incoming slot 18 - q):
+//  moving wall      f_q - 2 w_q rho (u_w . c_q) / cs2           BLB/boundaryconditions/BlockMovingWall.cpp:23-28
+//  pressure in/out  -f_q + feq_{18-q} + feq_q of (rho_t, u_t)    BlockAbstractPressureCondition.cpp:24-37;
+//                   outlet: rho_t = rho_L, u_t = cell velocity    BlockPressureOutletCondition.h:32-50
+//                   inlet:  rho_t = cell density, u_t = u_L       BlockPressureInletCondition.h:32-50
+//  velocity inlet   feq_{18-q}(rho_L, u_L)                        BlockVelocityInletEquilibrium.cpp:19-23
+// computeFeq (BLB/BlockComputeCellValues.cpp:14-20) for one direction, reference order:
+__device__ __forceinline__ double feq_f64(int q, double rho, double u0, double u1, double u2, double uu) {
+  const double cu = cdot64(q, u0, u1, u2);
+  return W64(q) * rho *
+         (1.0 + cu / (1.0 / 3.0) + (cu * cu) / (2.0 * (1.0 / 3.0) * (1.0 / 3.0)) - (uu / (2.0 * (1.0 / 3.0))));
+}
+
+// (the cell velocity comes by value: a pointer select between it and the table would force it
+// into scratch memory)
+__device__ __forceinline__ void apply_boundary_f64(double (&f)[Q], uint32_t m, const BoundaryClass& b,
+                                                   const double* __restrict__ bc_u, int64_t i, double rho,
+                                                   double u0, double u1, double u2) {
+  const uint32_t ty = b.type & BC_TYPE_MASK;
+  if (ty == BC_MOVING_WALL) {
+#pragma unroll
+    for (int q = 0; q < Q; q++)
+      if ((m >> q) & 1u) f[q] -= moving_wall_f64(q, b.u, rho);
+    return;
+  }
+  const double* ub = (b.type & BC_CELL_VELOCITY) ? bc_u + 3 * i : b.u;
+  double rt, t0, t1, t2;
+  if (ty == BC_PRESSURE_OUTLET) {
+    rt = b.rho; t0 = u0; t1 = u1; t2 = u2;
+  } else {
+    rt = ty == BC_PRESSURE_INLET ? rho : b.rho;
+    t0 = ub[0]; t1 = ub[1]; t2 = ub[2];
+  }
+  double uu = t0 * t0;
+  uu += t1 * t1;
+  uu += t2 * t2;
+  const bool vin = ty == BC_VELOCITY_INLET;
+#pragma unroll
+  for (int q = 0; q < Q; q++) {
+    if ((m >> q) & 1u) {
+      const double fin = feq_f64(Q - 1 - q, rt, t0, t1, t2, uu);
+      f[q] = vin ? fin : -f[q] + fin + feq_f64(q, rt, t0, t1, t2, uu);
+    }
+  }
+}
+
+// FP32 shifted form: with geq_q = feq_q - w_q = w_q (drho_t + rho_t (3 cu + 4.5 cu^2 - 1.5 u^2))
+// and w_q = w_{18-q}, the pressure rule is g_q <- -g_q + geq_{18-q} + geq_q and the velocity
+// inlet g_q <- geq_{18-q}.
+__device__ __forceinline__ void apply_boundary_f32(float (&g)[Q], uint32_t m, const BoundaryClass& b,
+                                                   const double* __restrict__ bc_u, int64_t i, float rho,
+                                                   float drho, float u0, float u1, float u2) {
+  const uint32_t ty = b.type & BC_TYPE_MASK;
+  if (ty == BC_MOVING_WALL) {
+#pragma unroll
+    for (int q = 0; q < Q; q++)
+      if ((m >> q) & 1u) g[q] -= moving_wall_f32(q, b.u, rho);
+    return;
+  }
+  const double* ub = (b.type & BC_CELL_VELOCITY) ? bc_u + 3 * i : b.u;
+  float rt, drt, t0, t1, t2;
+  if (ty == BC_PRESSURE_OUTLET) {
+    rt = (float)b.rho; drt = (float)(b.rho - 1.0); t0 = u0; t1 = u1; t2 = u2;
+  } else {
+    if (ty == BC_PRESSURE_INLET) {
+      rt = rho; drt = drho;
+    } else {
+      rt = (float)b.rho; drt = (float)(b.rho - 1.0);
+    }
+    t0 = (float)ub[0]; t1 = (float)ub[1]; t2 = (float)ub[2];
+  }
+  const float uu15 = 1.5f * (t0 * t0 + t1 * t1 + t2 * t2);
+  const bool vin = ty == BC_VELOCITY_INLET;
+#pragma unroll
+  for (int q = 0; q < Q; q++) {
+    if ((m >> q) & 1u) {
+      const float cu = (float)CX(q) * t0 + (float)CY(q) * t1 + (float)CZ(q) * t2;
+      // W_q = W_{18-q}; c_{18-q} . u = -cu
+      const float gin = W32(q) * (drt + rt * (-3.0f * cu + 4.5f * cu * cu - uu15));
+      const float gout = W32(q) * (drt + rt * (3.0f * cu + 4.5f * cu * cu - uu15));
+      g[q] = vin ? gin : -g[q] + gin + gout;
+    }
+  }
 }
 
 }  // namespace lbm

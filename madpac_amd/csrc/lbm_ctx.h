@@ -31,7 +31,9 @@ struct lbm_ctx {
   bool flags_set = false, state_set = false;
   bool walls = false, force = false;
   bool spec = false;                   // step-kernel load schedule (lbm_tune)
-  lbm::WallTable wt{};
+  lbm::BoundaryClass bct[lbm::MAX_CLASSES]{};  // host copy of the boundary-class table
+  lbm::BoundaryClass* d_bct = nullptr;          // device copy (read by the step kernels)
+  double* d_bcu = nullptr;                      // per-cell target velocities [3*N] or null
   lbm::CollideParams p64{};
   lbm::CollideParams32 p32{};
   double* partials = nullptr;

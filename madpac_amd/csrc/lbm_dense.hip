@@ -44,7 +44,7 @@ __global__ __launch_bounds__(BX * BY) void k_build_words(DenseGeom g, const uint
     return;
   }
   // class ids index an 8-entry table; the host rejects larger ids, the mask keeps it in range
-  uint32_t w = FLUID | ((cls ? (uint32_t)cls[c.i] & (MAX_CLASSES - 1) : 0u) << CLASS_SHIFT);
+  uint32_t w = FLUID | ((cls ? (uint32_t)cls[c.i] & CLASS_BITS : 0u) << CLASS_SHIFT);
 #pragma unroll
   for (int q = 0; q < Q; q++) {
     if (q == REST) continue;
@@ -74,7 +74,7 @@ template <typename T, bool FORCE, bool ALPHA, bool WALLS, bool SPEC>
 __global__ __launch_bounds__(BX * BY) void k_dense_step(DenseGeom g, const T* __restrict__ src,
                                                         T* __restrict__ dst, const uint32_t* __restrict__ words,
                                                         const T* __restrict__ alpha, CollideParams p64,
-                                                        CollideParams32 p32, WallTable wt) {
+                                                        CollideParams32 p32, BoundaryArgs bc) {
   const CellPos c = map_cell(g);
   if (!c.valid) return;
   const NbrOff o = nbr_offsets(g, c);
@@ -99,21 +99,20 @@ __global__ __launch_bounds__(BX * BY) void k_dense_step(DenseGeom g, const T* __
   }
   const T a = ALPHA ? alpha[c.i] : T(0);
   T rho;
+  T uc[sizeof(T) == 8 ? 3 : 4];
   if constexpr (sizeof(T) == 8) {
-    rho = collide_f64<FORCE, ALPHA>(f, p64, a);
+    rho = collide_f64<FORCE, ALPHA>(f, p64, a, uc);
   } else {
-    rho = collide_f32s<FORCE, ALPHA>(f, p32, a);
+    rho = collide_f32s<FORCE, ALPHA>(f, p32, a, uc);
   }
   if (WALLS) {
-    const uint32_t k = (w >> CLASS_SHIFT) & (MAX_CLASSES - 1);
+    const uint32_t k = (w >> CLASS_SHIFT) & CLASS_BITS;
     if (k) {
-      const uint32_t m = w & wt.mask[k];
-#pragma unroll
-      for (int q = 0; q < Q; q++) {
-        if ((m >> q) & 1u) {
-          if constexpr (sizeof(T) == 8) f[q] -= moving_wall_f64(q, wt.uw[k], rho);
-          else f[q] -= moving_wall_f32(q, wt.uw[k], rho);
-        }
+      const BoundaryClass& b = bc.table[k];
+      const uint32_t m = w & b.mask;
+      if (m) {
+        if constexpr (sizeof(T) == 8) apply_boundary_f64(f, m, b, bc.cell_u, c.i, rho, uc[0], uc[1], uc[2]);
+        else apply_boundary_f32(f, m, b, bc.cell_u, c.i, rho, uc[3], uc[0], uc[1], uc[2]);
       }
     }
   }
@@ -399,13 +398,13 @@ hipError_t launch_build_words(const DenseGeom& g, const uint8_t* inner, const ui
 template <typename T, bool SPEC>
 static hipError_t step_t(const DenseGeom& g, const void* src, void* dst, const uint32_t* words, const void* alpha,
                          bool force, bool walls, const CollideParams& p64, const CollideParams32& p32,
-                         const WallTable& wt, hipStream_t s) {
+                         const BoundaryArgs& bc, hipStream_t s) {
   const T* a = (const T*)alpha;
   const T* sp = (const T*)src;
   T* dp = (T*)dst;
 #define LBM_STEP(F, A, W)                                                                                        \
   hipLaunchKernelGGL((k_dense_step<T, F, A, W, SPEC>), dense_grid(g), dense_block(), 0, s, g, sp, dp, words, a, \
-                     p64, p32, wt)
+                     p64, p32, bc)
   const int key = (force ? 4 : 0) | (alpha ? 2 : 0) | (walls ? 1 : 0);
   switch (key) {
     case 0: LBM_STEP(false, false, false); break;
@@ -423,12 +422,12 @@ static hipError_t step_t(const DenseGeom& g, const void* src, void* dst, const u
 
 hipError_t launch_dense_step(const DenseGeom& g, bool fp32, const void* src, void* dst, const uint32_t* words,
                              const void* alpha, bool force, bool walls, const CollideParams& p64,
-                             const CollideParams32& p32, const WallTable& wt, bool spec, hipStream_t s) {
+                             const CollideParams32& p32, const BoundaryArgs& bc, bool spec, hipStream_t s) {
   if (fp32)
-    return spec ? step_t<float, true>(g, src, dst, words, alpha, force, walls, p64, p32, wt, s)
-                : step_t<float, false>(g, src, dst, words, alpha, force, walls, p64, p32, wt, s);
-  return spec ? step_t<double, true>(g, src, dst, words, alpha, force, walls, p64, p32, wt, s)
-              : step_t<double, false>(g, src, dst, words, alpha, force, walls, p64, p32, wt, s);
+    return spec ? step_t<float, true>(g, src, dst, words, alpha, force, walls, p64, p32, bc, s)
+                : step_t<float, false>(g, src, dst, words, alpha, force, walls, p64, p32, bc, s);
+  return spec ? step_t<double, true>(g, src, dst, words, alpha, force, walls, p64, p32, bc, s)
+              : step_t<double, false>(g, src, dst, words, alpha, force, walls, p64, p32, bc, s);
 }
 
 hipError_t launch_dense_pull(const DenseGeom& g, bool fp32, const void* src, double* f, const uint32_t* words,

@@ -40,9 +40,15 @@ hipError_t launch_build_words(const DenseGeom& g, const uint8_t* inner, const ui
 // one step src -> dst
 // spec: issue the 19 neighbour loads alongside the flag word and re-load the bounce-back
 // populations afterwards, instead of reading the word first (same results either way)
+// boundary classes of a context: the device table (MAX_CLASSES entries) and the optional
+// per-cell target velocities [3*N] of BC_CELL_VELOCITY classes
+struct BoundaryArgs {
+  const BoundaryClass* table;
+  const double* cell_u;
+};
 hipError_t launch_dense_step(const DenseGeom& g, bool fp32, const void* src, void* dst, const uint32_t* words,
                              const void* alpha, bool force, bool walls, const CollideParams& p64,
-                             const CollideParams32& p32, const WallTable& wt, bool spec, hipStream_t stream);
+                             const CollideParams32& p32, const BoundaryArgs& bc, bool spec, hipStream_t stream);
 // reference state f (FP64) from the stored state
 hipError_t launch_dense_pull(const DenseGeom& g, bool fp32, const void* src, double* f, const uint32_t* words,
                              hipStream_t stream);

@@ -52,10 +52,12 @@ template <typename T>
 __device__ __forceinline__ void collide_patch(T (&f)[Q], double omega) {
   if constexpr (sizeof(T) == 8) {
     CollideParams p{1.0 / omega, omega, {0.0, 0.0, 0.0}};
-    collide_f64<false, false>(f, p, 0.0);
+    double u[3];
+    collide_f64<false, false>(f, p, 0.0, u);
   } else {
     CollideParams32 p{(float)omega, 0.f, {0.f, 0.f, 0.f}};
-    collide_f32s<false, false>(f, p, 0.f);
+    float u[4];
+    collide_f32s<false, false>(f, p, 0.f, u);
   }
 }
 

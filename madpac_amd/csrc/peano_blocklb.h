@@ -34,7 +34,9 @@ constexpr int LB_BLOCKSIZE = 6;                                    // LB/UserInp
 constexpr int LB_BLOCK_NUMBER_OF_CELLS = 216;                      // BLB/BlockDefinitions.h:37
 constexpr int LB_CURRENT_DIR = 19;
 
-enum class Scenario { LidDrivenCavity, PeriodicBox };
+// lid-driven-cavity and default-channel of SLB/configurations/BlockLatticeBoltzmannScenarioConfiguration.cpp:52-56;
+// PeriodicBox (all axes periodic, body force) has no reference counterpart
+enum class Scenario { LidDrivenCavity, DefaultChannel, PeriodicBox };
 enum class Precision { FP64 = LBM_FP64, FP32Shifted = LBM_FP32_SHIFTED };
 
 // Run configuration: the XML tags regular-grid, lattice-boltzmann-configuration and
@@ -50,7 +52,8 @@ struct Configuration {
   int numberOfTimesteps = 1000;
   bool measurePerformance = false;
   Scenario scenario = Scenario::LidDrivenCavity;
-  double characteristicVelocityL = 0.05;     // lid velocity of the cavity
+  double characteristicVelocityL = 0.05;     // lid velocity (cavity) or mean inlet velocity (channel)
+  bool useParabolicProfile = false;          // channel inlet profile ("use-parabolic-profile")
   double bodyForce[3] = {0.0, 0.0, 0.0};     // Guo forcing (periodic box)
   Precision precision = Precision::FP64;
   int device = 0;
@@ -119,11 +122,15 @@ class Repository {
     c.device = cfg_.device;
     check(lbm_create(&ctx_, &c), nullptr, "lbm_create");
     n_ = (int64_t)c.nx * c.ny * c.nz;
-    // every cell of the box is fluid: the cavity's walls and lid lie outside the box
+    // every cell of the box is fluid: the walls, lid, inlet and outlet lie outside the box
     // (GridManagementService::createVertex flags, BLB/services/GridManagementService.cpp:91-111)
     std::vector<uint8_t> inner((size_t)n_, 1);
     check(lbm_set_flags(ctx_, inner.data()), ctx_, "lbm_set_flags");
-    if (!box) check(lbm_set_cavity_lid(ctx_, cfg_.characteristicVelocityL), ctx_, "lbm_set_cavity_lid");
+    if (cfg_.scenario == Scenario::LidDrivenCavity)
+      check(lbm_set_cavity_lid(ctx_, cfg_.characteristicVelocityL), ctx_, "lbm_set_cavity_lid");
+    else if (cfg_.scenario == Scenario::DefaultChannel)
+      check(lbm_set_channel(ctx_, cfg_.characteristicVelocityL, cfg_.useParabolicProfile ? 1 : 0), ctx_,
+            "lbm_set_channel");
     // f = w, rho = 1, u = 0 (GridManagementService.cpp:121-148)
     check(lbm_init_equilibrium(ctx_, nullptr, nullptr), ctx_, "lbm_init_equilibrium");
     state_.tau_ = lbm_tau_eff(cfg_.tau);

@@ -17,6 +17,9 @@ LIB_PATH = os.path.join(_HERE, "liblbm_hip.so")
 Q = 19
 FP64 = 0
 FP32_SHIFTED = 1
+# boundary-condition types of a class (include/lbm_capi.h)
+BC_MOVING_WALL, BC_PRESSURE_OUTLET, BC_PRESSURE_INLET, BC_VELOCITY_INLET = 0, 1, 2, 3
+BC_CELL_VELOCITY = 16
 
 # D3Q19 ordering (madpac/peano/applications/latticeboltzmann/LatticeBoltzmannDefinitions.cpp:113-131)
 C = np.array([
@@ -28,7 +31,8 @@ C = np.array([
 # every symbol include/lbm_capi.h declares
 EXPORTS = (
     "lbm_create", "lbm_destroy", "lbm_last_error", "lbm_fluid_cells", "lbm_set_flags",
-    "lbm_set_moving_wall_class", "lbm_set_wall_classes", "lbm_set_cavity_lid", "lbm_set_porosity",
+    "lbm_set_boundary_class", "lbm_set_moving_wall_class", "lbm_set_wall_classes", "lbm_set_boundary_velocity",
+    "lbm_set_cavity_lid", "lbm_set_channel", "lbm_set_porosity",
     "lbm_init_equilibrium", "lbm_set_populations", "lbm_get_populations", "lbm_step", "lbm_step_timed",
     "lbm_tune", "lbm_synchronize", "lbm_get_fields", "lbm_reduce_moments", "lbm_kinetic_energy",
     "lbm_pressure_drop",
@@ -85,9 +89,12 @@ def load_library(path: str = LIB_PATH):
     L.lbm_fluid_cells.argtypes = [vp]
     L.lbm_fluid_cells.restype = ctypes.c_int64
     L.lbm_set_flags.argtypes = [vp, u8p]
+    L.lbm_set_boundary_class.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, dp, ctypes.c_uint32]
     L.lbm_set_moving_wall_class.argtypes = [vp, ctypes.c_int, dp, ctypes.c_uint32]
     L.lbm_set_wall_classes.argtypes = [vp, u8p]
+    L.lbm_set_boundary_velocity.argtypes = [vp, dp]
     L.lbm_set_cavity_lid.argtypes = [vp, ctypes.c_double]
+    L.lbm_set_channel.argtypes = [vp, ctypes.c_double, ctypes.c_int]
     L.lbm_set_porosity.argtypes = [vp, dp]
     L.lbm_init_equilibrium.argtypes = [vp, dp, dp]
     L.lbm_set_populations.argtypes = [vp, dp]
@@ -176,6 +183,22 @@ class Lattice:
 
     def set_cavity_lid(self, ulid):
         self._check(self._L.lbm_set_cavity_lid(self._h, float(ulid)), "lbm_set_cavity_lid")
+
+    def set_channel(self, u_mean, parabolic=False):
+        self._check(self._L.lbm_set_channel(self._h, float(u_mean), int(bool(parabolic))), "lbm_set_channel")
+
+    def set_boundary_class(self, k, bc_type, rho=1.0, u=(0.0, 0.0, 0.0), mask=0):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        self._check(self._L.lbm_set_boundary_class(self._h, int(k), int(bc_type), float(rho), _dp(u),
+                                                   ctypes.c_uint32(mask)), "lbm_set_boundary_class")
+
+    def set_boundary_velocity(self, u):
+        if u is None:
+            self._check(self._L.lbm_set_boundary_velocity(self._h, None), "lbm_set_boundary_velocity")
+            return
+        a = np.ascontiguousarray(np.asarray(u, dtype=np.float64).reshape(-1))
+        assert a.size == 3 * self.n
+        self._check(self._L.lbm_set_boundary_velocity(self._h, _dp(a)), "lbm_set_boundary_velocity")
 
     def set_moving_wall_class(self, k, uw, mask):
         uw = np.ascontiguousarray(uw, dtype=np.float64)
@@ -346,6 +369,16 @@ class PatchSet:
         out = np.empty(4)
         self._check(self._L.lbm_patches_reduce_moments(self._h, _dp(out)), "lbm_patches_reduce_moments")
         return out
+
+
+def channel(nx, ny, nz, tau, u_mean, parabolic=False, solid=None, precision=FP64, device=0):
+    """The reference's default channel: no-slip y/z walls, velocity inlet on x- (uniform or
+    parabolic), pressure outlet rho = 1 on x+, optional solid obstacles; f = w."""
+    lat = Lattice(nx, ny, nz, tau, precision=precision, device=device)
+    lat.set_flags(np.ones(lat.n, np.uint8) if solid is None else ~np.asarray(solid, bool).reshape(-1))
+    lat.set_channel(u_mean, parabolic)
+    lat.init_equilibrium()
+    return lat
 
 
 def cavity(n, tau, ulid, precision=FP64, device=0):

@@ -23,7 +23,10 @@ Q = 19
 REST = 9
 FLUID = 0x80000000
 CLASS_SHIFT = 24
-MAX_CLASSES = 8
+MAX_CLASSES = 32
+# boundary-condition types of a class (lbm_oracle.h)
+BC_MOVING_WALL, BC_PRESSURE_OUTLET, BC_PRESSURE_INLET, BC_VELOCITY_INLET = 0, 1, 2, 3
+BC_CELL_VELOCITY = 16
 CS2 = 1.0 / 3.0
 
 # LB/LatticeBoltzmannDefinitions.cpp:113-131 (z-major, then y, then x)
@@ -49,7 +52,10 @@ class Params(ctypes.Structure):
                 ("g", ctypes.c_double * 3),
                 ("use_force", ctypes.c_int),
                 ("alpha", ctypes.POINTER(ctypes.c_double)),
-                ("tau_cell", ctypes.POINTER(ctypes.c_double))]
+                ("tau_cell", ctypes.POINTER(ctypes.c_double)),
+                ("bc_type", ctypes.c_int * MAX_CLASSES),
+                ("bc_rho", ctypes.c_double * MAX_CLASSES),
+                ("bc_u", ctypes.POINTER(ctypes.c_double))]
 
 
 _lib = None
@@ -79,6 +85,11 @@ def lib():
         L.lbo_moving_wall_term.restype = ctypes.c_double
         L.lbo_moving_wall_term.argtypes = [dp, ctypes.c_int, ctypes.c_double]
         L.lbo_build_cavity.argtypes = [ctypes.c_int, up, ctypes.POINTER(Params), ctypes.c_double]
+        L.lbo_build_channel.argtypes = [ctypes.POINTER(Dims), up, ctypes.POINTER(Params), ctypes.c_double,
+                                        ctypes.c_int, dp]
+        L.lbo_boundary_value.restype = ctypes.c_double
+        L.lbo_boundary_value.argtypes = [ctypes.POINTER(Params), ctypes.c_uint, ctypes.c_int64, ctypes.c_int,
+                                         ctypes.c_double, ctypes.c_double, dp]
         L.lbo_build_words.argtypes = [ctypes.POINTER(Dims), ctypes.POINTER(ctypes.c_uint8), up]
         L.lbo_step.argtypes = [ctypes.POINTER(Dims), up, ctypes.POINTER(Params), dp, dp, ctypes.c_int]
         L.lbo_stream.argtypes = [ctypes.POINTER(Dims), up, dp, dp]
@@ -122,6 +133,20 @@ class Lattice:
     def cavity(cls, n, tau, ulid):
         lat = cls(n, n, n)
         lib().lbo_build_cavity(n, _up(lat.words), ctypes.byref(lat.params), ulid)
+        lat.params.tau = tau
+        return lat
+
+    @classmethod
+    def channel(cls, nx, ny, nz, tau, umean, parabolic=False, solid=None):
+        """The reference's default channel: no-slip y/z walls, velocity inlet at x-, pressure
+        outlet rho = 1 at x+, optional solid obstacles."""
+        lat = cls(nx, ny, nz)
+        lat.set_solid(np.zeros(lat.n, np.uint8) if solid is None else solid)
+        lat._bc_u = np.zeros(3 * lat.n)
+        lib().lbo_build_channel(ctypes.byref(lat.dims), _up(lat.words), ctypes.byref(lat.params), float(umean),
+                                int(bool(parabolic)), _dp(lat._bc_u))
+        if lat.params.bc_type[0] != 0:
+            raise ValueError("not a channel (nx < 2 or more than 31 boundary classes)")
         lat.params.tau = tau
         return lat
 

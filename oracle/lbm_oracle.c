@@ -90,6 +90,33 @@ double lbo_moving_wall_term(const double* uw, int q, double rho) {
   return civel;
 }
 
+/* Value stored into the outgoing slot q of a boundary link (cell i, class k), after the
+ * collision; the swap streaming later moves it into the incoming slot 18-q.
+ *  moving wall      BLB/boundaryconditions/BlockMovingWall.cpp:23-28: f_q - ciVel;
+ *  pressure in/out  BLB/boundaryconditions/BlockAbstractPressureCondition.cpp:24-37:
+ *                   -f_q + feq_{18-q} + feq_q with feq of (rho_t, u_t); the outlet takes
+ *                   rho_t = rho_L and u_t = the cell's velocity (BlockPressureOutletCondition.h:32-50),
+ *                   the inlet u_t = u_L and rho_t = the cell's density (BlockPressureInletCondition.h:32-50);
+ *  velocity inlet   BLB/boundaryconditions/BlockVelocityInletEquilibrium.cpp:19-23: feq_{18-q}(rho_L, u_L).
+ * The cell's velocity is the one the collision used before any porosity scaling (with a
+ * body force, u + g/2). */
+double lbo_boundary_value(const lbo_params* p, unsigned k, int64_t i, int q, double fq, double rho,
+                          const double* u) {
+  const int t = p->bc_type[k];
+  const int ty = t & LBO_BC_TYPE_MASK;
+  if (ty == LBO_BC_MOVING_WALL) return fq - lbo_moving_wall_term(p->uw[k], q, rho);
+  const double* ub = ((t & LBO_BC_CELL_VELOCITY) && p->bc_u) ? p->bc_u + 3 * i : p->uw[k];
+  const int qin = LBO_Q - 1 - q;
+  double feq[LBO_Q];
+  if (ty == LBO_BC_VELOCITY_INLET) {
+    lbo_feq(p->bc_rho[k], ub, feq);
+    return feq[qin];
+  }
+  if (ty == LBO_BC_PRESSURE_OUTLET) lbo_feq(p->bc_rho[k], u, feq);
+  else lbo_feq(rho, ub, feq);
+  return -fq + feq[qin] + feq[q];
+}
+
 /* Collision of one fluid cell, in place on f[19].  Without force and porosity this is
  * the reference sequence computeDensity -> computeVelocity -> BGK (mapping
  * BLB/mappings/RegularGrid2RegularBlockSolver.cpp:126-137).  With a body acceleration g
@@ -98,13 +125,14 @@ double lbo_moving_wall_term(const double* uw, int q, double rho) {
  * (madpac/peanocoupling/PeanoDataExchangeService.cpp:161-186) squares c.u in the second
  * term, which is not Guo's scheme, so the published scheme is used here.  With a
  * porosity alpha the equilibrium velocity is scaled by (1-alpha) (Brinkman penalisation). */
-static double collide_cell(double* f, double tau_eff, const lbo_params* p, double alpha) {
+static double collide_cell(double* f, double tau_eff, const lbo_params* p, double alpha, double* uout) {
   double rho = lbo_density(f);
   double u[3];
   lbo_velocity(f, rho, u);
   if (p->use_force) {
     for (int d = 0; d < 3; d++) u[d] = u[d] + 0.5 * p->g[d];
   }
+  for (int d = 0; d < 3; d++) uout[d] = u[d];
   if (p->alpha) {
     for (int d = 0; d < 3; d++) u[d] = (1.0 - alpha) * u[d];
   }
@@ -178,6 +206,69 @@ void lbo_build_cavity(int n, uint32_t* words, lbo_params* p, double ulid) {
   p->moving_mask[1] = lidmask;
 }
 
+/* The reference's default channel (SLB/AbstractBoundaryData4Channel.cpp:17-50 with
+ * SLB/BoundaryData4ChannelWithNoSlipBoundaries.cpp:17-27), link by link: a link whose target
+ * leaves the box through a y or z face is a no-slip wall (halfway bounce-back); otherwise one
+ * leaving through x- is a velocity inlet (rho_L = 1, the BoundaryData default,
+ * LB/BoundaryData.cpp:5; u_L = (umean, 0, 0), or the parabolic profile of :72-77 evaluated
+ * at the cell centre in lattice units), one leaving through x+ a pressure outlet with
+ * rho_L = 1 (:85-92), and any other (an obstacle) a wall.  The links of one cell with the
+ * same treatment form one class; classes are numbered in first-appearance order. */
+void lbo_build_channel(const lbo_dims* d, uint32_t* words, lbo_params* p, double umean, int parabolic,
+                       double* bc_u) {
+  memset(p->uw, 0, sizeof(p->uw));
+  memset(p->moving_mask, 0, sizeof(p->moving_mask));
+  memset(p->bc_type, 0, sizeof(p->bc_type));
+  for (int k = 0; k < LBO_MAX_CLASSES; k++) p->bc_rho[k] = 1.0;
+  p->bc_u = parabolic ? bc_u : NULL;
+  int ncls = 1;
+  for (int z = 0; z < d->nz; z++)
+    for (int y = 0; y < d->ny; y++)
+      for (int x = 0; x < d->nx; x++) {
+        int64_t i = (int64_t)x + (int64_t)d->nx * ((int64_t)y + (int64_t)d->ny * z);
+        uint32_t w = words[i];
+        words[i] = w & ~(LBO_CLASS_MASK << LBO_CLASS_SHIFT);
+        if (!(w & LBO_FLUID)) continue;
+        uint32_t inm = 0, outm = 0;
+        for (int q = 0; q < LBO_Q; q++) {
+          if (!(w & (1u << q))) continue;
+          int nxx = x + LBO_C[q][0], nyy = y + LBO_C[q][1], nzz = z + LBO_C[q][2];
+          if (nyy < 0 || nyy >= d->ny || nzz < 0 || nzz >= d->nz) continue;
+          if (nxx < 0) inm |= 1u << q;
+          else if (nxx >= d->nx) outm |= 1u << q;
+        }
+        if (bc_u) {
+          double v = 0.0;
+          if (inm) {
+            v = umean;
+            if (parabolic) {
+              const double L[2] = {(double)d->ny, (double)d->nz};
+              const double a[2] = {y + 0.5, z + 0.5};
+              for (int e = 0; e < 2; e++) v = v * 6.0 / (L[e] * L[e]) * a[e] * (L[e] - a[e]);
+            }
+          }
+          bc_u[3 * i] = v;
+          bc_u[3 * i + 1] = 0.0;
+          bc_u[3 * i + 2] = 0.0;
+        }
+        if (!inm && !outm) continue;
+        if (inm && outm) { p->bc_type[0] = -1; return; } /* nx == 1: not a channel */
+        const uint32_t m = inm ? inm : outm;
+        const int t = inm ? (LBO_BC_VELOCITY_INLET | (parabolic ? LBO_BC_CELL_VELOCITY : 0))
+                          : LBO_BC_PRESSURE_OUTLET;
+        int k = 1;
+        while (k < ncls && !(p->moving_mask[k] == m && p->bc_type[k] == t)) k++;
+        if (k == ncls) {
+          if (ncls == LBO_MAX_CLASSES) { p->bc_type[0] = -1; return; }
+          ncls++;
+          p->moving_mask[k] = m;
+          p->bc_type[k] = t;
+          if (inm) p->uw[k][0] = umean;
+        }
+        words[i] |= (uint32_t)k << LBO_CLASS_SHIFT;
+      }
+}
+
 /* One step in push form.  Equivalent to the reference's swap streaming
  * (BLB/nonlocaltreatment/AABlockStreaming.cpp:19-92): f*_q(x) goes to x+c_q when that cell
  * is fluid, otherwise it is reflected into q' = 18-q of x (halfway bounce-back is implicit,
@@ -201,12 +292,13 @@ void lbo_step(const lbo_dims* d, const uint32_t* words, const lbo_params* p,
         double f[LBO_Q];
         for (int q = 0; q < LBO_Q; q++) f[q] = src[(int64_t)q * N + i];
         double alpha = p->alpha ? p->alpha[i] : 0.0;
-        double rho = collide_cell(f, p->tau_cell ? lbo_tau_eff(p->tau_cell[i]) : tau_eff, p, alpha);
+        double uc[3];
+        double rho = collide_cell(f, p->tau_cell ? lbo_tau_eff(p->tau_cell[i]) : tau_eff, p, alpha, uc);
         unsigned cls = (w >> LBO_CLASS_SHIFT) & LBO_CLASS_MASK;
         for (int q = 0; q < LBO_Q; q++) {
           if (w & (1u << q)) {
             double v = f[q];
-            if (cls && ((p->moving_mask[cls] >> q) & 1u)) v -= lbo_moving_wall_term(p->uw[cls], q, rho);
+            if (cls && ((p->moving_mask[cls] >> q) & 1u)) v = lbo_boundary_value(p, cls, i, q, v, rho, uc);
             dst[(int64_t)(LBO_Q - 1 - q) * N + i] = v;
           } else {
             int64_t j = neighbour(d, x, y, z, q);
@@ -245,12 +337,13 @@ void lbo_step_swap_aos(const lbo_dims* d, const uint32_t* words, const lbo_param
     if (!(w & LBO_FLUID)) continue;
     double* fi = f + i * LBO_Q;
     double alpha = p->alpha ? p->alpha[i] : 0.0;
-    double rho = collide_cell(fi, p->tau_cell ? lbo_tau_eff(p->tau_cell[i]) : tau_eff, p, alpha);
+    double uc[3];
+    double rho = collide_cell(fi, p->tau_cell ? lbo_tau_eff(p->tau_cell[i]) : tau_eff, p, alpha, uc);
     unsigned cls = (w >> LBO_CLASS_SHIFT) & LBO_CLASS_MASK;
     if (cls)
       for (int q = 0; q < LBO_Q; q++)
         if ((w & (1u << q)) && ((p->moving_mask[cls] >> q) & 1u))
-          fi[q] -= lbo_moving_wall_term(p->uw[cls], q, rho);
+          fi[q] = lbo_boundary_value(p, cls, i, q, fi[q], rho, uc);
   }
   for (int z = 0; z < d->nz; z++)
     for (int y = 0; y < d->ny; y++)

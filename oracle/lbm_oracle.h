@@ -34,11 +34,23 @@ extern "C" {
 #define LBO_CS2 (1.0 / 3.0)
 
 /* Cell word: bit 31 fluid, bit q (0..18) = neighbour x+c_q is not fluid (bounce-back),
- * bits 24..30 = moving-wall class (0 = none). */
+ * bits 24..28 = boundary class (0 = none, classes 1..31). */
 #define LBO_FLUID 0x80000000u
 #define LBO_CLASS_SHIFT 24
-#define LBO_CLASS_MASK 0x7Fu
-#define LBO_MAX_CLASSES 8
+#define LBO_CLASS_MASK 0x1Fu
+#define LBO_MAX_CLASSES 32
+
+/* Boundary condition of a class's links (LB/BoundaryData.h:44-61 and the handlers
+ * registered in BLB/boundaryconditions/BlockBoundaryConditionManager.cpp:15-20).  Links
+ * outside a class's mask, and cells of class 0, are plain halfway bounce-back; a moving
+ * wall with u_w = 0 is the same.  LBO_BC_CELL_VELOCITY or-ed into a type takes the target
+ * velocity per cell from bc_u. */
+#define LBO_BC_MOVING_WALL 0
+#define LBO_BC_PRESSURE_OUTLET 1
+#define LBO_BC_PRESSURE_INLET 2
+#define LBO_BC_VELOCITY_INLET 3
+#define LBO_BC_TYPE_MASK 15
+#define LBO_BC_CELL_VELOCITY 16
 
 extern const int LBO_C[LBO_Q][3];
 extern const double LBO_W[LBO_Q];
@@ -50,12 +62,15 @@ typedef struct {
 
 typedef struct {
   double tau;                              /* configured tau (LB/configurations/...:269) */
-  double uw[LBO_MAX_CLASSES][3];           /* wall velocity per moving-wall class */
-  uint32_t moving_mask[LBO_MAX_CLASSES];   /* bit q: outgoing link q of a class-k cell is a moving wall */
+  double uw[LBO_MAX_CLASSES][3];           /* per class: wall velocity / target velocity u_L */
+  uint32_t moving_mask[LBO_MAX_CLASSES];   /* bit q: outgoing link q of a class-k cell takes the class's condition */
   double g[3];                             /* body acceleration (Guo forcing); 0 = off */
   int use_force;
   const double* alpha;                     /* per-cell Brinkman porosity or NULL */
   const double* tau_cell;                  /* per-cell configured tau (overrides tau) or NULL */
+  int bc_type[LBO_MAX_CLASSES];            /* LBO_BC_* per class */
+  double bc_rho[LBO_MAX_CLASSES];          /* per class: target density rho_L */
+  const double* bc_u;                      /* per-cell target velocity [3*N] for LBO_BC_CELL_VELOCITY classes */
 } lbo_params;
 
 /* a1/a8 */
@@ -68,8 +83,19 @@ void lbo_bgk(double* f, double tau_eff, const double* feq);
 /* a13: the amount subtracted from outgoing population q of a moving-wall link */
 double lbo_moving_wall_term(const double* uw, int q, double rho);
 
+/* a13 generalised: the value stored into the outgoing slot q of a boundary link of class k
+ * at cell i, given the post-collision population fq and the cell's rho and velocity u. */
+double lbo_boundary_value(const lbo_params* p, unsigned k, int64_t i, int q, double fq, double rho,
+                          const double* u);
+
 /* geometry: lid-driven cavity of n^3 fluid cells, lid on +z moving with (ulid,0,0). */
 void lbo_build_cavity(int n, uint32_t* words, lbo_params* p, double ulid);
+/* geometry: the reference's default channel on the whole box (walls on the y/z faces,
+ * velocity inlet on x-, pressure outlet rho = 1 on x+), solid cells as obstacles.
+ * words must come from lbo_build_words on a non-periodic box; bc_u receives the per-cell
+ * inlet velocity (3*N doubles) when parabolic != 0 and is then referenced by p. */
+void lbo_build_channel(const lbo_dims* d, uint32_t* words, lbo_params* p, double umean, int parabolic,
+                       double* bc_u);
 /* generic words from a solid mask (1 = solid); moving-wall classes left at 0. */
 void lbo_build_words(const lbo_dims* d, const uint8_t* solid, uint32_t* words);
 

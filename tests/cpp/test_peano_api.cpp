@@ -1,9 +1,11 @@
 // Drives the reference-shaped C++ API (madpac_amd/csrc/peano_blocklb.h) like reference code drives
 // the Peano block-LB: configure, run the time loop, read blocks through the block-store view.
-// Usage: test_peano_api n tau ulid steps out.bin   (writes the SoA populations, FP64)
-// Exit code 0 = the block view agrees with the dense fields and the flag layout of the reference.
+// Usage: test_peano_api cavity|channel|channel-parabolic nx ny nz tau u steps out.bin
+// (writes the SoA populations, FP64).  Exit code 0 = the block view agrees with the dense fields
+// and the flag layout of the reference.
 #include <cstdio>
 #include <cstdlib>
+#include <string>
 #include <vector>
 
 #include "peano_blocklb.h"
@@ -11,44 +13,58 @@
 using namespace madpac_amd::blocklatticeboltzmann;
 
 int main(int argc, char** argv) {
-  if (argc != 6) {
-    std::fprintf(stderr, "usage: %s n tau ulid steps out.bin\n", argv[0]);
+  if (argc != 9) {
+    std::fprintf(stderr, "usage: %s cavity|channel|channel-parabolic nx ny nz tau u steps out.bin\n", argv[0]);
     return 2;
   }
   Configuration cfg;
-  const int n = std::atoi(argv[1]);
-  for (int d = 0; d < 3; d++) cfg.numberOfGridPoints[d] = n;
-  cfg.tau = std::atof(argv[2]);
-  cfg.characteristicVelocityL = std::atof(argv[3]);
-  cfg.numberOfTimesteps = std::atoi(argv[4]);
-  cfg.scenario = Scenario::LidDrivenCavity;
+  const std::string scenario = argv[1];
+  int n3[3];
+  for (int d = 0; d < 3; d++) cfg.numberOfGridPoints[d] = n3[d] = std::atoi(argv[2 + d]);
+  for (int d = 0; d < 3; d++) cfg.domainSize[d] = (double)n3[d] / n3[1];
+  cfg.tau = std::atof(argv[5]);
+  cfg.characteristicVelocityL = std::atof(argv[6]);
+  cfg.numberOfTimesteps = std::atoi(argv[7]);
+  const char* outpath = argv[8];
+  if (scenario == "cavity") {
+    cfg.scenario = Scenario::LidDrivenCavity;
+  } else if (scenario == "channel" || scenario == "channel-parabolic") {
+    cfg.scenario = Scenario::DefaultChannel;
+    cfg.useParabolicProfile = scenario == "channel-parabolic";
+  } else {
+    return 2;
+  }
+  const int64_t ncell = (int64_t)n3[0] * n3[1] * n3[2];
   try {
     Repository repo(cfg);
     Runner runner;
     runner.runAsMaster(repo);
     RegularGridBlockState& st = repo.getRegularGridState();
     if (st.getTimestep() != cfg.numberOfTimesteps) return 3;
-    if (st.getLocalNumberOfCells() != (int64_t)n * n * n) return 4;
+    if (st.getLocalNumberOfCells() != ncell) return 4;
     std::vector<double> f, rho, u;
     repo.getPopulations(f);
     repo.getFields(rho, u);
-    FILE* fp = std::fopen(argv[5], "wb");
+    FILE* fp = std::fopen(outpath, "wb");
     if (!fp) return 5;
     std::fwrite(f.data(), sizeof(double), f.size(), fp);
     std::fclose(fp);
     // block view: every block, every cell, against the dense arrays and the reference's layout
-    const int b = repo.blocksPerDimension(0);
-    const int expect_b = n / 6 + 1 + (n % 6 ? 1 : 0);
-    if (b != expect_b || repo.numberOfVertices() != (unsigned)(b * b * b)) return 6;
+    int b[3];
+    for (int d = 0; d < 3; d++) {
+      b[d] = repo.blocksPerDimension(d);
+      if (b[d] != n3[d] / 6 + 1 + (n3[d] % 6 ? 1 : 0)) return 6;
+    }
+    if (repo.numberOfVertices() != (unsigned)(b[0] * b[1] * b[2])) return 6;
     int64_t inner_total = 0;
     for (unsigned v = 8; v < 8 + repo.numberOfVertices(); v++) {
       BlockView bv = repo.loadVertex(v);
       const unsigned k = v - 8;
-      const int bx = k % b, by = (k / b) % b, bz = k / (b * b);
+      const int bx = k % b[0], by = (k / b[0]) % b[1], bz = k / (b[0] * b[1]);
       for (int cell = 0; cell < LB_BLOCK_NUMBER_OF_CELLS; cell++) {
         const int x = cell % 6, y = (cell / 6) % 6, z = cell / 36;
         const int gx = 6 * bx + x - 3, gy = 6 * by + y - 3, gz = 6 * bz + z - 3;
-        const bool in = gx >= 0 && gy >= 0 && gz >= 0 && gx < n && gy < n && gz < n;
+        const bool in = gx >= 0 && gy >= 0 && gz >= 0 && gx < n3[0] && gy < n3[1] && gz < n3[2];
         if ((bv.inner[cell] != 0) != in) return 7;
         if (!in) {
           for (int q = 0; q < LB_CURRENT_DIR; q++)
@@ -56,13 +72,13 @@ int main(int argc, char** argv) {
           continue;
         }
         inner_total++;
-        const int64_t i = gx + (int64_t)n * (gy + (int64_t)n * gz);
+        const int64_t i = gx + (int64_t)n3[0] * (gy + (int64_t)n3[1] * gz);
         for (int q = 0; q < LB_CURRENT_DIR; q++)
-          if (bv.pdf[cell * LB_CURRENT_DIR + q] != f[(size_t)q * n * n * n + i]) return 9;
+          if (bv.pdf[cell * LB_CURRENT_DIR + q] != f[(size_t)q * ncell + i]) return 9;
         if (bv.density[cell] != rho[i]) return 10;
       }
     }
-    if (inner_total != (int64_t)n * n * n) return 11;
+    if (inner_total != ncell) return 11;
     bool threw = false;
     try {
       repo.loadVertex(3);

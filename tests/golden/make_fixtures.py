@@ -13,6 +13,9 @@ its outputs as small ``.npz`` fixtures next to this file:
 * ``cavity_n64_t1000.npz`` -- the 64^3 cavity (tau 0.6, lid 0.05) after 1000 steps:
   SHA-256 of the full SoA population field, the rho/u mid-planes and centre lines, and
   global mass and momentum.
+* ``channel_*.npz`` -- the reference's default channel (no-slip y/z walls, velocity inlet on
+  x-, pressure outlet on x+; uniform or parabolic inlet profile), full FP64 populations over
+  the fluid box after ``steps`` time steps.
 
 Usage:  python tests/golden/make_fixtures.py [--skip-64]
 """
@@ -36,16 +39,18 @@ CELLK = os.path.join(REF_DIR, "peano_lb_cellkernels")
 # Cavity XML for the reference's regular-grid block-LB batch job.  Tags and attributes are the
 # ones its configuration parsers read (blocklatticeboltzmann/configurations/
 # BlockLatticeBoltzmannBatchJobConfigurationForRegularGrid.cpp:41-91).
+# Scenario attributes: lid-driven-cavity takes none beyond the velocity; default-channel also
+# needs use-parabolic-profile (SLB/configurations/BlockLatticeBoltzmannScenarioConfiguration.cpp:52-109).
 XML = """<?xml version="1.0"?>
 <peano-configuration>
   <run-blocklatticeboltzmann-on-regular-grid>
     <log-output column-separator=" " log-time-stamp="no" log-time-stamp-human-readable="no" log-machine-name="no" log-message-type="yes" log-trace="yes" />
-    <regular-grid number-of-vertices="{n};{n};{n}" domain-size="1.0;1.0;1.0" domain-offset="0.0;0.0;0.0" />
+    <regular-grid number-of-vertices="{nx};{ny};{nz}" domain-size="{lx!r};{ly!r};{lz!r}" domain-offset="0.0;0.0;0.0" />
     <lattice-boltzmann-configuration tau="{tau!r}" dt="1.0" density="1.0" number-of-timesteps="{steps}" measure-performance="no" />
-    <lattice-boltzmann-scenario scenario="lid-driven-cavity" characteristic-velocity_L="{ulid!r}" dynamic-geometry="no" compute-forces="no"></lattice-boltzmann-scenario>
+    <lattice-boltzmann-scenario scenario="{scenario}" characteristic-velocity_L="{ulid!r}"{extra} dynamic-geometry="no" compute-forces="no"></lattice-boltzmann-scenario>
     <collision-model name="BGK" hMin="0.00001" hMax="10.0" />
     <builtin-geometry-configuration>
-      <hexahedron geometry-id="0" invert="false" width="1.0;1.0;1.0" offset="0.0;0.0;0.0" />
+      <hexahedron geometry-id="0" invert="false" width="{lx!r};{ly!r};{lz!r}" offset="0.0;0.0;0.0" />
     </builtin-geometry-configuration>
   </run-blocklatticeboltzmann-on-regular-grid>
 </peano-configuration>
@@ -75,32 +80,39 @@ def read_dump(path):
     return out
 
 
-def run_cavity(n, tau, ulid, steps):
-    """Run the reference and return the SoA populations over the n^3 fluid box."""
-    nb = blocks_per_dim(n)
+def run_reference(dims, tau, ulid, steps, scenario="lid-driven-cavity", parabolic=None):
+    """Run the reference on an nx x ny x nz box of unit cells (domain size n_d / ny) and return
+    the SoA populations over the fluid box, [q, z, y, x]."""
+    nx, ny, nz = dims
+    nb = [blocks_per_dim(n) for n in dims]
+    extra = "" if parabolic is None else f' use-parabolic-profile="{"yes" if parabolic else "no"}"'
     with tempfile.TemporaryDirectory() as td:
-        xml = os.path.join(td, "cavity.xml")
+        xml = os.path.join(td, "run.xml")
         with open(xml, "w") as fh:
-            fh.write(XML.format(n=n, tau=tau, ulid=ulid, steps=steps))
+            fh.write(XML.format(nx=nx, ny=ny, nz=nz, lx=nx / ny, ly=1.0, lz=nz / ny, tau=tau, ulid=ulid,
+                                steps=steps, scenario=scenario, extra=extra))
         out = os.path.join(td, "dump.bin")
-        subprocess.run([HARNESS, xml, str(nb ** 3), out], check=True, cwd=td,
+        subprocess.run([HARNESS, xml, str(nb[0] * nb[1] * nb[2]), out], check=True, cwd=td,
                        stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
         blocks = read_dump(out)
-    s = 6 * nb
-    dense = np.zeros((s, s, s, 19))
-    inner = np.zeros((s, s, s), np.uint8)
+    dense = np.zeros((6 * nb[2], 6 * nb[1], 6 * nb[0], 19))
+    inner = np.zeros(dense.shape[:3], np.uint8)
     # blocks are created in lexicographic vertex order, x fastest
     for k, (inn, f) in enumerate(blocks):
-        bx, by, bz = k % nb, (k // nb) % nb, k // (nb * nb)
+        bx, by, bz = k % nb[0], (k // nb[0]) % nb[1], k // (nb[0] * nb[1])
         sl = (slice(6 * bz, 6 * bz + 6), slice(6 * by, 6 * by + 6), slice(6 * bx, 6 * bx + 6))
         dense[sl] = f.reshape(6, 6, 6, 19)
         inner[sl] = inn.reshape(6, 6, 6)
-    # fluid cell centres (s - 2.5) dx lie in [0, 1] for slots s = 3 .. n+2
-    box = (slice(3, 3 + n),) * 3
-    assert inner[box].all() and int(inner.sum()) == n ** 3, "unexpected cavity flag layout"
+    # fluid cell centres (s - 2.5) dx lie in the domain for slots s = 3 .. n+2
+    box = (slice(3, 3 + nz), slice(3, 3 + ny), slice(3, 3 + nx))
+    assert inner[box].all() and int(inner.sum()) == nx * ny * nz, "unexpected flag layout"
     outside = np.where(inner[..., None] == 0, dense, 0.0)
     assert not outside.any(), "non-fluid cells must keep zero populations"
     return np.ascontiguousarray(np.moveaxis(dense[box], -1, 0))  # [q, z, y, x]
+
+
+def run_cavity(n, tau, ulid, steps):
+    return run_reference((n, n, n), tau, ulid, steps)
 
 
 def cellkernel_fixture(seed, tau):
@@ -165,6 +177,15 @@ def main():
         np.savez_compressed(os.path.join(HERE, f"cavity_n{n}_t{steps}.npz"), f=f, n=np.int64(n),
                             tau=np.float64(tau), ulid=np.float64(ulid), steps=np.int64(steps))
         print("cavity", n, steps, file=sys.stderr)
+
+    # default channel: (nx, ny, nz), tau, u_mean, parabolic, steps
+    for dims, tau, um, par, steps in (((8, 8, 8), 0.6, 0.05, False, 10), ((16, 8, 8), 0.7, 0.04, True, 200),
+                                      ((10, 7, 9), 0.55, 0.03, True, 13)):
+        f = run_reference(dims, tau, um, steps, scenario="default-channel", parabolic=par)
+        name = f"channel_{dims[0]}x{dims[1]}x{dims[2]}_t{steps}_{'parabolic' if par else 'uniform'}.npz"
+        np.savez_compressed(os.path.join(HERE, name), f=f, dims=np.array(dims, np.int64), tau=np.float64(tau),
+                            umean=np.float64(um), parabolic=np.int64(par), steps=np.int64(steps))
+        print("channel", dims, steps, file=sys.stderr)
 
     if not args.skip_64:
         n, tau, ulid, steps = 64, 0.6, 0.05, 1000

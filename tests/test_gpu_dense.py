@@ -214,7 +214,7 @@ def test_error_paths_return_codes():
         with pytest.raises(lbm.LbmError, match="state"):
             lat.step(1)
         with pytest.raises(lbm.LbmError, match="class"):
-            lat.set_moving_wall_class(9, (0.1, 0, 0), 0xFF)
+            lat.set_moving_wall_class(32, (0.1, 0, 0), 0xFF)
         with pytest.raises(lbm.LbmError, match="porosity"):
             lat.set_porosity(np.full(512, 2.0))
         lat.init_equilibrium()

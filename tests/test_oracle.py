@@ -15,7 +15,7 @@ import numpy as np
 import pytest
 
 import oracle as O
-from conftest import GOLDEN, golden
+from conftest import GOLDEN, channel_tolerance, golden
 
 DP = ctypes.POINTER(ctypes.c_double)
 
@@ -168,6 +168,91 @@ def test_cavity64_1000_steps_matches_reference():
     u_p = np.moveaxis(u.reshape(64, 64, 64, 3), -1, 0)[:, :, h, :]
     assert np.max(np.abs(rho_p - d["rho_mid_y"])) <= 1e-12
     assert np.max(np.abs(u_p - d["u_mid_y"])) <= 1e-12 * np.max(np.abs(d["u_mid_y"]))
+
+
+CHANNEL_FIXTURES = ["channel_8x8x8_t10_uniform.npz", "channel_16x8x8_t200_parabolic.npz",
+                    "channel_10x7x9_t13_parabolic.npz"]
+
+
+@pytest.mark.parametrize("name", CHANNEL_FIXTURES)
+def test_channel_step_matches_reference(name):
+    d = golden(name)
+    nx, ny, nz = (int(v) for v in d["dims"])
+    lat = O.Lattice.channel(nx, ny, nz, float(d["tau"]), float(d["umean"]), parabolic=bool(d["parabolic"]))
+    f = lat.step(lat.equilibrium(), int(d["steps"]))
+    ref = d["f"].reshape(-1)
+    tol = channel_tolerance(d)
+    if tol == 0.0:
+        assert np.array_equal(f, ref), f"max diff {np.abs(f - ref).max()}"
+    else:
+        assert np.max(np.abs(f - ref)) <= tol * np.max(np.abs(ref))
+
+
+def test_channel_swap_streaming_equals_push():
+    d = golden("channel_8x8x8_t10_uniform.npz")
+    lat = O.Lattice.channel(8, 8, 8, float(d["tau"]), float(d["umean"]))
+    f = lat.step_swap(lat.equilibrium(), int(d["steps"]))
+    assert np.array_equal(f, d["f"].reshape(-1))
+
+
+def test_channel_classification_of_edge_links():
+    # corner cell (0,0,0): only links that leave through x- alone are inlet links; any link that
+    # also leaves through y- or z- is a wall (the y/z test comes first in the reference)
+    lat = O.Lattice.channel(6, 5, 4, 0.6, 0.02)
+    k = (int(lat.words[0]) >> O.CLASS_SHIFT) & 31
+    mask = lat.params.moving_mask[k]
+    inlet = [q for q in range(19) if (mask >> q) & 1]
+    assert inlet == [q for q in range(19) if O.C[q][0] < 0 and O.C[q][1] >= 0 and O.C[q][2] >= 0]
+    assert lat.params.bc_type[k] == O.BC_VELOCITY_INLET
+    # the interior of the outlet face: all five x+ links
+    i = 5 + 6 * (2 + 5 * 2)
+    k = (int(lat.words[i]) >> O.CLASS_SHIFT) & 31
+    assert lat.params.bc_type[k] == O.BC_PRESSURE_OUTLET
+    assert [q for q in range(19) if (lat.params.moving_mask[k] >> q) & 1] == [q for q in range(19) if O.C[q][0] > 0]
+    # an interior cell carries no class
+    assert (int(lat.words[2 + 6 * (2 + 5 * 2)]) >> O.CLASS_SHIFT) & 31 == 0
+
+
+@pytest.mark.parametrize("bc", ["outlet", "pressure_inlet", "velocity_inlet"])
+def test_boundary_value_returns_equilibrium_of_the_target_state(bc):
+    # At equilibrium with the boundary's own target state the incoming population must be that
+    # equilibrium's value for the incoming direction q' = 18 - q (anti-bounce-back consistency;
+    # fails on a swapped q/q', a sign error or a wrong target).
+    rng = np.random.default_rng(7)
+    p = O.Params()
+    rho, u = 1.0 + 0.1 * rng.random(), 0.05 * rng.standard_normal(3)
+    k = 1
+    p.bc_type[k] = {"outlet": O.BC_PRESSURE_OUTLET, "pressure_inlet": O.BC_PRESSURE_INLET,
+                    "velocity_inlet": O.BC_VELOCITY_INLET}[bc]
+    p.bc_rho[k] = rho
+    for d in range(3):
+        p.uw[k][d] = u[d]
+    feq = np.zeros(19)
+    O.lib().lbo_feq(rho, _p(np.ascontiguousarray(u)), _p(feq))
+    L = O.lib()
+    for q in range(19):
+        if q == O.REST:
+            continue
+        # velocity inlet ignores the cell; feed it an unrelated post-collision value
+        fq = feq[q] if bc != "velocity_inlet" else 0.123
+        v = L.lbo_boundary_value(ctypes.byref(p), k, 0, q, fq, rho, _p(np.ascontiguousarray(u)))
+        assert abs(v - feq[18 - q]) <= 4e-16 * abs(feq[18 - q]) + 1e-18, (q, v, feq[18 - q])
+    # and away from equilibrium the pressure conditions are linear in f_q with slope -1
+    if bc != "velocity_inlet":
+        a = L.lbo_boundary_value(ctypes.byref(p), k, 0, 3, 0.01, rho, _p(np.ascontiguousarray(u)))
+        b = L.lbo_boundary_value(ctypes.byref(p), k, 0, 3, 0.02, rho, _p(np.ascontiguousarray(u)))
+        assert np.isclose(b - a, -0.01, rtol=1e-12)
+
+
+def test_channel_reaches_uniform_plane_flux():
+    # steady continuity: after the transient, the mass flux through every x-plane is the same
+    nx, ny, nz = 16, 6, 6
+    lat = O.Lattice.channel(nx, ny, nz, 0.8, 0.03)
+    f = lat.step(lat.equilibrium(), 3000, nthreads=os.cpu_count() or 1)
+    rho, u = lat.fields(f)
+    flux = (rho * u[:, 0]).reshape(nz, ny, nx).sum(axis=(0, 1))
+    assert np.max(np.abs(flux - flux.mean())) <= 1e-4 * flux.mean()
+    assert flux.mean() > 0
 
 
 # ---------------------------------------------------------------- invariants
