@@ -170,7 +170,7 @@ typedef struct {
 } lbm_patch;
 typedef struct {
   int tile_x, tile_y, tile_z;  /* tile extents in cells (one workgroup per tile) */
-  int blocks_per_cu;           /* persistent grid = CUs * blocks_per_cu */
+  int blocks_per_cu;           /* persistent grid = CUs * blocks_per_cu (1..64); 0 = one workgroup per tile */
 } lbm_tiling;
 int lbm_patches_create(lbm_patchset** ps, const lbm_patch* table, int npatches, int precision,
                        int device);

@@ -373,16 +373,25 @@ extern "C" int lbm_adjoint_pressure_drop(lbm_ctx* c, int nsteps, int snapshots, 
   int64_t recompute = 0, reversed = 0;
   int rc_err = 0;
   ACHK(hipEventRecord(c->ev0, st));
-  auto load = [&](int t) -> int {
-    auto it = held.find(t);
+  // Checkpoints are read where they lie: the state s_{t0+j} is the slot itself for j = 0, else the
+  // first recomputed step reads the slot and writes the context's ping-pong buffers.
+  auto state_from = [&](int t0, int j, const void** out) -> int {
+    auto it = held.find(t0);
     if (it == held.end()) return adj_fail(c, LBM_ERR_STATE, "checkpoint schedule lost a state");
-    hipError_t e = hipMemcpyAsync(c->buf[c->cur], a->slots[it->second], sbytes, hipMemcpyDeviceToDevice, st);
-    return e == hipSuccess ? 0 : adj_fail(c, LBM_ERR_HIP, "checkpoint restore failed");
+    const void* s0 = a->slots[it->second];
+    if (j == 0) {
+      *out = s0;
+      return 0;
+    }
+    int r = lbm_enqueue_step_from(c, s0, c->buf[c->cur]);
+    if (!r) r = lbm_enqueue_steps(c, j - 1);
+    *out = c->buf[c->cur];
+    return r;
   };
-  auto adj_step = [&]() -> int {
-    hipError_t e = c->fp32 ? adj_step_t<float>(c->g, c->buf[c->cur], a->lam[lc], a->lam[lc ^ 1], c->words, c->alpha,
+  auto adj_step = [&](const void* sp) -> int {
+    hipError_t e = c->fp32 ? adj_step_t<float>(c->g, sp, a->lam[lc], a->lam[lc ^ 1], c->words, c->alpha,
                                                a->galpha, c->force, c->walls, c->p64, c->d_bct, st)
-                           : adj_step_t<double>(c->g, c->buf[c->cur], a->lam[lc], a->lam[lc ^ 1], c->words, c->alpha,
+                           : adj_step_t<double>(c->g, sp, a->lam[lc], a->lam[lc ^ 1], c->words, c->alpha,
                                                 a->galpha, c->force, c->walls, c->p64, c->d_bct, st);
     if (e != hipSuccess) return adj_fail(c, LBM_ERR_HIP, std::string("adjoint step: ") + hipGetErrorString(e));
     lc ^= 1;
@@ -392,18 +401,17 @@ extern "C" int lbm_adjoint_pressure_drop(lbm_ctx* c, int nsteps, int snapshots, 
   // reverse steps t0 .. t0+l-1 with s_t0 held and cc free slots
   std::function<int(int, int, int)> rev = [&](int t0, int l, int cc) -> int {
     if (rc_err) return rc_err;
+    const void* sp = nullptr;
     if (l == 1) {
-      int r = load(t0);
-      return r ? r : adj_step();
+      int r = state_from(t0, 0, &sp);
+      return r ? r : adj_step(sp);
     }
     if (cc == 0) {
       for (int j = l - 1; j >= 0; j--) {
-        int r = load(t0);
-        if (r) return r;
-        r = lbm_enqueue_steps(c, j);
+        int r = state_from(t0, j, &sp);
         if (r) return r;
         recompute += j;
-        r = adj_step();
+        r = adj_step(sp);
         if (r) return r;
       }
       return 0;
@@ -412,11 +420,6 @@ extern "C" int lbm_adjoint_pressure_drop(lbm_ctx* c, int nsteps, int snapshots, 
     const int tm = t0 + m;
     bool placed_here = false;
     if (!held.count(tm)) {
-      int r = load(t0);
-      if (r) return r;
-      r = lbm_enqueue_steps(c, m);
-      if (r) return r;
-      recompute += m;
       int slot = -1;
       for (int k = 1; k < nslots; k++)
         if (!busy[k]) {
@@ -424,8 +427,18 @@ extern "C" int lbm_adjoint_pressure_drop(lbm_ctx* c, int nsteps, int snapshots, 
           break;
         }
       if (slot < 0) return adj_fail(c, LBM_ERR_STATE, "no free checkpoint slot");
-      hipError_t e = hipMemcpyAsync(a->slots[slot], c->buf[c->cur], sbytes, hipMemcpyDeviceToDevice, st);
-      if (e != hipSuccess) return adj_fail(c, LBM_ERR_HIP, "checkpoint store failed");
+      // s_tm = m steps from the checkpoint at t0, the last one written straight into the new slot
+      auto it = held.find(t0);
+      if (it == held.end()) return adj_fail(c, LBM_ERR_STATE, "checkpoint schedule lost a state");
+      int r;
+      if (m == 1) {
+        r = lbm_enqueue_step_from(c, a->slots[it->second], a->slots[slot]);
+      } else {
+        r = state_from(t0, m - 1, &sp);
+        if (!r) r = lbm_enqueue_step_from(c, sp, a->slots[slot]);
+      }
+      if (r) return r;
+      recompute += m;
       busy[slot] = true;
       held[tm] = slot;
       placed_here = true;

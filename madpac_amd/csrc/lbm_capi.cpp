@@ -387,6 +387,12 @@ int lbm_get_populations(lbm_ctx* c, double* f) {
 
 }  // extern "C"
 
+int lbm_enqueue_step_from(lbm_ctx* c, const void* src, void* dst) {
+  HIPCHK(c, lbm::launch_dense_step(c->g, c->fp32, src, dst, c->words, c->alpha, c->force, c->walls, c->p64, c->p32,
+                                   lbm::BoundaryArgs{c->d_bct, c->d_bcu}, c->spec, c->stream));
+  return LBM_OK;
+}
+
 int lbm_enqueue_steps(lbm_ctx* c, int nsteps) {
   for (int s = 0; s < nsteps; s++) {
     HIPCHK(c, lbm::launch_dense_step(c->g, c->fp32, c->buf[c->cur], c->buf[c->cur ^ 1], c->words, c->alpha, c->force,

@@ -47,4 +47,6 @@ struct lbm_ctx {
 
 double lbm_tau_effective(double tau);
 int lbm_enqueue_steps(lbm_ctx* c, int nsteps);
+// one step from src into dst (any two state-sized device buffers), leaving buf/cur alone
+int lbm_enqueue_step_from(lbm_ctx* c, const void* src, void* dst);
 void lbm_adjoint_release(lbm_ctx* c);

@@ -29,17 +29,35 @@
 namespace lbm {
 
 struct PatchDev {
-  int64_t base;    // element offset of the patch's SoA block (19 planes of np^3)
-  int n;           // interior edge
-  int np;          // n + 2
-  double omega;    // 1 / tau_eff of the patch
-  int nbr[Q];      // neighbour patch in direction c_q
+  int64_t base;      // element offset of the patch's SoA block (19 planes of np^3)
+  int n;             // interior edge
+  int np;            // n + 2
+  double omega;      // 1 / tau_eff of the patch
+  int nbr[Q];        // neighbour patch in direction c_q
+  int64_t nbase[Q];  // that neighbour's base (one load fewer on the ghost path)
 };
 
+// Everything one tile iteration needs, so that a workgroup fetches a single record per tile (and
+// the next one while it works on the current).
 struct TileDev {
+  int64_t base;      // the patch's base
+  double omega;
   int patch;
-  int x0, y0, z0;  // tile origin, interior coordinates
+  int n, np;
+  int x0, y0, z0;    // tile origin, interior coordinates
 };
+
+// D3Q19 index of a face or edge direction e (at most two non-zero components), by arithmetic on
+// the z-major ordering: z = 0 holds the 3x3 block 5..13, z = -1 / +1 the five directions 0..4 / 14..18.
+__host__ __device__ constexpr int dir_index(int ex, int ey, int ez) {
+  return ez == 0 ? 9 + ex + 3 * ey : (ez < 0 ? 2 : 16) + ex + 2 * ey;
+}
+constexpr bool dir_index_matches_table() {
+  for (int q = 0; q < Q; q++)
+    if (dir_index(CX(q), CY(q), CZ(q)) != q) return false;
+  return true;
+}
+static_assert(dir_index_matches_table(), "dir_index must invert the D3Q19 ordering");
 
 // D3Q19 index of a direction vector e in {-1,0,1}^3 (-1 where not a D3Q19 direction)
 __host__ __device__ constexpr int dir_of(int ex, int ey, int ez) {
@@ -66,13 +84,22 @@ __global__ __launch_bounds__(256) void k_patch_step(const PatchDev* __restrict__
                                                     const TileDev* __restrict__ tiles, int ntiles,
                                                     const T* __restrict__ src, T* __restrict__ dst) {
   constexpr int TC = TX * TY * TZ;
-  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
-    const TileDev tl = tiles[t];
-    const PatchDev pd = patches[tl.patch];
-    const int n = pd.n, np = pd.np;
+  int t = blockIdx.x;
+  if (t >= ntiles) return;
+  TileDev tl = tiles[t];
+  for (;;) {
+    // prefetch the next tile's record; it is waited for only at the end of this iteration
+    const int tn = t + gridDim.x;
+    TileDev next{};
+    if (tn < ntiles) next = tiles[tn];
+    // per-lane lookups go through the global table (a private copy would live in scratch memory)
+    const PatchDev* __restrict__ pp = patches + tl.patch;
+    const int64_t base = tl.base;
+    const int n = tl.n, np = tl.np;
+    const double omega = tl.omega;
     const int64_t P = (int64_t)np * np * np;
-    const T* __restrict__ s = src + pd.base;
-    T* __restrict__ d = dst + pd.base;
+    const T* __restrict__ s = src + base;
+    T* __restrict__ d = dst + base;
     for (int c = threadIdx.x; c < TC; c += 256) {
       const int x = tl.x0 + c % TX, y = tl.y0 + (c / TX) % TY, z = tl.z0 + c / (TX * TY);
       if (x >= n || y >= n || z >= n) continue;
@@ -83,7 +110,7 @@ __global__ __launch_bounds__(256) void k_patch_step(const PatchDev* __restrict__
         const int64_t off = (int64_t)CX(q) + (int64_t)np * (CY(q) + (int64_t)np * CZ(q));
         f[q] = s[(int64_t)q * P + i - off];
       }
-      collide_patch<T>(f, pd.omega);
+      collide_patch<T>(f, omega);
 #pragma unroll
       for (int q = 0; q < Q; q++) __builtin_nontemporal_store(f[q], &d[(int64_t)q * P + i]);
       // ghost-layer exchange: populations leaving the patch go to the neighbour's ghost layer
@@ -96,14 +123,16 @@ __global__ __launch_bounds__(256) void k_patch_step(const PatchDev* __restrict__
           const int ey = ty < 0 ? -1 : (ty >= n ? 1 : 0);
           const int ez = tz < 0 ? -1 : (tz >= n ? 1 : 0);
           if ((ex | ey | ez) == 0) continue;
-          const int k = dir_of(ex, ey, ez);  // a face or edge direction: always a D3Q19 direction
-          const PatchDev& nb = patches[pd.nbr[k]];
+          const int k = dir_index(ex, ey, ez);  // a face or edge direction: always a D3Q19 direction
           const int gx = x - n * ex, gy = y - n * ey, gz = z - n * ez;  // ghost position in the neighbour
           const int64_t gi = (int64_t)(gx + 1) + (int64_t)np * ((gy + 1) + (int64_t)np * (gz + 1));
-          dst[nb.base + (int64_t)q * P + gi] = f[q];
+          dst[pp->nbase[k] + (int64_t)q * P + gi] = f[q];
         }
       }
     }
+    if (tn >= ntiles) break;
+    t = tn;
+    tl = next;
   }
 }
 
@@ -115,8 +144,9 @@ __global__ __launch_bounds__(256) void k_patch_init(const PatchDev* __restrict__
                                                     const double* __restrict__ u, T* __restrict__ s) {
   const int p = blockIdx.y;
   if (p >= npatch) return;
-  const PatchDev pd = patches[p];
-  const int n = pd.n, np = pd.np;
+  const PatchDev* __restrict__ pp = patches + p;  // no private copy: nbr is indexed per lane
+  const int64_t base = pp->base;
+  const int n = pp->n, np = pp->np;
   const int64_t P = (int64_t)np * np * np;
   const int64_t cells = (int64_t)n * n * n;
   for (int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x; c < cells; c += (int64_t)gridDim.x * 256) {
@@ -134,12 +164,12 @@ __global__ __launch_bounds__(256) void k_patch_init(const PatchDev* __restrict__
           (1.0 + cu / (1.0 / 3.0) + (cu * cu) / (2.0 * (1.0 / 3.0) * (1.0 / 3.0)) - (uu / (2.0 * (1.0 / 3.0))));
       const T v = (sizeof(T) == 8) ? (T)feq : (T)(feq - W64(q));
       const int sx = x - CX(q), sy = y - CY(q), sz = z - CZ(q);  // the slot x - c_q
-      s[pd.base + (int64_t)q * P + (int64_t)(sx + 1) + (int64_t)np * ((sy + 1) + (int64_t)np * (sz + 1))] = v;
+      s[base + (int64_t)q * P + (int64_t)(sx + 1) + (int64_t)np * ((sy + 1) + (int64_t)np * (sz + 1))] = v;
       const int ex = sx < 0 ? -1 : (sx >= n ? 1 : 0);
       const int ey = sy < 0 ? -1 : (sy >= n ? 1 : 0);
       const int ez = sz < 0 ? -1 : (sz >= n ? 1 : 0);
       if ((ex | ey | ez) != 0) {
-        const PatchDev& nb = patches[pd.nbr[dir_of(ex, ey, ez)]];
+        const PatchDev& nb = patches[pp->nbr[dir_of(ex, ey, ez)]];
         const int gx = sx - n * ex, gy = sy - n * ey, gz = sz - n * ez;  // interior of the neighbour
         s[nb.base + (int64_t)q * P + (int64_t)(gx + 1) + (int64_t)np * ((gy + 1) + (int64_t)np * (gz + 1))] = v;
       }
@@ -292,7 +322,10 @@ int build_tiles(lbm_patchset* ps, const lbm_tiling& t) {
     const int n = ps->host[p].n;
     for (int z = 0; z < n; z += t.tile_z)
       for (int y = 0; y < n; y += t.tile_y)
-        for (int x = 0; x < n; x += t.tile_x) tiles.push_back(TileDev{p, x, y, z});
+        for (int x = 0; x < n; x += t.tile_x) {
+          const PatchDev& d = ps->host[p];
+          tiles.push_back(TileDev{d.base, d.omega, p, d.n, d.np, x, y, z});
+        }
   }
   if (ps->d_tiles) (void)hipFree(ps->d_tiles);
   ps->d_tiles = nullptr;
@@ -308,7 +341,9 @@ int build_tiles(lbm_patchset* ps, const lbm_tiling& t) {
 template <typename T>
 hipError_t launch_step_t(lbm_patchset* ps, const void* src, void* dst) {
   const lbm_tiling& t = ps->tiling;
-  const dim3 grid((unsigned)std::max(1, std::min(ps->ntiles, ps->cus * t.blocks_per_cu))), block(256);
+  // blocks_per_cu = 0: one workgroup per tile (the loop body runs once)
+  const int wgs = t.blocks_per_cu == 0 ? ps->ntiles : std::min(ps->ntiles, ps->cus * t.blocks_per_cu);
+  const dim3 grid((unsigned)std::max(1, wgs)), block(256);
   const T* s = (const T*)src;
   T* d = (T*)dst;
 #define LBM_PATCH(X, Y, Z)                                                                                     \
@@ -380,6 +415,8 @@ int lbm_patches_create(lbm_patchset** out, const lbm_patch* table, int npatches,
     base += ((int64_t)Q * d.np * d.np * d.np + 63) / 64 * 64;  // 64-element aligned blocks
     cb += (int64_t)d.n * d.n * d.n;
   }
+  for (auto& d : ps->host)
+    for (int q = 0; q < Q; q++) d.nbase[q] = ps->host[d.nbr[q]].base;
   ps->elems = base;
   ps->cells = cb;
   hipDeviceProp_t prop;
@@ -492,7 +529,7 @@ int lbm_patches_get_populations(lbm_patchset* ps, double* f) {
 int lbm_patches_set_tiling(lbm_patchset* ps, const lbm_tiling* t) {
   if (!ps) return LBM_ERR_ARG;
   if (!t || !shape_supported(t)) return pfail(ps, LBM_ERR_ARG, "unsupported tile shape");
-  if (t->blocks_per_cu < 1 || t->blocks_per_cu > 32) return pfail(ps, LBM_ERR_ARG, "blocks_per_cu must be in 1..32");
+  if (t->blocks_per_cu < 0 || t->blocks_per_cu > 64) return pfail(ps, LBM_ERR_ARG, "blocks_per_cu must be in 0..64");
   PCHK(ps, hipSetDevice(ps->device));
   return build_tiles(ps, *t);
 }
@@ -544,7 +581,7 @@ int lbm_patches_autotune(lbm_patchset* ps, lbm_tiling* chosen) {
   PCHK(ps, hipMemcpy(save, ps->buf[cur0], bytes, hipMemcpyDeviceToDevice));
   lbm_tiling best{};
   float best_ms = 1e30f;
-  const int bpcs[] = {2, 4, 8};
+  const int bpcs[] = {4, 8, 16, 32, 64, 0};
   int rc = LBM_OK;
   for (const Shape& sh : kShapes) {
     for (int bpc : bpcs) {
