@@ -107,6 +107,23 @@ int lbm_set_cavity_lid(lbm_ctx* ctx, double ulid);
  * u_mean * prod_{d=y,z} 6 a_d (L_d - a_d) / L_d^2 at the cell centre (lattice units), rho_L = 1;
  * links leaving through x+ are a pressure outlet with rho_L = 1.  Uses classes 1..18. */
 int lbm_set_channel(lbm_ctx* ctx, double u_mean, int parabolic);
+/* ---- MD <-> LB coupling (madpac/peanocoupling/PeanoDataExchangeService.cpp:69-218) ------ */
+/* At a list of coupled cells the MD side either overwrites the equilibrium velocity (and, with
+ * rho != NULL, the density) of the collision (getAndSetV :110-129), or, in the Dirichlet mode,
+ * forces the cell toward the MD velocity: g = (u_MD - u)/dt, u <- u + dt/2 g, and after the BGK
+ * relaxation f_i += dt g_i with the reference's computeGi (:161-186, its (c.u)^2 term included).
+ * Velocities are in lattice units (the caller applies the reference's _lb2ms / _densityLB2MD).
+ * Coupled cells use the plain collision: contexts with a body force or porosity are rejected. */
+enum { LBM_COUPLE_OVERWRITE = 0, LBM_COUPLE_DIRICHLET = 1 };
+/* the coupled cells (distinct indices; n = 0 clears) */
+int lbm_set_coupling_cells(lbm_ctx* ctx, const int64_t* cells, int64_t n);
+/* MD data applied by every following step until changed: u[3k+d], rho[k] (NULL: no density
+ * coupling); u == NULL switches the coupling off.  dt is the Dirichlet forcing's time step. */
+int lbm_set_coupling_data(lbm_ctx* ctx, int mode, const double* u, const double* rho, double dt);
+/* The export to MD (getAndSetV :136-151 before its scaling): rho[k] and u[3k+d] = (sum f c)/rho
+ * of the current state at the coupled cells (0 at non-fluid cells); either pointer may be NULL. */
+int lbm_get_coupling_data(lbm_ctx* ctx, double* u, double* rho);
+
 /* Per-cell Brinkman porosity alpha in [0,1] (NULL disables).  Equilibrium velocity is
  * scaled by (1 - alpha).  No reference counterpart (design variable of the adjoint). */
 int lbm_set_porosity(lbm_ctx* ctx, const double* alpha);

@@ -146,6 +146,7 @@ int lbm_destroy(lbm_ctx* c) {
   if (!c) return LBM_OK;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   lbm_adjoint_release(c);
+  lbm_coupling_release(c);
   void* ptrs[] = {c->buf[0], c->buf[1], c->words,   c->d_inner, c->d_cls, c->alpha,
                   c->partials, c->small,  c->scratch, c->d_bct,   c->d_bcu};
   for (void* p : ptrs)
@@ -317,6 +318,7 @@ int lbm_set_porosity(lbm_ctx* c, const double* alpha) {
   }
   for (int64_t i = 0; i < c->g.Nu; i++)
     if (!(alpha[i] >= 0.0 && alpha[i] <= 1.0)) return fail(c, LBM_ERR_ARG, "porosity must lie in [0,1]");
+  if (c->couple_on) return fail(c, LBM_ERR_STATE, "coupled cells use the plain collision: no porosity");
   if (!c->alpha && hipMalloc(&c->alpha, (size_t)c->g.N * c->esize) != hipSuccess)
     return fail(c, LBM_ERR_NOMEM, "porosity allocation failed");
   // the kernels index the porosity by padded cell; halo cells get 0
@@ -398,6 +400,8 @@ int lbm_enqueue_steps(lbm_ctx* c, int nsteps) {
     HIPCHK(c, lbm::launch_dense_step(c->g, c->fp32, c->buf[c->cur], c->buf[c->cur ^ 1], c->words, c->alpha, c->force,
                                      c->walls, c->p64, c->p32, lbm::BoundaryArgs{c->d_bct, c->d_bcu},
                                      c->spec, c->stream));
+    const int rc = lbm_enqueue_coupling(c, c->buf[c->cur], c->buf[c->cur ^ 1]);
+    if (rc) return rc;
     c->cur ^= 1;
   }
   return LBM_OK;

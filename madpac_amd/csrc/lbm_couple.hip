@@ -1,0 +1,304 @@
+// MD <-> LB coupling hooks at a list of cells (the MaDPaC use of the block-LB solver).
+//
+// Reference: madpac/peanocoupling/PeanoDataExchangeService.cpp.  getAndSetV (:69-158) runs per
+// block between computeVelocity and the collision: at coupled cells it either overwrites the LB
+// velocity (and, with density coupling, the density) with the MD values, or, in the Dirichlet
+// mode, computes g = (u_MD - u)/dt and shifts u by dt/2 g; adjustF (:189-218) then adds dt g_i
+// (computeGi, :161-186) after the BGK relaxation; and it exports the LB density and velocity of
+// the same cells to the MD side.  Here the fused step kernel collides every cell as usual, and a
+// second small kernel re-collides only the coupled cells from the same source state and overwrites
+// their (own, pull-form) output slots; the hot kernel is unchanged.  A gather kernel returns rho
+// and u of the coupled cells (the export).  Coupled cells use the reference's block-solver
+// collision: no body force and no porosity (the host rejects those combinations).
+#include <cmath>
+#include <string>
+#include <vector>
+
+#include "lbm_ctx.h"
+#include "lbm_geom.h"
+
+namespace lbm {
+
+struct CoupleArgs {
+  const int64_t* cells;   // coupled cell indices
+  int64_t n;
+  const double* u;        // [3n] MD velocity (lattice units)
+  const double* rho;      // [n] MD density or null
+  int mode;               // LBM_COUPLE_*
+  double dt;              // LB time step of the Dirichlet forcing
+  double tau;             // configured tau (computeGi's _tau)
+  double omega;           // 1 / tau_eff (BlockBGK)
+};
+
+__device__ __forceinline__ CellPos pos_of(const DenseGeom& g, int64_t i) {
+  CellPos c;
+  c.i = i;
+  c.x = (int)(i % g.nx);
+  c.y = (int)((i / g.nx) % g.ny);
+  c.z = (int)(i / ((int64_t)g.nx * g.ny));
+  c.valid = true;
+  return c;
+}
+
+// FP64 collision of one coupled cell (reference operation order, as the oracle restates it).
+__device__ __forceinline__ double collide_coupled_f64(double (&f)[Q], const CoupleArgs& a, int64_t k,
+                                                      double (&uo)[3]) {
+  double rho = 0.0;
+#pragma unroll
+  for (int q = 0; q < Q; q++) rho += f[q];
+  double b0 = 0.0, b1 = 0.0, b2 = 0.0;
+#pragma unroll
+  for (int q = 0; q < Q; q++) {
+    if (CX(q) == 1) b0 += f[q];
+    if (CX(q) == -1) b0 -= f[q];
+    if (CY(q) == 1) b1 += f[q];
+    if (CY(q) == -1) b1 -= f[q];
+    if (CZ(q) == 1) b2 += f[q];
+    if (CZ(q) == -1) b2 -= f[q];
+  }
+  const double inv = 1.0 / rho;
+  double u[3] = {b0 * inv, b1 * inv, b2 * inv};
+  const double* um = a.u + 3 * k;
+  double g[3] = {0.0, 0.0, 0.0};
+  const bool dir = a.mode == LBM_COUPLE_DIRICHLET;
+  if (dir) {
+    // getAndSetV :110-117: g = (u_MD - u)/dt, u += dt/2 g
+    for (int d = 0; d < 3; d++) {
+      g[d] = (um[d] - u[d]) / a.dt;
+      u[d] += a.dt / 2.0 * g[d];
+    }
+  } else {
+    for (int d = 0; d < 3; d++) u[d] = um[d];  // :121-122
+  }
+  if (a.rho) rho = a.rho[k];  // :124-129
+  uo[0] = u[0];
+  uo[1] = u[1];
+  uo[2] = u[2];
+  double uu = u[0] * u[0];
+  uu += u[1] * u[1];
+  uu += u[2] * u[2];
+#pragma unroll
+  for (int q = 0; q < Q; q++) f[q] -= a.omega * (f[q] - feq_f64(q, rho, u[0], u[1], u[2], uu));
+  if (dir) {
+    // adjustF / computeGi, with (c.u)^2 as the reference writes it
+    const double pref = 1.0 - 0.5 / a.tau;
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+      const double m0 = (double)CX(q) - u[0], m1 = (double)CY(q) - u[1], m2 = (double)CZ(q) - u[2];
+      double cmg = m0 * g[0];
+      cmg += m1 * g[1];
+      cmg += m2 * g[2];
+      const double cu = cdot64(q, u[0], u[1], u[2]);
+      const double cg = cdot64(q, g[0], g[1], g[2]);
+      const double gi = pref * W64(q) * rho *
+                        (1.0 / (1.0 / 3.0) * cmg + (cu * cu) / ((1.0 / 3.0) * (1.0 / 3.0)) * cg);
+      f[q] += a.dt * gi;
+    }
+  }
+  return rho;
+}
+
+// Re-collide the coupled cells from src and overwrite their own output slots in dst (run after the
+// step kernel that wrote dst from src).  FP32-shifted contexts are lifted to FP64 for these cells.
+template <typename T, bool WALLS>
+__global__ __launch_bounds__(256) void k_couple(DenseGeom g, const T* __restrict__ src, T* __restrict__ dst,
+                                                const uint32_t* __restrict__ words, CoupleArgs a, BoundaryArgs bc) {
+  const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (k >= a.n) return;
+  const int64_t i = a.cells[k];
+  const uint32_t w = words[i];
+  if (!(w & FLUID)) return;
+  const NbrOff o = nbr_offsets(g, pos_of(g, i));
+  const int64_t N = g.N;
+  double f[Q];
+#pragma unroll
+  for (int q = 0; q < Q; q++) {
+    const T v = src[pull_index(q, w, i, N, o)];
+    f[q] = sizeof(T) == 8 ? (double)v : (double)v + W64(q);
+  }
+  double u[3];
+  const double rho = collide_coupled_f64(f, a, k, u);
+  if (WALLS) {
+    const uint32_t cls = (w >> CLASS_SHIFT) & CLASS_BITS;
+    if (cls) {
+      const BoundaryClass& b = bc.table[cls];
+      const uint32_t m = w & b.mask;
+      if (m) apply_boundary_f64(f, m, b, bc.cell_u, i, rho, u[0], u[1], u[2]);
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < Q; q++) dst[(int64_t)q * N + i] = sizeof(T) == 8 ? (T)f[q] : (T)(f[q] - W64(q));
+}
+
+// Export: rho and u = (sum f c) / rho of the reference state at the coupled cells (0 at solid cells).
+template <typename T>
+__global__ __launch_bounds__(256) void k_couple_gather(DenseGeom g, const T* __restrict__ s,
+                                                       const uint32_t* __restrict__ words, const int64_t* __restrict__ cells,
+                                                       int64_t n, double* __restrict__ out) {
+  const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (k >= n) return;
+  const int64_t i = cells[k];
+  const uint32_t w = words[i];
+  double rho = 0.0, u0 = 0.0, u1 = 0.0, u2 = 0.0;
+  if (w & FLUID) {
+    const NbrOff o = nbr_offsets(g, pos_of(g, i));
+    double f[Q];
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+      const T v = s[pull_index(q, w, i, g.N, o)];
+      f[q] = sizeof(T) == 8 ? (double)v : (double)v + W64(q);
+    }
+    double b0 = 0.0, b1 = 0.0, b2 = 0.0;
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+      rho += f[q];
+      if (CX(q) == 1) b0 += f[q];
+      if (CX(q) == -1) b0 -= f[q];
+      if (CY(q) == 1) b1 += f[q];
+      if (CY(q) == -1) b1 -= f[q];
+      if (CZ(q) == 1) b2 += f[q];
+      if (CZ(q) == -1) b2 -= f[q];
+    }
+    const double inv = 1.0 / rho;
+    u0 = b0 * inv;
+    u1 = b1 * inv;
+    u2 = b2 * inv;
+  }
+  out[4 * k] = rho;
+  out[4 * k + 1] = u0;
+  out[4 * k + 2] = u1;
+  out[4 * k + 3] = u2;
+}
+
+}  // namespace lbm
+
+using lbm::Q;
+
+namespace {
+int cfail(lbm_ctx* c, int code, const std::string& msg) {
+  c->err = msg;
+  return code;
+}
+}  // namespace
+
+// Launch the re-collision of the coupled cells for the step src -> dst (called by the step loop).
+int lbm_enqueue_coupling(lbm_ctx* c, const void* src, void* dst) {
+  if (!c->couple_on || c->ncouple == 0) return LBM_OK;
+  lbm::CoupleArgs a{c->d_ccells, c->ncouple, c->d_cu, c->couple_rho ? c->d_crho : nullptr, c->couple_mode,
+                    c->couple_dt, c->cfg.tau, c->p64.omega};
+  const lbm::BoundaryArgs bc{c->d_bct, c->d_bcu};
+  const dim3 grid((unsigned)((c->ncouple + 255) / 256)), block(256);
+  if (c->fp32) {
+    if (c->walls)
+      hipLaunchKernelGGL((lbm::k_couple<float, true>), grid, block, 0, c->stream, c->g, (const float*)src, (float*)dst,
+                         c->words, a, bc);
+    else
+      hipLaunchKernelGGL((lbm::k_couple<float, false>), grid, block, 0, c->stream, c->g, (const float*)src,
+                         (float*)dst, c->words, a, bc);
+  } else {
+    if (c->walls)
+      hipLaunchKernelGGL((lbm::k_couple<double, true>), grid, block, 0, c->stream, c->g, (const double*)src,
+                         (double*)dst, c->words, a, bc);
+    else
+      hipLaunchKernelGGL((lbm::k_couple<double, false>), grid, block, 0, c->stream, c->g, (const double*)src,
+                         (double*)dst, c->words, a, bc);
+  }
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? LBM_OK : cfail(c, LBM_ERR_HIP, std::string("coupling kernel: ") + hipGetErrorString(e));
+}
+
+void lbm_coupling_release(lbm_ctx* c) {
+  void* ptrs[] = {c->d_ccells, c->d_cu, c->d_crho, c->d_cout};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  c->d_ccells = nullptr;
+  c->d_cu = c->d_crho = c->d_cout = nullptr;
+  c->ncouple = 0;
+  c->couple_on = false;
+}
+
+extern "C" {
+
+int lbm_set_coupling_cells(lbm_ctx* c, const int64_t* cells, int64_t n) {
+  if (!c) return LBM_ERR_ARG;
+  if (n < 0 || (n > 0 && !cells)) return cfail(c, LBM_ERR_ARG, "coupling cell list is null or negative");
+  if (hipSetDevice(c->cfg.device) != hipSuccess || hipStreamSynchronize(c->stream) != hipSuccess)
+    return cfail(c, LBM_ERR_HIP, "device synchronisation failed");
+  lbm_coupling_release(c);
+  if (n == 0) return LBM_OK;
+  std::vector<uint8_t> seen((size_t)c->g.Nu, 0);
+  for (int64_t k = 0; k < n; k++) {
+    if (cells[k] < 0 || cells[k] >= c->g.Nu) return cfail(c, LBM_ERR_ARG, "coupling cell out of range");
+    if (seen[(size_t)cells[k]]++) return cfail(c, LBM_ERR_ARG, "coupling cells must be distinct");
+  }
+  if (hipMalloc(&c->d_ccells, (size_t)n * sizeof(int64_t)) != hipSuccess ||
+      hipMalloc(&c->d_cu, (size_t)n * 3 * sizeof(double)) != hipSuccess ||
+      hipMalloc(&c->d_crho, (size_t)n * sizeof(double)) != hipSuccess ||
+      hipMalloc(&c->d_cout, (size_t)n * 4 * sizeof(double)) != hipSuccess) {
+    lbm_coupling_release(c);
+    return cfail(c, LBM_ERR_NOMEM, "coupling buffers allocation failed");
+  }
+  if (hipMemcpy(c->d_ccells, cells, (size_t)n * sizeof(int64_t), hipMemcpyHostToDevice) != hipSuccess) {
+    lbm_coupling_release(c);
+    return cfail(c, LBM_ERR_HIP, "coupling cell upload failed");
+  }
+  c->ncouple = n;
+  return LBM_OK;
+}
+
+int lbm_set_coupling_data(lbm_ctx* c, int mode, const double* u, const double* rho, double dt) {
+  if (!c) return LBM_ERR_ARG;
+  if (!u) {
+    c->couple_on = false;
+    return LBM_OK;
+  }
+  if (c->ncouple == 0) return cfail(c, LBM_ERR_STATE, "no coupling cells (lbm_set_coupling_cells)");
+  if (mode != LBM_COUPLE_OVERWRITE && mode != LBM_COUPLE_DIRICHLET) return cfail(c, LBM_ERR_ARG, "unknown coupling mode");
+  if (mode == LBM_COUPLE_DIRICHLET && !(dt > 0.0 && std::isfinite(dt)))
+    return cfail(c, LBM_ERR_ARG, "coupling time step must be positive");
+  if (c->force || c->alpha)
+    return cfail(c, LBM_ERR_STATE, "coupled cells use the plain collision: no body force or porosity");
+  for (int64_t k = 0; k < 3 * c->ncouple; k++)
+    if (!std::isfinite(u[k])) return cfail(c, LBM_ERR_ARG, "coupling velocities must be finite");
+  if (rho)
+    for (int64_t k = 0; k < c->ncouple; k++)
+      if (!(rho[k] > 0.0) || !std::isfinite(rho[k])) return cfail(c, LBM_ERR_ARG, "coupling densities must be positive");
+  if (hipSetDevice(c->cfg.device) != hipSuccess ||
+      hipMemcpyAsync(c->d_cu, u, (size_t)c->ncouple * 3 * sizeof(double), hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+      (rho && hipMemcpyAsync(c->d_crho, rho, (size_t)c->ncouple * sizeof(double), hipMemcpyHostToDevice, c->stream) !=
+                  hipSuccess) ||
+      hipStreamSynchronize(c->stream) != hipSuccess)
+    return cfail(c, LBM_ERR_HIP, "coupling data upload failed");
+  c->couple_mode = mode;
+  c->couple_dt = dt;
+  c->couple_rho = rho != nullptr;
+  c->couple_on = true;
+  return LBM_OK;
+}
+
+int lbm_get_coupling_data(lbm_ctx* c, double* u, double* rho) {
+  if (!c) return LBM_ERR_ARG;
+  if (c->ncouple == 0) return cfail(c, LBM_ERR_STATE, "no coupling cells (lbm_set_coupling_cells)");
+  if (!c->flags_set || !c->state_set) return cfail(c, LBM_ERR_STATE, "flags and state must be set first");
+  if (hipSetDevice(c->cfg.device) != hipSuccess) return cfail(c, LBM_ERR_HIP, "hipSetDevice failed");
+  const dim3 grid((unsigned)((c->ncouple + 255) / 256)), block(256);
+  if (c->fp32)
+    hipLaunchKernelGGL(lbm::k_couple_gather<float>, grid, block, 0, c->stream, c->g, (const float*)c->buf[c->cur],
+                       c->words, c->d_ccells, c->ncouple, c->d_cout);
+  else
+    hipLaunchKernelGGL(lbm::k_couple_gather<double>, grid, block, 0, c->stream, c->g, (const double*)c->buf[c->cur],
+                       c->words, c->d_ccells, c->ncouple, c->d_cout);
+  std::vector<double> out((size_t)c->ncouple * 4);
+  if (hipGetLastError() != hipSuccess ||
+      hipMemcpyAsync(out.data(), c->d_cout, out.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+      hipStreamSynchronize(c->stream) != hipSuccess)
+    return cfail(c, LBM_ERR_HIP, "coupling gather failed");
+  for (int64_t k = 0; k < c->ncouple; k++) {
+    if (rho) rho[k] = out[4 * k];
+    if (u)
+      for (int d = 0; d < 3; d++) u[3 * k + d] = out[4 * k + 1 + d];
+  }
+  return LBM_OK;
+}
+
+}  // extern "C"

@@ -34,6 +34,13 @@ struct lbm_ctx {
   lbm::BoundaryClass bct[lbm::MAX_CLASSES]{};  // host copy of the boundary-class table
   lbm::BoundaryClass* d_bct = nullptr;          // device copy (read by the step kernels)
   double* d_bcu = nullptr;                      // per-cell target velocities [3*N] or null
+  // MD coupling (lbm_couple.hip)
+  int64_t* d_ccells = nullptr;                  // coupled cells
+  int64_t ncouple = 0;
+  double *d_cu = nullptr, *d_crho = nullptr, *d_cout = nullptr;
+  bool couple_on = false, couple_rho = false;
+  int couple_mode = 0;
+  double couple_dt = 1.0;
   lbm::CollideParams p64{};
   lbm::CollideParams32 p32{};
   double* partials = nullptr;
@@ -49,4 +56,7 @@ double lbm_tau_effective(double tau);
 int lbm_enqueue_steps(lbm_ctx* c, int nsteps);
 // one step from src into dst (any two state-sized device buffers), leaving buf/cur alone
 int lbm_enqueue_step_from(lbm_ctx* c, const void* src, void* dst);
+// re-collision of the MD-coupled cells for the step src -> dst (no-op without coupling)
+int lbm_enqueue_coupling(lbm_ctx* c, const void* src, void* dst);
+void lbm_coupling_release(lbm_ctx* c);
 void lbm_adjoint_release(lbm_ctx* c);

@@ -20,6 +20,7 @@ FP32_SHIFTED = 1
 # boundary-condition types of a class (include/lbm_capi.h)
 BC_MOVING_WALL, BC_PRESSURE_OUTLET, BC_PRESSURE_INLET, BC_VELOCITY_INLET = 0, 1, 2, 3
 BC_CELL_VELOCITY = 16
+COUPLE_OVERWRITE, COUPLE_DIRICHLET = 0, 1
 
 # D3Q19 ordering (madpac/peano/applications/latticeboltzmann/LatticeBoltzmannDefinitions.cpp:113-131)
 C = np.array([
@@ -33,6 +34,7 @@ EXPORTS = (
     "lbm_create", "lbm_destroy", "lbm_last_error", "lbm_fluid_cells", "lbm_set_flags",
     "lbm_set_boundary_class", "lbm_set_moving_wall_class", "lbm_set_wall_classes", "lbm_set_boundary_velocity",
     "lbm_set_cavity_lid", "lbm_set_channel", "lbm_set_porosity",
+    "lbm_set_coupling_cells", "lbm_set_coupling_data", "lbm_get_coupling_data",
     "lbm_init_equilibrium", "lbm_set_populations", "lbm_get_populations", "lbm_step", "lbm_step_timed",
     "lbm_tune", "lbm_synchronize", "lbm_get_fields", "lbm_reduce_moments", "lbm_kinetic_energy",
     "lbm_pressure_drop",
@@ -95,6 +97,9 @@ def load_library(path: str = LIB_PATH):
     L.lbm_set_boundary_velocity.argtypes = [vp, dp]
     L.lbm_set_cavity_lid.argtypes = [vp, ctypes.c_double]
     L.lbm_set_channel.argtypes = [vp, ctypes.c_double, ctypes.c_int]
+    L.lbm_set_coupling_cells.argtypes = [vp, ctypes.POINTER(ctypes.c_int64), ctypes.c_int64]
+    L.lbm_set_coupling_data.argtypes = [vp, ctypes.c_int, dp, dp, ctypes.c_double]
+    L.lbm_get_coupling_data.argtypes = [vp, dp, dp]
     L.lbm_set_porosity.argtypes = [vp, dp]
     L.lbm_init_equilibrium.argtypes = [vp, dp, dp]
     L.lbm_set_populations.argtypes = [vp, dp]
@@ -199,6 +204,33 @@ class Lattice:
         a = np.ascontiguousarray(np.asarray(u, dtype=np.float64).reshape(-1))
         assert a.size == 3 * self.n
         self._check(self._L.lbm_set_boundary_velocity(self._h, _dp(a)), "lbm_set_boundary_velocity")
+
+    # ---- MD coupling
+    def set_coupling_cells(self, cells):
+        if cells is None or len(cells) == 0:
+            self._check(self._L.lbm_set_coupling_cells(self._h, None, 0), "lbm_set_coupling_cells")
+            self._ncouple = 0
+            return
+        a = np.ascontiguousarray(np.asarray(cells, dtype=np.int64).reshape(-1))
+        self._check(self._L.lbm_set_coupling_cells(self._h, a.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), a.size),
+                    "lbm_set_coupling_cells")
+        self._ncouple = a.size
+
+    def set_coupling_data(self, mode, u, rho=None, dt=1.0):
+        if u is None:
+            self._check(self._L.lbm_set_coupling_data(self._h, int(mode), None, None, float(dt)),
+                        "lbm_set_coupling_data")
+            return
+        u = np.ascontiguousarray(np.asarray(u, dtype=np.float64).reshape(-1))
+        r = None if rho is None else np.ascontiguousarray(np.asarray(rho, dtype=np.float64).reshape(-1))
+        self._check(self._L.lbm_set_coupling_data(self._h, int(mode), _dp(u), None if r is None else _dp(r), float(dt)),
+                    "lbm_set_coupling_data")
+
+    def get_coupling_data(self):
+        n = getattr(self, "_ncouple", 0)
+        rho, u = np.empty(n), np.empty((n, 3))
+        self._check(self._L.lbm_get_coupling_data(self._h, _dp(u.reshape(-1)), _dp(rho)), "lbm_get_coupling_data")
+        return rho, u
 
     def set_moving_wall_class(self, k, uw, mask):
         uw = np.ascontiguousarray(uw, dtype=np.float64)

@@ -27,6 +27,7 @@ MAX_CLASSES = 32
 # boundary-condition types of a class (lbm_oracle.h)
 BC_MOVING_WALL, BC_PRESSURE_OUTLET, BC_PRESSURE_INLET, BC_VELOCITY_INLET = 0, 1, 2, 3
 BC_CELL_VELOCITY = 16
+COUPLE_OVERWRITE, COUPLE_DIRICHLET = 0, 1
 CS2 = 1.0 / 3.0
 
 # LB/LatticeBoltzmannDefinitions.cpp:113-131 (z-major, then y, then x)
@@ -55,7 +56,12 @@ class Params(ctypes.Structure):
                 ("tau_cell", ctypes.POINTER(ctypes.c_double)),
                 ("bc_type", ctypes.c_int * MAX_CLASSES),
                 ("bc_rho", ctypes.c_double * MAX_CLASSES),
-                ("bc_u", ctypes.POINTER(ctypes.c_double))]
+                ("bc_u", ctypes.POINTER(ctypes.c_double)),
+                ("couple_row", ctypes.POINTER(ctypes.c_int64)),
+                ("couple_mode", ctypes.c_int),
+                ("couple_u", ctypes.POINTER(ctypes.c_double)),
+                ("couple_rho", ctypes.POINTER(ctypes.c_double)),
+                ("couple_dt", ctypes.c_double)]
 
 
 _lib = None
@@ -90,6 +96,7 @@ def lib():
         L.lbo_boundary_value.restype = ctypes.c_double
         L.lbo_boundary_value.argtypes = [ctypes.POINTER(Params), ctypes.c_uint, ctypes.c_int64, ctypes.c_int,
                                          ctypes.c_double, ctypes.c_double, dp]
+        L.lbo_couple_gi.argtypes = [ctypes.c_double, ctypes.c_double, dp, dp, dp]
         L.lbo_build_words.argtypes = [ctypes.POINTER(Dims), ctypes.POINTER(ctypes.c_uint8), up]
         L.lbo_step.argtypes = [ctypes.POINTER(Dims), up, ctypes.POINTER(Params), dp, dp, ctypes.c_int]
         L.lbo_stream.argtypes = [ctypes.POINTER(Dims), up, dp, dp]
@@ -149,6 +156,25 @@ class Lattice:
             raise ValueError("not a channel (nx < 2 or more than 31 boundary classes)")
         lat.params.tau = tau
         return lat
+
+    def set_coupling(self, cells, mode, u, rho=None, dt=1.0):
+        """MD coupling at the listed cells: mode COUPLE_OVERWRITE or COUPLE_DIRICHLET, MD velocity
+        u[k] (lattice units) and optional density rho[k] for cells[k].  cells=None clears."""
+        if cells is None:
+            self.params.couple_row = None
+            self._couple = None
+            return
+        cells = np.asarray(cells, dtype=np.int64).reshape(-1)
+        row = np.full(self.n, -1, dtype=np.int64)
+        row[cells] = np.arange(cells.size)
+        u = np.ascontiguousarray(np.asarray(u, dtype=np.float64).reshape(-1))
+        r = None if rho is None else np.ascontiguousarray(np.asarray(rho, dtype=np.float64).reshape(-1))
+        self._couple = (row, u, r)
+        self.params.couple_row = row.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))
+        self.params.couple_mode = int(mode)
+        self.params.couple_u = _dp(u)
+        self.params.couple_rho = None if r is None else _dp(r)
+        self.params.couple_dt = float(dt)
 
     def set_solid(self, solid):
         solid = np.ascontiguousarray(solid.reshape(-1), dtype=np.uint8)

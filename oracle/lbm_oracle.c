@@ -151,6 +151,51 @@ static double collide_cell(double* f, double tau_eff, const lbo_params* p, doubl
   return rho;
 }
 
+/* computeGi (madpac/peanocoupling/PeanoDataExchangeService.cpp:161-186), as written there:
+ * g_i = (1 - 0.5/tau) w_i rho [ (c_i - u).g / cs2 + (c_i.u)^2 (c_i.g) / cs2^2 ].  The squared
+ * c.u differs from Guo's scheme and makes sum_i g_i = -(1 - 0.5/tau) rho u.g / cs2 (not 0);
+ * it is kept because this is the coupling the reference runs. */
+void lbo_couple_gi(double rho, double tau, const double* u, const double* g, double* gi) {
+  for (int q = 0; q < LBO_Q; q++) {
+    double c[3] = {cd(q, 0), cd(q, 1), cd(q, 2)};
+    double cmu[3] = {c[0] - u[0], c[1] - u[1], c[2] - u[2]};
+    double cu = dot3(c, u);
+    gi[q] = (1.0 - 0.5 / tau) * LBO_W[q] * rho *
+            (1 / (LBO_CS2) * dot3(cmu, g) + (cu * cu) / (LBO_CS2 * LBO_CS2) * dot3(c, g));
+  }
+}
+
+/* Collision of a coupled cell (getAndSetV :69-158 before BlockBGK, adjustF :189-218 after it):
+ * overwrite: the equilibrium velocity is the MD velocity; Dirichlet: g = (u_MD - u)/dt,
+ * u <- u + dt/2 g, then f += dt g_i after the BGK relaxation.  With density coupling rho_MD
+ * replaces the density of the equilibrium (and of g_i).  The reference's adjustF applies its
+ * term to cell index `_level` of each block only; here it applies to every coupled cell. */
+static double collide_coupled(double* f, double tau_eff, const lbo_params* p, int64_t row, double* uout) {
+  double rho = lbo_density(f);
+  double u[3], g[3] = {0.0, 0.0, 0.0};
+  lbo_velocity(f, rho, u);
+  const double* um = p->couple_u + 3 * row;
+  if (p->couple_mode == LBO_COUPLE_DIRICHLET) {
+    for (int d = 0; d < 3; d++) {
+      g[d] = (um[d] - u[d]) / p->couple_dt;
+      u[d] += p->couple_dt / 2.0 * g[d];
+    }
+  } else {
+    for (int d = 0; d < 3; d++) u[d] = um[d];
+  }
+  if (p->couple_rho) rho = p->couple_rho[row];
+  for (int d = 0; d < 3; d++) uout[d] = u[d];
+  double feq[LBO_Q];
+  lbo_feq(rho, u, feq);
+  lbo_bgk(f, tau_eff, feq);
+  if (p->couple_mode == LBO_COUPLE_DIRICHLET) {
+    double gi[LBO_Q];
+    lbo_couple_gi(rho, p->tau, u, g, gi);
+    for (int q = 0; q < LBO_Q; q++) f[q] += p->couple_dt * gi[q];
+  }
+  return rho;
+}
+
 static int64_t ncells(const lbo_dims* d) { return (int64_t)d->nx * d->ny * d->nz; }
 
 /* neighbour of (x,y,z) along c_q, or -1 if it leaves a non-periodic domain */
@@ -293,7 +338,9 @@ void lbo_step(const lbo_dims* d, const uint32_t* words, const lbo_params* p,
         for (int q = 0; q < LBO_Q; q++) f[q] = src[(int64_t)q * N + i];
         double alpha = p->alpha ? p->alpha[i] : 0.0;
         double uc[3];
-        double rho = collide_cell(f, p->tau_cell ? lbo_tau_eff(p->tau_cell[i]) : tau_eff, p, alpha, uc);
+        const double te = p->tau_cell ? lbo_tau_eff(p->tau_cell[i]) : tau_eff;
+        const int64_t row = p->couple_row ? p->couple_row[i] : -1;
+        double rho = row >= 0 ? collide_coupled(f, te, p, row, uc) : collide_cell(f, te, p, alpha, uc);
         unsigned cls = (w >> LBO_CLASS_SHIFT) & LBO_CLASS_MASK;
         for (int q = 0; q < LBO_Q; q++) {
           if (w & (1u << q)) {
@@ -338,7 +385,9 @@ void lbo_step_swap_aos(const lbo_dims* d, const uint32_t* words, const lbo_param
     double* fi = f + i * LBO_Q;
     double alpha = p->alpha ? p->alpha[i] : 0.0;
     double uc[3];
-    double rho = collide_cell(fi, p->tau_cell ? lbo_tau_eff(p->tau_cell[i]) : tau_eff, p, alpha, uc);
+    const double te = p->tau_cell ? lbo_tau_eff(p->tau_cell[i]) : tau_eff;
+    const int64_t row = p->couple_row ? p->couple_row[i] : -1;
+    double rho = row >= 0 ? collide_coupled(fi, te, p, row, uc) : collide_cell(fi, te, p, alpha, uc);
     unsigned cls = (w >> LBO_CLASS_SHIFT) & LBO_CLASS_MASK;
     if (cls)
       for (int q = 0; q < LBO_Q; q++)

@@ -52,6 +52,11 @@ extern "C" {
 #define LBO_BC_TYPE_MASK 15
 #define LBO_BC_CELL_VELOCITY 16
 
+/* MD coupling modes (madpac/peanocoupling/PeanoDataExchangeService.cpp:69-158): the MD velocity
+ * replaces the LB velocity, or drives a Dirichlet-type forcing toward it. */
+#define LBO_COUPLE_OVERWRITE 0
+#define LBO_COUPLE_DIRICHLET 1
+
 extern const int LBO_C[LBO_Q][3];
 extern const double LBO_W[LBO_Q];
 
@@ -71,6 +76,11 @@ typedef struct {
   int bc_type[LBO_MAX_CLASSES];            /* LBO_BC_* per class */
   double bc_rho[LBO_MAX_CLASSES];          /* per class: target density rho_L */
   const double* bc_u;                      /* per-cell target velocity [3*N] for LBO_BC_CELL_VELOCITY classes */
+  const int64_t* couple_row;               /* per cell: row of the coupling data, -1 = not coupled; NULL = none */
+  int couple_mode;                         /* LBO_COUPLE_* */
+  const double* couple_u;                  /* [3*rows] MD velocity in lattice units */
+  const double* couple_rho;                /* [rows] MD density, or NULL (no density coupling) */
+  double couple_dt;                        /* LB time step of the Dirichlet forcing */
 } lbo_params;
 
 /* a1/a8 */
@@ -87,6 +97,9 @@ double lbo_moving_wall_term(const double* uw, int q, double rho);
  * at cell i, given the post-collision population fq and the cell's rho and velocity u. */
 double lbo_boundary_value(const lbo_params* p, unsigned k, int64_t i, int q, double fq, double rho,
                           const double* u);
+
+/* the reference's coupling force of one cell (PeanoDataExchangeService.cpp:161-186) */
+void lbo_couple_gi(double rho, double tau, const double* u, const double* g, double* gi);
 
 /* geometry: lid-driven cavity of n^3 fluid cells, lid on +z moving with (ulid,0,0). */
 void lbo_build_cavity(int n, uint32_t* words, lbo_params* p, double ulid);

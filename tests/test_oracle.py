@@ -255,6 +255,90 @@ def test_channel_reaches_uniform_plane_flux():
     assert flux.mean() > 0
 
 
+# ---------------------------------------------------------------- MD coupling (madpac/peanocoupling)
+def test_coupling_force_moments_closed_form():
+    # With the D3Q19 moments sum w c c = cs2 I and sum w c_a c_b c_c c_d = cs2^2 (dd + dd + dd):
+    # sum_i g_i c_i = P [g + 2 u (u.g) + g |u|^2] and sum_i g_i = -P u.g / cs2, P = (1 - 1/(2 tau)) rho.
+    # Guo's scheme, with (c.u) in place of (c.u)^2, would give P g and 0.
+    rng = np.random.default_rng(11)
+    L = O.lib()
+    for _ in range(10):
+        rho, tau = 0.9 + 0.2 * rng.random(), 0.55 + rng.random()
+        u, g = 0.05 * rng.standard_normal(3), 1e-3 * rng.standard_normal(3)
+        gi = np.empty(19)
+        L.lbo_couple_gi(rho, tau, _p(np.ascontiguousarray(u)), _p(np.ascontiguousarray(g)), _p(gi))
+        pref = (1 - 0.5 / tau) * rho
+        assert np.allclose(gi @ O.C, pref * (g + 2 * u * (u @ g) + g * (u @ u)), rtol=1e-11, atol=1e-19)
+        assert np.isclose(gi.sum(), -pref * (u @ g) / O.CS2, rtol=1e-10, atol=1e-19)
+
+
+def _periodic_random(n, seed):
+    lat = O.Lattice(n, n, n, periodic=(True, True, True))
+    lat.set_solid(np.zeros(n ** 3, np.uint8))
+    rng = np.random.default_rng(seed)
+    f0 = lat.equilibrium(rho=1 + 0.02 * rng.standard_normal(n ** 3), u=0.02 * rng.standard_normal((n ** 3, 3)))
+    f0 = f0 * (1 + 0.01 * rng.standard_normal(f0.size))  # off equilibrium
+    return lat, f0, rng
+
+
+def _gather_outgoing(f, n, x):
+    # the populations that left cell x in one push-form step sit at x + c_q
+    out = np.empty(19)
+    z, y, xx = np.unravel_index(x, (n, n, n))
+    F = f.reshape(19, n, n, n)
+    for q in range(19):
+        c = O.C[q]
+        out[q] = F[q, (z + c[2]) % n, (y + c[1]) % n, (xx + c[0]) % n]
+    return out
+
+
+@pytest.mark.parametrize("density", [False, True])
+def test_coupling_overwrite_relaxes_to_md_equilibrium(density):
+    # tau_eff = 1: a coupled cell's post-collision populations are feq(rho, u_MD) (rho_MD with
+    # density coupling); uncoupled cells are untouched by the coupling
+    n = 6
+    lat, f0, rng = _periodic_random(n, 3)
+    lat.params.tau = 1.0
+    cells = np.array([7, 50, 131])
+    umd = 0.03 * rng.standard_normal((3, 3))
+    rmd = 1 + 0.01 * rng.standard_normal(3) if density else None
+    plain = lat.step(f0, 1)
+    lat.set_coupling(cells, O.COUPLE_OVERWRITE, umd, rmd)
+    f1 = lat.step(f0, 1)
+    rho0, _ = lat.fields(f0)
+    for k, x in enumerate(cells):
+        feq = np.empty(19)
+        r = rmd[k] if density else rho0[x]
+        O.lib().lbo_feq(float(r), _p(np.ascontiguousarray(umd[k])), _p(feq))
+        assert np.allclose(_gather_outgoing(f1, n, x), feq, rtol=2e-15, atol=0)
+    moved = np.zeros(19 * n ** 3, bool).reshape(19, n, n, n)
+    for x in cells:
+        z, y, xx = np.unravel_index(x, (n, n, n))
+        for q in range(19):
+            c = O.C[q]
+            moved[q, (z + c[2]) % n, (y + c[1]) % n, (xx + c[0]) % n] = True
+    assert np.array_equal(f1[~moved.reshape(-1)], plain[~moved.reshape(-1)])
+
+
+def test_coupling_dirichlet_drives_momentum_to_md_velocity():
+    # tau_eff = 1, dt = 1: the relaxed momentum rho v (v = u + g/2) plus sum g_i c_i with
+    # P = rho/2 gives rho u_MD + (rho/2) (2 v (v.g) + g |v|^2), g = u_MD - u
+    n = 6
+    lat, f0, rng = _periodic_random(n, 5)
+    lat.params.tau = 1.0
+    cells = np.array([3, 77, 200])
+    umd = 0.03 * rng.standard_normal((3, 3))
+    lat.set_coupling(cells, O.COUPLE_DIRICHLET, umd, None, dt=1.0)
+    f1 = lat.step(f0, 1)
+    rho0, u0 = lat.fields(f0)
+    for k, x in enumerate(cells):
+        out = _gather_outgoing(f1, n, x)
+        g = umd[k] - u0[x]
+        v = u0[x] + g / 2
+        expect = rho0[x] * umd[k] + rho0[x] / 2 * (2 * v * (v @ g) + g * (v @ v))
+        assert np.allclose(out @ O.C, expect, rtol=1e-12, atol=1e-17)
+
+
 # ---------------------------------------------------------------- invariants
 def test_rest_state_stays_at_rest():
     # sum(w) is 1 only to round-off, so rest is a fixed point to round-off, not bitwise
