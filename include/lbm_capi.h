@@ -39,8 +39,11 @@ enum lbm_status {
 };
 
 enum lbm_precision {
-  LBM_FP64 = 0,          /* FP64 populations, FP64 collision (reference arithmetic order) */
-  LBM_FP32_SHIFTED = 1   /* FP32 populations stored as f - w_q, FP32 collision */
+  LBM_FP64 = 0,             /* FP64 populations, FP64 collision (reference arithmetic order) */
+  LBM_FP32_SHIFTED = 1,     /* FP32 populations stored as f - w_q (FP32 state and traffic); the
+                               collision arithmetic runs in FP64 registers and rounds once at the store */
+  LBM_FP32_SHIFTED_F32 = 2  /* FP32 populations as above with FP32 collision arithmetic: persistent
+                               rounding of the equilibrium biases the momentum at low Mach */
 };
 
 typedef struct lbm_ctx lbm_ctx;

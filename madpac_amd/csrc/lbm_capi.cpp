@@ -89,7 +89,7 @@ int lbm_create(lbm_ctx** out, const lbm_config* cfg) {
   if (cfg->nx < 1 || cfg->ny < 1 || cfg->nz < 1) return fail(nullptr, LBM_ERR_ARG, "lattice extent must be >= 1");
   if ((int64_t)cfg->nx * cfg->ny * cfg->nz > (int64_t)1 << 33)
     return fail(nullptr, LBM_ERR_ARG, "lattice too large");
-  if (cfg->precision != LBM_FP64 && cfg->precision != LBM_FP32_SHIFTED)
+  if (cfg->precision != LBM_FP64 && cfg->precision != LBM_FP32_SHIFTED && cfg->precision != LBM_FP32_SHIFTED_F32)
     return fail(nullptr, LBM_ERR_ARG, "unknown precision");
   if (!(cfg->tau > 0.5) || !(cfg->tau < 1e6)) return fail(nullptr, LBM_ERR_ARG, "tau must be > 0.5");
   for (int k = 0; k < 3; k++)
@@ -100,7 +100,8 @@ int lbm_create(lbm_ctx** out, const lbm_config* cfg) {
 
   lbm_ctx* c = new lbm_ctx();
   c->cfg = *cfg;
-  c->fp32 = cfg->precision == LBM_FP32_SHIFTED;
+  c->fp32 = cfg->precision != LBM_FP64;
+  c->f32math = cfg->precision == LBM_FP32_SHIFTED_F32;
   c->esize = c->fp32 ? sizeof(float) : sizeof(double);
   c->g = lbm::make_geom(cfg->nx, cfg->ny, cfg->nz, cfg->periodic);
   c->force = cfg->force[0] != 0.0 || cfg->force[1] != 0.0 || cfg->force[2] != 0.0;
@@ -390,16 +391,16 @@ int lbm_get_populations(lbm_ctx* c, double* f) {
 }  // extern "C"
 
 int lbm_enqueue_step_from(lbm_ctx* c, const void* src, void* dst) {
-  HIPCHK(c, lbm::launch_dense_step(c->g, c->fp32, src, dst, c->words, c->alpha, c->force, c->walls, c->p64, c->p32,
-                                   lbm::BoundaryArgs{c->d_bct, c->d_bcu}, c->spec, c->stream));
+  HIPCHK(c, lbm::launch_dense_step(c->g, c->fp32, c->f32math, src, dst, c->words, c->alpha, c->force, c->walls,
+                                   c->p64, c->p32, lbm::BoundaryArgs{c->d_bct, c->d_bcu}, c->spec, c->stream));
   return LBM_OK;
 }
 
 int lbm_enqueue_steps(lbm_ctx* c, int nsteps) {
   for (int s = 0; s < nsteps; s++) {
-    HIPCHK(c, lbm::launch_dense_step(c->g, c->fp32, c->buf[c->cur], c->buf[c->cur ^ 1], c->words, c->alpha, c->force,
-                                     c->walls, c->p64, c->p32, lbm::BoundaryArgs{c->d_bct, c->d_bcu},
-                                     c->spec, c->stream));
+    HIPCHK(c, lbm::launch_dense_step(c->g, c->fp32, c->f32math, c->buf[c->cur], c->buf[c->cur ^ 1], c->words,
+                                     c->alpha, c->force, c->walls, c->p64, c->p32,
+                                     lbm::BoundaryArgs{c->d_bct, c->d_bcu}, c->spec, c->stream));
     const int rc = lbm_enqueue_coupling(c, c->buf[c->cur], c->buf[c->cur ^ 1]);
     if (rc) return rc;
     c->cur ^= 1;

@@ -213,6 +213,60 @@ __device__ __forceinline__ float collide_f32s(float (&g)[Q], const CollideParams
   return rho;
 }
 
+// FP32-shifted storage with FP64 arithmetic: the shifted populations are widened, the collision runs
+// in double (rho = 1 + drho is never rounded to FP32) and each population is rounded once when
+// stored.  FP32 arithmetic repeats the same rounding of the equilibrium step after step while the
+// state changes slowly, which accumulates into a momentum bias of ~1e-8 of the momentum per step at
+// low Mach; here the only rounding is the store's.  Outputs as collide_f32s.
+template <bool FORCE, bool ALPHA>
+__device__ __forceinline__ float collide_f32s_x(float (&gs)[Q], const CollideParams& p, float alpha,
+                                                float (&uo)[4]) {
+  double drho = 0.0, j0 = 0.0, j1 = 0.0, j2 = 0.0;
+#pragma unroll
+  for (int q = 0; q < Q; q++) {
+    const double g = gs[q];
+    drho += g;
+    if (CX(q) == 1) j0 += g;
+    if (CX(q) == -1) j0 -= g;
+    if (CY(q) == 1) j1 += g;
+    if (CY(q) == -1) j1 -= g;
+    if (CZ(q) == 1) j2 += g;
+    if (CZ(q) == -1) j2 -= g;
+  }
+  const double rho = 1.0 + drho;
+  const double inv = 1.0 / rho;
+  double u0 = j0 * inv, u1 = j1 * inv, u2 = j2 * inv;
+  if (FORCE) {
+    u0 += 0.5 * p.g[0];
+    u1 += 0.5 * p.g[1];
+    u2 += 0.5 * p.g[2];
+  }
+  uo[0] = (float)u0;
+  uo[1] = (float)u1;
+  uo[2] = (float)u2;
+  uo[3] = (float)drho;
+  if (ALPHA) {
+    const double s = 1.0 - (double)alpha;
+    u0 *= s; u1 *= s; u2 *= s;
+  }
+  const double uu15 = 1.5 * (u0 * u0 + u1 * u1 + u2 * u2);
+  const double ug = FORCE ? u0 * p.g[0] + u1 * p.g[1] + u2 * p.g[2] : 0.0;
+  const double fpref = FORCE ? (1.0 - 0.5 * p.omega) * rho : 0.0;
+#pragma unroll
+  for (int q = 0; q < Q; q++) {
+    const double cu = cdot64(q, u0, u1, u2);
+    const double geq = W64(q) * (drho + rho * (3.0 * cu + 4.5 * cu * cu - uu15));
+    const double g = gs[q];
+    double v = g - p.omega * (g - geq);
+    if (FORCE) {
+      const double cg = cdot64(q, p.g[0], p.g[1], p.g[2]);
+      v += fpref * W64(q) * (3.0 * (cg - ug) + 9.0 * cu * cg);
+    }
+    gs[q] = (float)v;
+  }
+  return (float)rho;
+}
+
 __device__ __forceinline__ float moving_wall_f32(int q, const double* uw, float rho) {
   const float cu = (float)CX(q) * (float)uw[0] + (float)CY(q) * (float)uw[1] + (float)CZ(q) * (float)uw[2];
   return 6.0f * W32(q) * rho * cu;

@@ -19,7 +19,8 @@ struct lbm_ctx {
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   std::string err;
-  bool fp32 = false;
+  bool fp32 = false;                   // FP32-shifted storage
+  bool f32math = false;                // ... with FP32 collision arithmetic (LBM_FP32_SHIFTED_F32)
   size_t esize = 8;
   void* buf[2] = {nullptr, nullptr};  // populations (double or shifted float), ping-pong
   int cur = 0;                         // buffer holding the current stored state

@@ -70,7 +70,7 @@ __global__ __launch_bounds__(BX * BY) void k_build_words(DenseGeom g, const uint
 // dependent memory round trips; solid lanes load nothing).  SPEC = true: the 19 neighbour loads
 // issue together with the word, and the few bounce-back populations are re-loaded from the cell's
 // own reflected slots once the word has arrived (one round trip on most lanes; solid lanes load).
-template <typename T, bool FORCE, bool ALPHA, bool WALLS, bool SPEC>
+template <typename T, bool FORCE, bool ALPHA, bool WALLS, bool SPEC, bool M32 = false>
 __global__ __launch_bounds__(BX * BY) void k_dense_step(DenseGeom g, const T* __restrict__ src,
                                                         T* __restrict__ dst, const uint32_t* __restrict__ words,
                                                         const T* __restrict__ alpha, CollideParams p64,
@@ -102,8 +102,10 @@ __global__ __launch_bounds__(BX * BY) void k_dense_step(DenseGeom g, const T* __
   T uc[sizeof(T) == 8 ? 3 : 4];
   if constexpr (sizeof(T) == 8) {
     rho = collide_f64<FORCE, ALPHA>(f, p64, a, uc);
-  } else {
+  } else if constexpr (M32) {
     rho = collide_f32s<FORCE, ALPHA>(f, p32, a, uc);
+  } else {
+    rho = collide_f32s_x<FORCE, ALPHA>(f, p64, a, uc);
   }
   if (WALLS) {
     const uint32_t k = (w >> CLASS_SHIFT) & CLASS_BITS;
@@ -395,7 +397,7 @@ hipError_t launch_build_words(const DenseGeom& g, const uint8_t* inner, const ui
   return hipGetLastError();
 }
 
-template <typename T, bool SPEC>
+template <typename T, bool SPEC, bool M32>
 static hipError_t step_t(const DenseGeom& g, const void* src, void* dst, const uint32_t* words, const void* alpha,
                          bool force, bool walls, const CollideParams& p64, const CollideParams32& p32,
                          const BoundaryArgs& bc, hipStream_t s) {
@@ -403,7 +405,7 @@ static hipError_t step_t(const DenseGeom& g, const void* src, void* dst, const u
   const T* sp = (const T*)src;
   T* dp = (T*)dst;
 #define LBM_STEP(F, A, W)                                                                                        \
-  hipLaunchKernelGGL((k_dense_step<T, F, A, W, SPEC>), dense_grid(g), dense_block(), 0, s, g, sp, dp, words, a, \
+  hipLaunchKernelGGL((k_dense_step<T, F, A, W, SPEC, M32>), dense_grid(g), dense_block(), 0, s, g, sp, dp, words, a, \
                      p64, p32, bc)
   const int key = (force ? 4 : 0) | (alpha ? 2 : 0) | (walls ? 1 : 0);
   switch (key) {
@@ -420,14 +422,18 @@ static hipError_t step_t(const DenseGeom& g, const void* src, void* dst, const u
   return hipGetLastError();
 }
 
-hipError_t launch_dense_step(const DenseGeom& g, bool fp32, const void* src, void* dst, const uint32_t* words,
-                             const void* alpha, bool force, bool walls, const CollideParams& p64,
-                             const CollideParams32& p32, const BoundaryArgs& bc, bool spec, hipStream_t s) {
+hipError_t launch_dense_step(const DenseGeom& g, bool fp32, bool f32math, const void* src, void* dst,
+                             const uint32_t* words, const void* alpha, bool force, bool walls,
+                             const CollideParams& p64, const CollideParams32& p32, const BoundaryArgs& bc, bool spec,
+                             hipStream_t s) {
+  if (fp32 && f32math)
+    return spec ? step_t<float, true, true>(g, src, dst, words, alpha, force, walls, p64, p32, bc, s)
+                : step_t<float, false, true>(g, src, dst, words, alpha, force, walls, p64, p32, bc, s);
   if (fp32)
-    return spec ? step_t<float, true>(g, src, dst, words, alpha, force, walls, p64, p32, bc, s)
-                : step_t<float, false>(g, src, dst, words, alpha, force, walls, p64, p32, bc, s);
-  return spec ? step_t<double, true>(g, src, dst, words, alpha, force, walls, p64, p32, bc, s)
-              : step_t<double, false>(g, src, dst, words, alpha, force, walls, p64, p32, bc, s);
+    return spec ? step_t<float, true, false>(g, src, dst, words, alpha, force, walls, p64, p32, bc, s)
+                : step_t<float, false, false>(g, src, dst, words, alpha, force, walls, p64, p32, bc, s);
+  return spec ? step_t<double, true, false>(g, src, dst, words, alpha, force, walls, p64, p32, bc, s)
+              : step_t<double, false, false>(g, src, dst, words, alpha, force, walls, p64, p32, bc, s);
 }
 
 hipError_t launch_dense_pull(const DenseGeom& g, bool fp32, const void* src, double* f, const uint32_t* words,

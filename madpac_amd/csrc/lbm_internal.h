@@ -46,7 +46,9 @@ struct BoundaryArgs {
   const BoundaryClass* table;
   const double* cell_u;
 };
-hipError_t launch_dense_step(const DenseGeom& g, bool fp32, const void* src, void* dst, const uint32_t* words,
+// fp32: FP32-shifted storage; f32math: FP32 collision arithmetic (else FP64 arithmetic on the FP32 state)
+hipError_t launch_dense_step(const DenseGeom& g, bool fp32, bool f32math, const void* src, void* dst,
+                             const uint32_t* words,
                              const void* alpha, bool force, bool walls, const CollideParams& p64,
                              const CollideParams32& p32, const BoundaryArgs& bc, bool spec, hipStream_t stream);
 // reference state f (FP64) from the stored state

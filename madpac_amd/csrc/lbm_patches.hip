@@ -66,20 +66,60 @@ __host__ __device__ constexpr int dir_of(int ex, int ey, int ez) {
   return -1;
 }
 
-template <typename T>
+// M32: FP32 arithmetic for FP32 storage (else FP64 arithmetic, rounded once at the store)
+template <typename T, bool M32>
 __device__ __forceinline__ void collide_patch(T (&f)[Q], double omega) {
   if constexpr (sizeof(T) == 8) {
     CollideParams p{1.0 / omega, omega, {0.0, 0.0, 0.0}};
     double u[3];
     collide_f64<false, false>(f, p, 0.0, u);
-  } else {
+  } else if constexpr (M32) {
     CollideParams32 p{(float)omega, 0.f, {0.f, 0.f, 0.f}};
     float u[4];
     collide_f32s<false, false>(f, p, 0.f, u);
+  } else {
+    CollideParams p{1.0 / omega, omega, {0.0, 0.0, 0.0}};
+    float u[4];
+    collide_f32s_x<false, false>(f, p, 0.f, u);
   }
 }
 
-template <typename T, int TX, int TY, int TZ>
+// One interior cell (x, y, z) of a patch: pull (ghost layer included), collide, store the own slots,
+// and write the populations that leave the patch into the neighbours' ghost layers.
+template <typename T, bool M32>
+__device__ __forceinline__ void patch_cell(const PatchDev* __restrict__ pp, const T* __restrict__ s,
+                                           T* __restrict__ d, T* __restrict__ dst, int n, int np, int64_t P,
+                                           double omega, int x, int y, int z) {
+  const int64_t i = (int64_t)(x + 1) + (int64_t)np * ((y + 1) + (int64_t)np * (z + 1));
+  T f[Q];
+#pragma unroll
+  for (int q = 0; q < Q; q++) {
+    const int64_t off = (int64_t)CX(q) + (int64_t)np * (CY(q) + (int64_t)np * CZ(q));
+    f[q] = s[(int64_t)q * P + i - off];
+  }
+  collide_patch<T, M32>(f, omega);
+#pragma unroll
+  for (int q = 0; q < Q; q++) __builtin_nontemporal_store(f[q], &d[(int64_t)q * P + i]);
+  // ghost-layer exchange: populations leaving the patch go to the neighbour's ghost layer
+  if (x == 0 || y == 0 || z == 0 || x == n - 1 || y == n - 1 || z == n - 1) {
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+      if (q == REST) continue;
+      const int tx = x + CX(q), ty = y + CY(q), tz = z + CZ(q);
+      const int ex = tx < 0 ? -1 : (tx >= n ? 1 : 0);
+      const int ey = ty < 0 ? -1 : (ty >= n ? 1 : 0);
+      const int ez = tz < 0 ? -1 : (tz >= n ? 1 : 0);
+      if ((ex | ey | ez) == 0) continue;
+      const int k = dir_index(ex, ey, ez);  // a face or edge direction: always a D3Q19 direction
+      const int gx = x - n * ex, gy = y - n * ey, gz = z - n * ez;  // ghost position in the neighbour
+      const int64_t gi = (int64_t)(gx + 1) + (int64_t)np * ((gy + 1) + (int64_t)np * (gz + 1));
+      dst[pp->nbase[k] + (int64_t)q * P + gi] = f[q];
+    }
+  }
+}
+
+// Box tiles of TX x TY x TZ cells (cells outside the patch idle).
+template <typename T, int TX, int TY, int TZ, bool M32>
 __global__ __launch_bounds__(256) void k_patch_step(const PatchDev* __restrict__ patches,
                                                     const TileDev* __restrict__ tiles, int ntiles,
                                                     const T* __restrict__ src, T* __restrict__ dst) {
@@ -94,41 +134,40 @@ __global__ __launch_bounds__(256) void k_patch_step(const PatchDev* __restrict__
     if (tn < ntiles) next = tiles[tn];
     // per-lane lookups go through the global table (a private copy would live in scratch memory)
     const PatchDev* __restrict__ pp = patches + tl.patch;
-    const int64_t base = tl.base;
     const int n = tl.n, np = tl.np;
-    const double omega = tl.omega;
     const int64_t P = (int64_t)np * np * np;
-    const T* __restrict__ s = src + base;
-    T* __restrict__ d = dst + base;
     for (int c = threadIdx.x; c < TC; c += 256) {
       const int x = tl.x0 + c % TX, y = tl.y0 + (c / TX) % TY, z = tl.z0 + c / (TX * TY);
       if (x >= n || y >= n || z >= n) continue;
-      const int64_t i = (int64_t)(x + 1) + (int64_t)np * ((y + 1) + (int64_t)np * (z + 1));
-      T f[Q];
-#pragma unroll
-      for (int q = 0; q < Q; q++) {
-        const int64_t off = (int64_t)CX(q) + (int64_t)np * (CY(q) + (int64_t)np * CZ(q));
-        f[q] = s[(int64_t)q * P + i - off];
-      }
-      collide_patch<T>(f, omega);
-#pragma unroll
-      for (int q = 0; q < Q; q++) __builtin_nontemporal_store(f[q], &d[(int64_t)q * P + i]);
-      // ghost-layer exchange: populations leaving the patch go to the neighbour's ghost layer
-      if (x == 0 || y == 0 || z == 0 || x == n - 1 || y == n - 1 || z == n - 1) {
-#pragma unroll
-        for (int q = 0; q < Q; q++) {
-          if (q == REST) continue;
-          const int tx = x + CX(q), ty = y + CY(q), tz = z + CZ(q);
-          const int ex = tx < 0 ? -1 : (tx >= n ? 1 : 0);
-          const int ey = ty < 0 ? -1 : (ty >= n ? 1 : 0);
-          const int ez = tz < 0 ? -1 : (tz >= n ? 1 : 0);
-          if ((ex | ey | ez) == 0) continue;
-          const int k = dir_index(ex, ey, ez);  // a face or edge direction: always a D3Q19 direction
-          const int gx = x - n * ex, gy = y - n * ey, gz = z - n * ez;  // ghost position in the neighbour
-          const int64_t gi = (int64_t)(gx + 1) + (int64_t)np * ((gy + 1) + (int64_t)np * (gz + 1));
-          dst[pp->nbase[k] + (int64_t)q * P + gi] = f[q];
-        }
-      }
+      patch_cell<T, M32>(pp, src + tl.base, dst + tl.base, dst, n, np, P, tl.omega, x, y, z);
+    }
+    if (tn >= ntiles) break;
+    t = tn;
+    tl = next;
+  }
+}
+
+// Linear tiles: 256 consecutive interior cells of one patch in x-fastest order, one per thread, so
+// no lane idles whatever the edge (tile.x0 holds the first cell's linear index).
+template <typename T, bool M32>
+__global__ __launch_bounds__(256) void k_patch_step_lin(const PatchDev* __restrict__ patches,
+                                                        const TileDev* __restrict__ tiles, int ntiles,
+                                                        const T* __restrict__ src, T* __restrict__ dst) {
+  int t = blockIdx.x;
+  if (t >= ntiles) return;
+  TileDev tl = tiles[t];
+  for (;;) {
+    const int tn = t + gridDim.x;
+    TileDev next{};
+    if (tn < ntiles) next = tiles[tn];
+    const PatchDev* __restrict__ pp = patches + tl.patch;
+    const int n = tl.n, np = tl.np;
+    const int64_t P = (int64_t)np * np * np;
+    const int c = tl.x0 + (int)threadIdx.x;
+    if (c < n * n * n) {
+      const int z = c / (n * n), r = c - z * n * n;
+      const int y = r / n, x = r - y * n;
+      patch_cell<T, M32>(pp, src + tl.base, dst + tl.base, dst, n, np, P, tl.omega, x, y, z);
     }
     if (tn >= ntiles) break;
     t = tn;
@@ -254,7 +293,10 @@ namespace {
 struct Shape {
   int x, y, z;
 };
-const Shape kShapes[] = {{8, 8, 4}, {8, 8, 8}, {16, 8, 2}, {16, 16, 1}, {16, 16, 2}, {32, 8, 1}, {32, 8, 2}, {16, 4, 4}};
+// box tiles, and {256, 0, 0}: linear runs of 256 consecutive cells of a patch
+const Shape kShapes[] = {{8, 8, 4},  {8, 8, 8},  {16, 8, 2}, {16, 16, 1}, {16, 16, 2},
+                         {32, 8, 1}, {32, 8, 2}, {16, 4, 4}, {256, 0, 0}};
+bool linear_tiles(const lbm_tiling& t) { return t.tile_y == 0; }
 
 double tau_eff_of(double tau) {
   double nu = 1.0 / 3.0 * (tau - 0.5);
@@ -266,6 +308,7 @@ double tau_eff_of(double tau) {
 struct lbm_patchset {
   int device = 0;
   bool fp32 = false;
+  bool f32math = false;
   size_t esize = 8;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -320,6 +363,11 @@ int build_tiles(lbm_patchset* ps, const lbm_tiling& t) {
   std::vector<TileDev> tiles;
   for (int p : order) {
     const int n = ps->host[p].n;
+    if (linear_tiles(t)) {
+      const PatchDev& d = ps->host[p];
+      for (int c = 0; c < n * n * n; c += t.tile_x) tiles.push_back(TileDev{d.base, d.omega, p, d.n, d.np, c, 0, 0});
+      continue;
+    }
     for (int z = 0; z < n; z += t.tile_z)
       for (int y = 0; y < n; y += t.tile_y)
         for (int x = 0; x < n; x += t.tile_x) {
@@ -338,7 +386,7 @@ int build_tiles(lbm_patchset* ps, const lbm_tiling& t) {
   return LBM_OK;
 }
 
-template <typename T>
+template <typename T, bool M32>
 hipError_t launch_step_t(lbm_patchset* ps, const void* src, void* dst) {
   const lbm_tiling& t = ps->tiling;
   // blocks_per_cu = 0: one workgroup per tile (the loop body runs once)
@@ -348,7 +396,7 @@ hipError_t launch_step_t(lbm_patchset* ps, const void* src, void* dst) {
   T* d = (T*)dst;
 #define LBM_PATCH(X, Y, Z)                                                                                     \
   if (t.tile_x == X && t.tile_y == Y && t.tile_z == Z) {                                                       \
-    hipLaunchKernelGGL((k_patch_step<T, X, Y, Z>), grid, block, 0, ps->stream, ps->d_patches, ps->d_tiles,    \
+    hipLaunchKernelGGL((k_patch_step<T, X, Y, Z, M32>), grid, block, 0, ps->stream, ps->d_patches, ps->d_tiles, \
                        ps->ntiles, s, d);                                                                      \
     return hipGetLastError();                                                                                  \
   }
@@ -361,12 +409,19 @@ hipError_t launch_step_t(lbm_patchset* ps, const void* src, void* dst) {
   LBM_PATCH(32, 8, 2)
   LBM_PATCH(16, 4, 4)
 #undef LBM_PATCH
+  if (linear_tiles(t)) {
+    hipLaunchKernelGGL((k_patch_step_lin<T, M32>), grid, block, 0, ps->stream, ps->d_patches, ps->d_tiles, ps->ntiles,
+                       s, d);
+    return hipGetLastError();
+  }
   return hipErrorInvalidValue;
 }
 
 hipError_t launch_step(lbm_patchset* ps, int from) {
-  return ps->fp32 ? launch_step_t<float>(ps, ps->buf[from], ps->buf[from ^ 1])
-                  : launch_step_t<double>(ps, ps->buf[from], ps->buf[from ^ 1]);
+  if (ps->fp32)
+    return ps->f32math ? launch_step_t<float, true>(ps, ps->buf[from], ps->buf[from ^ 1])
+                       : launch_step_t<float, false>(ps, ps->buf[from], ps->buf[from ^ 1]);
+  return launch_step_t<double, false>(ps, ps->buf[from], ps->buf[from ^ 1]);
 }
 
 bool shape_supported(const lbm_tiling* t) {
@@ -388,7 +443,8 @@ int lbm_patches_create(lbm_patchset** out, const lbm_patch* table, int npatches,
     return code;
   };
   if (!table || npatches < 1) return bail(LBM_ERR_ARG, "empty patch table");
-  if (precision != LBM_FP64 && precision != LBM_FP32_SHIFTED) return bail(LBM_ERR_ARG, "unknown precision");
+  if (precision != LBM_FP64 && precision != LBM_FP32_SHIFTED && precision != LBM_FP32_SHIFTED_F32)
+    return bail(LBM_ERR_ARG, "unknown precision");
   for (int p = 0; p < npatches; p++) {
     const lbm_patch& P = table[p];
     if (P.n < 2 || P.n > 256) return bail(LBM_ERR_ARG, "patch edge must be in 2..256");
@@ -400,7 +456,8 @@ int lbm_patches_create(lbm_patchset** out, const lbm_patch* table, int npatches,
     }
   }
   ps->device = device;
-  ps->fp32 = precision == LBM_FP32_SHIFTED;
+  ps->fp32 = precision != LBM_FP64;
+  ps->f32math = precision == LBM_FP32_SHIFTED_F32;
   ps->esize = ps->fp32 ? 4 : 8;
   int64_t base = 0, cb = 0;
   for (int p = 0; p < npatches; p++) {

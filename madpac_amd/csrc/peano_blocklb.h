@@ -37,7 +37,7 @@ constexpr int LB_CURRENT_DIR = 19;
 // lid-driven-cavity and default-channel of SLB/configurations/BlockLatticeBoltzmannScenarioConfiguration.cpp:52-56;
 // PeriodicBox (all axes periodic, body force) has no reference counterpart
 enum class Scenario { LidDrivenCavity, DefaultChannel, PeriodicBox };
-enum class Precision { FP64 = LBM_FP64, FP32Shifted = LBM_FP32_SHIFTED };
+enum class Precision { FP64 = LBM_FP64, FP32Shifted = LBM_FP32_SHIFTED, FP32ShiftedF32Math = LBM_FP32_SHIFTED_F32 };
 
 // Run configuration: the XML tags regular-grid, lattice-boltzmann-configuration and
 // lattice-boltzmann-scenario of the reference (LB/configurations/LatticeBoltzmannConfiguration.cpp:3-14,

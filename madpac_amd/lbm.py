@@ -16,7 +16,8 @@ LIB_PATH = os.path.join(_HERE, "liblbm_hip.so")
 
 Q = 19
 FP64 = 0
-FP32_SHIFTED = 1
+FP32_SHIFTED = 1       # FP32 state, FP64 collision arithmetic
+FP32_SHIFTED_F32 = 2   # FP32 state, FP32 collision arithmetic
 # boundary-condition types of a class (include/lbm_capi.h)
 BC_MOVING_WALL, BC_PRESSURE_OUTLET, BC_PRESSURE_INLET, BC_VELOCITY_INLET = 0, 1, 2, 3
 BC_CELL_VELOCITY = 16

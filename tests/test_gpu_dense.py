@@ -190,13 +190,14 @@ def test_kinetic_energy_matches_oracle_taylor_green():
         assert abs(ke - ke_o) <= tol * ke_o, (prec, ke, ke_o)
 
 
-def test_fp32_shifted_cavity_error_vs_fp64_oracle():
+@pytest.mark.parametrize("prec", [lbm.FP32_SHIFTED, lbm.FP32_SHIFTED_F32])
+def test_fp32_shifted_cavity_error_vs_fp64_oracle(prec):
     # FP32 shifted populations against the FP64 oracle: velocity L2 / Linf relative errors
     n, steps = 24, 600
     lat64 = O.Lattice.cavity(n, 0.6, 0.05)
     fo = lat64.step(lat64.equilibrium(), steps, nthreads=os.cpu_count() or 1)
     rho_o, u_o = lat64.fields(fo)
-    with lbm.cavity(n, 0.6, 0.05, precision=lbm.FP32_SHIFTED) as lat:
+    with lbm.cavity(n, 0.6, 0.05, precision=prec) as lat:
         lat.step(steps)
         rho, u = lat.fields()
     l2 = np.linalg.norm(u - u_o) / np.linalg.norm(u_o)
@@ -204,6 +205,34 @@ def test_fp32_shifted_cavity_error_vs_fp64_oracle():
     print(f"[gpu] FP32-shifted cavity {n}^3 x {steps}: u L2 {l2:.3e} Linf {linf:.3e}", flush=True)
     assert l2 <= 1e-5 and linf <= 1e-4
     assert np.max(np.abs(rho - rho_o)) <= 1e-6
+
+
+def test_fp32_shifted_has_no_momentum_drift_in_forced_porous_flow():
+    # Low-Mach forced flow in a porous box: FP32 collision arithmetic repeats the same rounding of
+    # the equilibrium every step and drifts the momentum (~1e-5 relative after 500 steps here);
+    # the FP32-shifted state with FP64 collision arithmetic rounds only at the store.
+    n, steps, tau, g = 24, 500, 0.55, (1e-6, 0.0, 0.0)
+    solid = _porous_solid(n, 17, frac=0.2, rmin=2, rmax=4)
+    ora = O.Lattice(n, n, n, periodic=(True, True, True))
+    ora.set_solid(solid)
+    ora.set_force(g)
+    ora.params.tau = tau
+    fo = ora.step(ora.equilibrium(), steps, nthreads=os.cpu_count() or 1)
+    _, u_o = ora.fields(fo)
+    fl = ~solid.reshape(-1)
+    out = {}
+    for prec in (lbm.FP32_SHIFTED, lbm.FP32_SHIFTED_F32):
+        with lbm.Lattice(n, n, n, tau, precision=prec, periodic=(True, True, True), force=g) as lat:
+            lat.set_flags(~solid)
+            lat.init_equilibrium()
+            lat.step(steps)
+            _, u = lat.fields()
+        out[prec] = (np.linalg.norm(u[fl] - u_o[fl]) / np.linalg.norm(u_o[fl]),
+                     (u[fl, 0].sum() - u_o[fl, 0].sum()) / u_o[fl, 0].sum())
+    print(f"[gpu] forced porous {n}^3 x {steps}: FP64-arith u L2 {out[lbm.FP32_SHIFTED][0]:.2e} "
+          f"drift {out[lbm.FP32_SHIFTED][1]:.2e}; FP32-arith u L2 {out[lbm.FP32_SHIFTED_F32][0]:.2e} "
+          f"drift {out[lbm.FP32_SHIFTED_F32][1]:.2e}", flush=True)
+    assert out[lbm.FP32_SHIFTED][0] <= 5e-7 and abs(out[lbm.FP32_SHIFTED][1]) <= 2e-7
 
 
 def test_error_paths_return_codes():

@@ -98,6 +98,22 @@ def test_graph_replay_and_autotune_are_bit_identical():
     assert abs(res[0][2][0] - m0) < 1e-9 * m0
 
 
+@pytest.mark.parametrize("prec", [lbm.FP64, lbm.FP32_SHIFTED])
+def test_every_tiling_is_bit_identical(prec):
+    # linear runs and box tiles, persistent grids and one workgroup per tile: same bits
+    table, info, u = _setup([((2, 1, 1), 24), ((2, 2, 2), 8), ((1, 1, 1), 32), ((2, 1, 1), 16)], seed=9)
+    out = []
+    for tiling in ((32, 8, 1, 4), (256, 0, 0, 0), (256, 0, 0, 8), (8, 8, 4, 0), (16, 16, 2, 2), (16, 4, 4, 64)):
+        with lbm.PatchSet(table, precision=prec) as ps:
+            ps.init(u=u)
+            ps.set_tiling(*tiling)
+            ps.step(9, graph=True)
+            out.append(ps.get_populations())
+    for other in out[1:]:
+        for a, b in zip(out[0], other):
+            assert np.array_equal(a, b)
+
+
 def test_patches_fp32_shifted_close_to_fp64_oracle():
     table, info, u = _setup([((2, 2, 2), 8)], seed=3)
     steps = 50
