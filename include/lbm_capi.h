@@ -184,9 +184,11 @@ int lbm_adjoint_pressure_drop(lbm_ctx* ctx, int nsteps, int snapshots, int xin, 
 /* ---- batched uneven patches -------------------------------------------------------------- */
 typedef struct lbm_patchset lbm_patchset;
 typedef struct {
-  int n;                   /* cubic patch edge in cells (each patch holds (n+2)^3 slots with ghosts) */
+  int n;                   /* cubic patch edge in cells (n^3 cells, plus ghost records for the
+                              populations entering across its 6 faces and 12 edges) */
   double tau;              /* per-patch relaxation time */
-  int nbr[LBM_Q];          /* neighbour patch in direction c_q (q != 9); must share edge n */
+  int nbr[LBM_Q];          /* neighbour patch in direction c_q (q != 9); must share edge n and list
+                              this patch in direction c_{18-q} */
 } lbm_patch;
 typedef struct {
   int tile_x, tile_y, tile_z;  /* tile extents in cells (one workgroup per tile) */

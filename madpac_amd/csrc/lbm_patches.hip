@@ -29,9 +29,9 @@
 namespace lbm {
 
 struct PatchDev {
-  int64_t base;      // element offset of the patch's SoA block (19 planes of np^3)
+  int64_t base;      // element offset of the patch: 19 interior planes of n^3, then its ghost records
   int n;             // interior edge
-  int np;            // n + 2
+  int pad_;
   double omega;      // 1 / tau_eff of the patch
   int nbr[Q];        // neighbour patch in direction c_q
   int64_t nbase[Q];  // that neighbour's base (one load fewer on the ghost path)
@@ -43,9 +43,49 @@ struct TileDev {
   int64_t base;      // the patch's base
   double omega;
   int patch;
-  int n, np;
-  int x0, y0, z0;    // tile origin, interior coordinates
+  int n;
+  int x0, y0, z0;    // tile origin (interior coordinates), or the first cell's linear index
 };
+
+// Storage of one patch (element offsets from its base):
+//   19 interior planes of n^3 cells, x fastest: plane q holds the post-collision population q;
+//   then the ghost records of the populations that enter the patch from outside:
+//   6 faces (axis a, side e = -1/+1; id 2a + (e > 0)) x the 5 populations with c_a = -e, each an
+//   n x n array indexed by the two other coordinates of the source cell (lower axis fastest);
+//   12 edges (axes a < b, sides e_a, e_b; id 4 pair + 2 (e_a > 0) + (e_b > 0), pair xy 0, xz 1,
+//   yz 2) x the one population c = -(e_a, e_b), an n-array indexed by the remaining coordinate.
+// The neighbour across the face or edge writes a ghost record when it streams out; the patch reads
+// it when it pulls.  Interior rows and planes are contiguous, and the ghost records of a face are
+// contiguous along its lower in-face axis, so both sides access them with consecutive lanes.
+__host__ __device__ inline int64_t patch_elems(int n) {
+  const int64_t n1 = n;
+  return ((int64_t)Q * n1 * n1 * n1 + 30 * n1 * n1 + 12 * n1 + 63) / 64 * 64;
+}
+
+// index of q among the 5 populations that cross a face normal to axis a (c_{q,a} != 0)
+__host__ __device__ constexpr int face_local(int q, int a) {
+  const int cb = a == 0 ? CY(q) : CX(q);  // the two other components, lower axis first
+  const int cc = a == 2 ? CY(q) : CZ(q);
+  return (cb == 0 && cc == 0) ? 0 : (cb != 0 ? (cb < 0 ? 1 : 2) : (cc < 0 ? 3 : 4));
+}
+
+// Offset of slot s = (s0, s1, s2) of population q: the own plane when s is interior, else the
+// ghost record filled by the neighbour that owns s (s outside along one axis: a face; two: an edge).
+__host__ __device__ __forceinline__ int64_t slot_offset(int q, int n, int s0, int s1, int s2) {
+  const int64_t n1 = n, n2 = n1 * n1, n3 = n2 * n1;
+  const int e0 = s0 < 0 ? -1 : (s0 >= n ? 1 : 0);
+  const int e1 = s1 < 0 ? -1 : (s1 >= n ? 1 : 0);
+  const int e2 = s2 < 0 ? -1 : (s2 >= n ? 1 : 0);
+  if ((e0 | e1 | e2) == 0) return (int64_t)q * n3 + s0 + n1 * (s1 + n1 * s2);
+  const int64_t G0 = (int64_t)Q * n3;
+  if (e1 == 0 && e2 == 0) return G0 + (5 * (0 + (e0 > 0)) + face_local(q, 0)) * n2 + s1 + n1 * s2;
+  if (e0 == 0 && e2 == 0) return G0 + (5 * (2 + (e1 > 0)) + face_local(q, 1)) * n2 + s0 + n1 * s2;
+  if (e0 == 0 && e1 == 0) return G0 + (5 * (4 + (e2 > 0)) + face_local(q, 2)) * n2 + s0 + n1 * s1;
+  const int64_t E0 = G0 + 30 * n2;
+  if (e2 == 0) return E0 + (0 + 2 * (e0 > 0) + (e1 > 0)) * n1 + s2;
+  if (e1 == 0) return E0 + (4 + 2 * (e0 > 0) + (e2 > 0)) * n1 + s1;
+  return E0 + (8 + 2 * (e1 > 0) + (e2 > 0)) * n1 + s0;
+}
 
 // D3Q19 index of a face or edge direction e (at most two non-zero components), by arithmetic on
 // the z-major ordering: z = 0 holds the 3x3 block 5..13, z = -1 / +1 the five directions 0..4 / 14..18.
@@ -58,13 +98,6 @@ constexpr bool dir_index_matches_table() {
   return true;
 }
 static_assert(dir_index_matches_table(), "dir_index must invert the D3Q19 ordering");
-
-// D3Q19 index of a direction vector e in {-1,0,1}^3 (-1 where not a D3Q19 direction)
-__host__ __device__ constexpr int dir_of(int ex, int ey, int ez) {
-  for (int q = 0; q < Q; q++)
-    if (CX(q) == ex && CY(q) == ey && CZ(q) == ez) return q;
-  return -1;
-}
 
 // M32: FP32 arithmetic for FP32 storage (else FP64 arithmetic, rounded once at the store)
 template <typename T, bool M32>
@@ -84,23 +117,21 @@ __device__ __forceinline__ void collide_patch(T (&f)[Q], double omega) {
   }
 }
 
-// One interior cell (x, y, z) of a patch: pull (ghost layer included), collide, store the own slots,
-// and write the populations that leave the patch into the neighbours' ghost layers.
+// One interior cell (x, y, z) of a patch: pull (ghost records included), collide, store the own
+// planes, and write each population that leaves the patch into the receiving neighbour's ghost
+// record.  The receiver sees the source at s = (x, y, z) - n e (e = the crossed face or edge).
 template <typename T, bool M32>
 __device__ __forceinline__ void patch_cell(const PatchDev* __restrict__ pp, const T* __restrict__ s,
-                                           T* __restrict__ d, T* __restrict__ dst, int n, int np, int64_t P,
-                                           double omega, int x, int y, int z) {
-  const int64_t i = (int64_t)(x + 1) + (int64_t)np * ((y + 1) + (int64_t)np * (z + 1));
+                                           T* __restrict__ d, T* __restrict__ dst, int n, double omega, int x, int y,
+                                           int z) {
   T f[Q];
 #pragma unroll
-  for (int q = 0; q < Q; q++) {
-    const int64_t off = (int64_t)CX(q) + (int64_t)np * (CY(q) + (int64_t)np * CZ(q));
-    f[q] = s[(int64_t)q * P + i - off];
-  }
+  for (int q = 0; q < Q; q++) f[q] = s[slot_offset(q, n, x - CX(q), y - CY(q), z - CZ(q))];
   collide_patch<T, M32>(f, omega);
+  const int64_t n1 = n, n3 = n1 * n1 * n1;
+  const int64_t i = x + n1 * (y + n1 * z);
 #pragma unroll
-  for (int q = 0; q < Q; q++) __builtin_nontemporal_store(f[q], &d[(int64_t)q * P + i]);
-  // ghost-layer exchange: populations leaving the patch go to the neighbour's ghost layer
+  for (int q = 0; q < Q; q++) __builtin_nontemporal_store(f[q], &d[(int64_t)q * n3 + i]);
   if (x == 0 || y == 0 || z == 0 || x == n - 1 || y == n - 1 || z == n - 1) {
 #pragma unroll
     for (int q = 0; q < Q; q++) {
@@ -111,9 +142,7 @@ __device__ __forceinline__ void patch_cell(const PatchDev* __restrict__ pp, cons
       const int ez = tz < 0 ? -1 : (tz >= n ? 1 : 0);
       if ((ex | ey | ez) == 0) continue;
       const int k = dir_index(ex, ey, ez);  // a face or edge direction: always a D3Q19 direction
-      const int gx = x - n * ex, gy = y - n * ey, gz = z - n * ez;  // ghost position in the neighbour
-      const int64_t gi = (int64_t)(gx + 1) + (int64_t)np * ((gy + 1) + (int64_t)np * (gz + 1));
-      dst[pp->nbase[k] + (int64_t)q * P + gi] = f[q];
+      dst[pp->nbase[k] + slot_offset(q, n, x - n * ex, y - n * ey, z - n * ez)] = f[q];
     }
   }
 }
@@ -134,12 +163,11 @@ __global__ __launch_bounds__(256) void k_patch_step(const PatchDev* __restrict__
     if (tn < ntiles) next = tiles[tn];
     // per-lane lookups go through the global table (a private copy would live in scratch memory)
     const PatchDev* __restrict__ pp = patches + tl.patch;
-    const int n = tl.n, np = tl.np;
-    const int64_t P = (int64_t)np * np * np;
+    const int n = tl.n;
     for (int c = threadIdx.x; c < TC; c += 256) {
       const int x = tl.x0 + c % TX, y = tl.y0 + (c / TX) % TY, z = tl.z0 + c / (TX * TY);
       if (x >= n || y >= n || z >= n) continue;
-      patch_cell<T, M32>(pp, src + tl.base, dst + tl.base, dst, n, np, P, tl.omega, x, y, z);
+      patch_cell<T, M32>(pp, src + tl.base, dst + tl.base, dst, n, tl.omega, x, y, z);
     }
     if (tn >= ntiles) break;
     t = tn;
@@ -161,13 +189,12 @@ __global__ __launch_bounds__(256) void k_patch_step_lin(const PatchDev* __restri
     TileDev next{};
     if (tn < ntiles) next = tiles[tn];
     const PatchDev* __restrict__ pp = patches + tl.patch;
-    const int n = tl.n, np = tl.np;
-    const int64_t P = (int64_t)np * np * np;
+    const int n = tl.n;
     const int c = tl.x0 + (int)threadIdx.x;
     if (c < n * n * n) {
       const int z = c / (n * n), r = c - z * n * n;
       const int y = r / n, x = r - y * n;
-      patch_cell<T, M32>(pp, src + tl.base, dst + tl.base, dst, n, np, P, tl.omega, x, y, z);
+      patch_cell<T, M32>(pp, src + tl.base, dst + tl.base, dst, n, tl.omega, x, y, z);
     }
     if (tn >= ntiles) break;
     t = tn;
@@ -175,18 +202,16 @@ __global__ __launch_bounds__(256) void k_patch_step_lin(const PatchDev* __restri
   }
 }
 
-// initial state: f = feq(rho, u) per interior cell, stored so that f_q(x) = s_q(x - c_q); a slot
-// in the ghost layer also has its counterpart in the neighbour's interior, which is written too
+// initial state: f = feq(rho, u) per interior cell, written to the slot each population is pulled
+// from (own plane or own ghost record)
 template <typename T>
 __global__ __launch_bounds__(256) void k_patch_init(const PatchDev* __restrict__ patches, int npatch,
                                                     const int64_t* __restrict__ cell_base, const double* __restrict__ rho,
                                                     const double* __restrict__ u, T* __restrict__ s) {
   const int p = blockIdx.y;
   if (p >= npatch) return;
-  const PatchDev* __restrict__ pp = patches + p;  // no private copy: nbr is indexed per lane
-  const int64_t base = pp->base;
-  const int n = pp->n, np = pp->np;
-  const int64_t P = (int64_t)np * np * np;
+  const int64_t base = patches[p].base;
+  const int n = patches[p].n;
   const int64_t cells = (int64_t)n * n * n;
   for (int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x; c < cells; c += (int64_t)gridDim.x * 256) {
     const int x = (int)(c % n), y = (int)((c / n) % n), z = (int)(c / ((int64_t)n * n));
@@ -202,16 +227,7 @@ __global__ __launch_bounds__(256) void k_patch_init(const PatchDev* __restrict__
       const double feq = W64(q) * r *
           (1.0 + cu / (1.0 / 3.0) + (cu * cu) / (2.0 * (1.0 / 3.0) * (1.0 / 3.0)) - (uu / (2.0 * (1.0 / 3.0))));
       const T v = (sizeof(T) == 8) ? (T)feq : (T)(feq - W64(q));
-      const int sx = x - CX(q), sy = y - CY(q), sz = z - CZ(q);  // the slot x - c_q
-      s[base + (int64_t)q * P + (int64_t)(sx + 1) + (int64_t)np * ((sy + 1) + (int64_t)np * (sz + 1))] = v;
-      const int ex = sx < 0 ? -1 : (sx >= n ? 1 : 0);
-      const int ey = sy < 0 ? -1 : (sy >= n ? 1 : 0);
-      const int ez = sz < 0 ? -1 : (sz >= n ? 1 : 0);
-      if ((ex | ey | ez) != 0) {
-        const PatchDev& nb = patches[pp->nbr[dir_of(ex, ey, ez)]];
-        const int gx = sx - n * ex, gy = sy - n * ey, gz = sz - n * ez;  // interior of the neighbour
-        s[nb.base + (int64_t)q * P + (int64_t)(gx + 1) + (int64_t)np * ((gy + 1) + (int64_t)np * (gz + 1))] = v;
-      }
+      s[base + slot_offset(q, n, x - CX(q), y - CY(q), z - CZ(q))] = v;
     }
   }
 }
@@ -223,17 +239,14 @@ __global__ __launch_bounds__(256) void k_patch_pull(const PatchDev* __restrict__
                                                     double* __restrict__ f) {
   const int p = blockIdx.y;
   if (p >= npatch) return;
-  const PatchDev pd = patches[p];
-  const int n = pd.n, np = pd.np;
-  const int64_t P = (int64_t)np * np * np;
+  const int64_t base = patches[p].base;
+  const int n = patches[p].n;
   const int64_t cells = (int64_t)n * n * n;
   for (int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x; c < cells; c += (int64_t)gridDim.x * 256) {
     const int x = (int)(c % n), y = (int)((c / n) % n), z = (int)(c / ((int64_t)n * n));
-    const int64_t i = (int64_t)(x + 1) + (int64_t)np * ((y + 1) + (int64_t)np * (z + 1));
 #pragma unroll
     for (int q = 0; q < Q; q++) {
-      const int64_t off = (int64_t)CX(q) + (int64_t)np * (CY(q) + (int64_t)np * CZ(q));
-      const T v = s[pd.base + (int64_t)q * P + i - off];
+      const T v = s[base + slot_offset(q, n, x - CX(q), y - CY(q), z - CZ(q))];
       f[Q * cell_base[p] + (int64_t)q * cells + c] = (sizeof(T) == 8) ? (double)v : (double)v + W64(q);
     }
   }
@@ -246,17 +259,14 @@ __global__ __launch_bounds__(256) void k_patch_moments(const PatchDev* __restric
   const int p = blockIdx.y;
   double v[4] = {0.0, 0.0, 0.0, 0.0};
   if (p < npatch) {
-    const PatchDev pd = patches[p];
-    const int n = pd.n, np = pd.np;
-    const int64_t P = (int64_t)np * np * np;
+    const int64_t base = patches[p].base;
+    const int n = patches[p].n;
     const int64_t cells = (int64_t)n * n * n;
     for (int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x; c < cells; c += (int64_t)gridDim.x * 256) {
       const int x = (int)(c % n), y = (int)((c / n) % n), z = (int)(c / ((int64_t)n * n));
-      const int64_t i = (int64_t)(x + 1) + (int64_t)np * ((y + 1) + (int64_t)np * (z + 1));
 #pragma unroll
       for (int q = 0; q < Q; q++) {
-        const int64_t off = (int64_t)CX(q) + (int64_t)np * (CY(q) + (int64_t)np * CZ(q));
-        const T sv = s[pd.base + (int64_t)q * P + i - off];
+        const T sv = s[base + slot_offset(q, n, x - CX(q), y - CY(q), z - CZ(q))];
         const double fv = (sizeof(T) == 8) ? (double)sv : (double)sv + W64(q);
         v[0] += (sizeof(T) == 8) ? (double)sv - W64(q) : (double)sv;
         if (CX(q) == 1) v[1] += fv;
@@ -365,14 +375,14 @@ int build_tiles(lbm_patchset* ps, const lbm_tiling& t) {
     const int n = ps->host[p].n;
     if (linear_tiles(t)) {
       const PatchDev& d = ps->host[p];
-      for (int c = 0; c < n * n * n; c += t.tile_x) tiles.push_back(TileDev{d.base, d.omega, p, d.n, d.np, c, 0, 0});
+      for (int c = 0; c < n * n * n; c += t.tile_x) tiles.push_back(TileDev{d.base, d.omega, p, d.n, c, 0, 0});
       continue;
     }
     for (int z = 0; z < n; z += t.tile_z)
       for (int y = 0; y < n; y += t.tile_y)
         for (int x = 0; x < n; x += t.tile_x) {
           const PatchDev& d = ps->host[p];
-          tiles.push_back(TileDev{d.base, d.omega, p, d.n, d.np, x, y, z});
+          tiles.push_back(TileDev{d.base, d.omega, p, d.n, x, y, z});
         }
   }
   if (ps->d_tiles) (void)hipFree(ps->d_tiles);
@@ -453,6 +463,8 @@ int lbm_patches_create(lbm_patchset** out, const lbm_patch* table, int npatches,
       if (q == REST) continue;
       if (P.nbr[q] < 0 || P.nbr[q] >= npatches) return bail(LBM_ERR_ARG, "neighbour index out of range");
       if (table[P.nbr[q]].n != P.n) return bail(LBM_ERR_ARG, "neighbouring patches must share their edge");
+      // the neighbour across c_q must see this patch across c_{18-q}: each ghost record has one writer
+      if (table[P.nbr[q]].nbr[Q - 1 - q] != p) return bail(LBM_ERR_ARG, "neighbour table must be reciprocal");
     }
   }
   ps->device = device;
@@ -463,13 +475,12 @@ int lbm_patches_create(lbm_patchset** out, const lbm_patch* table, int npatches,
   for (int p = 0; p < npatches; p++) {
     PatchDev d{};
     d.n = table[p].n;
-    d.np = d.n + 2;
     d.base = base;
     d.omega = 1.0 / tau_eff_of(table[p].tau);
     for (int q = 0; q < Q; q++) d.nbr[q] = q == REST ? p : table[p].nbr[q];
     ps->host.push_back(d);
     ps->cell_base.push_back(cb);
-    base += ((int64_t)Q * d.np * d.np * d.np + 63) / 64 * 64;  // 64-element aligned blocks
+    base += patch_elems(d.n);  // 64-element aligned blocks
     cb += (int64_t)d.n * d.n * d.n;
   }
   for (auto& d : ps->host)
