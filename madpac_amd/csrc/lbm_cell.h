@@ -267,6 +267,58 @@ __device__ __forceinline__ float collide_f32s_x(float (&gs)[Q], const CollidePar
   return (float)rho;
 }
 
+// FP32-shifted storage, hybrid arithmetic: the moments, rho, u and each equilibrium value in FP64
+// (the parts whose repeated FP32 rounding biases the momentum), the relaxation and the force term
+// in FP32.  Fewer live FP64 values than collide_f32s_x; outputs as collide_f32s.
+template <bool FORCE, bool ALPHA>
+__device__ __forceinline__ float collide_f32s_h(float (&gs)[Q], const CollideParams& p, const CollideParams32& p32,
+                                                float alpha, float (&uo)[4]) {
+  double drho = 0.0, j0 = 0.0, j1 = 0.0, j2 = 0.0;
+#pragma unroll
+  for (int q = 0; q < Q; q++) {
+    const double g = gs[q];
+    drho += g;
+    if (CX(q) == 1) j0 += g;
+    if (CX(q) == -1) j0 -= g;
+    if (CY(q) == 1) j1 += g;
+    if (CY(q) == -1) j1 -= g;
+    if (CZ(q) == 1) j2 += g;
+    if (CZ(q) == -1) j2 -= g;
+  }
+  const double rho = 1.0 + drho;
+  const double inv = 1.0 / rho;
+  double u0 = j0 * inv, u1 = j1 * inv, u2 = j2 * inv;
+  if (FORCE) {
+    u0 += 0.5 * p.g[0];
+    u1 += 0.5 * p.g[1];
+    u2 += 0.5 * p.g[2];
+  }
+  uo[0] = (float)u0;
+  uo[1] = (float)u1;
+  uo[2] = (float)u2;
+  uo[3] = (float)drho;
+  if (ALPHA) {
+    const double s = 1.0 - (double)alpha;
+    u0 *= s; u1 *= s; u2 *= s;
+  }
+  const double uu15 = 1.5 * (u0 * u0 + u1 * u1 + u2 * u2);
+  const float rho32 = (float)rho, v0 = (float)u0, v1 = (float)u1, v2 = (float)u2;
+  const float ug = FORCE ? v0 * p32.g[0] + v1 * p32.g[1] + v2 * p32.g[2] : 0.f;
+#pragma unroll
+  for (int q = 0; q < Q; q++) {
+    const double cu = cdot64(q, u0, u1, u2);
+    const float geq = (float)(W64(q) * (drho + rho * (3.0 * cu + 4.5 * cu * cu - uu15)));
+    float v = gs[q] - p32.omega * (gs[q] - geq);
+    if (FORCE) {
+      const float cg = (float)CX(q) * p32.g[0] + (float)CY(q) * p32.g[1] + (float)CZ(q) * p32.g[2];
+      const float cuf = (float)cu;
+      v += p32.fpref * W32(q) * rho32 * (3.0f * (cg - ug) + 9.0f * cuf * cg);
+    }
+    gs[q] = v;
+  }
+  return rho32;
+}
+
 __device__ __forceinline__ float moving_wall_f32(int q, const double* uw, float rho) {
   const float cu = (float)CX(q) * (float)uw[0] + (float)CY(q) * (float)uw[1] + (float)CZ(q) * (float)uw[2];
   return 6.0f * W32(q) * rho * cu;

@@ -112,8 +112,9 @@ __device__ __forceinline__ void collide_patch(T (&f)[Q], double omega) {
     collide_f32s<false, false>(f, p, 0.f, u);
   } else {
     CollideParams p{1.0 / omega, omega, {0.0, 0.0, 0.0}};
+    CollideParams32 p32{(float)omega, 0.f, {0.f, 0.f, 0.f}};
     float u[4];
-    collide_f32s_x<false, false>(f, p, 0.f, u);
+    collide_f32s_h<false, false>(f, p, p32, 0.f, u);
   }
 }
 
@@ -177,8 +178,13 @@ __global__ __launch_bounds__(256) void k_patch_step(const PatchDev* __restrict__
 
 // Linear tiles: 256 consecutive interior cells of one patch in x-fastest order, one per thread, so
 // no lane idles whatever the edge (tile.x0 holds the first cell's linear index).
+// FP32 state: at most 64 VGPRs, 8 waves per SIMD (the hybrid collision fits without spills);
+// FP64: at least 4 waves per SIMD (its natural allocation).
+template <typename T>
+constexpr int patch_waves_per_simd() { return sizeof(T) == 4 ? 8 : 4; }
+
 template <typename T, bool M32>
-__global__ __launch_bounds__(256) void k_patch_step_lin(const PatchDev* __restrict__ patches,
+__global__ __launch_bounds__(256, patch_waves_per_simd<T>()) void k_patch_step_lin(const PatchDev* __restrict__ patches,
                                                         const TileDev* __restrict__ tiles, int ntiles,
                                                         const T* __restrict__ src, T* __restrict__ dst) {
   int t = blockIdx.x;
@@ -303,9 +309,8 @@ namespace {
 struct Shape {
   int x, y, z;
 };
-// box tiles, and {256, 0, 0}: linear runs of 256 consecutive cells of a patch
-const Shape kShapes[] = {{8, 8, 4},  {8, 8, 8},  {16, 8, 2}, {16, 16, 1}, {16, 16, 2},
-                         {32, 8, 1}, {32, 8, 2}, {16, 4, 4}, {256, 0, 0}};
+// box tiles of 256 cells, and {256, 0, 0}: linear runs of 256 consecutive cells of a patch
+const Shape kShapes[] = {{8, 8, 4}, {16, 8, 2}, {16, 16, 1}, {32, 8, 1}, {16, 4, 4}, {256, 0, 0}};
 bool linear_tiles(const lbm_tiling& t) { return t.tile_y == 0; }
 
 double tau_eff_of(double tau) {
@@ -411,12 +416,9 @@ hipError_t launch_step_t(lbm_patchset* ps, const void* src, void* dst) {
     return hipGetLastError();                                                                                  \
   }
   LBM_PATCH(8, 8, 4)
-  LBM_PATCH(8, 8, 8)
   LBM_PATCH(16, 8, 2)
   LBM_PATCH(16, 16, 1)
-  LBM_PATCH(16, 16, 2)
   LBM_PATCH(32, 8, 1)
-  LBM_PATCH(32, 8, 2)
   LBM_PATCH(16, 4, 4)
 #undef LBM_PATCH
   if (linear_tiles(t)) {

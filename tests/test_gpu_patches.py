@@ -103,7 +103,7 @@ def test_every_tiling_is_bit_identical(prec):
     # linear runs and box tiles, persistent grids and one workgroup per tile: same bits
     table, info, u = _setup([((2, 1, 1), 24), ((2, 2, 2), 8), ((1, 1, 1), 32), ((2, 1, 1), 16)], seed=9)
     out = []
-    for tiling in ((32, 8, 1, 4), (256, 0, 0, 0), (256, 0, 0, 8), (8, 8, 4, 0), (16, 16, 2, 2), (16, 4, 4, 64)):
+    for tiling in ((32, 8, 1, 4), (256, 0, 0, 0), (256, 0, 0, 8), (8, 8, 4, 0), (16, 16, 1, 2), (16, 4, 4, 64)):
         with lbm.PatchSet(table, precision=prec) as ps:
             ps.init(u=u)
             ps.set_tiling(*tiling)
@@ -137,6 +137,12 @@ def test_patch_table_validation():
     bad = [(16, table[0][1], table[0][2])] + table[1:]  # neighbour edges differ
     with pytest.raises(lbm.LbmError, match="edge"):
         lbm.PatchSet(bad)
+    # patch 0 sees itself across +x while patch 1 still sees patch 0 across -x: two writers would
+    # share patch 0's -x ghost records
+    nbr = list(table[0][2])
+    nbr[10] = 0
+    with pytest.raises(lbm.LbmError, match="reciprocal"):
+        lbm.PatchSet([(table[0][0], table[0][1], nbr)] + table[1:])
     with lbm.PatchSet(table) as ps:
         with pytest.raises(lbm.LbmError, match="tile"):
             ps.set_tiling(5, 5, 5, 4)
