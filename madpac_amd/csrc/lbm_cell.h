@@ -133,16 +133,17 @@ __device__ __forceinline__ double collide_f64(double (&f)[Q], const CollideParam
     f[q] -= omega * (f[q] - feq);
   }
   if (FORCE) {
+    // Guo source (1 - 1/(2 tau)) w rho [3 (c - u).g + 9 (c.u)(c.g)] with cs2 = 1/3 folded into the
+    // factors (not a reference routine: the multiplications replace two divisions per population)
     const double pref = (1.0 - 0.5 / p.tau_eff) * rho;
+    double ug = u0 * p.g[0];
+    ug += u1 * p.g[1];
+    ug += u2 * p.g[2];
 #pragma unroll
     for (int q = 0; q < Q; q++) {
-      const double m0 = (double)CX(q) - u0, m1 = (double)CY(q) - u1, m2 = (double)CZ(q) - u2;
-      double cmg = m0 * p.g[0];
-      cmg += m1 * p.g[1];
-      cmg += m2 * p.g[2];
       const double cu = cdot64(q, u0, u1, u2);
       const double cg = cdot64(q, p.g[0], p.g[1], p.g[2]);
-      f[q] += pref * W64(q) * (cmg / (1.0 / 3.0) + cu * cg / ((1.0 / 3.0) * (1.0 / 3.0)));
+      f[q] += pref * W64(q) * (3.0 * (cg - ug) + 9.0 * cu * cg);
     }
   }
   return rho;

@@ -140,12 +140,14 @@ static double collide_cell(double* f, double tau_eff, const lbo_params* p, doubl
   lbo_feq(rho, u, feq);
   lbo_bgk(f, tau_eff, feq);
   if (p->use_force) {
+    /* Guo's source with cs2 = 1/3 folded into the factors: 3 (c - u).g + 9 (c.u)(c.g) */
     double pref = (1.0 - 0.5 / tau_eff) * rho;
+    double ug = dot3(u, p->g);
     for (int q = 0; q < LBO_Q; q++) {
       double c[3] = {cd(q, 0), cd(q, 1), cd(q, 2)};
-      double cmu[3] = {c[0] - u[0], c[1] - u[1], c[2] - u[2]};
       double cu = dot3(c, u);
-      f[q] += pref * LBO_W[q] * (dot3(cmu, p->g) / LBO_CS2 + cu * dot3(c, p->g) / (LBO_CS2 * LBO_CS2));
+      double cg = dot3(c, p->g);
+      f[q] += pref * LBO_W[q] * (3.0 * (cg - ug) + 9.0 * cu * cg);
     }
   }
   return rho;
