@@ -63,7 +63,7 @@ def run_dense(lbm, solid, n, tau, prec, steps, warmup, g):
     per_launch_s = ms * 1e-3 / steps
     bytes_per_launch = nfluid * 2 * 19 * esize
     achieved = bytes_per_launch / per_launch_s / 1e9
-    key = f"{'fp64' if prec == lbm.FP64 else 'fp32s'} {n}^3"
+    key = f"{'fp64' if prec == lbm.FP64 else 'fp32s'} {n}^3 {'loads-with-flag-word' if variant else 'flag-word-first'}"
     traffic = read_traffic(key)
     return {
         "mlups": nfluid * steps / (ms * 1e-3) / 1e6,

@@ -4,6 +4,7 @@ large-scale Taylor-Green vortex.
   python -m madpac_amd.validate precision [--size 256 --steps 2000]   porous-channel tau sweep
   python -m madpac_amd.validate tgv [--size 512 --steps 5000]          Taylor-Green energy decay
   python -m madpac_amd.validate tgv --size 384 --precisions fp64
+  python -m madpac_amd.validate adjoint [--size 128 --steps 500]       gradient vs central differences
 
 The FP64 device path is bit-identical to the CPU oracle (tests/test_gpu_dense.py), so it serves as
 the FP64 reference at sizes the CPU oracle cannot finish in time.  Output: one JSON object.
@@ -125,9 +126,73 @@ def taylor_green(n=512, steps=5000, u0=0.04, tau=0.52, precisions=("fp32s", "fp6
     return out
 
 
+def adjoint_fd(n=128, steps=500, snapshots=23, ncells=64, h=1e-4, g=1e-5, tau=0.6, seed=wl.DEFAULT_SEED):
+    """The adjoint workload: n^3 duct (walls on the y/z faces, periodic along x, body force g along x as
+    the driving pressure gradient), Brinkman porosity from 8 Fourier modes, J = FP64 pressure drop
+    between the planes x = n/4 and 3n/4 after `steps` steps.  The gradient dJ/dalpha from the FP32-
+    shifted and FP64 adjoints (revolve with `snapshots` checkpoints) against FP64 central differences
+    with step h on `ncells` seeded fluid cells."""
+    solid = wl.channel_walls(n, n, n)
+    alpha = wl.fourier_porosity(n, n, n).reshape(-1)
+    xin, xout = n // 4, 3 * n // 4
+    fluid = ~solid.reshape(-1)
+
+    def make(prec):
+        lat = lbm.Lattice(n, n, n, tau, precision=prec, periodic=(True, False, False), force=(g, 0.0, 0.0))
+        lat.set_flags(~solid)
+        return lat
+
+    out = {"workload": f"{n}^3 duct, Brinkman porosity (8 Fourier modes), g={g} along x, tau={tau}, {steps} steps, "
+                       f"J = FP64 pressure drop x={xin} -> x={xout}, revolve {snapshots} snapshots, "
+                       f"FD h={h} (FP64) on {ncells} cells", "adjoint": {}}
+    grads = {}
+    for name, prec in (("fp64", lbm.FP64), ("fp32s", lbm.FP32_SHIFTED)):
+        with make(prec) as lat:
+            lat.set_porosity(alpha)
+            lat.init_equilibrium()
+            t0 = time.perf_counter()
+            J, grad, st = lat.adjoint_pressure_drop(steps, snapshots, xin, xout)
+            wall = time.perf_counter() - t0
+        grads[name] = grad
+        out["adjoint"][name] = {"J": J, "wall_s": wall, "forward_ms": st.forward_ms, "reverse_ms": st.reverse_ms,
+                                "recompute_steps": st.recompute_steps, "snapshots": st.snapshots_used}
+        _log(f"[validate] adjoint {name}: J = {J:.12e}, {wall:.2f} s")
+    rng = np.random.default_rng(seed)
+    ok = np.flatnonzero(fluid & (alpha >= h) & (alpha <= 1.0 - h))
+    cells = np.sort(rng.choice(ok, size=ncells, replace=False))
+    fd = np.empty(ncells)
+    with make(lbm.FP64) as lat:
+        t0 = time.perf_counter()
+        for k, i in enumerate(cells):
+            val = []
+            for sgn in (1.0, -1.0):
+                a = alpha.copy()
+                a[i] += sgn * h
+                lat.set_porosity(a)
+                lat.init_equilibrium()
+                lat.step(steps)
+                val.append(lat.pressure_drop(xin, xout))
+            fd[k] = (val[0] - val[1]) / (2 * h)
+        _log(f"[validate] {ncells} central differences in {time.perf_counter() - t0:.1f} s")
+    big = np.abs(fd) >= 1e-2 * np.abs(fd).max()
+    out["cells"] = cells.tolist()
+    out["fd"] = fd.tolist()
+    for name, grad in grads.items():
+        gk = grad[cells]
+        rel = np.abs(gk - fd) / np.abs(fd)
+        out["adjoint"][name].update({
+            "grad": gk.tolist(),
+            "normwise_rel_err": float(np.linalg.norm(gk - fd) / np.linalg.norm(fd)),
+            "max_rel_err_large": float(rel[big].max()),
+            "max_abs_err_over_max": float(np.abs(gk - fd).max() / np.abs(fd).max())})
+        _log(f"[validate] {name} adjoint vs FD: normwise {out['adjoint'][name]['normwise_rel_err']:.2e}, "
+             f"max rel (|fd| >= 1e-2 max) {out['adjoint'][name]['max_rel_err_large']:.2e}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("what", choices=["precision", "tgv"])
+    ap.add_argument("what", choices=["precision", "tgv", "adjoint"])
     ap.add_argument("--size", type=int, default=None)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--precisions", default="fp32s,fp64")
@@ -135,6 +200,8 @@ def main():
     a = ap.parse_args()
     if a.what == "precision":
         r = porous_precision_sweep(n=a.size or 256, steps=a.steps or 2000)
+    elif a.what == "adjoint":
+        r = adjoint_fd(n=a.size or 128, steps=a.steps or 500)
     else:
         r = taylor_green(n=a.size or 512, steps=a.steps or 5000, precisions=tuple(a.precisions.split(",")))
     s = json.dumps(r)
