@@ -354,11 +354,13 @@ __global__ __launch_bounds__(256) void k_plane_rho(DenseGeom g, const T* __restr
       const uint32_t w = words[c.i];
       if (w & FLUID) {
         const NbrOff o = nbr_offsets(g, c);
+        // density deviation rho - 1 = sum (f_q - w_q) (the FP32 state stores f_q - w_q): the rest
+        // density cancels in the pressure drop, and the small deviations keep its relative precision
         double r = 0.0;
 #pragma unroll
         for (int q = 0; q < Q; q++) {
           const T s = src[pull_index(q, w, c.i, g.N, o)];
-          r += (sizeof(T) == 8) ? (double)s : (double)s + W64(q);
+          r += (sizeof(T) == 8) ? (double)s - W64(q) : (double)s;
         }
         v[2 * k] = r;
         v[2 * k + 1] = 1.0;
