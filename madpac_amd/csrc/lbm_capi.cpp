@@ -313,7 +313,10 @@ int lbm_set_porosity(lbm_ctx* c, const double* alpha) {
   if (!c) return LBM_ERR_ARG;
   HIPCHK(c, hipSetDevice(c->cfg.device));
   if (!alpha) {
-    if (c->alpha) (void)hipFree(c->alpha);
+    if (c->alpha) {
+      HIPCHK(c, hipStreamSynchronize(c->stream));  // queued steps may still read it
+      (void)hipFree(c->alpha);
+    }
     c->alpha = nullptr;
     return LBM_OK;
   }
