@@ -45,8 +45,8 @@ def read_traffic(key):
     return None
 
 
-def run_dense(lbm, solid, n, tau, prec, steps, warmup, g):
-    lat = lbm.Lattice(n, n, n, tau, precision=prec, periodic=(True, True, True), force=(g, 0.0, 0.0))
+def run_dense(lbm, solid, n, tau, prec, steps, warmup, g, layout=0):
+    lat = lbm.Lattice(n, n, n, tau, precision=prec, periodic=(True, True, True), force=(g, 0.0, 0.0), layout=layout)
     lat.set_flags(~solid)
     lat.init_equilibrium()
     nfluid = lat.fluid_cells
@@ -63,7 +63,8 @@ def run_dense(lbm, solid, n, tau, prec, steps, warmup, g):
     per_launch_s = ms * 1e-3 / steps
     bytes_per_launch = nfluid * 2 * 19 * esize
     achieved = bytes_per_launch / per_launch_s / 1e9
-    key = f"{'fp64' if prec == lbm.FP64 else 'fp32s'} {n}^3 {'loads-with-flag-word' if variant else 'flag-word-first'}"
+    schedule = "aa-in-place" if layout == lbm.LAYOUT_AA else ("loads-with-flag-word" if variant else "flag-word-first")
+    key = f"{'fp64' if prec == lbm.FP64 else 'fp32s'} {n}^3 {schedule}"
     traffic = read_traffic(key)
     return {
         "mlups": nfluid * steps / (ms * 1e-3) / 1e6,
@@ -72,7 +73,7 @@ def run_dense(lbm, solid, n, tau, prec, steps, warmup, g):
         "wall_s": wall,
         "fluid_cells": nfluid,
         "mass": m[0],
-        "variant": "loads-with-flag-word" if variant else "flag-word-first",
+        "variant": schedule,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "algorithmic_bytes_per_launch": bytes_per_launch},
@@ -193,6 +194,10 @@ def main():
         res[name] = run_dense(lbm, solid, n, args.tau, prec, args.steps, args.warmup, 1e-6)
         log(f"[bench] {name}: {res[name]['mlups']:.0f} MLUPS, {res[name]['ms_per_step']:.3f} ms/step, "
             f"{res[name]['roofline']['achieved']:.0f} GB/s ({100 * res[name]['roofline']['frac']:.1f} % of peak)")
+    for name, prec in (("fp64_aa", lbm.FP64), ("fp32s_aa", lbm.FP32_SHIFTED)):
+        res[name] = run_dense(lbm, solid, n, args.tau, prec, args.steps, args.warmup, 1e-6, layout=lbm.LAYOUT_AA)
+        log(f"[bench] {name}: {res[name]['mlups']:.0f} MLUPS, {res[name]['ms_per_step']:.3f} ms/step, "
+            f"{res[name]['roofline']['achieved']:.0f} GB/s ({100 * res[name]['roofline']['frac']:.1f} % of peak)")
     patches = None
     if not args.no_patches:
         patches = {}
@@ -225,6 +230,10 @@ def main():
         "fp32_shifted_f32_arithmetic": {"value": round(res["fp32s_f32math"]["mlups"], 1), "unit": "MLUPS",
                                         "dtype": "f32", "ms_per_step": round(res["fp32s_f32math"]["ms_per_step"], 4),
                                         "roofline": res["fp32s_f32math"]["roofline"]},
+        "aa_in_place": {k: {"value": round(res[k]["mlups"], 1), "unit": "MLUPS",
+                            "ms_per_step": round(res[k]["ms_per_step"], 4), "roofline": res[k]["roofline"],
+                            "state_bytes": 19 * n ** 3 * (8 if k == "fp64_aa" else 4)}
+                        for k in ("fp64_aa", "fp32s_aa")},
         "wall_mlups": {"fp64": round(r64["wall_mlups"], 1), "fp32s": round(r32["wall_mlups"], 1)},
         "batched_patches": patches,
         "adjoint": adj,

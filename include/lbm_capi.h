@@ -55,7 +55,14 @@ typedef struct {
   double tau;            /* configured relaxation time; tau_eff follows LB/MultiLevelSimData.h:160 */
   double force[3];       /* body acceleration for Guo forcing; all zero disables it */
   int device;            /* HIP device ordinal */
+  int layout;            /* enum lbm_layout (0 = two buffers) */
 } lbm_config;
+
+/* Storage of the state.  TWO_BUFFER: pull-form steps between two buffers.  AA: one buffer updated
+ * in place, alternating a step that touches only the cell's own slots with one that pulls from and
+ * pushes to the neighbours; half the memory at the same traffic (e.g. a 1024^3 FP64 lattice fits
+ * one MI355X).  The AA layout does not support the adjoint or the MD coupling. */
+enum lbm_layout { LBM_LAYOUT_TWO_BUFFER = 0, LBM_LAYOUT_AA = 1 };
 
 /* ---- lifetime ---------------------------------------------------------------------- */
 /* Replaces the regular-grid runner's set-up (BLB/runners/...ForRegularGrid.cpp:49-121) and

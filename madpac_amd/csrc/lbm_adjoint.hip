@@ -271,6 +271,7 @@ extern "C" int lbm_adjoint_pressure_drop(lbm_ctx* c, int nsteps, int snapshots, 
   if (xin < 0 || xin >= c->g.nx || xout < 0 || xout >= c->g.nx) return adj_fail(c, LBM_ERR_ARG, "plane out of range");
   if (!c->flags_set || !c->state_set) return adj_fail(c, LBM_ERR_STATE, "flags and state must be set first");
   if (c->couple_on) return adj_fail(c, LBM_ERR_STATE, "the adjoint does not cover MD-coupled cells");
+  if (c->aa) return adj_fail(c, LBM_ERR_STATE, "the adjoint needs the two-buffer layout");
   for (int k = 1; k < MAX_CLASSES; k++)
     if (c->bct[k].mask && (c->bct[k].type & BC_TYPE_MASK) != BC_MOVING_WALL)
       return adj_fail(c, LBM_ERR_STATE, "the adjoint supports halfway bounce-back and moving walls only");

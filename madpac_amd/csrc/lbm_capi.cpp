@@ -53,8 +53,9 @@ int rebuild_words(lbm_ctx* c) {
   // the stored state is a permutation that depends on the flags: it must be set again
   const size_t bytes = (size_t)Q * c->g.N * c->esize;
   HIPCHK(c, hipMemsetAsync(c->buf[0], 0, bytes, c->stream));
-  HIPCHK(c, hipMemsetAsync(c->buf[1], 0, bytes, c->stream));
+  if (c->buf[1]) HIPCHK(c, hipMemsetAsync(c->buf[1], 0, bytes, c->stream));
   c->state_set = false;
+  c->parity = 0;
   return LBM_OK;
 }
 
@@ -91,6 +92,8 @@ int lbm_create(lbm_ctx** out, const lbm_config* cfg) {
     return fail(nullptr, LBM_ERR_ARG, "lattice too large");
   if (cfg->precision != LBM_FP64 && cfg->precision != LBM_FP32_SHIFTED && cfg->precision != LBM_FP32_SHIFTED_F32)
     return fail(nullptr, LBM_ERR_ARG, "unknown precision");
+  if (cfg->layout != LBM_LAYOUT_TWO_BUFFER && cfg->layout != LBM_LAYOUT_AA)
+    return fail(nullptr, LBM_ERR_ARG, "unknown layout");
   if (!(cfg->tau > 0.5) || !(cfg->tau < 1e6)) return fail(nullptr, LBM_ERR_ARG, "tau must be > 0.5");
   for (int k = 0; k < 3; k++)
     if (!std::isfinite(cfg->force[k])) return fail(nullptr, LBM_ERR_ARG, "force must be finite");
@@ -102,6 +105,7 @@ int lbm_create(lbm_ctx** out, const lbm_config* cfg) {
   c->cfg = *cfg;
   c->fp32 = cfg->precision != LBM_FP64;
   c->f32math = cfg->precision == LBM_FP32_SHIFTED_F32;
+  c->aa = cfg->layout == LBM_LAYOUT_AA;
   c->esize = c->fp32 ? sizeof(float) : sizeof(double);
   c->g = lbm::make_geom(cfg->nx, cfg->ny, cfg->nz, cfg->periodic);
   c->force = cfg->force[0] != 0.0 || cfg->force[1] != 0.0 || cfg->force[2] != 0.0;
@@ -125,7 +129,7 @@ int lbm_create(lbm_ctx** out, const lbm_config* cfg) {
     return bail(LBM_ERR_HIP, "event creation failed");
   const size_t pbytes = (size_t)Q * c->g.N * c->esize;
   c->npartials = lbm::dense_moment_partials(c->g);
-  if (hipMalloc(&c->buf[0], pbytes) != hipSuccess || hipMalloc(&c->buf[1], pbytes) != hipSuccess ||
+  if (hipMalloc(&c->buf[0], pbytes) != hipSuccess || (!c->aa && hipMalloc(&c->buf[1], pbytes) != hipSuccess) ||
       hipMalloc(&c->words, (size_t)c->g.N * sizeof(uint32_t)) != hipSuccess ||
       hipMalloc(&c->d_inner, (size_t)c->g.Nu) != hipSuccess || hipMalloc(&c->d_cls, (size_t)c->g.Nu) != hipSuccess ||
       hipMalloc(&c->partials, (size_t)(c->npartials + 1024) * 4 * sizeof(double)) != hipSuccess ||
@@ -133,7 +137,7 @@ int lbm_create(lbm_ctx** out, const lbm_config* cfg) {
       hipMalloc(&c->d_bct, sizeof(c->bct)) != hipSuccess)
     return bail(LBM_ERR_NOMEM, "device allocation failed");
   if (hipMemsetAsync(c->buf[0], 0, pbytes, c->stream) != hipSuccess ||
-      hipMemsetAsync(c->buf[1], 0, pbytes, c->stream) != hipSuccess ||
+      (c->buf[1] && hipMemsetAsync(c->buf[1], 0, pbytes, c->stream) != hipSuccess) ||
       hipMemsetAsync(c->words, 0, (size_t)c->g.N * sizeof(uint32_t), c->stream) != hipSuccess ||
       hipMemsetAsync(c->d_cls, 0, (size_t)c->g.Nu, c->stream) != hipSuccess ||
       hipMemcpyAsync(c->d_bct, c->bct, sizeof(c->bct), hipMemcpyHostToDevice, c->stream) != hipSuccess ||
@@ -355,7 +359,8 @@ int lbm_init_equilibrium(lbm_ctx* c, const double* rho, const double* u) {
   double* d_u = u ? (double*)c->scratch + (rho ? n : 0) : nullptr;
   if (rho) HIPCHK(c, hipMemcpyAsync(d_rho, rho, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
   if (u) HIPCHK(c, hipMemcpyAsync(d_u, u, 3 * n * sizeof(double), hipMemcpyHostToDevice, c->stream));
-  HIPCHK(c, lbm::launch_dense_init_eq(c->g, c->fp32, d_rho, d_u, c->buf[c->cur], c->words, c->stream));
+  c->parity = 0;  // AA: written NATURAL
+  HIPCHK(c, lbm::launch_dense_init_eq(lbm_state_geom(c), c->fp32, d_rho, d_u, c->buf[c->cur], c->words, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->state_set = true;
   return LBM_OK;
@@ -370,7 +375,9 @@ int lbm_set_populations(lbm_ctx* c, const double* f) {
   int rc = ensure_scratch(c, bytes);
   if (rc) return rc;
   HIPCHK(c, hipMemcpyAsync(c->scratch, f, bytes, hipMemcpyHostToDevice, c->stream));
-  HIPCHK(c, lbm::launch_dense_unpull(c->g, c->fp32, (const double*)c->scratch, c->buf[c->cur], c->words, c->stream));
+  c->parity = 0;  // AA: written NATURAL
+  HIPCHK(c, lbm::launch_dense_unpull(lbm_state_geom(c), c->fp32, (const double*)c->scratch, c->buf[c->cur], c->words,
+                                     c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->state_set = true;
   return LBM_OK;
@@ -385,7 +392,8 @@ int lbm_get_populations(lbm_ctx* c, double* f) {
   const size_t bytes = (size_t)Q * c->g.Nu * sizeof(double);
   rc = ensure_scratch(c, bytes);
   if (rc) return rc;
-  HIPCHK(c, lbm::launch_dense_pull(c->g, c->fp32, c->buf[c->cur], (double*)c->scratch, c->words, c->stream));
+  HIPCHK(c, lbm::launch_dense_pull(lbm_state_geom(c), c->fp32, c->buf[c->cur], (double*)c->scratch, c->words,
+                                   c->stream));
   HIPCHK(c, hipMemcpyAsync(f, c->scratch, bytes, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return LBM_OK;
@@ -400,6 +408,14 @@ int lbm_enqueue_step_from(lbm_ctx* c, const void* src, void* dst) {
 }
 
 int lbm_enqueue_steps(lbm_ctx* c, int nsteps) {
+  if (c->aa) {
+    for (int s = 0; s < nsteps; s++) {
+      HIPCHK(c, lbm::launch_aa_step(c->g, c->fp32, c->f32math, c->parity, c->buf[0], c->words, c->alpha, c->force,
+                                    c->walls, c->p64, c->p32, lbm::BoundaryArgs{c->d_bct, c->d_bcu}, c->stream));
+      c->parity ^= 1;
+    }
+    return LBM_OK;
+  }
   for (int s = 0; s < nsteps; s++) {
     HIPCHK(c, lbm::launch_dense_step(c->g, c->fp32, c->f32math, c->buf[c->cur], c->buf[c->cur ^ 1], c->words,
                                      c->alpha, c->force, c->walls, c->p64, c->p32,
@@ -444,6 +460,10 @@ int lbm_tune(lbm_ctx* c, int nsteps, int* variant) {
   if (nsteps == 0) {  // explicit choice
     if (!variant || (*variant != 0 && *variant != 1)) return fail(c, LBM_ERR_ARG, "variant must be 0 or 1");
     c->spec = *variant == 1;
+    return LBM_OK;
+  }
+  if (c->aa) {  // the AA kernels have one load schedule
+    if (variant) *variant = 0;
     return LBM_OK;
   }
   int rc = need_state(c);
@@ -499,7 +519,7 @@ int lbm_get_fields(lbm_ctx* c, double* rho, double* u) {
   if (rc) return rc;
   double* d_rho = (double*)c->scratch;
   double* d_u = d_rho + n;
-  HIPCHK(c, lbm::launch_dense_fields(c->g, c->fp32, c->buf[c->cur], c->words, d_rho, d_u, c->stream));
+  HIPCHK(c, lbm::launch_dense_fields(lbm_state_geom(c), c->fp32, c->buf[c->cur], c->words, d_rho, d_u, c->stream));
   HIPCHK(c, hipMemcpyAsync(rho, d_rho, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipMemcpyAsync(u, d_u, 3 * n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -513,7 +533,8 @@ int lbm_reduce_moments(lbm_ctx* c, double out[4]) {
   if (rc) return rc;
   HIPCHK(c, hipSetDevice(c->cfg.device));
   double h[4];
-  HIPCHK(c, lbm::launch_dense_moments(c->g, c->fp32, c->buf[c->cur], c->words, c->partials, c->small, c->stream));
+  HIPCHK(c, lbm::launch_dense_moments(lbm_state_geom(c), c->fp32, c->buf[c->cur], c->words, c->partials, c->small,
+                                      c->stream));
   HIPCHK(c, hipMemcpyAsync(h, c->small, sizeof(h), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   // the kernel sums deviations from rest (f - w); the rest mass is one per fluid cell
@@ -532,7 +553,8 @@ int lbm_kinetic_energy(lbm_ctx* c, double* ke) {
   int rc = need_state(c);
   if (rc) return rc;
   HIPCHK(c, hipSetDevice(c->cfg.device));
-  HIPCHK(c, lbm::launch_dense_energy(c->g, c->fp32, c->buf[c->cur], c->words, c->partials, c->small, c->stream));
+  HIPCHK(c, lbm::launch_dense_energy(lbm_state_geom(c), c->fp32, c->buf[c->cur], c->words, c->partials, c->small,
+                                     c->stream));
   HIPCHK(c, hipMemcpyAsync(ke, c->small, sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   if (!std::isfinite(*ke)) return fail(c, LBM_ERR_NUMERIC, "non-finite kinetic energy");
@@ -547,8 +569,8 @@ int lbm_pressure_drop(lbm_ctx* c, int xin, int xout, double* J) {
   if (rc) return rc;
   HIPCHK(c, hipSetDevice(c->cfg.device));
   double h[4];
-  HIPCHK(c, lbm::launch_dense_plane_rho(c->g, c->fp32, c->buf[c->cur], c->words, xin, xout, c->partials, c->small,
-                                         c->stream));
+  HIPCHK(c, lbm::launch_dense_plane_rho(lbm_state_geom(c), c->fp32, c->buf[c->cur], c->words, xin, xout, c->partials,
+                                         c->small, c->stream));
   HIPCHK(c, hipMemcpyAsync(h, c->small, sizeof(h), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   if (h[1] < 1.0 || h[3] < 1.0) return fail(c, LBM_ERR_ARG, "a plane holds no fluid cell");

@@ -145,7 +145,7 @@ __global__ __launch_bounds__(256) void k_couple_gather(DenseGeom g, const T* __r
     double f[Q];
 #pragma unroll
     for (int q = 0; q < Q; q++) {
-      const T v = s[pull_index(q, w, i, g.N, o)];
+      const T v = s[state_index(g, q, w, i, o)];
       f[q] = sizeof(T) == 8 ? (double)v : (double)v + W64(q);
     }
     double b0 = 0.0, b1 = 0.0, b2 = 0.0;
@@ -222,6 +222,7 @@ extern "C" {
 int lbm_set_coupling_cells(lbm_ctx* c, const int64_t* cells, int64_t n) {
   if (!c) return LBM_ERR_ARG;
   if (n < 0 || (n > 0 && !cells)) return cfail(c, LBM_ERR_ARG, "coupling cell list is null or negative");
+  if (c->aa && n > 0) return cfail(c, LBM_ERR_STATE, "MD coupling needs the two-buffer layout");
   if (hipSetDevice(c->cfg.device) != hipSuccess || hipStreamSynchronize(c->stream) != hipSuccess)
     return cfail(c, LBM_ERR_HIP, "device synchronisation failed");
   lbm_coupling_release(c);

@@ -24,6 +24,8 @@ struct lbm_ctx {
   size_t esize = 8;
   void* buf[2] = {nullptr, nullptr};  // populations (double or shifted float), ping-pong
   int cur = 0;                         // buffer holding the current stored state
+  bool aa = false;                     // LBM_LAYOUT_AA: one buffer (buf[0]) updated in place
+  int parity = 0;                      // AA: 0 = NATURAL layout (next step even), 1 = SWAPPED
   uint32_t* words = nullptr;           // per-cell fluid / bounce / wall-class word
   uint8_t* d_inner = nullptr;
   uint8_t* d_cls = nullptr;
@@ -55,6 +57,12 @@ struct lbm_ctx {
 
 double lbm_tau_effective(double tau);
 int lbm_enqueue_steps(lbm_ctx* c, int nsteps);
+// the geometry with the layout of the current stored state (for the reading/writing kernels)
+inline lbm::DenseGeom lbm_state_geom(const lbm_ctx* c) {
+  lbm::DenseGeom g = c->g;
+  g.layout = !c->aa ? lbm::LAYOUT_PULL : (c->parity ? lbm::LAYOUT_AA_SWAPPED : lbm::LAYOUT_AA_NATURAL);
+  return g;
+}
 // one step from src into dst (any two state-sized device buffers), leaving buf/cur alone
 int lbm_enqueue_step_from(lbm_ctx* c, const void* src, void* dst);
 // re-collision of the MD-coupled cells for the step src -> dst (no-op without coupling)

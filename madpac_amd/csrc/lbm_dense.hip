@@ -30,6 +30,7 @@ DenseGeom make_geom(int nx, int ny, int nz, const int periodic[3]) {
   g.hx = g.hy = g.hz = 0;
   g.NX = nx; g.NY = ny; g.NZ = nz;
   g.Nu = g.N;
+  g.layout = LAYOUT_PULL;
   return g;
 }
 
@@ -65,6 +66,37 @@ __global__ __launch_bounds__(BX * BY) void k_build_words(DenseGeom g, const uint
 }
 
 // ------------------------------------------------------------------------------------------
+// Collision of one fluid cell plus the boundary classes of its bounce-back links (shared by the
+// two-buffer and the AA steps).  T = double: reference arithmetic; T = float: shifted populations,
+// FP64 collision arithmetic unless M32.
+template <typename T, bool FORCE, bool ALPHA, bool WALLS, bool M32>
+__device__ __forceinline__ void collide_cell_bc(T (&f)[Q], uint32_t w, int64_t i, const T* __restrict__ alpha,
+                                                const CollideParams& p64, const CollideParams32& p32,
+                                                const BoundaryArgs& bc) {
+  const T a = ALPHA ? alpha[i] : T(0);
+  T rho;
+  T uc[sizeof(T) == 8 ? 3 : 4];
+  if constexpr (sizeof(T) == 8) {
+    rho = collide_f64<FORCE, ALPHA>(f, p64, a, uc);
+  } else if constexpr (M32) {
+    rho = collide_f32s<FORCE, ALPHA>(f, p32, a, uc);
+  } else {
+    rho = collide_f32s_x<FORCE, ALPHA>(f, p64, a, uc);
+  }
+  if (WALLS) {
+    const uint32_t k = (w >> CLASS_SHIFT) & CLASS_BITS;
+    if (k) {
+      const BoundaryClass& b = bc.table[k];
+      const uint32_t m = w & b.mask;
+      if (m) {
+        if constexpr (sizeof(T) == 8) apply_boundary_f64(f, m, b, bc.cell_u, i, rho, uc[0], uc[1], uc[2]);
+        else apply_boundary_f32(f, m, b, bc.cell_u, i, rho, uc[3], uc[0], uc[1], uc[2]);
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------------------------------------
 // The hot kernel.  T = double: reference arithmetic; T = float: shifted populations.
 // SPEC = false: the flag word is read first, then the 19 loads at the pulled addresses (two
 // dependent memory round trips; solid lanes load nothing).  SPEC = true: the 19 neighbour loads
@@ -97,29 +129,46 @@ __global__ __launch_bounds__(BX * BY) void k_dense_step(DenseGeom g, const T* __
 #pragma unroll
     for (int q = 0; q < Q; q++) f[q] = src[pull_index(q, w, c.i, N, o)];
   }
-  const T a = ALPHA ? alpha[c.i] : T(0);
-  T rho;
-  T uc[sizeof(T) == 8 ? 3 : 4];
-  if constexpr (sizeof(T) == 8) {
-    rho = collide_f64<FORCE, ALPHA>(f, p64, a, uc);
-  } else if constexpr (M32) {
-    rho = collide_f32s<FORCE, ALPHA>(f, p32, a, uc);
-  } else {
-    rho = collide_f32s_x<FORCE, ALPHA>(f, p64, a, uc);
-  }
-  if (WALLS) {
-    const uint32_t k = (w >> CLASS_SHIFT) & CLASS_BITS;
-    if (k) {
-      const BoundaryClass& b = bc.table[k];
-      const uint32_t m = w & b.mask;
-      if (m) {
-        if constexpr (sizeof(T) == 8) apply_boundary_f64(f, m, b, bc.cell_u, c.i, rho, uc[0], uc[1], uc[2]);
-        else apply_boundary_f32(f, m, b, bc.cell_u, c.i, rho, uc[3], uc[0], uc[1], uc[2]);
-      }
-    }
-  }
+  collide_cell_bc<T, FORCE, ALPHA, WALLS, M32>(f, w, c.i, alpha, p64, p32, bc);
 #pragma unroll
   for (int q = 0; q < Q; q++) __builtin_nontemporal_store(f[q], &dst[(int64_t)q * N + c.i]);
+}
+
+// ------------------------------------------------------------------------------------------
+// AA in-place steps on one buffer a (half the memory of two buffers, the same traffic).
+// Even (ODD = false): a holds the reference state NATURAL, a_q(x) = f_q(x); the cell reads its own
+// 19 slots and writes f*_q(x) into its own slot 18 - q (SWAPPED): no neighbour access at all.
+// Odd: the cell reads f_q(x) = a_{18-q}(x - c_q), or a_q(x) across a bounce-back link, and writes
+// f*_q(x) to a_q(x + c_q), or to a_{18-q}(x) across a bounce-back link (NATURAL again).  In both, a
+// cell writes exactly the slots it read, and no other cell touches them, so the update is in place
+// without races.  Boundary classes act on the outgoing population of a bounce-back link, as in the
+// two-buffer step.
+template <typename T, bool FORCE, bool ALPHA, bool WALLS, bool M32, bool ODD>
+__global__ __launch_bounds__(BX * BY) void k_aa_step(DenseGeom g, T* __restrict__ a, const uint32_t* __restrict__ words,
+                                                     const T* __restrict__ alpha, CollideParams p64,
+                                                     CollideParams32 p32, BoundaryArgs bc) {
+  const CellPos c = map_cell(g);
+  if (!c.valid) return;
+  const uint32_t w = words[c.i];
+  if (!(w & FLUID)) return;
+  const int64_t N = g.N;
+  const NbrOff o = nbr_offsets(g, c);
+  T f[Q];
+#pragma unroll
+  for (int q = 0; q < Q; q++) f[q] = ODD ? a[swapped_index(q, w, c.i, N, o)] : a[(int64_t)q * N + c.i];
+  collide_cell_bc<T, FORCE, ALPHA, WALLS, M32>(f, w, c.i, alpha, p64, p32, bc);
+  // The stores go to the slots just loaded.  Left to itself the compiler keeps the 19 load
+  // addresses live through the collision (38 more VGPRs, one wave less per SIMD); an opaque copy
+  // of the cell index makes it recompute them here instead.
+  int64_t i = c.i;
+  asm volatile("" : "+v"(i));
+#pragma unroll
+  for (int q = 0; q < Q; q++) {
+    const bool bounce = (w >> q) & 1u;  // x + c_q is not fluid
+    const int64_t di = !ODD ? (int64_t)(Q - 1 - q) * N + i
+                            : (bounce ? (int64_t)(Q - 1 - q) * N + i : (int64_t)q * N + i + dst_off(q, o));
+    __builtin_nontemporal_store(f[q], &a[di]);
+  }
 }
 
 // ------------------------------------------------------------------------------------------
@@ -138,7 +187,7 @@ __global__ __launch_bounds__(BX * BY) void k_dense_pull(DenseGeom g, const T* __
   const NbrOff o = nbr_offsets(g, c);
 #pragma unroll
   for (int q = 0; q < Q; q++) {
-    const T v = src[pull_index(q, w, c.i, N, o)];
+    const T v = src[state_index(g, q, w, c.i, o)];
     f[(int64_t)q * N + c.i] = (sizeof(T) == 8) ? (double)v : (double)v + W64(q);
   }
 }
@@ -156,7 +205,7 @@ __global__ __launch_bounds__(BX * BY) void k_dense_unpull(DenseGeom g, const dou
 #pragma unroll
   for (int q = 0; q < Q; q++) {
     const double v = f[(int64_t)q * N + c.i];
-    dst[pull_index(q, w, c.i, N, o)] = (sizeof(T) == 8) ? (T)v : (T)(v - W64(q));
+    dst[state_index(g, q, w, c.i, o)] = (sizeof(T) == 8) ? (T)v : (T)(v - W64(q));
   }
 }
 
@@ -181,7 +230,7 @@ __global__ __launch_bounds__(BX * BY) void k_dense_init_eq(DenseGeom g, const do
     const double cu = cdot64(q, u0, u1, u2);
     const double feq = W64(q) * r *
         (1.0 + cu / (1.0 / 3.0) + (cu * cu) / (2.0 * (1.0 / 3.0) * (1.0 / 3.0)) - (uu / (2.0 * (1.0 / 3.0))));
-    dst[pull_index(q, w, c.i, N, o)] = (sizeof(T) == 8) ? (T)feq : (T)(feq - W64(q));
+    dst[state_index(g, q, w, c.i, o)] = (sizeof(T) == 8) ? (T)feq : (T)(feq - W64(q));
   }
 }
 
@@ -202,7 +251,7 @@ __global__ __launch_bounds__(BX * BY) void k_dense_fields(DenseGeom g, const T* 
   double r = 0.0, b0 = 0.0, b1 = 0.0, b2 = 0.0;
 #pragma unroll
   for (int q = 0; q < Q; q++) {
-    const T v = src[pull_index(q, w, c.i, g.N, o)];
+    const T v = src[state_index(g, q, w, c.i, o)];
     const double fv = (sizeof(T) == 8) ? (double)v : (double)v + W64(q);
     r += fv;
     if (CX(q) == 1) b0 += fv;
@@ -260,7 +309,7 @@ __global__ __launch_bounds__(BX * BY) void k_dense_moments(DenseGeom g, const T*
       const NbrOff o = nbr_offsets(g, c);
 #pragma unroll
       for (int q = 0; q < Q; q++) {
-        const T s = src[pull_index(q, w, c.i, g.N, o)];
+        const T s = src[state_index(g, q, w, c.i, o)];
         // deviation from the rest state: f - w (FP64) or the shifted value itself (FP32)
         const double fv = (sizeof(T) == 8) ? (double)s : (double)s + W64(q);
         const double dv = (sizeof(T) == 8) ? (double)s - W64(q) : (double)s;
@@ -291,7 +340,7 @@ __global__ __launch_bounds__(BX * BY) void k_dense_energy(DenseGeom g, const T* 
       double r = 0.0, j0 = 0.0, j1 = 0.0, j2 = 0.0;
 #pragma unroll
       for (int q = 0; q < Q; q++) {
-        const T s = src[pull_index(q, w, c.i, g.N, o)];
+        const T s = src[state_index(g, q, w, c.i, o)];
         const double fv = (sizeof(T) == 8) ? (double)s : (double)s + W64(q);
         r += fv;
         if (CX(q) == 1) j0 += fv;
@@ -359,7 +408,7 @@ __global__ __launch_bounds__(256) void k_plane_rho(DenseGeom g, const T* __restr
         double r = 0.0;
 #pragma unroll
         for (int q = 0; q < Q; q++) {
-          const T s = src[pull_index(q, w, c.i, g.N, o)];
+          const T s = src[state_index(g, q, w, c.i, o)];
           r += (sizeof(T) == 8) ? (double)s - W64(q) : (double)s;
         }
         v[2 * k] = r;
@@ -436,6 +485,43 @@ hipError_t launch_dense_step(const DenseGeom& g, bool fp32, bool f32math, const 
                 : step_t<float, false, false>(g, src, dst, words, alpha, force, walls, p64, p32, bc, s);
   return spec ? step_t<double, true, false>(g, src, dst, words, alpha, force, walls, p64, p32, bc, s)
               : step_t<double, false, false>(g, src, dst, words, alpha, force, walls, p64, p32, bc, s);
+}
+
+template <typename T, bool M32, bool ODD>
+static hipError_t aa_t(const DenseGeom& g, void* buf, const uint32_t* words, const void* alpha, bool force, bool walls,
+                       const CollideParams& p64, const CollideParams32& p32, const BoundaryArgs& bc, hipStream_t s) {
+  T* ap = (T*)buf;
+  const T* al = (const T*)alpha;
+#define LBM_AA(F, A, W)                                                                                        \
+  hipLaunchKernelGGL((k_aa_step<T, F, A, W, M32, ODD>), dense_grid(g), dense_block(), 0, s, g, ap, words, al, \
+                     p64, p32, bc)
+  const int key = (force ? 4 : 0) | (alpha ? 2 : 0) | (walls ? 1 : 0);
+  switch (key) {
+    case 0: LBM_AA(false, false, false); break;
+    case 1: LBM_AA(false, false, true); break;
+    case 2: LBM_AA(false, true, false); break;
+    case 3: LBM_AA(false, true, true); break;
+    case 4: LBM_AA(true, false, false); break;
+    case 5: LBM_AA(true, false, true); break;
+    case 6: LBM_AA(true, true, false); break;
+    default: LBM_AA(true, true, true); break;
+  }
+#undef LBM_AA
+  return hipGetLastError();
+}
+
+hipError_t launch_aa_step(const DenseGeom& g, bool fp32, bool f32math, int parity, void* a, const uint32_t* words,
+                          const void* alpha, bool force, bool walls, const CollideParams& p64,
+                          const CollideParams32& p32, const BoundaryArgs& bc, hipStream_t s) {
+  const bool odd = parity != 0;
+  if (fp32 && f32math)
+    return odd ? aa_t<float, true, true>(g, a, words, alpha, force, walls, p64, p32, bc, s)
+               : aa_t<float, true, false>(g, a, words, alpha, force, walls, p64, p32, bc, s);
+  if (fp32)
+    return odd ? aa_t<float, false, true>(g, a, words, alpha, force, walls, p64, p32, bc, s)
+               : aa_t<float, false, false>(g, a, words, alpha, force, walls, p64, p32, bc, s);
+  return odd ? aa_t<double, false, true>(g, a, words, alpha, force, walls, p64, p32, bc, s)
+             : aa_t<double, false, false>(g, a, words, alpha, force, walls, p64, p32, bc, s);
 }
 
 hipError_t launch_dense_pull(const DenseGeom& g, bool fp32, const void* src, double* f, const uint32_t* words,

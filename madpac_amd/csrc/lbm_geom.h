@@ -80,4 +80,17 @@ __device__ __forceinline__ int64_t pull_index(int q, uint32_t w, int64_t i, int6
   return bounce ? (int64_t)(Q - 1 - q) * N + i : (int64_t)q * N + i + src_off(q, o);
 }
 
+// the same for AA's SWAPPED layout (post-collision values reflected in the own slots)
+__device__ __forceinline__ int64_t swapped_index(int q, uint32_t w, int64_t i, int64_t N, const NbrOff& o) {
+  const bool bounce = (w >> (Q - 1 - q)) & 1u;
+  return bounce ? (int64_t)q * N + i : (int64_t)(Q - 1 - q) * N + i + src_off(q, o);
+}
+
+// index of population q of the reference state at cell i in the layout g.layout
+__device__ __forceinline__ int64_t state_index(const DenseGeom& g, int q, uint32_t w, int64_t i, const NbrOff& o) {
+  if (g.layout == LAYOUT_AA_NATURAL) return (int64_t)q * g.N + i;
+  if (g.layout == LAYOUT_AA_SWAPPED) return swapped_index(q, w, i, g.N, o);
+  return pull_index(q, w, i, g.N, o);
+}
+
 }  // namespace lbm

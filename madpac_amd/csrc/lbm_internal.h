@@ -30,7 +30,14 @@ struct DenseGeom {
   int hx, hy, hz;
   int NX, NY, NZ;
   int64_t Nu;
+  int layout;                // how the stored state maps to the reference state (LAYOUT_*)
 };
+
+// Stored-state layouts.  PULL: two buffers, post-collision values in the cell's own slots, so the
+// reference population q of cell x is s_q(x - c_q).  AA (one buffer, in place): NATURAL holds the
+// reference state itself, a_q(x) = f_q(x); SWAPPED (after an even AA step) holds post-collision
+// values reflected in the cell's own slots, a_{18-q}(x) = f*_q(x).
+enum : int { LAYOUT_PULL = 0, LAYOUT_AA_NATURAL = 1, LAYOUT_AA_SWAPPED = 2 };
 
 DenseGeom make_geom(int nx, int ny, int nz, const int periodic[3]);
 
@@ -52,6 +59,11 @@ hipError_t launch_dense_step(const DenseGeom& g, bool fp32, bool f32math, const 
                              const void* alpha, bool force, bool walls, const CollideParams& p64,
                              const CollideParams32& p32, const BoundaryArgs& bc, bool spec, hipStream_t stream);
 // reference state f (FP64) from the stored state
+// one in-place AA step on a single buffer: parity 0 reads the NATURAL layout and leaves SWAPPED,
+// parity 1 the reverse
+hipError_t launch_aa_step(const DenseGeom& g, bool fp32, bool f32math, int parity, void* a, const uint32_t* words,
+                          const void* alpha, bool force, bool walls, const CollideParams& p64,
+                          const CollideParams32& p32, const BoundaryArgs& bc, hipStream_t stream);
 hipError_t launch_dense_pull(const DenseGeom& g, bool fp32, const void* src, double* f, const uint32_t* words,
                              hipStream_t stream);
 // stored state from a reference state f (FP64)

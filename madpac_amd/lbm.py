@@ -18,6 +18,7 @@ Q = 19
 FP64 = 0
 FP32_SHIFTED = 1       # FP32 state, FP64 collision arithmetic
 FP32_SHIFTED_F32 = 2   # FP32 state, FP32 collision arithmetic
+LAYOUT_TWO_BUFFER, LAYOUT_AA = 0, 1
 # boundary-condition types of a class (include/lbm_capi.h)
 BC_MOVING_WALL, BC_PRESSURE_OUTLET, BC_PRESSURE_INLET, BC_VELOCITY_INLET = 0, 1, 2, 3
 BC_CELL_VELOCITY = 16
@@ -53,7 +54,7 @@ class LbmError(RuntimeError):
 class Config(ctypes.Structure):
     _fields_ = [("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int),
                 ("periodic", ctypes.c_int * 3), ("precision", ctypes.c_int), ("tau", ctypes.c_double),
-                ("force", ctypes.c_double * 3), ("device", ctypes.c_int)]
+                ("force", ctypes.c_double * 3), ("device", ctypes.c_int), ("layout", ctypes.c_int)]
 
 
 class AdjointStats(ctypes.Structure):
@@ -147,10 +148,10 @@ class Lattice:
     """A dense D3Q19 lattice on one GPU (an lbm_ctx)."""
 
     def __init__(self, nx, ny, nz, tau, precision=FP64, periodic=(False, False, False), force=(0.0, 0.0, 0.0),
-                 device=0):
+                 device=0, layout=LAYOUT_TWO_BUFFER):
         self._L = load_library()
         cfg = Config(nx, ny, nz, (ctypes.c_int * 3)(*[int(p) for p in periodic]), int(precision), float(tau),
-                     (ctypes.c_double * 3)(*[float(v) for v in force]), int(device))
+                     (ctypes.c_double * 3)(*[float(v) for v in force]), int(device), int(layout))
         h = ctypes.c_void_p()
         rc = self._L.lbm_create(ctypes.byref(h), ctypes.byref(cfg))
         if rc != 0:
@@ -404,19 +405,19 @@ class PatchSet:
         return out
 
 
-def channel(nx, ny, nz, tau, u_mean, parabolic=False, solid=None, precision=FP64, device=0):
+def channel(nx, ny, nz, tau, u_mean, parabolic=False, solid=None, precision=FP64, device=0, layout=LAYOUT_TWO_BUFFER):
     """The reference's default channel: no-slip y/z walls, velocity inlet on x- (uniform or
     parabolic), pressure outlet rho = 1 on x+, optional solid obstacles; f = w."""
-    lat = Lattice(nx, ny, nz, tau, precision=precision, device=device)
+    lat = Lattice(nx, ny, nz, tau, precision=precision, device=device, layout=layout)
     lat.set_flags(np.ones(lat.n, np.uint8) if solid is None else ~np.asarray(solid, bool).reshape(-1))
     lat.set_channel(u_mean, parabolic)
     lat.init_equilibrium()
     return lat
 
 
-def cavity(n, tau, ulid, precision=FP64, device=0):
+def cavity(n, tau, ulid, precision=FP64, device=0, layout=LAYOUT_TWO_BUFFER):
     """The reference's lid-driven cavity: n^3 fluid cells, lid (ulid, 0, 0) on +z, f = w."""
-    lat = Lattice(n, n, n, tau, precision=precision, device=device)
+    lat = Lattice(n, n, n, tau, precision=precision, device=device, layout=layout)
     lat.set_flags(np.ones(lat.n, np.uint8))
     lat.set_cavity_lid(ulid)
     lat.init_equilibrium()

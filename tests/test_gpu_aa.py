@@ -1,0 +1,121 @@
+"""The in-place AA layout (one state buffer, alternating own-slot and neighbour steps) against
+the reference fixtures, the oracle and the two-buffer layout, for odd and even step counts (the
+state sits in the NATURAL layout after an even count and in the SWAPPED one after an odd count).
+Bars: FP64 bit-identical to the fixtures / oracle; FP32-shifted bit-identical to the two-buffer
+FP32-shifted path (same arithmetic, same rounding at the store)."""
+import numpy as np
+import pytest
+
+import oracle as O
+from conftest import channel_tolerance, golden
+from test_gpu_dense import _porous_solid
+
+pytestmark = pytest.mark.gpu
+lbm = pytest.importorskip("madpac_amd.lbm")
+AA = lbm.LAYOUT_AA
+
+
+@pytest.mark.parametrize("name", ["cavity_n8_t1.npz", "cavity_n8_t10.npz", "cavity_n10_t37.npz",
+                                  "cavity_n13_t5.npz", "cavity_n8_t100.npz"])
+def test_aa_cavity_bitwise_equals_reference(name):
+    d = golden(name)
+    n, steps = int(d["n"]), int(d["steps"])
+    print(f"[gpu] AA cavity fixture {name}", flush=True)
+    with lbm.cavity(n, float(d["tau"]), float(d["ulid"]), layout=AA) as lat:
+        lat.step(steps)
+        f = lat.get_populations()
+    ref = d["f"].reshape(-1)
+    assert np.array_equal(f, ref), f"max |diff| = {np.abs(f - ref).max()}"
+
+
+@pytest.mark.parametrize("name", ["channel_8x8x8_t10_uniform.npz", "channel_10x7x9_t13_parabolic.npz"])
+def test_aa_channel_equals_reference(name):
+    d = golden(name)
+    nx, ny, nz = (int(v) for v in d["dims"])
+    steps, par = int(d["steps"]), bool(d["parabolic"])
+    print(f"[gpu] AA channel fixture {name}", flush=True)
+    with lbm.channel(nx, ny, nz, float(d["tau"]), float(d["umean"]), parabolic=par, layout=AA) as lat:
+        lat.step(steps)
+        f = lat.get_populations()
+    ref = d["f"].reshape(-1)
+    tol = channel_tolerance(d)
+    assert np.max(np.abs(f - ref)) <= tol * np.max(np.abs(ref))
+    ora = O.Lattice.channel(nx, ny, nz, float(d["tau"]), float(d["umean"]), parabolic=par)
+    assert np.array_equal(f, ora.step(ora.equilibrium(), steps))
+
+
+@pytest.mark.parametrize("steps", [1, 6, 13])
+def test_aa_porous_force_brinkman_bitwise_equals_oracle(steps):
+    # obstacles, a wall pair, a periodic axis, Guo force and Brinkman porosity in one run
+    n = 14
+    solid = _porous_solid(n, 31, frac=0.12)
+    solid[:, 0, :] = solid[:, -1, :] = True
+    rng = np.random.default_rng(steps)
+    alpha = 0.3 * rng.random(n ** 3)
+    g = (1e-5, 0.0, 2e-6)
+    ora = O.Lattice(n, n, n, periodic=(True, False, True))
+    ora.set_solid(solid)
+    ora.set_force(g)
+    ora.set_alpha(alpha)
+    ora.params.tau = 0.6
+    f0 = ora.equilibrium(u=0.01 * rng.standard_normal((n ** 3, 3)))
+    fo = ora.step(f0, steps)
+    print(f"[gpu] AA porous+force+Brinkman n={n} steps={steps}", flush=True)
+    with lbm.Lattice(n, n, n, 0.6, periodic=(True, False, True), force=g, layout=AA) as lat:
+        lat.set_flags(~solid)
+        lat.set_porosity(alpha)
+        lat.set_populations(f0)
+        lat.step(steps)
+        fg = lat.get_populations()
+        m = lat.moments()
+        J = lat.pressure_drop(2, 11)
+    assert np.array_equal(fg, fo), f"max |diff| = {np.abs(fg - fo).max()}"
+    mo = ora.moments(fo)
+    assert abs(m[0] - mo[0]) <= 1e-13 * mo[0]
+    assert abs(J - ora.pressure_drop(fo, 2, 11)) <= 1e-13
+
+
+@pytest.mark.parametrize("prec", [lbm.FP32_SHIFTED, lbm.FP32_SHIFTED_F32])
+def test_aa_fp32_bit_identical_to_two_buffer(prec):
+    nx, ny, nz = 20, 12, 10
+    z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    solid = (x - 7) ** 2 + (y - 6) ** 2 + (z - 5) ** 2 <= 9
+    out = {}
+    for layout in (lbm.LAYOUT_TWO_BUFFER, AA):
+        with lbm.channel(nx, ny, nz, 0.62, 0.04, parabolic=True, solid=solid, precision=prec, layout=layout) as lat:
+            res = []
+            for steps in (7, 4):  # odd, then back to the NATURAL layout
+                lat.step(steps)
+                res.append((lat.get_populations(), *lat.fields(), lat.kinetic_energy()))
+            out[layout] = res
+    for a, b in zip(out[lbm.LAYOUT_TWO_BUFFER], out[AA]):
+        for x_, y_ in zip(a, b):
+            assert np.array_equal(np.asarray(x_), np.asarray(y_))
+
+
+def test_aa_set_get_round_trip_across_parities():
+    n = 10
+    solid = _porous_solid(n, 5)
+    rng = np.random.default_rng(1)
+    f = (O.W[:, None] * (1.0 + 0.05 * rng.standard_normal((19, n ** 3))))
+    f[:, solid.reshape(-1)] = 0.0
+    f = f.reshape(-1)
+    with lbm.Lattice(n, n, n, 0.7, periodic=(True, True, True), layout=AA) as lat:
+        lat.set_flags(~solid)
+        lat.set_populations(f)
+        assert np.array_equal(lat.get_populations(), f)
+        lat.step(3)  # SWAPPED
+        g = lat.get_populations()
+        lat.set_populations(f)  # back to NATURAL
+        lat.step(3)
+        assert np.array_equal(lat.get_populations(), g)
+
+
+def test_aa_rejects_adjoint_and_coupling():
+    n = 8
+    with lbm.cavity(n, 0.7, 0.05, layout=AA) as lat:
+        with pytest.raises(lbm.LbmError, match="two-buffer"):
+            lat.set_coupling_cells(np.arange(4))
+        with pytest.raises(lbm.LbmError, match="two-buffer"):
+            lat.adjoint_pressure_drop(4, 2, 1, 6)
+        assert lat.tune(variant=0) == 0
