@@ -19,6 +19,7 @@
 
 #include <chrono>
 #include <cmath>
+#include <cinttypes>
 #include <cstdint>
 #include <cstdio>
 #include <stdexcept>
@@ -57,6 +58,11 @@ struct Configuration {
   double bodyForce[3] = {0.0, 0.0, 0.0};     // Guo forcing (periodic box)
   Precision precision = Precision::FP64;
   int device = 0;
+  double domainOffset[3] = {0.0, 0.0, 0.0};
+  // the plotter tag (LB/configurations/LatticeBoltzmannPlotterConfiguration.cpp:25-55): VTK files
+  // <vtkOutputName>_<iteration>.vtk every timestepsPerPlotting iterations; empty name = no plotting
+  std::string vtkOutputName;
+  int timestepsPerPlotting = 0;
 
   // LB/configurations/LatticeBoltzmannConfiguration.cpp:247-269: tau <-> viscosity_L
   void computeUndefinedValues() {
@@ -81,7 +87,17 @@ class RegularGridBlockState {
   double getDt() const { return dt_; }
   double getDensity() const { return density_; }
   int getNumberTimesteps() const { return numberTimesteps_; }
-  int getTimestep() const { return timestep_; }
+  int getTimestep() const { return timestep_; }              // iterations done
+  // BLB/RegularGridBlockState.cpp:253-270: with plotting on, returns whether this iteration plots
+  // and advances the plot counter
+  bool plotVTK() const { return !vtkFilename_.empty(); }
+  bool triggerPlotting() {
+    if (!plotVTK()) return false;
+    const bool plot = plotTimestep_ % timestepsPerPlotting_ == 0;
+    plotTimestep_++;
+    return plot;
+  }
+  const std::string& getVTKFilename() const { return vtkFilename_; }
   int64_t getLocalNumberOfCells() const { return cells_; }   // inner cells updated last iteration
   bool measurePerformance() const { return measure_; }
   double getLastMLUPS() const { return mlups_; }
@@ -90,9 +106,83 @@ class RegularGridBlockState {
   friend class Repository;
   double tau_ = 0, viscosityL_ = 0, dx_ = 0, dt_ = 0, density_ = 1, mlups_ = 0;
   int numberTimesteps_ = 0, timestep_ = 0;
+  int plotTimestep_ = 0, timestepsPerPlotting_ = 1;
+  std::string vtkFilename_;
   int64_t cells_ = 0;
   bool measure_ = false;
 };
+
+// Legacy-VTK text file of the per-cell fields, laid out as the reference's block VTK output
+// (BLB/mappings/RegularGrid2BlockVTKOutput.cpp:95-190 through
+// tarch/plotter/griddata/unstructured/vtk/VTKTextFileWriter.cpp:49-83): an unstructured grid of
+// voxels (cell type 11), one per fluid cell, visited block by block in the block store's vertex
+// order (6^3 blocks, x fastest; cells z, y, x inside a block); corner points numbered on first
+// use, corners of a voxel in x-fastest order; cell data density, velocity and block-id (the
+// linear block number, empty blocks counted), values written as float with six significant
+// digits.  rho and u are per lattice cell, u interleaved (3 per cell).
+inline void writeBlockVTK(const std::string& filename, const int n[3], double dx, const double offset[3],
+                          const std::vector<uint8_t>& fluid, const std::vector<double>& rho,
+                          const std::vector<double>& u) {
+  const int64_t ncell = (int64_t)n[0] * n[1] * n[2];
+  if ((int64_t)rho.size() != ncell || (int64_t)u.size() != 3 * ncell || (int64_t)fluid.size() != ncell)
+    throw std::runtime_error("writeBlockVTK: field sizes do not match the lattice");
+  int nb[3];
+  for (int d = 0; d < 3; d++) nb[d] = n[d] / LB_BLOCKSIZE + 1 + (n[d] % LB_BLOCKSIZE ? 1 : 0);
+  const int64_t vx = n[0] + 1, vy = n[1] + 1;
+  std::vector<int64_t> vnum((size_t)(vx * vy * (n[2] + 1)), -1);
+  std::string pts, cells, data_rho, data_u, data_id;
+  int64_t npts = 0, ncells = 0;
+  char buf[160];
+  for (int bz = 0; bz < nb[2]; bz++)
+    for (int by = 0; by < nb[1]; by++)
+      for (int bx = 0; bx < nb[0]; bx++) {
+        const int64_t block = bx + (int64_t)nb[0] * (by + (int64_t)nb[1] * bz);
+        for (int z = 0; z < LB_BLOCKSIZE; z++)
+          for (int y = 0; y < LB_BLOCKSIZE; y++)
+            for (int x = 0; x < LB_BLOCKSIZE; x++) {
+              const int g[3] = {LB_BLOCKSIZE * bx + x - 3, LB_BLOCKSIZE * by + y - 3, LB_BLOCKSIZE * bz + z - 3};
+              if (g[0] < 0 || g[1] < 0 || g[2] < 0 || g[0] >= n[0] || g[1] >= n[1] || g[2] >= n[2]) continue;
+              const int64_t i = g[0] + (int64_t)n[0] * (g[1] + (int64_t)n[1] * g[2]);
+              if (!fluid[(size_t)i]) continue;
+              int64_t corner[8];
+              for (int c = 0; c < 8; c++) {
+                const int cx = g[0] + (c & 1), cy = g[1] + ((c >> 1) & 1), cz = g[2] + (c >> 2);
+                int64_t& v = vnum[(size_t)(cx + vx * (cy + vy * cz))];
+                if (v < 0) {
+                  v = npts++;
+                  std::snprintf(buf, sizeof buf, "%g  %g  %g\n", offset[0] + cx * dx, offset[1] + cy * dx,
+                                offset[2] + cz * dx);
+                  pts += buf;
+                }
+                corner[c] = v;
+              }
+              std::snprintf(buf, sizeof buf, "8 %" PRId64 " %" PRId64 " %" PRId64 " %" PRId64 " %" PRId64 " %" PRId64
+                            " %" PRId64 " %" PRId64 " \n", corner[0], corner[1], corner[2], corner[3], corner[4],
+                            corner[5], corner[6], corner[7]);
+              cells += buf;
+              std::snprintf(buf, sizeof buf, "%g \n", (double)(float)rho[(size_t)i]);
+              data_rho += buf;
+              std::snprintf(buf, sizeof buf, "%g %g %g \n", (double)(float)u[(size_t)(3 * i)],
+                            (double)(float)u[(size_t)(3 * i + 1)], (double)(float)u[(size_t)(3 * i + 2)]);
+              data_u += buf;
+              std::snprintf(buf, sizeof buf, "%g \n", (double)(float)block);
+              data_id += buf;
+              ncells++;
+            }
+      }
+  FILE* fp = std::fopen(filename.c_str(), "w");
+  if (!fp) throw std::runtime_error("cannot open " + filename);
+  std::fprintf(fp, "# vtk DataFile Version 2.0\nMI355X block-LB cell fields\nASCII\n\n");
+  std::fprintf(fp, "DATASET UNSTRUCTURED_GRID\nPOINTS %" PRId64 " float\n\n%s\n\n", npts, pts.c_str());
+  std::fprintf(fp, "CELLS %" PRId64 " %" PRId64 "\n\n%s\n\n", ncells, 9 * ncells, cells.c_str());
+  std::fprintf(fp, "CELL_TYPES %" PRId64 "\n\n", ncells);
+  for (int64_t k = 0; k < ncells; k++) std::fputs("11\n", fp);
+  std::fprintf(fp, "\n\nCELL_DATA %" PRId64 "\n\n", ncells);
+  std::fprintf(fp, "SCALARS density float 1\nLOOKUP_TABLE default\n%s\n", data_rho.c_str());
+  std::fprintf(fp, "VECTORS velocity float\n%s\n", data_u.c_str());
+  std::fprintf(fp, "SCALARS block-id float 1\nLOOKUP_TABLE default\n%s\n\n\n", data_id.c_str());
+  if (std::fclose(fp) != 0) throw std::runtime_error("error writing " + filename);
+}
 
 // One 6^3 block as GridManagementService holds it (BLB/services/GridManagementService.h:74-86):
 // AoS pdf[cell*19 + q], cell = x + 6y + 36z.
@@ -108,6 +198,8 @@ class Repository {
  public:
   explicit Repository(const Configuration& cfg) : cfg_(cfg) {
     cfg_.computeUndefinedValues();
+    if (!cfg_.vtkOutputName.empty() && cfg_.timestepsPerPlotting < 1)
+      throw std::runtime_error("'timesteps-per-plotting' is smaller than one");
     lbm_config c{};
     c.nx = cfg_.numberOfGridPoints[0];
     c.ny = cfg_.numberOfGridPoints[1];
@@ -140,19 +232,39 @@ class Repository {
     state_.density_ = cfg_.density;
     state_.numberTimesteps_ = cfg_.numberOfTimesteps;
     state_.measure_ = cfg_.measurePerformance;
+    if (!cfg_.vtkOutputName.empty()) {
+      state_.vtkFilename_ = cfg_.vtkOutputName;
+      state_.timestepsPerPlotting_ = cfg_.timestepsPerPlotting;
+    }
+    fluid_ = inner;
+    // the block store's initial moments (GridManagementService.cpp:121-130)
+    heldRho_.assign((size_t)n_, 1.0);
+    heldU_.assign((size_t)n_ * 3, 0.0);
   }
   Repository(const Repository&) = delete;
   Repository& operator=(const Repository&) = delete;
   ~Repository() { terminate(); }
 
   void switchToRegularBlockSolverAdapter() { vtk_ = false; }
-  void switchToRegularBlockSolverAndVTKOutputAdapter(const std::string& basename) {
+  // the file name comes from the state (the configuration's vtkOutputName)
+  void switchToRegularBlockSolverAndVTKOutputAdapter() {
+    if (!state_.plotVTK()) throw std::runtime_error("no VTK output name configured");
     vtk_ = true;
-    vtkBase_ = basename;
   }
 
-  // one time step: beginIteration, the fused A/B/C sweeps, endIteration
+  // One time step: beginIteration, the fused A/B/C sweeps, endIteration.  With the VTK adapter the
+  // file shows what the reference's block store holds when its output mapping runs: the density
+  // and velocity its previous iteration computed before colliding (the state one step back; the
+  // initial rho = 1, u = 0 in the first iteration), named <vtkOutputName>_<iteration>.vtk.
   void iterate() {
+    if (state_.plotVTK()) {
+      if (vtk_) {
+        const int n3[3] = {cfg_.numberOfGridPoints[0], cfg_.numberOfGridPoints[1], cfg_.numberOfGridPoints[2]};
+        writeBlockVTK(state_.getVTKFilename() + "_" + std::to_string(state_.timestep_) + ".vtk", n3, cfg_.getDx(),
+                      cfg_.domainOffset, fluid_, heldRho_, heldU_);
+      }
+      getFields(heldRho_, heldU_);  // the moments this iteration's collision computes
+    }
     const auto t0 = std::chrono::steady_clock::now();
     check(lbm_step(ctx_, 1), ctx_, "lbm_step");
     check(lbm_synchronize(ctx_), ctx_, "lbm_synchronize");
@@ -162,7 +274,6 @@ class Repository {
     totalSeconds_ += s;
     iterations_++;
     state_.timestep_++;
-    if (vtk_) writeVTK(vtkBase_ + "_" + std::to_string(state_.timestep_) + ".vtk");
   }
 
   RegularGridBlockState& getRegularGridState() { return state_; }
@@ -239,23 +350,12 @@ class Repository {
   }
   void reduceMoments(double out[4]) { check(lbm_reduce_moments(ctx_, out), ctx_, "lbm_reduce_moments"); }
 
-  // legacy VTK (structured points) of rho and u per LB cell, the quantities the reference's
-  // VTK output mapping plots (BLB/mappings/RegularGrid2BlockVTKOutput.cpp:95-273)
+  // the current state's fields in the block VTK layout (writeBlockVTK)
   void writeVTK(const std::string& filename) {
     std::vector<double> rho, u;
     getFields(rho, u);
-    FILE* fp = std::fopen(filename.c_str(), "w");
-    if (!fp) throw std::runtime_error("cannot open " + filename);
-    const int nx = cfg_.numberOfGridPoints[0], ny = cfg_.numberOfGridPoints[1], nz = cfg_.numberOfGridPoints[2];
-    const double dx = cfg_.getDx();
-    std::fprintf(fp, "# vtk DataFile Version 2.0\nMI355X block-LB fields\nASCII\nDATASET STRUCTURED_POINTS\n");
-    std::fprintf(fp, "DIMENSIONS %d %d %d\nORIGIN %.17g %.17g %.17g\nSPACING %.17g %.17g %.17g\n", nx, ny, nz,
-                 0.5 * dx, 0.5 * dx, 0.5 * dx, dx, dx, dx);
-    std::fprintf(fp, "POINT_DATA %ld\nSCALARS density double 1\nLOOKUP_TABLE default\n", (long)n_);
-    for (int64_t i = 0; i < n_; i++) std::fprintf(fp, "%.17g\n", rho[i]);
-    std::fprintf(fp, "VECTORS velocity double\n");
-    for (int64_t i = 0; i < n_; i++) std::fprintf(fp, "%.17g %.17g %.17g\n", u[3 * i], u[3 * i + 1], u[3 * i + 2]);
-    std::fclose(fp);
+    const int n3[3] = {cfg_.numberOfGridPoints[0], cfg_.numberOfGridPoints[1], cfg_.numberOfGridPoints[2]};
+    writeBlockVTK(filename, n3, cfg_.getDx(), cfg_.domainOffset, fluid_, rho, u);
   }
 
   lbm_ctx* handle() { return ctx_; }
@@ -280,7 +380,8 @@ class Repository {
   int64_t n_ = 0;
   RegularGridBlockState state_;
   bool vtk_ = false;
-  std::string vtkBase_;
+  std::vector<uint8_t> fluid_;
+  std::vector<double> heldRho_, heldU_;
   double totalSeconds_ = 0.0;
   int64_t iterations_ = 0;
   std::vector<double> f_, rho_, u_;
@@ -309,7 +410,10 @@ class Runner {
   int runAsMaster(Repository& repo) {
     RegularGridBlockState& st = repo.getRegularGridState();
     for (int t = 0; t < st.getNumberTimesteps(); t++) {
-      repo.switchToRegularBlockSolverAdapter();
+      if (st.triggerPlotting())
+        repo.switchToRegularBlockSolverAndVTKOutputAdapter();
+      else
+        repo.switchToRegularBlockSolverAdapter();
       repo.iterate();
       // LB/PerformanceMeasurement.cpp:45-66 (MLUPS = inner cells / (1e6 s), wall clock here)
       if (st.measurePerformance()) std::printf("MLUPS: %g\n", st.getLastMLUPS());

@@ -1,8 +1,9 @@
 // Drives the reference-shaped C++ API (madpac_amd/csrc/peano_blocklb.h) like reference code drives
 // the Peano block-LB: configure, run the time loop, read blocks through the block-store view.
-// Usage: test_peano_api cavity|channel|channel-parabolic nx ny nz tau u steps out.bin
-// (writes the SoA populations, FP64).  Exit code 0 = the block view agrees with the dense fields
-// and the flag layout of the reference.
+// Usage: test_peano_api cavity|channel|channel-parabolic nx ny nz tau u steps out.bin [vtkname every]
+// (writes the SoA populations, FP64; with vtkname, VTK field files every `every` iterations as the
+// reference's plotter tag does).  Exit code 0 = the block view agrees with the dense fields and the
+// flag layout of the reference.
 #include <cstdio>
 #include <cstdlib>
 #include <string>
@@ -13,8 +14,9 @@
 using namespace madpac_amd::blocklatticeboltzmann;
 
 int main(int argc, char** argv) {
-  if (argc != 9) {
-    std::fprintf(stderr, "usage: %s cavity|channel|channel-parabolic nx ny nz tau u steps out.bin\n", argv[0]);
+  if (argc != 9 && argc != 11) {
+    std::fprintf(stderr, "usage: %s cavity|channel|channel-parabolic nx ny nz tau u steps out.bin [vtkname every]\n",
+                 argv[0]);
     return 2;
   }
   Configuration cfg;
@@ -26,6 +28,10 @@ int main(int argc, char** argv) {
   cfg.characteristicVelocityL = std::atof(argv[6]);
   cfg.numberOfTimesteps = std::atoi(argv[7]);
   const char* outpath = argv[8];
+  if (argc == 11) {
+    cfg.vtkOutputName = argv[9];
+    cfg.timestepsPerPlotting = std::atoi(argv[10]);
+  }
   if (scenario == "cavity") {
     cfg.scenario = Scenario::LidDrivenCavity;
   } else if (scenario == "channel" || scenario == "channel-parabolic") {

@@ -16,8 +16,12 @@ its outputs as small ``.npz`` fixtures next to this file:
 * ``channel_*.npz`` -- the reference's default channel (no-slip y/z walls, velocity inlet on
   x-, pressure outlet on x+; uniform or parabolic inlet profile), full FP64 populations over
   the fluid box after ``steps`` time steps.
+* ``vtk_*.npz`` -- the legacy-VTK field files the reference writes with its plotter tag
+  (``<plotter vtk-outputname=... timesteps-per-plotting=...>``), read back with
+  ``tests/vtk_io.py``: per plotted iteration the points, voxels, cell types, density, velocity
+  and block ids, plus the section header lines.
 
-Usage:  python tests/golden/make_fixtures.py [--skip-64]
+Usage:  python tests/golden/make_fixtures.py [--skip-64] [--only-vtk]
 """
 from __future__ import annotations
 
@@ -111,6 +115,42 @@ def run_reference(dims, tau, ulid, steps, scenario="lid-driven-cavity", paraboli
     return np.ascontiguousarray(np.moveaxis(dense[box], -1, 0))  # [q, z, y, x]
 
 
+def run_reference_vtk(dims, tau, u, steps, every, scenario="lid-driven-cavity", parabolic=None):
+    """Run the reference with VTK plotting every `every` iterations; return {iteration: parsed file}."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from vtk_io import read_vtk
+    nx, ny, nz = dims
+    nb = [blocks_per_dim(n) for n in dims]
+    extra = "" if parabolic is None else f' use-parabolic-profile="{"yes" if parabolic else "no"}"'
+    xml = XML.format(nx=nx, ny=ny, nz=nz, lx=nx / ny, ly=1.0, lz=nz / ny, tau=tau, ulid=u, steps=steps,
+                     scenario=scenario, extra=extra)
+    tag = f'    <plotter vtk-outputname="ref" timesteps-per-plotting="{every}" />\n'
+    xml = xml.replace("    <collision-model", tag + "    <collision-model")
+    out = {}
+    with tempfile.TemporaryDirectory() as td:
+        with open(os.path.join(td, "run.xml"), "w") as fh:
+            fh.write(xml)
+        subprocess.run([HARNESS, "run.xml", str(nb[0] * nb[1] * nb[2]), "dump.bin"], check=True, cwd=td,
+                       stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+        for k in range(0, steps, every):
+            path = os.path.join(td, f"ref_{k}.vtk")
+            d = read_vtk(path)
+            d["first_line"] = open(path).readline().rstrip("\n")
+            out[k] = d
+    return out
+
+
+def save_vtk_fixture(name, files, **meta):
+    arrays = {}
+    for k, d in files.items():
+        for key in ("points", "cells", "types", "density", "velocity", "block-id"):
+            arrays[f"{key.replace('-', '_')}_{k}"] = d[key]
+    first = next(iter(files.values()))
+    arrays["headers"] = np.array([first["first_line"]] + first["headers"])
+    arrays["plotted"] = np.array(sorted(files), np.int64)
+    np.savez_compressed(os.path.join(HERE, name), **arrays, **meta)
+
+
 def run_cavity(n, tau, ulid, steps):
     return run_reference((n, n, n), tau, ulid, steps)
 
@@ -163,8 +203,21 @@ def moments(f):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--skip-64", action="store_true")
+    ap.add_argument("--only-vtk", action="store_true")
     args = ap.parse_args()
     subprocess.run(["make", "-s", "-f", os.path.join(ROOT, "oracle", "ref", "Makefile"), "-j8"], check=True)
+
+    # VTK field output: cavity 8^3 every iteration; parabolic channel 10x7x9 every second one
+    files = run_reference_vtk((8, 8, 8), 0.6, 0.05, 3, 1)
+    save_vtk_fixture("vtk_cavity_8x8x8_t3_every1.npz", files, dims=np.array((8, 8, 8), np.int64), tau=np.float64(0.6),
+                     u=np.float64(0.05), steps=np.int64(3), every=np.int64(1), scenario=np.array("cavity"))
+    files = run_reference_vtk((10, 7, 9), 0.55, 0.03, 6, 2, scenario="default-channel", parabolic=True)
+    save_vtk_fixture("vtk_channel_10x7x9_t6_every2.npz", files, dims=np.array((10, 7, 9), np.int64),
+                     tau=np.float64(0.55), u=np.float64(0.03), steps=np.int64(6), every=np.int64(2),
+                     scenario=np.array("channel-parabolic"))
+    print("vtk", file=sys.stderr)
+    if args.only_vtk:
+        return
 
     for seed, tau in ((1, 0.6), (2, 0.93)):
         d = cellkernel_fixture(seed, tau)

@@ -43,3 +43,27 @@ def test_cpp_runner_channel_matches_reference(name, tmp_path):
     ref = d["f"].reshape(-1)
     tol = channel_tolerance(d)
     assert np.max(np.abs(f - ref)) <= tol * np.max(np.abs(ref))
+
+
+@pytest.mark.parametrize("name", ["vtk_cavity_8x8x8_t3_every1.npz", "vtk_channel_10x7x9_t6_every2.npz"])
+def test_cpp_runner_vtk_output_matches_reference(name, tmp_path):
+    # the runner's plotter path end to end (triggerPlotting, VTK adapter, the moments the block
+    # store holds when the output mapping runs) against the reference's own VTK files; bars as in
+    # tests/test_vtk.py
+    from vtk_io import assert_same_vtk, fixture_file, read_vtk
+    if not os.path.exists(EXE):
+        pytest.fail(f"{EXE} is not built (run __graft_entry__.build())")
+    d = golden(name)
+    print(f"[gpu] C++ API VTK output {name}", flush=True)
+    scen = "cavity" if str(d["scenario"]) == "cavity" else "channel-parabolic"
+    base = str(tmp_path / "mine")
+    r = subprocess.run([EXE, scen] + [str(int(v)) for v in d["dims"]] +
+                       [repr(float(d["tau"])), repr(float(d["u"])), str(int(d["steps"])), str(tmp_path / "f.bin"),
+                        base, str(int(d["every"]))], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, (r.returncode, r.stdout, r.stderr)
+    written = sorted(int(p.name[5:-4]) for p in tmp_path.glob("mine_*.vtk"))
+    assert written == [int(k) for k in d["plotted"]]
+    exact = scen == "cavity"
+    for k in d["plotted"]:
+        assert_same_vtk(read_vtk(f"{base}_{int(k)}.vtk"), fixture_file(d, k), [str(h) for h in d["headers"]],
+                        0.0 if exact else 2e-6, 0.0 if exact else 1e-15)
