@@ -459,10 +459,17 @@ extern "C" int lbm_adjoint_pressure_drop(lbm_ctx* c, int nsteps, int snapshots, 
   ACHK(hipEventSynchronize(c->ev1));
   ACHK(hipEventElapsedTime(&rev_ms, c->ev0, c->ev1));
 
-  ACHK(hipMemcpyAsync(grad_alpha, a->galpha, (size_t)N * sizeof(double), hipMemcpyDeviceToHost, st));
+  std::vector<double> gstored((size_t)N);
+  ACHK(hipMemcpyAsync(gstored.data(), a->galpha, (size_t)N * sizeof(double), hipMemcpyDeviceToHost, st));
   // leave the context at the state after nsteps
   ACHK(hipMemcpyAsync(c->buf[c->cur], a->final_state, sbytes, hipMemcpyDeviceToDevice, st));
   ACHK(hipStreamSynchronize(st));
+  // stored (bricked) order -> the caller's row-major order
+  const DenseGeom& g = c->g;
+  for (int z = 0; z < g.nz; z++)
+    for (int y = 0; y < g.ny; y++)
+      for (int x = 0; x < g.nx; x++)
+        grad_alpha[(int64_t)x + (int64_t)g.nx * ((int64_t)y + (int64_t)g.ny * z)] = gstored[(size_t)stored_index(g, x, y, z)];
   *J = Jv;
   if (stats) {
     stats->snapshots_used = nslots - 1;
@@ -472,7 +479,7 @@ extern "C" int lbm_adjoint_pressure_drop(lbm_ctx* c, int nsteps, int snapshots, 
     stats->forward_ms = fwd_ms;
     stats->reverse_ms = rev_ms;
   }
-  for (int64_t i = 0; i < N; i++)
+  for (int64_t i = 0; i < g.Nu; i++)
     if (!std::isfinite(grad_alpha[i])) return adj_fail(c, LBM_ERR_NUMERIC, "non-finite gradient");
 #undef ACHK
   return LBM_OK;

@@ -212,12 +212,22 @@ int lbm_set_boundary_velocity(lbm_ctx* c, const double* u) {
     c->d_bcu = nullptr;
     return LBM_OK;
   }
-  const size_t n = 3 * (size_t)c->g.Nu;
+  const lbm::DenseGeom& g = c->g;
+  const size_t n = 3 * (size_t)g.Nu;
   for (size_t i = 0; i < n; i++)
     if (!std::isfinite(u[i])) return fail(c, LBM_ERR_ARG, "boundary velocities must be finite");
-  if (!c->d_bcu && hipMalloc(&c->d_bcu, n * sizeof(double)) != hipSuccess)
+  // the kernels read the velocities by stored (bricked) cell
+  std::vector<double> dev(3 * (size_t)g.N, 0.0);
+  for (int z = 0; z < g.nz; z++)
+    for (int y = 0; y < g.ny; y++)
+      for (int x = 0; x < g.nx; x++) {
+        const int64_t r = (int64_t)x + (int64_t)g.nx * ((int64_t)y + (int64_t)g.ny * z);
+        const int64_t p = lbm::stored_index(g, x, y, z);
+        for (int d = 0; d < 3; d++) dev[(size_t)(3 * p + d)] = u[3 * r + d];
+      }
+  if (!c->d_bcu && hipMalloc(&c->d_bcu, dev.size() * sizeof(double)) != hipSuccess)
     return fail(c, LBM_ERR_NOMEM, "boundary velocity allocation failed");
-  HIPCHK(c, hipMemcpyAsync(c->d_bcu, u, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->d_bcu, dev.data(), dev.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return LBM_OK;
 }
@@ -329,7 +339,7 @@ int lbm_set_porosity(lbm_ctx* c, const double* alpha) {
   if (c->couple_on) return fail(c, LBM_ERR_STATE, "coupled cells use the plain collision: no porosity");
   if (!c->alpha && hipMalloc(&c->alpha, (size_t)c->g.N * c->esize) != hipSuccess)
     return fail(c, LBM_ERR_NOMEM, "porosity allocation failed");
-  // the kernels index the porosity by padded cell; halo cells get 0
+  // the kernels index the porosity by stored (bricked) cell; padding cells get 0
   const lbm::DenseGeom& g = c->g;
   std::vector<double> a64;
   std::vector<float> a32;
@@ -339,7 +349,7 @@ int lbm_set_porosity(lbm_ctx* c, const double* alpha) {
     for (int y = 0; y < g.ny; y++)
       for (int x = 0; x < g.nx; x++) {
         const int64_t u = (int64_t)x + (int64_t)g.nx * ((int64_t)y + (int64_t)g.ny * z);
-        const int64_t p = (int64_t)(x + g.hx) + (int64_t)g.NX * ((int64_t)(y + g.hy) + (int64_t)g.NY * (z + g.hz));
+        const int64_t p = lbm::stored_index(g, x, y, z);
         if (c->fp32) a32[p] = (float)alpha[u];
         else a64[p] = alpha[u];
       }

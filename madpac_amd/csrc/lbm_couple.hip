@@ -30,12 +30,16 @@ struct CoupleArgs {
   double omega;           // 1 / tau_eff (BlockBGK)
 };
 
+// position of stored (bricked) cell i
 __device__ __forceinline__ CellPos pos_of(const DenseGeom& g, int64_t i) {
   CellPos c;
   c.i = i;
-  c.x = (int)(i % g.nx);
-  c.y = (int)((i / g.nx) % g.ny);
-  c.z = (int)(i / ((int64_t)g.nx * g.ny));
+  const int64_t brick = i >> 6;
+  const int l = (int)(i & 63);
+  c.x = 4 * (int)(brick % g.bnx) + (l & 3);
+  c.y = 4 * (int)((brick / g.bnx) % g.bny) + ((l >> 2) & 3);
+  c.z = 4 * (int)(brick / ((int64_t)g.bnx * g.bny)) + (l >> 4);
+  c.r = (int64_t)c.x + (int64_t)g.nx * ((int64_t)c.y + (int64_t)g.ny * c.z);
   c.valid = true;
   return c;
 }
@@ -239,7 +243,15 @@ int lbm_set_coupling_cells(lbm_ctx* c, const int64_t* cells, int64_t n) {
     lbm_coupling_release(c);
     return cfail(c, LBM_ERR_NOMEM, "coupling buffers allocation failed");
   }
-  if (hipMemcpy(c->d_ccells, cells, (size_t)n * sizeof(int64_t), hipMemcpyHostToDevice) != hipSuccess) {
+  // the kernels address the stored (bricked) cells
+  const lbm::DenseGeom& g = c->g;
+  std::vector<int64_t> stored((size_t)n);
+  for (int64_t k = 0; k < n; k++) {
+    const int64_t r = cells[k];
+    const int x = (int)(r % g.nx), y = (int)((r / g.nx) % g.ny), z = (int)(r / ((int64_t)g.nx * g.ny));
+    stored[(size_t)k] = lbm::stored_index(g, x, y, z);
+  }
+  if (hipMemcpy(c->d_ccells, stored.data(), (size_t)n * sizeof(int64_t), hipMemcpyHostToDevice) != hipSuccess) {
     lbm_coupling_release(c);
     return cfail(c, LBM_ERR_HIP, "coupling cell upload failed");
   }

@@ -4,10 +4,11 @@
 // cell; it gathers the 19 incoming populations from the SoA state (population q of cell
 // x - c_q, or the reflected population 18-q of its own cell when x - c_q is not fluid), then
 // collides, applies the moving-wall term and writes its 19 post-collision populations back to
-// its own slots of the other buffer.  Reads of one population plane are contiguous along x, so
-// a wavefront of 64 lanes reads 64 consecutive values (512 B in FP64, 256 B in FP32) offset
-// by at most one element; writes are aligned.  Nothing is staged in LDS: each value is read
-// once and written once, and the neighbour rows a workgroup re-reads sit in L2.
+// its own slots of the other buffer.  A wavefront owns one 4 x 4 x 4 brick, stored as 64
+// consecutive values per population plane (512 B in FP64, 256 B in FP32): writes are whole
+// aligned lines, and a shifted read touches the brick's own lines plus the facing layer of one
+// to three neighbour bricks, which their own wavefronts read at about the same time from L2.
+// Nothing is staged in LDS: each value is read once and written once.
 //
 // The reference's own update is the three-phase swap algorithm
 // (BLB/mappings/RegularGrid2RegularBlockSolver.cpp:83-248 with
@@ -22,14 +23,12 @@ DenseGeom make_geom(int nx, int ny, int nz, const int periodic[3]) {
   DenseGeom g;
   g.nx = nx; g.ny = ny; g.nz = nz;
   g.px = periodic[0] ? 1 : 0; g.py = periodic[1] ? 1 : 0; g.pz = periodic[2] ? 1 : 0;
-  g.N = (int64_t)nx * ny * nz;
-  g.nbx = (nx + BX - 1) / BX;
-  g.nby = (ny + BY - 1) / BY;
-  g.nblocks = (int64_t)g.nbx * g.nby * nz;
+  g.bnx = (nx + 3) / 4; g.bny = (ny + 3) / 4; g.bnz = (nz + 3) / 4;
+  const int64_t bricks = (int64_t)g.bnx * g.bny * g.bnz;
+  g.N = bricks * 64;
+  g.Nu = (int64_t)nx * ny * nz;
+  g.nblocks = (bricks + BY - 1) / BY;
   g.nblocks_padded = (g.nblocks + NXCD - 1) / NXCD * NXCD;
-  g.hx = g.hy = g.hz = 0;
-  g.NX = nx; g.NY = ny; g.NZ = nz;
-  g.Nu = g.N;
   g.layout = LAYOUT_PULL;
   return g;
 }
@@ -40,12 +39,12 @@ __global__ __launch_bounds__(BX * BY) void k_build_words(DenseGeom g, const uint
                                                          uint32_t* __restrict__ words) {
   const CellPos c = map_cell(g);
   if (!c.valid) return;
-  if (!inner[c.i]) {
+  if (!inner[c.r]) {
     words[c.i] = 0u;
     return;
   }
-  // class ids index an 8-entry table; the host rejects larger ids, the mask keeps it in range
-  uint32_t w = FLUID | ((cls ? (uint32_t)cls[c.i] & CLASS_BITS : 0u) << CLASS_SHIFT);
+  // class ids index a MAX_CLASSES-entry table; the host rejects larger ids, the mask keeps it in range
+  uint32_t w = FLUID | ((cls ? (uint32_t)cls[c.r] & CLASS_BITS : 0u) << CLASS_SHIFT);
 #pragma unroll
   for (int q = 0; q < Q; q++) {
     if (q == REST) continue;
@@ -178,17 +177,17 @@ __global__ __launch_bounds__(BX * BY) void k_dense_pull(DenseGeom g, const T* __
   const CellPos c = map_cell(g);
   if (!c.valid) return;
   const uint32_t w = words[c.i];
-  const int64_t N = g.N;
+  const int64_t Nu = g.Nu;
   if (!(w & FLUID)) {
 #pragma unroll
-    for (int q = 0; q < Q; q++) f[(int64_t)q * N + c.i] = 0.0;
+    for (int q = 0; q < Q; q++) f[(int64_t)q * Nu + c.r] = 0.0;
     return;
   }
   const NbrOff o = nbr_offsets(g, c);
 #pragma unroll
   for (int q = 0; q < Q; q++) {
     const T v = src[state_index(g, q, w, c.i, o)];
-    f[(int64_t)q * N + c.i] = (sizeof(T) == 8) ? (double)v : (double)v + W64(q);
+    f[(int64_t)q * Nu + c.r] = (sizeof(T) == 8) ? (double)v : (double)v + W64(q);
   }
 }
 
@@ -201,10 +200,10 @@ __global__ __launch_bounds__(BX * BY) void k_dense_unpull(DenseGeom g, const dou
   const uint32_t w = words[c.i];
   if (!(w & FLUID)) return;
   const NbrOff o = nbr_offsets(g, c);
-  const int64_t N = g.N;
+  const int64_t Nu = g.Nu;
 #pragma unroll
   for (int q = 0; q < Q; q++) {
-    const double v = f[(int64_t)q * N + c.i];
+    const double v = f[(int64_t)q * Nu + c.r];
     dst[state_index(g, q, w, c.i, o)] = (sizeof(T) == 8) ? (T)v : (T)(v - W64(q));
   }
 }
@@ -219,9 +218,8 @@ __global__ __launch_bounds__(BX * BY) void k_dense_init_eq(DenseGeom g, const do
   const uint32_t w = words[c.i];
   if (!(w & FLUID)) return;
   const NbrOff o = nbr_offsets(g, c);
-  const int64_t N = g.N;
-  const double r = rho ? rho[c.i] : 1.0;
-  const double u0 = u ? u[3 * c.i] : 0.0, u1 = u ? u[3 * c.i + 1] : 0.0, u2 = u ? u[3 * c.i + 2] : 0.0;
+  const double r = rho ? rho[c.r] : 1.0;
+  const double u0 = u ? u[3 * c.r] : 0.0, u1 = u ? u[3 * c.r + 1] : 0.0, u2 = u ? u[3 * c.r + 2] : 0.0;
   double uu = u0 * u0;
   uu += u1 * u1;
   uu += u2 * u2;
@@ -243,8 +241,8 @@ __global__ __launch_bounds__(BX * BY) void k_dense_fields(DenseGeom g, const T* 
   if (!c.valid) return;
   const uint32_t w = words[c.i];
   if (!(w & FLUID)) {
-    rho[c.i] = 0.0;
-    u[3 * c.i] = u[3 * c.i + 1] = u[3 * c.i + 2] = 0.0;
+    rho[c.r] = 0.0;
+    u[3 * c.r] = u[3 * c.r + 1] = u[3 * c.r + 2] = 0.0;
     return;
   }
   const NbrOff o = nbr_offsets(g, c);
@@ -262,10 +260,10 @@ __global__ __launch_bounds__(BX * BY) void k_dense_fields(DenseGeom g, const T* 
     if (CZ(q) == -1) b2 -= fv;
   }
   const double inv = 1.0 / r;
-  rho[c.i] = r;
-  u[3 * c.i] = b0 * inv;
-  u[3 * c.i + 1] = b1 * inv;
-  u[3 * c.i + 2] = b2 * inv;
+  rho[c.r] = r;
+  u[3 * c.r] = b0 * inv;
+  u[3 * c.r + 1] = b1 * inv;
+  u[3 * c.r + 2] = b2 * inv;
 }
 
 // ------------------------------------------------------------------------------------------
@@ -399,7 +397,7 @@ __global__ __launch_bounds__(256) void k_plane_rho(DenseGeom g, const T* __restr
       c.x = k == 0 ? xin : xout;
       c.y = y;
       c.z = z;
-      c.i = (int64_t)c.x + (int64_t)g.nx * ((int64_t)y + (int64_t)g.ny * z);
+      c.i = stored_index(g, c.x, y, z);
       const uint32_t w = words[c.i];
       if (w & FLUID) {
         const NbrOff o = nbr_offsets(g, c);

@@ -6,12 +6,14 @@
 
 namespace lbm {
 
-// XCD-aware block mapping.  Workgroups are dispatched round-robin over the 8 XCDs, so the
-// linear id is remapped to give each XCD one contiguous range of blocks (a stack of z-planes):
-// the y- and z-neighbour rows a block pulls from were then mostly fetched into the same L2.
+// XCD-aware workgroup mapping.  Workgroups are dispatched round-robin over the 8 XCDs, so the
+// linear id is remapped to give each XCD one contiguous range of bricks (a slab of brick layers):
+// the neighbour bricks a workgroup pulls from were then mostly fetched into the same L2.  Lane
+// threadIdx.x of wavefront threadIdx.y owns cell threadIdx.x of brick 4 * workgroup + threadIdx.y.
 struct CellPos {
   int x, y, z;
-  int64_t i;
+  int64_t i;   // stored (bricked) index
+  int64_t r;   // row-major user index x + nx * (y + ny * z)
   bool valid;
 };
 
@@ -20,14 +22,18 @@ __device__ __forceinline__ CellPos map_cell(const DenseGeom& g) {
   const int64_t bid = blockIdx.x;
   const int64_t per = g.nblocks_padded / NXCD;
   const int64_t b = (bid % NXCD) * per + bid / NXCD;
-  const int bx = (int)(b % g.nbx);
-  const int64_t r = b / g.nbx;
-  const int by = (int)(r % g.nby);
-  c.z = (int)(r / g.nby);
-  c.x = bx * BX + threadIdx.x;
-  c.y = by * BY + threadIdx.y;
-  c.valid = (b < g.nblocks) && c.x < g.nx && c.y < g.ny;
-  c.i = (int64_t)c.x + (int64_t)g.nx * ((int64_t)c.y + (int64_t)g.ny * c.z);
+  const int64_t brick = b * BY + threadIdx.y;
+  const int l = threadIdx.x;
+  const int bx = (int)(brick % g.bnx);
+  const int64_t rest = brick / g.bnx;
+  const int by = (int)(rest % g.bny);
+  const int64_t bz = rest / g.bny;
+  c.x = 4 * bx + (l & 3);
+  c.y = 4 * by + ((l >> 2) & 3);
+  c.z = (int)(4 * bz + (l >> 4));
+  c.valid = bz < g.bnz && c.x < g.nx && c.y < g.ny && c.z < g.nz;
+  c.i = brick * 64 + l;
+  c.r = (int64_t)c.x + (int64_t)g.nx * ((int64_t)c.y + (int64_t)g.ny * c.z);
   return c;
 }
 
@@ -39,15 +45,25 @@ struct NbrOff {
   int64_t xm, xp, ym, yp, zm, zp;
 };
 
+// The stored index is a sum of one term per coordinate (stored_index), so a neighbour's index is
+// the cell's plus the change of each term.
+__device__ __forceinline__ int64_t brick_term_x(int x) { return (int64_t)((x >> 2) << 6) + (x & 3); }
+__device__ __forceinline__ int64_t brick_term_y(const DenseGeom& g, int y) {
+  return (int64_t)(y >> 2) * 64 * g.bnx + ((y & 3) << 2);
+}
+__device__ __forceinline__ int64_t brick_term_z(const DenseGeom& g, int z) {
+  return (int64_t)(z >> 2) * 64 * g.bnx * g.bny + ((z & 3) << 4);
+}
+
 __device__ __forceinline__ NbrOff nbr_offsets(const DenseGeom& g, const CellPos& c) {
   NbrOff o;
-  const int64_t sy = g.nx, sz = (int64_t)g.nx * g.ny;
-  o.xm = c.x == 0 ? (g.px ? (int64_t)(g.nx - 1) : 0) : -1;
-  o.xp = c.x == g.nx - 1 ? (g.px ? -(int64_t)(g.nx - 1) : 0) : 1;
-  o.ym = c.y == 0 ? (g.py ? (int64_t)(g.ny - 1) * sy : 0) : -sy;
-  o.yp = c.y == g.ny - 1 ? (g.py ? -(int64_t)(g.ny - 1) * sy : 0) : sy;
-  o.zm = c.z == 0 ? (g.pz ? (int64_t)(g.nz - 1) * sz : 0) : -sz;
-  o.zp = c.z == g.nz - 1 ? (g.pz ? -(int64_t)(g.nz - 1) * sz : 0) : sz;
+  const int64_t tx = brick_term_x(c.x), ty = brick_term_y(g, c.y), tz = brick_term_z(g, c.z);
+  o.xm = c.x == 0 ? (g.px ? brick_term_x(g.nx - 1) - tx : 0) : brick_term_x(c.x - 1) - tx;
+  o.xp = c.x == g.nx - 1 ? (g.px ? -tx : 0) : brick_term_x(c.x + 1) - tx;
+  o.ym = c.y == 0 ? (g.py ? brick_term_y(g, g.ny - 1) - ty : 0) : brick_term_y(g, c.y - 1) - ty;
+  o.yp = c.y == g.ny - 1 ? (g.py ? -ty : 0) : brick_term_y(g, c.y + 1) - ty;
+  o.zm = c.z == 0 ? (g.pz ? brick_term_z(g, g.nz - 1) - tz : 0) : brick_term_z(g, c.z - 1) - tz;
+  o.zp = c.z == g.nz - 1 ? (g.pz ? -tz : 0) : brick_term_z(g, c.z + 1) - tz;
   return o;
 }
 

@@ -10,9 +10,16 @@
 
 namespace lbm {
 
-// One lattice stored as SoA populations s[q*N + i], i = x + nx*(y + ny*z).  The stored state is
-// the post-collision state of the last step; the reference's post-streaming state is its pull
-// (see k_dense_pull).  Blocks are BX x BY cells of one z-plane.
+// One lattice stored as SoA populations s[q*N + i] over bricks of 4 x 4 x 4 cells: the 64 cells
+// of a brick are consecutive (x fastest, then y, then z) and bricks follow in x-fastest order, so
+// i = 64 * (bx + bnx * (by + bny * bz)) + (x & 3) + 4 * (y & 3) + 16 * (z & 3).  One wavefront
+// updates one brick.  A 128-byte line then holds a compact 4 x 4 x 1 (FP64) or 4 x 4 x 2 (FP32)
+// cell group instead of a row segment, so far fewer lines straddle a solid surface and get
+// fetched for a few fluid cells.  The lattice is padded to whole bricks (padding cells are never
+// fluid and never touched).  Arrays that cross the C-ABI stay in the row-major user order
+// r = x + nx * (y + ny * z); the kernels that read or write them index both.
+// The stored state is the post-collision state of the last step; the reference's post-streaming
+// state is its pull (see k_dense_pull).  A workgroup is BY bricks of BX = 64 cells.
 constexpr int BX = 64;
 constexpr int BY = 4;
 constexpr int NXCD = 8;  // MI355X: 8 XCDs, each with its own L2
@@ -20,18 +27,19 @@ constexpr int NXCD = 8;  // MI355X: 8 XCDs, each with its own L2
 struct DenseGeom {
   int nx, ny, nz;
   int px, py, pz;            // periodic flags
-  int64_t N;                 // cells
-  int nbx, nby;              // blocks along x and y
-  int64_t nblocks;           // real blocks
+  int bnx, bny, bnz;         // bricks per dimension
+  int64_t N;                 // stored cells (whole bricks)
+  int64_t Nu;                // lattice cells nx * ny * nz (the row-major user arrays)
+  int64_t nblocks;           // workgroups holding a brick
   int64_t nblocks_padded;    // rounded up to a multiple of NXCD
-  // The stored lattice has no halo, so the user layout (nx, ny, nz; Nu cells) and the stored
-  // layout (NX, NY, NZ; N cells, offset hx, hy, hz) coincide; host code addresses both through
-  // these fields.
-  int hx, hy, hz;
-  int NX, NY, NZ;
-  int64_t Nu;
   int layout;                // how the stored state maps to the reference state (LAYOUT_*)
 };
+
+// stored index of cell (x, y, z)
+__host__ __device__ inline int64_t stored_index(const DenseGeom& g, int x, int y, int z) {
+  return ((((int64_t)(z >> 2) * g.bny + (y >> 2)) * g.bnx + (x >> 2)) << 6) + ((z & 3) << 4) + ((y & 3) << 2) +
+         (x & 3);
+}
 
 // Stored-state layouts.  PULL: two buffers, post-collision values in the cell's own slots, so the
 // reference population q of cell x is s_q(x - c_q).  AA (one buffer, in place): NATURAL holds the
