@@ -50,6 +50,7 @@ def run_dense(lbm, solid, n, tau, prec, steps, warmup, g, layout=0):
     lat.set_flags(~solid)
     lat.init_equilibrium()
     nfluid = lat.fluid_cells
+    order = "bricks" if lat.cell_order == lbm.CELL_ORDER_BRICKS else "rows"
     variant = lat.tune(10)  # load schedule of the step kernel (bit-identical results either way)
     lat.step(warmup)
     lat.synchronize()
@@ -74,6 +75,7 @@ def run_dense(lbm, solid, n, tau, prec, steps, warmup, g, layout=0):
         "fluid_cells": nfluid,
         "mass": m[0],
         "variant": schedule,
+        "cell_order": order,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "algorithmic_bytes_per_launch": bytes_per_launch},
@@ -222,10 +224,11 @@ def main():
                                f"(seed {wl.DEFAULT_SEED}) to porosity 0.8, tau={args.tau}, pull-form fused "
                                f"collide+stream", "cells": n ** 3, "fluid_cells": r64["fluid_cells"],
                    "mlups_counts": "fluid cells",
-                   "step_variant": {"fp64": r64["variant"], "fp32s": r32["variant"]}},
+                   "step_variant": {"fp64": r64["variant"], "fp32s": r32["variant"]},
+                   "cell_order": {"fp64": r64["cell_order"], "fp32s": r32["cell_order"]}},
         "roofline": r64["roofline"],
         "fp32_shifted": {"value": round(r32["mlups"], 1), "unit": "MLUPS", "dtype": "f32",
-                         "arithmetic": "FP32 state and traffic, FP64 collision registers",
+                         "arithmetic": "FP32 state and traffic; FP64 moments, equilibrium and relaxation product",
                          "ms_per_step": round(r32["ms_per_step"], 4), "roofline": r32["roofline"]},
         "fp32_shifted_f32_arithmetic": {"value": round(res["fp32s_f32math"]["mlups"], 1), "unit": "MLUPS",
                                         "dtype": "f32", "ms_per_step": round(res["fp32s_f32math"]["ms_per_step"], 4),

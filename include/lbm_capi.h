@@ -40,8 +40,9 @@ enum lbm_status {
 
 enum lbm_precision {
   LBM_FP64 = 0,             /* FP64 populations, FP64 collision (reference arithmetic order) */
-  LBM_FP32_SHIFTED = 1,     /* FP32 populations stored as f - w_q (FP32 state and traffic); the
-                               collision arithmetic runs in FP64 registers and rounds once at the store */
+  LBM_FP32_SHIFTED = 1,     /* FP32 populations stored as f - w_q (FP32 state and traffic); moments,
+                               equilibrium and the relaxation product in FP64, differences and the
+                               store in FP32 */
   LBM_FP32_SHIFTED_F32 = 2  /* FP32 populations as above with FP32 collision arithmetic: persistent
                                rounding of the equilibrium biases the momentum at low Mach */
 };
@@ -56,7 +57,16 @@ typedef struct {
   double force[3];       /* body acceleration for Guo forcing; all zero disables it */
   int device;            /* HIP device ordinal */
   int layout;            /* enum lbm_layout (0 = two buffers) */
+  int cell_order;        /* enum lbm_cell_order (0 = automatic) */
 } lbm_config;
+
+/* Order of the cells in device memory; the C-ABI always exchanges row-major arrays.  ROWS:
+ * x + nx (y + ny z).  BRICKS: 4 x 4 x 4 bricks of consecutive cells, one wavefront per brick: far
+ * fewer 128-byte lines straddle solid surfaces (porous 256^3: FP32-shifted +15-22 %, FP64 +5 %),
+ * but shifted reads touch more lines (all-fluid 512^3: 9-18 % slower).  AUTO decides at each
+ * lbm_set_flags from the flags: BRICKS when reading rows would fetch at least 1.12 cells per fluid
+ * cell in 128-byte lines (solid cells sharing lines with fluid ones), ROWS otherwise. */
+enum lbm_cell_order { LBM_CELL_ORDER_AUTO = 0, LBM_CELL_ORDER_ROWS = 1, LBM_CELL_ORDER_BRICKS = 2 };
 
 /* Storage of the state.  TWO_BUFFER: pull-form steps between two buffers.  AA: one buffer updated
  * in place, alternating a step that touches only the cell's own slots with one that pulls from and
@@ -72,6 +82,9 @@ int lbm_destroy(lbm_ctx* ctx);
 const char* lbm_last_error(const lbm_ctx* ctx);
 /* cells with the fluid ("inner") flag */
 int64_t lbm_fluid_cells(const lbm_ctx* ctx);
+/* the cell order in use (LBM_CELL_ORDER_ROWS or LBM_CELL_ORDER_BRICKS; -1 for a null context).  No
+ * reference counterpart: the reference stores 6^3 blocks of AoS cells. */
+int lbm_cell_order(const lbm_ctx* ctx);
 
 /* ---- boundary flags ------------------------------------------------------------------ */
 /* inner[i] != 0 marks a fluid cell: the reference's per-block `inner` bitset

@@ -2,6 +2,7 @@
 // device buffers, validates every argument and enqueues the kernels of lbm_dense.hip on the
 // context's stream.  Citations: LB/ = madpac/peano/applications/latticeboltzmann/,
 // BLB/ = LB/blocklatticeboltzmann/.
+#include <algorithm>
 #include "lbm_capi.h"
 
 #include <cmath>
@@ -59,6 +60,51 @@ int rebuild_words(lbm_ctx* c) {
   return LBM_OK;
 }
 
+// Cells fetched per fluid cell when rows are read in 128-byte lines (`per_line` cells): the number
+// of line-sized row segments holding any fluid cell, times their length, over the fluid cells.
+// 1 for an all-fluid box or walls that fill whole rows; 1.2-1.25 for the 20 %-solid sphere packing.
+// Bricks cut that waste (1.10-1.11 there) but a shifted read touches up to 4x more lines, which
+// costs 9 % (FP64) to 18 % (FP32) on an all-fluid lattice; interpolating the measured gains and
+// losses puts the break-even near 1.12.
+double row_line_waste(const lbm::DenseGeom& g, const uint8_t* inner, int per_line) {
+  int64_t fetched = 0, fluid = 0;
+  for (int64_t row = 0; row < (int64_t)g.ny * g.nz; row++) {
+    const uint8_t* r = inner + row * g.nx;
+    for (int x0 = 0; x0 < g.nx; x0 += per_line) {
+      const int len = std::min(per_line, g.nx - x0);
+      int nf = 0;
+      for (int x = x0; x < x0 + len; x++) nf += r[x] ? 1 : 0;
+      fluid += nf;
+      if (nf) fetched += len;
+    }
+  }
+  return fluid ? (double)fetched / (double)fluid : 1.0;
+}
+
+// Switch the device cell order; the per-cell arrays already on the device (porosity, inlet
+// velocities, coupling cell ids) follow.  The state is rebuilt by the caller (rebuild_words).
+int switch_cell_order(lbm_ctx* c, bool bricks) {
+  const lbm::DenseGeom from = c->g, to = bricks ? c->g_bricks : c->g_rows;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (c->alpha || c->d_bcu) {
+    int rc = ensure_scratch(c, (size_t)std::max(from.N, to.N) * 3 * sizeof(double));
+    if (rc) return rc;
+  }
+  if (c->alpha) {
+    HIPCHK(c, lbm::launch_relayout(from, to, c->alpha, c->scratch, 1, (int)c->esize, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->alpha, c->scratch, (size_t)to.N * c->esize, hipMemcpyDeviceToDevice, c->stream));
+  }
+  if (c->d_bcu) {
+    HIPCHK(c, lbm::launch_relayout(from, to, c->d_bcu, c->scratch, 3, 8, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->d_bcu, c->scratch, (size_t)to.N * 3 * sizeof(double), hipMemcpyDeviceToDevice,
+                             c->stream));
+  }
+  if (c->ncouple) HIPCHK(c, lbm::launch_convert_ids(from, to, c->d_ccells, c->ncouple, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->g = to;
+  return LBM_OK;
+}
+
 int upload_boundary_table(lbm_ctx* c) {
   HIPCHK(c, hipMemcpyAsync(c->d_bct, c->bct, sizeof(c->bct), hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -94,6 +140,8 @@ int lbm_create(lbm_ctx** out, const lbm_config* cfg) {
     return fail(nullptr, LBM_ERR_ARG, "unknown precision");
   if (cfg->layout != LBM_LAYOUT_TWO_BUFFER && cfg->layout != LBM_LAYOUT_AA)
     return fail(nullptr, LBM_ERR_ARG, "unknown layout");
+  if (cfg->cell_order < LBM_CELL_ORDER_AUTO || cfg->cell_order > LBM_CELL_ORDER_BRICKS)
+    return fail(nullptr, LBM_ERR_ARG, "unknown cell order");
   if (!(cfg->tau > 0.5) || !(cfg->tau < 1e6)) return fail(nullptr, LBM_ERR_ARG, "tau must be > 0.5");
   for (int k = 0; k < 3; k++)
     if (!std::isfinite(cfg->force[k])) return fail(nullptr, LBM_ERR_ARG, "force must be finite");
@@ -107,7 +155,9 @@ int lbm_create(lbm_ctx** out, const lbm_config* cfg) {
   c->f32math = cfg->precision == LBM_FP32_SHIFTED_F32;
   c->aa = cfg->layout == LBM_LAYOUT_AA;
   c->esize = c->fp32 ? sizeof(float) : sizeof(double);
-  c->g = lbm::make_geom(cfg->nx, cfg->ny, cfg->nz, cfg->periodic);
+  c->g_rows = lbm::make_geom(cfg->nx, cfg->ny, cfg->nz, cfg->periodic, false, c->esize);
+  c->g_bricks = lbm::make_geom(cfg->nx, cfg->ny, cfg->nz, cfg->periodic, true, c->esize);
+  c->g = cfg->cell_order == LBM_CELL_ORDER_BRICKS ? c->g_bricks : c->g_rows;
   c->force = cfg->force[0] != 0.0 || cfg->force[1] != 0.0 || cfg->force[2] != 0.0;
   c->p64.tau_eff = lbm_tau_effective(cfg->tau);
   c->p64.omega = 1.0 / c->p64.tau_eff;
@@ -127,10 +177,11 @@ int lbm_create(lbm_ctx** out, const lbm_config* cfg) {
     return bail(LBM_ERR_HIP, "stream creation failed");
   if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess)
     return bail(LBM_ERR_HIP, "event creation failed");
-  const size_t pbytes = (size_t)Q * c->g.N * c->esize;
-  c->npartials = lbm::dense_moment_partials(c->g);
+  // sized for the larger (bricked) extent, so that lbm_set_flags can switch the cell order
+  const size_t pbytes = (size_t)Q * c->g_bricks.N * c->esize;
+  c->npartials = std::max(lbm::dense_moment_partials(c->g_rows), lbm::dense_moment_partials(c->g_bricks));
   if (hipMalloc(&c->buf[0], pbytes) != hipSuccess || (!c->aa && hipMalloc(&c->buf[1], pbytes) != hipSuccess) ||
-      hipMalloc(&c->words, (size_t)c->g.N * sizeof(uint32_t)) != hipSuccess ||
+      hipMalloc(&c->words, (size_t)c->g_bricks.N * sizeof(uint32_t)) != hipSuccess ||
       hipMalloc(&c->d_inner, (size_t)c->g.Nu) != hipSuccess || hipMalloc(&c->d_cls, (size_t)c->g.Nu) != hipSuccess ||
       hipMalloc(&c->partials, (size_t)(c->npartials + 1024) * 4 * sizeof(double)) != hipSuccess ||
       hipMalloc(&c->small, 8 * sizeof(double)) != hipSuccess ||
@@ -138,7 +189,7 @@ int lbm_create(lbm_ctx** out, const lbm_config* cfg) {
     return bail(LBM_ERR_NOMEM, "device allocation failed");
   if (hipMemsetAsync(c->buf[0], 0, pbytes, c->stream) != hipSuccess ||
       (c->buf[1] && hipMemsetAsync(c->buf[1], 0, pbytes, c->stream) != hipSuccess) ||
-      hipMemsetAsync(c->words, 0, (size_t)c->g.N * sizeof(uint32_t), c->stream) != hipSuccess ||
+      hipMemsetAsync(c->words, 0, (size_t)c->g_bricks.N * sizeof(uint32_t), c->stream) != hipSuccess ||
       hipMemsetAsync(c->d_cls, 0, (size_t)c->g.Nu, c->stream) != hipSuccess ||
       hipMemcpyAsync(c->d_bct, c->bct, sizeof(c->bct), hipMemcpyHostToDevice, c->stream) != hipSuccess ||
       hipStreamSynchronize(c->stream) != hipSuccess)
@@ -167,10 +218,20 @@ const char* lbm_last_error(const lbm_ctx* c) { return c ? c->err.c_str() : g_cre
 
 int64_t lbm_fluid_cells(const lbm_ctx* c) { return c ? c->nfluid : -1; }
 
+int lbm_cell_order(const lbm_ctx* c) {
+  return c ? (c->g.bricked ? LBM_CELL_ORDER_BRICKS : LBM_CELL_ORDER_ROWS) : -1;
+}
+
 int lbm_set_flags(lbm_ctx* c, const uint8_t* inner) {
   if (!c) return LBM_ERR_ARG;
   if (!inner) return fail(c, LBM_ERR_ARG, "inner flags are null");
   HIPCHK(c, hipSetDevice(c->cfg.device));
+  bool bricks = c->cfg.cell_order == LBM_CELL_ORDER_BRICKS;
+  if (c->cfg.cell_order == LBM_CELL_ORDER_AUTO) bricks = row_line_waste(c->g, inner, c->fp32 ? 32 : 16) >= 1.12;
+  if (bricks != (c->g.bricked != 0)) {
+    int rc = switch_cell_order(c, bricks);
+    if (rc) return rc;
+  }
   HIPCHK(c, hipMemcpyAsync(c->d_inner, inner, (size_t)c->g.Nu, hipMemcpyHostToDevice, c->stream));
   c->flags_set = true;
   return rebuild_words(c);
@@ -225,7 +286,8 @@ int lbm_set_boundary_velocity(lbm_ctx* c, const double* u) {
         const int64_t p = lbm::stored_index(g, x, y, z);
         for (int d = 0; d < 3; d++) dev[(size_t)(3 * p + d)] = u[3 * r + d];
       }
-  if (!c->d_bcu && hipMalloc(&c->d_bcu, dev.size() * sizeof(double)) != hipSuccess)
+  // sized for either cell order (lbm_set_flags may switch it)
+  if (!c->d_bcu && hipMalloc(&c->d_bcu, 3 * (size_t)c->g_bricks.N * sizeof(double)) != hipSuccess)
     return fail(c, LBM_ERR_NOMEM, "boundary velocity allocation failed");
   HIPCHK(c, hipMemcpyAsync(c->d_bcu, dev.data(), dev.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -337,7 +399,7 @@ int lbm_set_porosity(lbm_ctx* c, const double* alpha) {
   for (int64_t i = 0; i < c->g.Nu; i++)
     if (!(alpha[i] >= 0.0 && alpha[i] <= 1.0)) return fail(c, LBM_ERR_ARG, "porosity must lie in [0,1]");
   if (c->couple_on) return fail(c, LBM_ERR_STATE, "coupled cells use the plain collision: no porosity");
-  if (!c->alpha && hipMalloc(&c->alpha, (size_t)c->g.N * c->esize) != hipSuccess)
+  if (!c->alpha && hipMalloc(&c->alpha, (size_t)c->g_bricks.N * c->esize) != hipSuccess)
     return fail(c, LBM_ERR_NOMEM, "porosity allocation failed");
   // the kernels index the porosity by stored (bricked) cell; padding cells get 0
   const lbm::DenseGeom& g = c->g;

@@ -214,63 +214,13 @@ __device__ __forceinline__ float collide_f32s(float (&g)[Q], const CollideParams
   return rho;
 }
 
-// FP32-shifted storage with FP64 arithmetic: the shifted populations are widened, the collision runs
-// in double (rho = 1 + drho is never rounded to FP32) and each population is rounded once when
-// stored.  FP32 arithmetic repeats the same rounding of the equilibrium step after step while the
-// state changes slowly, which accumulates into a momentum bias of ~1e-8 of the momentum per step at
-// low Mach; here the only rounding is the store's.  Outputs as collide_f32s.
-template <bool FORCE, bool ALPHA>
-__device__ __forceinline__ float collide_f32s_x(float (&gs)[Q], const CollideParams& p, float alpha,
-                                                float (&uo)[4]) {
-  double drho = 0.0, j0 = 0.0, j1 = 0.0, j2 = 0.0;
-#pragma unroll
-  for (int q = 0; q < Q; q++) {
-    const double g = gs[q];
-    drho += g;
-    if (CX(q) == 1) j0 += g;
-    if (CX(q) == -1) j0 -= g;
-    if (CY(q) == 1) j1 += g;
-    if (CY(q) == -1) j1 -= g;
-    if (CZ(q) == 1) j2 += g;
-    if (CZ(q) == -1) j2 -= g;
-  }
-  const double rho = 1.0 + drho;
-  const double inv = 1.0 / rho;
-  double u0 = j0 * inv, u1 = j1 * inv, u2 = j2 * inv;
-  if (FORCE) {
-    u0 += 0.5 * p.g[0];
-    u1 += 0.5 * p.g[1];
-    u2 += 0.5 * p.g[2];
-  }
-  uo[0] = (float)u0;
-  uo[1] = (float)u1;
-  uo[2] = (float)u2;
-  uo[3] = (float)drho;
-  if (ALPHA) {
-    const double s = 1.0 - (double)alpha;
-    u0 *= s; u1 *= s; u2 *= s;
-  }
-  const double uu15 = 1.5 * (u0 * u0 + u1 * u1 + u2 * u2);
-  const double ug = FORCE ? u0 * p.g[0] + u1 * p.g[1] + u2 * p.g[2] : 0.0;
-  const double fpref = FORCE ? (1.0 - 0.5 * p.omega) * rho : 0.0;
-#pragma unroll
-  for (int q = 0; q < Q; q++) {
-    const double cu = cdot64(q, u0, u1, u2);
-    const double geq = W64(q) * (drho + rho * (3.0 * cu + 4.5 * cu * cu - uu15));
-    const double g = gs[q];
-    double v = g - p.omega * (g - geq);
-    if (FORCE) {
-      const double cg = cdot64(q, p.g[0], p.g[1], p.g[2]);
-      v += fpref * W64(q) * (3.0 * (cg - ug) + 9.0 * cu * cg);
-    }
-    gs[q] = (float)v;
-  }
-  return (float)rho;
-}
-
 // FP32-shifted storage, hybrid arithmetic: the moments, rho, u and each equilibrium value in FP64
-// (the parts whose repeated FP32 rounding biases the momentum), the relaxation and the force term
-// in FP32.  Fewer live FP64 values than collide_f32s_x; outputs as collide_f32s.
+// (the parts whose repeated FP32 rounding biases the momentum), the difference g - geq and the
+// force term in FP32, the product omega (g - geq) in FP64: an FP32 omega would change the
+// viscosity by up to ~1e-6 relative near tau = 0.5, where nu = (1/omega - 1/2) / 3 is small.
+// Against an all-FP64 evaluation of the same update (widen, collide, round once at the store) it
+// holds fewer FP64 values live: 60 instead of 76 VGPRs in the dense step, 8 instead of 6 waves per
+// SIMD, +6 % throughput.  Outputs as collide_f32s.
 template <bool FORCE, bool ALPHA>
 __device__ __forceinline__ float collide_f32s_h(float (&gs)[Q], const CollideParams& p, const CollideParams32& p32,
                                                 float alpha, float (&uo)[4]) {
@@ -309,7 +259,7 @@ __device__ __forceinline__ float collide_f32s_h(float (&gs)[Q], const CollidePar
   for (int q = 0; q < Q; q++) {
     const double cu = cdot64(q, u0, u1, u2);
     const float geq = (float)(W64(q) * (drho + rho * (3.0 * cu + 4.5 * cu * cu - uu15)));
-    float v = gs[q] - p32.omega * (gs[q] - geq);
+    float v = gs[q] - (float)(p.omega * (double)(gs[q] - geq));
     if (FORCE) {
       const float cg = (float)CX(q) * p32.g[0] + (float)CY(q) * p32.g[1] + (float)CZ(q) * p32.g[2];
       const float cuf = (float)cu;

@@ -30,19 +30,6 @@ struct CoupleArgs {
   double omega;           // 1 / tau_eff (BlockBGK)
 };
 
-// position of stored (bricked) cell i
-__device__ __forceinline__ CellPos pos_of(const DenseGeom& g, int64_t i) {
-  CellPos c;
-  c.i = i;
-  const int64_t brick = i >> 6;
-  const int l = (int)(i & 63);
-  c.x = 4 * (int)(brick % g.bnx) + (l & 3);
-  c.y = 4 * (int)((brick / g.bnx) % g.bny) + ((l >> 2) & 3);
-  c.z = 4 * (int)(brick / ((int64_t)g.bnx * g.bny)) + (l >> 4);
-  c.r = (int64_t)c.x + (int64_t)g.nx * ((int64_t)c.y + (int64_t)g.ny * c.z);
-  c.valid = true;
-  return c;
-}
 
 // FP64 collision of one coupled cell (reference operation order, as the oracle restates it).
 __device__ __forceinline__ double collide_coupled_f64(double (&f)[Q], const CoupleArgs& a, int64_t k,

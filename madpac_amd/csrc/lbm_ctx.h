@@ -15,7 +15,8 @@ struct AdjointState;  // lbm_adjoint.hip
 
 struct lbm_ctx {
   lbm_config cfg{};
-  lbm::DenseGeom g{};
+  lbm::DenseGeom g{};                  // geometry of the cell order in use (one of the two below)
+  lbm::DenseGeom g_rows{}, g_bricks{};
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   std::string err;

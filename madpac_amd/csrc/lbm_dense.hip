@@ -19,15 +19,19 @@
 
 namespace lbm {
 
-DenseGeom make_geom(int nx, int ny, int nz, const int periodic[3]) {
+int64_t plane_pad(size_t esize) { return esize == 8 ? 2112 : 64; }
+
+DenseGeom make_geom(int nx, int ny, int nz, const int periodic[3], bool bricked, size_t esize) {
   DenseGeom g;
   g.nx = nx; g.ny = ny; g.nz = nz;
   g.px = periodic[0] ? 1 : 0; g.py = periodic[1] ? 1 : 0; g.pz = periodic[2] ? 1 : 0;
+  g.bricked = bricked ? 1 : 0;
+  g.rnx = (nx + BX - 1) / BX; g.rny = (ny + BY - 1) / BY;
   g.bnx = (nx + 3) / 4; g.bny = (ny + 3) / 4; g.bnz = (nz + 3) / 4;
   const int64_t bricks = (int64_t)g.bnx * g.bny * g.bnz;
-  g.N = bricks * 64;
   g.Nu = (int64_t)nx * ny * nz;
-  g.nblocks = (bricks + BY - 1) / BY;
+  g.N = (bricked ? bricks * 64 : g.Nu) + plane_pad(esize);
+  g.nblocks = bricked ? (bricks + BY - 1) / BY : (int64_t)g.rnx * g.rny * nz;
   g.nblocks_padded = (g.nblocks + NXCD - 1) / NXCD * NXCD;
   g.layout = LAYOUT_PULL;
   return g;
@@ -67,7 +71,7 @@ __global__ __launch_bounds__(BX * BY) void k_build_words(DenseGeom g, const uint
 // ------------------------------------------------------------------------------------------
 // Collision of one fluid cell plus the boundary classes of its bounce-back links (shared by the
 // two-buffer and the AA steps).  T = double: reference arithmetic; T = float: shifted populations,
-// FP64 collision arithmetic unless M32.
+// hybrid arithmetic (collide_f32s_h) unless M32 (all FP32).
 template <typename T, bool FORCE, bool ALPHA, bool WALLS, bool M32>
 __device__ __forceinline__ void collide_cell_bc(T (&f)[Q], uint32_t w, int64_t i, const T* __restrict__ alpha,
                                                 const CollideParams& p64, const CollideParams32& p32,
@@ -80,7 +84,7 @@ __device__ __forceinline__ void collide_cell_bc(T (&f)[Q], uint32_t w, int64_t i
   } else if constexpr (M32) {
     rho = collide_f32s<FORCE, ALPHA>(f, p32, a, uc);
   } else {
-    rho = collide_f32s_x<FORCE, ALPHA>(f, p64, a, uc);
+    rho = collide_f32s_h<FORCE, ALPHA>(f, p64, p32, a, uc);
   }
   if (WALLS) {
     const uint32_t k = (w >> CLASS_SHIFT) & CLASS_BITS;
@@ -436,6 +440,24 @@ __global__ __launch_bounds__(BX * BY) void k_count_fluid(DenseGeom g, const uint
   block_reduce_store<1>(v, partials + (int64_t)blockIdx.x);
 }
 
+// Cell-order conversion of per-cell arrays (one thread per cell of the target order).
+template <typename E>
+__global__ __launch_bounds__(BX * BY) void k_relayout(DenseGeom from, DenseGeom to, const E* __restrict__ in,
+                                                      E* __restrict__ out, int comps) {
+  const CellPos c = map_cell(to);
+  if (!c.valid) return;
+  const int64_t j = stored_index(from, c.x, c.y, c.z);
+  for (int k = 0; k < comps; k++) out[comps * c.i + k] = in[comps * j + k];
+}
+
+__global__ __launch_bounds__(256) void k_convert_ids(DenseGeom from, DenseGeom to, int64_t* __restrict__ ids,
+                                                     int64_t n) {
+  const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (k >= n) return;
+  const CellPos c = pos_of(from, ids[k]);
+  ids[k] = stored_index(to, c.x, c.y, c.z);
+}
+
 // ------------------------------------------------------------------------------------------
 static dim3 dense_block() { return dim3(BX, BY, 1); }
 static dim3 dense_grid(const DenseGeom& g) { return dim3((unsigned)g.nblocks_padded, 1, 1); }
@@ -556,6 +578,25 @@ hipError_t launch_dense_fields(const DenseGeom& g, bool fp32, const void* src, c
 }
 
 int64_t dense_moment_partials(const DenseGeom& g) { return g.nblocks_padded; }
+
+hipError_t launch_relayout(const DenseGeom& from, const DenseGeom& to, const void* in, void* out, int comps,
+                           int esize, hipStream_t s) {
+  if (esize == 8)
+    hipLaunchKernelGGL(k_relayout<double>, dense_grid(to), dense_block(), 0, s, from, to, (const double*)in,
+                       (double*)out, comps);
+  else if (esize == 4)
+    hipLaunchKernelGGL(k_relayout<float>, dense_grid(to), dense_block(), 0, s, from, to, (const float*)in,
+                       (float*)out, comps);
+  else
+    return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+hipError_t launch_convert_ids(const DenseGeom& from, const DenseGeom& to, int64_t* ids, int64_t n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_convert_ids, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, from, to, ids, n);
+  return hipGetLastError();
+}
 
 hipError_t launch_dense_moments(const DenseGeom& g, bool fp32, const void* src, const uint32_t* words,
                                 double* partials, double* out4, hipStream_t s) {

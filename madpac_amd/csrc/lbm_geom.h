@@ -7,8 +7,9 @@
 namespace lbm {
 
 // XCD-aware workgroup mapping.  Workgroups are dispatched round-robin over the 8 XCDs, so the
-// linear id is remapped to give each XCD one contiguous range of bricks (a slab of brick layers):
-// the neighbour bricks a workgroup pulls from were then mostly fetched into the same L2.  Lane
+// linear id is remapped to give each XCD one contiguous range of workgroups (a slab of z-planes or
+// brick layers): the neighbours a workgroup pulls from were then mostly fetched into the same L2.
+// Rows: thread (x, y) of a workgroup owns cell (64 bx + x, 4 by + y, z).  Bricks: lane
 // threadIdx.x of wavefront threadIdx.y owns cell threadIdx.x of brick 4 * workgroup + threadIdx.y.
 struct CellPos {
   int x, y, z;
@@ -22,6 +23,17 @@ __device__ __forceinline__ CellPos map_cell(const DenseGeom& g) {
   const int64_t bid = blockIdx.x;
   const int64_t per = g.nblocks_padded / NXCD;
   const int64_t b = (bid % NXCD) * per + bid / NXCD;
+  if (!g.bricked) {
+    const int bx = (int)(b % g.rnx);
+    const int64_t r = b / g.rnx;
+    const int by = (int)(r % g.rny);
+    c.z = (int)(r / g.rny);
+    c.x = bx * BX + threadIdx.x;
+    c.y = by * BY + threadIdx.y;
+    c.valid = c.z < g.nz && c.x < g.nx && c.y < g.ny;
+    c.i = c.r = (int64_t)c.x + (int64_t)g.nx * ((int64_t)c.y + (int64_t)g.ny * c.z);
+    return c;
+  }
   const int64_t brick = b * BY + threadIdx.y;
   const int l = threadIdx.x;
   const int bx = (int)(brick % g.bnx);
@@ -34,6 +46,28 @@ __device__ __forceinline__ CellPos map_cell(const DenseGeom& g) {
   c.valid = bz < g.bnz && c.x < g.nx && c.y < g.ny && c.z < g.nz;
   c.i = brick * 64 + l;
   c.r = (int64_t)c.x + (int64_t)g.nx * ((int64_t)c.y + (int64_t)g.ny * c.z);
+  return c;
+}
+
+// position of stored cell i
+__device__ __forceinline__ CellPos pos_of(const DenseGeom& g, int64_t i) {
+  CellPos c;
+  c.i = i;
+  c.valid = true;
+  if (!g.bricked) {
+    c.x = (int)(i % g.nx);
+    c.y = (int)((i / g.nx) % g.ny);
+    c.z = (int)(i / ((int64_t)g.nx * g.ny));
+    c.r = i;
+    return c;
+  }
+  const int64_t brick = i >> 6;
+  const int l = (int)(i & 63);
+  c.x = 4 * (int)(brick % g.bnx) + (l & 3);
+  c.y = 4 * (int)((brick / g.bnx) % g.bny) + ((l >> 2) & 3);
+  c.z = 4 * (int)(brick / ((int64_t)g.bnx * g.bny)) + (l >> 4);
+  c.r = (int64_t)c.x + (int64_t)g.nx * ((int64_t)c.y + (int64_t)g.ny * c.z);
+  c.valid = true;
   return c;
 }
 
@@ -57,6 +91,16 @@ __device__ __forceinline__ int64_t brick_term_z(const DenseGeom& g, int z) {
 
 __device__ __forceinline__ NbrOff nbr_offsets(const DenseGeom& g, const CellPos& c) {
   NbrOff o;
+  if (!g.bricked) {
+    const int64_t sy = g.nx, sz = (int64_t)g.nx * g.ny;
+    o.xm = c.x == 0 ? (g.px ? (int64_t)(g.nx - 1) : 0) : -1;
+    o.xp = c.x == g.nx - 1 ? (g.px ? -(int64_t)(g.nx - 1) : 0) : 1;
+    o.ym = c.y == 0 ? (g.py ? (int64_t)(g.ny - 1) * sy : 0) : -sy;
+    o.yp = c.y == g.ny - 1 ? (g.py ? -(int64_t)(g.ny - 1) * sy : 0) : sy;
+    o.zm = c.z == 0 ? (g.pz ? (int64_t)(g.nz - 1) * sz : 0) : -sz;
+    o.zp = c.z == g.nz - 1 ? (g.pz ? -(int64_t)(g.nz - 1) * sz : 0) : sz;
+    return o;
+  }
   const int64_t tx = brick_term_x(c.x), ty = brick_term_y(g, c.y), tz = brick_term_z(g, c.z);
   o.xm = c.x == 0 ? (g.px ? brick_term_x(g.nx - 1) - tx : 0) : brick_term_x(c.x - 1) - tx;
   o.xp = c.x == g.nx - 1 ? (g.px ? -tx : 0) : brick_term_x(c.x + 1) - tx;

@@ -10,16 +10,19 @@
 
 namespace lbm {
 
-// One lattice stored as SoA populations s[q*N + i] over bricks of 4 x 4 x 4 cells: the 64 cells
-// of a brick are consecutive (x fastest, then y, then z) and bricks follow in x-fastest order, so
-// i = 64 * (bx + bnx * (by + bny * bz)) + (x & 3) + 4 * (y & 3) + 16 * (z & 3).  One wavefront
-// updates one brick.  A 128-byte line then holds a compact 4 x 4 x 1 (FP64) or 4 x 4 x 2 (FP32)
-// cell group instead of a row segment, so far fewer lines straddle a solid surface and get
-// fetched for a few fluid cells.  The lattice is padded to whole bricks (padding cells are never
-// fluid and never touched).  Arrays that cross the C-ABI stay in the row-major user order
-// r = x + nx * (y + ny * z); the kernels that read or write them index both.
-// The stored state is the post-collision state of the last step; the reference's post-streaming
-// state is its pull (see k_dense_pull).  A workgroup is BY bricks of BX = 64 cells.
+// One lattice stored as SoA populations s[q*N + i] in one of two cell orders.
+//  * Rows: i = x + nx * (y + ny * z); a workgroup is BX x BY cells of one z-plane, a wavefront a
+//    64-cell row segment.
+//  * Bricks of 4 x 4 x 4 cells: the 64 cells of a brick are consecutive (x fastest, then y, then
+//    z) and bricks follow in x-fastest order, i = 64 * (bx + bnx * (by + bny * bz)) + (x & 3) +
+//    4 * (y & 3) + 16 * (z & 3), padded to whole bricks (padding cells are never fluid and never
+//    touched); a wavefront is one brick, a workgroup BY bricks.  A 128-byte line holds a compact
+//    4 x 4 x 1 (FP64) or 4 x 4 x 2 (FP32) cell group, so far fewer lines straddle a solid surface
+//    and get fetched for a few fluid cells; but a shifted read touches up to 4x more lines than a
+//    row segment does.  Bricks win on porous lattices, rows on (nearly) all-fluid ones.
+// Arrays that cross the C-ABI stay in the row-major user order r; kernels that read or write them
+// index both.  The stored state is the post-collision state of the last step; the reference's
+// post-streaming state is its pull (see k_dense_pull).
 constexpr int BX = 64;
 constexpr int BY = 4;
 constexpr int NXCD = 8;  // MI355X: 8 XCDs, each with its own L2
@@ -27,16 +30,19 @@ constexpr int NXCD = 8;  // MI355X: 8 XCDs, each with its own L2
 struct DenseGeom {
   int nx, ny, nz;
   int px, py, pz;            // periodic flags
+  int bricked;               // 0 = rows, 1 = bricks
+  int rnx, rny;              // rows: workgroups along x and y
   int bnx, bny, bnz;         // bricks per dimension
-  int64_t N;                 // stored cells (whole bricks)
+  int64_t N;                 // plane stride: stored cells (rows: Nu; bricks: whole bricks) + plane_pad
   int64_t Nu;                // lattice cells nx * ny * nz (the row-major user arrays)
-  int64_t nblocks;           // workgroups holding a brick
+  int64_t nblocks;           // workgroups holding a cell
   int64_t nblocks_padded;    // rounded up to a multiple of NXCD
   int layout;                // how the stored state maps to the reference state (LAYOUT_*)
 };
 
 // stored index of cell (x, y, z)
 __host__ __device__ inline int64_t stored_index(const DenseGeom& g, int x, int y, int z) {
+  if (!g.bricked) return (int64_t)x + (int64_t)g.nx * ((int64_t)y + (int64_t)g.ny * z);
   return ((((int64_t)(z >> 2) * g.bny + (y >> 2)) * g.bnx + (x >> 2)) << 6) + ((z & 3) << 4) + ((y & 3) << 2) +
          (x & 3);
 }
@@ -47,7 +53,18 @@ __host__ __device__ inline int64_t stored_index(const DenseGeom& g, int x, int y
 // values reflected in the cell's own slots, a_{18-q}(x) = f*_q(x).
 enum : int { LAYOUT_PULL = 0, LAYOUT_AA_NATURAL = 1, LAYOUT_AA_SWAPPED = 2 };
 
-DenseGeom make_geom(int nx, int ny, int nz, const int periodic[3]);
+// Population planes are N apart, N = stored cells + plane_pad(esize): with a power-of-two stride
+// (2^24 cells at 256^3) the 19 plane streams of a wavefront alias on the same HBM channels and the
+// step time varies by 15 % from one allocation to the next.  Pads measured on the 256^3 porous
+// lattice (bricks): FP64 +6 % with 2112 cells (16.5 KiB), FP32 stable and +1 % with 64 cells
+// (256 B); pads of 1M cells were 20 % slower.
+int64_t plane_pad(size_t esize);
+DenseGeom make_geom(int nx, int ny, int nz, const int periodic[3], bool bricked, size_t esize);
+// per-cell array of `comps` values of `esize` bytes from the cell order of `from` to that of `to`
+hipError_t launch_relayout(const DenseGeom& from, const DenseGeom& to, const void* in, void* out, int comps,
+                           int esize, hipStream_t stream);
+// stored cell ids of `from` -> those of `to`, in place
+hipError_t launch_convert_ids(const DenseGeom& from, const DenseGeom& to, int64_t* ids, int64_t n, hipStream_t stream);
 
 // ---- launchers (all enqueue on `stream`; return hipError_t) ----
 hipError_t launch_build_words(const DenseGeom& g, const uint8_t* inner, const uint8_t* cls, uint32_t* words,
@@ -61,7 +78,7 @@ struct BoundaryArgs {
   const BoundaryClass* table;
   const double* cell_u;
 };
-// fp32: FP32-shifted storage; f32math: FP32 collision arithmetic (else FP64 arithmetic on the FP32 state)
+// fp32: FP32-shifted storage; f32math: FP32 collision arithmetic (else the hybrid arithmetic of collide_f32s_h)
 hipError_t launch_dense_step(const DenseGeom& g, bool fp32, bool f32math, const void* src, void* dst,
                              const uint32_t* words,
                              const void* alpha, bool force, bool walls, const CollideParams& p64,

@@ -99,7 +99,7 @@ constexpr bool dir_index_matches_table() {
 }
 static_assert(dir_index_matches_table(), "dir_index must invert the D3Q19 ordering");
 
-// M32: FP32 arithmetic for FP32 storage (else FP64 arithmetic, rounded once at the store)
+// M32: FP32 arithmetic for FP32 storage (else the hybrid arithmetic of collide_f32s_h)
 template <typename T, bool M32>
 __device__ __forceinline__ void collide_patch(T (&f)[Q], double omega) {
   if constexpr (sizeof(T) == 8) {

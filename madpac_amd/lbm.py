@@ -16,9 +16,10 @@ LIB_PATH = os.path.join(_HERE, "liblbm_hip.so")
 
 Q = 19
 FP64 = 0
-FP32_SHIFTED = 1       # FP32 state, FP64 collision arithmetic
+FP32_SHIFTED = 1       # FP32 state; FP64 moments, equilibrium and relaxation product
 FP32_SHIFTED_F32 = 2   # FP32 state, FP32 collision arithmetic
 LAYOUT_TWO_BUFFER, LAYOUT_AA = 0, 1
+CELL_ORDER_AUTO, CELL_ORDER_ROWS, CELL_ORDER_BRICKS = 0, 1, 2
 # boundary-condition types of a class (include/lbm_capi.h)
 BC_MOVING_WALL, BC_PRESSURE_OUTLET, BC_PRESSURE_INLET, BC_VELOCITY_INLET = 0, 1, 2, 3
 BC_CELL_VELOCITY = 16
@@ -33,7 +34,7 @@ C = np.array([
 
 # every symbol include/lbm_capi.h declares
 EXPORTS = (
-    "lbm_create", "lbm_destroy", "lbm_last_error", "lbm_fluid_cells", "lbm_set_flags",
+    "lbm_create", "lbm_destroy", "lbm_last_error", "lbm_fluid_cells", "lbm_cell_order", "lbm_set_flags",
     "lbm_set_boundary_class", "lbm_set_moving_wall_class", "lbm_set_wall_classes", "lbm_set_boundary_velocity",
     "lbm_set_cavity_lid", "lbm_set_channel", "lbm_set_porosity",
     "lbm_set_coupling_cells", "lbm_set_coupling_data", "lbm_get_coupling_data",
@@ -54,7 +55,8 @@ class LbmError(RuntimeError):
 class Config(ctypes.Structure):
     _fields_ = [("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int),
                 ("periodic", ctypes.c_int * 3), ("precision", ctypes.c_int), ("tau", ctypes.c_double),
-                ("force", ctypes.c_double * 3), ("device", ctypes.c_int), ("layout", ctypes.c_int)]
+                ("force", ctypes.c_double * 3), ("device", ctypes.c_int), ("layout", ctypes.c_int),
+                ("cell_order", ctypes.c_int)]
 
 
 class AdjointStats(ctypes.Structure):
@@ -92,6 +94,8 @@ def load_library(path: str = LIB_PATH):
     L.lbm_last_error.restype = ctypes.c_char_p
     L.lbm_fluid_cells.argtypes = [vp]
     L.lbm_fluid_cells.restype = ctypes.c_int64
+    L.lbm_cell_order.argtypes = [vp]
+    L.lbm_cell_order.restype = ctypes.c_int
     L.lbm_set_flags.argtypes = [vp, u8p]
     L.lbm_set_boundary_class.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, dp, ctypes.c_uint32]
     L.lbm_set_moving_wall_class.argtypes = [vp, ctypes.c_int, dp, ctypes.c_uint32]
@@ -148,10 +152,10 @@ class Lattice:
     """A dense D3Q19 lattice on one GPU (an lbm_ctx)."""
 
     def __init__(self, nx, ny, nz, tau, precision=FP64, periodic=(False, False, False), force=(0.0, 0.0, 0.0),
-                 device=0, layout=LAYOUT_TWO_BUFFER):
+                 device=0, layout=LAYOUT_TWO_BUFFER, cell_order=CELL_ORDER_AUTO):
         self._L = load_library()
         cfg = Config(nx, ny, nz, (ctypes.c_int * 3)(*[int(p) for p in periodic]), int(precision), float(tau),
-                     (ctypes.c_double * 3)(*[float(v) for v in force]), int(device), int(layout))
+                     (ctypes.c_double * 3)(*[float(v) for v in force]), int(device), int(layout), int(cell_order))
         h = ctypes.c_void_p()
         rc = self._L.lbm_create(ctypes.byref(h), ctypes.byref(cfg))
         if rc != 0:
@@ -256,6 +260,11 @@ class Lattice:
     @property
     def fluid_cells(self):
         return int(self._L.lbm_fluid_cells(self._h))
+
+    @property
+    def cell_order(self):
+        """CELL_ORDER_ROWS or CELL_ORDER_BRICKS, as chosen at the last set_flags."""
+        return int(self._L.lbm_cell_order(self._h))
 
     # ---- state
     def init_equilibrium(self, rho=None, u=None):

@@ -209,8 +209,9 @@ def test_fp32_shifted_cavity_error_vs_fp64_oracle(prec):
 
 def test_fp32_shifted_has_no_momentum_drift_in_forced_porous_flow():
     # Low-Mach forced flow in a porous box: FP32 collision arithmetic repeats the same rounding of
-    # the equilibrium every step and drifts the momentum (~1e-5 relative after 500 steps here);
-    # the FP32-shifted state with FP64 collision arithmetic rounds only at the store.
+    # the moments and equilibrium every step and drifts the momentum (~1e-5 relative after 500
+    # steps here); the FP32-shifted collision keeps them and the relaxation product in FP64
+    # (measured: u L2 2.1e-7, momentum offset 8.8e-8).  Bars: u L2 <= 5e-7, |offset| <= 2e-7.
     n, steps, tau, g = 24, 500, 0.55, (1e-6, 0.0, 0.0)
     solid = _porous_solid(n, 17, frac=0.2, rmin=2, rmax=4)
     ora = O.Lattice(n, n, n, periodic=(True, True, True))
