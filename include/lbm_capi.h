@@ -162,6 +162,14 @@ int lbm_get_populations(lbm_ctx* ctx, double* f);
 /* nsteps fused collide+stream steps: phases A, B and C of one reference iteration
  * (BLB/mappings/RegularGrid2RegularBlockSolver.cpp:83-248) per step. */
 int lbm_step(lbm_ctx* ctx, int nsteps);
+/* Same steps, replayed from a hipGraph of 32 steps captured on the first call (one graph per
+ * starting buffer or AA parity; the remainder runs as plain steps).  Anything the launches bake in
+ * (cell order, porosity, per-cell velocities, wall classes present, body force, load schedule, MD
+ * coupling) is checked on every call and a change re-captures.  Results are bit-identical to
+ * lbm_step, whose launches are capture-safe (no memsets, copies or synchronisation).  On MI355X /
+ * ROCm 7.2 replays are not faster than plain launches (both bound by the dispatch latency of
+ * dependent kernels, ~6 us per step on small lattices).  No reference counterpart. */
+int lbm_step_graph(lbm_ctx* ctx, int nsteps);
 /* Same, timed with HIP events on the context stream; *ms = elapsed device time. */
 int lbm_step_timed(lbm_ctx* ctx, int nsteps, float* ms);
 /* Times the step kernel's two load schedules for nsteps steps each on a copy of the current state

@@ -13,6 +13,21 @@ namespace lbm {
 struct AdjointState;  // lbm_adjoint.hip
 }
 
+// What a captured pair of steps bakes into its launches (besides fixed buffers and parameters):
+// a change of any of these re-captures the graph (lbm_step_graph).
+struct lbm_graph_key {
+  int bricked = -1, walls = 0, force = 0, spec = 0, couple_on = 0, couple_mode = 0, couple_rho = 0;
+  double couple_dt = 0.0;
+  const void *alpha = nullptr, *bcu = nullptr, *ccells = nullptr, *cu = nullptr, *crho = nullptr;
+  int64_t ncouple = 0;
+  bool operator==(const lbm_graph_key& o) const {
+    return bricked == o.bricked && walls == o.walls && force == o.force && spec == o.spec &&
+           couple_on == o.couple_on && couple_mode == o.couple_mode && couple_rho == o.couple_rho &&
+           couple_dt == o.couple_dt && alpha == o.alpha && bcu == o.bcu && ccells == o.ccells && cu == o.cu &&
+           crho == o.crho && ncouple == o.ncouple;
+  }
+};
+
 struct lbm_ctx {
   lbm_config cfg{};
   lbm::DenseGeom g{};                  // geometry of the cell order in use (one of the two below)
@@ -54,6 +69,10 @@ struct lbm_ctx {
   size_t scratch_bytes = 0;
   int64_t nfluid = 0;
   lbm::AdjointState* adj = nullptr;
+  // lbm_step_graph: two steps captured per starting phase (buffer `cur`, or the AA parity)
+  hipGraph_t graph[2] = {nullptr, nullptr};
+  hipGraphExec_t gexec[2] = {nullptr, nullptr};
+  lbm_graph_key gkey[2];
 };
 
 double lbm_tau_effective(double tau);
@@ -69,4 +88,5 @@ int lbm_enqueue_step_from(lbm_ctx* c, const void* src, void* dst);
 // re-collision of the MD-coupled cells for the step src -> dst (no-op without coupling)
 int lbm_enqueue_coupling(lbm_ctx* c, const void* src, void* dst);
 void lbm_coupling_release(lbm_ctx* c);
+void lbm_graphs_release(lbm_ctx* c);
 void lbm_adjoint_release(lbm_ctx* c);

@@ -34,7 +34,8 @@ C = np.array([
 
 # every symbol include/lbm_capi.h declares
 EXPORTS = (
-    "lbm_create", "lbm_destroy", "lbm_last_error", "lbm_fluid_cells", "lbm_cell_order", "lbm_set_flags",
+    "lbm_create", "lbm_destroy", "lbm_last_error", "lbm_fluid_cells", "lbm_cell_order", "lbm_step_graph",
+    "lbm_set_flags",
     "lbm_set_boundary_class", "lbm_set_moving_wall_class", "lbm_set_wall_classes", "lbm_set_boundary_velocity",
     "lbm_set_cavity_lid", "lbm_set_channel", "lbm_set_porosity",
     "lbm_set_coupling_cells", "lbm_set_coupling_data", "lbm_get_coupling_data",
@@ -111,6 +112,7 @@ def load_library(path: str = LIB_PATH):
     L.lbm_set_populations.argtypes = [vp, dp]
     L.lbm_get_populations.argtypes = [vp, dp]
     L.lbm_step.argtypes = [vp, ctypes.c_int]
+    L.lbm_step_graph.argtypes = [vp, ctypes.c_int]
     L.lbm_step_timed.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_float)]
     L.lbm_tune.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
     L.lbm_synchronize.argtypes = [vp]
@@ -283,8 +285,11 @@ class Lattice:
         self._check(self._L.lbm_get_populations(self._h, _dp(f)), "lbm_get_populations")
         return f
 
-    def step(self, n=1):
-        self._check(self._L.lbm_step(self._h, int(n)), "lbm_step")
+    def step(self, n=1, graph=False):
+        if graph:
+            self._check(self._L.lbm_step_graph(self._h, int(n)), "lbm_step_graph")
+        else:
+            self._check(self._L.lbm_step(self._h, int(n)), "lbm_step")
 
     def step_timed(self, n):
         ms = ctypes.c_float()
