@@ -28,10 +28,12 @@ DenseGeom make_geom(int nx, int ny, int nz, const int periodic[3], bool bricked,
   g.bricked = bricked ? 1 : 0;
   g.rnx = (nx + BX - 1) / BX; g.rny = (ny + BY - 1) / BY;
   g.bnx = (nx + 3) / 4; g.bny = (ny + 3) / 4; g.bnz = (nz + 3) / 4;
+  g.tnx = (g.bnx + 1) / 2; g.tny = (g.bny + 1) / 2; g.tnz = (g.bnz + 1) / 2;
   const int64_t bricks = (int64_t)g.bnx * g.bny * g.bnz;
   g.Nu = (int64_t)nx * ny * nz;
   g.N = (bricked ? bricks * 64 : g.Nu) + plane_pad(esize);
-  g.nblocks = bricked ? (bricks + BY - 1) / BY : (int64_t)g.rnx * g.rny * nz;
+  // bricks: 8 brick slots per traversal tile, BY per workgroup
+  g.nblocks = bricked ? (int64_t)g.tnx * g.tny * g.tnz * 8 / BY : (int64_t)g.rnx * g.rny * nz;
   g.nblocks_padded = (g.nblocks + NXCD - 1) / NXCD * NXCD;
   g.layout = LAYOUT_PULL;
   return g;

@@ -10,7 +10,10 @@ namespace lbm {
 // linear id is remapped to give each XCD one contiguous range of workgroups (a slab of z-planes or
 // brick layers): the neighbours a workgroup pulls from were then mostly fetched into the same L2.
 // Rows: thread (x, y) of a workgroup owns cell (64 bx + x, 4 by + y, z).  Bricks: lane
-// threadIdx.x of wavefront threadIdx.y owns cell threadIdx.x of brick 4 * workgroup + threadIdx.y.
+// threadIdx.x of wavefront threadIdx.y owns cell threadIdx.x of the brick in traversal slot
+// 4 * workgroup + threadIdx.y; slots run over tiles of 2 x 2 x 2 bricks (x fastest inside a tile,
+// tiles in x-fastest order), which keeps a brick's z- and y-neighbours in flight together more
+// often than the storage order does (FP64 +3 %; larger tiles were slower).
 struct CellPos {
   int x, y, z;
   int64_t i;   // stored (bricked) index
@@ -34,17 +37,21 @@ __device__ __forceinline__ CellPos map_cell(const DenseGeom& g) {
     c.i = c.r = (int64_t)c.x + (int64_t)g.nx * ((int64_t)c.y + (int64_t)g.ny * c.z);
     return c;
   }
-  const int64_t brick = b * BY + threadIdx.y;
+  const int64_t slot = b * BY + threadIdx.y;
   const int l = threadIdx.x;
-  const int bx = (int)(brick % g.bnx);
-  const int64_t rest = brick / g.bnx;
-  const int by = (int)(rest % g.bny);
-  const int64_t bz = rest / g.bny;
+  const int64_t tile = slot >> 3;
+  const int loc = (int)(slot & 7);
+  const int tx = (int)(tile % g.tnx);
+  const int64_t rest = tile / g.tnx;
+  const int ty = (int)(rest % g.tny);
+  const int64_t tz = rest / g.tny;
+  const int bx = 2 * tx + (loc & 1), by = 2 * ty + ((loc >> 1) & 1);
+  const int64_t bz = 2 * tz + (loc >> 2);
   c.x = 4 * bx + (l & 3);
   c.y = 4 * by + ((l >> 2) & 3);
   c.z = (int)(4 * bz + (l >> 4));
-  c.valid = bz < g.bnz && c.x < g.nx && c.y < g.ny && c.z < g.nz;
-  c.i = brick * 64 + l;
+  c.valid = bz < g.bnz && bx < g.bnx && by < g.bny && c.x < g.nx && c.y < g.ny && c.z < g.nz;
+  c.i = ((int64_t)bx + (int64_t)g.bnx * ((int64_t)by + (int64_t)g.bny * bz)) * 64 + l;
   c.r = (int64_t)c.x + (int64_t)g.nx * ((int64_t)c.y + (int64_t)g.ny * c.z);
   return c;
 }

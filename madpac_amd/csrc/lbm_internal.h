@@ -33,6 +33,7 @@ struct DenseGeom {
   int bricked;               // 0 = rows, 1 = bricks
   int rnx, rny;              // rows: workgroups along x and y
   int bnx, bny, bnz;         // bricks per dimension
+  int tnx, tny, tnz;         // bricks: traversal tiles of 2 x 2 x 2 bricks per dimension
   int64_t N;                 // plane stride: stored cells (rows: Nu; bricks: whole bricks) + plane_pad
   int64_t Nu;                // lattice cells nx * ny * nz (the row-major user arrays)
   int64_t nblocks;           // workgroups holding a cell
