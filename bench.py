@@ -51,7 +51,7 @@ def run_dense(lbm, solid, n, tau, prec, steps, warmup, g, layout=0):
     lat.init_equilibrium()
     nfluid = lat.fluid_cells
     order = "bricks" if lat.cell_order == lbm.CELL_ORDER_BRICKS else "rows"
-    variant = lat.tune(10)  # load schedule of the step kernel (bit-identical results either way)
+    variant = lat.tune(40)  # load schedule of the step kernel (bit-identical results either way)
     lat.step(warmup)
     lat.synchronize()
     t0 = time.perf_counter()

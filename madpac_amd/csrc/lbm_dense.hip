@@ -102,6 +102,18 @@ __device__ __forceinline__ void collide_cell_bc(T (&f)[Q], uint32_t w, int64_t i
 }
 
 // ------------------------------------------------------------------------------------------
+// Whether this lane's 128-byte line of a population plane holds a fluid lane of the wavefront
+// (`fluid` = the wavefront's ballot of fluid lanes).  A solid lane in such a line stores zeros into
+// its own slots (never read: pulls and readers skip non-fluid cells) together with the fluid lanes'
+// stores, so the line is written whole: partial-line writes at the solid surfaces cost more.
+template <typename T>
+__device__ __forceinline__ bool line_has_fluid(unsigned long long fluid) {
+  constexpr int LANES = 128 / (int)sizeof(T);
+  const int first = (int)threadIdx.x & ~(LANES - 1);
+  const unsigned long long line = (LANES >= 64) ? ~0ull : (((1ull << LANES) - 1ull) << first);
+  return (fluid & line) != 0;
+}
+
 // The hot kernel.  T = double: reference arithmetic; T = float: shifted populations.
 // SPEC = false: the flag word is read first, then the 19 loads at the pulled addresses (two
 // dependent memory round trips; solid lanes load nothing).  SPEC = true: the 19 neighbour loads
@@ -122,19 +134,34 @@ __global__ __launch_bounds__(BX * BY) void k_dense_step(DenseGeom g, const T* __
 #pragma unroll
     for (int q = 0; q < Q; q++) f[q] = src[(int64_t)q * N + c.i + src_off(q, o)];
     w = words[c.i];
-    if (!(w & FLUID)) return;
-    if (w & 0x7FFFFu) {
+    const bool fl = (w & FLUID) != 0;
+    const unsigned long long fluid = __ballot(fl);  // every lane votes before any returns
+    if (!fl && !line_has_fluid<T>(fluid)) return;
+    if (fl) {
+      if (w & 0x7FFFFu) {
 #pragma unroll
-      for (int q = 0; q < Q; q++)
-        if ((w >> (Q - 1 - q)) & 1u) f[q] = src[(int64_t)(Q - 1 - q) * N + c.i];
+        for (int q = 0; q < Q; q++)
+          if ((w >> (Q - 1 - q)) & 1u) f[q] = src[(int64_t)(Q - 1 - q) * N + c.i];
+      }
+      collide_cell_bc<T, FORCE, ALPHA, WALLS, M32>(f, w, c.i, alpha, p64, p32, bc);
+    } else {
+#pragma unroll
+      for (int q = 0; q < Q; q++) f[q] = T(0);
     }
   } else {
     w = words[c.i];
-    if (!(w & FLUID)) return;
+    const bool fl = (w & FLUID) != 0;
+    const unsigned long long fluid = __ballot(fl);  // every lane votes before any returns
+    if (!fl && !line_has_fluid<T>(fluid)) return;
+    if (fl) {
 #pragma unroll
-    for (int q = 0; q < Q; q++) f[q] = src[pull_index(q, w, c.i, N, o)];
+      for (int q = 0; q < Q; q++) f[q] = src[pull_index(q, w, c.i, N, o)];
+      collide_cell_bc<T, FORCE, ALPHA, WALLS, M32>(f, w, c.i, alpha, p64, p32, bc);
+    } else {
+#pragma unroll
+      for (int q = 0; q < Q; q++) f[q] = T(0);
+    }
   }
-  collide_cell_bc<T, FORCE, ALPHA, WALLS, M32>(f, w, c.i, alpha, p64, p32, bc);
 #pragma unroll
   for (int q = 0; q < Q; q++) __builtin_nontemporal_store(f[q], &dst[(int64_t)q * N + c.i]);
 }
@@ -155,13 +182,21 @@ __global__ __launch_bounds__(BX * BY) void k_aa_step(DenseGeom g, T* __restrict_
   const CellPos c = map_cell(g);
   if (!c.valid) return;
   const uint32_t w = words[c.i];
-  if (!(w & FLUID)) return;
   const int64_t N = g.N;
+  const bool fl = (w & FLUID) != 0;
+  const unsigned long long fluid = __ballot(fl);  // every lane votes before any returns
+  // the even step writes the cell's own slots: solid lanes complete the lines as in k_dense_step
+  if (!fl && (ODD || !line_has_fluid<T>(fluid))) return;
   const NbrOff o = nbr_offsets(g, c);
   T f[Q];
+  if (fl) {
 #pragma unroll
-  for (int q = 0; q < Q; q++) f[q] = ODD ? a[swapped_index(q, w, c.i, N, o)] : a[(int64_t)q * N + c.i];
-  collide_cell_bc<T, FORCE, ALPHA, WALLS, M32>(f, w, c.i, alpha, p64, p32, bc);
+    for (int q = 0; q < Q; q++) f[q] = ODD ? a[swapped_index(q, w, c.i, N, o)] : a[(int64_t)q * N + c.i];
+    collide_cell_bc<T, FORCE, ALPHA, WALLS, M32>(f, w, c.i, alpha, p64, p32, bc);
+  } else {
+#pragma unroll
+    for (int q = 0; q < Q; q++) f[q] = T(0);
+  }
   // The stores go to the slots just loaded.  Left to itself the compiler keeps the 19 load
   // addresses live through the collision (38 more VGPRs, one wave less per SIMD); an opaque copy
   // of the cell index makes it recompute them here instead.
