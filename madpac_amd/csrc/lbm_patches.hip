@@ -176,6 +176,16 @@ __global__ __launch_bounds__(256) void k_patch_step(const PatchDev* __restrict__
   }
 }
 
+// (x, y, z) of linear cell c of an n^3 patch (x fastest); NN = n when known at compile time, else 0
+template <int NN>
+__device__ __forceinline__ void split_cell(int c, int n_rt, int& x, int& y, int& z) {
+  const int n = NN ? NN : n_rt;
+  z = c / (n * n);
+  const int r = c - z * n * n;
+  y = r / n;
+  x = r - y * n;
+}
+
 // Linear tiles: 256 consecutive interior cells of one patch in x-fastest order, one per thread, so
 // no lane idles whatever the edge (tile.x0 holds the first cell's linear index).
 // FP32 state: at most 64 VGPRs, 8 waves per SIMD (the hybrid collision fits without spills);
@@ -198,8 +208,15 @@ __global__ __launch_bounds__(256, patch_waves_per_simd<T>()) void k_patch_step_l
     const int n = tl.n;
     const int c = tl.x0 + (int)threadIdx.x;
     if (c < n * n * n) {
-      const int z = c / (n * n), r = c - z * n * n;
-      const int y = r / n, x = r - y * n;
+      // the edge is uniform over the workgroup: the common edges divide by constants
+      int x, y, z;
+      switch (n) {
+        case 8: split_cell<8>(c, n, x, y, z); break;
+        case 16: split_cell<16>(c, n, x, y, z); break;
+        case 24: split_cell<24>(c, n, x, y, z); break;
+        case 32: split_cell<32>(c, n, x, y, z); break;
+        default: split_cell<0>(c, n, x, y, z); break;
+      }
       patch_cell<T, M32>(pp, src + tl.base, dst + tl.base, dst, n, tl.omega, x, y, z);
     }
     if (tn >= ntiles) break;
