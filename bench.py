@@ -180,8 +180,8 @@ def main():
     ap.add_argument("--no-patches", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path (the domain would shard "
-              "into slabs with xGMI halo exchange; see the sharding notes).  Run with --gpus 1.",
+        print("bench.py measures one GPU only: madpac_amd has no multi-GPU path (the domain would shard "
+              "into slabs with an xGMI halo exchange).  Run with --gpus 1.",
               file=sys.stderr)
         sys.exit(2)
 

@@ -1,7 +1,7 @@
 """CPU oracle for the D3Q19 block-LB hot path -- TEST INFRASTRUCTURE ONLY.
 
 Only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
-``bench.py`` may import this package, and only as the checker.  The product
+``bench.py`` may import this package, and only to check results against.  The product
 package ``madpac_amd`` never imports it.
 
 The numerics live in ``lbm_oracle.c`` (a plain-C restatement of the reference's

@@ -2,7 +2,7 @@
  * CPU oracle for the D3Q19 BGK block-LB hot path (TEST INFRASTRUCTURE ONLY).
  *
  * Only tests/, __graft_entry__.smoke() and the cpu_baseline leg of bench.py use
- * this file, and only as the checker.  See lbm_oracle.h for the pinning summary.
+ * this file, and only to check results against.  See lbm_oracle.h for the pinning summary.
  * Reference citations: LB/ = madpac/peano/applications/latticeboltzmann/,
  * BLB/ = LB/blocklatticeboltzmann/, SLB/ = madpac/scenario/latticeboltzmann/blocklatticeboltzmann/.
  *

@@ -2,8 +2,8 @@
  * CPU oracle for the D3Q19 BGK block-LB hot path (TEST INFRASTRUCTURE ONLY).
  *
  * This is a plain-C restatement of the reference Peano block-LB arithmetic, used
- * only as the checker by tests/, __graft_entry__.smoke() and the cpu_baseline leg
- * of bench.py.  The product path (madpac_amd/) never links, loads or calls it.
+ * only to check results against, by tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py.  The product path (madpac_amd/) never links, loads or calls it.
  *
  * Every function reproduces the reference operation order so that the FP64 result
  * is bit-identical to the reference on x86-64 (no FMA contraction: build with

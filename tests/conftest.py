@@ -28,7 +28,7 @@ def golden(name):
 def channel_tolerance(d):
     """Relative bar of a channel fixture: 0 (bitwise) unless the inlet is parabolic on a box whose
     side ratios are not powers of two.  The reference evaluates the parabolic profile at physical
-    cell centres (domain size n_d / n_y), this build in lattice units; the two round alike when the
+    cell centres (domain size n_d / n_y), madpac_amd in lattice units; the two round alike when the
     ratios are powers of two and to a few ulps of the profile otherwise."""
     import numpy as np
     dims = [int(v) for v in d["dims"]]
