@@ -62,10 +62,11 @@ typedef struct {
 
 /* Order of the cells in device memory; the C-ABI always exchanges row-major arrays.  ROWS:
  * x + nx (y + ny z).  BRICKS: 4 x 4 x 4 bricks of consecutive cells, one wavefront per brick: far
- * fewer 128-byte lines straddle solid surfaces (porous 256^3: FP32-shifted +15-22 %, FP64 +5 %),
- * but shifted reads touch more lines (all-fluid 512^3: 9-18 % slower).  AUTO decides at each
- * lbm_set_flags from the flags: BRICKS when reading rows would fetch at least 1.12 cells per fluid
- * cell in 128-byte lines (solid cells sharing lines with fluid ones), ROWS otherwise. */
+ * fewer 128-byte lines straddle solid surfaces (porous 256^3 FP64: +3 %), but shifted reads touch
+ * more lines (all-fluid 512^3: 9-18 % slower).  AUTO decides at each lbm_set_flags: BRICKS for an
+ * FP64 context whose flags would make row reads fetch at least 1.12 cells per fluid cell in
+ * 128-byte lines (solid cells sharing lines with fluid ones), ROWS otherwise (FP32 measured
+ * faster in rows even on the porous lattice). */
 enum lbm_cell_order { LBM_CELL_ORDER_AUTO = 0, LBM_CELL_ORDER_ROWS = 1, LBM_CELL_ORDER_BRICKS = 2 };
 
 /* Storage of the state.  TWO_BUFFER: pull-form steps between two buffers.  AA: one buffer updated

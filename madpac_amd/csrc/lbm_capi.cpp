@@ -63,10 +63,10 @@ int rebuild_words(lbm_ctx* c) {
 
 // Cells fetched per fluid cell when rows are read in 128-byte lines (`per_line` cells): the number
 // of line-sized row segments holding any fluid cell, times their length, over the fluid cells.
-// 1 for an all-fluid box or walls that fill whole rows; 1.2-1.25 for the 20 %-solid sphere packing.
-// Bricks cut that waste (1.10-1.11 there) but a shifted read touches up to 4x more lines, which
-// costs 9 % (FP64) to 18 % (FP32) on an all-fluid lattice; interpolating the measured gains and
-// losses puts the break-even near 1.12.
+// 1 for an all-fluid box or walls that fill whole rows; 1.2 (FP64 lines) for the 20 %-solid sphere
+// packing.  Bricks cut that waste (1.10 there) but a shifted read touches up to 4x more lines,
+// which costs 9 % on an all-fluid FP64 lattice; interpolating the measured gains and losses puts
+// the break-even near 1.12.
 double row_line_waste(const lbm::DenseGeom& g, const uint8_t* inner, int per_line) {
   int64_t fetched = 0, fluid = 0;
   for (int64_t row = 0; row < (int64_t)g.ny * g.nz; row++) {
@@ -229,7 +229,9 @@ int lbm_set_flags(lbm_ctx* c, const uint8_t* inner) {
   if (!inner) return fail(c, LBM_ERR_ARG, "inner flags are null");
   HIPCHK(c, hipSetDevice(c->cfg.device));
   bool bricks = c->cfg.cell_order == LBM_CELL_ORDER_BRICKS;
-  if (c->cfg.cell_order == LBM_CELL_ORDER_AUTO) bricks = row_line_waste(c->g, inner, c->fp32 ? 32 : 16) >= 1.12;
+  // FP32 keeps rows: with whole-line writes they measured faster than bricks on the 256^3 sphere
+  // packing too (27.3k vs 26.7k MLUPS two-buffer, 22.3k vs 20.4k AA)
+  if (c->cfg.cell_order == LBM_CELL_ORDER_AUTO) bricks = !c->fp32 && row_line_waste(c->g, inner, 16) >= 1.12;
   if (bricks != (c->g.bricked != 0)) {
     int rc = switch_cell_order(c, bricks);
     if (rc) return rc;

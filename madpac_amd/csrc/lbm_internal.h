@@ -19,7 +19,7 @@ namespace lbm {
 //    touched); a wavefront is one brick, a workgroup BY bricks.  A 128-byte line holds a compact
 //    4 x 4 x 1 (FP64) or 4 x 4 x 2 (FP32) cell group, so far fewer lines straddle a solid surface
 //    and get fetched for a few fluid cells; but a shifted read touches up to 4x more lines than a
-//    row segment does.  Bricks win on porous lattices, rows on (nearly) all-fluid ones.
+//    row segment does.  Bricks win for FP64 on porous lattices, rows elsewhere.
 // Arrays that cross the C-ABI stay in the row-major user order r; kernels that read or write them
 // index both.  The stored state is the post-collision state of the last step; the reference's
 // post-streaming state is its pull (see k_dense_pull).

@@ -62,14 +62,14 @@ def _row_line_waste(solid, per_line):
 
 
 def test_automatic_choice_follows_row_line_waste():
-    # whole-plane walls waste no row lines (rows); a sphere packing does (bricks)
+    # whole-plane walls waste no row lines (rows); a sphere packing does (bricks, FP64 only)
     n = 32
     walls = np.zeros((n, n, n), bool)
     walls[:, 0, :] = walls[:, -1, :] = walls[0] = walls[-1] = True
     spheres = _porous_solid(n, 2, frac=0.2)
     for solid, prec in ((walls, lbm.FP64), (spheres, lbm.FP64), (spheres, lbm.FP32_SHIFTED)):
-        per_line = 16 if prec == lbm.FP64 else 32
-        expect = lbm.CELL_ORDER_BRICKS if _row_line_waste(solid, per_line) >= 1.12 else lbm.CELL_ORDER_ROWS
+        bricks = prec == lbm.FP64 and _row_line_waste(solid, 16) >= 1.12
+        expect = lbm.CELL_ORDER_BRICKS if bricks else lbm.CELL_ORDER_ROWS
         with lbm.Lattice(n, n, n, 0.7, precision=prec, periodic=(True, False, False)) as lat:
             lat.set_flags(~solid)
             assert lat.cell_order == expect
