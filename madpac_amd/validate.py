@@ -84,7 +84,8 @@ def porous_precision_sweep(n=256, steps=2000, taus=(0.55, 0.6, 0.7, 0.8, 1.0), g
 PRECISIONS = {"fp64": lbm.FP64, "fp32s": lbm.FP32_SHIFTED, "fp32s_f32math": lbm.FP32_SHIFTED_F32}
 
 
-def taylor_green(n=512, steps=5000, u0=0.04, tau=0.52, precisions=("fp32s", "fp64"), every=250):
+def taylor_green(n=512, steps=5000, u0=0.04, tau=0.52, precisions=("fp32s", "fp64"), every=250,
+                 layout=lbm.LAYOUT_TWO_BUFFER):
     nu = (tau - 0.5) / 3.0
     k = 2 * np.pi / n
     t0 = time.perf_counter()
@@ -93,11 +94,13 @@ def taylor_green(n=512, steps=5000, u0=0.04, tau=0.52, precisions=("fp32s", "fp6
     u = u.reshape(-1, 3)
     _log(f"[validate] TGV {n}^3 initial fields ({time.perf_counter() - t0:.1f} s)")
     out = {"workload": f"Taylor-Green vortex {n}^3 periodic, u0={u0}, tau={tau} (Re = u0 n / (2 pi nu) = "
-                       f"{u0 * n / (2 * np.pi * nu):.0f}), {steps} steps", "runs": {}}
+                       f"{u0 * n / (2 * np.pi * nu):.0f}), {steps} steps"
+                       + (", in-place AA layout (one state buffer)" if layout == lbm.LAYOUT_AA else ""),
+           "runs": {}}
     curves = {}
     for name in precisions:
         prec = PRECISIONS[name]
-        with lbm.Lattice(n, n, n, tau, precision=prec, periodic=(True, True, True)) as lat:
+        with lbm.Lattice(n, n, n, tau, precision=prec, periodic=(True, True, True), layout=layout) as lat:
             lat.set_flags(np.ones(n ** 3, np.uint8))
             lat.init_equilibrium(rho=rho, u=u)
             lat.tune(3)
@@ -113,7 +116,8 @@ def taylor_green(n=512, steps=5000, u0=0.04, tau=0.52, precisions=("fp32s", "fp6
         gbs = n ** 3 * 38 * esize / per / 1e9
         curves[name] = np.array(E)
         out["runs"][name] = {"ms_per_step": per * 1e3, "mlups": n ** 3 / per / 1e6, "achieved_GBs": gbs,
-                             "hbm_frac": gbs / 8000.0, "energy": E, "t": ts, "mass_drift": float(m[0] - n ** 3)}
+                             "hbm_frac": gbs / 8000.0, "energy": E, "t": ts, "mass_drift": float(m[0] - n ** 3),
+                             "state_GB": 19 * n ** 3 * esize * (1 if layout == lbm.LAYOUT_AA else 2) / 1e9}
         _log(f"[validate] TGV {name}: {n ** 3 / per / 1e6:.0f} MLUPS, {gbs:.0f} GB/s, E(T)/E0 {E[-1] / E[0]:.5f}")
     # early-time viscous decay E(t) = E0 exp(-2 nu k_eff^2 t) with k_eff^2 = 3 k^2 for this vortex
     first = next(iter(curves.values()))
@@ -205,6 +209,7 @@ def main():
     ap.add_argument("--size", type=int, default=None)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--precisions", default="fp32s,fp64")
+    ap.add_argument("--layout", choices=["two-buffer", "aa"], default="two-buffer")
     ap.add_argument("--out", default=None)
     a = ap.parse_args()
     if a.what == "precision":
@@ -212,7 +217,8 @@ def main():
     elif a.what == "adjoint":
         r = adjoint_fd(n=a.size or 128, steps=a.steps or 500)
     else:
-        r = taylor_green(n=a.size or 512, steps=a.steps or 5000, precisions=tuple(a.precisions.split(",")))
+        r = taylor_green(n=a.size or 512, steps=a.steps or 5000, precisions=tuple(a.precisions.split(",")),
+                         layout=lbm.LAYOUT_AA if a.layout == "aa" else lbm.LAYOUT_TWO_BUFFER)
     s = json.dumps(r)
     if a.out:
         with open(a.out, "w") as fh:
