@@ -6,9 +6,11 @@
 
 namespace lbm {
 
-// XCD-aware workgroup mapping.  Workgroups are dispatched round-robin over the 8 XCDs, so the
-// linear id is remapped to give each XCD one contiguous range of workgroups (a slab of z-planes or
-// brick layers): the neighbours a workgroup pulls from were then mostly fetched into the same L2.
+// Workgroup mapping.  Workgroups are dispatched round-robin over the 8 XCDs.  Rows keep that order,
+// so all XCDs stream the same few planes at a time; remapping the id to give each XCD its own slab
+// of planes was 3.5 % slower on the porous FP32 step and made the all-fluid rate depend on the
+// allocation (35.5k-38.1k MLUPS at 512^3 against a steady 35.6k).  Bricks (FP64) do remap: each
+// XCD owns one contiguous range of bricks (+1.5 %).
 // Rows: thread (x, y) of a workgroup owns cell (64 bx + x, 4 by + y, z).  Bricks: lane
 // threadIdx.x of wavefront threadIdx.y owns cell threadIdx.x of the brick in traversal slot
 // 4 * workgroup + threadIdx.y; slots run over tiles of 2 x 2 x 2 bricks (x fastest inside a tile,
@@ -25,8 +27,8 @@ __device__ __forceinline__ CellPos map_cell(const DenseGeom& g) {
   CellPos c;
   const int64_t bid = blockIdx.x;
   const int64_t per = g.nblocks_padded / NXCD;
-  const int64_t b = (bid % NXCD) * per + bid / NXCD;
   if (!g.bricked) {
+    const int64_t b = bid;
     const int bx = (int)(b % g.rnx);
     const int64_t r = b / g.rnx;
     const int by = (int)(r % g.rny);
@@ -37,6 +39,7 @@ __device__ __forceinline__ CellPos map_cell(const DenseGeom& g) {
     c.i = c.r = (int64_t)c.x + (int64_t)g.nx * ((int64_t)c.y + (int64_t)g.ny * c.z);
     return c;
   }
+  const int64_t b = (bid % NXCD) * per + bid / NXCD;
   const int64_t slot = b * BY + threadIdx.y;
   const int l = threadIdx.x;
   const int64_t tile = slot >> 3;
