@@ -173,12 +173,15 @@ int lbm_step(lbm_ctx* ctx, int nsteps);
 int lbm_step_graph(lbm_ctx* ctx, int nsteps);
 /* Same, timed with HIP events on the context stream; *ms = elapsed device time. */
 int lbm_step_timed(lbm_ctx* ctx, int nsteps, float* ms);
-/* Times the step kernel's two load schedules for nsteps steps each on a copy of the current state
- * (the state is restored) and keeps the faster one; *variant (may be NULL) receives it: 0 = flag
- * word first, then the 19 pulled loads (solid cells load nothing); 1 = the 19 neighbour loads issue
- * alongside the flag word and bounce-back populations are re-loaded afterwards.  With
- * nsteps == 0, *variant is an input and selects the schedule directly.  Both give bit-identical
- * results.  No reference counterpart (the reference's CPU traversal has no such choice). */
+/* Times the step kernel's variants for nsteps steps each on a copy of the current state (the state
+ * is restored) and keeps the fastest; *variant (may be NULL) receives it.  Bit 0, the load
+ * schedule: 0 = flag word first, then the 19 pulled loads (solid cells load nothing); 1 = the 19
+ * neighbour loads issue alongside the flag word and bounce-back populations are re-loaded
+ * afterwards (two-buffer layout only).  Bit 1, the workgroup order: 0 = the hardware's round-robin
+ * dispatch over the XCDs; 1 = each XCD a contiguous range of the lattice.  Which is faster depends
+ * on lattice size, cell order and allocation.  With nsteps == 0, *variant (0..3) is an input and
+ * selects directly.  When the state copy does not fit the device, the current choice is kept.  All
+ * variants give bit-identical results.  No reference counterpart. */
 int lbm_tune(lbm_ctx* ctx, int nsteps, int* variant);
 int lbm_synchronize(lbm_ctx* ctx);
 

@@ -507,6 +507,7 @@ namespace {
 lbm_graph_key graph_key(const lbm_ctx* c) {
   lbm_graph_key k;
   k.bricked = c->g.bricked;
+  k.remap = c->g.remap;
   k.walls = c->walls;
   k.force = c->force;
   k.spec = c->spec;
@@ -609,28 +610,33 @@ int lbm_tune(lbm_ctx* c, int nsteps, int* variant) {
   if (!c) return LBM_ERR_ARG;
   if (nsteps < 0) return fail(c, LBM_ERR_ARG, "nsteps must be >= 0");
   if (nsteps == 0) {  // explicit choice
-    if (!variant || (*variant != 0 && *variant != 1)) return fail(c, LBM_ERR_ARG, "variant must be 0 or 1");
-    c->spec = *variant == 1;
-    return LBM_OK;
-  }
-  if (c->aa) {  // the AA kernels have one load schedule
-    if (variant) *variant = 0;
+    if (!variant || *variant < 0 || *variant > 3) return fail(c, LBM_ERR_ARG, "variant must be in 0..3");
+    c->spec = !c->aa && (*variant & 1);
+    c->g.remap = (*variant >> 1) & 1;
     return LBM_OK;
   }
   int rc = need_state(c);
   if (rc) return rc;
   HIPCHK(c, hipSetDevice(c->cfg.device));
+  // time each variant on a copy of the state (both buffers hold the same start, so two-buffer and
+  // in-place steps begin from it alike), then restore it
   const size_t bytes = (size_t)Q * c->g.N * c->esize;
-  rc = ensure_scratch(c, bytes);
-  if (rc) return rc;
-  const int cur0 = c->cur;
+  if (ensure_scratch(c, bytes) != LBM_OK) {  // no room for the copy (e.g. a 1024^3 AA state): keep the defaults
+    c->err.clear();
+    if (variant) *variant = (c->spec ? 1 : 0) | (c->g.remap << 1);
+    return LBM_OK;
+  }
+  const int cur0 = c->cur, par0 = c->parity;
   HIPCHK(c, hipMemcpyAsync(c->scratch, c->buf[cur0], bytes, hipMemcpyDeviceToDevice, c->stream));
   float best = 0.f;
-  bool best_spec = false;
-  for (int v = 0; v < 2; v++) {
-    c->spec = v == 1;
+  int best_v = -1;
+  for (int v = 0; v < 4; v++) {
+    if (c->aa && (v & 1)) continue;  // the AA kernels have one load schedule
+    c->spec = (v & 1) != 0;
+    c->g.remap = v >> 1;
     HIPCHK(c, hipMemcpyAsync(c->buf[cur0], c->scratch, bytes, hipMemcpyDeviceToDevice, c->stream));
     c->cur = cur0;
+    c->parity = par0;
     rc = lbm_enqueue_steps(c, 2);  // warm-up
     if (rc) return rc;
     float ms = 0.f;
@@ -640,16 +646,18 @@ int lbm_tune(lbm_ctx* c, int nsteps, int* variant) {
     HIPCHK(c, hipEventRecord(c->ev1, c->stream));
     HIPCHK(c, hipEventSynchronize(c->ev1));
     HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
-    if (v == 0 || ms < best) {
+    if (best_v < 0 || ms < best) {
       best = ms;
-      best_spec = c->spec;
+      best_v = v;
     }
   }
-  c->spec = best_spec;
+  c->spec = (best_v & 1) != 0;
+  c->g.remap = best_v >> 1;
   HIPCHK(c, hipMemcpyAsync(c->buf[cur0], c->scratch, bytes, hipMemcpyDeviceToDevice, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->cur = cur0;
-  if (variant) *variant = c->spec ? 1 : 0;
+  c->parity = par0;
+  if (variant) *variant = best_v;
   return LBM_OK;
 }
 

@@ -26,6 +26,7 @@ DenseGeom make_geom(int nx, int ny, int nz, const int periodic[3], bool bricked,
   g.nx = nx; g.ny = ny; g.nz = nz;
   g.px = periodic[0] ? 1 : 0; g.py = periodic[1] ? 1 : 0; g.pz = periodic[2] ? 1 : 0;
   g.bricked = bricked ? 1 : 0;
+  g.remap = bricked ? 1 : 0;  // the measured better default of each order; lbm_tune times both
   g.rnx = (nx + BX - 1) / BX; g.rny = (ny + BY - 1) / BY;
   g.bnx = (nx + 3) / 4; g.bny = (ny + 3) / 4; g.bnz = (nz + 3) / 4;
   g.tnx = (g.bnx + 1) / 2; g.tny = (g.bny + 1) / 2; g.tnz = (g.bnz + 1) / 2;

@@ -6,11 +6,12 @@
 
 namespace lbm {
 
-// Workgroup mapping.  Workgroups are dispatched round-robin over the 8 XCDs.  Rows keep that order,
-// so all XCDs stream the same few planes at a time; remapping the id to give each XCD its own slab
-// of planes was 3.5 % slower on the porous FP32 step and made the all-fluid rate depend on the
-// allocation (35.5k-38.1k MLUPS at 512^3 against a steady 35.6k).  Bricks (FP64) do remap: each
-// XCD owns one contiguous range of bricks (+1.5 %).
+// Workgroup mapping.  Workgroups are dispatched round-robin over the 8 XCDs.  With g.remap the id is
+// remapped so that each XCD owns one contiguous range of workgroups (a slab of planes or brick
+// layers); without it all XCDs stream the same few planes at a time.  Which is faster depends on
+// the lattice and the allocation (porous 256^3 FP32 rows: dispatch order +3.5 %; FP64 bricks: remap
+// +1.5 %; all-fluid 512^3 rows: dispatch order 38.3k MLUPS, remap 35.5k-38.1k; 1024^3: the reverse),
+// so lbm_tune times both; the defaults are dispatch order for rows and remap for bricks.
 // Rows: thread (x, y) of a workgroup owns cell (64 bx + x, 4 by + y, z).  Bricks: lane
 // threadIdx.x of wavefront threadIdx.y owns cell threadIdx.x of the brick in traversal slot
 // 4 * workgroup + threadIdx.y; slots run over tiles of 2 x 2 x 2 bricks (x fastest inside a tile,
@@ -27,8 +28,8 @@ __device__ __forceinline__ CellPos map_cell(const DenseGeom& g) {
   CellPos c;
   const int64_t bid = blockIdx.x;
   const int64_t per = g.nblocks_padded / NXCD;
+  const int64_t b = g.remap ? (bid % NXCD) * per + bid / NXCD : bid;
   if (!g.bricked) {
-    const int64_t b = bid;
     const int bx = (int)(b % g.rnx);
     const int64_t r = b / g.rnx;
     const int by = (int)(r % g.rny);
@@ -39,7 +40,6 @@ __device__ __forceinline__ CellPos map_cell(const DenseGeom& g) {
     c.i = c.r = (int64_t)c.x + (int64_t)g.nx * ((int64_t)c.y + (int64_t)g.ny * c.z);
     return c;
   }
-  const int64_t b = (bid % NXCD) * per + bid / NXCD;
   const int64_t slot = b * BY + threadIdx.y;
   const int l = threadIdx.x;
   const int64_t tile = slot >> 3;

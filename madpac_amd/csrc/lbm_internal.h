@@ -31,6 +31,7 @@ struct DenseGeom {
   int nx, ny, nz;
   int px, py, pz;            // periodic flags
   int bricked;               // 0 = rows, 1 = bricks
+  int remap;                 // 1: XCD-contiguous workgroup ranges (map_cell), 0: dispatch order
   int rnx, rny;              // rows: workgroups along x and y
   int bnx, bny, bnz;         // bricks per dimension
   int tnx, tny, tnz;         // bricks: traversal tiles of 2 x 2 x 2 bricks per dimension

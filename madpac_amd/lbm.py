@@ -297,8 +297,8 @@ class Lattice:
         return float(ms.value)
 
     def tune(self, nsteps=10, variant=None):
-        """Pick the faster step-kernel load schedule on the current state (returns 0 or 1), or with
-        `variant` given, select that schedule directly."""
+        """Pick the fastest step-kernel variant on the current state (returns 0..3: bit 0 the load
+        schedule, bit 1 the XCD-contiguous workgroup order), or with `variant` given, select it."""
         v = ctypes.c_int(0 if variant is None else int(variant))
         n = 0 if variant is not None else int(nsteps)
         self._check(self._L.lbm_tune(self._h, n, ctypes.byref(v)), "lbm_tune")

@@ -103,7 +103,7 @@ def taylor_green(n=512, steps=5000, u0=0.04, tau=0.52, precisions=("fp32s", "fp6
         with lbm.Lattice(n, n, n, tau, precision=prec, periodic=(True, True, True), layout=layout) as lat:
             lat.set_flags(np.ones(n ** 3, np.uint8))
             lat.init_equilibrium(rho=rho, u=u)
-            lat.tune(3)
+            lat.tune(10)
             lat.init_equilibrium(rho=rho, u=u)
             E, ts, ms_total = [lat.kinetic_energy()], [0], 0.0
             for s in range(every, steps + 1, every):
