@@ -290,10 +290,10 @@ extern "C" int lbm_adjoint_pressure_drop(lbm_ctx* c, int nsteps, int snapshots, 
     c->adj = a;
     a->slots.assign(nslots, nullptr);
     bool ok = true;
-    for (int k = 0; k < nslots && ok; k++) ok = hipMalloc(&a->slots[k], sbytes) == hipSuccess;
-    ok = ok && hipMalloc(&a->final_state, sbytes) == hipSuccess;
-    ok = ok && hipMalloc(&a->lam[0], sbytes) == hipSuccess && hipMalloc(&a->lam[1], sbytes) == hipSuccess;
-    ok = ok && hipMalloc(&a->galpha, (size_t)N * sizeof(double)) == hipSuccess;
+    for (int k = 0; k < nslots && ok; k++) ok = lbm::device_alloc(&a->slots[k], sbytes) == hipSuccess;
+    ok = ok && lbm::device_alloc(&a->final_state, sbytes) == hipSuccess;
+    ok = ok && lbm::device_alloc(&a->lam[0], sbytes) == hipSuccess && lbm::device_alloc(&a->lam[1], sbytes) == hipSuccess;
+    ok = ok && lbm::device_alloc(&a->galpha, (size_t)N * sizeof(double)) == hipSuccess;
     if (!ok) {
       lbm_adjoint_release(c);
       return adj_fail(c, LBM_ERR_NOMEM, "adjoint workspace allocation failed");

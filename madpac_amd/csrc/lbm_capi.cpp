@@ -40,7 +40,7 @@ int ensure_scratch(lbm_ctx* c, size_t bytes) {
   if (c->scratch) (void)hipFree(c->scratch);
   c->scratch = nullptr;
   c->scratch_bytes = 0;
-  if (hipMalloc(&c->scratch, bytes) != hipSuccess) return fail(c, LBM_ERR_NOMEM, "scratch allocation failed");
+  if (lbm::device_alloc(&c->scratch, bytes) != hipSuccess) return fail(c, LBM_ERR_NOMEM, "scratch allocation failed");
   c->scratch_bytes = bytes;
   return LBM_OK;
 }
@@ -181,12 +181,12 @@ int lbm_create(lbm_ctx** out, const lbm_config* cfg) {
   // sized for the larger (bricked) extent, so that lbm_set_flags can switch the cell order
   const size_t pbytes = (size_t)Q * c->g_bricks.N * c->esize;
   c->npartials = std::max(lbm::dense_moment_partials(c->g_rows), lbm::dense_moment_partials(c->g_bricks));
-  if (hipMalloc(&c->buf[0], pbytes) != hipSuccess || (!c->aa && hipMalloc(&c->buf[1], pbytes) != hipSuccess) ||
-      hipMalloc(&c->words, (size_t)c->g_bricks.N * sizeof(uint32_t)) != hipSuccess ||
-      hipMalloc(&c->d_inner, (size_t)c->g.Nu) != hipSuccess || hipMalloc(&c->d_cls, (size_t)c->g.Nu) != hipSuccess ||
-      hipMalloc(&c->partials, (size_t)(c->npartials + 1024) * 4 * sizeof(double)) != hipSuccess ||
-      hipMalloc(&c->small, 8 * sizeof(double)) != hipSuccess ||
-      hipMalloc(&c->d_bct, sizeof(c->bct)) != hipSuccess)
+  if (lbm::device_alloc(&c->buf[0], pbytes) != hipSuccess || (!c->aa && lbm::device_alloc(&c->buf[1], pbytes) != hipSuccess) ||
+      lbm::device_alloc(&c->words, (size_t)c->g_bricks.N * sizeof(uint32_t)) != hipSuccess ||
+      lbm::device_alloc(&c->d_inner, (size_t)c->g.Nu) != hipSuccess || lbm::device_alloc(&c->d_cls, (size_t)c->g.Nu) != hipSuccess ||
+      lbm::device_alloc(&c->partials, (size_t)(c->npartials + 1024) * 4 * sizeof(double)) != hipSuccess ||
+      lbm::device_alloc(&c->small, 8 * sizeof(double)) != hipSuccess ||
+      lbm::device_alloc(&c->d_bct, sizeof(c->bct)) != hipSuccess)
     return bail(LBM_ERR_NOMEM, "device allocation failed");
   if (hipMemsetAsync(c->buf[0], 0, pbytes, c->stream) != hipSuccess ||
       (c->buf[1] && hipMemsetAsync(c->buf[1], 0, pbytes, c->stream) != hipSuccess) ||
@@ -291,7 +291,7 @@ int lbm_set_boundary_velocity(lbm_ctx* c, const double* u) {
         for (int d = 0; d < 3; d++) dev[(size_t)(3 * p + d)] = u[3 * r + d];
       }
   // sized for either cell order (lbm_set_flags may switch it)
-  if (!c->d_bcu && hipMalloc(&c->d_bcu, 3 * (size_t)c->g_bricks.N * sizeof(double)) != hipSuccess)
+  if (!c->d_bcu && lbm::device_alloc(&c->d_bcu, 3 * (size_t)c->g_bricks.N * sizeof(double)) != hipSuccess)
     return fail(c, LBM_ERR_NOMEM, "boundary velocity allocation failed");
   HIPCHK(c, hipMemcpyAsync(c->d_bcu, dev.data(), dev.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -403,7 +403,7 @@ int lbm_set_porosity(lbm_ctx* c, const double* alpha) {
   for (int64_t i = 0; i < c->g.Nu; i++)
     if (!(alpha[i] >= 0.0 && alpha[i] <= 1.0)) return fail(c, LBM_ERR_ARG, "porosity must lie in [0,1]");
   if (c->couple_on) return fail(c, LBM_ERR_STATE, "coupled cells use the plain collision: no porosity");
-  if (!c->alpha && hipMalloc(&c->alpha, (size_t)c->g_bricks.N * c->esize) != hipSuccess)
+  if (!c->alpha && lbm::device_alloc(&c->alpha, (size_t)c->g_bricks.N * c->esize) != hipSuccess)
     return fail(c, LBM_ERR_NOMEM, "porosity allocation failed");
   // the kernels index the porosity by stored (bricked) cell; padding cells get 0
   const lbm::DenseGeom& g = c->g;

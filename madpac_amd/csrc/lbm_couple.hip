@@ -223,10 +223,10 @@ int lbm_set_coupling_cells(lbm_ctx* c, const int64_t* cells, int64_t n) {
     if (cells[k] < 0 || cells[k] >= c->g.Nu) return cfail(c, LBM_ERR_ARG, "coupling cell out of range");
     if (seen[(size_t)cells[k]]++) return cfail(c, LBM_ERR_ARG, "coupling cells must be distinct");
   }
-  if (hipMalloc(&c->d_ccells, (size_t)n * sizeof(int64_t)) != hipSuccess ||
-      hipMalloc(&c->d_cu, (size_t)n * 3 * sizeof(double)) != hipSuccess ||
-      hipMalloc(&c->d_crho, (size_t)n * sizeof(double)) != hipSuccess ||
-      hipMalloc(&c->d_cout, (size_t)n * 4 * sizeof(double)) != hipSuccess) {
+  if (lbm::device_alloc(&c->d_ccells, (size_t)n * sizeof(int64_t)) != hipSuccess ||
+      lbm::device_alloc(&c->d_cu, (size_t)n * 3 * sizeof(double)) != hipSuccess ||
+      lbm::device_alloc(&c->d_crho, (size_t)n * sizeof(double)) != hipSuccess ||
+      lbm::device_alloc(&c->d_cout, (size_t)n * 4 * sizeof(double)) != hipSuccess) {
     lbm_coupling_release(c);
     return cfail(c, LBM_ERR_NOMEM, "coupling buffers allocation failed");
   }

@@ -26,12 +26,15 @@ DenseGeom make_geom(int nx, int ny, int nz, const int periodic[3], bool bricked,
   g.nx = nx; g.ny = ny; g.nz = nz;
   g.px = periodic[0] ? 1 : 0; g.py = periodic[1] ? 1 : 0; g.pz = periodic[2] ? 1 : 0;
   g.bricked = bricked ? 1 : 0;
-  g.remap = bricked ? 1 : 0;  // the measured better default of each order; lbm_tune times both
   g.rnx = (nx + BX - 1) / BX; g.rny = (ny + BY - 1) / BY;
   g.bnx = (nx + 3) / 4; g.bny = (ny + 3) / 4; g.bnz = (nz + 3) / 4;
   g.tnx = (g.bnx + 1) / 2; g.tny = (g.bny + 1) / 2; g.tnz = (g.bnz + 1) / 2;
   const int64_t bricks = (int64_t)g.bnx * g.bny * g.bnz;
   g.Nu = (int64_t)nx * ny * nz;
+  // untuned defaults (lbm_tune times both): measured better for bricks, and for rows from 2^29 cells
+  // up (1024^3: FP32 38.2k against 33.7k MLUPS, FP64 in place 19.2k against 17.8k; 256^3 and 512^3
+  // rows run faster in dispatch order)
+  g.remap = (bricked || g.Nu >= ((int64_t)1 << 29)) ? 1 : 0;
   g.N = (bricked ? bricks * 64 : g.Nu) + plane_pad(esize);
   // bricks: 8 brick slots per traversal tile, BY per workgroup
   g.nblocks = bricked ? (int64_t)g.tnx * g.tny * g.tnz * 8 / BY : (int64_t)g.rnx * g.rny * nz;

@@ -11,7 +11,8 @@ namespace lbm {
 // layers); without it all XCDs stream the same few planes at a time.  Which is faster depends on
 // the lattice and the allocation (porous 256^3 FP32 rows: dispatch order +3.5 %; FP64 bricks: remap
 // +1.5 %; all-fluid 512^3 rows: dispatch order 38.3k MLUPS, remap 35.5k-38.1k; 1024^3: the reverse),
-// so lbm_tune times both; the defaults are dispatch order for rows and remap for bricks.
+// so lbm_tune times both; the untuned defaults (make_geom) are the remap for bricks and for lattices
+// of 2^29 cells and more, dispatch order otherwise.
 // Rows: thread (x, y) of a workgroup owns cell (64 bx + x, 4 by + y, z).  Bricks: lane
 // threadIdx.x of wavefront threadIdx.y owns cell threadIdx.x of the brick in traversal slot
 // 4 * workgroup + threadIdx.y; slots run over tiles of 2 x 2 x 2 bricks (x fastest inside a tile,

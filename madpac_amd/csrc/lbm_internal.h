@@ -42,6 +42,19 @@ struct DenseGeom {
   int layout;                // how the stored state maps to the reference state (LAYOUT_*)
 };
 
+// hipMalloc that leaves no error behind when it fails: HIP keeps the failure as the thread's last
+// error, and the next launch check (hipGetLastError) would report it against an unrelated kernel.
+// The caller handles the returned error (e.g. lbm_tune falls back when a state copy does not fit).
+template <typename T>
+inline hipError_t device_alloc(T** p, size_t bytes) {
+  const hipError_t e = hipMalloc((void**)p, bytes);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    *p = nullptr;
+  }
+  return e;
+}
+
 // stored index of cell (x, y, z)
 __host__ __device__ inline int64_t stored_index(const DenseGeom& g, int x, int y, int z) {
   if (!g.bricked) return (int64_t)x + (int64_t)g.nx * ((int64_t)y + (int64_t)g.ny * z);

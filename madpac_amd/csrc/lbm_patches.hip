@@ -25,6 +25,7 @@
 
 #include "lbm_capi.h"
 #include "lbm_cell.h"
+#include "lbm_internal.h"
 
 namespace lbm {
 
@@ -409,7 +410,7 @@ int build_tiles(lbm_patchset* ps, const lbm_tiling& t) {
   }
   if (ps->d_tiles) (void)hipFree(ps->d_tiles);
   ps->d_tiles = nullptr;
-  if (hipMalloc(&ps->d_tiles, tiles.size() * sizeof(TileDev)) != hipSuccess)
+  if (lbm::device_alloc(&ps->d_tiles, tiles.size() * sizeof(TileDev)) != hipSuccess)
     return pfail(ps, LBM_ERR_NOMEM, "tile table allocation failed");
   PCHK(ps, hipMemcpy(ps->d_tiles, tiles.data(), tiles.size() * sizeof(TileDev), hipMemcpyHostToDevice));
   ps->ntiles = (int)tiles.size();
@@ -517,11 +518,11 @@ int lbm_patches_create(lbm_patchset** out, const lbm_patch* table, int npatches,
   int maxn = 0;
   for (auto& d : ps->host) maxn = std::max(maxn, d.n);
   ps->npartials = (int64_t)npatches * 64;
-  if (hipMalloc(&ps->buf[0], bytes) != hipSuccess || hipMalloc(&ps->buf[1], bytes) != hipSuccess ||
-      hipMalloc(&ps->d_patches, ps->host.size() * sizeof(PatchDev)) != hipSuccess ||
-      hipMalloc(&ps->d_cell_base, ps->cell_base.size() * sizeof(int64_t)) != hipSuccess ||
-      hipMalloc(&ps->partials, (size_t)ps->npartials * 4 * sizeof(double)) != hipSuccess ||
-      hipMalloc(&ps->small, 8 * sizeof(double)) != hipSuccess)
+  if (lbm::device_alloc(&ps->buf[0], bytes) != hipSuccess || lbm::device_alloc(&ps->buf[1], bytes) != hipSuccess ||
+      lbm::device_alloc(&ps->d_patches, ps->host.size() * sizeof(PatchDev)) != hipSuccess ||
+      lbm::device_alloc(&ps->d_cell_base, ps->cell_base.size() * sizeof(int64_t)) != hipSuccess ||
+      lbm::device_alloc(&ps->partials, (size_t)ps->npartials * 4 * sizeof(double)) != hipSuccess ||
+      lbm::device_alloc(&ps->small, 8 * sizeof(double)) != hipSuccess)
     return bail(LBM_ERR_NOMEM, "device allocation failed");
   if (hipMemcpy(ps->d_patches, ps->host.data(), ps->host.size() * sizeof(PatchDev), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(ps->d_cell_base, ps->cell_base.data(), ps->cell_base.size() * sizeof(int64_t), hipMemcpyHostToDevice) !=
@@ -557,9 +558,9 @@ int lbm_patches_init(lbm_patchset* ps, const double* rho, const double* u) {
   if (!ps) return LBM_ERR_ARG;
   PCHK(ps, hipSetDevice(ps->device));
   double *d_rho = nullptr, *d_u = nullptr;
-  if (rho && hipMalloc(&d_rho, (size_t)ps->cells * sizeof(double)) != hipSuccess)
+  if (rho && lbm::device_alloc(&d_rho, (size_t)ps->cells * sizeof(double)) != hipSuccess)
     return pfail(ps, LBM_ERR_NOMEM, "staging allocation failed");
-  if (u && hipMalloc(&d_u, (size_t)ps->cells * 3 * sizeof(double)) != hipSuccess) {
+  if (u && lbm::device_alloc(&d_u, (size_t)ps->cells * 3 * sizeof(double)) != hipSuccess) {
     if (d_rho) (void)hipFree(d_rho);
     return pfail(ps, LBM_ERR_NOMEM, "staging allocation failed");
   }
@@ -594,7 +595,7 @@ int lbm_patches_get_populations(lbm_patchset* ps, double* f) {
   PCHK(ps, hipSetDevice(ps->device));
   double* d_f = nullptr;
   const size_t bytes = (size_t)ps->cells * Q * sizeof(double);
-  if (hipMalloc(&d_f, bytes) != hipSuccess) return pfail(ps, LBM_ERR_NOMEM, "staging allocation failed");
+  if (lbm::device_alloc(&d_f, bytes) != hipSuccess) return pfail(ps, LBM_ERR_NOMEM, "staging allocation failed");
   int maxn = 0;
   for (auto& d : ps->host) maxn = std::max(maxn, d.n);
   const int64_t maxc = (int64_t)maxn * maxn * maxn;
@@ -663,7 +664,7 @@ int lbm_patches_autotune(lbm_patchset* ps, lbm_tiling* chosen) {
   // time on a copy of the state, restore afterwards
   const size_t bytes = (size_t)ps->elems * ps->esize;
   void* save = nullptr;
-  if (hipMalloc(&save, bytes) != hipSuccess) return pfail(ps, LBM_ERR_NOMEM, "autotune staging allocation failed");
+  if (lbm::device_alloc(&save, bytes) != hipSuccess) return pfail(ps, LBM_ERR_NOMEM, "autotune staging allocation failed");
   const int cur0 = ps->cur;
   PCHK(ps, hipMemcpy(save, ps->buf[cur0], bytes, hipMemcpyDeviceToDevice));
   lbm_tiling best{};

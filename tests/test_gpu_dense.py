@@ -262,3 +262,16 @@ def test_nan_is_detected_by_reduction_guard():
         lat.set_populations(f)
         with pytest.raises(lbm.LbmError, match="non-finite"):
             lat.moments()
+
+
+def test_failed_allocation_leaves_no_error_behind():
+    # a 2048^3 lattice (the largest extent accepted) needs 1.3 TB per FP64 buffer: lbm_create's
+    # device allocation fails cleanly, and the failure must not surface later as the launch error
+    # of an unrelated context
+    with pytest.raises(lbm.LbmError, match="allocation"):
+        lbm.Lattice(2048, 2048, 2048, 0.7)
+    with lbm.cavity(8, 0.6, 0.05) as lat:
+        lat.step(7)
+        f = lat.get_populations()
+    ora = O.Lattice.cavity(8, 0.6, 0.05)
+    assert np.array_equal(f, ora.step(ora.equilibrium(), 7))
