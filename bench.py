@@ -200,6 +200,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-patches", action="store_true")
+    ap.add_argument("--no-all-fluid", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: madpac_amd has no multi-GPU path (the domain would shard "
@@ -228,8 +229,11 @@ def main():
         for name, prec in (("fp32s", lbm.FP32_SHIFTED), ("fp64", lbm.FP64)):
             patches[name] = run_patches(lbm, wl, prec)
             log(f"[bench] patches {name}: {patches[name]['value']:.0f} MLUPS, tiling {patches[name]['tiling']}")
-    tgv = run_taylor_green(lbm, wl)
-    log(f"[bench] Taylor-Green 512^3: fp64 {tgv['fp64']['value']:.0f} MLUPS, fp32s {tgv['fp32s']['value']:.0f} MLUPS")
+    tgv = None
+    if not args.no_all_fluid:
+        tgv = run_taylor_green(lbm, wl)
+        log(f"[bench] Taylor-Green 512^3: fp64 {tgv['fp64']['value']:.0f} MLUPS, "
+            f"fp32s {tgv['fp32s']['value']:.0f} MLUPS")
     adj = None
     if not args.no_adjoint:
         adj = run_adjoint(lbm, wl)
