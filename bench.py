@@ -104,6 +104,30 @@ def run_taylor_green(lbm, wl, n=512, steps=200, tau=0.52, u0=0.04):
     return out
 
 
+def run_shard_share(lbm, solid, tau, g, planes=32, steps=300):
+    """One GPU's share of an 8-way z-slab split of the porous channel: the first `planes` + 2 planes of
+    the same geometry (the 2 extra planes stand in for the ghost planes a sharded run would fill from
+    its neighbours; here they are updated too, with bounce-back beyond them)."""
+    n = solid.shape[1]
+    nz = planes + 2
+    sub = np.ascontiguousarray(solid[:nz])
+    out = {"workload": f"porous channel slab {n}x{n}x{nz} (1/8 of {n}^3 plus 2 ghost-plane stand-ins)"}
+    for name, prec in (("fp64", lbm.FP64), ("fp32s", lbm.FP32_SHIFTED)):
+        with lbm.Lattice(n, n, nz, tau, precision=prec, periodic=(True, True, False), force=(g, 0.0, 0.0)) as lat:
+            lat.set_flags(~sub)
+            lat.init_equilibrium()
+            lat.tune(20)
+            lat.step(20)
+            ms = lat.step_timed(steps) / steps
+            nfl = lat.fluid_cells
+            order = "bricks" if lat.cell_order == lbm.CELL_ORDER_BRICKS else "rows"
+        esize = 8 if prec == lbm.FP64 else 4
+        out[name] = {"value": round(nfl / (ms * 1e3), 1), "unit": "MLUPS", "ms_per_step": round(ms, 4),
+                     "roofline_frac": round(nfl * 38 * esize / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                     "cell_order": order}
+    return out
+
+
 def run_adjoint(lbm, wl, n=128, steps=500, snapshots=23):
     solid = wl.channel_walls(n, n, n)
     alpha = wl.fourier_porosity(n, n, n).reshape(-1)
@@ -229,6 +253,9 @@ def main():
         for name, prec in (("fp32s", lbm.FP32_SHIFTED), ("fp64", lbm.FP64)):
             patches[name] = run_patches(lbm, wl, prec)
             log(f"[bench] patches {name}: {patches[name]['value']:.0f} MLUPS, tiling {patches[name]['tiling']}")
+    shard = run_shard_share(lbm, solid, args.tau, 1e-6)
+    log(f"[bench] slab share {n}x{n}x34: fp64 {shard['fp64']['value']:.0f} MLUPS, "
+        f"fp32s {shard['fp32s']['value']:.0f} MLUPS")
     tgv = None
     if not args.no_all_fluid:
         tgv = run_taylor_green(lbm, wl)
@@ -269,6 +296,7 @@ def main():
         "wall_mlups": {"fp64": round(r64["wall_mlups"], 1), "fp32s": round(r32["wall_mlups"], 1)},
         "batched_patches": patches,
         "all_fluid": tgv,
+        "slab_share": shard,
         "adjoint": adj,
         "cpu_baseline": cpu,
     }

@@ -119,32 +119,87 @@ __device__ __forceinline__ void collide_patch(T (&f)[Q], double omega) {
   }
 }
 
+// Slot arithmetic of one interior cell without branches (the step kernels' form of slot_offset).
+// Along each axis a cell touches at most one face (n >= 2).  The pull source x - c_q lies outside
+// along axis a iff c_{q,a} = +1 at coordinate 0 or -1 at n - 1; the push target x + c_q iff
+// c_{q,a} = +1 at n - 1 or -1 at 0.  Every alternative (own plane, face record, edge record) is a
+// tile-uniform base plus a per-lane term, picked by those flags; all offsets fit 32 bits (n <= 256).
+struct PatchLane {
+  int x, y, z, i, n, n2, n3;
+  bool lx, hx, ly, hy, lz, hz;  // coordinate == 0 / == n - 1
+};
+
+__device__ __forceinline__ PatchLane patch_lane(int n, int x, int y, int z) {
+  PatchLane L;
+  L.x = x; L.y = y; L.z = z;
+  L.n = n; L.n2 = n * n; L.n3 = n * n * n;
+  L.i = x + n * (y + n * z);
+  L.lx = x == 0; L.hx = x == n - 1;
+  L.ly = y == 0; L.hy = y == n - 1;
+  L.lz = z == 0; L.hz = z == n - 1;
+  return L;
+}
+
+__device__ __forceinline__ bool src_outside(int c, bool lo, bool hi) { return c > 0 ? lo : (c < 0 ? hi : false); }
+__device__ __forceinline__ bool dst_outside(int c, bool lo, bool hi) { return c > 0 ? hi : (c < 0 ? lo : false); }
+
+// Offset from the patch base of the slot that population q of the cell is pulled from: equal to
+// slot_offset(q, n, x - c_x, y - c_y, z - c_z).
+__device__ __forceinline__ int pull_slot(int q, const PatchLane& L) {
+  const int cx = CX(q), cy = CY(q), cz = CZ(q);
+  const bool ox = src_outside(cx, L.lx, L.hx), oy = src_outside(cy, L.ly, L.hy), oz = src_outside(cz, L.lz, L.hz);
+  const int n = L.n, n2 = L.n2, G0 = Q * L.n3, E0 = G0 + 30 * n2;
+  int off = q * L.n3 + L.i - (cx + n * cy + n2 * cz);
+  // a face record holds the source's two other coordinates (lower axis fastest), an edge record the third
+  if (cx != 0 && ox) off = G0 + (5 * (cx < 0) + face_local(q, 0)) * n2 + (L.y - cy) + n * (L.z - cz);
+  if (cy != 0 && oy) off = G0 + (5 * (2 + (cy < 0)) + face_local(q, 1)) * n2 + (L.x - cx) + n * (L.z - cz);
+  if (cz != 0 && oz) off = G0 + (5 * (4 + (cz < 0)) + face_local(q, 2)) * n2 + (L.x - cx) + n * (L.y - cy);
+  if (cx != 0 && cy != 0 && ox && oy) off = E0 + (2 * (cx < 0) + (cy < 0)) * n + L.z;
+  if (cx != 0 && cz != 0 && ox && oz) off = E0 + (4 + 2 * (cx < 0) + (cz < 0)) * n + L.y;
+  if (cy != 0 && cz != 0 && oy && oz) off = E0 + (8 + 2 * (cy < 0) + (cz < 0)) * n + L.x;
+  return off;
+}
+
 // One interior cell (x, y, z) of a patch: pull (ghost records included), collide, store the own
 // planes, and write each population that leaves the patch into the receiving neighbour's ghost
-// record.  The receiver sees the source at s = (x, y, z) - n e (e = the crossed face or edge).
+// record.  The receiver (direction e = the crossed face or edge, e = c_q on the crossed axes) sees
+// the source at s = (x, y, z) - n e, so the record is indexed by the cell's own other coordinates:
+// the write goes to slot_offset(q, n, x - n e_x, y - n e_y, z - n e_z) of that neighbour.  The
+// neighbours' bases are tile-uniform loads from the patch record.
 template <typename T, bool M32>
 __device__ __forceinline__ void patch_cell(const PatchDev* __restrict__ pp, const T* __restrict__ s,
                                            T* __restrict__ d, T* __restrict__ dst, int n, double omega, int x, int y,
                                            int z) {
   T f[Q];
+  {
+    const PatchLane L = patch_lane(n, x, y, z);
 #pragma unroll
-  for (int q = 0; q < Q; q++) f[q] = s[slot_offset(q, n, x - CX(q), y - CY(q), z - CZ(q))];
+    for (int q = 0; q < Q; q++) f[q] = s[pull_slot(q, L)];
+  }
   collide_patch<T, M32>(f, omega);
-  const int64_t n1 = n, n3 = n1 * n1 * n1;
-  const int64_t i = x + n1 * (y + n1 * z);
+  // Rebuild the lane terms from opaque copies of the coordinates: kept live through the collision
+  // they cost more registers than the collision leaves (spills).
+  asm volatile("" : "+v"(x), "+v"(y), "+v"(z));
+  const PatchLane L = patch_lane(n, x, y, z);
 #pragma unroll
-  for (int q = 0; q < Q; q++) __builtin_nontemporal_store(f[q], &d[(int64_t)q * n3 + i]);
-  if (x == 0 || y == 0 || z == 0 || x == n - 1 || y == n - 1 || z == n - 1) {
+  for (int q = 0; q < Q; q++) __builtin_nontemporal_store(f[q], &d[q * L.n3 + L.i]);
+  if (L.lx | L.hx | L.ly | L.hy | L.lz | L.hz) {
+    const int n2 = L.n2, G0 = Q * L.n3, E0 = G0 + 30 * n2;
 #pragma unroll
     for (int q = 0; q < Q; q++) {
       if (q == REST) continue;
-      const int tx = x + CX(q), ty = y + CY(q), tz = z + CZ(q);
-      const int ex = tx < 0 ? -1 : (tx >= n ? 1 : 0);
-      const int ey = ty < 0 ? -1 : (ty >= n ? 1 : 0);
-      const int ez = tz < 0 ? -1 : (tz >= n ? 1 : 0);
-      if ((ex | ey | ez) == 0) continue;
-      const int k = dir_index(ex, ey, ez);  // a face or edge direction: always a D3Q19 direction
-      dst[pp->nbase[k] + slot_offset(q, n, x - n * ex, y - n * ey, z - n * ez)] = f[q];
+      const int cx = CX(q), cy = CY(q), cz = CZ(q);
+      const bool ox = dst_outside(cx, L.lx, L.hx), oy = dst_outside(cy, L.ly, L.hy), oz = dst_outside(cz, L.lz, L.hz);
+      if (!(ox | oy | oz)) continue;
+      int64_t nb = 0;
+      int off = 0;
+      if (cx != 0 && ox) { nb = pp->nbase[dir_index(cx, 0, 0)]; off = G0 + (5 * (cx < 0) + face_local(q, 0)) * n2 + L.y + n * L.z; }
+      if (cy != 0 && oy) { nb = pp->nbase[dir_index(0, cy, 0)]; off = G0 + (5 * (2 + (cy < 0)) + face_local(q, 1)) * n2 + L.x + n * L.z; }
+      if (cz != 0 && oz) { nb = pp->nbase[dir_index(0, 0, cz)]; off = G0 + (5 * (4 + (cz < 0)) + face_local(q, 2)) * n2 + L.x + n * L.y; }
+      if (cx != 0 && cy != 0 && ox && oy) { nb = pp->nbase[q]; off = E0 + (2 * (cx < 0) + (cy < 0)) * n + L.z; }
+      if (cx != 0 && cz != 0 && ox && oz) { nb = pp->nbase[q]; off = E0 + (4 + 2 * (cx < 0) + (cz < 0)) * n + L.y; }
+      if (cy != 0 && cz != 0 && oy && oz) { nb = pp->nbase[q]; off = E0 + (8 + 2 * (cy < 0) + (cz < 0)) * n + L.x; }
+      dst[nb + off] = f[q];
     }
   }
 }

@@ -58,6 +58,9 @@ def _oracle_torus(table, first, dims, n, u_patch, steps):
 @pytest.mark.parametrize("tori", [
     [((2, 2, 2), 8)],
     [((2, 1, 1), 16), ((1, 1, 1), 8), ((3, 2, 1), 8)],
+    [((1, 1, 1), 32), ((2, 1, 1), 24)],
+    # edges outside {8, 16, 24, 32} take the generic index split; n = 2 puts every cell on every face
+    [((1, 1, 1), 2), ((1, 2, 1), 5), ((1, 1, 2), 12), ((2, 2, 1), 3)],
 ])
 def test_patches_fp64_bitwise_equal_dense_oracle(tori):
     table, info, u = _setup(tori, seed=len(tori))
