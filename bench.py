@@ -179,6 +179,8 @@ def run_patches(lbm, wl, prec, steps=500):
             "mass_drift": float(m[0] - ps.cells),
             "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": round(achieved / HBM_PEAK_GBS, 4),
+                         "traffic": read_traffic(f"patches {'fp64' if prec == lbm.FP64 else 'fp32s'} step"),
+                         "algorithmic_bytes_per_launch": interior_bytes,
                          "ghost_exchange_GBs": round(ghost_vals * 2 * esize / per / 1e9, 1)}}
 
 
@@ -225,6 +227,7 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-patches", action="store_true")
     ap.add_argument("--no-all-fluid", action="store_true")
+    ap.add_argument("--patch-steps", type=int, default=500)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: madpac_amd has no multi-GPU path (the domain would shard "
@@ -251,7 +254,7 @@ def main():
     if not args.no_patches:
         patches = {}
         for name, prec in (("fp32s", lbm.FP32_SHIFTED), ("fp64", lbm.FP64)):
-            patches[name] = run_patches(lbm, wl, prec)
+            patches[name] = run_patches(lbm, wl, prec, steps=args.patch_steps)
             log(f"[bench] patches {name}: {patches[name]['value']:.0f} MLUPS, tiling {patches[name]['tiling']}")
     shard = run_shard_share(lbm, solid, args.tau, 1e-6)
     log(f"[bench] slab share {n}x{n}x34: fp64 {shard['fp64']['value']:.0f} MLUPS, "
