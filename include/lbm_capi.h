@@ -235,6 +235,10 @@ int64_t lbm_patches_cells(const lbm_patchset* ps);
  * patch order, x fastest (sum n^3 entries; u has 3 per cell); NULL = rest. */
 int lbm_patches_init(lbm_patchset* ps, const double* rho, const double* u);
 int lbm_patches_get_populations(lbm_patchset* ps, double* f);   /* 19 * sum n^3, per patch SoA */
+/* rho[i] and u[3*i+d] of every interior cell, patch order, x fastest (sum n^3 cells): the block
+ * store's getDensity()/getVelocity() (BLB/services/GridManagementService.cpp:295-327) computed as
+ * BLB/BlockComputeCellValues.cpp:26-68 does. */
+int lbm_patches_get_fields(lbm_patchset* ps, double* rho, double* u);
 /* Pick a tiling by timing candidates on the device (the autotuner); *chosen may be NULL. */
 int lbm_patches_autotune(lbm_patchset* ps, lbm_tiling* chosen);
 int lbm_patches_set_tiling(lbm_patchset* ps, const lbm_tiling* t);

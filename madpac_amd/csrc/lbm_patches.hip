@@ -331,6 +331,41 @@ __global__ __launch_bounds__(256) void k_patch_pull(const PatchDev* __restrict__
   }
 }
 
+// rho and u of every interior cell (patch order, x fastest), in the operation order of the dense
+// readout (BLB/BlockComputeCellValues.cpp:26-68: density, then momentum times the reciprocal)
+template <typename T>
+__global__ __launch_bounds__(256) void k_patch_fields(const PatchDev* __restrict__ patches, int npatch,
+                                                      const int64_t* __restrict__ cell_base, const T* __restrict__ s,
+                                                      double* __restrict__ rho, double* __restrict__ u) {
+  const int p = blockIdx.y;
+  if (p >= npatch) return;
+  const int64_t base = patches[p].base;
+  const int n = patches[p].n;
+  const int64_t cells = (int64_t)n * n * n;
+  for (int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x; c < cells; c += (int64_t)gridDim.x * 256) {
+    const int x = (int)(c % n), y = (int)((c / n) % n), z = (int)(c / ((int64_t)n * n));
+    double r = 0.0, b0 = 0.0, b1 = 0.0, b2 = 0.0;
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+      const T v = s[base + slot_offset(q, n, x - CX(q), y - CY(q), z - CZ(q))];
+      const double fv = (sizeof(T) == 8) ? (double)v : (double)v + W64(q);
+      r += fv;
+      if (CX(q) == 1) b0 += fv;
+      if (CX(q) == -1) b0 -= fv;
+      if (CY(q) == 1) b1 += fv;
+      if (CY(q) == -1) b1 -= fv;
+      if (CZ(q) == 1) b2 += fv;
+      if (CZ(q) == -1) b2 -= fv;
+    }
+    const double inv = 1.0 / r;
+    const int64_t k = cell_base[p] + c;
+    rho[k] = r;
+    u[3 * k] = b0 * inv;
+    u[3 * k + 1] = b1 * inv;
+    u[3 * k + 2] = b2 * inv;
+  }
+}
+
 // per-block partial mass / momentum deviations over interior cells, summed in fixed order later
 template <typename T>
 __global__ __launch_bounds__(256) void k_patch_moments(const PatchDev* __restrict__ patches, int npatch,
@@ -666,6 +701,36 @@ int lbm_patches_get_populations(lbm_patchset* ps, double* f) {
   if (e == hipSuccess) e = hipStreamSynchronize(ps->stream);
   (void)hipFree(d_f);
   if (e != hipSuccess) return pfail(ps, LBM_ERR_HIP, std::string("patch readout: ") + hipGetErrorString(e));
+  return LBM_OK;
+}
+
+int lbm_patches_get_fields(lbm_patchset* ps, double* rho, double* u) {
+  if (!ps) return LBM_ERR_ARG;
+  if (!rho || !u) return pfail(ps, LBM_ERR_ARG, "output is null");
+  if (!ps->state_set) return pfail(ps, LBM_ERR_STATE, "state not initialised (lbm_patches_init)");
+  PCHK(ps, hipSetDevice(ps->device));
+  double* d_f = nullptr;
+  const size_t cells = (size_t)ps->cells;
+  if (lbm::device_alloc(&d_f, 4 * cells * sizeof(double)) != hipSuccess)
+    return pfail(ps, LBM_ERR_NOMEM, "staging allocation failed");
+  double* d_rho = d_f;
+  double* d_u = d_f + cells;
+  int maxn = 0;
+  for (auto& d : ps->host) maxn = std::max(maxn, d.n);
+  const int64_t maxc = (int64_t)maxn * maxn * maxn;
+  const dim3 grid((unsigned)std::min<int64_t>((maxc + 255) / 256, 64), (unsigned)ps->host.size());
+  if (ps->fp32)
+    hipLaunchKernelGGL(k_patch_fields<float>, grid, dim3(256), 0, ps->stream, ps->d_patches, (int)ps->host.size(),
+                       ps->d_cell_base, (const float*)ps->buf[ps->cur], d_rho, d_u);
+  else
+    hipLaunchKernelGGL(k_patch_fields<double>, grid, dim3(256), 0, ps->stream, ps->d_patches, (int)ps->host.size(),
+                       ps->d_cell_base, (const double*)ps->buf[ps->cur], d_rho, d_u);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(rho, d_rho, cells * sizeof(double), hipMemcpyDeviceToHost, ps->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(u, d_u, 3 * cells * sizeof(double), hipMemcpyDeviceToHost, ps->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ps->stream);
+  (void)hipFree(d_f);
+  if (e != hipSuccess) return pfail(ps, LBM_ERR_HIP, std::string("patch fields: ") + hipGetErrorString(e));
   return LBM_OK;
 }
 

@@ -44,8 +44,8 @@ EXPORTS = (
     "lbm_pressure_drop",
     "lbm_adjoint_pressure_drop",
     "lbm_patches_create", "lbm_patches_destroy", "lbm_patches_last_error", "lbm_patches_cells",
-    "lbm_patches_init", "lbm_patches_get_populations", "lbm_patches_autotune", "lbm_patches_set_tiling",
-    "lbm_patches_step", "lbm_patches_reduce_moments",
+    "lbm_patches_init", "lbm_patches_get_populations", "lbm_patches_get_fields", "lbm_patches_autotune",
+    "lbm_patches_set_tiling", "lbm_patches_step", "lbm_patches_reduce_moments",
 )
 
 
@@ -132,6 +132,7 @@ def load_library(path: str = LIB_PATH):
         L.lbm_patches_cells.restype = ctypes.c_int64
         L.lbm_patches_init.argtypes = [vp, dp, dp]
         L.lbm_patches_get_populations.argtypes = [vp, dp]
+        L.lbm_patches_get_fields.argtypes = [vp, dp, dp]
         L.lbm_patches_autotune.argtypes = [vp, ctypes.POINTER(Tiling)]
         L.lbm_patches_set_tiling.argtypes = [vp, ctypes.POINTER(Tiling)]
         L.lbm_patches_step.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_float)]
@@ -397,6 +398,13 @@ class PatchSet:
             out.append(f[o:o + Q * n ** 3].reshape(Q, n, n, n))
             o += Q * n ** 3
         return out
+
+    def fields(self):
+        """rho (cells,) and u (cells, 3) of every interior cell, patch order, x fastest."""
+        rho = np.empty(self.cells)
+        u = np.empty(3 * self.cells)
+        self._check(self._L.lbm_patches_get_fields(self._h, _dp(rho), _dp(u)), "lbm_patches_get_fields")
+        return rho, u.reshape(-1, 3)
 
     def autotune(self):
         t = Tiling()

@@ -52,7 +52,8 @@ def _oracle_torus(table, first, dims, n, u_patch, steps):
     lat.set_solid(np.zeros(tau_d.shape, np.uint8))
     lat.set_tau_cells(tau_d)
     f = lat.step(lat.equilibrium(u=u_d.reshape(-1, 3)), steps)
-    return f.reshape(19, c * n, b * n, a * n), ids
+    rho, u = lat.fields(f)
+    return f.reshape(19, c * n, b * n, a * n), ids, rho.reshape(c * n, b * n, a * n), u.reshape(c * n, b * n, a * n, 3)
 
 
 @pytest.mark.parametrize("tori", [
@@ -70,10 +71,17 @@ def test_patches_fp64_bitwise_equal_dense_oracle(tori):
         ps.init(u=u)
         ps.step(steps)
         fp = ps.get_populations()
+        rho_p, u_p = ps.fields()
+    offs = np.cumsum([0] + [t[0] ** 3 for t in table])
     for first, dims, n in info:
-        fo, ids = _oracle_torus(table, first, dims, n, u, steps)
+        fo, ids, rho_o, u_o = _oracle_torus(table, first, dims, n, u, steps)
         fg = _dense_of([fp[p] for p in ids], dims, n)
         assert np.array_equal(fg, fo), f"torus {dims}x{n}: max |diff| {np.abs(fg - fo).max()}"
+        # lbm_patches_get_fields: the reference's density and velocity of the same state
+        rg = _dense_of([rho_p[offs[p]:offs[p + 1]].reshape(n, n, n) for p in ids], dims, n)
+        ug = _dense_of([u_p[offs[p]:offs[p + 1]].reshape(n, n, n, 3).transpose(3, 0, 1, 2) for p in ids], dims, n)
+        assert np.array_equal(rg, rho_o)
+        assert np.array_equal(ug, np.moveaxis(u_o, -1, 0))
 
 
 def test_graph_replay_and_autotune_are_bit_identical():
@@ -125,7 +133,7 @@ def test_patches_fp32_shifted_close_to_fp64_oracle():
         ps.step(steps, graph=True)
         fp = ps.get_populations()
     first, dims, n = info[0]
-    fo, ids = _oracle_torus(table, first, dims, n, u, steps)
+    fo, ids, _, _ = _oracle_torus(table, first, dims, n, u, steps)
     fg = _dense_of([fp[p] for p in ids], dims, n)
     c = O.C.astype(float)
     j_o = np.tensordot(c.T, fo, axes=([1], [0]))
