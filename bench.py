@@ -51,7 +51,7 @@ def run_dense(lbm, solid, n, tau, prec, steps, warmup, g, layout=0):
     lat.init_equilibrium()
     nfluid = lat.fluid_cells
     order = "bricks" if lat.cell_order == lbm.CELL_ORDER_BRICKS else "rows"
-    variant = lat.tune(40)  # load schedule and workgroup order (bit-identical results either way)
+    variant = lat.tune(40)  # load schedule, workgroup order and shape (bit-identical results either way)
     lat.step(warmup)
     lat.synchronize()
     t0 = time.perf_counter()
@@ -66,6 +66,8 @@ def run_dense(lbm, solid, n, tau, prec, steps, warmup, g, layout=0):
     achieved = bytes_per_launch / per_launch_s / 1e9
     schedule = "aa-in-place" if layout == lbm.LAYOUT_AA else ("loads-with-flag-word" if variant & 1 else "flag-word-first")
     dispatch = "xcd-contiguous" if variant & 2 else "dispatch-order"
+    shape = (variant >> 2) & 3
+    wg_rows = (2, 4, 8)[shape - 1] if shape else "default"
     key = f"{'fp64' if prec == lbm.FP64 else 'fp32s'} {n}^3 {schedule}"
     traffic = read_traffic(key)
     return {
@@ -77,6 +79,7 @@ def run_dense(lbm, solid, n, tau, prec, steps, warmup, g, layout=0):
         "mass": m[0],
         "variant": schedule,
         "workgroup_order": dispatch,
+        "workgroup_rows": wg_rows,
         "cell_order": order,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
@@ -284,7 +287,8 @@ def main():
                    "mlups_counts": "fluid cells",
                    "step_variant": {"fp64": r64["variant"], "fp32s": r32["variant"]},
                    "cell_order": {"fp64": r64["cell_order"], "fp32s": r32["cell_order"]},
-                   "workgroup_order": {"fp64": r64["workgroup_order"], "fp32s": r32["workgroup_order"]}},
+                   "workgroup_order": {"fp64": r64["workgroup_order"], "fp32s": r32["workgroup_order"]},
+                   "workgroup_rows": {"fp64": r64["workgroup_rows"], "fp32s": r32["workgroup_rows"]}},
         "roofline": r64["roofline"],
         "fp32_shifted": {"value": round(r32["mlups"], 1), "unit": "MLUPS", "dtype": "f32",
                          "arithmetic": "FP32 state and traffic; FP64 moments, equilibrium and relaxation product",

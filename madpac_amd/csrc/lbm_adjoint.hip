@@ -44,7 +44,7 @@ struct AdjointState {
 // ---------------------------------------------------------------------------------------------
 // dJ/df of J = cs^2 (<rho>_xin - <rho>_xout): uniform per population on the two planes.
 template <typename L>
-__global__ __launch_bounds__(BX * BY) void k_adj_terminal(DenseGeom g, const uint32_t* __restrict__ words, int xin,
+__global__ __launch_bounds__(BX * BY_MAX) void k_adj_terminal(DenseGeom g, const uint32_t* __restrict__ words, int xin,
                                                           int xout, double vin, double vout, L* __restrict__ lam) {
   const CellPos c = map_cell(g);
   if (!c.valid) return;
@@ -58,7 +58,7 @@ __global__ __launch_bounds__(BX * BY) void k_adj_terminal(DenseGeom g, const uin
 
 // The adjoint step.  T: stored-state type; L: adjoint type (double or float); gradient in double.
 template <typename T, bool FORCE, bool ALPHA, bool WALLS>
-__global__ __launch_bounds__(BX * BY) void k_adj_step(DenseGeom g, const T* __restrict__ s,
+__global__ __launch_bounds__(BX * BY_MAX) void k_adj_step(DenseGeom g, const T* __restrict__ s,
                                                       const T* __restrict__ lam_out, T* __restrict__ lam_in,
                                                       const uint32_t* __restrict__ words, const T* __restrict__ alpha,
                                                       double* __restrict__ galpha, CollideParams p,
@@ -185,7 +185,7 @@ template <typename T>
 static hipError_t adj_step_t(const DenseGeom& g, const void* s, const void* lo, void* li, const uint32_t* words,
                              const void* alpha, double* galpha, bool force, bool walls, const CollideParams& p,
                              const BoundaryClass* bct, hipStream_t st) {
-  const dim3 grid((unsigned)g.nblocks_padded), block(BX, BY);
+  const dim3 grid((unsigned)g.nblocks_padded), block(BX, g.by);
   const T* sp = (const T*)s;
   const T* lop = (const T*)lo;
   T* lip = (T*)li;
@@ -361,7 +361,7 @@ extern "C" int lbm_adjoint_pressure_drop(lbm_ctx* c, int nsteps, int snapshots, 
   ACHK(hipStreamSynchronize(st));
   const double vin = (1.0 / 3.0) / hsum[1], vout = -(1.0 / 3.0) / hsum[3];
   int lc = 0;
-  const dim3 grid((unsigned)c->g.nblocks_padded), block(BX, BY);
+  const dim3 grid((unsigned)c->g.nblocks_padded), block(BX, c->g.by);
   if (c->fp32)
     hipLaunchKernelGGL(k_adj_terminal<float>, grid, block, 0, st, c->g, c->words, xin, xout, vin, vout,
                        (float*)a->lam[lc]);

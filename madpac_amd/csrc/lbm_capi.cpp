@@ -508,6 +508,7 @@ lbm_graph_key graph_key(const lbm_ctx* c) {
   lbm_graph_key k;
   k.bricked = c->g.bricked;
   k.remap = c->g.remap;
+  k.by = c->g.by;
   k.walls = c->walls;
   k.force = c->force;
   k.spec = c->spec;
@@ -606,13 +607,30 @@ int lbm_step_timed(lbm_ctx* c, int nsteps, float* ms) {
   return LBM_OK;
 }
 
+// Tuning variant: bit 0 the load schedule, bit 1 the workgroup order, bits 2-3 the workgroup
+// shape (0: the default for the cell order and precision, 1/2/3: 2/4/8 rows or bricks).
+static const int kShapeRows[4] = {0, 2, 4, 8};
+
+static void apply_variant(lbm_ctx* c, int v) {
+  c->spec = !c->aa && (v & 1);
+  c->g.remap = (v >> 1) & 1;
+  const int s = (v >> 2) & 3;
+  lbm::set_workgroup_rows(c->g, s ? kShapeRows[s] : lbm::default_workgroup_rows(c->g.bricked, c->esize));
+}
+
+static int current_variant(const lbm_ctx* c) {
+  int s = 0;
+  for (int k = 1; k < 4; k++)
+    if (kShapeRows[k] == c->g.by) s = k;
+  return (c->spec ? 1 : 0) | (c->g.remap << 1) | (s << 2);
+}
+
 int lbm_tune(lbm_ctx* c, int nsteps, int* variant) {
   if (!c) return LBM_ERR_ARG;
   if (nsteps < 0) return fail(c, LBM_ERR_ARG, "nsteps must be >= 0");
   if (nsteps == 0) {  // explicit choice
-    if (!variant || *variant < 0 || *variant > 3) return fail(c, LBM_ERR_ARG, "variant must be in 0..3");
-    c->spec = !c->aa && (*variant & 1);
-    c->g.remap = (*variant >> 1) & 1;
+    if (!variant || *variant < 0 || *variant > 15) return fail(c, LBM_ERR_ARG, "variant must be in 0..15");
+    apply_variant(c, *variant);
     return LBM_OK;
   }
   int rc = need_state(c);
@@ -623,17 +641,16 @@ int lbm_tune(lbm_ctx* c, int nsteps, int* variant) {
   const size_t bytes = (size_t)Q * c->g.N * c->esize;
   if (ensure_scratch(c, bytes) != LBM_OK) {  // no room for the copy (e.g. a 1024^3 AA state): keep the defaults
     c->err.clear();
-    if (variant) *variant = (c->spec ? 1 : 0) | (c->g.remap << 1);
+    if (variant) *variant = current_variant(c);
     return LBM_OK;
   }
   const int cur0 = c->cur, par0 = c->parity;
   HIPCHK(c, hipMemcpyAsync(c->scratch, c->buf[cur0], bytes, hipMemcpyDeviceToDevice, c->stream));
   float best = 0.f;
   int best_v = -1;
-  for (int v = 0; v < 4; v++) {
+  for (int v = 4; v < 16; v++) {  // the three explicit shapes
     if (c->aa && (v & 1)) continue;  // the AA kernels have one load schedule
-    c->spec = (v & 1) != 0;
-    c->g.remap = v >> 1;
+    apply_variant(c, v);
     HIPCHK(c, hipMemcpyAsync(c->buf[cur0], c->scratch, bytes, hipMemcpyDeviceToDevice, c->stream));
     c->cur = cur0;
     c->parity = par0;
@@ -651,8 +668,7 @@ int lbm_tune(lbm_ctx* c, int nsteps, int* variant) {
       best_v = v;
     }
   }
-  c->spec = (best_v & 1) != 0;
-  c->g.remap = best_v >> 1;
+  apply_variant(c, best_v);
   HIPCHK(c, hipMemcpyAsync(c->buf[cur0], c->scratch, bytes, hipMemcpyDeviceToDevice, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->cur = cur0;

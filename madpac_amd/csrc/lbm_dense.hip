@@ -26,7 +26,7 @@ DenseGeom make_geom(int nx, int ny, int nz, const int periodic[3], bool bricked,
   g.nx = nx; g.ny = ny; g.nz = nz;
   g.px = periodic[0] ? 1 : 0; g.py = periodic[1] ? 1 : 0; g.pz = periodic[2] ? 1 : 0;
   g.bricked = bricked ? 1 : 0;
-  g.rnx = (nx + BX - 1) / BX; g.rny = (ny + BY - 1) / BY;
+  g.rnx = (nx + BX - 1) / BX;
   g.bnx = (nx + 3) / 4; g.bny = (ny + 3) / 4; g.bnz = (nz + 3) / 4;
   g.tnx = (g.bnx + 1) / 2; g.tny = (g.bny + 1) / 2; g.tnz = (g.bnz + 1) / 2;
   const int64_t bricks = (int64_t)g.bnx * g.bny * g.bnz;
@@ -36,15 +36,35 @@ DenseGeom make_geom(int nx, int ny, int nz, const int periodic[3], bool bricked,
   // rows run faster in dispatch order)
   g.remap = (bricked || g.Nu >= ((int64_t)1 << 29)) ? 1 : 0;
   g.N = (bricked ? bricks * 64 : g.Nu) + plane_pad(esize);
-  // bricks: 8 brick slots per traversal tile, BY per workgroup
-  g.nblocks = bricked ? (int64_t)g.tnx * g.tny * g.tnz * 8 / BY : (int64_t)g.rnx * g.rny * nz;
-  g.nblocks_padded = (g.nblocks + NXCD - 1) / NXCD * NXCD;
+  set_workgroup_rows(g, default_workgroup_rows(bricked, esize));
   g.layout = LAYOUT_PULL;
   return g;
 }
 
+// FP64 rows: 8 rows per workgroup, FP32 rows: 2 (all-fluid 512^3: FP64 18.9k against 18.1k MLUPS
+// with 4 rows, FP32 37.7k against 35.6k); bricks: 4 (the 256^3 porous channel in FP64: 14.8k
+// against 14.6k with 2 and 14.2k with 8)
+int default_workgroup_rows(bool bricked, size_t esize) { return bricked ? 4 : (esize == 4 ? 2 : 8); }
+
+void set_workgroup_rows(DenseGeom& g, int by) {
+  g.by = by;
+  g.rny = (g.ny + by - 1) / by;
+  // bricks: 8 brick slots per traversal tile, by per workgroup (by divides 8)
+  g.nblocks = g.bricked ? (int64_t)g.tnx * g.tny * g.tnz * 8 / by : (int64_t)g.rnx * g.rny * g.nz;
+  g.nblocks_padded = (g.nblocks + NXCD - 1) / NXCD * NXCD;
+}
+
+// the geometry the reductions launch with: BY_REDUCE rows or bricks per workgroup in dispatch
+// order, so that each partial sums the same cells into the same slot whatever the step's tuning
+static DenseGeom reduce_geom(const DenseGeom& g) {
+  DenseGeom r = g;
+  set_workgroup_rows(r, BY_REDUCE);
+  r.remap = 0;
+  return r;
+}
+
 // ------------------------------------------------------------------------------------------
-__global__ __launch_bounds__(BX * BY) void k_build_words(DenseGeom g, const uint8_t* __restrict__ inner,
+__global__ __launch_bounds__(BX * BY_MAX) void k_build_words(DenseGeom g, const uint8_t* __restrict__ inner,
                                                          const uint8_t* __restrict__ cls,
                                                          uint32_t* __restrict__ words) {
   const CellPos c = map_cell(g);
@@ -124,7 +144,7 @@ __device__ __forceinline__ bool line_has_fluid(unsigned long long fluid) {
 // issue together with the word, and the few bounce-back populations are re-loaded from the cell's
 // own reflected slots once the word has arrived (one round trip on most lanes; solid lanes load).
 template <typename T, bool FORCE, bool ALPHA, bool WALLS, bool SPEC, bool M32 = false>
-__global__ __launch_bounds__(BX * BY) void k_dense_step(DenseGeom g, const T* __restrict__ src,
+__global__ __launch_bounds__(BX * BY_MAX) void k_dense_step(DenseGeom g, const T* __restrict__ src,
                                                         T* __restrict__ dst, const uint32_t* __restrict__ words,
                                                         const T* __restrict__ alpha, CollideParams p64,
                                                         CollideParams32 p32, BoundaryArgs bc) {
@@ -180,7 +200,7 @@ __global__ __launch_bounds__(BX * BY) void k_dense_step(DenseGeom g, const T* __
 // without races.  Boundary classes act on the outgoing population of a bounce-back link, as in the
 // two-buffer step.
 template <typename T, bool FORCE, bool ALPHA, bool WALLS, bool M32, bool ODD>
-__global__ __launch_bounds__(BX * BY) void k_aa_step(DenseGeom g, T* __restrict__ a, const uint32_t* __restrict__ words,
+__global__ __launch_bounds__(BX * BY_MAX) void k_aa_step(DenseGeom g, T* __restrict__ a, const uint32_t* __restrict__ words,
                                                      const T* __restrict__ alpha, CollideParams p64,
                                                      CollideParams32 p32, BoundaryArgs bc) {
   const CellPos c = map_cell(g);
@@ -226,7 +246,7 @@ __global__ __launch_bounds__(BX * BY) void k_aa_step(DenseGeom g, T* __restrict_
 
 // ------------------------------------------------------------------------------------------
 template <typename T>
-__global__ __launch_bounds__(BX * BY) void k_dense_pull(DenseGeom g, const T* __restrict__ src,
+__global__ __launch_bounds__(BX * BY_MAX) void k_dense_pull(DenseGeom g, const T* __restrict__ src,
                                                         double* __restrict__ f, const uint32_t* __restrict__ words) {
   const CellPos c = map_cell(g);
   if (!c.valid) return;
@@ -247,7 +267,7 @@ __global__ __launch_bounds__(BX * BY) void k_dense_pull(DenseGeom g, const T* __
 
 // Scatter of a reference state into the stored representation (inverse of the pull).
 template <typename T>
-__global__ __launch_bounds__(BX * BY) void k_dense_unpull(DenseGeom g, const double* __restrict__ f,
+__global__ __launch_bounds__(BX * BY_MAX) void k_dense_unpull(DenseGeom g, const double* __restrict__ f,
                                                           T* __restrict__ dst, const uint32_t* __restrict__ words) {
   const CellPos c = map_cell(g);
   if (!c.valid) return;
@@ -264,7 +284,7 @@ __global__ __launch_bounds__(BX * BY) void k_dense_unpull(DenseGeom g, const dou
 
 // feq(rho, u) exactly as BLB/BlockComputeCellValues.cpp:9-22 evaluates it, scattered like unpull.
 template <typename T>
-__global__ __launch_bounds__(BX * BY) void k_dense_init_eq(DenseGeom g, const double* __restrict__ rho,
+__global__ __launch_bounds__(BX * BY_MAX) void k_dense_init_eq(DenseGeom g, const double* __restrict__ rho,
                                                            const double* __restrict__ u, T* __restrict__ dst,
                                                            const uint32_t* __restrict__ words) {
   const CellPos c = map_cell(g);
@@ -288,7 +308,7 @@ __global__ __launch_bounds__(BX * BY) void k_dense_init_eq(DenseGeom g, const do
 
 // rho and u of the reference state, in the reference's operation order for FP64.
 template <typename T>
-__global__ __launch_bounds__(BX * BY) void k_dense_fields(DenseGeom g, const T* __restrict__ src,
+__global__ __launch_bounds__(BX * BY_MAX) void k_dense_fields(DenseGeom g, const T* __restrict__ src,
                                                           const uint32_t* __restrict__ words,
                                                           double* __restrict__ rho, double* __restrict__ u) {
   const CellPos c = map_cell(g);
@@ -322,11 +342,11 @@ __global__ __launch_bounds__(BX * BY) void k_dense_fields(DenseGeom g, const T* 
 
 // ------------------------------------------------------------------------------------------
 // Deterministic FP64 reductions: each block reduces its cells with 64-lane wavefront shuffles
-// then an LDS step over its 4 wavefronts, writes one partial, and a single block sums the
+// then an LDS step over its wavefronts, writes one partial, and a single block sums the
 // partials in a fixed order.
 template <int K>
 __device__ __forceinline__ void block_reduce_store(double (&v)[K], double* __restrict__ out) {
-  __shared__ double lds[K][BX * BY / 64];
+  __shared__ double lds[K][BX * BY_MAX / 64];
 #pragma unroll
   for (int k = 0; k < K; k++) {
 #pragma unroll
@@ -343,14 +363,14 @@ __device__ __forceinline__ void block_reduce_store(double (&v)[K], double* __res
 #pragma unroll
     for (int k = 0; k < K; k++) {
       double s = 0.0;
-      for (int j = 0; j < BX * BY / 64; j++) s += lds[k][j];
+      for (int j = 0; j < (int)blockDim.y; j++) s += lds[k][j];  // one wavefront per workgroup row
       out[k] = s;
     }
   }
 }
 
 template <typename T>
-__global__ __launch_bounds__(BX * BY) void k_dense_moments(DenseGeom g, const T* __restrict__ src,
+__global__ __launch_bounds__(BX * BY_MAX) void k_dense_moments(DenseGeom g, const T* __restrict__ src,
                                                            const uint32_t* __restrict__ words,
                                                            double* __restrict__ partials) {
   const CellPos c = map_cell(g);
@@ -380,7 +400,7 @@ __global__ __launch_bounds__(BX * BY) void k_dense_moments(DenseGeom g, const T*
 
 // kinetic energy sum over fluid cells of (j . j) / (2 rho), FP64
 template <typename T>
-__global__ __launch_bounds__(BX * BY) void k_dense_energy(DenseGeom g, const T* __restrict__ src,
+__global__ __launch_bounds__(BX * BY_MAX) void k_dense_energy(DenseGeom g, const T* __restrict__ src,
                                                           const uint32_t* __restrict__ words,
                                                           double* __restrict__ partials) {
   const CellPos c = map_cell(g);
@@ -483,7 +503,7 @@ __global__ __launch_bounds__(256) void k_plane_rho(DenseGeom g, const T* __restr
     for (int k = 0; k < 4; k++) partials[4 * (int64_t)blockIdx.x + k] = lds[k][0] + lds[k][1] + lds[k][2] + lds[k][3];
 }
 
-__global__ __launch_bounds__(BX * BY) void k_count_fluid(DenseGeom g, const uint32_t* __restrict__ words,
+__global__ __launch_bounds__(BX * BY_MAX) void k_count_fluid(DenseGeom g, const uint32_t* __restrict__ words,
                                                          double* __restrict__ partials) {
   const CellPos c = map_cell(g);
   double v[1] = {(c.valid && (words[c.i] & FLUID)) ? 1.0 : 0.0};
@@ -492,7 +512,7 @@ __global__ __launch_bounds__(BX * BY) void k_count_fluid(DenseGeom g, const uint
 
 // Cell-order conversion of per-cell arrays (one thread per cell of the target order).
 template <typename E>
-__global__ __launch_bounds__(BX * BY) void k_relayout(DenseGeom from, DenseGeom to, const E* __restrict__ in,
+__global__ __launch_bounds__(BX * BY_MAX) void k_relayout(DenseGeom from, DenseGeom to, const E* __restrict__ in,
                                                       E* __restrict__ out, int comps) {
   const CellPos c = map_cell(to);
   if (!c.valid) return;
@@ -509,12 +529,12 @@ __global__ __launch_bounds__(256) void k_convert_ids(DenseGeom from, DenseGeom t
 }
 
 // ------------------------------------------------------------------------------------------
-static dim3 dense_block() { return dim3(BX, BY, 1); }
+static dim3 dense_block(const DenseGeom& g) { return dim3(BX, g.by, 1); }
 static dim3 dense_grid(const DenseGeom& g) { return dim3((unsigned)g.nblocks_padded, 1, 1); }
 
 hipError_t launch_build_words(const DenseGeom& g, const uint8_t* inner, const uint8_t* cls, uint32_t* words,
                               hipStream_t s) {
-  hipLaunchKernelGGL(k_build_words, dense_grid(g), dense_block(), 0, s, g, inner, cls, words);
+  hipLaunchKernelGGL(k_build_words, dense_grid(g), dense_block(g), 0, s, g, inner, cls, words);
   return hipGetLastError();
 }
 
@@ -526,7 +546,7 @@ static hipError_t step_t(const DenseGeom& g, const void* src, void* dst, const u
   const T* sp = (const T*)src;
   T* dp = (T*)dst;
 #define LBM_STEP(F, A, W)                                                                                        \
-  hipLaunchKernelGGL((k_dense_step<T, F, A, W, SPEC, M32>), dense_grid(g), dense_block(), 0, s, g, sp, dp, words, a, \
+  hipLaunchKernelGGL((k_dense_step<T, F, A, W, SPEC, M32>), dense_grid(g), dense_block(g), 0, s, g, sp, dp, words, a, \
                      p64, p32, bc)
   const int key = (force ? 4 : 0) | (alpha ? 2 : 0) | (walls ? 1 : 0);
   switch (key) {
@@ -563,7 +583,7 @@ static hipError_t aa_t(const DenseGeom& g, void* buf, const uint32_t* words, con
   T* ap = (T*)buf;
   const T* al = (const T*)alpha;
 #define LBM_AA(F, A, W)                                                                                        \
-  hipLaunchKernelGGL((k_aa_step<T, F, A, W, M32, ODD>), dense_grid(g), dense_block(), 0, s, g, ap, words, al, \
+  hipLaunchKernelGGL((k_aa_step<T, F, A, W, M32, ODD>), dense_grid(g), dense_block(g), 0, s, g, ap, words, al, \
                      p64, p32, bc)
   const int key = (force ? 4 : 0) | (alpha ? 2 : 0) | (walls ? 1 : 0);
   switch (key) {
@@ -596,46 +616,46 @@ hipError_t launch_aa_step(const DenseGeom& g, bool fp32, bool f32math, int parit
 
 hipError_t launch_dense_pull(const DenseGeom& g, bool fp32, const void* src, double* f, const uint32_t* words,
                              hipStream_t s) {
-  if (fp32) hipLaunchKernelGGL(k_dense_pull<float>, dense_grid(g), dense_block(), 0, s, g, (const float*)src, f, words);
-  else hipLaunchKernelGGL(k_dense_pull<double>, dense_grid(g), dense_block(), 0, s, g, (const double*)src, f, words);
+  if (fp32) hipLaunchKernelGGL(k_dense_pull<float>, dense_grid(g), dense_block(g), 0, s, g, (const float*)src, f, words);
+  else hipLaunchKernelGGL(k_dense_pull<double>, dense_grid(g), dense_block(g), 0, s, g, (const double*)src, f, words);
   return hipGetLastError();
 }
 
 hipError_t launch_dense_unpull(const DenseGeom& g, bool fp32, const double* f, void* dst, const uint32_t* words,
                                hipStream_t s) {
-  if (fp32) hipLaunchKernelGGL(k_dense_unpull<float>, dense_grid(g), dense_block(), 0, s, g, f, (float*)dst, words);
-  else hipLaunchKernelGGL(k_dense_unpull<double>, dense_grid(g), dense_block(), 0, s, g, f, (double*)dst, words);
+  if (fp32) hipLaunchKernelGGL(k_dense_unpull<float>, dense_grid(g), dense_block(g), 0, s, g, f, (float*)dst, words);
+  else hipLaunchKernelGGL(k_dense_unpull<double>, dense_grid(g), dense_block(g), 0, s, g, f, (double*)dst, words);
   return hipGetLastError();
 }
 
 hipError_t launch_dense_init_eq(const DenseGeom& g, bool fp32, const double* rho, const double* u, void* dst,
                                 const uint32_t* words, hipStream_t s) {
   if (fp32)
-    hipLaunchKernelGGL(k_dense_init_eq<float>, dense_grid(g), dense_block(), 0, s, g, rho, u, (float*)dst, words);
+    hipLaunchKernelGGL(k_dense_init_eq<float>, dense_grid(g), dense_block(g), 0, s, g, rho, u, (float*)dst, words);
   else
-    hipLaunchKernelGGL(k_dense_init_eq<double>, dense_grid(g), dense_block(), 0, s, g, rho, u, (double*)dst, words);
+    hipLaunchKernelGGL(k_dense_init_eq<double>, dense_grid(g), dense_block(g), 0, s, g, rho, u, (double*)dst, words);
   return hipGetLastError();
 }
 
 hipError_t launch_dense_fields(const DenseGeom& g, bool fp32, const void* src, const uint32_t* words, double* rho,
                                double* u, hipStream_t s) {
   if (fp32)
-    hipLaunchKernelGGL(k_dense_fields<float>, dense_grid(g), dense_block(), 0, s, g, (const float*)src, words, rho, u);
+    hipLaunchKernelGGL(k_dense_fields<float>, dense_grid(g), dense_block(g), 0, s, g, (const float*)src, words, rho, u);
   else
-    hipLaunchKernelGGL(k_dense_fields<double>, dense_grid(g), dense_block(), 0, s, g, (const double*)src, words, rho,
+    hipLaunchKernelGGL(k_dense_fields<double>, dense_grid(g), dense_block(g), 0, s, g, (const double*)src, words, rho,
                        u);
   return hipGetLastError();
 }
 
-int64_t dense_moment_partials(const DenseGeom& g) { return g.nblocks_padded; }
+int64_t dense_moment_partials(const DenseGeom& g) { return reduce_geom(g).nblocks_padded; }
 
 hipError_t launch_relayout(const DenseGeom& from, const DenseGeom& to, const void* in, void* out, int comps,
                            int esize, hipStream_t s) {
   if (esize == 8)
-    hipLaunchKernelGGL(k_relayout<double>, dense_grid(to), dense_block(), 0, s, from, to, (const double*)in,
+    hipLaunchKernelGGL(k_relayout<double>, dense_grid(to), dense_block(to), 0, s, from, to, (const double*)in,
                        (double*)out, comps);
   else if (esize == 4)
-    hipLaunchKernelGGL(k_relayout<float>, dense_grid(to), dense_block(), 0, s, from, to, (const float*)in,
+    hipLaunchKernelGGL(k_relayout<float>, dense_grid(to), dense_block(to), 0, s, from, to, (const float*)in,
                        (float*)out, comps);
   else
     return hipErrorInvalidValue;
@@ -648,25 +668,27 @@ hipError_t launch_convert_ids(const DenseGeom& from, const DenseGeom& to, int64_
   return hipGetLastError();
 }
 
-hipError_t launch_dense_moments(const DenseGeom& g, bool fp32, const void* src, const uint32_t* words,
+hipError_t launch_dense_moments(const DenseGeom& gs, bool fp32, const void* src, const uint32_t* words,
                                 double* partials, double* out4, hipStream_t s) {
+  const DenseGeom g = reduce_geom(gs);
   if (fp32)
-    hipLaunchKernelGGL(k_dense_moments<float>, dense_grid(g), dense_block(), 0, s, g, (const float*)src, words,
+    hipLaunchKernelGGL(k_dense_moments<float>, dense_grid(g), dense_block(g), 0, s, g, (const float*)src, words,
                        partials);
   else
-    hipLaunchKernelGGL(k_dense_moments<double>, dense_grid(g), dense_block(), 0, s, g, (const double*)src, words,
+    hipLaunchKernelGGL(k_dense_moments<double>, dense_grid(g), dense_block(g), 0, s, g, (const double*)src, words,
                        partials);
   hipLaunchKernelGGL(k_sum_partials<4>, dim3(1), dim3(256), 0, s, (const double*)partials, g.nblocks_padded, out4);
   return hipGetLastError();
 }
 
-hipError_t launch_dense_energy(const DenseGeom& g, bool fp32, const void* src, const uint32_t* words,
+hipError_t launch_dense_energy(const DenseGeom& gs, bool fp32, const void* src, const uint32_t* words,
                                double* partials, double* out1, hipStream_t s) {
+  const DenseGeom g = reduce_geom(gs);
   if (fp32)
-    hipLaunchKernelGGL(k_dense_energy<float>, dense_grid(g), dense_block(), 0, s, g, (const float*)src, words,
+    hipLaunchKernelGGL(k_dense_energy<float>, dense_grid(g), dense_block(g), 0, s, g, (const float*)src, words,
                        partials);
   else
-    hipLaunchKernelGGL(k_dense_energy<double>, dense_grid(g), dense_block(), 0, s, g, (const double*)src, words,
+    hipLaunchKernelGGL(k_dense_energy<double>, dense_grid(g), dense_block(g), 0, s, g, (const double*)src, words,
                        partials);
   hipLaunchKernelGGL(k_sum_partials<1>, dim3(1), dim3(256), 0, s, (const double*)partials, g.nblocks_padded, out1);
   return hipGetLastError();
@@ -685,9 +707,10 @@ hipError_t launch_dense_plane_rho(const DenseGeom& g, bool fp32, const void* src
   return hipGetLastError();
 }
 
-hipError_t launch_count_fluid(const DenseGeom& g, const uint32_t* words, double* partials, double* out1,
+hipError_t launch_count_fluid(const DenseGeom& gs, const uint32_t* words, double* partials, double* out1,
                               hipStream_t s) {
-  hipLaunchKernelGGL(k_count_fluid, dense_grid(g), dense_block(), 0, s, g, words, partials);
+  const DenseGeom g = reduce_geom(gs);
+  hipLaunchKernelGGL(k_count_fluid, dense_grid(g), dense_block(g), 0, s, g, words, partials);
   hipLaunchKernelGGL(k_sum_partials<1>, dim3(1), dim3(256), 0, s, (const double*)partials, g.nblocks_padded, out1);
   return hipGetLastError();
 }

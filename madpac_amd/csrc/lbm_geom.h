@@ -13,9 +13,9 @@ namespace lbm {
 // +1.5 %; all-fluid 512^3 rows: dispatch order 38.3k MLUPS, remap 35.5k-38.1k; 1024^3: the reverse),
 // so lbm_tune times both; the untuned defaults (make_geom) are the remap for bricks and for lattices
 // of 2^29 cells and more, dispatch order otherwise.
-// Rows: thread (x, y) of a workgroup owns cell (64 bx + x, 4 by + y, z).  Bricks: lane
+// Rows: thread (x, y) of a workgroup owns cell (64 bx + x, g.by * by + y, z).  Bricks: lane
 // threadIdx.x of wavefront threadIdx.y owns cell threadIdx.x of the brick in traversal slot
-// 4 * workgroup + threadIdx.y; slots run over tiles of 2 x 2 x 2 bricks (x fastest inside a tile,
+// g.by * workgroup + threadIdx.y; slots run over tiles of 2 x 2 x 2 bricks (x fastest inside a tile,
 // tiles in x-fastest order), which keeps a brick's z- and y-neighbours in flight together more
 // often than the storage order does (FP64 +3 %; larger tiles were slower).
 struct CellPos {
@@ -36,12 +36,12 @@ __device__ __forceinline__ CellPos map_cell(const DenseGeom& g) {
     const int by = (int)(r % g.rny);
     c.z = (int)(r / g.rny);
     c.x = bx * BX + threadIdx.x;
-    c.y = by * BY + threadIdx.y;
+    c.y = by * g.by + threadIdx.y;
     c.valid = c.z < g.nz && c.x < g.nx && c.y < g.ny;
     c.i = c.r = (int64_t)c.x + (int64_t)g.nx * ((int64_t)c.y + (int64_t)g.ny * c.z);
     return c;
   }
-  const int64_t slot = b * BY + threadIdx.y;
+  const int64_t slot = b * g.by + threadIdx.y;
   const int l = threadIdx.x;
   const int64_t tile = slot >> 3;
   const int loc = (int)(slot & 7);

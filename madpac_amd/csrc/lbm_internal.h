@@ -11,12 +11,12 @@
 namespace lbm {
 
 // One lattice stored as SoA populations s[q*N + i] in one of two cell orders.
-//  * Rows: i = x + nx * (y + ny * z); a workgroup is BX x BY cells of one z-plane, a wavefront a
+//  * Rows: i = x + nx * (y + ny * z); a workgroup is BX x g.by cells of one z-plane, a wavefront a
 //    64-cell row segment.
 //  * Bricks of 4 x 4 x 4 cells: the 64 cells of a brick are consecutive (x fastest, then y, then
 //    z) and bricks follow in x-fastest order, i = 64 * (bx + bnx * (by + bny * bz)) + (x & 3) +
 //    4 * (y & 3) + 16 * (z & 3), padded to whole bricks (padding cells are never fluid and never
-//    touched); a wavefront is one brick, a workgroup BY bricks.  A 128-byte line holds a compact
+//    touched); a wavefront is one brick, a workgroup g.by bricks.  A 128-byte line holds a compact
 //    4 x 4 x 1 (FP64) or 4 x 4 x 2 (FP32) cell group, so far fewer lines straddle a solid surface
 //    and get fetched for a few fluid cells; but a shifted read touches up to 4x more lines than a
 //    row segment does.  Bricks win for FP64 on porous lattices, rows elsewhere.
@@ -24,14 +24,18 @@ namespace lbm {
 // index both.  The stored state is the post-collision state of the last step; the reference's
 // post-streaming state is its pull (see k_dense_pull).
 constexpr int BX = 64;
-constexpr int BY = 4;
-constexpr int NXCD = 8;  // MI355X: 8 XCDs, each with its own L2
+constexpr int BY_MAX = 8;     // workgroup rows (rows order) or bricks (bricks order): 2, 4 or 8
+// The reductions run with this many rows per workgroup, in dispatch order, whatever the step uses,
+// so that their fixed summation order (and their bits) do not depend on the tuned variant.
+constexpr int BY_REDUCE = 4;
+constexpr int NXCD = 8;       // MI355X: 8 XCDs, each with its own L2
 
 struct DenseGeom {
   int nx, ny, nz;
   int px, py, pz;            // periodic flags
   int bricked;               // 0 = rows, 1 = bricks
   int remap;                 // 1: XCD-contiguous workgroup ranges (map_cell), 0: dispatch order
+  int by;                    // workgroup = BX x by cells of one plane (rows) or by bricks (bricks)
   int rnx, rny;              // rows: workgroups along x and y
   int bnx, bny, bnz;         // bricks per dimension
   int tnx, tny, tnz;         // bricks: traversal tiles of 2 x 2 x 2 bricks per dimension
@@ -75,6 +79,12 @@ enum : int { LAYOUT_PULL = 0, LAYOUT_AA_NATURAL = 1, LAYOUT_AA_SWAPPED = 2 };
 // (256 B); pads of 1M cells were 20 % slower.
 int64_t plane_pad(size_t esize);
 DenseGeom make_geom(int nx, int ny, int nz, const int periodic[3], bool bricked, size_t esize);
+// Workgroup shape: `by` rows of BX cells (rows order) or `by` bricks (bricks order), by in {2, 4, 8};
+// it fixes the grid (rny, nblocks) but not the stored order.  The default is the measured best
+// without tuning: 8 rows in FP64, 2 in FP32, 4 bricks.
+void set_workgroup_rows(DenseGeom& g, int by);
+int default_workgroup_rows(bool bricked, size_t esize);
+inline bool valid_workgroup_rows(int by) { return by == 2 || by == 4 || by == 8; }
 // per-cell array of `comps` values of `esize` bytes from the cell order of `from` to that of `to`
 hipError_t launch_relayout(const DenseGeom& from, const DenseGeom& to, const void* in, void* out, int comps,
                            int esize, hipStream_t stream);

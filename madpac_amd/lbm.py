@@ -298,8 +298,9 @@ class Lattice:
         return float(ms.value)
 
     def tune(self, nsteps=10, variant=None):
-        """Pick the fastest step-kernel variant on the current state (returns 0..3: bit 0 the load
-        schedule, bit 1 the XCD-contiguous workgroup order), or with `variant` given, select it."""
+        """Pick the fastest step-kernel variant on the current state (returns 4..15: bit 0 the load
+        schedule, bit 1 the XCD-contiguous workgroup order, bits 2-3 the workgroup shape 2/4/8 rows
+        or bricks), or with `variant` given (0..15; shape bits 0 = the default), select it."""
         v = ctypes.c_int(0 if variant is None else int(variant))
         n = 0 if variant is not None else int(nsteps)
         self._check(self._L.lbm_tune(self._h, n, ctypes.byref(v)), "lbm_tune")

@@ -136,7 +136,7 @@ def test_aa_tune_restores_the_state_and_changes_no_bits():
             before = lat.get_populations()
             if tuned:
                 v = lat.tune(4)
-                assert v in (0, 2)  # one load schedule in place; either workgroup order
+                assert v & 1 == 0 and 4 <= v < 16  # one load schedule in place; any order and shape
                 assert np.array_equal(lat.get_populations(), before)
             lat.step(9)
             out.append(lat.get_populations())

@@ -83,7 +83,7 @@ def test_graph_recaptures_after_changes():
             lat.step(33, graph=graph)
             out.append(lat.get_populations())
             lat.set_coupling_data(0, None)
-            lat.tune(variant=3)  # the other load schedule and workgroup order
+            lat.tune(variant=7)  # the other load schedule and workgroup order, 2-row workgroups
             lat.step(35, graph=graph)
             out.append(lat.get_populations())
             lat.set_flags(np.ones(n ** 3, np.uint8))  # all fluid: the other cell order
