@@ -67,8 +67,9 @@ def run_dense(lbm, solid, n, tau, prec, steps, warmup, g, layout=0):
     schedule = "aa-in-place" if layout == lbm.LAYOUT_AA else ("loads-with-flag-word" if variant & 1 else "flag-word-first")
     dispatch = "xcd-contiguous" if variant & 2 else "dispatch-order"
     shape = (variant >> 2) & 3
-    wg_rows = (2, 4, 8)[shape - 1] if shape else "default"
-    key = f"{'fp64' if prec == lbm.FP64 else 'fp32s'} {n}^3 {schedule}"
+    # the shape bits are 0 only when the tuner could not copy the state: then the default holds
+    wg_rows = (2, 4, 8)[shape - 1] if shape else (4 if order == "bricks" else (8 if prec == lbm.FP64 else 2))
+    key = f"{'fp64' if prec == lbm.FP64 else 'fp32s'} {n}^3 {schedule} ({wg_rows} {order}/workgroup)"
     traffic = read_traffic(key)
     return {
         "mlups": nfluid * steps / (ms * 1e-3) / 1e6,
@@ -298,6 +299,7 @@ def main():
                                         "roofline": res["fp32s_f32math"]["roofline"]},
         "aa_in_place": {k: {"value": round(res[k]["mlups"], 1), "unit": "MLUPS",
                             "ms_per_step": round(res[k]["ms_per_step"], 4), "roofline": res[k]["roofline"],
+                            "cell_order": res[k]["cell_order"], "workgroup_rows": res[k]["workgroup_rows"],
                             "state_bytes": 19 * n ** 3 * (8 if k == "fp64_aa" else 4)}
                         for k in ("fp64_aa", "fp32s_aa")},
         "wall_mlups": {"fp64": round(r64["wall_mlups"], 1), "fp32s": round(r32["wall_mlups"], 1)},
