@@ -141,3 +141,25 @@ def test_aa_tune_restores_the_state_and_changes_no_bits():
             lat.step(9)
             out.append(lat.get_populations())
     assert np.array_equal(out[0], out[1])
+
+
+@pytest.mark.parametrize("variant", [4, 10, 14])
+def test_aa_each_workgroup_shape_bitwise_equals_two_buffer(variant):
+    # 2, 4 and 8 rows or bricks per workgroup (both orders by size), in place against two buffers
+    nx, ny, nz, steps = 22, 13, 10, 9
+    z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    solid = (x - 8) ** 2 + (y - 6) ** 2 + (z - 5) ** 2 <= 12
+    out = []
+    for layout, order in ((lbm.LAYOUT_TWO_BUFFER, lbm.CELL_ORDER_ROWS), (AA, lbm.CELL_ORDER_ROWS),
+                          (AA, lbm.CELL_ORDER_BRICKS)):
+        print(f"[gpu] AA shape variant {variant} layout {layout} order {order}", flush=True)
+        with lbm.Lattice(nx, ny, nz, 0.6, periodic=(True, True, True), force=(1e-5, 0, 0), layout=layout,
+                         cell_order=order) as lat:
+            lat.set_flags(~solid)
+            lat.init_equilibrium()
+            lat.tune(variant=variant)
+            lat.step(steps)
+            out.append((lat.get_populations(), lat.moments()))
+    for f, _ in out[1:]:
+        assert np.array_equal(f, out[0][0])
+    assert np.array_equal(out[1][1], out[0][1])  # same cell order: the reductions sum alike

@@ -111,3 +111,23 @@ def test_fp32_adjoint_tracks_fp64():
     rel = np.linalg.norm(g32 - g64) / np.linalg.norm(g64)
     print(f"[gpu] FP32 adjoint vs FP64: relative L2 {rel:.3e}", flush=True)
     assert rel <= 1e-2
+
+
+def test_adjoint_gradient_is_the_same_for_every_workgroup_shape():
+    # the forward and reverse kernels launch with the tuned workgroup shape and order; the gradient
+    # is per cell and the objective's reduction has its own launch, so every variant gives the same bits
+    nx, ny, nz, steps, tau, g = 18, 10, 9, 10, 0.7, (1e-4, 0.0, 0.0)
+    solid, alpha = _channel(nx, ny, nz, 5)
+    out = []
+    for variant in (0, 4, 10, 15):
+        print(f"[gpu] adjoint variant {variant}", flush=True)
+        with lbm.Lattice(nx, ny, nz, tau, periodic=(True, False, False), force=g) as lat:
+            lat.set_flags(~solid)
+            lat.set_porosity(alpha)
+            lat.init_equilibrium()
+            lat.tune(variant=variant)
+            J, grad, _ = lat.adjoint_pressure_drop(steps, 3, 2, 14)
+            out.append((J, grad))
+    for J, grad in out[1:]:
+        assert J == out[0][0]
+        assert np.array_equal(grad, out[0][1])
