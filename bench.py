@@ -68,7 +68,7 @@ def run_dense(lbm, solid, n, tau, prec, steps, warmup, g, layout=0):
     dispatch = "xcd-contiguous" if variant & 2 else "dispatch-order"
     shape = (variant >> 2) & 3
     # the shape bits are 0 only when the tuner could not copy the state: then the default holds
-    wg_rows = (2, 4, 8)[shape - 1] if shape else (4 if order == "bricks" else (8 if prec == lbm.FP64 else 2))
+    wg_rows = (2, 4, 8)[shape - 1] if shape else 4
     key = f"{'fp64' if prec == lbm.FP64 else 'fp32s'} {n}^3 {schedule} ({wg_rows} {order}/workgroup)"
     traffic = read_traffic(key)
     return {

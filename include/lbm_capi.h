@@ -179,8 +179,8 @@ int lbm_step_timed(lbm_ctx* ctx, int nsteps, float* ms);
  * neighbour loads issue alongside the flag word and bounce-back populations are re-loaded
  * afterwards (two-buffer layout only).  Bit 1, the workgroup order: 0 = the hardware's round-robin
  * dispatch over the XCDs; 1 = each XCD a contiguous range of the lattice.  Bits 2-3, the workgroup
- * shape: 0 = the default for the cell order and precision (8 rows of 64 cells in FP64, 2 in FP32,
- * 4 bricks), 1/2/3 = 2/4/8 rows or bricks; the timing covers the three explicit shapes.  Which is
+ * shape: 0 = the default (4 rows of 64 cells, or 4 bricks), 1/2/3 = 2/4/8 rows or bricks; the
+ * timing covers the three explicit shapes.  Which is
  * faster depends on lattice size, cell order and allocation.  With nsteps == 0, *variant (0..15) is
  * an input and selects directly.  When the state copy does not fit the device, the current choice
  * is kept.  All variants give bit-identical states, and the reductions run with one fixed shape.

@@ -608,7 +608,7 @@ int lbm_step_timed(lbm_ctx* c, int nsteps, float* ms) {
 }
 
 // Tuning variant: bit 0 the load schedule, bit 1 the workgroup order, bits 2-3 the workgroup
-// shape (0: the default for the cell order and precision, 1/2/3: 2/4/8 rows or bricks).
+// shape (0: the default, 1/2/3: 2/4/8 rows or bricks).
 static const int kShapeRows[4] = {0, 2, 4, 8};
 
 static void apply_variant(lbm_ctx* c, int v) {

@@ -41,10 +41,15 @@ DenseGeom make_geom(int nx, int ny, int nz, const int periodic[3], bool bricked,
   return g;
 }
 
-// FP64 rows: 8 rows per workgroup, FP32 rows: 2 (all-fluid 512^3: FP64 18.9k against 18.1k MLUPS
-// with 4 rows, FP32 37.7k against 35.6k); bricks: 4 (the 256^3 porous channel in FP64: 14.8k
-// against 14.6k with 2 and 14.2k with 8)
-int default_workgroup_rows(bool bricked, size_t esize) { return bricked ? 4 : (esize == 4 ? 2 : 8); }
+// 4 rows or bricks per workgroup: in a sweep of all tuned variants the best shape, or within 1 % of
+// it, on the porous 256^3 channel and the all-fluid 512^3 lattice in both precisions (8 rows led FP32
+// on the porous channel by 1 %; single runs had suggested 2 rows for FP32 and 8 for FP64, which the
+// sweep did not confirm)
+int default_workgroup_rows(bool bricked, size_t esize) {
+  (void)bricked;
+  (void)esize;
+  return 4;
+}
 
 void set_workgroup_rows(DenseGeom& g, int by) {
   g.by = by;

@@ -80,8 +80,7 @@ enum : int { LAYOUT_PULL = 0, LAYOUT_AA_NATURAL = 1, LAYOUT_AA_SWAPPED = 2 };
 int64_t plane_pad(size_t esize);
 DenseGeom make_geom(int nx, int ny, int nz, const int periodic[3], bool bricked, size_t esize);
 // Workgroup shape: `by` rows of BX cells (rows order) or `by` bricks (bricks order), by in {2, 4, 8};
-// it fixes the grid (rny, nblocks) but not the stored order.  The default is the measured best
-// without tuning: 8 rows in FP64, 2 in FP32, 4 bricks.
+// it fixes the grid (rny, nblocks) but not the stored order.  The untuned default is 4.
 void set_workgroup_rows(DenseGeom& g, int by);
 int default_workgroup_rows(bool bricked, size_t esize);
 inline bool valid_workgroup_rows(int by) { return by == 2 || by == 4 || by == 8; }
