@@ -615,7 +615,7 @@ static void apply_variant(lbm_ctx* c, int v) {
   c->spec = !c->aa && (v & 1);
   c->g.remap = (v >> 1) & 1;
   const int s = (v >> 2) & 3;
-  lbm::set_workgroup_rows(c->g, s ? kShapeRows[s] : lbm::default_workgroup_rows(c->g.bricked, c->esize));
+  lbm::set_workgroup_rows(c->g, s ? kShapeRows[s] : lbm::BY_DEFAULT);
 }
 
 static int current_variant(const lbm_ctx* c) {

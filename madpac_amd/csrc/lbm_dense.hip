@@ -36,19 +36,9 @@ DenseGeom make_geom(int nx, int ny, int nz, const int periodic[3], bool bricked,
   // rows run faster in dispatch order)
   g.remap = (bricked || g.Nu >= ((int64_t)1 << 29)) ? 1 : 0;
   g.N = (bricked ? bricks * 64 : g.Nu) + plane_pad(esize);
-  set_workgroup_rows(g, default_workgroup_rows(bricked, esize));
+  set_workgroup_rows(g, BY_DEFAULT);
   g.layout = LAYOUT_PULL;
   return g;
-}
-
-// 4 rows or bricks per workgroup: in a sweep of all tuned variants the best shape, or within 1 % of
-// it, on the porous 256^3 channel and the all-fluid 512^3 lattice in both precisions (8 rows led FP32
-// on the porous channel by 1 %; single runs had suggested 2 rows for FP32 and 8 for FP64, which the
-// sweep did not confirm)
-int default_workgroup_rows(bool bricked, size_t esize) {
-  (void)bricked;
-  (void)esize;
-  return 4;
 }
 
 void set_workgroup_rows(DenseGeom& g, int by) {

@@ -25,6 +25,7 @@ namespace lbm {
 // post-streaming state is its pull (see k_dense_pull).
 constexpr int BX = 64;
 constexpr int BY_MAX = 8;     // workgroup rows (rows order) or bricks (bricks order): 2, 4 or 8
+constexpr int BY_DEFAULT = 4;
 // The reductions run with this many rows per workgroup, in dispatch order, whatever the step uses,
 // so that their fixed summation order (and their bits) do not depend on the tuned variant.
 constexpr int BY_REDUCE = 4;
@@ -80,9 +81,11 @@ enum : int { LAYOUT_PULL = 0, LAYOUT_AA_NATURAL = 1, LAYOUT_AA_SWAPPED = 2 };
 int64_t plane_pad(size_t esize);
 DenseGeom make_geom(int nx, int ny, int nz, const int periodic[3], bool bricked, size_t esize);
 // Workgroup shape: `by` rows of BX cells (rows order) or `by` bricks (bricks order), by in {2, 4, 8};
-// it fixes the grid (rny, nblocks) but not the stored order.  The untuned default is 4.
+// it fixes the grid (rny, nblocks) but not the stored order.  The untuned default, BY_DEFAULT = 4,
+// was the best shape or within 1 % of it in a sweep of all tuned variants on the porous 256^3
+// channel and the all-fluid 512^3 lattice in both precisions (8 rows led FP32 on the porous
+// channel by 1 %).
 void set_workgroup_rows(DenseGeom& g, int by);
-int default_workgroup_rows(bool bricked, size_t esize);
 inline bool valid_workgroup_rows(int by) { return by == 2 || by == 4 || by == 8; }
 // per-cell array of `comps` values of `esize` bytes from the cell order of `from` to that of `to`
 hipError_t launch_relayout(const DenseGeom& from, const DenseGeom& to, const void* in, void* out, int comps,
