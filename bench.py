@@ -45,7 +45,7 @@ def read_traffic(key):
     return None
 
 
-def run_dense(lbm, solid, n, tau, prec, steps, warmup, g, layout=0):
+def run_dense(lbm, solid, n, tau, prec, steps, warmup, g, layout=0, readout=False):
     lat = lbm.Lattice(n, n, n, tau, precision=prec, periodic=(True, True, True), force=(g, 0.0, 0.0), layout=layout)
     lat.set_flags(~solid)
     lat.init_equilibrium()
@@ -59,6 +59,15 @@ def run_dense(lbm, solid, n, tau, prec, steps, warmup, g, layout=0):
     lat.synchronize()
     wall = time.perf_counter() - t0
     m = lat.moments()
+    host = None
+    if readout:  # what a caller pays to bring results to the host (PCIe), outside the timed steps
+        t0 = time.perf_counter()
+        lat.fields()
+        t1 = time.perf_counter()
+        lat.get_populations()
+        t2 = time.perf_counter()
+        host = {"fields_s": round(t1 - t0, 4), "fields_bytes": n ** 3 * 4 * 8,
+                "populations_s": round(t2 - t1, 4), "populations_bytes": n ** 3 * 19 * 8}
     lat.close()
     esize = 8 if prec == lbm.FP64 else 4
     per_launch_s = ms * 1e-3 / steps
@@ -82,6 +91,7 @@ def run_dense(lbm, solid, n, tau, prec, steps, warmup, g, layout=0):
         "workgroup_order": dispatch,
         "workgroup_rows": wg_rows,
         "cell_order": order,
+        "host_readout": host,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "algorithmic_bytes_per_launch": bytes_per_launch},
@@ -247,7 +257,7 @@ def main():
     log(f"[bench] porous geometry {n}^3: solid fraction {solid.mean():.4f} ({time.perf_counter() - t0:.1f} s)")
     res = {}
     for name, prec in (("fp64", lbm.FP64), ("fp32s", lbm.FP32_SHIFTED), ("fp32s_f32math", lbm.FP32_SHIFTED_F32)):
-        res[name] = run_dense(lbm, solid, n, args.tau, prec, args.steps, args.warmup, 1e-6)
+        res[name] = run_dense(lbm, solid, n, args.tau, prec, args.steps, args.warmup, 1e-6, readout=name == "fp64")
         log(f"[bench] {name}: {res[name]['mlups']:.0f} MLUPS, {res[name]['ms_per_step']:.3f} ms/step, "
             f"{res[name]['roofline']['achieved']:.0f} GB/s ({100 * res[name]['roofline']['frac']:.1f} % of peak)")
     for name, prec in (("fp64_aa", lbm.FP64), ("fp32s_aa", lbm.FP32_SHIFTED)):
@@ -303,6 +313,7 @@ def main():
                             "state_bytes": 19 * n ** 3 * (8 if k == "fp64_aa" else 4)}
                         for k in ("fp64_aa", "fp32s_aa")},
         "wall_mlups": {"fp64": round(r64["wall_mlups"], 1), "fp32s": round(r32["wall_mlups"], 1)},
+        "host_readout_fp64": r64["host_readout"],
         "batched_patches": patches,
         "all_fluid": tgv,
         "slab_share": shard,
