@@ -76,3 +76,7 @@ def test_null_handles_are_rejected():
     assert L.lbm_fluid_cells(None) == -1
     assert L.lbm_patches_step(None, 1, 0, None) == -1
     assert L.lbm_patches_destroy(None) == 0
+    assert L.lbm_patches_get_fields(None, None, None) == -1
+    assert L.lbm_patches_get_populations(None, None) == -1
+    assert L.lbm_get_fields(None, None, None) == -1
+    assert L.lbm_tune(None, 0, None) == -1
