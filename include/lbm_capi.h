@@ -72,7 +72,8 @@ enum lbm_cell_order { LBM_CELL_ORDER_AUTO = 0, LBM_CELL_ORDER_ROWS = 1, LBM_CELL
 /* Storage of the state.  TWO_BUFFER: pull-form steps between two buffers.  AA: one buffer updated
  * in place, alternating a step that touches only the cell's own slots with one that pulls from and
  * pushes to the neighbours; half the memory at the same traffic (e.g. a 1024^3 FP64 lattice fits
- * one MI355X).  The AA layout does not support the adjoint or the MD coupling. */
+ * one MI355X).  The AA layout does not support the adjoint; MD coupling works in both layouts
+ * (in place, the coupled cells' populations are saved before each step and re-collided after it). */
 enum lbm_layout { LBM_LAYOUT_TWO_BUFFER = 0, LBM_LAYOUT_AA = 1 };
 
 /* ---- lifetime ---------------------------------------------------------------------- */

@@ -486,8 +486,12 @@ int lbm_enqueue_step_from(lbm_ctx* c, const void* src, void* dst) {
 int lbm_enqueue_steps(lbm_ctx* c, int nsteps) {
   if (c->aa) {
     for (int s = 0; s < nsteps; s++) {
+      int rc = lbm_enqueue_coupling_save(c);
+      if (rc) return rc;
       HIPCHK(c, lbm::launch_aa_step(c->g, c->fp32, c->f32math, c->parity, c->buf[0], c->words, c->alpha, c->force,
                                     c->walls, c->p64, c->p32, lbm::BoundaryArgs{c->d_bct, c->d_bcu}, c->stream));
+      rc = lbm_enqueue_coupling_aa(c, c->parity);
+      if (rc) return rc;
       c->parity ^= 1;
     }
     return LBM_OK;

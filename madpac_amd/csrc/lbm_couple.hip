@@ -7,7 +7,8 @@
 // (computeGi, :161-186) after the BGK relaxation; and it exports the LB density and velocity of
 // the same cells to the MD side.  Here the fused step kernel collides every cell as usual, and a
 // second small kernel re-collides only the coupled cells from the same source state and overwrites
-// their (own, pull-form) output slots; the hot kernel is unchanged.  A gather kernel returns rho
+// their output slots (own slots in the pull form; in the in-place layout the slots the step wrote,
+// from populations saved before the step); the hot kernel is unchanged.  A gather kernel returns rho
 // and u of the coupled cells (the export).  Coupled cells use the reference's block-solver
 // collision: no body force and no porosity (the host rejects those combinations).
 #include <cmath>
@@ -89,6 +90,22 @@ __device__ __forceinline__ double collide_coupled_f64(double (&f)[Q], const Coup
   return rho;
 }
 
+// Coupled collision of cell k (stored index i, flag word w), then the boundary class of its links.
+template <bool WALLS>
+__device__ __forceinline__ void couple_collide(double (&f)[Q], const CoupleArgs& a, int64_t k, int64_t i, uint32_t w,
+                                               const BoundaryArgs& bc) {
+  double u[3];
+  const double rho = collide_coupled_f64(f, a, k, u);
+  if (WALLS) {
+    const uint32_t cls = (w >> CLASS_SHIFT) & CLASS_BITS;
+    if (cls) {
+      const BoundaryClass& b = bc.table[cls];
+      const uint32_t m = w & b.mask;
+      if (m) apply_boundary_f64(f, m, b, bc.cell_u, i, rho, u[0], u[1], u[2]);
+    }
+  }
+}
+
 // Re-collide the coupled cells from src and overwrite their own output slots in dst (run after the
 // step kernel that wrote dst from src).  FP32-shifted contexts are lifted to FP64 for these cells.
 template <typename T, bool WALLS>
@@ -107,18 +124,54 @@ __global__ __launch_bounds__(256) void k_couple(DenseGeom g, const T* __restrict
     const T v = src[pull_index(q, w, i, N, o)];
     f[q] = sizeof(T) == 8 ? (double)v : (double)v + W64(q);
   }
-  double u[3];
-  const double rho = collide_coupled_f64(f, a, k, u);
-  if (WALLS) {
-    const uint32_t cls = (w >> CLASS_SHIFT) & CLASS_BITS;
-    if (cls) {
-      const BoundaryClass& b = bc.table[cls];
-      const uint32_t m = w & b.mask;
-      if (m) apply_boundary_f64(f, m, b, bc.cell_u, i, rho, u[0], u[1], u[2]);
-    }
-  }
+  couple_collide<WALLS>(f, a, k, i, w, bc);
 #pragma unroll
   for (int q = 0; q < Q; q++) dst[(int64_t)q * N + i] = sizeof(T) == 8 ? (T)f[q] : (T)(f[q] - W64(q));
+}
+
+// In-place (AA) layout: the step overwrites the populations the re-collision starts from, so they
+// are saved before it: save[q * n + k] = population q of coupled cell k in the reference state, read
+// through the layout the step starts from (g.layout).
+template <typename T>
+__global__ __launch_bounds__(256) void k_couple_save(DenseGeom g, const T* __restrict__ s,
+                                                     const uint32_t* __restrict__ words, const int64_t* __restrict__ cells,
+                                                     int64_t n, double* __restrict__ save) {
+  const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (k >= n) return;
+  const int64_t i = cells[k];
+  const uint32_t w = words[i];
+  if (!(w & FLUID)) return;
+  const NbrOff o = nbr_offsets(g, pos_of(g, i));
+#pragma unroll
+  for (int q = 0; q < Q; q++) {
+    const T v = s[state_index(g, q, w, i, o)];
+    save[(int64_t)q * n + k] = sizeof(T) == 8 ? (double)v : (double)v + W64(q);
+  }
+}
+
+// After the in-place step: re-collide from the saved populations and overwrite the slots the step
+// wrote for this cell (k_aa_step): an even step the own slot 18 - q, an odd step the slot q of
+// x + c_q, or the own slot 18 - q across a bounce-back link.
+template <typename T, bool WALLS, bool ODD>
+__global__ __launch_bounds__(256) void k_couple_aa(DenseGeom g, const double* __restrict__ save, T* __restrict__ s,
+                                                   const uint32_t* __restrict__ words, CoupleArgs a, BoundaryArgs bc) {
+  const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (k >= a.n) return;
+  const int64_t i = a.cells[k];
+  const uint32_t w = words[i];
+  if (!(w & FLUID)) return;
+  const NbrOff o = nbr_offsets(g, pos_of(g, i));
+  const int64_t N = g.N;
+  double f[Q];
+#pragma unroll
+  for (int q = 0; q < Q; q++) f[q] = save[(int64_t)q * a.n + k];
+  couple_collide<WALLS>(f, a, k, i, w, bc);
+#pragma unroll
+  for (int q = 0; q < Q; q++) {
+    const bool bounce = (w >> q) & 1u;
+    const int64_t di = (!ODD || bounce) ? (int64_t)(Q - 1 - q) * N + i : (int64_t)q * N + i + dst_off(q, o);
+    s[di] = sizeof(T) == 8 ? (T)f[q] : (T)(f[q] - W64(q));
+  }
 }
 
 // Export: rho and u = (sum f c) / rho of the reference state at the coupled cells (0 at solid cells).
@@ -198,12 +251,54 @@ int lbm_enqueue_coupling(lbm_ctx* c, const void* src, void* dst) {
   return e == hipSuccess ? LBM_OK : cfail(c, LBM_ERR_HIP, std::string("coupling kernel: ") + hipGetErrorString(e));
 }
 
+// In-place layout: save the coupled cells' populations before the step (layout of the step's start).
+int lbm_enqueue_coupling_save(lbm_ctx* c) {
+  if (!c->couple_on || c->ncouple == 0) return LBM_OK;
+  const dim3 grid((unsigned)((c->ncouple + 255) / 256)), block(256);
+  const lbm::DenseGeom g = lbm_state_geom(c);
+  if (c->fp32)
+    hipLaunchKernelGGL(lbm::k_couple_save<float>, grid, block, 0, c->stream, g, (const float*)c->buf[0], c->words,
+                       c->d_ccells, c->ncouple, c->d_csave);
+  else
+    hipLaunchKernelGGL(lbm::k_couple_save<double>, grid, block, 0, c->stream, g, (const double*)c->buf[0], c->words,
+                       c->d_ccells, c->ncouple, c->d_csave);
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? LBM_OK : cfail(c, LBM_ERR_HIP, std::string("coupling save kernel: ") + hipGetErrorString(e));
+}
+
+// In-place layout: re-collide the coupled cells after the step of parity `parity` (0 even, 1 odd).
+int lbm_enqueue_coupling_aa(lbm_ctx* c, int parity) {
+  if (!c->couple_on || c->ncouple == 0) return LBM_OK;
+  lbm::CoupleArgs a{c->d_ccells, c->ncouple, c->d_cu, c->couple_rho ? c->d_crho : nullptr, c->couple_mode,
+                    c->couple_dt, c->cfg.tau, c->p64.omega};
+  const lbm::BoundaryArgs bc{c->d_bct, c->d_bcu};
+  const dim3 grid((unsigned)((c->ncouple + 255) / 256)), block(256);
+#define LBM_COUPLE_AA(T, W, O)                                                                                   \
+  hipLaunchKernelGGL((lbm::k_couple_aa<T, W, O>), grid, block, 0, c->stream, c->g, c->d_csave, (T*)c->buf[0],     \
+                     c->words, a, bc)
+  const int key = (c->fp32 ? 4 : 0) | (c->walls ? 2 : 0) | (parity ? 1 : 0);
+  switch (key) {
+    case 0: LBM_COUPLE_AA(double, false, false); break;
+    case 1: LBM_COUPLE_AA(double, false, true); break;
+    case 2: LBM_COUPLE_AA(double, true, false); break;
+    case 3: LBM_COUPLE_AA(double, true, true); break;
+    case 4: LBM_COUPLE_AA(float, false, false); break;
+    case 5: LBM_COUPLE_AA(float, false, true); break;
+    case 6: LBM_COUPLE_AA(float, true, false); break;
+    default: LBM_COUPLE_AA(float, true, true); break;
+  }
+#undef LBM_COUPLE_AA
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? LBM_OK : cfail(c, LBM_ERR_HIP, std::string("coupling kernel: ") + hipGetErrorString(e));
+}
+
 void lbm_coupling_release(lbm_ctx* c) {
-  void* ptrs[] = {c->d_ccells, c->d_cu, c->d_crho, c->d_cout};
+  void* ptrs[] = {c->d_ccells, c->d_cu, c->d_crho, c->d_cout, c->d_csave};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   c->d_ccells = nullptr;
   c->d_cu = c->d_crho = c->d_cout = nullptr;
+  c->d_csave = nullptr;
   c->ncouple = 0;
   c->couple_on = false;
 }
@@ -213,7 +308,6 @@ extern "C" {
 int lbm_set_coupling_cells(lbm_ctx* c, const int64_t* cells, int64_t n) {
   if (!c) return LBM_ERR_ARG;
   if (n < 0 || (n > 0 && !cells)) return cfail(c, LBM_ERR_ARG, "coupling cell list is null or negative");
-  if (c->aa && n > 0) return cfail(c, LBM_ERR_STATE, "MD coupling needs the two-buffer layout");
   if (hipSetDevice(c->cfg.device) != hipSuccess || hipStreamSynchronize(c->stream) != hipSuccess)
     return cfail(c, LBM_ERR_HIP, "device synchronisation failed");
   lbm_coupling_release(c);
@@ -226,7 +320,8 @@ int lbm_set_coupling_cells(lbm_ctx* c, const int64_t* cells, int64_t n) {
   if (lbm::device_alloc(&c->d_ccells, (size_t)n * sizeof(int64_t)) != hipSuccess ||
       lbm::device_alloc(&c->d_cu, (size_t)n * 3 * sizeof(double)) != hipSuccess ||
       lbm::device_alloc(&c->d_crho, (size_t)n * sizeof(double)) != hipSuccess ||
-      lbm::device_alloc(&c->d_cout, (size_t)n * 4 * sizeof(double)) != hipSuccess) {
+      lbm::device_alloc(&c->d_cout, (size_t)n * 4 * sizeof(double)) != hipSuccess ||
+      (c->aa && lbm::device_alloc(&c->d_csave, (size_t)n * Q * sizeof(double)) != hipSuccess)) {
     lbm_coupling_release(c);
     return cfail(c, LBM_ERR_NOMEM, "coupling buffers allocation failed");
   }
@@ -283,10 +378,10 @@ int lbm_get_coupling_data(lbm_ctx* c, double* u, double* rho) {
   if (hipSetDevice(c->cfg.device) != hipSuccess) return cfail(c, LBM_ERR_HIP, "hipSetDevice failed");
   const dim3 grid((unsigned)((c->ncouple + 255) / 256)), block(256);
   if (c->fp32)
-    hipLaunchKernelGGL(lbm::k_couple_gather<float>, grid, block, 0, c->stream, c->g, (const float*)c->buf[c->cur],
+    hipLaunchKernelGGL(lbm::k_couple_gather<float>, grid, block, 0, c->stream, lbm_state_geom(c), (const float*)c->buf[c->cur],
                        c->words, c->d_ccells, c->ncouple, c->d_cout);
   else
-    hipLaunchKernelGGL(lbm::k_couple_gather<double>, grid, block, 0, c->stream, c->g, (const double*)c->buf[c->cur],
+    hipLaunchKernelGGL(lbm::k_couple_gather<double>, grid, block, 0, c->stream, lbm_state_geom(c), (const double*)c->buf[c->cur],
                        c->words, c->d_ccells, c->ncouple, c->d_cout);
   std::vector<double> out((size_t)c->ncouple * 4);
   if (hipGetLastError() != hipSuccess ||

@@ -58,6 +58,7 @@ struct lbm_ctx {
   int64_t* d_ccells = nullptr;                  // coupled cells
   int64_t ncouple = 0;
   double *d_cu = nullptr, *d_crho = nullptr, *d_cout = nullptr;
+  double* d_csave = nullptr;                    // in-place layout: coupled cells' populations before a step
   bool couple_on = false, couple_rho = false;
   int couple_mode = 0;
   double couple_dt = 1.0;
@@ -88,6 +89,9 @@ inline lbm::DenseGeom lbm_state_geom(const lbm_ctx* c) {
 int lbm_enqueue_step_from(lbm_ctx* c, const void* src, void* dst);
 // re-collision of the MD-coupled cells for the step src -> dst (no-op without coupling)
 int lbm_enqueue_coupling(lbm_ctx* c, const void* src, void* dst);
+// in-place layout: save the coupled cells' populations before a step, re-collide them after it
+int lbm_enqueue_coupling_save(lbm_ctx* c);
+int lbm_enqueue_coupling_aa(lbm_ctx* c, int parity);
 void lbm_coupling_release(lbm_ctx* c);
 void lbm_graphs_release(lbm_ctx* c);
 void lbm_adjoint_release(lbm_ctx* c);

@@ -114,14 +114,44 @@ def test_aa_set_get_round_trip_across_parities():
         assert np.array_equal(lat.get_populations(), g)
 
 
-def test_aa_rejects_adjoint_and_coupling():
+def test_aa_rejects_adjoint():
     n = 8
     with lbm.cavity(n, 0.7, 0.05, layout=AA) as lat:
         with pytest.raises(lbm.LbmError, match="two-buffer"):
-            lat.set_coupling_cells(np.arange(4))
-        with pytest.raises(lbm.LbmError, match="two-buffer"):
             lat.adjoint_pressure_drop(4, 2, 1, 6)
         assert lat.tune(variant=0) == 0
+
+
+@pytest.mark.parametrize("prec", [lbm.FP64, lbm.FP32_SHIFTED])
+@pytest.mark.parametrize("order", [lbm.CELL_ORDER_ROWS, lbm.CELL_ORDER_BRICKS])
+@pytest.mark.parametrize("mode,density", [(0, True), (1, False)])
+def test_aa_md_coupling_bitwise_equals_two_buffer(prec, order, mode, density):
+    # the in-place step saves the coupled cells' populations before it and re-collides them into the
+    # slots it wrote; states and the MD export equal the two-buffer run at odd and even step counts
+    n = 12
+    rng = np.random.default_rng(40 + mode)
+    cells = rng.choice(n ** 3, size=n ** 3 // 8, replace=False)
+    umd = 0.03 * rng.standard_normal((cells.size, 3))
+    rmd = 1.0 + 0.01 * rng.standard_normal(cells.size)
+    rho0 = 1.0 + 0.01 * rng.standard_normal(n ** 3)
+    u0 = 0.02 * rng.standard_normal((n ** 3, 3))
+    out = {}
+    for layout in (lbm.LAYOUT_TWO_BUFFER, AA):
+        print(f"[gpu] AA coupling prec {prec} order {order} mode {mode} layout {layout}", flush=True)
+        with lbm.Lattice(n, n, n, 0.7, precision=prec, layout=layout, cell_order=order) as lat:
+            lat.set_flags(np.ones(n ** 3, np.uint8))
+            lat.set_cavity_lid(0.04)
+            lat.init_equilibrium(rho=rho0, u=u0)
+            lat.set_coupling_cells(cells)
+            lat.set_coupling_data(mode, umd, rmd if density else None, dt=0.8)
+            res = []
+            for steps in (7, 6):
+                lat.step(steps)
+                res.append((lat.get_populations(), *lat.get_coupling_data()))
+            out[layout] = res
+    for a, b in zip(out[lbm.LAYOUT_TWO_BUFFER], out[AA]):
+        for x_, y_ in zip(a, b):
+            assert np.array_equal(np.asarray(x_), np.asarray(y_))
 
 
 def test_aa_tune_restores_the_state_and_changes_no_bits():
