@@ -39,6 +39,9 @@ constexpr int LB_CURRENT_DIR = 19;
 // PeriodicBox (all axes periodic, body force) has no reference counterpart
 enum class Scenario { LidDrivenCavity, DefaultChannel, PeriodicBox };
 enum class Precision { FP64 = LBM_FP64, FP32Shifted = LBM_FP32_SHIFTED, FP32ShiftedF32Math = LBM_FP32_SHIFTED_F32 };
+// Storage of the state on the device: two buffers, or one updated in place (half the memory, the
+// same results); the reference keeps one block store on the host and has no such choice.
+enum class Layout { TwoBuffer = LBM_LAYOUT_TWO_BUFFER, InPlace = LBM_LAYOUT_AA };
 
 // Run configuration: the XML tags regular-grid, lattice-boltzmann-configuration and
 // lattice-boltzmann-scenario of the reference (LB/configurations/LatticeBoltzmannConfiguration.cpp:3-14,
@@ -57,6 +60,7 @@ struct Configuration {
   bool useParabolicProfile = false;          // channel inlet profile ("use-parabolic-profile")
   double bodyForce[3] = {0.0, 0.0, 0.0};     // Guo forcing (periodic box)
   Precision precision = Precision::FP64;
+  Layout layout = Layout::TwoBuffer;
   int device = 0;
   double domainOffset[3] = {0.0, 0.0, 0.0};
   // the plotter tag (LB/configurations/LatticeBoltzmannPlotterConfiguration.cpp:25-55): VTK files
@@ -210,6 +214,7 @@ class Repository {
       c.force[d] = box ? cfg_.bodyForce[d] : 0.0;
     }
     c.precision = static_cast<int>(cfg_.precision);
+    c.layout = static_cast<int>(cfg_.layout);
     c.tau = cfg_.tau;
     c.device = cfg_.device;
     check(lbm_create(&ctx_, &c), nullptr, "lbm_create");

@@ -32,6 +32,9 @@ int main(int argc, char** argv) {
     cfg.vtkOutputName = argv[9];
     cfg.timestepsPerPlotting = std::atoi(argv[10]);
   }
+  // LBM_TEST_LAYOUT=in-place runs the same case with the state updated in place
+  if (const char* l = std::getenv("LBM_TEST_LAYOUT"))
+    if (std::string(l) == "in-place") cfg.layout = Layout::InPlace;
   if (scenario == "cavity") {
     cfg.scenario = Scenario::LidDrivenCavity;
   } else if (scenario == "channel" || scenario == "channel-parabolic") {

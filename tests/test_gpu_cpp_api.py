@@ -12,16 +12,18 @@ pytestmark = pytest.mark.gpu
 EXE = os.path.join(ROOT, "tests", "cpp", "test_peano_api")
 
 
+@pytest.mark.parametrize("layout", ["two-buffer", "in-place"])
 @pytest.mark.parametrize("name", ["cavity_n13_t5.npz", "cavity_n8_t100.npz"])
-def test_cpp_runner_matches_reference(name, tmp_path):
+def test_cpp_runner_matches_reference(name, layout, tmp_path):
     if not os.path.exists(EXE):
         pytest.fail(f"{EXE} is not built (run __graft_entry__.build())")
     d = golden(name)
     out = tmp_path / "f.bin"
-    print(f"[gpu] C++ API cavity {name}", flush=True)
+    print(f"[gpu] C++ API cavity {name} {layout}", flush=True)
     n = str(int(d["n"]))
+    env = dict(os.environ, LBM_TEST_LAYOUT=layout)
     r = subprocess.run([EXE, "cavity", n, n, n, repr(float(d["tau"])), repr(float(d["ulid"])), str(int(d["steps"])),
-                        str(out)], capture_output=True, text=True, timeout=120)
+                        str(out)], capture_output=True, text=True, timeout=120, env=env)
     assert r.returncode == 0, (r.returncode, r.stdout, r.stderr)
     f = np.fromfile(out, dtype="<f8")
     assert np.array_equal(f, d["f"].reshape(-1))
