@@ -72,8 +72,8 @@ enum lbm_cell_order { LBM_CELL_ORDER_AUTO = 0, LBM_CELL_ORDER_ROWS = 1, LBM_CELL
 /* Storage of the state.  TWO_BUFFER: pull-form steps between two buffers.  AA: one buffer updated
  * in place, alternating a step that touches only the cell's own slots with one that pulls from and
  * pushes to the neighbours; half the memory at the same traffic (e.g. a 1024^3 FP64 lattice fits
- * one MI355X).  The AA layout does not support the adjoint; MD coupling works in both layouts
- * (in place, the coupled cells' populations are saved before each step and re-collided after it). */
+ * one MI355X).  MD coupling and the adjoint work in both layouts with the same results (in place,
+ * coupled cells' populations are saved before each step; the adjoint borrows a second buffer). */
 enum lbm_layout { LBM_LAYOUT_TWO_BUFFER = 0, LBM_LAYOUT_AA = 1 };
 
 /* ---- lifetime ---------------------------------------------------------------------- */
@@ -213,7 +213,9 @@ typedef struct {
 /* Runs nsteps forward steps from the current state, evaluates J = pressure drop (FP64) and
  * returns dJ/dalpha for every cell (0 on non-fluid cells), by a hand-written reverse sweep of
  * the collide+stream kernel with revolve (binomial) checkpointing on `snapshots` device
- * checkpoints.  The state afterwards equals the state after nsteps forward steps. */
+ * checkpoints.  The state afterwards equals the state after nsteps forward steps.  On an in-place
+ * (LBM_LAYOUT_AA) context it allocates a second state buffer for the call, runs in the two-buffer
+ * form and leaves the final state in place again. */
 int lbm_adjoint_pressure_drop(lbm_ctx* ctx, int nsteps, int snapshots, int xin, int xout,
                               double* J, double* grad_alpha, lbm_adjoint_stats* stats);
 

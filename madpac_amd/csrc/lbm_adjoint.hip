@@ -261,9 +261,11 @@ int adj_fail(lbm_ctx* c, int code, const std::string& m) {
 }
 }  // namespace
 
-extern "C" int lbm_adjoint_pressure_drop(lbm_ctx* c, int nsteps, int snapshots, int xin, int xout, double* J,
-                                         double* grad_alpha, lbm_adjoint_stats* stats) {
-  if (!c) return LBM_ERR_ARG;
+static int adjoint_two_buffer(lbm_ctx* c, int nsteps, int snapshots, int xin, int xout, double* J,
+                              double* grad_alpha, lbm_adjoint_stats* stats);
+
+static int adjoint_args(lbm_ctx* c, int nsteps, int snapshots, int xin, int xout, const double* J,
+                        const double* grad_alpha) {
   if (!J || !grad_alpha) return adj_fail(c, LBM_ERR_ARG, "output is null");
   // the schedule search is O(nsteps^2 * snapshots) on the host
   if (nsteps < 1 || nsteps > 2048) return adj_fail(c, LBM_ERR_ARG, "nsteps must be in 1..2048");
@@ -271,10 +273,66 @@ extern "C" int lbm_adjoint_pressure_drop(lbm_ctx* c, int nsteps, int snapshots, 
   if (xin < 0 || xin >= c->g.nx || xout < 0 || xout >= c->g.nx) return adj_fail(c, LBM_ERR_ARG, "plane out of range");
   if (!c->flags_set || !c->state_set) return adj_fail(c, LBM_ERR_STATE, "flags and state must be set first");
   if (c->couple_on) return adj_fail(c, LBM_ERR_STATE, "the adjoint does not cover MD-coupled cells");
-  if (c->aa) return adj_fail(c, LBM_ERR_STATE, "the adjoint needs the two-buffer layout");
   for (int k = 1; k < MAX_CLASSES; k++)
     if (c->bct[k].mask && (c->bct[k].type & BC_TYPE_MASK) != BC_MOVING_WALL)
       return adj_fail(c, LBM_ERR_STATE, "the adjoint supports halfway bounce-back and moving walls only");
+  return LBM_OK;
+}
+
+// Re-store the current state in another layout: through the reference state (FP64, user order),
+// which the pull kernel reads and the unpull kernel scatters; FP32 values convert exactly.
+static int restore_layout(lbm_ctx* c, const lbm::DenseGeom& from, const void* src, const lbm::DenseGeom& to,
+                          void* dst) {
+  double* f = nullptr;
+  if (lbm::device_alloc(&f, (size_t)Q * c->g.Nu * sizeof(double)) != hipSuccess)
+    return adj_fail(c, LBM_ERR_NOMEM, "layout conversion buffer allocation failed");
+  hipError_t e = lbm::launch_dense_pull(from, c->fp32, src, f, c->words, c->stream);
+  if (e == hipSuccess) e = lbm::launch_dense_unpull(to, c->fp32, f, dst, c->words, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  (void)hipFree(f);
+  return e == hipSuccess ? LBM_OK : adj_fail(c, LBM_ERR_HIP, std::string("layout conversion: ") + hipGetErrorString(e));
+}
+
+// On an in-place (AA) context the adjoint borrows a second state buffer for its duration: the state
+// is re-stored in the two-buffer pull form, the adjoint runs as on a two-buffer context, and the
+// final state goes back to the in-place layout (NATURAL, parity 0).
+extern "C" int lbm_adjoint_pressure_drop(lbm_ctx* c, int nsteps, int snapshots, int xin, int xout, double* J,
+                                         double* grad_alpha, lbm_adjoint_stats* stats) {
+  if (!c) return LBM_ERR_ARG;
+  int rc = adjoint_args(c, nsteps, snapshots, xin, xout, J, grad_alpha);
+  if (rc) return rc;
+  if (!c->aa) return adjoint_two_buffer(c, nsteps, snapshots, xin, xout, J, grad_alpha, stats);
+  if (hipSetDevice(c->cfg.device) != hipSuccess) return adj_fail(c, LBM_ERR_HIP, "hipSetDevice failed");
+  const size_t pbytes = (size_t)Q * c->g.N * c->esize;
+  if (lbm::device_alloc(&c->buf[1], pbytes) != hipSuccess) {
+    c->buf[1] = nullptr;
+    return adj_fail(c, LBM_ERR_NOMEM, "the adjoint on an in-place context needs a second state buffer");
+  }
+  rc = restore_layout(c, lbm_state_geom(c), c->buf[0], [&] {
+    lbm::DenseGeom g = c->g;
+    g.layout = lbm::LAYOUT_PULL;
+    return g;
+  }(), c->buf[1]);
+  if (rc == LBM_OK) {
+    c->aa = false;
+    c->cur = 1;
+    rc = adjoint_two_buffer(c, nsteps, snapshots, xin, xout, J, grad_alpha, stats);
+    // back in place, whatever the adjoint returned (the state is the final one on success)
+    lbm::DenseGeom natural = c->g;
+    natural.layout = lbm::LAYOUT_AA_NATURAL;
+    const int rc2 = restore_layout(c, lbm_state_geom(c), c->buf[c->cur], natural, c->buf[0]);
+    c->aa = true;
+    c->cur = 0;
+    c->parity = 0;
+    if (!rc) rc = rc2;
+  }
+  (void)hipFree(c->buf[1]);
+  c->buf[1] = nullptr;
+  return rc;
+}
+
+static int adjoint_two_buffer(lbm_ctx* c, int nsteps, int snapshots, int xin, int xout, double* J,
+                              double* grad_alpha, lbm_adjoint_stats* stats) {
   if (hipSetDevice(c->cfg.device) != hipSuccess) return adj_fail(c, LBM_ERR_HIP, "hipSetDevice failed");
 
   const int64_t N = c->g.N;

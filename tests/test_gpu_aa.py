@@ -114,11 +114,40 @@ def test_aa_set_get_round_trip_across_parities():
         assert np.array_equal(lat.get_populations(), g)
 
 
-def test_aa_rejects_adjoint():
+@pytest.mark.parametrize("prec", [lbm.FP64, lbm.FP32_SHIFTED])
+def test_aa_adjoint_bitwise_equals_two_buffer(prec):
+    # the in-place context lends itself a second buffer for the adjoint: J, the gradient and the final
+    # state equal the two-buffer run; it starts from the SWAPPED parity (an odd step count before)
+    nx, ny, nz, tau, g = 16, 9, 8, 0.7, (1e-4, 0.0, 0.0)
+    solid = np.zeros((nz, ny, nx), bool)
+    solid[0], solid[-1], solid[:, 0], solid[:, -1] = True, True, True, True
+    alpha = np.clip(np.random.default_rng(6).random(nx * ny * nz) * 0.5, 0, 1)
+    out = {}
+    for layout in (lbm.LAYOUT_TWO_BUFFER, AA):
+        print(f"[gpu] AA adjoint prec {prec} layout {layout}", flush=True)
+        with lbm.Lattice(nx, ny, nz, tau, precision=prec, periodic=(True, False, False), force=g, layout=layout) as lat:
+            lat.set_flags(~solid)
+            lat.set_porosity(alpha)
+            lat.init_equilibrium()
+            lat.step(3)
+            J, grad, _ = lat.adjoint_pressure_drop(10, 3, 2, 13)
+            f_end = lat.get_populations()
+            lat.step(5)  # the context steps on from the final state in its own layout
+            out[layout] = (J, grad, f_end, lat.get_populations())
+    a, b = out[lbm.LAYOUT_TWO_BUFFER], out[AA]
+    assert a[0] == b[0]
+    for x_, y_ in zip(a[1:], b[1:]):
+        assert np.array_equal(x_, y_)
+
+
+def test_aa_adjoint_argument_errors_leave_the_state_alone():
     n = 8
     with lbm.cavity(n, 0.7, 0.05, layout=AA) as lat:
-        with pytest.raises(lbm.LbmError, match="two-buffer"):
-            lat.adjoint_pressure_drop(4, 2, 1, 6)
+        lat.step(3)
+        before = lat.get_populations()
+        with pytest.raises(lbm.LbmError, match="plane"):
+            lat.adjoint_pressure_drop(4, 2, 1, 60)
+        assert np.array_equal(lat.get_populations(), before)
         assert lat.tune(variant=0) == 0
 
 
