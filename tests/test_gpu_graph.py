@@ -103,3 +103,27 @@ def test_graph_step_errors():
         lat.init_equilibrium()
         with pytest.raises(lbm.LbmError, match="nsteps"):
             lat.step(-1, graph=True)
+
+
+def test_graph_replays_in_place_with_md_coupling():
+    # in place, each captured step holds the save and re-collision kernels of the coupled cells
+    n = 12
+    solid = _porous_solid(n, 8, frac=0.15, rmin=1, rmax=2)
+    cells = np.flatnonzero(~solid.reshape(-1))[::11]
+    umd = 0.02 * np.random.default_rng(9).standard_normal((cells.size, 3))
+    res = []
+    for graph in (False, True):
+        print(f"[gpu] graph in place with coupling, graph={graph}", flush=True)
+        with lbm.Lattice(n, n, n, 0.7, periodic=(True, True, True), layout=lbm.LAYOUT_AA) as lat:
+            lat.set_flags(~solid)
+            lat.init_equilibrium()
+            lat.set_coupling_cells(cells)
+            lat.set_coupling_data(0, umd, None, dt=1.0)
+            lat.step(37, graph=graph)
+            out = [lat.get_populations(), *lat.get_coupling_data()]
+            lat.set_coupling_data(1, umd, None, dt=0.5)  # other mode and dt: a re-capture
+            lat.step(34, graph=graph)
+            out += [lat.get_populations(), *lat.get_coupling_data()]
+            res.append(out)
+    for a, b in zip(*res):
+        assert np.array_equal(a, b)
