@@ -15,9 +15,11 @@
  * BGK relaxation are pinned by closed-form identities; the full step (collide,
  * swap streaming with implicit halfway bounce-back, moving wall) is pinned
  * bit-for-bit against fixtures produced by the compiled reference
- * (tests/golden/); Guo forcing, Brinkman porosity, periodic wrap and the adjoint
- * step go beyond the reference and are pinned by analytic solutions (Poiseuille
- * flow, Taylor-Green decay), conservation laws and central finite differences.
+ * (tests/golden/); Guo forcing, Brinkman porosity, periodic wrap, the pressure-drop
+ * objective and the adjoint step go beyond the reference and are pinned by analytic
+ * solutions (Poiseuille flow, Taylor-Green decay, the uniform-porosity fixed point
+ * u + g/2 = g tau_eff / alpha, the objective of a linear density ramp), conservation
+ * laws and central finite differences.
  */
 #ifndef LBM_ORACLE_H
 #define LBM_ORACLE_H

@@ -456,3 +456,76 @@ def test_adjoint_matches_central_difference():
         am[i] -= h
         fd = (J(ap) - J(am)) / (2 * h)
         assert abs(fd - galpha[i]) <= 1e-6 * max(abs(fd), 1e-3 * np.abs(galpha).max()), (i, fd, galpha[i])
+
+
+# ---------------------------------------------------------------- Brinkman porosity and the objective
+@pytest.mark.parametrize("alpha,tau", [(0.3, 0.7), (0.75, 0.55), (0.05, 1.3)])
+def test_brinkman_uniform_porosity_reaches_closed_form_velocity(alpha, tau):
+    # Uniform periodic field, uniform alpha, Guo force g.  Streaming leaves a uniform field
+    # unchanged, so only the collision acts on j = rho u:
+    #   j* = j - omega (j - rho (1 - alpha)(u + g/2)) + (1 - omega/2) rho g.
+    # Its fixed point is u + g/2 = g tau_eff / alpha.  Scaling u + g/2 by (1 - alpha) only after
+    # the shift is what gives this; scaling before the shift, dropping the (1 - omega/2) factor,
+    # or a wrong sign of the source each move the fixed point.
+    g = np.array([1e-5, -2e-6, 0.0])
+    lat = O.Lattice(4, 4, 4, periodic=(True, True, True))
+    lat.set_solid(np.zeros(lat.n, np.uint8))
+    lat.set_force(g)
+    lat.set_alpha(np.full(lat.n, alpha))
+    lat.params.tau = tau
+    # relaxation factor per step of the deviation is 1 - omega alpha; run until it is below 1e-14
+    omega = 1.0 / O.tau_eff(tau)
+    steps = int(np.ceil(np.log(1e-14) / np.log(abs(1.0 - omega * alpha)))) + 10
+    f = lat.step(lat.equilibrium(), steps)
+    rho, u = lat.fields(f)
+    np.testing.assert_allclose(rho, 1.0, rtol=0, atol=1e-12)  # the Guo source has zero mass
+    expect = g * O.tau_eff(tau) / alpha
+    got = u + 0.5 * g
+    for d in (0, 1):
+        assert np.max(np.abs(got[:, d] / expect[d] - 1.0)) <= 1e-10, (d, got[0], expect)
+    assert np.max(np.abs(got[:, 2])) <= 1e-10 * abs(expect[0])  # round-off of the populations' sums
+
+
+def test_pressure_drop_of_a_density_ramp():
+    # J = cs^2 (<rho>_{x=xin} - <rho>_{x=xout}) over the fluid cells of the two planes.  For the
+    # state feq(rho, 0) with rho = 1 + a x + b y this is cs^2 (a (xin - xout) + b (<y>_in - <y>_out)),
+    # where <y> is the mean y of the fluid cells of each plane.  Solid cells hold f = 0 and must not
+    # count; the two planes have different solid sets, so a plane sum, a wrong count or a
+    # transposed plane index all fail.
+    nx, ny, nz = 9, 7, 5
+    lat = O.Lattice(nx, ny, nz)
+    solid = np.zeros((nz, ny, nx), np.uint8)
+    solid[:, 0, 2] = 1          # xin plane loses its y = 0 row
+    solid[1:3, 4:7, 6] = 1      # xout plane loses a block at large y
+    lat.set_solid(solid)
+    a, b = 2e-3, -7e-4
+    z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    rho = (1.0 + a * x + b * y).astype(np.float64)
+    f = lat.equilibrium(rho=rho)
+    fluid = ~solid.astype(bool)
+    for xin, xout in [(2, 6), (6, 2), (0, 8)]:
+        ybar = [y[:, :, xx][fluid[:, :, xx]].mean() for xx in (xin, xout)]
+        expect = (a * (xin - xout) + b * (ybar[0] - ybar[1])) / 3.0
+        assert abs(lat.pressure_drop(f, xin, xout) - expect) <= 1e-15, (xin, xout)
+    # the rest state f = w has no pressure drop
+    assert lat.pressure_drop(lat.equilibrium(), 2, 6) == 0.0
+
+
+def test_pressure_drop_gradient_is_the_objective_linear_map():
+    # J is affine in f: J(f) = lam . (f - w) exactly, with lam the oracle's dJ/df, and lam is zero
+    # outside the two planes' fluid cells.
+    lat = O.Lattice(8, 5, 4)
+    solid = np.zeros((4, 5, 8), np.uint8)
+    solid[0, 1, 1] = solid[2, 3, 6] = 1
+    lat.set_solid(solid)
+    rng = np.random.default_rng(4)
+    f = lat.equilibrium(rho=1.0 + 0.01 * rng.random(lat.n), u=0.02 * rng.standard_normal((lat.n, 3)))
+    f = f + 1e-3 * rng.standard_normal(f.size) * np.tile(lat.fluid, O.Q)
+    lam = lat.pressure_drop_grad(1, 6).reshape(O.Q, 4, 5, 8)
+    w = np.repeat(O.W, lat.n) * np.tile(lat.fluid, O.Q)
+    J = lat.pressure_drop(f, 1, 6)
+    assert abs(np.dot(lam.reshape(-1), f - w) - J) <= 1e-15 * max(1.0, abs(J)) + 1e-17
+    mask = np.zeros((4, 5, 8), bool)
+    mask[:, :, [1, 6]] = True
+    assert np.all(lam[:, ~mask] == 0.0)
+    assert np.all(lam[:, solid.astype(bool)] == 0.0)
