@@ -145,8 +145,11 @@ int lbm_set_coupling_cells(lbm_ctx* ctx, const int64_t* cells, int64_t n);
 /* MD data applied by every following step until changed: u[3k+d], rho[k] (NULL: no density
  * coupling); u == NULL switches the coupling off.  dt is the Dirichlet forcing's time step. */
 int lbm_set_coupling_data(lbm_ctx* ctx, int mode, const double* u, const double* rho, double dt);
-/* The export to MD (getAndSetV :136-151 before its scaling): rho[k] and u[3k+d] = (sum f c)/rho
- * of the current state at the coupled cells (0 at non-fluid cells); either pointer may be NULL. */
+/* The export to MD (getAndSetV :100-151 before its scaling by _lb2ms / _densityLB2MD): rho[k] and
+ * u[3k+d] = (sum f c)/rho of the current state at the coupled cells, passed through the GET block
+ * while coupling data is set, as the reference's SET block sends the values GET changed: u_MD in the
+ * overwrite mode, u + dt/2 (u_MD - u)/dt in the Dirichlet mode, rho_MD with density coupling.  0 at
+ * non-fluid cells; either pointer may be NULL. */
 int lbm_get_coupling_data(lbm_ctx* ctx, double* u, double* rho);
 
 /* Per-cell Brinkman porosity alpha in [0,1] (NULL disables).  Equilibrium velocity is

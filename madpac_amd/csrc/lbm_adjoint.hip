@@ -279,17 +279,12 @@ static int adjoint_args(lbm_ctx* c, int nsteps, int snapshots, int xin, int xout
   return LBM_OK;
 }
 
-// Re-store the current state in another layout: through the reference state (FP64, user order),
-// which the pull kernel reads and the unpull kernel scatters; FP32 values convert exactly.
+// Re-store the current state in another layout of the same cell order, value by value into a second
+// buffer (no staging memory, so it cannot fail for lack of memory once both buffers exist).
 static int restore_layout(lbm_ctx* c, const lbm::DenseGeom& from, const void* src, const lbm::DenseGeom& to,
                           void* dst) {
-  double* f = nullptr;
-  if (lbm::device_alloc(&f, (size_t)Q * c->g.Nu * sizeof(double)) != hipSuccess)
-    return adj_fail(c, LBM_ERR_NOMEM, "layout conversion buffer allocation failed");
-  hipError_t e = lbm::launch_dense_pull(from, c->fp32, src, f, c->words, c->stream);
-  if (e == hipSuccess) e = lbm::launch_dense_unpull(to, c->fp32, f, dst, c->words, c->stream);
+  hipError_t e = lbm::launch_state_relayout(from, to, c->fp32, src, dst, c->words, c->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-  (void)hipFree(f);
   return e == hipSuccess ? LBM_OK : adj_fail(c, LBM_ERR_HIP, std::string("layout conversion: ") + hipGetErrorString(e));
 }
 
@@ -318,12 +313,16 @@ extern "C" int lbm_adjoint_pressure_drop(lbm_ctx* c, int nsteps, int snapshots, 
     c->cur = 1;
     rc = adjoint_two_buffer(c, nsteps, snapshots, xin, xout, J, grad_alpha, stats);
     // back in place, whatever the adjoint returned (the state is the final one on success)
+    // The in-place layout is chosen for capacity: its adjoint workspace (checkpoints, adjoint states)
+    // is not kept between calls.
+    lbm_adjoint_release(c);
     lbm::DenseGeom natural = c->g;
     natural.layout = lbm::LAYOUT_AA_NATURAL;
     const int rc2 = restore_layout(c, lbm_state_geom(c), c->buf[c->cur], natural, c->buf[0]);
     c->aa = true;
     c->cur = 0;
     c->parity = 0;
+    if (rc2) c->state_set = false;  // buf[0] holds no valid state: it must be set again
     if (!rc) rc = rc2;
   }
   (void)hipFree(c->buf[1]);

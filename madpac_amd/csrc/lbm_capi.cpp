@@ -524,6 +524,7 @@ lbm_graph_key graph_key(const lbm_ctx* c) {
     k.ccells = c->d_ccells;
     k.cu = c->d_cu;
     k.crho = c->d_crho;
+    k.csave = c->d_csave;
     k.ncouple = c->ncouple;
   }
   k.alpha = c->alpha;

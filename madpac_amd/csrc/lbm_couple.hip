@@ -9,7 +9,7 @@
 // second small kernel re-collides only the coupled cells from the same source state and overwrites
 // their output slots (own slots in the pull form; in the in-place layout the slots the step wrote,
 // from populations saved before the step); the hot kernel is unchanged.  A gather kernel returns rho
-// and u of the coupled cells (the export).  Coupled cells use the reference's block-solver
+// and u of the coupled cells as getAndSetV exports them (after its GET block).  Coupled cells use the reference's block-solver
 // collision: no body force and no porosity (the host rejects those combinations).
 #include <cmath>
 #include <string>
@@ -32,6 +32,24 @@ struct CoupleArgs {
 };
 
 
+// The GET block of getAndSetV (:100-129) for coupled cell k, on the cell's rho and u: overwrite sets
+// u = u_MD (:121-122); Dirichlet computes g = (u_MD - u)/dt and shifts u += dt/2 g (:110-117); with
+// density coupling rho = rho_MD (:124-129).  g stays 0 in the overwrite mode.
+__device__ __forceinline__ void couple_get(const CoupleArgs& a, int64_t k, double& rho, double (&u)[3],
+                                           double (&g)[3]) {
+  const double* um = a.u + 3 * k;
+  g[0] = g[1] = g[2] = 0.0;
+  if (a.mode == LBM_COUPLE_DIRICHLET) {
+    for (int d = 0; d < 3; d++) {
+      g[d] = (um[d] - u[d]) / a.dt;
+      u[d] += a.dt / 2.0 * g[d];
+    }
+  } else {
+    for (int d = 0; d < 3; d++) u[d] = um[d];
+  }
+  if (a.rho) rho = a.rho[k];
+}
+
 // FP64 collision of one coupled cell (reference operation order, as the oracle restates it).
 __device__ __forceinline__ double collide_coupled_f64(double (&f)[Q], const CoupleArgs& a, int64_t k,
                                                       double (&uo)[3]) {
@@ -50,19 +68,9 @@ __device__ __forceinline__ double collide_coupled_f64(double (&f)[Q], const Coup
   }
   const double inv = 1.0 / rho;
   double u[3] = {b0 * inv, b1 * inv, b2 * inv};
-  const double* um = a.u + 3 * k;
-  double g[3] = {0.0, 0.0, 0.0};
+  double g[3];
+  couple_get(a, k, rho, u, g);
   const bool dir = a.mode == LBM_COUPLE_DIRICHLET;
-  if (dir) {
-    // getAndSetV :110-117: g = (u_MD - u)/dt, u += dt/2 g
-    for (int d = 0; d < 3; d++) {
-      g[d] = (um[d] - u[d]) / a.dt;
-      u[d] += a.dt / 2.0 * g[d];
-    }
-  } else {
-    for (int d = 0; d < 3; d++) u[d] = um[d];  // :121-122
-  }
-  if (a.rho) rho = a.rho[k];  // :124-129
   uo[0] = u[0];
   uo[1] = u[1];
   uo[2] = u[2];
@@ -174,14 +182,16 @@ __global__ __launch_bounds__(256) void k_couple_aa(DenseGeom g, const double* __
   }
 }
 
-// Export: rho and u = (sum f c) / rho of the reference state at the coupled cells (0 at solid cells).
+// Export (getAndSetV :136-151): rho and u = (sum f c) / rho of the reference state at the coupled
+// cells, passed through the GET block when the coupling is on (`on`), as the reference sends the
+// block's velocity and density after GET changed them; 0 at solid cells.
 template <typename T>
 __global__ __launch_bounds__(256) void k_couple_gather(DenseGeom g, const T* __restrict__ s,
-                                                       const uint32_t* __restrict__ words, const int64_t* __restrict__ cells,
-                                                       int64_t n, double* __restrict__ out) {
+                                                       const uint32_t* __restrict__ words, CoupleArgs a, int on,
+                                                       double* __restrict__ out) {
   const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (k >= n) return;
-  const int64_t i = cells[k];
+  if (k >= a.n) return;
+  const int64_t i = a.cells[k];
   const uint32_t w = words[i];
   double rho = 0.0, u0 = 0.0, u1 = 0.0, u2 = 0.0;
   if (w & FLUID) {
@@ -204,9 +214,11 @@ __global__ __launch_bounds__(256) void k_couple_gather(DenseGeom g, const T* __r
       if (CZ(q) == -1) b2 -= f[q];
     }
     const double inv = 1.0 / rho;
-    u0 = b0 * inv;
-    u1 = b1 * inv;
-    u2 = b2 * inv;
+    double u[3] = {b0 * inv, b1 * inv, b2 * inv}, gd[3];
+    if (on) couple_get(a, k, rho, u, gd);
+    u0 = u[0];
+    u1 = u[1];
+    u2 = u[2];
   }
   out[4 * k] = rho;
   out[4 * k + 1] = u0;
@@ -293,6 +305,8 @@ int lbm_enqueue_coupling_aa(lbm_ctx* c, int parity) {
 }
 
 void lbm_coupling_release(lbm_ctx* c) {
+  // captured steps hold the buffers' addresses: drop them before the buffers go
+  if (c->d_ccells) lbm_graphs_release(c);
   void* ptrs[] = {c->d_ccells, c->d_cu, c->d_crho, c->d_cout, c->d_csave};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -377,12 +391,15 @@ int lbm_get_coupling_data(lbm_ctx* c, double* u, double* rho) {
   if (!c->flags_set || !c->state_set) return cfail(c, LBM_ERR_STATE, "flags and state must be set first");
   if (hipSetDevice(c->cfg.device) != hipSuccess) return cfail(c, LBM_ERR_HIP, "hipSetDevice failed");
   const dim3 grid((unsigned)((c->ncouple + 255) / 256)), block(256);
+  const lbm::CoupleArgs a{c->d_ccells, c->ncouple, c->d_cu, c->couple_rho ? c->d_crho : nullptr, c->couple_mode,
+                          c->couple_dt, c->cfg.tau, c->p64.omega};
+  const int on = c->couple_on ? 1 : 0;
   if (c->fp32)
     hipLaunchKernelGGL(lbm::k_couple_gather<float>, grid, block, 0, c->stream, lbm_state_geom(c), (const float*)c->buf[c->cur],
-                       c->words, c->d_ccells, c->ncouple, c->d_cout);
+                       c->words, a, on, c->d_cout);
   else
     hipLaunchKernelGGL(lbm::k_couple_gather<double>, grid, block, 0, c->stream, lbm_state_geom(c), (const double*)c->buf[c->cur],
-                       c->words, c->d_ccells, c->ncouple, c->d_cout);
+                       c->words, a, on, c->d_cout);
   std::vector<double> out((size_t)c->ncouple * 4);
   if (hipGetLastError() != hipSuccess ||
       hipMemcpyAsync(out.data(), c->d_cout, out.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||

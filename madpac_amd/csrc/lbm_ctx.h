@@ -19,13 +19,13 @@ struct lbm_graph_key {
   int bricked = -1, remap = -1, by = -1, walls = 0, force = 0, spec = 0, couple_on = 0, couple_mode = 0,
       couple_rho = 0;
   double couple_dt = 0.0;
-  const void *alpha = nullptr, *bcu = nullptr, *ccells = nullptr, *cu = nullptr, *crho = nullptr;
+  const void *alpha = nullptr, *bcu = nullptr, *ccells = nullptr, *cu = nullptr, *crho = nullptr, *csave = nullptr;
   int64_t ncouple = 0;
   bool operator==(const lbm_graph_key& o) const {
     return bricked == o.bricked && remap == o.remap && by == o.by && walls == o.walls && force == o.force && spec == o.spec &&
            couple_on == o.couple_on && couple_mode == o.couple_mode && couple_rho == o.couple_rho &&
            couple_dt == o.couple_dt && alpha == o.alpha && bcu == o.bcu && ccells == o.ccells && cu == o.cu &&
-           crho == o.crho && ncouple == o.ncouple;
+           crho == o.crho && csave == o.csave && ncouple == o.ncouple;
   }
 };
 

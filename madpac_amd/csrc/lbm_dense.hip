@@ -277,6 +277,22 @@ __global__ __launch_bounds__(BX * BY_MAX) void k_dense_unpull(DenseGeom g, const
   }
 }
 
+// The same state in another layout of the same cell order (e.g. pull form <-> AA NATURAL): each
+// fluid cell copies its 19 values of the reference state from where `from` keeps them to where `to`
+// does.  src and dst are distinct buffers; the stored values are copied as they are (exact in both
+// precisions) and non-fluid cells are left alone.
+template <typename T>
+__global__ __launch_bounds__(BX * BY_MAX) void k_state_relayout(DenseGeom from, DenseGeom to, const T* __restrict__ src,
+                                                            T* __restrict__ dst, const uint32_t* __restrict__ words) {
+  const CellPos c = map_cell(to);
+  if (!c.valid) return;
+  const uint32_t w = words[c.i];
+  if (!(w & FLUID)) return;
+  const NbrOff o = nbr_offsets(to, c);
+#pragma unroll
+  for (int q = 0; q < Q; q++) dst[state_index(to, q, w, c.i, o)] = src[state_index(from, q, w, c.i, o)];
+}
+
 // feq(rho, u) exactly as BLB/BlockComputeCellValues.cpp:9-22 evaluates it, scattered like unpull.
 template <typename T>
 __global__ __launch_bounds__(BX * BY_MAX) void k_dense_init_eq(DenseGeom g, const double* __restrict__ rho,
@@ -620,6 +636,18 @@ hipError_t launch_dense_unpull(const DenseGeom& g, bool fp32, const double* f, v
                                hipStream_t s) {
   if (fp32) hipLaunchKernelGGL(k_dense_unpull<float>, dense_grid(g), dense_block(g), 0, s, g, f, (float*)dst, words);
   else hipLaunchKernelGGL(k_dense_unpull<double>, dense_grid(g), dense_block(g), 0, s, g, f, (double*)dst, words);
+  return hipGetLastError();
+}
+
+hipError_t launch_state_relayout(const DenseGeom& from, const DenseGeom& to, bool fp32, const void* src, void* dst,
+                                 const uint32_t* words, hipStream_t s) {
+  if (from.bricked != to.bricked || from.N != to.N) return hipErrorInvalidValue;
+  if (fp32)
+    hipLaunchKernelGGL(k_state_relayout<float>, dense_grid(to), dense_block(to), 0, s, from, to, (const float*)src,
+                       (float*)dst, words);
+  else
+    hipLaunchKernelGGL(k_state_relayout<double>, dense_grid(to), dense_block(to), 0, s, from, to, (const double*)src,
+                       (double*)dst, words);
   return hipGetLastError();
 }
 

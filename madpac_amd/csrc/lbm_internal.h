@@ -121,6 +121,10 @@ hipError_t launch_dense_pull(const DenseGeom& g, bool fp32, const void* src, dou
 // stored state from a reference state f (FP64)
 hipError_t launch_dense_unpull(const DenseGeom& g, bool fp32, const double* f, void* dst, const uint32_t* words,
                                hipStream_t stream);
+// the stored state of `from` re-stored in the layout of `to` (same cell order and plane stride) into
+// another buffer, values copied exactly, no staging memory
+hipError_t launch_state_relayout(const DenseGeom& from, const DenseGeom& to, bool fp32, const void* src, void* dst,
+                                 const uint32_t* words, hipStream_t stream);
 // stored state from feq(rho, u); rho/u may be null (1 / 0)
 hipError_t launch_dense_init_eq(const DenseGeom& g, bool fp32, const double* rho, const double* u, void* dst,
                                 const uint32_t* words, hipStream_t stream);

@@ -151,6 +151,13 @@ def _u8(a):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))
 
 
+def _sized(a, size, what):
+    """a, after checking that it holds exactly `size` elements: the C side reads that many."""
+    if a.size != size:
+        raise ValueError(f"{what}: expected {size} values, got {a.size}")
+    return a
+
+
 class Lattice:
     """A dense D3Q19 lattice on one GPU (an lbm_ctx)."""
 
@@ -191,8 +198,7 @@ class Lattice:
 
     # ---- geometry
     def set_flags(self, inner):
-        a = np.ascontiguousarray(np.asarray(inner).reshape(-1) != 0, dtype=np.uint8)
-        assert a.size == self.n
+        a = _sized(np.ascontiguousarray(np.asarray(inner).reshape(-1) != 0, dtype=np.uint8), self.n, "flags")
         self._check(self._L.lbm_set_flags(self._h, _u8(a)), "lbm_set_flags")
 
     def set_cavity_lid(self, ulid):
@@ -202,7 +208,7 @@ class Lattice:
         self._check(self._L.lbm_set_channel(self._h, float(u_mean), int(bool(parabolic))), "lbm_set_channel")
 
     def set_boundary_class(self, k, bc_type, rho=1.0, u=(0.0, 0.0, 0.0), mask=0):
-        u = np.ascontiguousarray(u, dtype=np.float64)
+        u = _sized(np.ascontiguousarray(u, dtype=np.float64).reshape(-1), 3, "boundary velocity")
         self._check(self._L.lbm_set_boundary_class(self._h, int(k), int(bc_type), float(rho), _dp(u),
                                                    ctypes.c_uint32(mask)), "lbm_set_boundary_class")
 
@@ -210,8 +216,8 @@ class Lattice:
         if u is None:
             self._check(self._L.lbm_set_boundary_velocity(self._h, None), "lbm_set_boundary_velocity")
             return
-        a = np.ascontiguousarray(np.asarray(u, dtype=np.float64).reshape(-1))
-        assert a.size == 3 * self.n
+        a = _sized(np.ascontiguousarray(np.asarray(u, dtype=np.float64).reshape(-1)), 3 * self.n,
+                   "boundary velocities")
         self._check(self._L.lbm_set_boundary_velocity(self._h, _dp(a)), "lbm_set_boundary_velocity")
 
     # ---- MD coupling
@@ -230,8 +236,10 @@ class Lattice:
             self._check(self._L.lbm_set_coupling_data(self._h, int(mode), None, None, float(dt)),
                         "lbm_set_coupling_data")
             return
-        u = np.ascontiguousarray(np.asarray(u, dtype=np.float64).reshape(-1))
-        r = None if rho is None else np.ascontiguousarray(np.asarray(rho, dtype=np.float64).reshape(-1))
+        n = getattr(self, "_ncouple", 0)
+        u = _sized(np.ascontiguousarray(np.asarray(u, dtype=np.float64).reshape(-1)), 3 * n, "coupling velocities")
+        r = None if rho is None else _sized(np.ascontiguousarray(np.asarray(rho, dtype=np.float64).reshape(-1)), n,
+                                            "coupling densities")
         self._check(self._L.lbm_set_coupling_data(self._h, int(mode), _dp(u), None if r is None else _dp(r), float(dt)),
                     "lbm_set_coupling_data")
 
@@ -242,7 +250,7 @@ class Lattice:
         return rho, u
 
     def set_moving_wall_class(self, k, uw, mask):
-        uw = np.ascontiguousarray(uw, dtype=np.float64)
+        uw = _sized(np.ascontiguousarray(uw, dtype=np.float64).reshape(-1), 3, "wall velocity")
         self._check(self._L.lbm_set_moving_wall_class(self._h, int(k), _dp(uw), ctypes.c_uint32(mask)),
                     "lbm_set_moving_wall_class")
 
@@ -250,14 +258,14 @@ class Lattice:
         if cls is None:
             self._check(self._L.lbm_set_wall_classes(self._h, None), "lbm_set_wall_classes")
             return
-        a = np.ascontiguousarray(np.asarray(cls).reshape(-1), dtype=np.uint8)
+        a = _sized(np.ascontiguousarray(np.asarray(cls).reshape(-1), dtype=np.uint8), self.n, "wall classes")
         self._check(self._L.lbm_set_wall_classes(self._h, _u8(a)), "lbm_set_wall_classes")
 
     def set_porosity(self, alpha):
         if alpha is None:
             self._check(self._L.lbm_set_porosity(self._h, None), "lbm_set_porosity")
             return
-        a = np.ascontiguousarray(np.asarray(alpha, dtype=np.float64).reshape(-1))
+        a = _sized(np.ascontiguousarray(np.asarray(alpha, dtype=np.float64).reshape(-1)), self.n, "porosity")
         self._check(self._L.lbm_set_porosity(self._h, _dp(a)), "lbm_set_porosity")
 
     @property
@@ -271,14 +279,15 @@ class Lattice:
 
     # ---- state
     def init_equilibrium(self, rho=None, u=None):
-        r = None if rho is None else np.ascontiguousarray(np.asarray(rho, dtype=np.float64).reshape(-1))
-        v = None if u is None else np.ascontiguousarray(np.asarray(u, dtype=np.float64).reshape(-1))
+        r = None if rho is None else _sized(np.ascontiguousarray(np.asarray(rho, dtype=np.float64).reshape(-1)),
+                                            self.n, "density")
+        v = None if u is None else _sized(np.ascontiguousarray(np.asarray(u, dtype=np.float64).reshape(-1)),
+                                          3 * self.n, "velocity")
         self._check(self._L.lbm_init_equilibrium(self._h, None if r is None else _dp(r), None if v is None else _dp(v)),
                     "lbm_init_equilibrium")
 
     def set_populations(self, f):
-        a = np.ascontiguousarray(np.asarray(f, dtype=np.float64).reshape(-1))
-        assert a.size == Q * self.n
+        a = _sized(np.ascontiguousarray(np.asarray(f, dtype=np.float64).reshape(-1)), Q * self.n, "populations")
         self._check(self._L.lbm_set_populations(self._h, _dp(a)), "lbm_set_populations")
 
     def get_populations(self):
@@ -385,8 +394,10 @@ class PatchSet:
         self.close()
 
     def init(self, rho=None, u=None):
-        r = None if rho is None else np.ascontiguousarray(np.asarray(rho, dtype=np.float64).reshape(-1))
-        v = None if u is None else np.ascontiguousarray(np.asarray(u, dtype=np.float64).reshape(-1))
+        r = None if rho is None else _sized(np.ascontiguousarray(np.asarray(rho, dtype=np.float64).reshape(-1)),
+                                            self.cells, "density")
+        v = None if u is None else _sized(np.ascontiguousarray(np.asarray(u, dtype=np.float64).reshape(-1)),
+                                          3 * self.cells, "velocity")
         self._check(self._L.lbm_patches_init(self._h, None if r is None else _dp(r), None if v is None else _dp(v)),
                     "lbm_patches_init")
 

@@ -102,6 +102,8 @@ def lib():
         L.lbo_stream.argtypes = [ctypes.POINTER(Dims), up, dp, dp]
         L.lbo_step_swap_aos.argtypes = [ctypes.POINTER(Dims), up, ctypes.POINTER(Params), dp]
         L.lbo_fields.argtypes = [ctypes.POINTER(Dims), up, dp, dp, dp]
+        L.lbo_couple_export.argtypes = [ctypes.POINTER(Dims), up, ctypes.POINTER(Params), dp,
+                                        ctypes.POINTER(ctypes.c_int64), ctypes.c_int64, dp, dp]
         L.lbo_init_equilibrium.argtypes = [ctypes.POINTER(Dims), up, dp, dp, dp]
         L.lbo_moments.argtypes = [ctypes.POINTER(Dims), up, dp, dp]
         L.lbo_pressure_drop.restype = ctypes.c_double
@@ -234,6 +236,17 @@ class Lattice:
         rho = np.empty(self.n)
         u = np.empty((self.n, 3))
         lib().lbo_fields(ctypes.byref(self.dims), _up(self.words), _dp(np.ascontiguousarray(f)), _dp(rho), _dp(u))
+        return rho, u
+
+    def coupling_export(self, f, cells):
+        """What getAndSetV sends to MD for `cells` from state f: (rho[k], u[k]) after its GET
+        block (the coupling data set by set_coupling, if any, applied)."""
+        cells = np.ascontiguousarray(np.asarray(cells, dtype=np.int64).reshape(-1))
+        rho = np.empty(cells.size)
+        u = np.empty((cells.size, 3))
+        lib().lbo_couple_export(ctypes.byref(self.dims), _up(self.words), ctypes.byref(self.params),
+                                _dp(np.ascontiguousarray(f)), cells.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                                cells.size, _dp(rho), _dp(u))
         return rho, u
 
     def moments(self, f):

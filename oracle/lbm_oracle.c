@@ -172,11 +172,13 @@ void lbo_couple_gi(double rho, double tau, const double* u, const double* g, dou
  * u <- u + dt/2 g, then f += dt g_i after the BGK relaxation.  With density coupling rho_MD
  * replaces the density of the equilibrium (and of g_i).  The reference's adjustF applies its
  * term to cell index `_level` of each block only; here it applies to every coupled cell. */
-static double collide_coupled(double* f, double tau_eff, const lbo_params* p, int64_t row, double* uout) {
-  double rho = lbo_density(f);
-  double u[3], g[3] = {0.0, 0.0, 0.0};
-  lbo_velocity(f, rho, u);
+/* The GET block of getAndSetV (madpac/peanocoupling/PeanoDataExchangeService.cpp:100-129) for
+ * coupled row `row`: overwrite sets u = u_MD (:121-122); Dirichlet computes g = (u_MD - u)/dt and
+ * shifts u += dt/2 g (:110-117); with density coupling rho = rho_MD (:124-129).  g is 0 in the
+ * overwrite mode. */
+static void couple_get(const lbo_params* p, int64_t row, double* rho, double* u, double* g) {
   const double* um = p->couple_u + 3 * row;
+  for (int d = 0; d < 3; d++) g[d] = 0.0;
   if (p->couple_mode == LBO_COUPLE_DIRICHLET) {
     for (int d = 0; d < 3; d++) {
       g[d] = (um[d] - u[d]) / p->couple_dt;
@@ -185,7 +187,19 @@ static double collide_coupled(double* f, double tau_eff, const lbo_params* p, in
   } else {
     for (int d = 0; d < 3; d++) u[d] = um[d];
   }
-  if (p->couple_rho) rho = p->couple_rho[row];
+  if (p->couple_rho) *rho = p->couple_rho[row];
+}
+
+/* Collision of a coupled cell (getAndSetV :69-158 before BlockBGK, adjustF :189-218 after it):
+ * overwrite: the equilibrium velocity is the MD velocity; Dirichlet: g = (u_MD - u)/dt,
+ * u <- u + dt/2 g, then f += dt g_i after the BGK relaxation.  With density coupling rho_MD
+ * replaces the density of the equilibrium (and of g_i).  The reference's adjustF applies its
+ * term to cell index `_level` of each block only; here it applies to every coupled cell. */
+static double collide_coupled(double* f, double tau_eff, const lbo_params* p, int64_t row, double* uout) {
+  double rho = lbo_density(f);
+  double u[3], g[3];
+  lbo_velocity(f, rho, u);
+  couple_get(p, row, &rho, u, g);
   for (int d = 0; d < 3; d++) uout[d] = u[d];
   double feq[LBO_Q];
   lbo_feq(rho, u, feq);
@@ -430,6 +444,31 @@ void lbo_fields(const lbo_dims* d, const uint32_t* words, const double* f, doubl
     for (int q = 0; q < LBO_Q; q++) fi[q] = f[(int64_t)q * N + i];
     rho[i] = lbo_density(fi);
     lbo_velocity(fi, rho[i], u + 3 * i);
+  }
+}
+
+/* The export to MD of getAndSetV (madpac/peanocoupling/PeanoDataExchangeService.cpp:69-151):
+ * the SET block (:136-151) sends the block's velocity and density after the GET block (:100-129)
+ * has changed them, i.e. for the listed cells rho and u = (sum f c)/rho of state f
+ * (BLB/BlockComputeCellValues.cpp:26-68) passed through couple_get when the cell has coupling data
+ * (p->couple_row non-NULL and row >= 0), before the scaling by _lb2ms / _densityLB2MD.  Non-fluid
+ * cells give 0. */
+void lbo_couple_export(const lbo_dims* d, const uint32_t* words, const lbo_params* p, const double* f,
+                       const int64_t* cells, int64_t n, double* rho, double* u) {
+  const int64_t N = ncells(d);
+  for (int64_t k = 0; k < n; k++) {
+    const int64_t i = cells[k];
+    double r = 0.0, v[3] = {0.0, 0.0, 0.0}, g[3];
+    if (words[i] & LBO_FLUID) {
+      double fi[LBO_Q];
+      for (int q = 0; q < LBO_Q; q++) fi[q] = f[(int64_t)q * N + i];
+      r = lbo_density(fi);
+      lbo_velocity(fi, r, v);
+      const int64_t row = p->couple_row ? p->couple_row[i] : -1;
+      if (row >= 0) couple_get(p, row, &r, v, g);
+    }
+    rho[k] = r;
+    for (int dd = 0; dd < 3; dd++) u[3 * k + dd] = v[dd];
   }
 }
 

@@ -126,6 +126,9 @@ void lbo_stream(const lbo_dims* d, const uint32_t* words, const double* src, dou
 
 /* fields: rho, u (u = 0 and rho untouched = 0 on non-fluid cells) */
 void lbo_fields(const lbo_dims* d, const uint32_t* words, const double* f, double* rho, double* u);
+/* MD export of getAndSetV: rho[k], u[3k+d] of cells[k] after the coupling's GET block */
+void lbo_couple_export(const lbo_dims* d, const uint32_t* words, const lbo_params* p, const double* f,
+                       const int64_t* cells, int64_t n, double* rho, double* u);
 /* equilibrium initialisation from rho/u fields (non-fluid cells: 0) */
 void lbo_init_equilibrium(const lbo_dims* d, const uint32_t* words, const double* rho,
                           const double* u, double* f);

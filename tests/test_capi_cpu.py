@@ -80,3 +80,22 @@ def test_null_handles_are_rejected():
     assert L.lbm_patches_get_populations(None, None) == -1
     assert L.lbm_get_fields(None, None, None) == -1
     assert L.lbm_tune(None, 0, None) == -1
+
+
+def test_binding_rejects_short_arrays_before_calling_the_library():
+    # the C side reads n (or 3n, 19n) values from these pointers: a short array must not get there
+    class Never:
+        def __getattr__(self, name):
+            raise AssertionError(f"{name} called with a wrongly sized array")
+
+    lat = object.__new__(lbm.Lattice)
+    lat._L, lat._h, lat.n, lat._ncouple = Never(), None, 27, 4
+    import numpy as np
+    for call in (lambda: lat.set_flags(np.ones(26)), lambda: lat.set_porosity(np.zeros(26)),
+                 lambda: lat.set_wall_classes(np.zeros(28, np.uint8)), lambda: lat.init_equilibrium(rho=np.ones(26)),
+                 lambda: lat.init_equilibrium(u=np.zeros(3 * 27 - 1)), lambda: lat.set_populations(np.zeros(19 * 26)),
+                 lambda: lat.set_boundary_velocity(np.zeros(3 * 26)), lambda: lat.set_coupling_data(0, np.zeros(11)),
+                 lambda: lat.set_coupling_data(0, np.zeros(12), rho=np.ones(3)),
+                 lambda: lat.set_moving_wall_class(1, (0.1, 0.0), 0)):
+        with pytest.raises(ValueError, match="expected"):
+            call()

@@ -1,7 +1,8 @@
 """GPU parity of the MD <-> LB coupling hooks (madpac/peanocoupling/PeanoDataExchangeService.cpp)
 through the C-ABI, against the oracle's restatement.  Bars: FP64 bit-identical to the oracle;
 FP32-shifted contexts (coupled cells re-collided in FP64) at u L2 <= 1e-5; the export equals the
-oracle's fields at the coupled cells."""
+oracle's getAndSetV export (after its GET block) bit for bit, and the plain fields once the coupling
+is off."""
 import numpy as np
 import pytest
 
@@ -38,9 +39,14 @@ def test_fp64_coupled_cavity_equals_oracle(mode, density):
         lat.step(steps)
         f = lat.get_populations()
         r_out, u_out = lat.get_coupling_data()
+        lat.set_coupling_data(mode, None)
+        r_off, u_off = lat.get_coupling_data()
     assert np.array_equal(f, fo), f"max |diff| = {np.abs(f - fo).max()}"
-    ro, uo = ora.fields(fo)
-    assert np.array_equal(r_out, ro[cells]) and np.array_equal(u_out, uo[cells])
+    # the export is what getAndSetV sends after its GET block: MD-modified while coupling is on
+    ro, uo = ora.coupling_export(fo, cells)
+    assert np.array_equal(r_out, ro) and np.array_equal(u_out, uo)
+    rf, uf = ora.fields(fo)
+    assert np.array_equal(r_off, rf[cells]) and np.array_equal(u_off, uf[cells])
 
 
 @pytest.mark.parametrize("mode", [0, 1])

@@ -292,6 +292,35 @@ def _gather_outgoing(f, n, x):
     return out
 
 
+
+@pytest.mark.parametrize("mode,density", [(O.COUPLE_OVERWRITE, False), (O.COUPLE_OVERWRITE, True),
+                                          (O.COUPLE_DIRICHLET, False), (O.COUPLE_DIRICHLET, True)])
+def test_coupling_export_is_the_velocity_after_the_get_block(mode, density):
+    # getAndSetV (PeanoDataExchangeService.cpp:100-151) sends the block velocity and density after
+    # its GET block changed them: overwrite sends u_MD; Dirichlet sends u + dt/2 (u_MD - u)/dt, which
+    # is (u + u_MD)/2 whatever dt; density coupling sends rho_MD, else the cell's rho.  Cells without
+    # coupling data, and every cell once the coupling is off, send their own moments.
+    lat = O.Lattice.cavity(6, 0.7, 0.0)
+    rng = np.random.default_rng(11)
+    f = lat.equilibrium(rho=1.0 + 0.01 * rng.random(lat.n), u=0.02 * rng.standard_normal((lat.n, 3)))
+    rho, u = lat.fields(f)
+    coupled = rng.choice(lat.n, 20, replace=False)
+    umd = 0.03 * rng.standard_normal((20, 3))
+    rmd = 1.0 + 0.02 * rng.random(20)
+    lat.set_coupling(coupled, mode, umd, rmd if density else None, dt=0.37)
+    others = np.setdiff1d(np.arange(lat.n), coupled)[:15]
+    r_c, u_c = lat.coupling_export(f, coupled)
+    if mode == O.COUPLE_OVERWRITE:
+        assert np.array_equal(u_c, umd)
+    else:
+        np.testing.assert_allclose(u_c, 0.5 * (u[coupled] + umd), rtol=0, atol=1e-16)
+    assert np.array_equal(r_c, rmd if density else rho[coupled])
+    r_o, u_o = lat.coupling_export(f, others)
+    assert np.array_equal(r_o, rho[others]) and np.array_equal(u_o, u[others])
+    lat.set_coupling(None, mode, None)
+    r_off, u_off = lat.coupling_export(f, coupled)
+    assert np.array_equal(r_off, rho[coupled]) and np.array_equal(u_off, u[coupled])
+
 @pytest.mark.parametrize("density", [False, True])
 def test_coupling_overwrite_relaxes_to_md_equilibrium(density):
     # tau_eff = 1: a coupled cell's post-collision populations are feq(rho, u_MD) (rho_MD with
