@@ -59,17 +59,24 @@ def build(verbose: bool = False) -> str:
     return LIB
 
 
+CPP_TESTS = (("test_peano_api", True), ("test_peano_events", True), ("test_geometry", False))
+
+
 def build_cpp_test(verbose: bool = False) -> str:
-    """Compile tests/cpp/test_peano_api (a C++ program on the reference-shaped API) against the library."""
-    src = os.path.join(ROOT, "tests", "cpp", "test_peano_api.cpp")
-    exe = os.path.join(ROOT, "tests", "cpp", "test_peano_api")
-    cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-I" + os.path.join(ROOT, "include"), "-I" + CSRC, src,
-           "-L" + HERE, "-llbm_hip", "-Wl,-rpath,$ORIGIN/../../madpac_amd", "-o", exe]
-    r = subprocess.run(cmd, capture_output=True, text=True)
-    if r.returncode != 0:
-        raise RuntimeError(f"C++ API test build failed:\n{r.stderr}")
-    if verbose:
-        print("built", exe, file=sys.stderr)
+    """Compile the C++ programs of tests/cpp: test_peano_api and test_peano_events (the reference-shaped
+    API, linked against the library) and test_geometry (the block store's geometry flags, host only)."""
+    exe = None
+    for name, links in CPP_TESTS:
+        src = os.path.join(ROOT, "tests", "cpp", name + ".cpp")
+        exe = os.path.join(ROOT, "tests", "cpp", name)
+        cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-I" + os.path.join(ROOT, "include"), "-I" + CSRC, src]
+        if links:
+            cmd += ["-L" + HERE, "-llbm_hip", "-Wl,-rpath,$ORIGIN/../../madpac_amd"]
+        r = subprocess.run(cmd + ["-o", exe], capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"C++ test {name} build failed:\n{r.stderr}")
+        if verbose:
+            print("built", exe, file=sys.stderr)
     return exe
 
 

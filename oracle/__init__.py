@@ -139,9 +139,16 @@ class Lattice:
 
     # ----- geometry -----
     @classmethod
-    def cavity(cls, n, tau, ulid):
+    def cavity(cls, n, tau, ulid, solid=None):
+        """The reference's lid-driven cavity on n^3 cells; `solid` (n^3, 1 = solid) adds obstacles,
+        whose links stay halfway bounce-back while the top layer's c_z > 0 links are the lid."""
         lat = cls(n, n, n)
         lib().lbo_build_cavity(n, _up(lat.words), ctypes.byref(lat.params), ulid)
+        if solid is not None:
+            lat.set_solid(solid)
+            top = np.zeros((n, n, n), bool)
+            top[n - 1] = True
+            lat.words[top.reshape(-1) & lat.fluid] |= np.uint32(1 << CLASS_SHIFT)
         lat.params.tau = tau
         return lat
 

@@ -16,12 +16,17 @@ its outputs as small ``.npz`` fixtures next to this file:
 * ``channel_*.npz`` -- the reference's default channel (no-slip y/z walls, velocity inlet on
   x-, pressure outlet on x+; uniform or parabolic inlet profile), full FP64 populations over
   the fluid box after ``steps`` time steps.
+* ``geometry_*.npz`` -- runs with builtin obstacles in the ``<builtin-geometry-configuration>``
+  tag (an inverted ``<sphere>``, an inverted ``<hexahedron>``) besides the domain box: the whole
+  block store's ``inner`` and ``boundary`` flags (peano/geometry/builtin/Sphere.cpp,
+  Hexahedron.cpp through GridManagementService::createVertex) and the populations over the box
+  after ``steps`` time steps (0 in non-inner cells).
 * ``vtk_*.npz`` -- the legacy-VTK field files the reference writes with its plotter tag
   (``<plotter vtk-outputname=... timesteps-per-plotting=...>``), read back with
   ``tests/vtk_io.py``: per plotted iteration the points, voxels, cell types, density, velocity
   and block ids, plus the section header lines.
 
-Usage:  python tests/golden/make_fixtures.py [--skip-64] [--only-vtk]
+Usage:  python tests/golden/make_fixtures.py [--skip-64] [--only-vtk] [--only-geometry]
 """
 from __future__ import annotations
 
@@ -54,7 +59,7 @@ XML = """<?xml version="1.0"?>
     <lattice-boltzmann-scenario scenario="{scenario}" characteristic-velocity_L="{ulid!r}"{extra} dynamic-geometry="no" compute-forces="no"></lattice-boltzmann-scenario>
     <collision-model name="BGK" hMin="0.00001" hMax="10.0" />
     <builtin-geometry-configuration>
-      <hexahedron geometry-id="0" invert="false" width="{lx!r};{ly!r};{lz!r}" offset="0.0;0.0;0.0" />
+      <hexahedron geometry-id="0" invert="false" width="{lx!r};{ly!r};{lz!r}" offset="0.0;0.0;0.0" />{objects}
     </builtin-geometry-configuration>
   </run-blocklatticeboltzmann-on-regular-grid>
 </peano-configuration>
@@ -94,7 +99,7 @@ def run_reference(dims, tau, ulid, steps, scenario="lid-driven-cavity", paraboli
         xml = os.path.join(td, "run.xml")
         with open(xml, "w") as fh:
             fh.write(XML.format(nx=nx, ny=ny, nz=nz, lx=nx / ny, ly=1.0, lz=nz / ny, tau=tau, ulid=ulid,
-                                steps=steps, scenario=scenario, extra=extra))
+                                steps=steps, scenario=scenario, extra=extra, objects=""))
         out = os.path.join(td, "dump.bin")
         subprocess.run([HARNESS, xml, str(nb[0] * nb[1] * nb[2]), out], check=True, cwd=td,
                        stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
@@ -115,6 +120,82 @@ def run_reference(dims, tau, ulid, steps, scenario="lid-driven-cavity", paraboli
     return np.ascontiguousarray(np.moveaxis(dense[box], -1, 0))  # [q, z, y, x]
 
 
+def _vec(v):
+    return ";".join(repr(float(x)) for x in v)
+
+
+def object_xml(obj):
+    """One builtin geometry object as the reference's configuration reads it
+    (peano/geometry/builtin/configurations/SphereConfiguration.cpp, HexahedronConfiguration.cpp)."""
+    kind, gid, invert = obj[0], obj[1], "true" if obj[2] else "false"
+    if kind == "sphere":
+        return (f'\n      <sphere geometry-id="{gid}" invert="{invert}" center="{_vec(obj[3])}" '
+                f'radius="{float(obj[4])!r}" />')
+    return (f'\n      <hexahedron geometry-id="{gid}" invert="{invert}" width="{_vec(obj[4])}" '
+            f'offset="{_vec(obj[3])}" />')
+
+
+def run_reference_geometry(dims, tau, u, steps, objects, scenario="lid-driven-cavity", parabolic=None):
+    """Run the reference with extra builtin geometry objects; return (f over the box [q, z, y, x],
+    inner and boundary flags of the whole block store [6 nbz, 6 nby, 6 nbx])."""
+    nx, ny, nz = dims
+    nb = [blocks_per_dim(n) for n in dims]
+    extra = "" if parabolic is None else f' use-parabolic-profile="{"yes" if parabolic else "no"}"'
+    xml = XML.format(nx=nx, ny=ny, nz=nz, lx=nx / ny, ly=1.0, lz=nz / ny, tau=tau, ulid=u, steps=steps,
+                     scenario=scenario, extra=extra, objects="".join(object_xml(o) for o in objects))
+    with tempfile.TemporaryDirectory() as td:
+        with open(os.path.join(td, "run.xml"), "w") as fh:
+            fh.write(xml)
+        subprocess.run([HARNESS, "run.xml", str(nb[0] * nb[1] * nb[2]), "dump.bin"], check=True, cwd=td,
+                       stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+        raw = open(os.path.join(td, "dump.bin"), "rb").read()
+    rec = 216 * 2 + 8 * (216 + 648 + 4104)
+    shape = (6 * nb[2], 6 * nb[1], 6 * nb[0])
+    dense = np.zeros(shape + (19,))
+    inner = np.zeros(shape, np.uint8)
+    boundary = np.zeros(shape, np.uint8)
+    for k in range(nb[0] * nb[1] * nb[2]):
+        o = 16 + k * rec
+        bx, by, bz = k % nb[0], (k // nb[0]) % nb[1], k // (nb[0] * nb[1])
+        sl = (slice(6 * bz, 6 * bz + 6), slice(6 * by, 6 * by + 6), slice(6 * bx, 6 * bx + 6))
+        inner[sl] = np.frombuffer(raw[o:o + 216], np.uint8).reshape(6, 6, 6)
+        boundary[sl] = np.frombuffer(raw[o + 216:o + 432], np.uint8).reshape(6, 6, 6)
+        dense[sl] = np.frombuffer(raw[o + 432 + 8 * 864:o + rec], "<f8").reshape(6, 6, 6, 19)
+    box = (slice(3, 3 + nz), slice(3, 3 + ny), slice(3, 3 + nx))
+    outside = inner.copy()
+    outside[box] = 0
+    assert not outside.any(), "inner cells outside the box"
+    assert not np.where(inner[..., None] == 0, dense, 0.0).any(), "non-inner cells must keep zero populations"
+    f = np.ascontiguousarray(np.moveaxis(dense[box], -1, 0))
+    return f, inner, boundary
+
+
+def save_geometry_fixture(name, dims, tau, u, steps, objects, scenario, parabolic):
+    f, inner, boundary = run_reference_geometry(dims, tau, u, steps, objects, scenario, parabolic)
+    kinds = np.array([0 if o[0] == "hexahedron" else 1 for o in objects], np.int64)
+    inv = np.array([int(o[2]) for o in objects], np.int64)
+    # per object: hexahedron offset[3], width[3]; sphere centre[3], radius, 0, 0
+    params = np.array([list(o[3]) + (list(o[4]) if o[0] == "hexahedron" else [float(o[4]), 0.0, 0.0])
+                       for o in objects], np.float64)
+    np.savez_compressed(os.path.join(HERE, name), f=f, inner=inner, boundary=boundary,
+                        dims=np.array(dims, np.int64), tau=np.float64(tau), u=np.float64(u), steps=np.int64(steps),
+                        scenario=np.array(scenario), parabolic=np.int64(bool(parabolic)),
+                        object_kind=kinds, object_inverted=inv, object_params=params)
+    print("geometry", name, int(inner.sum()), "inner cells", file=sys.stderr)
+
+
+# Obstacle cases: (file, dims, tau, velocity, steps, objects, scenario, parabolic).  The domain box
+# (domain size n_d / n_y) is always the first object.  Sphere radii and centres keep every cell
+# centre and voxel corner at least 1e-3 dx away from a surface.
+GEOMETRY_CASES = (
+    ("geometry_cavity_sphere_n12_t40.npz", (12, 12, 12), 0.6, 0.08, 40,
+     [("sphere", 1, True, (0.45, 0.52, 0.40), 0.26)], "lid-driven-cavity", None),
+    ("geometry_channel_16x8x8_t60.npz", (16, 8, 8), 0.7, 0.04, 60,
+     [("sphere", 1, True, (0.61, 0.47, 0.53), 0.23),
+      ("hexahedron", 2, True, (1.25, 0.125, 0.25), (0.25, 0.5, 0.375))], "default-channel", True),
+)
+
+
 def run_reference_vtk(dims, tau, u, steps, every, scenario="lid-driven-cavity", parabolic=None):
     """Run the reference with VTK plotting every `every` iterations; return {iteration: parsed file}."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
@@ -123,7 +204,7 @@ def run_reference_vtk(dims, tau, u, steps, every, scenario="lid-driven-cavity", 
     nb = [blocks_per_dim(n) for n in dims]
     extra = "" if parabolic is None else f' use-parabolic-profile="{"yes" if parabolic else "no"}"'
     xml = XML.format(nx=nx, ny=ny, nz=nz, lx=nx / ny, ly=1.0, lz=nz / ny, tau=tau, ulid=u, steps=steps,
-                     scenario=scenario, extra=extra)
+                     scenario=scenario, extra=extra, objects="")
     tag = f'    <plotter vtk-outputname="ref" timesteps-per-plotting="{every}" />\n'
     xml = xml.replace("    <collision-model", tag + "    <collision-model")
     out = {}
@@ -204,8 +285,14 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--skip-64", action="store_true")
     ap.add_argument("--only-vtk", action="store_true")
+    ap.add_argument("--only-geometry", action="store_true")
     args = ap.parse_args()
     subprocess.run(["make", "-s", "-f", os.path.join(ROOT, "oracle", "ref", "Makefile"), "-j8"], check=True)
+
+    for case in GEOMETRY_CASES:
+        save_geometry_fixture(*case)
+    if args.only_geometry:
+        return
 
     # VTK field output: cavity 8^3 every iteration; parabolic channel 10x7x9 every second one
     files = run_reference_vtk((8, 8, 8), 0.6, 0.05, 3, 1)

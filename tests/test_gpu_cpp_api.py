@@ -69,3 +69,109 @@ def test_cpp_runner_vtk_output_matches_reference(name, tmp_path):
     for k in d["plotted"]:
         assert_same_vtk(read_vtk(f"{base}_{int(k)}.vtk"), fixture_file(d, k), [str(h) for h in d["headers"]],
                         0.0 if exact else 2e-6, 0.0 if exact else 1e-15)
+
+
+# ---- builtin obstacles, mapping events and writes through the block store (test_peano_events)
+EVENTS = os.path.join(ROOT, "tests", "cpp", "test_peano_events")
+GEOMETRY = ["geometry_cavity_sphere_n12_t40.npz", "geometry_channel_16x8x8_t60.npz"]
+
+
+def _object_args(d):
+    args = []
+    for k, inv, p in zip(d["object_kind"], d["object_inverted"], d["object_params"]):
+        args += [str(int(k)), str(int(inv))] + [repr(float(v)) for v in p]
+    return args
+
+
+def _scenario(d):
+    if str(d["scenario"]) == "lid-driven-cavity":
+        return "cavity"
+    return "channel-parabolic" if int(d["parabolic"]) else "channel"
+
+
+def _run_events(d, mode, tmp_path, layout="two-buffer", steps=None):
+    if not os.path.exists(EVENTS):
+        pytest.fail(f"{EVENTS} is not built (run __graft_entry__.build())")
+    out = tmp_path / f"f_{mode.replace(':', '_')}_{layout}.bin"
+    steps = int(d["steps"]) if steps is None else steps
+    env = dict(os.environ, LBM_TEST_LAYOUT=layout)
+    r = subprocess.run([EVENTS, _scenario(d)] + [str(int(v)) for v in d["dims"]] +
+                       [repr(float(d["tau"])), repr(float(d["u"])), str(steps), str(out), mode] + _object_args(d),
+                       capture_output=True, text=True, timeout=120, env=env)
+    assert r.returncode == 0, (r.returncode, r.stdout, r.stderr)
+    raw = out.read_bytes()
+    n = int(np.prod(d["dims"]))
+    f = np.frombuffer(raw[:8 * 19 * n], "<f8")
+    o = 8 * 19 * n
+    s = np.frombuffer(raw[o:o + 12], np.int32)
+    m = int(np.prod(s))
+    inner = np.frombuffer(raw[o + 12:o + 12 + m], np.uint8).reshape(s[2], s[1], s[0])
+    boundary = np.frombuffer(raw[o + 12 + m:o + 12 + 2 * m], np.uint8).reshape(s[2], s[1], s[0])
+    return f, inner, boundary
+
+
+@pytest.mark.parametrize("layout", ["two-buffer", "in-place"])
+@pytest.mark.parametrize("mode", ["plain", "roundtrip"])
+@pytest.mark.parametrize("name", GEOMETRY)
+def test_cpp_runner_with_obstacles_matches_reference(name, mode, layout, tmp_path):
+    # builtin sphere / hexahedron obstacles through the C++ configuration: the block store's flags and
+    # the populations are the reference's bit for bit, with or without every mapping event firing
+    # and every block's pdf written back through GridManagementService
+    d = golden(name)
+    print(f"[gpu] C++ API obstacles {name} {mode} {layout}", flush=True)
+    f, inner, boundary = _run_events(d, mode, tmp_path, layout)
+    assert np.array_equal(inner, d["inner"]) and np.array_equal(boundary, d["boundary"])
+    assert np.array_equal(f, d["f"].reshape(-1)), np.abs(f - d["f"].reshape(-1)).max()
+
+
+def _oracle_for(d):
+    import oracle as O
+    nx, ny, nz = (int(v) for v in d["dims"])
+    solid = (d["inner"][3:3 + nz, 3:3 + ny, 3:3 + nx] == 0).astype(np.uint8)
+    if _scenario(d) == "cavity":
+        return O.Lattice.cavity(nx, float(d["tau"]), float(d["u"]), solid=solid), solid
+    return O.Lattice.channel(nx, ny, nz, float(d["tau"]), float(d["u"]), parabolic=bool(int(d["parabolic"])),
+                             solid=solid), solid
+
+
+@pytest.mark.parametrize("layout", ["two-buffer", "in-place"])
+def test_block_store_writes_reach_the_device(layout, tmp_path):
+    # touchVertexFirstTime of iteration T scales the populations of one cell through getPdf(); the
+    # oracle applies the same change to its state after T steps
+    import oracle as O
+    d = golden(GEOMETRY[0])
+    n, T, steps = 12, 7, 15
+    fluid = np.flatnonzero(d["inner"][3:15, 3:15, 3:15].reshape(-1))
+    cell = int(fluid[len(fluid) // 3])
+    print(f"[gpu] block-store write: cell {cell} at iteration {T}, {layout}", flush=True)
+    f, _, _ = _run_events(d, f"perturb:{T}:{cell}", tmp_path, layout, steps=steps)
+    lat, _ = _oracle_for(d)
+    fo = lat.step(lat.equilibrium(), T).reshape(O.Q, -1)
+    fo[:, cell] *= 1.0 + np.arange(O.Q) / 1000.0
+    fo = lat.step(fo.reshape(-1), steps - T)
+    assert np.array_equal(f, fo), np.abs(f - fo).max()
+    plain, _, _ = _run_events(d, "plain", tmp_path, layout, steps=steps)
+    assert not np.array_equal(f, plain)
+
+
+def test_block_store_inner_flag_write_reflags_the_lattice(tmp_path):
+    # a fluid cell next to the sphere made non-inner through getInner() at iteration T: from then on
+    # it is solid (populations dropped, its links bounce back), as in the oracle re-flagged at T
+    import oracle as O
+    d = golden(GEOMETRY[0])
+    n, T, steps = 12, 5, 12
+    lat, solid = _oracle_for(d)
+    fluid = np.flatnonzero(solid.reshape(-1) == 0)
+    cell = int(fluid[len(fluid) // 2])
+    print(f"[gpu] block-store re-flag: cell {cell} at iteration {T}", flush=True)
+    f, inner, _ = _run_events(d, f"solidify:{T}:{cell}", tmp_path, steps=steps)
+    fo = lat.step(lat.equilibrium(), T)
+    solid2 = solid.reshape(-1).copy()
+    solid2[cell] = 1
+    lat2 = O.Lattice.cavity(n, float(d["tau"]), float(d["u"]), solid=solid2)
+    fo = fo.reshape(O.Q, -1)
+    fo[:, cell] = 0.0
+    fo = lat2.step(fo.reshape(-1), steps - T)
+    assert np.array_equal(f, fo), np.abs(f - fo).max()
+    z, y, x = cell // (n * n), (cell // n) % n, cell % n
+    assert inner[z + 3, y + 3, x + 3] == 0
