@@ -212,6 +212,8 @@ typedef struct {
   int64_t reverse_steps;   /* adjoint steps run (= nsteps) */
   float forward_ms;        /* device time of the first forward sweep */
   float reverse_ms;        /* device time of the reverse sweep, recomputation included */
+  float recompute_ms;      /* device time of the forward steps re-run during the reverse sweep */
+  float adjoint_ms;        /* device time of the adjoint (reverse) steps themselves */
 } lbm_adjoint_stats;
 /* Runs nsteps forward steps from the current state, evaluates J = pressure drop (FP64) and
  * returns dJ/dalpha for every cell (0 on non-fluid cells), by a hand-written reverse sweep of

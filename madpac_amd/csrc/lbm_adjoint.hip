@@ -241,6 +241,48 @@ struct Revolve {
 
 using namespace lbm;
 
+namespace {
+// Device time of the reverse sweep's two phases: a pair of events brackets each recompute segment
+// and each adjoint step on the context's stream; after the sweep the pairs are summed per phase.
+// Events are created as the sweep needs them (a few per step) and destroyed with the timer.
+struct PhaseTimer {
+  enum Phase { RECOMPUTE = 0, ADJOINT = 1 };
+  std::vector<hipEvent_t> ev;
+  std::vector<int> phase;
+  size_t used = 0;
+  hipError_t init(size_t pairs) {
+    ev.reserve(2 * pairs);
+    phase.reserve(pairs);
+    return hipSuccess;
+  }
+  hipError_t begin(hipStream_t s, Phase p) {
+    while (ev.size() < 2 * used + 2) {
+      hipEvent_t e = nullptr;
+      const hipError_t r = hipEventCreate(&e);
+      if (r != hipSuccess) return r;
+      ev.push_back(e);
+    }
+    if (phase.size() <= used) phase.resize(used + 1);
+    phase[used] = p;
+    return hipEventRecord(ev[2 * used], s);
+  }
+  hipError_t end(hipStream_t s) { return hipEventRecord(ev[2 * used++ + 1], s); }
+  hipError_t sum(float out[2]) {
+    for (size_t k = 0; k < used; k++) {
+      float ms = 0.f;
+      const hipError_t r = hipEventElapsedTime(&ms, ev[2 * k], ev[2 * k + 1]);
+      if (r != hipSuccess) return r;
+      out[phase[k]] += ms;
+    }
+    return hipSuccess;
+  }
+  ~PhaseTimer() {
+    for (hipEvent_t e : ev)
+      if (e) (void)hipEventDestroy(e);
+  }
+};
+}  // namespace
+
 void lbm_adjoint_release(lbm_ctx* c) {
   if (!c || !c->adj) return;
   AdjointState* a = c->adj;
@@ -428,9 +470,12 @@ static int adjoint_two_buffer(lbm_ctx* c, int nsteps, int snapshots, int xin, in
   ACHK(hipGetLastError());
   ACHK(hipMemsetAsync(a->galpha, 0, (size_t)N * sizeof(double), st));
 
-  // reverse sweep
+  // reverse sweep.  Each recompute segment and each adjoint step is bracketed by a pair of events
+  // from a pool, so the two phases are timed separately on the device (summed after the sweep).
   int64_t recompute = 0, reversed = 0;
   int rc_err = 0;
+  PhaseTimer timer;
+  ACHK(timer.init(4 * (size_t)nsteps + 8));
   ACHK(hipEventRecord(c->ev0, st));
   // Checkpoints are read where they lie: the state s_{t0+j} is the slot itself for j = 0, else the
   // first recomputed step reads the slot and writes the context's ping-pong buffers.
@@ -442,16 +487,23 @@ static int adjoint_two_buffer(lbm_ctx* c, int nsteps, int snapshots, int xin, in
       *out = s0;
       return 0;
     }
+    hipError_t e = timer.begin(st, PhaseTimer::RECOMPUTE);
+    if (e != hipSuccess) return adj_fail(c, LBM_ERR_HIP, std::string("recompute timing: ") + hipGetErrorString(e));
     int r = lbm_enqueue_step_from(c, s0, c->buf[c->cur]);
     if (!r) r = lbm_enqueue_steps(c, j - 1);
+    e = timer.end(st);
+    if (!r && e != hipSuccess) return adj_fail(c, LBM_ERR_HIP, std::string("recompute timing: ") + hipGetErrorString(e));
     *out = c->buf[c->cur];
     return r;
   };
   auto adj_step = [&](const void* sp) -> int {
-    hipError_t e = c->fp32 ? adj_step_t<float>(c->g, sp, a->lam[lc], a->lam[lc ^ 1], c->words, c->alpha,
-                                               a->galpha, c->force, c->walls, c->p64, c->d_bct, st)
-                           : adj_step_t<double>(c->g, sp, a->lam[lc], a->lam[lc ^ 1], c->words, c->alpha,
-                                                a->galpha, c->force, c->walls, c->p64, c->d_bct, st);
+    hipError_t e = timer.begin(st, PhaseTimer::ADJOINT);
+    if (e == hipSuccess)
+      e = c->fp32 ? adj_step_t<float>(c->g, sp, a->lam[lc], a->lam[lc ^ 1], c->words, c->alpha, a->galpha, c->force,
+                                       c->walls, c->p64, c->d_bct, st)
+                  : adj_step_t<double>(c->g, sp, a->lam[lc], a->lam[lc ^ 1], c->words, c->alpha, a->galpha, c->force,
+                                        c->walls, c->p64, c->d_bct, st);
+    if (e == hipSuccess) e = timer.end(st);
     if (e != hipSuccess) return adj_fail(c, LBM_ERR_HIP, std::string("adjoint step: ") + hipGetErrorString(e));
     lc ^= 1;
     reversed++;
@@ -491,10 +543,18 @@ static int adjoint_two_buffer(lbm_ctx* c, int nsteps, int snapshots, int xin, in
       if (it == held.end()) return adj_fail(c, LBM_ERR_STATE, "checkpoint schedule lost a state");
       int r;
       if (m == 1) {
-        r = lbm_enqueue_step_from(c, a->slots[it->second], a->slots[slot]);
+        hipError_t e = timer.begin(st, PhaseTimer::RECOMPUTE);
+        r = e == hipSuccess ? lbm_enqueue_step_from(c, a->slots[it->second], a->slots[slot])
+                            : adj_fail(c, LBM_ERR_HIP, "recompute timing failed");
+        if (!r && timer.end(st) != hipSuccess) r = adj_fail(c, LBM_ERR_HIP, "recompute timing failed");
       } else {
         r = state_from(t0, m - 1, &sp);
-        if (!r) r = lbm_enqueue_step_from(c, sp, a->slots[slot]);
+        if (!r) {
+          hipError_t e = timer.begin(st, PhaseTimer::RECOMPUTE);
+          r = e == hipSuccess ? lbm_enqueue_step_from(c, sp, a->slots[slot])
+                              : adj_fail(c, LBM_ERR_HIP, "recompute timing failed");
+          if (!r && timer.end(st) != hipSuccess) r = adj_fail(c, LBM_ERR_HIP, "recompute timing failed");
+        }
       }
       if (r) return r;
       recompute += m;
@@ -515,6 +575,8 @@ static int adjoint_two_buffer(lbm_ctx* c, int nsteps, int snapshots, int xin, in
   float rev_ms = 0.f;
   ACHK(hipEventSynchronize(c->ev1));
   ACHK(hipEventElapsedTime(&rev_ms, c->ev0, c->ev1));
+  float phase_ms[2] = {0.f, 0.f};
+  ACHK(timer.sum(phase_ms));
 
   std::vector<double> gstored((size_t)N);
   ACHK(hipMemcpyAsync(gstored.data(), a->galpha, (size_t)N * sizeof(double), hipMemcpyDeviceToHost, st));
@@ -535,6 +597,8 @@ static int adjoint_two_buffer(lbm_ctx* c, int nsteps, int snapshots, int xin, in
     stats->reverse_steps = reversed;
     stats->forward_ms = fwd_ms;
     stats->reverse_ms = rev_ms;
+    stats->recompute_ms = phase_ms[PhaseTimer::RECOMPUTE];
+    stats->adjoint_ms = phase_ms[PhaseTimer::ADJOINT];
   }
   for (int64_t i = 0; i < g.Nu; i++)
     if (!std::isfinite(grad_alpha[i])) return adj_fail(c, LBM_ERR_NUMERIC, "non-finite gradient");

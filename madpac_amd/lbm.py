@@ -64,7 +64,8 @@ class Config(ctypes.Structure):
 class AdjointStats(ctypes.Structure):
     _fields_ = [("snapshots_used", ctypes.c_int), ("forward_steps", ctypes.c_int64),
                 ("recompute_steps", ctypes.c_int64), ("reverse_steps", ctypes.c_int64),
-                ("forward_ms", ctypes.c_float), ("reverse_ms", ctypes.c_float)]
+                ("forward_ms", ctypes.c_float), ("reverse_ms", ctypes.c_float),
+                ("recompute_ms", ctypes.c_float), ("adjoint_ms", ctypes.c_float)]
 
 
 class Patch(ctypes.Structure):

@@ -131,3 +131,20 @@ def test_adjoint_gradient_is_the_same_for_every_workgroup_shape():
     for J, grad in out[1:]:
         assert J == out[0][0]
         assert np.array_equal(grad, out[0][1])
+
+
+def test_adjoint_stats_time_both_reverse_phases():
+    # the reverse sweep's recompute segments and adjoint steps are timed separately with events
+    nx, ny, nz, steps, tau, g = 32, 16, 16, 40, 0.7, (1e-4, 0.0, 0.0)
+    solid, alpha = _channel(nx, ny, nz, 9)
+    with lbm.Lattice(nx, ny, nz, tau, periodic=(True, False, False), force=g) as lat:
+        lat.set_flags(~solid)
+        lat.set_porosity(alpha)
+        lat.init_equilibrium()
+        _, _, st = lat.adjoint_pressure_drop(steps, 3, 4, 20)
+    print(f"[gpu] adjoint stats: forward {st.forward_ms:.3f} ms, reverse {st.reverse_ms:.3f} ms "
+          f"(recompute {st.recompute_ms:.3f}, adjoint {st.adjoint_ms:.3f})", flush=True)
+    assert st.reverse_steps == steps and st.recompute_steps > 0
+    assert st.adjoint_ms > 0.0 and st.recompute_ms > 0.0
+    # both phases lie inside the reverse sweep's own bracket
+    assert st.adjoint_ms + st.recompute_ms <= st.reverse_ms * 1.001 + 1e-3
