@@ -111,6 +111,15 @@ def lib():
         L.lbo_pressure_drop_grad.argtypes = [ctypes.POINTER(Dims), up, ctypes.c_int, ctypes.c_int, dp]
         L.lbo_step_adjoint.argtypes = [ctypes.POINTER(Dims), up, ctypes.POINTER(Params), dp, dp, dp, dp,
                                        ctypes.c_int]
+        ip = ctypes.POINTER(ctypes.c_int)
+        u8 = ctypes.POINTER(ctypes.c_uint8)
+        L.lbo_amr_father.argtypes = [ip, ip]
+        L.lbo_amr_subblock.argtypes = [ip, ip]
+        L.lbo_amr_from_neighbour.restype = ctypes.c_int
+        L.lbo_amr_from_neighbour.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.lbo_amr_find_overlap.argtypes = [dp, u8]
+        L.lbo_amr_prolongate.argtypes = [ip, dp, dp, dp, dp, u8]
+        L.lbo_amr_restrict.argtypes = [ip, dp, dp]
         _lib = L
     return _lib
 
@@ -280,3 +289,66 @@ class Lattice:
 
 def tau_eff(tau):
     return lib().lbo_tau_eff(float(tau))
+
+
+# ---- restriction / prolongation between grid levels on 6^3 blocks (AoS pdf[19 * cell + q]) ----
+BLOCK_CELLS = 216
+
+
+def _pos(pos):
+    a = (ctypes.c_int * 3)(*[int(v) for v in pos])
+    return ctypes.cast(a, ctypes.POINTER(ctypes.c_int)), a
+
+
+def _u8p(a):
+    assert a.dtype == np.uint8 and a.flags.c_contiguous
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))
+
+
+def amr_father(pos):
+    p, _keep = _pos(pos)
+    out = (ctypes.c_int * 3)()
+    lib().lbo_amr_father(p, ctypes.cast(out, ctypes.POINTER(ctypes.c_int)))
+    return tuple(out)
+
+
+def amr_subblock(pos):
+    p, _keep = _pos(pos)
+    out = (ctypes.c_int * 3)()
+    lib().lbo_amr_subblock(p, ctypes.cast(out, ctypes.POINTER(ctypes.c_int)))
+    return tuple(out)
+
+
+def amr_from_neighbour(nb, cell, q):
+    return bool(lib().lbo_amr_from_neighbour(int(nb), int(cell), int(q)))
+
+
+def amr_find_overlap(fine):
+    fine = np.ascontiguousarray(fine, dtype=np.float64).reshape(-1)
+    assert fine.size == BLOCK_CELLS * Q
+    out = np.zeros(BLOCK_CELLS, np.uint8)
+    lib().lbo_amr_find_overlap(_dp(fine), _u8p(out))
+    return out.astype(bool)
+
+
+def amr_prolongate(pos, coarse, fine, density, velocity, overlap):
+    """Returns (fine, density, velocity) after the prolongation (inputs are not modified)."""
+    p, _keep = _pos(pos)
+    coarse = np.ascontiguousarray(coarse, dtype=np.float64).reshape(-1)
+    fine = np.array(fine, dtype=np.float64).reshape(-1)
+    density = np.array(density, dtype=np.float64).reshape(-1)
+    velocity = np.array(velocity, dtype=np.float64).reshape(-1)
+    ov = np.ascontiguousarray(np.asarray(overlap).reshape(-1) != 0, dtype=np.uint8)
+    assert coarse.size == fine.size == BLOCK_CELLS * Q and density.size == BLOCK_CELLS and velocity.size == 3 * BLOCK_CELLS
+    lib().lbo_amr_prolongate(p, _dp(coarse), _dp(fine), _dp(density), _dp(velocity), _u8p(ov))
+    return fine, density, velocity
+
+
+def amr_restrict(pos, coarse, fine):
+    """Returns the coarse block after the restriction (inputs are not modified)."""
+    p, _keep = _pos(pos)
+    coarse = np.array(coarse, dtype=np.float64).reshape(-1)
+    fine = np.ascontiguousarray(fine, dtype=np.float64).reshape(-1)
+    assert coarse.size == fine.size == BLOCK_CELLS * Q
+    lib().lbo_amr_restrict(p, _dp(coarse), _dp(fine))
+    return coarse

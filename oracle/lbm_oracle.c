@@ -633,3 +633,125 @@ void lbo_step_adjoint(const lbo_dims* d, const uint32_t* words, const lbo_params
       }
     }
 }
+
+/* ---- restriction and prolongation between grid levels (BLB/BlockRestrictionProlongation.cpp) ----
+ * Blocks are the reference's 6^3 vertex blocks, populations AoS: pdf[19 * cell + q] with
+ * cell = x + 6 y + 36 z.  A refined coarse cell holds 4^3 fine vertices at child positions
+ * pos in {0..3}^3; the fine block at pos overlaps the 2^3 coarse cells of one sub-block of
+ * its father vertex's block, each coarse cell covering 3^3 fine cells. */
+#define LBO_BS 6
+#define LBO_BC (LBO_BS * LBO_BS * LBO_BS)
+
+/* getFatherVertex (:17-28): pos / 2 per axis. */
+void lbo_amr_father(const int pos[3], int out[3]) {
+  for (int d = 0; d < 3; d++) out[d] = pos[d] / 2;
+}
+
+/* getSubBlock with the table of createSubBlockVector (:264-288): positions 0..3 -> 1, 2, 0, 1. */
+void lbo_amr_subblock(const int pos[3], int out[3]) {
+  static const int t[4] = {1, 2, 0, 1};
+  for (int d = 0; d < 3; d++) out[d] = t[pos[d]];
+}
+
+/* Whether entry (cell, q) of a block is filled from neighbour block nb (nb = bx + 3 by + 9 bz,
+ * 0..26 without 13) by the three streaming steps of createMinusOneEntriesFromNeighbourBlock
+ * (:293-449): the reference marks the neighbour of a 3^3-block domain with -1, streams three
+ * times (populations leaving the domain are lost, empty slots become 0) and lists the centre
+ * entries that hold -1.  Streaming moves population q by c_q per step, so the entry holds -1
+ * exactly when its position 6 + x - 3 c_q lies inside the neighbour along every axis. */
+int lbo_amr_from_neighbour(int nb, int cell, int q) {
+  const int b[3] = {nb % 3, (nb / 3) % 3, nb / 9};
+  const int x[3] = {cell % LBO_BS, (cell / LBO_BS) % LBO_BS, cell / (LBO_BS * LBO_BS)};
+  for (int d = 0; d < 3; d++) {
+    const int s = LBO_BS + x[d] - 3 * LBO_C[q][d];
+    if (s < LBO_BS * b[d] || s >= LBO_BS * (b[d] + 1)) return 0;
+  }
+  return 1;
+}
+
+/* findFineOverlapCells (:32-62) with neighbourIsTemporaryVertex (:66-79): a neighbour is a
+ * temporary vertex if any entry it fills holds exactly -1.0; then every cell it fills is an
+ * overlap cell. */
+void lbo_amr_find_overlap(const double* fine, uint8_t* overlap) {
+  for (int c = 0; c < LBO_BC; c++) overlap[c] = 0;
+  for (int nb = 0; nb < 27; nb++) {
+    if (nb == 13) continue;
+    int temporary = 0;
+    for (int c = 0; c < LBO_BC && !temporary; c++)
+      for (int q = 0; q < LBO_Q; q++)
+        if (lbo_amr_from_neighbour(nb, c, q) && fine[LBO_Q * c + q] == -1.0) {
+          temporary = 1;
+          break;
+        }
+    if (!temporary) continue;
+    for (int c = 0; c < LBO_BC; c++)
+      for (int q = 0; q < LBO_Q; q++)
+        if (lbo_amr_from_neighbour(nb, c, q)) overlap[c] = 1;
+  }
+}
+
+/* prolongateToFineLevel (:83-160): every overlap cell of the fine block takes the populations of
+ * the coarse cell above it; its density is their sum (q order) and its velocity the momentum
+ * (accumulated per axis in q order) divided by the density. */
+void lbo_amr_prolongate(const int pos[3], const double* coarse, double* fine, double* density, double* velocity,
+                        const uint8_t* overlap) {
+  int sb[3];
+  lbo_amr_subblock(pos, sb);
+  const int h = LBO_BS / 3;
+  for (int z = 0; z < h; z++)
+    for (int y = 0; y < h; y++)
+      for (int x = 0; x < h; x++) {
+        const int cc = sb[0] * h + x + (sb[1] * h + y) * LBO_BS + (sb[2] * h + z) * LBO_BS * LBO_BS;
+        for (int zz = 0; zz < 3; zz++)
+          for (int yy = 0; yy < 3; yy++)
+            for (int xx = 0; xx < 3; xx++) {
+              const int fc = x * 3 + xx + (y * 3 + yy) * LBO_BS + (z * 3 + zz) * LBO_BS * LBO_BS;
+              if (!overlap[fc]) continue;
+              double rho = 0.0, m[3] = {0.0, 0.0, 0.0};
+              for (int q = 0; q < LBO_Q; q++) {
+                const double v = coarse[LBO_Q * cc + q];
+                fine[LBO_Q * fc + q] = v;
+                rho += v;
+                for (int d = 0; d < 3; d++) m[d] += v * cd(q, d);
+              }
+              density[fc] = rho;
+              for (int d = 0; d < 3; d++) velocity[3 * fc + d] = m[d] / rho;
+            }
+      }
+}
+
+/* restrictToCoarseLevel (:164-259): each coarse cell of the sub-block takes, per population, the
+ * mean (sum in z, y, x order times 1/27) of its 3^3 fine cells, unless one of them holds the
+ * marker -1.0 for that population; the selection is the reference's arithmetic blend
+ * b * coarse + (1 - b) * mean with b = 1.0 or 0.0. */
+void lbo_amr_restrict(const int pos[3], double* coarse, const double* fine) {
+  int sb[3];
+  lbo_amr_subblock(pos, sb);
+  const int h = LBO_BS / 3;
+  for (int z = 0; z < h; z++)
+    for (int y = 0; y < h; y++)
+      for (int x = 0; x < h; x++) {
+        const int cc = sb[0] * h + x + (sb[1] * h + y) * LBO_BS + (sb[2] * h + z) * LBO_BS * LBO_BS;
+        double r[LBO_Q];
+        int marked[LBO_Q];
+        for (int q = 0; q < LBO_Q; q++) {
+          r[q] = 0.0;
+          marked[q] = 0;
+        }
+        for (int zz = 0; zz < 3; zz++)
+          for (int yy = 0; yy < 3; yy++)
+            for (int xx = 0; xx < 3; xx++) {
+              const int fc = x * 3 + xx + (y * 3 + yy) * LBO_BS + (z * 3 + zz) * LBO_BS * LBO_BS;
+              for (int q = 0; q < LBO_Q; q++) {
+                const double v = fine[LBO_Q * fc + q];
+                marked[q] = marked[q] || v == -1.0;
+                r[q] += v;
+              }
+            }
+        for (int q = 0; q < LBO_Q; q++) {
+          const double mean = (1.0 / 27.0) * r[q];
+          const double b = (double)marked[q];
+          coarse[LBO_Q * cc + q] = b * coarse[LBO_Q * cc + q] + (double)(1 - marked[q]) * mean;
+        }
+      }
+}

@@ -145,6 +145,16 @@ void lbo_step_adjoint(const lbo_dims* d, const uint32_t* words, const lbo_params
 /* dJ/df of the pressure drop */
 void lbo_pressure_drop_grad(const lbo_dims* d, const uint32_t* words, int xin, int xout, double* lam);
 
+/* Restriction and prolongation between grid levels on 6^3 blocks (AoS pdf[19 * cell + q]),
+ * BLB/BlockRestrictionProlongation.cpp; pos is the fine vertex's child position in {0..3}^3. */
+void lbo_amr_father(const int pos[3], int out[3]);
+void lbo_amr_subblock(const int pos[3], int out[3]);
+int lbo_amr_from_neighbour(int nb, int cell, int q);
+void lbo_amr_find_overlap(const double* fine, uint8_t* overlap);
+void lbo_amr_prolongate(const int pos[3], const double* coarse, double* fine, double* density, double* velocity,
+                        const uint8_t* overlap);
+void lbo_amr_restrict(const int pos[3], double* coarse, const double* fine);
+
 #ifdef __cplusplus
 }
 #endif

@@ -558,3 +558,124 @@ def test_pressure_drop_gradient_is_the_objective_linear_map():
     mask[:, :, [1, 6]] = True
     assert np.all(lam[:, ~mask] == 0.0)
     assert np.all(lam[:, solid.astype(bool)] == 0.0)
+
+
+# ---- restriction / prolongation between grid levels (BLB/BlockRestrictionProlongation.cpp) ----
+import hashlib  # noqa: E402
+
+import amr_inputs  # noqa: E402
+
+AMR_SEEDS = (7, 8, 9)
+
+
+def _amr_fixture(seed):
+    return np.load(os.path.join(GOLDEN, f"amr_s{seed}.npz"), allow_pickle=False)
+
+
+def test_amr_overlap_table_equals_three_streaming_steps():
+    """The closed form of the overlap map (entry (cell, q) of the centre block holds the neighbour's
+    value iff 6 + x - 3 c_q lies in the neighbour) against the procedure the reference runs
+    (createMinusOneEntriesFromNeighbourBlock, :293-449): mark one neighbour of a 3^3-block domain
+    with -1, stream three times with populations leaving the domain lost, read the centre block."""
+    L = 18
+    for nb in range(27):
+        if nb == 13:
+            continue
+        b = (nb % 3, (nb // 3) % 3, nb // 9)
+        dom = np.zeros((O.Q, L, L, L))  # [q, z, y, x]
+        dom[:, 6 * b[2]:6 * b[2] + 6, 6 * b[1]:6 * b[1] + 6, 6 * b[0]:6 * b[0] + 6] = -1.0
+        for _ in range(3):
+            new = np.zeros_like(dom)
+            for q in range(O.Q):
+                cx, cy, cz = O.C[q]
+                src = dom[q, max(0, -cz):L - max(0, cz), max(0, -cy):L - max(0, cy), max(0, -cx):L - max(0, cx)]
+                new[q, max(0, cz):L - max(0, -cz), max(0, cy):L - max(0, -cy), max(0, cx):L - max(0, -cx)] = src
+            dom = new
+        centre = dom[:, 6:12, 6:12, 6:12].reshape(O.Q, 216)
+        for cell in range(216):
+            for q in range(O.Q):
+                assert O.amr_from_neighbour(nb, cell, q) == (centre[q, cell] == -1.0), (nb, cell, q)
+
+
+def test_amr_father_and_subblock_match_reference():
+    d = _amr_fixture(7)
+    for p, pos in enumerate(amr_inputs.positions()):
+        assert O.amr_father(pos) == tuple(d["father"][p])
+        assert O.amr_subblock(pos) == tuple(d["subblock"][p])
+
+
+@pytest.mark.parametrize("seed", AMR_SEEDS)
+def test_amr_oracle_bitwise_equals_reference(seed):
+    d = _amr_fixture(seed)
+    cases = amr_inputs.inputs(seed, float(d["pmark"]))
+    for p, c in enumerate(cases):
+        ov = O.amr_find_overlap(c["fine"])
+        assert np.array_equal(ov, d["overlap"][p]), p
+        fine, dens, vel = O.amr_prolongate(c["pos"], c["coarse"], c["fine"], c["density"], c["velocity"], ov)
+        coarse = O.amr_restrict(c["pos"], c["coarse"], c["fine"])
+        assert hashlib.sha256(fine.tobytes() + dens.tobytes() + vel.tobytes()).hexdigest() == \
+            str(d["prolongation_sha256"][p]), p
+        assert hashlib.sha256(coarse.tobytes()).hexdigest() == str(d["restriction_sha256"][p]), p
+        if p == int(d["full_position"]):
+            assert np.array_equal(fine, d["full_fine"]) and np.array_equal(coarse, d["full_coarse"])
+            assert np.array_equal(dens, d["full_density"]) and np.array_equal(vel, d["full_velocity"])
+    # sparse markers exercise both outcomes of the overlap search (dense ones make every cell overlap)
+    if seed == 7:
+        assert d["overlap"].any() and not d["overlap"].all()
+
+
+def test_amr_prolongation_reference_unit_test():
+    """BLB/tests/BlockRestrictionProlongationTest.cpp:41-146: coarse populations 1, fine -1, density 1,
+    velocity 0.05, overlap on the cells x, y, z < 3, child position (1, 1, 1): overlap cells get the
+    coarse values (density 19, velocity 0), all other cells keep theirs."""
+    coarse = np.ones(216 * O.Q)
+    fine = -np.ones(216 * O.Q)
+    ov = np.zeros(216, bool)
+    for z in range(3):
+        for y in range(3):
+            for x in range(3):
+                ov[x + 6 * y + 36 * z] = True
+    f, r, u = O.amr_prolongate((1, 1, 1), coarse, fine, np.ones(216), np.full(648, 0.05), ov)
+    f = f.reshape(216, O.Q)
+    u = u.reshape(216, 3)
+    for cell in range(216):
+        if ov[cell]:
+            assert r[cell] == 19.0 and not u[cell].any() and (f[cell] == 1.0).all()
+        else:
+            assert r[cell] == 1.0 and (u[cell] == 0.05).all() and (f[cell] == -1.0).all()
+
+
+def test_amr_restriction_is_the_mean_and_keeps_marked_populations():
+    """Restriction of a fine block whose 3^3 children of every coarse cell average to a known value
+    (a linear ramp across the children), with one marked population per coarse cell kept."""
+    rng = np.random.default_rng(3)
+    pos = (2, 1, 3)
+    sb = O.amr_subblock(pos)
+    coarse = rng.uniform(0.5, 1.5, 216 * O.Q)
+    fine = np.empty((216, O.Q))
+    for z in range(6):
+        for y in range(6):
+            for x in range(6):
+                off = ((x % 3) - 1) * 0.25 + ((y % 3) - 1) * 0.125 + ((z % 3) - 1) * 0.0625
+                fine[x + 6 * y + 36 * z] = 1.0 + off  # children of a coarse cell average to 1
+    marked_q = {}
+    for cz in range(2):
+        for cy in range(2):
+            for cx in range(2):
+                q = int(rng.integers(0, O.Q))
+                fine[3 * cx + 1 + 6 * (3 * cy + 2) + 36 * (3 * cz)][q] = -1.0
+                marked_q[(cx, cy, cz)] = q
+    out = O.amr_restrict(pos, coarse, fine.reshape(-1)).reshape(216, O.Q)
+    cin = coarse.reshape(216, O.Q)
+    for c in range(216):
+        x, y, z = c % 6, (c // 6) % 6, c // 36
+        inside = all(2 * s <= v < 2 * s + 2 for v, s in zip((x, y, z), sb))
+        if not inside:
+            assert np.array_equal(out[c], cin[c])
+            continue
+        q0 = marked_q[(x - 2 * sb[0], y - 2 * sb[1], z - 2 * sb[2])]
+        for q in range(O.Q):
+            if q == q0:
+                assert out[c, q] == cin[c, q]
+            else:
+                assert abs(out[c, q] - 1.0) < 1e-15
