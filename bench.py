@@ -58,6 +58,17 @@ def run_dense(lbm, solid, n, tau, prec, steps, warmup, g, layout=0, readout=Fals
     ms = lat.step_timed(steps)  # HIP events on the context stream around exactly `steps` launches
     lat.synchronize()
     wall = time.perf_counter() - t0
+    ceil = None
+    if layout == 0:
+        # the step's bandwidth references on the same buffers, grid and workgroup mapping: a 19-plane
+        # copy and the pull pattern over every stored cell, no flag word, no arithmetic
+        cells = n ** 3
+        esize_ = 8 if prec == lbm.FP64 else 4
+        ceil = {}
+        for key, pull in (("plane_copy", False), ("pull_copy", True)):
+            lat.measure_copy(2, pull=pull)
+            cms = lat.measure_copy(steps, pull=pull) / steps
+            ceil[key] = {"ms": round(cms, 4), "GBs": round(cells * 38 * esize_ / (cms * 1e-3) / 1e9, 1)}
     m = lat.moments()
     host = None
     if readout:  # what a caller pays to bring results to the host (PCIe), outside the timed steps
@@ -92,6 +103,7 @@ def run_dense(lbm, solid, n, tau, prec, steps, warmup, g, layout=0, readout=Fals
         "workgroup_rows": wg_rows,
         "cell_order": order,
         "host_readout": host,
+        "copy_reference": ceil,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "algorithmic_bytes_per_launch": bytes_per_launch},
@@ -204,6 +216,48 @@ def run_patches(lbm, wl, prec, steps=500):
                          "ghost_exchange_GBs": round(ghost_vals * 2 * esize / per / 1e9, 1)}}
 
 
+def run_block_store(lbm, ncoarse=4096, reps=5):
+    """Restriction and prolongation between two grid levels on the device block store: every coarse
+    block fathers 8 fine vertices (the positions of one octant of its refined cell), so one call per
+    operation covers 8 * ncoarse links.  Overlap search on fine blocks with sparse -1.0 markers;
+    prolongation with every cell an overlap cell (new vertices: the most data it moves); restriction
+    of all sub-blocks.  Device time of each launch from HIP events around it."""
+    rng = np.random.default_rng(5)
+    nc, q = 216, 19
+    nfine = 8 * ncoarse
+    links = []
+    for cb in range(ncoarse):
+        f = (cb % 2, (cb // 2) % 2, (cb // 4) % 2)
+        for k in range(8):
+            links.append((cb, ncoarse + 8 * cb + k, tuple(2 * f[d] + ((k >> d) & 1) for d in range(3))))
+    out = {"workload": f"{ncoarse} coarse and {nfine} fine 6^3 blocks, {len(links)} links per call, FP64"}
+    with lbm.BlockStore(ncoarse + nfine) as bs:
+        bs.set(0, pdf=rng.uniform(0.5, 1.5, (ncoarse, nc * q)))
+        fine = rng.uniform(0.5, 1.5, (nfine, nc * q))
+        fine[rng.random(fine.shape) < 1e-3] = -1.0
+        bs.set(ncoarse, pdf=fine, density=np.ones((nfine, nc)), velocity=np.zeros((nfine, 3 * nc)))
+        del fine
+        per_link = {"find_overlap": nc * q * 8 + nc,
+                    "prolongate": nc + 8 * q * 8 + nc * (q + 4) * 8,
+                    "restrict": nc * q * 8 + 2 * 8 * q * 8}
+        for name in ("find_overlap", "prolongate", "restrict"):
+            ms = []
+            for _ in range(reps):
+                if name == "find_overlap":
+                    bs.find_overlap(links)
+                elif name == "prolongate":
+                    bs.set_overlap(ncoarse, np.ones(nfine * nc, np.uint8))
+                    bs.prolongate(links)
+                else:
+                    bs.restrict(links)
+                ms.append(bs.last_ms())
+            t = float(np.mean(ms[1:]))
+            gbs = len(links) * per_link[name] / (t * 1e-3) / 1e9
+            out[name] = {"ms": round(t, 4), "algorithmic_bytes_per_link": per_link[name], "GBs": round(gbs, 1),
+                         "roofline_frac": round(gbs / HBM_PEAK_GBS, 4)}
+    return out
+
+
 def cpu_baseline(wl, n=128, seed=None, tau=0.6, g=1e-6, budget_s=12.0):
     import oracle
 
@@ -266,6 +320,9 @@ def main():
         res[name] = run_dense(lbm, solid, n, args.tau, prec, args.steps, args.warmup, 1e-6, readout=name == "fp64")
         log(f"[bench] {name}: {res[name]['mlups']:.0f} MLUPS, {res[name]['ms_per_step']:.3f} ms/step, "
             f"{res[name]['roofline']['achieved']:.0f} GB/s ({100 * res[name]['roofline']['frac']:.1f} % of peak)")
+        cr = res[name]["copy_reference"]
+        log(f"[bench] {name} references at {n}^3: 19-plane copy {cr['plane_copy']['GBs']:.0f} GB/s, "
+            f"pull copy {cr['pull_copy']['GBs']:.0f} GB/s")
     for name, prec in (("fp64_aa", lbm.FP64), ("fp32s_aa", lbm.FP32_SHIFTED)):
         res[name] = run_dense(lbm, solid, n, args.tau, prec, args.steps, args.warmup, 1e-6, layout=lbm.LAYOUT_AA)
         log(f"[bench] {name}: {res[name]['mlups']:.0f} MLUPS, {res[name]['ms_per_step']:.3f} ms/step, "
@@ -279,11 +336,19 @@ def main():
     shard = run_shard_share(lbm, solid, args.tau, 1e-6)
     log(f"[bench] slab share {n}x{n}x34: fp64 {shard['fp64']['value']:.0f} MLUPS, "
         f"fp32s {shard['fp32s']['value']:.0f} MLUPS")
-    tgv = None
+    tgv = tgv_same = None
     if not args.no_all_fluid:
         tgv = run_taylor_green(lbm, wl)
         log(f"[bench] Taylor-Green 512^3: fp64 {tgv['fp64']['value']:.0f} MLUPS, "
             f"fp32s {tgv['fp32s']['value']:.0f} MLUPS")
+        # the same kernels on an all-fluid lattice of the porous channel's size: separates the lattice
+        # size from the line waste at the solid surfaces
+        tgv_same = run_taylor_green(lbm, wl, n=n, steps=args.steps)
+        log(f"[bench] Taylor-Green {n}^3: fp64 {tgv_same['fp64']['value']:.0f} MLUPS, "
+            f"fp32s {tgv_same['fp32s']['value']:.0f} MLUPS")
+    amr = run_block_store(lbm)
+    log(f"[bench] block store {amr['workload']}: " + ", ".join(
+        f"{k} {amr[k]['ms']:.3f} ms ({amr[k]['GBs']:.0f} GB/s)" for k in ("find_overlap", "prolongate", "restrict")))
     adj = None
     if not args.no_adjoint:
         adj = run_adjoint(lbm, wl)
@@ -322,8 +387,13 @@ def main():
         "host_readout_fp64": r64["host_readout"],
         "batched_patches": patches,
         "all_fluid": tgv,
+        "all_fluid_same_size": tgv_same,
+        "copy_reference": {"fp64": r64["copy_reference"], "fp32s": r32["copy_reference"],
+                           "what": f"19-plane copy and pull-pattern copy of all {n}^3 cells on the step's buffers "
+                                   f"and workgroup mapping (lbm_measure_copy), GB/s = cells x 38 x sizeof(T) / time"},
         "slab_share": shard,
         "adjoint": adj,
+        "restriction_prolongation": amr,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -177,6 +177,13 @@ int lbm_step(lbm_ctx* ctx, int nsteps);
 int lbm_step_graph(lbm_ctx* ctx, int nsteps);
 /* Same, timed with HIP events on the context stream; *ms = elapsed device time. */
 int lbm_step_timed(lbm_ctx* ctx, int nsteps, float* ms);
+/* Bandwidth reference of the step, no reference counterpart (measurement): nsteps launches of a
+ * kernel with the step's grid, workgroup mapping and non-temporal stores that moves all 19 values
+ * of every stored cell (solid ones too) from the current state buffer into the other one, with no
+ * flag word and no arithmetic: mode 0 the cell's own slots (a 19-plane copy), mode 1 the slots the
+ * step pulls from (the all-fluid step's access pattern).  The state is left unchanged; *ms is the
+ * device time of all launches (HIP events).  Two-buffer contexts only. */
+int lbm_measure_copy(lbm_ctx* ctx, int mode, int nsteps, float* ms);
 /* Times the step kernel's variants for nsteps steps each on a copy of the current state (the state
  * is restored) and keeps the fastest; *variant (may be NULL) receives it.  Bit 0, the load
  * schedule: 0 = flag word first, then the 19 pulled loads (solid cells load nothing); 1 = the 19
@@ -242,6 +249,14 @@ int lbm_patches_create(lbm_patchset** ps, const lbm_patch* table, int npatches, 
 int lbm_patches_destroy(lbm_patchset* ps);
 const char* lbm_patches_last_error(const lbm_patchset* ps);
 int64_t lbm_patches_cells(const lbm_patchset* ps);
+/* Fluid flags of the interior cells (patch order, x fastest; nonzero = fluid), the per-block
+ * inner bitset of the reference's block store (BLB/services/GridManagementService.h:74-80) with its
+ * boundary links discovered across patch faces and edges as ReceiveBoundaryDataService::
+ * getBoundaryData does per block (SLB/services/ReceiveBoundaryDataService.cpp:107-185): every
+ * fluid-to-solid link is a halfway bounce-back link.  NULL makes every cell fluid again.  The state
+ * must be initialised (lbm_patches_init) after the flags change; solid cells read back as 0 and are
+ * left out of the moments. */
+int lbm_patches_set_flags(lbm_patchset* ps, const uint8_t* fluid);
 /* initial state: f = feq(rho, u) with per-cell rho/u, interior cells of all patches in
  * patch order, x fastest (sum n^3 entries; u has 3 per cell); NULL = rest. */
 int lbm_patches_init(lbm_patchset* ps, const double* rho, const double* u);
@@ -257,6 +272,49 @@ int lbm_patches_set_tiling(lbm_patchset* ps, const lbm_tiling* t);
  * captured once into a hipGraph and replayed. */
 int lbm_patches_step(lbm_patchset* ps, int nsteps, int use_graph, float* ms);
 int lbm_patches_reduce_moments(lbm_patchset* ps, double out[4]);
+
+/* ---- block store: restriction and prolongation between grid levels ----------------------
+ * Replaces BLB/BlockRestrictionProlongation.{h,cpp} as the spacetree solver mapping calls it
+ * (BLB/mappings/SpacetreeGrid2RegularBlockSolver.cpp:160-240, :340-366), batched: a device-resident
+ * store of 6^3 vertex blocks in the reference's layout (GridManagementService::VertexData,
+ * BLB/services/GridManagementService.h:74-86: pdf[19 * cell + q], density[216], velocity[648],
+ * cell = x + 6 y + 36 z) and one launch per list of links.  A link names the father vertex's block
+ * (coarse), the fine vertex's block (fine) and the fine vertex's child position in the refined
+ * coarse cell (pos, each 0..3: getFatherVertex is pos / 2, the sub-block getSubBlock maps 0..3 to
+ * 1, 2, 0, 1).  In one call a block is either fine or coarse (one level pair per call), the two
+ * blocks of a link differ, fine blocks are unique (overlap search, prolongation) and coarse
+ * sub-blocks are unique (restriction); violations return LBM_ERR_ARG before anything runs. */
+typedef struct lbm_blockstore lbm_blockstore;
+typedef struct {
+  int64_t coarse;  /* block of the father vertex */
+  int64_t fine;    /* block of the fine vertex */
+  int pos[3];      /* child position of the fine vertex, each 0..3 */
+  int new_vertex;  /* overlap search: the fine vertex is new, all populations become -1.0 and all
+                      cells overlap cells (SpacetreeGrid2RegularBlockSolver.cpp:186-193) */
+} lbm_amr_link;
+int lbm_blocks_create(lbm_blockstore** bs, int64_t nblocks, int device);
+int lbm_blocks_destroy(lbm_blockstore* bs);
+const char* lbm_blocks_last_error(const lbm_blockstore* bs);
+/* copy blocks first .. first + count - 1 in (set) or out (get); any pointer may be NULL */
+int lbm_blocks_set(lbm_blockstore* bs, int64_t first, int64_t count, const double* pdf,
+                   const double* density, const double* velocity);
+int lbm_blocks_get(lbm_blockstore* bs, int64_t first, int64_t count, double* pdf, double* density,
+                   double* velocity);
+/* findFineOverlapCells (BlockRestrictionProlongation.cpp:32-79) for the fine block of every link;
+ * the overlap cells stay with the block for lbm_blocks_prolongate and, if overlap != NULL, are
+ * copied out (216 bytes per link, 1 = overlap cell). */
+int lbm_blocks_find_overlap(lbm_blockstore* bs, const lbm_amr_link* links, int64_t n, uint8_t* overlap);
+/* overlap cells of blocks first .. first + count - 1 given by the caller (216 flags per block,
+ * nonzero = overlap cell), as the reference's prolongateToFineLevel takes them as an argument */
+int lbm_blocks_set_overlap(lbm_blockstore* bs, int64_t first, int64_t count, const uint8_t* overlap);
+/* prolongateToFineLevel (:83-160) on the overlap cells each fine block holds (from the last
+ * lbm_blocks_find_overlap or lbm_blocks_set_overlap on it) */
+int lbm_blocks_prolongate(lbm_blockstore* bs, const lbm_amr_link* links, int64_t n);
+/* restrictToCoarseLevel (:164-259) */
+int lbm_blocks_restrict(lbm_blockstore* bs, const lbm_amr_link* links, int64_t n);
+/* device time of the last find/prolongate/restrict launch in ms (HIP events); no reference
+ * counterpart (measurement) */
+float lbm_blocks_last_ms(const lbm_blockstore* bs);
 
 #ifdef __cplusplus
 }

@@ -360,7 +360,14 @@ extern "C" int lbm_adjoint_pressure_drop(lbm_ctx* c, int nsteps, int snapshots, 
     lbm_adjoint_release(c);
     lbm::DenseGeom natural = c->g;
     natural.layout = lbm::LAYOUT_AA_NATURAL;
-    const int rc2 = restore_layout(c, lbm_state_geom(c), c->buf[c->cur], natural, c->buf[0]);
+    // the relayout needs distinct buffers: a final state left in buf[0] moves to buf[1] first
+    int rc2 = LBM_OK;
+    if (c->cur == 0) {
+      hipError_t e = hipMemcpyAsync(c->buf[1], c->buf[0], pbytes, hipMemcpyDeviceToDevice, c->stream);
+      if (e != hipSuccess) rc2 = adj_fail(c, LBM_ERR_HIP, std::string("final state copy: ") + hipGetErrorString(e));
+      c->cur = 1;
+    }
+    if (!rc2) rc2 = restore_layout(c, lbm_state_geom(c), c->buf[1], natural, c->buf[0]);
     c->aa = true;
     c->cur = 0;
     c->parity = 0;

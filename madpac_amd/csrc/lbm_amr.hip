@@ -1,0 +1,380 @@
+// Restriction and prolongation between grid levels on a device-resident block store.
+//
+// Reference counterpart: BLB/BlockRestrictionProlongation.cpp, called per fine vertex from the
+// spacetree solver mapping (BLB/mappings/SpacetreeGrid2RegularBlockSolver.cpp:160-240 in
+// touchVertexFirstTime: overlap search, then prolongation after the collision; :340-366 in
+// touchVertexLastTime: restriction).  A block is one Peano vertex's 6^3 cells in the reference's
+// layout (populations AoS, pdf[19 * cell + q], cell = x + 6 y + 36 z; density[216]; velocity[648]),
+// so blocks move between this store and GridManagementService-style host code unchanged.
+//
+// A refined coarse cell has 4^3 fine vertices at child positions pos in {0..3}^3.  The fine block at
+// pos overlaps one sub-block (2^3 coarse cells) of its father vertex's block, each coarse cell
+// covering 3^3 fine cells.  One call handles a batch of (coarse block, fine block, pos) links, one
+// workgroup per link, so thousands of vertices of one level pair go in one launch:
+//   * overlap search: the fine block's -1.0 markers (populations streamed in from temporary
+//     vertices) name the neighbours that are temporary; each cell such a neighbour streams into is
+//     an overlap cell.  The map from entries to neighbours is arithmetic: entry (cell, q) is filled
+//     by the neighbour block holding position 6 + x - 3 c_q of the 3^3-block domain;
+//   * prolongation: overlap cells take the coarse cell's populations, density and velocity;
+//   * restriction: each coarse cell of the sub-block takes the mean of its 27 children per
+//     population, unless a child carries the marker there.
+// All three are HBM-bound byte movers (no arithmetic worth MFMA): the work per link is one fine
+// block (32.8 KB of populations) plus one coarse sub-block.
+#include <algorithm>
+#include <set>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "lbm_capi.h"
+#include "lbm_cell.h"
+#include "lbm_internal.h"
+
+namespace lbm {
+
+constexpr int BS = 6;                  // block edge
+constexpr int BC = BS * BS * BS;       // cells per block
+constexpr int BP = BC * Q;             // populations per block
+constexpr int BV = BC * 3;             // velocity entries per block
+
+struct AmrLinkDev {
+  int64_t coarse, fine;
+  int sx, sy, sz;    // sub-block of the coarse block (0..2 per axis)
+  int new_vertex;
+};
+
+// neighbour block (0..26, 13 = the block itself) whose populations fill entry (cell, q) after three
+// streaming steps: the one holding position 6 + x - 3 c_q per axis
+__device__ __forceinline__ int source_neighbour(int x, int y, int z, int q) {
+  const int bx = (BS + x - 3 * CX(q)) / BS, by = (BS + y - 3 * CY(q)) / BS, bz = (BS + z - 3 * CZ(q)) / BS;
+  return bx + 3 * by + 9 * bz;
+}
+
+// findFineOverlapCells (:32-79) for the fine block of each link; a new vertex is all markers and
+// all overlap cells (SpacetreeGrid2RegularBlockSolver.cpp:186-193)
+__global__ __launch_bounds__(256) void k_amr_overlap(const AmrLinkDev* __restrict__ links, int64_t nlinks,
+                                                     double* __restrict__ pdf, uint8_t* __restrict__ mask) {
+  const int64_t l = blockIdx.x;
+  if (l >= nlinks) return;
+  const AmrLinkDev L = links[l];
+  double* f = pdf + L.fine * BP;
+  uint8_t* m = mask + L.fine * BC;
+  const int c = threadIdx.x;
+  if (L.new_vertex) {
+    for (int k = c; k < BP; k += 256) f[k] = -1.0;
+    if (c < BC) m[c] = 1;
+    return;
+  }
+  __shared__ unsigned temporary;
+  if (c == 0) temporary = 0u;
+  __syncthreads();
+  const int x = c % BS, y = (c / BS) % BS, z = c / (BS * BS);
+  unsigned mine = 0u;  // neighbours this cell's entries come from
+  if (c < BC) {
+    unsigned marked = 0u;
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+      const int nb = source_neighbour(x, y, z, q);
+      if (nb == 13) continue;
+      mine |= 1u << nb;
+      if (f[c * Q + q] == -1.0) marked |= 1u << nb;
+    }
+    if (marked) atomicOr(&temporary, marked);
+  }
+  __syncthreads();
+  if (c < BC) m[c] = (mine & temporary) ? 1 : 0;
+}
+
+// prolongateToFineLevel (:83-160): one thread per fine cell
+__global__ __launch_bounds__(256) void k_amr_prolongate(const AmrLinkDev* __restrict__ links, int64_t nlinks,
+                                                        double* __restrict__ pdf, double* __restrict__ density,
+                                                        double* __restrict__ velocity, const uint8_t* __restrict__ mask) {
+  const int64_t l = blockIdx.x;
+  if (l >= nlinks) return;
+  const AmrLinkDev L = links[l];
+  const int fc = threadIdx.x;
+  if (fc >= BC || !mask[L.fine * BC + fc]) return;
+  const int x = fc % BS, y = (fc / BS) % BS, z = fc / (BS * BS);
+  const int h = BS / 3;
+  const int cc = (L.sx * h + x / 3) + (L.sy * h + y / 3) * BS + (L.sz * h + z / 3) * BS * BS;
+  const double* cf = pdf + L.coarse * BP + cc * Q;
+  double* ff = pdf + L.fine * BP + fc * Q;
+  double rho = 0.0, m0 = 0.0, m1 = 0.0, m2 = 0.0;
+#pragma unroll
+  for (int q = 0; q < Q; q++) {
+    const double v = cf[q];
+    ff[q] = v;
+    rho += v;
+    // the reference adds v * c_q per axis with c_q as a double (zero products included)
+    m0 += v * (double)CX(q);
+    m1 += v * (double)CY(q);
+    m2 += v * (double)CZ(q);
+  }
+  density[L.fine * BC + fc] = rho;
+  double* u = velocity + L.fine * BV + 3 * fc;
+  u[0] = m0 / rho;
+  u[1] = m1 / rho;
+  u[2] = m2 / rho;
+}
+
+// restrictToCoarseLevel (:164-259): one thread per (coarse cell of the sub-block, population)
+__global__ __launch_bounds__(256) void k_amr_restrict(const AmrLinkDev* __restrict__ links, int64_t nlinks,
+                                                      double* __restrict__ pdf) {
+  const int64_t l = blockIdx.x;
+  if (l >= nlinks) return;
+  const AmrLinkDev L = links[l];
+  const int t = threadIdx.x;
+  if (t >= 8 * Q) return;
+  const int q = t % Q, k = t / Q;
+  const int x = k & 1, y = (k >> 1) & 1, z = k >> 2;
+  const int h = BS / 3;
+  const int cc = (L.sx * h + x) + (L.sy * h + y) * BS + (L.sz * h + z) * BS * BS;
+  const double* ff = pdf + L.fine * BP;
+  double r = 0.0;
+  bool marked = false;
+  for (int zz = 0; zz < 3; zz++)
+    for (int yy = 0; yy < 3; yy++)
+      for (int xx = 0; xx < 3; xx++) {
+        const int fc = x * 3 + xx + (y * 3 + yy) * BS + (z * 3 + zz) * BS * BS;
+        const double v = ff[fc * Q + q];
+        marked = marked || v == -1.0;
+        r += v;
+      }
+  double* cf = pdf + L.coarse * BP + cc * Q + q;
+  const double mean = (1.0 / 27.0) * r;
+  const double b = marked ? 1.0 : 0.0;
+  // the reference's blend b * coarse + (1 - b) * mean, b = 1 or 0 (:246-248)
+  *cf = b * *cf + (1.0 - b) * mean;
+}
+
+}  // namespace lbm
+
+using namespace lbm;
+
+struct lbm_blockstore {
+  int device = 0;
+  int64_t nblocks = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  std::string err;
+  double* pdf = nullptr;       // nblocks * 4104
+  double* density = nullptr;   // nblocks * 216
+  double* velocity = nullptr;  // nblocks * 648
+  uint8_t* mask = nullptr;     // nblocks * 216 overlap cells of the last search
+  AmrLinkDev* d_links = nullptr;
+  int64_t cap_links = 0;
+  float last_ms = 0.f;
+};
+
+namespace {
+thread_local std::string g_blocks_create_error;
+
+int bfail(lbm_blockstore* bs, int code, const std::string& m) {
+  if (bs) bs->err = m;
+  return code;
+}
+
+#define BCHK(bs, call)                                                                                         \
+  do {                                                                                                         \
+    hipError_t e_ = (call);                                                                                    \
+    if (e_ != hipSuccess) return bfail(bs, LBM_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+enum Op { OVERLAP, PROLONGATE, RESTRICT };
+
+// Validates the links of one call and uploads them.  Within a call no block may be both a fine and
+// a coarse block (the levels of one call are one coarse/fine pair), a link's blocks differ, fine
+// blocks are unique (overlap search, prolongation) and coarse sub-blocks are unique (restriction),
+// so no two workgroups write the same values.
+int upload_links(lbm_blockstore* bs, const lbm_amr_link* links, int64_t n, Op op) {
+  if (n < 0 || (n > 0 && !links)) return bfail(bs, LBM_ERR_ARG, "link list is null");
+  if (n > (int64_t)1 << 31) return bfail(bs, LBM_ERR_ARG, "too many links in one call");
+  std::vector<AmrLinkDev> h((size_t)n);
+  std::set<int64_t> fines, coarses;
+  std::set<std::pair<int64_t, int>> subs;
+  static const int sub_of[4] = {1, 2, 0, 1};  // createSubBlockVector (:280-288)
+  for (int64_t k = 0; k < n; k++) {
+    const lbm_amr_link& L = links[k];
+    if (L.coarse < 0 || L.coarse >= bs->nblocks || L.fine < 0 || L.fine >= bs->nblocks)
+      return bfail(bs, LBM_ERR_ARG, "link " + std::to_string(k) + ": block id out of range");
+    if (L.coarse == L.fine) return bfail(bs, LBM_ERR_ARG, "link " + std::to_string(k) + ": coarse and fine block are the same");
+    for (int d = 0; d < 3; d++)
+      if (L.pos[d] < 0 || L.pos[d] > 3) return bfail(bs, LBM_ERR_ARG, "link " + std::to_string(k) + ": child position must be in 0..3");
+    AmrLinkDev& D = h[(size_t)k];
+    D.coarse = L.coarse;
+    D.fine = L.fine;
+    D.sx = sub_of[L.pos[0]];
+    D.sy = sub_of[L.pos[1]];
+    D.sz = sub_of[L.pos[2]];
+    D.new_vertex = L.new_vertex ? 1 : 0;
+    fines.insert(L.fine);
+    coarses.insert(L.coarse);
+    if (op == RESTRICT && !subs.insert({L.coarse, D.sx + 3 * D.sy + 9 * D.sz}).second)
+      return bfail(bs, LBM_ERR_ARG, "link " + std::to_string(k) + ": two links restrict into the same coarse sub-block");
+  }
+  if (op != RESTRICT && (int64_t)fines.size() != n)
+    return bfail(bs, LBM_ERR_ARG, "a fine block appears in two links");
+  for (int64_t b : fines)
+    if (coarses.count(b)) return bfail(bs, LBM_ERR_ARG, "a block is fine in one link and coarse in another (one level pair per call)");
+  if (n > bs->cap_links) {
+    if (bs->d_links) (void)hipFree(bs->d_links);
+    bs->d_links = nullptr;
+    bs->cap_links = 0;
+    if (lbm::device_alloc(&bs->d_links, (size_t)n * sizeof(AmrLinkDev)) != hipSuccess)
+      return bfail(bs, LBM_ERR_NOMEM, "link table allocation failed");
+    bs->cap_links = n;
+  }
+  if (n) BCHK(bs, hipMemcpyAsync(bs->d_links, h.data(), (size_t)n * sizeof(AmrLinkDev), hipMemcpyHostToDevice, bs->stream));
+  return LBM_OK;
+}
+
+int run_links(lbm_blockstore* bs, int64_t n, Op op) {
+  BCHK(bs, hipEventRecord(bs->ev0, bs->stream));
+  if (n) {
+    const dim3 grid((unsigned)n), block(256);
+    if (op == OVERLAP)
+      hipLaunchKernelGGL(k_amr_overlap, grid, block, 0, bs->stream, bs->d_links, n, bs->pdf, bs->mask);
+    else if (op == PROLONGATE)
+      hipLaunchKernelGGL(k_amr_prolongate, grid, block, 0, bs->stream, bs->d_links, n, bs->pdf, bs->density,
+                         bs->velocity, bs->mask);
+    else
+      hipLaunchKernelGGL(k_amr_restrict, grid, block, 0, bs->stream, bs->d_links, n, bs->pdf);
+    BCHK(bs, hipGetLastError());
+  }
+  BCHK(bs, hipEventRecord(bs->ev1, bs->stream));
+  BCHK(bs, hipEventSynchronize(bs->ev1));
+  BCHK(bs, hipEventElapsedTime(&bs->last_ms, bs->ev0, bs->ev1));
+  return LBM_OK;
+}
+
+int check_range(lbm_blockstore* bs, int64_t first, int64_t count) {
+  if (first < 0 || count < 0 || first > bs->nblocks || count > bs->nblocks - first)
+    return bfail(bs, LBM_ERR_ARG, "block range out of bounds");
+  return LBM_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int lbm_blocks_create(lbm_blockstore** out, int64_t nblocks, int device) {
+  if (!out) return LBM_ERR_ARG;
+  *out = nullptr;
+  lbm_blockstore* bs = new lbm_blockstore();
+  auto bail = [&](int code, const std::string& m) {
+    g_blocks_create_error = m;
+    lbm_blocks_destroy(bs);
+    return code;
+  };
+  if (nblocks < 1) return bail(LBM_ERR_ARG, "nblocks must be >= 1");
+  bs->device = device;
+  bs->nblocks = nblocks;
+  if (hipSetDevice(device) != hipSuccess) return bail(LBM_ERR_HIP, "hipSetDevice failed");
+  if (hipStreamCreateWithFlags(&bs->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&bs->ev0) != hipSuccess || hipEventCreate(&bs->ev1) != hipSuccess)
+    return bail(LBM_ERR_HIP, "stream/event creation failed");
+  const size_t nb = (size_t)nblocks;
+  if (lbm::device_alloc(&bs->pdf, nb * BP * sizeof(double)) != hipSuccess ||
+      lbm::device_alloc(&bs->density, nb * BC * sizeof(double)) != hipSuccess ||
+      lbm::device_alloc(&bs->velocity, nb * BV * sizeof(double)) != hipSuccess ||
+      lbm::device_alloc(&bs->mask, nb * BC) != hipSuccess)
+    return bail(LBM_ERR_NOMEM, "block store allocation failed");
+  if (hipMemsetAsync(bs->pdf, 0, nb * BP * sizeof(double), bs->stream) != hipSuccess ||
+      hipMemsetAsync(bs->density, 0, nb * BC * sizeof(double), bs->stream) != hipSuccess ||
+      hipMemsetAsync(bs->velocity, 0, nb * BV * sizeof(double), bs->stream) != hipSuccess ||
+      hipMemsetAsync(bs->mask, 0, nb * BC, bs->stream) != hipSuccess || hipStreamSynchronize(bs->stream) != hipSuccess)
+    return bail(LBM_ERR_HIP, "block store clear failed");
+  *out = bs;
+  return LBM_OK;
+}
+
+int lbm_blocks_destroy(lbm_blockstore* bs) {
+  if (!bs) return LBM_OK;
+  if (bs->stream) (void)hipStreamSynchronize(bs->stream);
+  void* ptrs[] = {bs->pdf, bs->density, bs->velocity, bs->mask, bs->d_links};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  if (bs->ev0) (void)hipEventDestroy(bs->ev0);
+  if (bs->ev1) (void)hipEventDestroy(bs->ev1);
+  if (bs->stream) (void)hipStreamDestroy(bs->stream);
+  delete bs;
+  return LBM_OK;
+}
+
+const char* lbm_blocks_last_error(const lbm_blockstore* bs) { return bs ? bs->err.c_str() : g_blocks_create_error.c_str(); }
+
+int lbm_blocks_set(lbm_blockstore* bs, int64_t first, int64_t count, const double* pdf, const double* density,
+                   const double* velocity) {
+  if (!bs) return LBM_ERR_ARG;
+  if (int rc = check_range(bs, first, count)) return rc;
+  BCHK(bs, hipSetDevice(bs->device));
+  if (pdf)
+    BCHK(bs, hipMemcpyAsync(bs->pdf + first * BP, pdf, (size_t)count * BP * sizeof(double), hipMemcpyHostToDevice, bs->stream));
+  if (density)
+    BCHK(bs, hipMemcpyAsync(bs->density + first * BC, density, (size_t)count * BC * sizeof(double), hipMemcpyHostToDevice,
+                            bs->stream));
+  if (velocity)
+    BCHK(bs, hipMemcpyAsync(bs->velocity + first * BV, velocity, (size_t)count * BV * sizeof(double), hipMemcpyHostToDevice,
+                            bs->stream));
+  BCHK(bs, hipStreamSynchronize(bs->stream));
+  return LBM_OK;
+}
+
+int lbm_blocks_get(lbm_blockstore* bs, int64_t first, int64_t count, double* pdf, double* density, double* velocity) {
+  if (!bs) return LBM_ERR_ARG;
+  if (int rc = check_range(bs, first, count)) return rc;
+  BCHK(bs, hipSetDevice(bs->device));
+  if (pdf)
+    BCHK(bs, hipMemcpyAsync(pdf, bs->pdf + first * BP, (size_t)count * BP * sizeof(double), hipMemcpyDeviceToHost, bs->stream));
+  if (density)
+    BCHK(bs, hipMemcpyAsync(density, bs->density + first * BC, (size_t)count * BC * sizeof(double), hipMemcpyDeviceToHost,
+                            bs->stream));
+  if (velocity)
+    BCHK(bs, hipMemcpyAsync(velocity, bs->velocity + first * BV, (size_t)count * BV * sizeof(double), hipMemcpyDeviceToHost,
+                            bs->stream));
+  BCHK(bs, hipStreamSynchronize(bs->stream));
+  return LBM_OK;
+}
+
+int lbm_blocks_set_overlap(lbm_blockstore* bs, int64_t first, int64_t count, const uint8_t* overlap) {
+  if (!bs) return LBM_ERR_ARG;
+  if (!overlap) return bfail(bs, LBM_ERR_ARG, "overlap is null");
+  if (int rc = check_range(bs, first, count)) return rc;
+  BCHK(bs, hipSetDevice(bs->device));
+  std::vector<uint8_t> m((size_t)count * BC);
+  for (size_t k = 0; k < m.size(); k++) m[k] = overlap[k] ? 1 : 0;
+  BCHK(bs, hipMemcpyAsync(bs->mask + first * BC, m.data(), m.size(), hipMemcpyHostToDevice, bs->stream));
+  BCHK(bs, hipStreamSynchronize(bs->stream));
+  return LBM_OK;
+}
+
+int lbm_blocks_find_overlap(lbm_blockstore* bs, const lbm_amr_link* links, int64_t n, uint8_t* overlap) {
+  if (!bs) return LBM_ERR_ARG;
+  BCHK(bs, hipSetDevice(bs->device));
+  if (int rc = upload_links(bs, links, n, OVERLAP)) return rc;
+  if (int rc = run_links(bs, n, OVERLAP)) return rc;
+  if (overlap && n) {
+    std::vector<uint8_t> all((size_t)bs->nblocks * BC);
+    BCHK(bs, hipMemcpyAsync(all.data(), bs->mask, all.size(), hipMemcpyDeviceToHost, bs->stream));
+    BCHK(bs, hipStreamSynchronize(bs->stream));
+    for (int64_t k = 0; k < n; k++) std::copy_n(all.data() + links[k].fine * BC, BC, overlap + k * BC);
+  }
+  return LBM_OK;
+}
+
+int lbm_blocks_prolongate(lbm_blockstore* bs, const lbm_amr_link* links, int64_t n) {
+  if (!bs) return LBM_ERR_ARG;
+  BCHK(bs, hipSetDevice(bs->device));
+  if (int rc = upload_links(bs, links, n, PROLONGATE)) return rc;
+  return run_links(bs, n, PROLONGATE);
+}
+
+int lbm_blocks_restrict(lbm_blockstore* bs, const lbm_amr_link* links, int64_t n) {
+  if (!bs) return LBM_ERR_ARG;
+  BCHK(bs, hipSetDevice(bs->device));
+  if (int rc = upload_links(bs, links, n, RESTRICT)) return rc;
+  return run_links(bs, n, RESTRICT);
+}
+
+float lbm_blocks_last_ms(const lbm_blockstore* bs) { return bs ? bs->last_ms : -1.f; }
+
+}  // extern "C"

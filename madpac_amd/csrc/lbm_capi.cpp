@@ -612,6 +612,26 @@ int lbm_step_timed(lbm_ctx* c, int nsteps, float* ms) {
   return LBM_OK;
 }
 
+int lbm_measure_copy(lbm_ctx* c, int mode, int nsteps, float* ms) {
+  if (!c) return LBM_ERR_ARG;
+  if (!ms) return fail(c, LBM_ERR_ARG, "ms is null");
+  if (nsteps < 1) return fail(c, LBM_ERR_ARG, "nsteps must be >= 1");
+  if (mode != 0 && mode != 1) return fail(c, LBM_ERR_ARG, "mode must be 0 (copy) or 1 (pull)");
+  if (c->aa) return fail(c, LBM_ERR_STATE, "the copy reference needs the two-buffer layout");
+  int rc = need_state(c);
+  if (rc) return rc;
+  HIPCHK(c, hipSetDevice(c->cfg.device));
+  // into the buffer the next step overwrites: the state in buf[cur] is left as it is
+  const void* src = c->buf[c->cur];
+  void* dst = c->buf[c->cur ^ 1];
+  HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+  for (int k = 0; k < nsteps; k++) HIPCHK(c, lbm::launch_dense_copy(c->g, c->fp32, mode == 1, src, dst, c->stream));
+  HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+  HIPCHK(c, hipEventSynchronize(c->ev1));
+  HIPCHK(c, hipEventElapsedTime(ms, c->ev0, c->ev1));
+  return LBM_OK;
+}
+
 // Tuning variant: bit 0 the load schedule, bit 1 the workgroup order, bits 2-3 the workgroup
 // shape (0: the default, 1/2/3: 2/4/8 rows or bricks).
 static const int kShapeRows[4] = {0, 2, 4, 8};

@@ -277,6 +277,29 @@ __global__ __launch_bounds__(BX * BY_MAX) void k_dense_unpull(DenseGeom g, const
   }
 }
 
+// Bandwidth reference of the step (measurement, not part of a time step): every stored cell of the
+// lattice (fluid or not) moves its 19 values from src to dst with the step's grid, workgroup mapping
+// and non-temporal stores, and no flag word or arithmetic.  PULL = false: each cell copies its own
+// slots (the 19-plane copy); PULL = true: it reads the slots the step pulls from (x - c_q, wrapped on
+// periodic axes, clamped otherwise), i.e. the all-fluid step's memory pattern.
+template <typename T, bool PULL>
+__global__ __launch_bounds__(BX * BY_MAX) void k_dense_copy(DenseGeom g, const T* __restrict__ src, T* __restrict__ dst) {
+  const CellPos c = map_cell(g);
+  if (!c.valid) return;
+  const int64_t N = g.N;
+  T f[Q];
+  if (PULL) {
+    const NbrOff o = nbr_offsets(g, c);
+#pragma unroll
+    for (int q = 0; q < Q; q++) f[q] = src[(int64_t)q * N + c.i + src_off(q, o)];
+  } else {
+#pragma unroll
+    for (int q = 0; q < Q; q++) f[q] = src[(int64_t)q * N + c.i];
+  }
+#pragma unroll
+  for (int q = 0; q < Q; q++) __builtin_nontemporal_store(f[q], &dst[(int64_t)q * N + c.i]);
+}
+
 // The same state in another layout of the same cell order (e.g. pull form <-> AA NATURAL): each
 // fluid cell copies its 19 values of the reference state from where `from` keeps them to where `to`
 // does.  src and dst are distinct buffers; the stored values are copied as they are (exact in both
@@ -636,6 +659,23 @@ hipError_t launch_dense_unpull(const DenseGeom& g, bool fp32, const double* f, v
                                hipStream_t s) {
   if (fp32) hipLaunchKernelGGL(k_dense_unpull<float>, dense_grid(g), dense_block(g), 0, s, g, f, (float*)dst, words);
   else hipLaunchKernelGGL(k_dense_unpull<double>, dense_grid(g), dense_block(g), 0, s, g, f, (double*)dst, words);
+  return hipGetLastError();
+}
+
+hipError_t launch_dense_copy(const DenseGeom& g, bool fp32, bool pull, const void* src, void* dst, hipStream_t s) {
+  if (fp32) {
+    if (pull)
+      hipLaunchKernelGGL((k_dense_copy<float, true>), dense_grid(g), dense_block(g), 0, s, g, (const float*)src, (float*)dst);
+    else
+      hipLaunchKernelGGL((k_dense_copy<float, false>), dense_grid(g), dense_block(g), 0, s, g, (const float*)src, (float*)dst);
+  } else {
+    if (pull)
+      hipLaunchKernelGGL((k_dense_copy<double, true>), dense_grid(g), dense_block(g), 0, s, g, (const double*)src,
+                         (double*)dst);
+    else
+      hipLaunchKernelGGL((k_dense_copy<double, false>), dense_grid(g), dense_block(g), 0, s, g, (const double*)src,
+                         (double*)dst);
+  }
   return hipGetLastError();
 }
 

@@ -121,6 +121,9 @@ hipError_t launch_dense_pull(const DenseGeom& g, bool fp32, const void* src, dou
 // stored state from a reference state f (FP64)
 hipError_t launch_dense_unpull(const DenseGeom& g, bool fp32, const double* f, void* dst, const uint32_t* words,
                                hipStream_t stream);
+// bandwidth reference of the step: the 19-plane copy (pull = false) or the pull pattern (pull = true)
+// of every stored cell from src to dst, no arithmetic
+hipError_t launch_dense_copy(const DenseGeom& g, bool fp32, bool pull, const void* src, void* dst, hipStream_t stream);
 // the stored state of `from` re-stored in the layout of `to` (same cell order and plane stride) into
 // another buffer, values copied exactly, no staging memory
 hipError_t launch_state_relayout(const DenseGeom& from, const DenseGeom& to, bool fp32, const void* src, void* dst,

@@ -36,6 +36,7 @@ struct PatchDev {
   double omega;      // 1 / tau_eff of the patch
   int nbr[Q];        // neighbour patch in direction c_q
   int64_t nbase[Q];  // that neighbour's base (one load fewer on the ghost path)
+  int64_t cbase;     // offset of the patch's first interior cell in per-cell arrays (flag words, user arrays)
 };
 
 // Everything one tile iteration needs, so that a workgroup fetches a single record per tile (and
@@ -166,15 +167,30 @@ __device__ __forceinline__ int pull_slot(int q, const PatchLane& L) {
 // the source at s = (x, y, z) - n e, so the record is indexed by the cell's own other coordinates:
 // the write goes to slot_offset(q, n, x - n e_x, y - n e_y, z - n e_z) of that neighbour.  The
 // neighbours' bases are tile-uniform loads from the patch record.
-template <typename T, bool M32>
-__device__ __forceinline__ void patch_cell(const PatchDev* __restrict__ pp, const T* __restrict__ s,
-                                           T* __restrict__ d, T* __restrict__ dst, int n, double omega, int x, int y,
-                                           int z) {
+// FL: the patch set carries fluid flags (the reference block's inner bitset plus its boundary links,
+// BLB/services/GridManagementService.h:74-80, SLB/services/ReceiveBoundaryDataService.cpp:107-185),
+// one word per interior cell as in the dense lattice: bit 31 fluid, bit q set when x + c_q is not
+// fluid.  A solid cell does nothing.  Across a bounce-back link (x - c_q not fluid) population q is
+// the cell's own outgoing population 18 - q of the previous step (halfway bounce-back,
+// BLB/boundaryconditions/BlockHalfwayBounceBack.cpp:12-27), and no ghost record is written toward a
+// solid cell (no fluid cell pulls from it).
+template <typename T, bool M32, bool FL>
+__device__ __forceinline__ void patch_cell(const PatchDev* __restrict__ pp, const uint32_t* __restrict__ words,
+                                           const T* __restrict__ s, T* __restrict__ d, T* __restrict__ dst, int n,
+                                           double omega, int x, int y, int z) {
   T f[Q];
+  uint32_t w = 0u;
   {
     const PatchLane L = patch_lane(n, x, y, z);
+    if (FL) {
+      w = words[pp->cbase + L.i];
+      if (!(w & FLUID)) return;
+    }
 #pragma unroll
-    for (int q = 0; q < Q; q++) f[q] = s[pull_slot(q, L)];
+    for (int q = 0; q < Q; q++) {
+      const bool bounce = FL && ((w >> (Q - 1 - q)) & 1u);
+      f[q] = s[bounce ? (Q - 1 - q) * L.n3 + L.i : pull_slot(q, L)];
+    }
   }
   collide_patch<T, M32>(f, omega);
   // Rebuild the lane terms from opaque copies of the coordinates: kept live through the collision
@@ -191,6 +207,7 @@ __device__ __forceinline__ void patch_cell(const PatchDev* __restrict__ pp, cons
       const int cx = CX(q), cy = CY(q), cz = CZ(q);
       const bool ox = dst_outside(cx, L.lx, L.hx), oy = dst_outside(cy, L.ly, L.hy), oz = dst_outside(cz, L.lz, L.hz);
       if (!(ox | oy | oz)) continue;
+      if (FL && ((w >> q) & 1u)) continue;
       int64_t nb = 0;
       int off = 0;
       if (cx != 0 && ox) { nb = pp->nbase[dir_index(cx, 0, 0)]; off = G0 + (5 * (cx < 0) + face_local(q, 0)) * n2 + L.y + n * L.z; }
@@ -205,9 +222,10 @@ __device__ __forceinline__ void patch_cell(const PatchDev* __restrict__ pp, cons
 }
 
 // Box tiles of TX x TY x TZ cells (cells outside the patch idle).
-template <typename T, int TX, int TY, int TZ, bool M32>
+template <typename T, int TX, int TY, int TZ, bool M32, bool FL>
 __global__ __launch_bounds__(256) void k_patch_step(const PatchDev* __restrict__ patches,
                                                     const TileDev* __restrict__ tiles, int ntiles,
+                                                    const uint32_t* __restrict__ words,
                                                     const T* __restrict__ src, T* __restrict__ dst) {
   constexpr int TC = TX * TY * TZ;
   int t = blockIdx.x;
@@ -224,7 +242,7 @@ __global__ __launch_bounds__(256) void k_patch_step(const PatchDev* __restrict__
     for (int c = threadIdx.x; c < TC; c += 256) {
       const int x = tl.x0 + c % TX, y = tl.y0 + (c / TX) % TY, z = tl.z0 + c / (TX * TY);
       if (x >= n || y >= n || z >= n) continue;
-      patch_cell<T, M32>(pp, src + tl.base, dst + tl.base, dst, n, tl.omega, x, y, z);
+      patch_cell<T, M32, FL>(pp, words, src + tl.base, dst + tl.base, dst, n, tl.omega, x, y, z);
     }
     if (tn >= ntiles) break;
     t = tn;
@@ -249,9 +267,10 @@ __device__ __forceinline__ void split_cell(int c, int n_rt, int& x, int& y, int&
 template <typename T>
 constexpr int patch_waves_per_simd() { return sizeof(T) == 4 ? 8 : 4; }
 
-template <typename T, bool M32>
+template <typename T, bool M32, bool FL>
 __global__ __launch_bounds__(256, patch_waves_per_simd<T>()) void k_patch_step_lin(const PatchDev* __restrict__ patches,
                                                         const TileDev* __restrict__ tiles, int ntiles,
+                                                        const uint32_t* __restrict__ words,
                                                         const T* __restrict__ src, T* __restrict__ dst) {
   int t = blockIdx.x;
   if (t >= ntiles) return;
@@ -273,7 +292,7 @@ __global__ __launch_bounds__(256, patch_waves_per_simd<T>()) void k_patch_step_l
         case 32: split_cell<32>(c, n, x, y, z); break;
         default: split_cell<0>(c, n, x, y, z); break;
       }
-      patch_cell<T, M32>(pp, src + tl.base, dst + tl.base, dst, n, tl.omega, x, y, z);
+      patch_cell<T, M32, FL>(pp, words, src + tl.base, dst + tl.base, dst, n, tl.omega, x, y, z);
     }
     if (tn >= ntiles) break;
     t = tn;
@@ -281,12 +300,63 @@ __global__ __launch_bounds__(256, patch_waves_per_simd<T>()) void k_patch_step_l
   }
 }
 
-// initial state: f = feq(rho, u) per interior cell, written to the slot each population is pulled
-// from (own plane or own ghost record)
+// Flag words from per-cell fluid bytes (patch order, x fastest): bit 31 fluid, bit q set when the
+// neighbour x + c_q is not fluid.  A neighbour outside the patch lies in the patch across the crossed
+// face or edge (nbr[dir_index(e)], e = the components of c_q that leave the patch), at x + c_q - n e;
+// neighbouring patches share the edge n.
+__global__ __launch_bounds__(256) void k_patch_words(const PatchDev* __restrict__ patches, int npatch,
+                                                     const uint8_t* __restrict__ fluid, uint32_t* __restrict__ words) {
+  const int p = blockIdx.y;
+  if (p >= npatch) return;
+  const PatchDev& P = patches[p];
+  const int n = P.n;
+  const int64_t cells = (int64_t)n * n * n;
+  for (int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x; c < cells; c += (int64_t)gridDim.x * 256) {
+    const int x = (int)(c % n), y = (int)((c / n) % n), z = (int)(c / ((int64_t)n * n));
+    if (!fluid[P.cbase + c]) {
+      words[P.cbase + c] = 0u;
+      continue;
+    }
+    uint32_t w = FLUID;
+    for (int q = 0; q < Q; q++) {
+      if (q == REST) continue;
+      int sx = x + CX(q), sy = y + CY(q), sz = z + CZ(q);
+      const int ex = sx < 0 ? -1 : (sx >= n ? 1 : 0), ey = sy < 0 ? -1 : (sy >= n ? 1 : 0),
+                ez = sz < 0 ? -1 : (sz >= n ? 1 : 0);
+      int np = p;
+      if (ex | ey | ez) {
+        np = P.nbr[dir_index(ex, ey, ez)];
+        sx -= n * ex;
+        sy -= n * ey;
+        sz -= n * ez;
+      }
+      if (!fluid[patches[np].cbase + sx + (int64_t)n * (sy + (int64_t)n * sz)]) w |= 1u << q;
+    }
+    words[P.cbase + c] = w;
+  }
+}
+
+// Flag word of interior cell c of patch p: FLUID without bounce bits when the set has no flags.
+__device__ __forceinline__ uint32_t patch_word(const uint32_t* __restrict__ words, const int64_t* __restrict__ cell_base,
+                                               int p, int64_t c) {
+  return words ? words[cell_base[p] + c] : FLUID;
+}
+
+// Slot that holds population q of the reference state of interior cell c = (x, y, z): the cell's own
+// plane 18 - q across a bounce-back link, else the pull source (own plane or ghost record).
+__device__ __forceinline__ int64_t ref_slot(int q, int n, int x, int y, int z, int64_t c, uint32_t w) {
+  if ((w >> (Q - 1 - q)) & 1u) return (int64_t)(Q - 1 - q) * n * n * n + c;
+  return slot_offset(q, n, x - CX(q), y - CY(q), z - CZ(q));
+}
+
+// initial state: f = feq(rho, u) per fluid interior cell, written to the slot each population is
+// pulled from (own plane or own ghost record, or the own reflected plane across a bounce-back link);
+// solid cells write nothing (no fluid cell reads their slots)
 template <typename T>
 __global__ __launch_bounds__(256) void k_patch_init(const PatchDev* __restrict__ patches, int npatch,
-                                                    const int64_t* __restrict__ cell_base, const double* __restrict__ rho,
-                                                    const double* __restrict__ u, T* __restrict__ s) {
+                                                    const int64_t* __restrict__ cell_base, const uint32_t* __restrict__ words,
+                                                    const double* __restrict__ rho, const double* __restrict__ u,
+                                                    T* __restrict__ s) {
   const int p = blockIdx.y;
   if (p >= npatch) return;
   const int64_t base = patches[p].base;
@@ -294,6 +364,8 @@ __global__ __launch_bounds__(256) void k_patch_init(const PatchDev* __restrict__
   const int64_t cells = (int64_t)n * n * n;
   for (int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x; c < cells; c += (int64_t)gridDim.x * 256) {
     const int x = (int)(c % n), y = (int)((c / n) % n), z = (int)(c / ((int64_t)n * n));
+    const uint32_t w = patch_word(words, cell_base, p, c);
+    if (!(w & FLUID)) continue;
     const int64_t ui = cell_base[p] + c;
     const double r = rho ? rho[ui] : 1.0;
     const double u0 = u ? u[3 * ui] : 0.0, u1 = u ? u[3 * ui + 1] : 0.0, u2 = u ? u[3 * ui + 2] : 0.0;
@@ -306,16 +378,16 @@ __global__ __launch_bounds__(256) void k_patch_init(const PatchDev* __restrict__
       const double feq = W64(q) * r *
           (1.0 + cu / (1.0 / 3.0) + (cu * cu) / (2.0 * (1.0 / 3.0) * (1.0 / 3.0)) - (uu / (2.0 * (1.0 / 3.0))));
       const T v = (sizeof(T) == 8) ? (T)feq : (T)(feq - W64(q));
-      s[base + slot_offset(q, n, x - CX(q), y - CY(q), z - CZ(q))] = v;
+      s[base + ref_slot(q, n, x, y, z, c, w)] = v;
     }
   }
 }
 
-// reference state of every interior cell (FP64), patch-major SoA blocks of 19 * n^3
+// reference state of every interior cell (FP64), patch-major SoA blocks of 19 * n^3; 0 at solid cells
 template <typename T>
 __global__ __launch_bounds__(256) void k_patch_pull(const PatchDev* __restrict__ patches, int npatch,
-                                                    const int64_t* __restrict__ cell_base, const T* __restrict__ s,
-                                                    double* __restrict__ f) {
+                                                    const int64_t* __restrict__ cell_base, const uint32_t* __restrict__ words,
+                                                    const T* __restrict__ s, double* __restrict__ f) {
   const int p = blockIdx.y;
   if (p >= npatch) return;
   const int64_t base = patches[p].base;
@@ -323,20 +395,28 @@ __global__ __launch_bounds__(256) void k_patch_pull(const PatchDev* __restrict__
   const int64_t cells = (int64_t)n * n * n;
   for (int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x; c < cells; c += (int64_t)gridDim.x * 256) {
     const int x = (int)(c % n), y = (int)((c / n) % n), z = (int)(c / ((int64_t)n * n));
+    const uint32_t w = patch_word(words, cell_base, p, c);
+    const bool fl = (w & FLUID) != 0;
 #pragma unroll
     for (int q = 0; q < Q; q++) {
-      const T v = s[base + slot_offset(q, n, x - CX(q), y - CY(q), z - CZ(q))];
-      f[Q * cell_base[p] + (int64_t)q * cells + c] = (sizeof(T) == 8) ? (double)v : (double)v + W64(q);
+      double v = 0.0;
+      if (fl) {
+        const T sv = s[base + ref_slot(q, n, x, y, z, c, w)];
+        v = (sizeof(T) == 8) ? (double)sv : (double)sv + W64(q);
+      }
+      f[Q * cell_base[p] + (int64_t)q * cells + c] = v;
     }
   }
 }
 
 // rho and u of every interior cell (patch order, x fastest), in the operation order of the dense
 // readout (BLB/BlockComputeCellValues.cpp:26-68: density, then momentum times the reciprocal)
+// (rho = 0, u = 0 at solid cells, as the dense readout)
 template <typename T>
 __global__ __launch_bounds__(256) void k_patch_fields(const PatchDev* __restrict__ patches, int npatch,
-                                                      const int64_t* __restrict__ cell_base, const T* __restrict__ s,
-                                                      double* __restrict__ rho, double* __restrict__ u) {
+                                                      const int64_t* __restrict__ cell_base, const uint32_t* __restrict__ words,
+                                                      const T* __restrict__ s, double* __restrict__ rho,
+                                                      double* __restrict__ u) {
   const int p = blockIdx.y;
   if (p >= npatch) return;
   const int64_t base = patches[p].base;
@@ -344,10 +424,17 @@ __global__ __launch_bounds__(256) void k_patch_fields(const PatchDev* __restrict
   const int64_t cells = (int64_t)n * n * n;
   for (int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x; c < cells; c += (int64_t)gridDim.x * 256) {
     const int x = (int)(c % n), y = (int)((c / n) % n), z = (int)(c / ((int64_t)n * n));
+    const uint32_t w = patch_word(words, cell_base, p, c);
+    if (!(w & FLUID)) {
+      const int64_t k = cell_base[p] + c;
+      rho[k] = 0.0;
+      u[3 * k] = u[3 * k + 1] = u[3 * k + 2] = 0.0;
+      continue;
+    }
     double r = 0.0, b0 = 0.0, b1 = 0.0, b2 = 0.0;
 #pragma unroll
     for (int q = 0; q < Q; q++) {
-      const T v = s[base + slot_offset(q, n, x - CX(q), y - CY(q), z - CZ(q))];
+      const T v = s[base + ref_slot(q, n, x, y, z, c, w)];
       const double fv = (sizeof(T) == 8) ? (double)v : (double)v + W64(q);
       r += fv;
       if (CX(q) == 1) b0 += fv;
@@ -367,9 +454,12 @@ __global__ __launch_bounds__(256) void k_patch_fields(const PatchDev* __restrict
 }
 
 // per-block partial mass / momentum deviations over interior cells, summed in fixed order later
+// (fluid cells only)
 template <typename T>
 __global__ __launch_bounds__(256) void k_patch_moments(const PatchDev* __restrict__ patches, int npatch,
-                                                       const T* __restrict__ s, double* __restrict__ partials) {
+                                                       const int64_t* __restrict__ cell_base,
+                                                       const uint32_t* __restrict__ words, const T* __restrict__ s,
+                                                       double* __restrict__ partials) {
   const int p = blockIdx.y;
   double v[4] = {0.0, 0.0, 0.0, 0.0};
   if (p < npatch) {
@@ -378,9 +468,11 @@ __global__ __launch_bounds__(256) void k_patch_moments(const PatchDev* __restric
     const int64_t cells = (int64_t)n * n * n;
     for (int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x; c < cells; c += (int64_t)gridDim.x * 256) {
       const int x = (int)(c % n), y = (int)((c / n) % n), z = (int)(c / ((int64_t)n * n));
+      const uint32_t w = patch_word(words, cell_base, p, c);
+      if (!(w & FLUID)) continue;
 #pragma unroll
       for (int q = 0; q < Q; q++) {
-        const T sv = s[base + slot_offset(q, n, x - CX(q), y - CY(q), z - CZ(q))];
+        const T sv = s[base + ref_slot(q, n, x, y, z, c, w)];
         const double fv = (sizeof(T) == 8) ? (double)sv : (double)sv + W64(q);
         v[0] += (sizeof(T) == 8) ? (double)sv - W64(q) : (double)sv;
         if (CX(q) == 1) v[1] += fv;
@@ -452,6 +544,8 @@ struct lbm_patchset {
   int64_t npartials = 0;
   double* small = nullptr;
   bool state_set = false;
+  uint32_t* d_words = nullptr;     // per-cell flag words (null: every cell is fluid)
+  int64_t nfluid = 0;              // fluid interior cells
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   int graph_cur = -1;  // buffer parity the graph was captured for
@@ -509,7 +603,7 @@ int build_tiles(lbm_patchset* ps, const lbm_tiling& t) {
   return LBM_OK;
 }
 
-template <typename T, bool M32>
+template <typename T, bool M32, bool FL>
 hipError_t launch_step_t(lbm_patchset* ps, const void* src, void* dst) {
   const lbm_tiling& t = ps->tiling;
   // blocks_per_cu = 0: one workgroup per tile (the loop body runs once)
@@ -519,8 +613,8 @@ hipError_t launch_step_t(lbm_patchset* ps, const void* src, void* dst) {
   T* d = (T*)dst;
 #define LBM_PATCH(X, Y, Z)                                                                                     \
   if (t.tile_x == X && t.tile_y == Y && t.tile_z == Z) {                                                       \
-    hipLaunchKernelGGL((k_patch_step<T, X, Y, Z, M32>), grid, block, 0, ps->stream, ps->d_patches, ps->d_tiles, \
-                       ps->ntiles, s, d);                                                                      \
+    hipLaunchKernelGGL((k_patch_step<T, X, Y, Z, M32, FL>), grid, block, 0, ps->stream, ps->d_patches,          \
+                       ps->d_tiles, ps->ntiles, ps->d_words, s, d);                                            \
     return hipGetLastError();                                                                                  \
   }
   LBM_PATCH(8, 8, 4)
@@ -530,18 +624,23 @@ hipError_t launch_step_t(lbm_patchset* ps, const void* src, void* dst) {
   LBM_PATCH(16, 4, 4)
 #undef LBM_PATCH
   if (linear_tiles(t)) {
-    hipLaunchKernelGGL((k_patch_step_lin<T, M32>), grid, block, 0, ps->stream, ps->d_patches, ps->d_tiles, ps->ntiles,
-                       s, d);
+    hipLaunchKernelGGL((k_patch_step_lin<T, M32, FL>), grid, block, 0, ps->stream, ps->d_patches, ps->d_tiles,
+                       ps->ntiles, ps->d_words, s, d);
     return hipGetLastError();
   }
   return hipErrorInvalidValue;
 }
 
-hipError_t launch_step(lbm_patchset* ps, int from) {
+template <bool FL>
+hipError_t launch_step_fl(lbm_patchset* ps, int from) {
   if (ps->fp32)
-    return ps->f32math ? launch_step_t<float, true>(ps, ps->buf[from], ps->buf[from ^ 1])
-                       : launch_step_t<float, false>(ps, ps->buf[from], ps->buf[from ^ 1]);
-  return launch_step_t<double, false>(ps, ps->buf[from], ps->buf[from ^ 1]);
+    return ps->f32math ? launch_step_t<float, true, FL>(ps, ps->buf[from], ps->buf[from ^ 1])
+                       : launch_step_t<float, false, FL>(ps, ps->buf[from], ps->buf[from ^ 1]);
+  return launch_step_t<double, false, FL>(ps, ps->buf[from], ps->buf[from ^ 1]);
+}
+
+hipError_t launch_step(lbm_patchset* ps, int from) {
+  return ps->d_words ? launch_step_fl<true>(ps, from) : launch_step_fl<false>(ps, from);
 }
 
 bool shape_supported(const lbm_tiling* t) {
@@ -588,6 +687,7 @@ int lbm_patches_create(lbm_patchset** out, const lbm_patch* table, int npatches,
     d.base = base;
     d.omega = 1.0 / tau_eff_of(table[p].tau);
     for (int q = 0; q < Q; q++) d.nbr[q] = q == REST ? p : table[p].nbr[q];
+    d.cbase = cb;
     ps->host.push_back(d);
     ps->cell_base.push_back(cb);
     base += patch_elems(d.n);  // 64-element aligned blocks
@@ -597,6 +697,7 @@ int lbm_patches_create(lbm_patchset** out, const lbm_patch* table, int npatches,
     for (int q = 0; q < Q; q++) d.nbase[q] = ps->host[d.nbr[q]].base;
   ps->elems = base;
   ps->cells = cb;
+  ps->nfluid = cb;
   hipDeviceProp_t prop;
   if (hipSetDevice(device) != hipSuccess || hipGetDeviceProperties(&prop, device) != hipSuccess)
     return bail(LBM_ERR_HIP, "device query failed");
@@ -628,7 +729,8 @@ int lbm_patches_destroy(lbm_patchset* ps) {
   if (!ps) return LBM_OK;
   if (ps->stream) (void)hipStreamSynchronize(ps->stream);
   drop_graph(ps);
-  void* ptrs[] = {ps->buf[0], ps->buf[1], ps->d_patches, ps->d_cell_base, ps->d_tiles, ps->partials, ps->small};
+  void* ptrs[] = {ps->buf[0], ps->buf[1], ps->d_patches, ps->d_cell_base, ps->d_tiles, ps->partials, ps->small,
+                  ps->d_words};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (ps->ev0) (void)hipEventDestroy(ps->ev0);
@@ -643,6 +745,47 @@ const char* lbm_patches_last_error(const lbm_patchset* ps) {
 }
 
 int64_t lbm_patches_cells(const lbm_patchset* ps) { return ps ? ps->cells : -1; }
+
+int lbm_patches_set_flags(lbm_patchset* ps, const uint8_t* fluid) {
+  if (!ps) return LBM_ERR_ARG;
+  PCHK(ps, hipSetDevice(ps->device));
+  PCHK(ps, hipStreamSynchronize(ps->stream));
+  drop_graph(ps);  // captured steps bake in the kernel variant and the word array
+  if (ps->d_words) (void)hipFree(ps->d_words);
+  ps->d_words = nullptr;
+  ps->state_set = false;  // populations sit in pull slots that depend on the flags: init again
+  ps->nfluid = ps->cells;
+  if (!fluid) return LBM_OK;
+  int64_t nf = 0;
+  for (int64_t k = 0; k < ps->cells; k++) nf += fluid[k] != 0;
+  uint8_t* d_fluid = nullptr;
+  if (lbm::device_alloc(&d_fluid, (size_t)ps->cells) != hipSuccess)
+    return pfail(ps, LBM_ERR_NOMEM, "flag staging allocation failed");
+  if (lbm::device_alloc(&ps->d_words, (size_t)ps->cells * sizeof(uint32_t)) != hipSuccess) {
+    ps->d_words = nullptr;
+    (void)hipFree(d_fluid);
+    return pfail(ps, LBM_ERR_NOMEM, "flag word allocation failed");
+  }
+  hipError_t e = hipMemcpyAsync(d_fluid, fluid, (size_t)ps->cells, hipMemcpyHostToDevice, ps->stream);
+  if (e == hipSuccess) {
+    int maxn = 0;
+    for (auto& d : ps->host) maxn = std::max(maxn, d.n);
+    const int64_t maxc = (int64_t)maxn * maxn * maxn;
+    const dim3 grid((unsigned)std::min<int64_t>((maxc + 255) / 256, 64), (unsigned)ps->host.size());
+    hipLaunchKernelGGL(k_patch_words, grid, dim3(256), 0, ps->stream, ps->d_patches, (int)ps->host.size(), d_fluid,
+                       ps->d_words);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(ps->stream);
+  (void)hipFree(d_fluid);
+  if (e != hipSuccess) {
+    (void)hipFree(ps->d_words);
+    ps->d_words = nullptr;
+    return pfail(ps, LBM_ERR_HIP, std::string("patch flags: ") + hipGetErrorString(e));
+  }
+  ps->nfluid = nf;
+  return LBM_OK;
+}
 
 int lbm_patches_init(lbm_patchset* ps, const double* rho, const double* u) {
   if (!ps) return LBM_ERR_ARG;
@@ -664,10 +807,10 @@ int lbm_patches_init(lbm_patchset* ps, const double* rho, const double* u) {
     const dim3 grid((unsigned)std::min<int64_t>((maxc + 255) / 256, 64), (unsigned)ps->host.size());
     if (ps->fp32)
       hipLaunchKernelGGL(k_patch_init<float>, grid, dim3(256), 0, ps->stream, ps->d_patches, (int)ps->host.size(),
-                         ps->d_cell_base, d_rho, d_u, (float*)ps->buf[ps->cur]);
+                         ps->d_cell_base, ps->d_words, d_rho, d_u, (float*)ps->buf[ps->cur]);
     else
       hipLaunchKernelGGL(k_patch_init<double>, grid, dim3(256), 0, ps->stream, ps->d_patches, (int)ps->host.size(),
-                         ps->d_cell_base, d_rho, d_u, (double*)ps->buf[ps->cur]);
+                         ps->d_cell_base, ps->d_words, d_rho, d_u, (double*)ps->buf[ps->cur]);
     e = hipGetLastError();
     if (e == hipSuccess) e = hipStreamSynchronize(ps->stream);
   }
@@ -692,10 +835,10 @@ int lbm_patches_get_populations(lbm_patchset* ps, double* f) {
   const dim3 grid((unsigned)std::min<int64_t>((maxc + 255) / 256, 64), (unsigned)ps->host.size());
   if (ps->fp32)
     hipLaunchKernelGGL(k_patch_pull<float>, grid, dim3(256), 0, ps->stream, ps->d_patches, (int)ps->host.size(),
-                       ps->d_cell_base, (const float*)ps->buf[ps->cur], d_f);
+                       ps->d_cell_base, ps->d_words, (const float*)ps->buf[ps->cur], d_f);
   else
     hipLaunchKernelGGL(k_patch_pull<double>, grid, dim3(256), 0, ps->stream, ps->d_patches, (int)ps->host.size(),
-                       ps->d_cell_base, (const double*)ps->buf[ps->cur], d_f);
+                       ps->d_cell_base, ps->d_words, (const double*)ps->buf[ps->cur], d_f);
   hipError_t e = hipGetLastError();
   if (e == hipSuccess) e = hipMemcpyAsync(f, d_f, bytes, hipMemcpyDeviceToHost, ps->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(ps->stream);
@@ -721,10 +864,10 @@ int lbm_patches_get_fields(lbm_patchset* ps, double* rho, double* u) {
   const dim3 grid((unsigned)std::min<int64_t>((maxc + 255) / 256, 64), (unsigned)ps->host.size());
   if (ps->fp32)
     hipLaunchKernelGGL(k_patch_fields<float>, grid, dim3(256), 0, ps->stream, ps->d_patches, (int)ps->host.size(),
-                       ps->d_cell_base, (const float*)ps->buf[ps->cur], d_rho, d_u);
+                       ps->d_cell_base, ps->d_words, (const float*)ps->buf[ps->cur], d_rho, d_u);
   else
     hipLaunchKernelGGL(k_patch_fields<double>, grid, dim3(256), 0, ps->stream, ps->d_patches, (int)ps->host.size(),
-                       ps->d_cell_base, (const double*)ps->buf[ps->cur], d_rho, d_u);
+                       ps->d_cell_base, ps->d_words, (const double*)ps->buf[ps->cur], d_rho, d_u);
   hipError_t e = hipGetLastError();
   if (e == hipSuccess) e = hipMemcpyAsync(rho, d_rho, cells * sizeof(double), hipMemcpyDeviceToHost, ps->stream);
   if (e == hipSuccess) e = hipMemcpyAsync(u, d_u, 3 * cells * sizeof(double), hipMemcpyDeviceToHost, ps->stream);
@@ -827,10 +970,10 @@ int lbm_patches_reduce_moments(lbm_patchset* ps, double out[4]) {
   const dim3 grid(64, (unsigned)np);
   if (ps->fp32)
     hipLaunchKernelGGL(k_patch_moments<float>, grid, dim3(256), 0, ps->stream, ps->d_patches, np,
-                       (const float*)ps->buf[ps->cur], ps->partials);
+                       ps->d_cell_base, ps->d_words, (const float*)ps->buf[ps->cur], ps->partials);
   else
     hipLaunchKernelGGL(k_patch_moments<double>, grid, dim3(256), 0, ps->stream, ps->d_patches, np,
-                       (const double*)ps->buf[ps->cur], ps->partials);
+                       ps->d_cell_base, ps->d_words, (const double*)ps->buf[ps->cur], ps->partials);
   PCHK(ps, hipGetLastError());
   std::vector<double> h((size_t)np * 64 * 4);
   PCHK(ps, hipMemcpyAsync(h.data(), ps->partials, h.size() * sizeof(double), hipMemcpyDeviceToHost, ps->stream));
@@ -838,7 +981,7 @@ int lbm_patches_reduce_moments(lbm_patchset* ps, double out[4]) {
   double s[4] = {0, 0, 0, 0};
   for (size_t j = 0; j < h.size() / 4; j++)
     for (int k = 0; k < 4; k++) s[k] += h[4 * j + k];
-  out[0] = (double)ps->cells + s[0];
+  out[0] = (double)ps->nfluid + s[0];
   for (int k = 1; k < 4; k++) out[k] = s[k];
   for (int k = 0; k < 4; k++)
     if (!std::isfinite(out[k])) return pfail(ps, LBM_ERR_NUMERIC, "non-finite population detected");

@@ -13,6 +13,7 @@
 //   BLB/services/GridManagementService loadVertex/getPdf/getInner/getBoundary/
 //        getDensity/getVelocity (6^3 blocks, vertex numbers from 8, written back)     -> GridManagementService
 //   peano/geometry/builtin Hexahedron, Sphere, intersected (builtin-geometry tag)     -> Configuration::geometry
+//   BLB/BlockRestrictionProlongation (restriction / prolongation between levels)      -> BlockRestrictionProlongation
 //   LB/PerformanceMeasurement (MLUPS per iteration; wall clock here, not CPU time)
 // The three sweeps of one reference iteration (per-block collide + local streaming, the
 // cross-block streaming of handleCell, the in-cell swap) are one fused device step here.  A mapping
@@ -22,6 +23,7 @@
 // reference reports through assertions throw std::runtime_error.
 #pragma once
 
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <cinttypes>
@@ -271,6 +273,90 @@ class RegularGrid2RegularBlockSolver {
   virtual void handleCell(RegularGridBlockVertex* const, RegularGridBlockCell&, const VertexEnumerator&) {}
   virtual void touchVertexLastTime(RegularGridBlockVertex&, const geometry::Vec3&) {}
   virtual void endIteration(RegularGridBlockState&) {}
+};
+
+// Restriction and prolongation between grid levels (BLB/BlockRestrictionProlongation.h:34-155), with
+// the reference's method names and block arguments (6^3 blocks in GridManagementService's layout,
+// std::vector in place of tarch::la::Vector, child positions 0..3 per axis).  Each call runs on the
+// device through an lbm_blockstore of two blocks (a host round trip per call, as the reference's
+// per-vertex calls are); a spacetree traversal that collects the vertices of one level pair calls
+// lbm_blocks_find_overlap / _prolongate / _restrict once for all of them instead.
+class BlockRestrictionProlongation {
+ public:
+  explicit BlockRestrictionProlongation(int device = 0) {
+    if (lbm_blocks_create(&bs_, 2, device) != LBM_OK)
+      throw std::runtime_error(std::string("lbm_blocks_create: ") + lbm_blocks_last_error(nullptr));
+  }
+  ~BlockRestrictionProlongation() { lbm_blocks_destroy(bs_); }
+  BlockRestrictionProlongation(const BlockRestrictionProlongation&) = delete;
+  BlockRestrictionProlongation& operator=(const BlockRestrictionProlongation&) = delete;
+
+  // :17-28
+  std::array<int, 3> getFatherVertex(const std::array<int, 3>& discretePosition) const {
+    return {discretePosition[0] / 2, discretePosition[1] / 2, discretePosition[2] / 2};
+  }
+  // :264-288
+  std::array<int, 3> getSubBlock(const std::array<int, 3>& discretePosition) const {
+    static const int t[4] = {1, 2, 0, 1};
+    return {t[discretePosition[0]], t[discretePosition[1]], t[discretePosition[2]]};
+  }
+  // :32-62; hasMinusOneEntries gets 216 flags
+  void findFineOverlapCells(const std::vector<double>& finePdf, std::vector<bool>& hasMinusOneEntries) const {
+    size(finePdf, kPdf, "finePdf");
+    check(lbm_blocks_set(bs_, 1, 1, finePdf.data(), nullptr, nullptr), "lbm_blocks_set");
+    const lbm_amr_link l = link({0, 0, 0});
+    uint8_t m[LB_BLOCK_NUMBER_OF_CELLS];
+    check(lbm_blocks_find_overlap(bs_, &l, 1, m), "lbm_blocks_find_overlap");
+    hasMinusOneEntries.assign(m, m + LB_BLOCK_NUMBER_OF_CELLS);
+  }
+  // :83-160
+  void prolongateToFineLevel(const std::array<int, 3>& discretePosition, const std::vector<double>& coarsePdf,
+                             std::vector<double>& finePdf, std::vector<double>& density, std::vector<double>& velocity,
+                             const std::vector<bool>& hasMinusOneEntries) const {
+    size(coarsePdf, kPdf, "coarsePdf");
+    size(finePdf, kPdf, "finePdf");
+    size(density, LB_BLOCK_NUMBER_OF_CELLS, "density");
+    size(velocity, 3 * LB_BLOCK_NUMBER_OF_CELLS, "velocity");
+    if (hasMinusOneEntries.size() != (size_t)LB_BLOCK_NUMBER_OF_CELLS)
+      throw std::runtime_error("hasMinusOneEntries must hold 216 flags");
+    uint8_t m[LB_BLOCK_NUMBER_OF_CELLS];
+    for (int c = 0; c < LB_BLOCK_NUMBER_OF_CELLS; c++) m[c] = hasMinusOneEntries[c] ? 1 : 0;
+    check(lbm_blocks_set(bs_, 0, 1, coarsePdf.data(), nullptr, nullptr), "lbm_blocks_set");
+    check(lbm_blocks_set(bs_, 1, 1, nullptr, density.data(), velocity.data()), "lbm_blocks_set");
+    check(lbm_blocks_set_overlap(bs_, 1, 1, m), "lbm_blocks_set_overlap");
+    const lbm_amr_link l = link(discretePosition);
+    check(lbm_blocks_set(bs_, 1, 1, finePdf.data(), nullptr, nullptr), "lbm_blocks_set");
+    check(lbm_blocks_prolongate(bs_, &l, 1), "lbm_blocks_prolongate");
+    check(lbm_blocks_get(bs_, 1, 1, finePdf.data(), density.data(), velocity.data()), "lbm_blocks_get");
+  }
+  // :164-259
+  void restrictToCoarseLevel(const std::array<int, 3>& discretePosition, std::vector<double>& coarsePdf,
+                             const std::vector<double>& finePdf) const {
+    size(coarsePdf, kPdf, "coarsePdf");
+    size(finePdf, kPdf, "finePdf");
+    check(lbm_blocks_set(bs_, 0, 1, coarsePdf.data(), nullptr, nullptr), "lbm_blocks_set");
+    check(lbm_blocks_set(bs_, 1, 1, finePdf.data(), nullptr, nullptr), "lbm_blocks_set");
+    const lbm_amr_link l = link(discretePosition);
+    check(lbm_blocks_restrict(bs_, &l, 1), "lbm_blocks_restrict");
+    check(lbm_blocks_get(bs_, 0, 1, coarsePdf.data(), nullptr, nullptr), "lbm_blocks_get");
+  }
+
+ private:
+  static constexpr int kPdf = LB_BLOCK_NUMBER_OF_CELLS * LB_CURRENT_DIR;
+  static lbm_amr_link link(const std::array<int, 3>& pos) {
+    lbm_amr_link l{};
+    l.coarse = 0;
+    l.fine = 1;
+    for (int d = 0; d < 3; d++) l.pos[d] = pos[d];
+    return l;
+  }
+  static void size(const std::vector<double>& v, int n, const char* what) {
+    if (v.size() != (size_t)n) throw std::runtime_error(std::string(what) + ": wrong size");
+  }
+  void check(int rc, const char* what) const {
+    if (rc != LBM_OK) throw std::runtime_error(std::string(what) + ": " + lbm_blocks_last_error(bs_));
+  }
+  lbm_blockstore* bs_ = nullptr;
 };
 
 class Repository;

@@ -39,14 +39,16 @@ EXPORTS = (
     "lbm_set_boundary_class", "lbm_set_moving_wall_class", "lbm_set_wall_classes", "lbm_set_boundary_velocity",
     "lbm_set_cavity_lid", "lbm_set_channel", "lbm_set_porosity",
     "lbm_set_coupling_cells", "lbm_set_coupling_data", "lbm_get_coupling_data",
-    "lbm_init_equilibrium", "lbm_set_populations", "lbm_get_populations", "lbm_step", "lbm_step_timed",
+    "lbm_init_equilibrium", "lbm_set_populations", "lbm_get_populations", "lbm_step", "lbm_step_timed", "lbm_measure_copy",
     "lbm_tune", "lbm_synchronize", "lbm_get_fields", "lbm_reduce_moments",
     "lbm_kinetic_energy",
     "lbm_pressure_drop",
     "lbm_adjoint_pressure_drop",
     "lbm_patches_create", "lbm_patches_destroy", "lbm_patches_last_error", "lbm_patches_cells",
     "lbm_patches_init", "lbm_patches_get_populations", "lbm_patches_get_fields", "lbm_patches_autotune",
-    "lbm_patches_set_tiling", "lbm_patches_step", "lbm_patches_reduce_moments",
+    "lbm_patches_set_tiling", "lbm_patches_step", "lbm_patches_reduce_moments", "lbm_patches_set_flags",
+    "lbm_blocks_create", "lbm_blocks_destroy", "lbm_blocks_last_error", "lbm_blocks_set", "lbm_blocks_get",
+    "lbm_blocks_find_overlap", "lbm_blocks_set_overlap", "lbm_blocks_prolongate", "lbm_blocks_restrict", "lbm_blocks_last_ms",
 )
 
 
@@ -70,6 +72,11 @@ class AdjointStats(ctypes.Structure):
 
 class Patch(ctypes.Structure):
     _fields_ = [("n", ctypes.c_int), ("tau", ctypes.c_double), ("nbr", ctypes.c_int * Q)]
+
+
+class AmrLink(ctypes.Structure):
+    _fields_ = [("coarse", ctypes.c_int64), ("fine", ctypes.c_int64), ("pos", ctypes.c_int * 3),
+                ("new_vertex", ctypes.c_int)]
 
 
 class Tiling(ctypes.Structure):
@@ -116,6 +123,7 @@ def load_library(path: str = LIB_PATH):
     L.lbm_step.argtypes = [vp, ctypes.c_int]
     L.lbm_step_graph.argtypes = [vp, ctypes.c_int]
     L.lbm_step_timed.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_float)]
+    L.lbm_measure_copy.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_float)]
     L.lbm_tune.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
     L.lbm_synchronize.argtypes = [vp]
     L.lbm_get_fields.argtypes = [vp, dp, dp]
@@ -133,12 +141,26 @@ def load_library(path: str = LIB_PATH):
         L.lbm_patches_cells.argtypes = [vp]
         L.lbm_patches_cells.restype = ctypes.c_int64
         L.lbm_patches_init.argtypes = [vp, dp, dp]
+        L.lbm_patches_set_flags.argtypes = [vp, u8p]
         L.lbm_patches_get_populations.argtypes = [vp, dp]
         L.lbm_patches_get_fields.argtypes = [vp, dp, dp]
         L.lbm_patches_autotune.argtypes = [vp, ctypes.POINTER(Tiling)]
         L.lbm_patches_set_tiling.argtypes = [vp, ctypes.POINTER(Tiling)]
         L.lbm_patches_step.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_float)]
         L.lbm_patches_reduce_moments.argtypes = [vp, dp]
+    lp = ctypes.POINTER(AmrLink)
+    L.lbm_blocks_create.argtypes = [ctypes.POINTER(vp), ctypes.c_int64, ctypes.c_int]
+    L.lbm_blocks_destroy.argtypes = [vp]
+    L.lbm_blocks_last_error.argtypes = [vp]
+    L.lbm_blocks_last_error.restype = ctypes.c_char_p
+    L.lbm_blocks_set.argtypes = [vp, ctypes.c_int64, ctypes.c_int64, dp, dp, dp]
+    L.lbm_blocks_get.argtypes = [vp, ctypes.c_int64, ctypes.c_int64, dp, dp, dp]
+    L.lbm_blocks_find_overlap.argtypes = [vp, lp, ctypes.c_int64, u8p]
+    L.lbm_blocks_set_overlap.argtypes = [vp, ctypes.c_int64, ctypes.c_int64, u8p]
+    L.lbm_blocks_prolongate.argtypes = [vp, lp, ctypes.c_int64]
+    L.lbm_blocks_restrict.argtypes = [vp, lp, ctypes.c_int64]
+    L.lbm_blocks_last_ms.argtypes = [vp]
+    L.lbm_blocks_last_ms.restype = ctypes.c_float
     _lib = L
     return L
 
@@ -331,6 +353,14 @@ class Lattice:
         self._check(self._L.lbm_reduce_moments(self._h, _dp(out)), "lbm_reduce_moments")
         return out
 
+    def measure_copy(self, steps, pull=False):
+        """Device ms of `steps` launches of the step's bandwidth reference (19-plane copy, or the
+        pull pattern with pull=True) over every stored cell; the state is unchanged."""
+        ms = ctypes.c_float()
+        self._check(self._L.lbm_measure_copy(self._h, 1 if pull else 0, int(steps), ctypes.byref(ms)),
+                    "lbm_measure_copy")
+        return ms.value
+
     def kinetic_energy(self):
         ke = ctypes.c_double()
         self._check(self._L.lbm_kinetic_energy(self._h, ctypes.byref(ke)), "lbm_kinetic_energy")
@@ -395,6 +425,14 @@ class PatchSet:
     def __exit__(self, *exc):
         self.close()
 
+    def set_flags(self, fluid):
+        """Per-cell fluid flags (patch order, x fastest; None = all fluid); init() must follow."""
+        if fluid is None:
+            self._check(self._L.lbm_patches_set_flags(self._h, None), "lbm_patches_set_flags")
+            return
+        a = _sized(np.ascontiguousarray(np.asarray(fluid).reshape(-1) != 0, dtype=np.uint8), self.cells, "flags")
+        self._check(self._L.lbm_patches_set_flags(self._h, _u8(a)), "lbm_patches_set_flags")
+
     def init(self, rho=None, u=None):
         r = None if rho is None else _sized(np.ascontiguousarray(np.asarray(rho, dtype=np.float64).reshape(-1)),
                                             self.cells, "density")
@@ -458,3 +496,102 @@ def cavity(n, tau, ulid, precision=FP64, device=0, layout=LAYOUT_TWO_BUFFER):
     lat.set_cavity_lid(ulid)
     lat.init_equilibrium()
     return lat
+
+
+BLOCK_CELLS = 216
+
+
+class BlockStore:
+    """Device-resident 6^3 vertex blocks in the reference's layout (an lbm_blockstore) with batched
+    restriction and prolongation between grid levels.  Links are sequences of (coarse, fine, pos) or
+    (coarse, fine, pos, new_vertex)."""
+
+    def __init__(self, nblocks, device=0):
+        self._L = load_library()
+        h = ctypes.c_void_p()
+        rc = self._L.lbm_blocks_create(ctypes.byref(h), int(nblocks), int(device))
+        if rc != 0:
+            raise LbmError(f"lbm_blocks_create failed ({rc}): {self._L.lbm_blocks_last_error(None).decode()}")
+        self._h = h
+        self.nblocks = int(nblocks)
+
+    def _check(self, rc, what):
+        if rc != 0:
+            raise LbmError(f"{what} failed ({rc}): {self._L.lbm_blocks_last_error(self._h).decode()}")
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.lbm_blocks_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def set(self, first, pdf=None, density=None, velocity=None):
+        arrs, count = [], None
+        for a, per, what in ((pdf, BLOCK_CELLS * Q, "pdf"), (density, BLOCK_CELLS, "density"),
+                             (velocity, 3 * BLOCK_CELLS, "velocity")):
+            if a is None:
+                arrs.append(None)
+                continue
+            a = np.ascontiguousarray(np.asarray(a, dtype=np.float64).reshape(-1))
+            if a.size % per:
+                raise ValueError(f"{what}: size {a.size} is not a whole number of blocks")
+            if count is not None and a.size // per != count:
+                raise ValueError("pdf, density and velocity cover different block counts")
+            count = a.size // per
+            arrs.append(a)
+        if count is None:
+            return
+        self._check(self._L.lbm_blocks_set(self._h, int(first), int(count), *[None if a is None else _dp(a) for a in arrs]),
+                    "lbm_blocks_set")
+
+    def get(self, first=0, count=None):
+        count = self.nblocks - first if count is None else int(count)
+        pdf = np.empty(count * BLOCK_CELLS * Q)
+        rho = np.empty(count * BLOCK_CELLS)
+        u = np.empty(count * 3 * BLOCK_CELLS)
+        self._check(self._L.lbm_blocks_get(self._h, int(first), count, _dp(pdf), _dp(rho), _dp(u)), "lbm_blocks_get")
+        return pdf.reshape(count, BLOCK_CELLS * Q), rho.reshape(count, BLOCK_CELLS), u.reshape(count, 3 * BLOCK_CELLS)
+
+    @staticmethod
+    def _links(links):
+        arr = (AmrLink * max(1, len(links)))()
+        for k, l in enumerate(links):
+            arr[k].coarse = int(l[0])
+            arr[k].fine = int(l[1])
+            for d in range(3):
+                arr[k].pos[d] = int(l[2][d])
+            arr[k].new_vertex = int(bool(l[3])) if len(l) > 3 else 0
+        return arr
+
+    def find_overlap(self, links):
+        arr = self._links(links)
+        out = np.zeros(len(links) * BLOCK_CELLS, np.uint8)
+        self._check(self._L.lbm_blocks_find_overlap(self._h, arr, len(links), _u8(out)), "lbm_blocks_find_overlap")
+        return out.reshape(len(links), BLOCK_CELLS).astype(bool)
+
+    def set_overlap(self, first, overlap):
+        m = np.ascontiguousarray(np.asarray(overlap).reshape(-1) != 0, dtype=np.uint8)
+        if m.size % BLOCK_CELLS:
+            raise ValueError("overlap: not a whole number of blocks")
+        self._check(self._L.lbm_blocks_set_overlap(self._h, int(first), m.size // BLOCK_CELLS, _u8(m)),
+                    "lbm_blocks_set_overlap")
+
+    def prolongate(self, links):
+        self._check(self._L.lbm_blocks_prolongate(self._h, self._links(links), len(links)), "lbm_blocks_prolongate")
+
+    def restrict(self, links):
+        self._check(self._L.lbm_blocks_restrict(self._h, self._links(links), len(links)), "lbm_blocks_restrict")
+
+    def last_ms(self):
+        return float(self._L.lbm_blocks_last_ms(self._h))

@@ -13,9 +13,10 @@
  *
  * Pinning (see tests/test_oracle.py): the lattice tables, moments, equilibrium and
  * BGK relaxation are pinned by closed-form identities; the full step (collide,
- * swap streaming with implicit halfway bounce-back, moving wall) is pinned
- * bit-for-bit against fixtures produced by the compiled reference
- * (tests/golden/); Guo forcing, Brinkman porosity, periodic wrap, the pressure-drop
+ * swap streaming with implicit halfway bounce-back, moving wall) and the restriction /
+ * prolongation between grid levels are pinned bit-for-bit against fixtures produced by
+ * the compiled reference (tests/golden/), the overlap map of the latter also against a
+ * brute-force run of the reference's own three-step streaming procedure; Guo forcing, Brinkman porosity, periodic wrap, the pressure-drop
  * objective and the adjoint step go beyond the reference and are pinned by analytic
  * solutions (Poiseuille flow, Taylor-Green decay, the uniform-porosity fixed point
  * u + g/2 = g tau_eff / alpha, the objective of a linear density ramp), conservation

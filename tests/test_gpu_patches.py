@@ -159,3 +159,125 @@ def test_patch_table_validation():
             ps.set_tiling(5, 5, 5, 4)
         with pytest.raises(lbm.LbmError, match="state"):
             ps.step(1)
+
+
+def _patches_of(dense, dims, n):
+    """Split a dense (c n, b n, a n) box into the per-patch (n, n, n) arrays of one torus, in the
+    patch order of workloads.torus_table (inverse of _dense_of)."""
+    a, b, c = dims
+    return [dense[kk * n:(kk + 1) * n, jj * n:(jj + 1) * n, ii * n:(ii + 1) * n]
+            for kk in range(c) for jj in range(b) for ii in range(a)]
+
+
+def _torus_obstacles(info, seed):
+    """Random periodic spheres (radius 1..3) in each torus: (fluid per cell in patch order, dense solid
+    per torus)."""
+    rng = np.random.default_rng(seed)
+    fluid, solids = [], []
+    for first, dims, n in info:
+        a, b, c = dims
+        shape = (c * n, b * n, a * n)
+        z, y, x = np.meshgrid(*[np.arange(s) for s in shape], indexing="ij")
+        solid = np.zeros(shape, bool)
+        for _ in range(max(2, (a * b * c * n ** 3) // 300)):
+            ctr = [rng.integers(0, s) for s in shape]
+            r = rng.integers(1, 4)
+            d2 = 0
+            for coord, cc, s in zip((z, y, x), ctr, shape):
+                d = np.abs(coord - cc)
+                d = np.minimum(d, s - d)
+                d2 = d2 + d * d
+            solid |= d2 <= r * r
+        solids.append(solid)
+        fluid += [~p.reshape(-1) for p in _patches_of(solid, dims, n)]
+    return np.concatenate(fluid), solids
+
+
+@pytest.mark.parametrize("tori", [
+    [((2, 2, 2), 8)],
+    # edge classes with and without the constant-divisor split; n = 3 puts most cells on faces and edges
+    [((2, 1, 1), 16), ((1, 2, 2), 6), ((3, 1, 1), 3)],
+])
+def test_patches_with_obstacles_bitwise_equal_dense_oracle(tori):
+    table, info, u = _setup(tori, seed=21)
+    fluid, solids = _torus_obstacles(info, seed=22)
+    assert 0 < (~fluid).sum() < fluid.size
+    steps = 12
+    print(f"[gpu] patches with obstacles {[(d, n) for d, n in tori]} FP64, {steps} steps", flush=True)
+    results = []
+    for tiling, graph in (((256, 0, 0, 0), False), ((16, 8, 2, 4), True)):
+        with lbm.PatchSet(table, precision=lbm.FP64) as ps:
+            ps.set_flags(fluid)
+            ps.init(u=u)
+            ps.set_tiling(*tiling)
+            ps.step(steps, graph=graph)
+            results.append((ps.get_populations(), ps.fields(), ps.moments()))
+    for fa, fb in zip(results[0][0], results[1][0]):
+        assert np.array_equal(fa, fb)
+    fp, (rho_p, u_p), m = results[0]
+    offs = np.cumsum([0] + [t[0] ** 3 for t in table])
+    mass = 0.0
+    for (first, dims, n), solid in zip(info, solids):
+        a, b, c = dims
+        ids = list(range(first, first + a * b * c))
+        u_p0 = [u[offs[p]:offs[p + 1]].reshape(n, n, n, 3).transpose(3, 0, 1, 2) for p in ids]
+        tau_d = _dense_of([np.full((n, n, n), table[p][1]) for p in ids], dims, n)
+        lat = O.Lattice(a * n, b * n, c * n, periodic=(True, True, True))
+        lat.set_solid(solid.astype(np.uint8))
+        lat.set_tau_cells(tau_d)
+        u_d = np.moveaxis(_dense_of(u_p0, dims, n), 0, -1)
+        fo = lat.step(lat.equilibrium(u=u_d.reshape(-1, 3)), steps)
+        rho_o, u_o = lat.fields(fo)
+        mass += lat.moments(fo)[0]
+        fl = ~solid
+        fo = fo.reshape(19, c * n, b * n, a * n)
+        fg = _dense_of([fp[p] for p in ids], dims, n)
+        assert np.array_equal(fg[:, fl], fo[:, fl]), f"torus {dims}x{n}: max |diff| {np.abs(fg - fo)[:, fl].max()}"
+        assert not fg[:, solid].any(), "solid cells read back as 0"
+        rg = _dense_of([rho_p[offs[p]:offs[p + 1]].reshape(n, n, n) for p in ids], dims, n)
+        ug = _dense_of([u_p[offs[p]:offs[p + 1]].reshape(n, n, n, 3).transpose(3, 0, 1, 2) for p in ids], dims, n)
+        assert np.array_equal(rg[fl], rho_o.reshape(rg.shape)[fl])
+        assert np.array_equal(ug[:, fl], np.moveaxis(u_o.reshape(c * n, b * n, a * n, 3), -1, 0)[:, fl])
+        assert not rg[solid].any()
+    # mass over fluid cells: conserved by bounce-back, equal to the oracle's up to summation order
+    assert abs(m[0] - mass) <= 1e-12 * mass
+    assert abs(m[0] - fluid.sum()) <= 1e-10 * fluid.sum()
+
+
+def test_patches_with_obstacles_fp32_shifted_close_to_fp64():
+    table, info, u = _setup([((2, 2, 2), 8), ((2, 1, 1), 12)], seed=31)
+    fluid, _ = _torus_obstacles(info, seed=32)
+    res = {}
+    for prec in (lbm.FP64, lbm.FP32_SHIFTED):
+        with lbm.PatchSet(table, precision=prec) as ps:
+            ps.set_flags(fluid)
+            ps.init(u=u)
+            ps.step(60, graph=True)
+            res[prec] = ps.fields()[1][fluid]
+    d = res[lbm.FP32_SHIFTED] - res[lbm.FP64]
+    rel = np.linalg.norm(d) / np.linalg.norm(res[lbm.FP64])
+    assert rel <= 1e-5, rel
+
+
+def test_patch_flags_require_init_and_can_be_cleared():
+    table, info, u = _setup([((2, 2, 1), 8)], seed=41)
+    fluid, _ = _torus_obstacles(info, seed=42)
+    with lbm.PatchSet(table) as ps:
+        ps.init(u=u)
+        ps.set_flags(fluid)
+        with pytest.raises(lbm.LbmError, match="state"):
+            ps.step(1)
+        with pytest.raises(ValueError, match="flags"):
+            ps.set_flags(fluid[:-1])
+        ps.init(u=u)
+        ps.step(3)
+        assert abs(ps.moments()[0] - fluid.sum()) < 1e-10 * fluid.sum()
+        ps.set_flags(None)  # all fluid again: the plain torus
+        ps.init(u=u)
+        ps.step(3)
+        f_clear = ps.get_populations()
+    with lbm.PatchSet(table) as ps:
+        ps.init(u=u)
+        ps.step(3)
+        for a_, b_ in zip(f_clear, ps.get_populations()):
+            assert np.array_equal(a_, b_)
