@@ -205,6 +205,16 @@ int lbm_get_fields(lbm_ctx* ctx, double* rho, double* u);
 /* FP64 global mass and momentum over fluid cells: out = {sum f, sum f c_x, sum f c_y, sum f c_z}.
  * Deterministic two-pass reduction; returns LBM_ERR_NUMERIC if a non-finite value is found. */
 int lbm_reduce_moments(lbm_ctx* ctx, double out[4]);
+/* lbm_reduce_moments that allocates nothing.  No counterpart in the reference: it is for callers
+ * that cannot allocate device memory (code between HIP graph captures, tight serving loops,
+ * harnesses that forbid hipMalloc).  The partial sums and results live in a fixed 16 MiB device
+ * array that the library holds on each device it runs on until the process ends (its address is
+ * looked up at lbm_create), instead of the context's own partials buffer; results are the same bits
+ * as lbm_reduce_moments.  Lattices whose reduction needs more than 524,414 workgroups of 256 cells
+ * (about 2^27 cells, e.g. beyond 512^3) are rejected with LBM_ERR_ARG.  All contexts on a device
+ * share its array: calls are serialised within a process (each returns only after its results are
+ * copied out), but two processes on one GPU each have their own copy. */
+int lbm_reduce_moments_noalloc(lbm_ctx* ctx, double out[4]);
 /* FP64 kinetic energy sum over fluid cells of |j|^2 / (2 rho) (deterministic two-pass reduction).
  * No reference counterpart (observable for the Taylor-Green validation). */
 int lbm_kinetic_energy(lbm_ctx* ctx, double* ke);

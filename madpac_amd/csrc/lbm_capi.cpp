@@ -176,6 +176,8 @@ int lbm_create(lbm_ctx** out, const lbm_config* cfg) {
   if (hipSetDevice(cfg->device) != hipSuccess) return bail(LBM_ERR_HIP, "hipSetDevice failed");
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
     return bail(LBM_ERR_HIP, "stream creation failed");
+  if (lbm_noalloc_setup(cfg->device) != hipSuccess)
+    return bail(LBM_ERR_HIP, "device address of the no-alloc reduction array not found");
   if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess)
     return bail(LBM_ERR_HIP, "event creation failed");
   // sized for the larger (bricked) extent, so that lbm_set_flags can switch the cell order

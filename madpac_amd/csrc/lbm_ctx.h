@@ -95,4 +95,6 @@ int lbm_enqueue_coupling_aa(lbm_ctx* c, int parity);
 void lbm_coupling_release(lbm_ctx* c);
 void lbm_graphs_release(lbm_ctx* c);
 void lbm_adjoint_release(lbm_ctx* c);
-
+// once per device, with the device current: the device address of the fixed array that
+// lbm_reduce_moments_noalloc uses (lbm_noalloc.hip)
+hipError_t lbm_noalloc_setup(int device);

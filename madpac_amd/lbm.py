@@ -40,7 +40,7 @@ EXPORTS = (
     "lbm_set_cavity_lid", "lbm_set_channel", "lbm_set_porosity",
     "lbm_set_coupling_cells", "lbm_set_coupling_data", "lbm_get_coupling_data",
     "lbm_init_equilibrium", "lbm_set_populations", "lbm_get_populations", "lbm_step", "lbm_step_timed", "lbm_measure_copy",
-    "lbm_tune", "lbm_synchronize", "lbm_get_fields", "lbm_reduce_moments",
+    "lbm_tune", "lbm_synchronize", "lbm_get_fields", "lbm_reduce_moments", "lbm_reduce_moments_noalloc",
     "lbm_kinetic_energy",
     "lbm_pressure_drop",
     "lbm_adjoint_pressure_drop",
@@ -128,6 +128,7 @@ def load_library(path: str = LIB_PATH):
     L.lbm_synchronize.argtypes = [vp]
     L.lbm_get_fields.argtypes = [vp, dp, dp]
     L.lbm_reduce_moments.argtypes = [vp, dp]
+    L.lbm_reduce_moments_noalloc.argtypes = [vp, dp]
     L.lbm_kinetic_energy.argtypes = [vp, dp]
     L.lbm_pressure_drop.argtypes = [vp, ctypes.c_int, ctypes.c_int, dp]
     L.lbm_adjoint_pressure_drop.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, dp, dp,
@@ -360,6 +361,12 @@ class Lattice:
         self._check(self._L.lbm_measure_copy(self._h, 1 if pull else 0, int(steps), ctypes.byref(ms)),
                     "lbm_measure_copy")
         return ms.value
+
+    def moments_noalloc(self):
+        """moments() through lbm_reduce_moments_noalloc (no device allocation; same bits)."""
+        out = np.empty(4)
+        self._check(self._L.lbm_reduce_moments_noalloc(self._h, _dp(out)), "lbm_reduce_moments_noalloc")
+        return out
 
     def kinetic_energy(self):
         ke = ctypes.c_double()
