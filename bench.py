@@ -207,7 +207,10 @@ def run_patches(lbm, wl, prec, steps=500):
     achieved = interior_bytes / per / 1e9
     return {"value": round(ps.cells * steps / (ms * 1e-3) / 1e6, 1), "unit": "MLUPS",
             "ms_per_step": round(ms / steps, 4), "patches": len(table), "cells": ps.cells,
-            "tiling": {"tile": list(tiling[:3]), "blocks_per_cu": tiling[3]}, "graph_steps": steps,
+            "tiling": {"tile": list(tiling[:3]), "blocks_per_cu": tiling[3],
+                       "schedule": ("one workgroup per tile" if tiling[3] == 0 else
+                                    "persistent, contiguous tile ranges" if tiling[4] else "persistent, strided")},
+            "graph_steps": steps,
             "mass_drift": float(m[0] - ps.cells),
             "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": round(achieved / HBM_PEAK_GBS, 4),
@@ -301,6 +304,8 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-patches", action="store_true")
     ap.add_argument("--no-all-fluid", action="store_true")
+    ap.add_argument("--all-fluid-sizes", default="512,256",
+                    help="edges of the all-fluid Taylor-Green legs (the porous channel's size runs --steps steps)")
     ap.add_argument("--patch-steps", type=int, default=500)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -338,14 +343,15 @@ def main():
         f"fp32s {shard['fp32s']['value']:.0f} MLUPS")
     tgv = tgv_same = None
     if not args.no_all_fluid:
-        tgv = run_taylor_green(lbm, wl)
-        log(f"[bench] Taylor-Green 512^3: fp64 {tgv['fp64']['value']:.0f} MLUPS, "
-            f"fp32s {tgv['fp32s']['value']:.0f} MLUPS")
-        # the same kernels on an all-fluid lattice of the porous channel's size: separates the lattice
-        # size from the line waste at the solid surfaces
-        tgv_same = run_taylor_green(lbm, wl, n=n, steps=args.steps)
-        log(f"[bench] Taylor-Green {n}^3: fp64 {tgv_same['fp64']['value']:.0f} MLUPS, "
-            f"fp32s {tgv_same['fp32s']['value']:.0f} MLUPS")
+        for m in [int(v) for v in args.all_fluid_sizes.split(",") if v]:
+            # at the porous channel's size the same kernels without solid surfaces separate the
+            # lattice size from the line waste at the surfaces
+            r = run_taylor_green(lbm, wl, n=m, steps=args.steps if m == n else 200)
+            if m == n:
+                tgv_same = r
+            else:
+                tgv = r
+            log(f"[bench] Taylor-Green {m}^3: fp64 {r['fp64']['value']:.0f} MLUPS, fp32s {r['fp32s']['value']:.0f} MLUPS")
     amr = run_block_store(lbm)
     log(f"[bench] block store {amr['workload']}: " + ", ".join(
         f"{k} {amr[k]['ms']:.3f} ms ({amr[k]['GBs']:.0f} GB/s)" for k in ("find_overlap", "prolongate", "restrict")))

@@ -58,6 +58,7 @@ typedef struct {
   int device;            /* HIP device ordinal */
   int layout;            /* enum lbm_layout (0 = two buffers) */
   int cell_order;        /* enum lbm_cell_order (0 = automatic) */
+  int populations;       /* enum lbm_populations (0 = automatic) */
 } lbm_config;
 
 /* Order of the cells in device memory; the C-ABI always exchanges row-major arrays.  ROWS:
@@ -68,6 +69,14 @@ typedef struct {
  * 128-byte lines (solid cells sharing lines with fluid ones), ROWS otherwise (FP32 measured
  * faster in rows even on the porous lattice). */
 enum lbm_cell_order { LBM_CELL_ORDER_AUTO = 0, LBM_CELL_ORDER_ROWS = 1, LBM_CELL_ORDER_BRICKS = 2 };
+
+/* Arrangement of the 19 populations in device memory (the C-ABI exchanges SoA arrays either way;
+ * results are bit-identical).  PLANES: plane q holds population q of every stored cell.
+ * INTERLEAVED: the 19 populations of each group of 64 consecutive stored cells (one wavefront's
+ * cells) lie together, so a wavefront's accesses stay within a few kilobytes instead of 19 planes:
+ * on the porous 256^3 channel FP64 +6 % (bricks and rows), FP32 rows +1 %, FP32 bricks -2 %; on
+ * all-fluid 256^3 / 512^3 lattices +3 to +5 %.  AUTO is INTERLEAVED. */
+enum lbm_populations { LBM_POPULATIONS_AUTO = 0, LBM_POPULATIONS_PLANES = 1, LBM_POPULATIONS_INTERLEAVED = 2 };
 
 /* Storage of the state.  TWO_BUFFER: pull-form steps between two buffers.  AA: one buffer updated
  * in place, alternating a step that touches only the cell's own slots with one that pulls from and
@@ -253,6 +262,9 @@ typedef struct {
 typedef struct {
   int tile_x, tile_y, tile_z;  /* tile extents in cells (one workgroup per tile) */
   int blocks_per_cu;           /* persistent grid = CUs * blocks_per_cu (1..64); 0 = one workgroup per tile */
+  int chunked;                 /* persistent grid: 0 = workgroup w takes tiles w, w + G, w + 2G, ...;
+                                  1 = workgroup w takes the contiguous range [w T / G, (w + 1) T / G) of
+                                  the T tiles, so a patch's neighbouring tiles run on one CU */
 } lbm_tiling;
 int lbm_patches_create(lbm_patchset** ps, const lbm_patch* table, int npatches, int precision,
                        int device);

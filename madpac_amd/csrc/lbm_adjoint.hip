@@ -76,7 +76,7 @@ __global__ __launch_bounds__(BX * BY_MAX) void k_adj_step(DenseGeom g, const T* 
   T f[Q], mu[Q];
 #pragma unroll
   for (int q = 0; q < Q; q++) {
-    f[q] = s[pull_index(q, w, c.i, N, o)];
+    f[q] = s[pull_index(g, q, w, c.i, o)];
     const bool bounce = (w >> q) & 1u;  // x + c_q is not fluid
     mu[q] = bounce ? lam_out[(int64_t)(Q - 1 - q) * N + c.i] : lam_out[(int64_t)q * N + c.i + dst_off(q, o)];
   }

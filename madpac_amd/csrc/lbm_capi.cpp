@@ -143,6 +143,8 @@ int lbm_create(lbm_ctx** out, const lbm_config* cfg) {
     return fail(nullptr, LBM_ERR_ARG, "unknown layout");
   if (cfg->cell_order < LBM_CELL_ORDER_AUTO || cfg->cell_order > LBM_CELL_ORDER_BRICKS)
     return fail(nullptr, LBM_ERR_ARG, "unknown cell order");
+  if (cfg->populations < LBM_POPULATIONS_AUTO || cfg->populations > LBM_POPULATIONS_INTERLEAVED)
+    return fail(nullptr, LBM_ERR_ARG, "unknown population arrangement");
   if (!(cfg->tau > 0.5) || !(cfg->tau < 1e6)) return fail(nullptr, LBM_ERR_ARG, "tau must be > 0.5");
   for (int k = 0; k < 3; k++)
     if (!std::isfinite(cfg->force[k])) return fail(nullptr, LBM_ERR_ARG, "force must be finite");
@@ -156,8 +158,12 @@ int lbm_create(lbm_ctx** out, const lbm_config* cfg) {
   c->f32math = cfg->precision == LBM_FP32_SHIFTED_F32;
   c->aa = cfg->layout == LBM_LAYOUT_AA;
   c->esize = c->fp32 ? sizeof(float) : sizeof(double);
-  c->g_rows = lbm::make_geom(cfg->nx, cfg->ny, cfg->nz, cfg->periodic, false, c->esize);
-  c->g_bricks = lbm::make_geom(cfg->nx, cfg->ny, cfg->nz, cfg->periodic, true, c->esize);
+  // interleaved unless planes are asked for: measured faster in every case of the porous 256^3
+  // channel and the all-fluid 256^3 / 512^3 lattices but FP32 in bricks (-2 %), which AUTO's cell
+  // order does not pick
+  const bool aos = cfg->populations != LBM_POPULATIONS_PLANES;
+  c->g_rows = lbm::make_geom(cfg->nx, cfg->ny, cfg->nz, cfg->periodic, false, c->esize, aos);
+  c->g_bricks = lbm::make_geom(cfg->nx, cfg->ny, cfg->nz, cfg->periodic, true, c->esize, aos);
   c->g = cfg->cell_order == LBM_CELL_ORDER_BRICKS ? c->g_bricks : c->g_rows;
   c->force = cfg->force[0] != 0.0 || cfg->force[1] != 0.0 || cfg->force[2] != 0.0;
   c->p64.tau_eff = lbm_tau_effective(cfg->tau);

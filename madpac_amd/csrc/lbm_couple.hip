@@ -125,16 +125,15 @@ __global__ __launch_bounds__(256) void k_couple(DenseGeom g, const T* __restrict
   const uint32_t w = words[i];
   if (!(w & FLUID)) return;
   const NbrOff o = nbr_offsets(g, pos_of(g, i));
-  const int64_t N = g.N;
   double f[Q];
 #pragma unroll
   for (int q = 0; q < Q; q++) {
-    const T v = src[pull_index(q, w, i, N, o)];
+    const T v = src[pull_index(g, q, w, i, o)];
     f[q] = sizeof(T) == 8 ? (double)v : (double)v + W64(q);
   }
   couple_collide<WALLS>(f, a, k, i, w, bc);
 #pragma unroll
-  for (int q = 0; q < Q; q++) dst[(int64_t)q * N + i] = sizeof(T) == 8 ? (T)f[q] : (T)(f[q] - W64(q));
+  for (int q = 0; q < Q; q++) dst[pop_at(g, q, i)] = sizeof(T) == 8 ? (T)f[q] : (T)(f[q] - W64(q));
 }
 
 // In-place (AA) layout: the step overwrites the populations the re-collision starts from, so they
@@ -169,7 +168,6 @@ __global__ __launch_bounds__(256) void k_couple_aa(DenseGeom g, const double* __
   const uint32_t w = words[i];
   if (!(w & FLUID)) return;
   const NbrOff o = nbr_offsets(g, pos_of(g, i));
-  const int64_t N = g.N;
   double f[Q];
 #pragma unroll
   for (int q = 0; q < Q; q++) f[q] = save[(int64_t)q * a.n + k];
@@ -177,7 +175,7 @@ __global__ __launch_bounds__(256) void k_couple_aa(DenseGeom g, const double* __
 #pragma unroll
   for (int q = 0; q < Q; q++) {
     const bool bounce = (w >> q) & 1u;
-    const int64_t di = (!ODD || bounce) ? (int64_t)(Q - 1 - q) * N + i : (int64_t)q * N + i + dst_off(q, o);
+    const int64_t di = (!ODD || bounce) ? pop_at(g, Q - 1 - q, i) : pop_at(g, q, i + dst_off(q, o));
     s[di] = sizeof(T) == 8 ? (T)f[q] : (T)(f[q] - W64(q));
   }
 }

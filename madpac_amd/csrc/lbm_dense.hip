@@ -21,7 +21,7 @@ namespace lbm {
 
 int64_t plane_pad(size_t esize) { return esize == 8 ? 2112 : 64; }
 
-DenseGeom make_geom(int nx, int ny, int nz, const int periodic[3], bool bricked, size_t esize) {
+DenseGeom make_geom(int nx, int ny, int nz, const int periodic[3], bool bricked, size_t esize, bool aos) {
   DenseGeom g;
   g.nx = nx; g.ny = ny; g.nz = nz;
   g.px = periodic[0] ? 1 : 0; g.py = periodic[1] ? 1 : 0; g.pz = periodic[2] ? 1 : 0;
@@ -38,6 +38,7 @@ DenseGeom make_geom(int nx, int ny, int nz, const int periodic[3], bool bricked,
   g.N = (bricked ? bricks * 64 : g.Nu) + plane_pad(esize);
   set_workgroup_rows(g, BY_DEFAULT);
   g.layout = LAYOUT_PULL;
+  g.aos = aos ? 1 : 0;
   return g;
 }
 
@@ -138,7 +139,7 @@ __device__ __forceinline__ bool line_has_fluid(unsigned long long fluid) {
 // dependent memory round trips; solid lanes load nothing).  SPEC = true: the 19 neighbour loads
 // issue together with the word, and the few bounce-back populations are re-loaded from the cell's
 // own reflected slots once the word has arrived (one round trip on most lanes; solid lanes load).
-template <typename T, bool FORCE, bool ALPHA, bool WALLS, bool SPEC, bool M32 = false>
+template <typename T, bool FORCE, bool ALPHA, bool WALLS, bool SPEC, bool M32, bool AOS>
 __global__ __launch_bounds__(BX * BY_MAX) void k_dense_step(DenseGeom g, const T* __restrict__ src,
                                                         T* __restrict__ dst, const uint32_t* __restrict__ words,
                                                         const T* __restrict__ alpha, CollideParams p64,
@@ -151,7 +152,7 @@ __global__ __launch_bounds__(BX * BY_MAX) void k_dense_step(DenseGeom g, const T
   uint32_t w;
   if (SPEC) {
 #pragma unroll
-    for (int q = 0; q < Q; q++) f[q] = src[(int64_t)q * N + c.i + src_off(q, o)];
+    for (int q = 0; q < Q; q++) f[q] = src[pop_at<AOS>(N, q, c.i + src_off(q, o))];
     w = words[c.i];
     const bool fl = (w & FLUID) != 0;
     const unsigned long long fluid = __ballot(fl);  // every lane votes before any returns
@@ -160,7 +161,7 @@ __global__ __launch_bounds__(BX * BY_MAX) void k_dense_step(DenseGeom g, const T
       if (w & 0x7FFFFu) {
 #pragma unroll
         for (int q = 0; q < Q; q++)
-          if ((w >> (Q - 1 - q)) & 1u) f[q] = src[(int64_t)(Q - 1 - q) * N + c.i];
+          if ((w >> (Q - 1 - q)) & 1u) f[q] = src[pop_at<AOS>(N, Q - 1 - q, c.i)];
       }
       collide_cell_bc<T, FORCE, ALPHA, WALLS, M32>(f, w, c.i, alpha, p64, p32, bc);
     } else {
@@ -174,7 +175,7 @@ __global__ __launch_bounds__(BX * BY_MAX) void k_dense_step(DenseGeom g, const T
     if (!fl && !line_has_fluid<T>(fluid)) return;
     if (fl) {
 #pragma unroll
-      for (int q = 0; q < Q; q++) f[q] = src[pull_index(q, w, c.i, N, o)];
+      for (int q = 0; q < Q; q++) f[q] = src[pull_index<AOS>(q, w, c.i, N, o)];
       collide_cell_bc<T, FORCE, ALPHA, WALLS, M32>(f, w, c.i, alpha, p64, p32, bc);
     } else {
 #pragma unroll
@@ -182,7 +183,7 @@ __global__ __launch_bounds__(BX * BY_MAX) void k_dense_step(DenseGeom g, const T
     }
   }
 #pragma unroll
-  for (int q = 0; q < Q; q++) __builtin_nontemporal_store(f[q], &dst[(int64_t)q * N + c.i]);
+  for (int q = 0; q < Q; q++) __builtin_nontemporal_store(f[q], &dst[pop_at<AOS>(N, q, c.i)]);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -194,7 +195,7 @@ __global__ __launch_bounds__(BX * BY_MAX) void k_dense_step(DenseGeom g, const T
 // cell writes exactly the slots it read, and no other cell touches them, so the update is in place
 // without races.  Boundary classes act on the outgoing population of a bounce-back link, as in the
 // two-buffer step.
-template <typename T, bool FORCE, bool ALPHA, bool WALLS, bool M32, bool ODD>
+template <typename T, bool FORCE, bool ALPHA, bool WALLS, bool M32, bool ODD, bool AOS>
 __global__ __launch_bounds__(BX * BY_MAX) void k_aa_step(DenseGeom g, T* __restrict__ a, const uint32_t* __restrict__ words,
                                                      const T* __restrict__ alpha, CollideParams p64,
                                                      CollideParams32 p32, BoundaryArgs bc) {
@@ -212,7 +213,7 @@ __global__ __launch_bounds__(BX * BY_MAX) void k_aa_step(DenseGeom g, T* __restr
   T f[Q];
   if (fl) {
 #pragma unroll
-    for (int q = 0; q < Q; q++) f[q] = ODD ? a[swapped_index(q, w, c.i, N, o)] : a[(int64_t)q * N + c.i];
+    for (int q = 0; q < Q; q++) f[q] = ODD ? a[swapped_index<AOS>(q, w, c.i, N, o)] : a[pop_at<AOS>(N, q, c.i)];
     collide_cell_bc<T, FORCE, ALPHA, WALLS, M32>(f, w, c.i, alpha, p64, p32, bc);
   } else {
 #pragma unroll
@@ -227,8 +228,8 @@ __global__ __launch_bounds__(BX * BY_MAX) void k_aa_step(DenseGeom g, T* __restr
   for (int q = 0; q < Q; q++) {
     // x + c_q is not fluid (a solid lane stores its zeros to its own slots)
     const bool bounce = ((fl ? w : 0x7FFFFu) >> q) & 1u;
-    const int64_t di = !ODD ? (int64_t)(Q - 1 - q) * N + i
-                            : (bounce ? (int64_t)(Q - 1 - q) * N + i : (int64_t)q * N + i + dst_off(q, o));
+    const int64_t di = !ODD ? pop_at<AOS>(N, Q - 1 - q, i)
+                            : (bounce ? pop_at<AOS>(N, Q - 1 - q, i) : pop_at<AOS>(N, q, i + dst_off(q, o)));
     // An even step writes whole lines of its own slots.  An odd step's pushes fill each line from
     // up to four wavefronts: ordinary stores let L2 merge the parts before write-back (measured
     // 5-13 % faster on a porous 256^3 channel than streaming stores, which write each part out).
@@ -282,7 +283,7 @@ __global__ __launch_bounds__(BX * BY_MAX) void k_dense_unpull(DenseGeom g, const
 // and non-temporal stores, and no flag word or arithmetic.  PULL = false: each cell copies its own
 // slots (the 19-plane copy); PULL = true: it reads the slots the step pulls from (x - c_q, wrapped on
 // periodic axes, clamped otherwise), i.e. the all-fluid step's memory pattern.
-template <typename T, bool PULL>
+template <typename T, bool PULL, bool AOS>
 __global__ __launch_bounds__(BX * BY_MAX) void k_dense_copy(DenseGeom g, const T* __restrict__ src, T* __restrict__ dst) {
   const CellPos c = map_cell(g);
   if (!c.valid) return;
@@ -291,13 +292,13 @@ __global__ __launch_bounds__(BX * BY_MAX) void k_dense_copy(DenseGeom g, const T
   if (PULL) {
     const NbrOff o = nbr_offsets(g, c);
 #pragma unroll
-    for (int q = 0; q < Q; q++) f[q] = src[(int64_t)q * N + c.i + src_off(q, o)];
+    for (int q = 0; q < Q; q++) f[q] = src[pop_at<AOS>(N, q, c.i + src_off(q, o))];
   } else {
 #pragma unroll
-    for (int q = 0; q < Q; q++) f[q] = src[(int64_t)q * N + c.i];
+    for (int q = 0; q < Q; q++) f[q] = src[pop_at<AOS>(N, q, c.i)];
   }
 #pragma unroll
-  for (int q = 0; q < Q; q++) __builtin_nontemporal_store(f[q], &dst[(int64_t)q * N + c.i]);
+  for (int q = 0; q < Q; q++) __builtin_nontemporal_store(f[q], &dst[pop_at<AOS>(N, q, c.i)]);
 }
 
 // The same state in another layout of the same cell order (e.g. pull form <-> AA NATURAL): each
@@ -572,7 +573,7 @@ hipError_t launch_build_words(const DenseGeom& g, const uint8_t* inner, const ui
   return hipGetLastError();
 }
 
-template <typename T, bool SPEC, bool M32>
+template <typename T, bool SPEC, bool M32, bool AOS>
 static hipError_t step_t(const DenseGeom& g, const void* src, void* dst, const uint32_t* words, const void* alpha,
                          bool force, bool walls, const CollideParams& p64, const CollideParams32& p32,
                          const BoundaryArgs& bc, hipStream_t s) {
@@ -580,7 +581,7 @@ static hipError_t step_t(const DenseGeom& g, const void* src, void* dst, const u
   const T* sp = (const T*)src;
   T* dp = (T*)dst;
 #define LBM_STEP(F, A, W)                                                                                        \
-  hipLaunchKernelGGL((k_dense_step<T, F, A, W, SPEC, M32>), dense_grid(g), dense_block(g), 0, s, g, sp, dp, words, a, \
+  hipLaunchKernelGGL((k_dense_step<T, F, A, W, SPEC, M32, AOS>), dense_grid(g), dense_block(g), 0, s, g, sp, dp, words, a, \
                      p64, p32, bc)
   const int key = (force ? 4 : 0) | (alpha ? 2 : 0) | (walls ? 1 : 0);
   switch (key) {
@@ -597,27 +598,35 @@ static hipError_t step_t(const DenseGeom& g, const void* src, void* dst, const u
   return hipGetLastError();
 }
 
+template <bool AOS>
+static hipError_t step_aos(const DenseGeom& g, bool fp32, bool f32math, const void* src, void* dst,
+                           const uint32_t* words, const void* alpha, bool force, bool walls, const CollideParams& p64,
+                           const CollideParams32& p32, const BoundaryArgs& bc, bool spec, hipStream_t s) {
+  if (fp32 && f32math)
+    return spec ? step_t<float, true, true, AOS>(g, src, dst, words, alpha, force, walls, p64, p32, bc, s)
+                : step_t<float, false, true, AOS>(g, src, dst, words, alpha, force, walls, p64, p32, bc, s);
+  if (fp32)
+    return spec ? step_t<float, true, false, AOS>(g, src, dst, words, alpha, force, walls, p64, p32, bc, s)
+                : step_t<float, false, false, AOS>(g, src, dst, words, alpha, force, walls, p64, p32, bc, s);
+  return spec ? step_t<double, true, false, AOS>(g, src, dst, words, alpha, force, walls, p64, p32, bc, s)
+              : step_t<double, false, false, AOS>(g, src, dst, words, alpha, force, walls, p64, p32, bc, s);
+}
+
 hipError_t launch_dense_step(const DenseGeom& g, bool fp32, bool f32math, const void* src, void* dst,
                              const uint32_t* words, const void* alpha, bool force, bool walls,
                              const CollideParams& p64, const CollideParams32& p32, const BoundaryArgs& bc, bool spec,
                              hipStream_t s) {
-  if (fp32 && f32math)
-    return spec ? step_t<float, true, true>(g, src, dst, words, alpha, force, walls, p64, p32, bc, s)
-                : step_t<float, false, true>(g, src, dst, words, alpha, force, walls, p64, p32, bc, s);
-  if (fp32)
-    return spec ? step_t<float, true, false>(g, src, dst, words, alpha, force, walls, p64, p32, bc, s)
-                : step_t<float, false, false>(g, src, dst, words, alpha, force, walls, p64, p32, bc, s);
-  return spec ? step_t<double, true, false>(g, src, dst, words, alpha, force, walls, p64, p32, bc, s)
-              : step_t<double, false, false>(g, src, dst, words, alpha, force, walls, p64, p32, bc, s);
+  return g.aos ? step_aos<true>(g, fp32, f32math, src, dst, words, alpha, force, walls, p64, p32, bc, spec, s)
+               : step_aos<false>(g, fp32, f32math, src, dst, words, alpha, force, walls, p64, p32, bc, spec, s);
 }
 
-template <typename T, bool M32, bool ODD>
+template <typename T, bool M32, bool ODD, bool AOS>
 static hipError_t aa_t(const DenseGeom& g, void* buf, const uint32_t* words, const void* alpha, bool force, bool walls,
                        const CollideParams& p64, const CollideParams32& p32, const BoundaryArgs& bc, hipStream_t s) {
   T* ap = (T*)buf;
   const T* al = (const T*)alpha;
 #define LBM_AA(F, A, W)                                                                                        \
-  hipLaunchKernelGGL((k_aa_step<T, F, A, W, M32, ODD>), dense_grid(g), dense_block(g), 0, s, g, ap, words, al, \
+  hipLaunchKernelGGL((k_aa_step<T, F, A, W, M32, ODD, AOS>), dense_grid(g), dense_block(g), 0, s, g, ap, words, al, \
                      p64, p32, bc)
   const int key = (force ? 4 : 0) | (alpha ? 2 : 0) | (walls ? 1 : 0);
   switch (key) {
@@ -634,18 +643,26 @@ static hipError_t aa_t(const DenseGeom& g, void* buf, const uint32_t* words, con
   return hipGetLastError();
 }
 
+template <bool AOS>
+static hipError_t aa_aos(const DenseGeom& g, bool fp32, bool f32math, bool odd, void* a, const uint32_t* words,
+                         const void* alpha, bool force, bool walls, const CollideParams& p64, const CollideParams32& p32,
+                         const BoundaryArgs& bc, hipStream_t s) {
+  if (fp32 && f32math)
+    return odd ? aa_t<float, true, true, AOS>(g, a, words, alpha, force, walls, p64, p32, bc, s)
+               : aa_t<float, true, false, AOS>(g, a, words, alpha, force, walls, p64, p32, bc, s);
+  if (fp32)
+    return odd ? aa_t<float, false, true, AOS>(g, a, words, alpha, force, walls, p64, p32, bc, s)
+               : aa_t<float, false, false, AOS>(g, a, words, alpha, force, walls, p64, p32, bc, s);
+  return odd ? aa_t<double, false, true, AOS>(g, a, words, alpha, force, walls, p64, p32, bc, s)
+             : aa_t<double, false, false, AOS>(g, a, words, alpha, force, walls, p64, p32, bc, s);
+}
+
 hipError_t launch_aa_step(const DenseGeom& g, bool fp32, bool f32math, int parity, void* a, const uint32_t* words,
                           const void* alpha, bool force, bool walls, const CollideParams& p64,
                           const CollideParams32& p32, const BoundaryArgs& bc, hipStream_t s) {
   const bool odd = parity != 0;
-  if (fp32 && f32math)
-    return odd ? aa_t<float, true, true>(g, a, words, alpha, force, walls, p64, p32, bc, s)
-               : aa_t<float, true, false>(g, a, words, alpha, force, walls, p64, p32, bc, s);
-  if (fp32)
-    return odd ? aa_t<float, false, true>(g, a, words, alpha, force, walls, p64, p32, bc, s)
-               : aa_t<float, false, false>(g, a, words, alpha, force, walls, p64, p32, bc, s);
-  return odd ? aa_t<double, false, true>(g, a, words, alpha, force, walls, p64, p32, bc, s)
-             : aa_t<double, false, false>(g, a, words, alpha, force, walls, p64, p32, bc, s);
+  return g.aos ? aa_aos<true>(g, fp32, f32math, odd, a, words, alpha, force, walls, p64, p32, bc, s)
+               : aa_aos<false>(g, fp32, f32math, odd, a, words, alpha, force, walls, p64, p32, bc, s);
 }
 
 hipError_t launch_dense_pull(const DenseGeom& g, bool fp32, const void* src, double* f, const uint32_t* words,
@@ -662,19 +679,21 @@ hipError_t launch_dense_unpull(const DenseGeom& g, bool fp32, const double* f, v
   return hipGetLastError();
 }
 
+template <typename T, bool AOS>
+static void copy_t(const DenseGeom& g, bool pull, const void* src, void* dst, hipStream_t s) {
+  if (pull)
+    hipLaunchKernelGGL((k_dense_copy<T, true, AOS>), dense_grid(g), dense_block(g), 0, s, g, (const T*)src, (T*)dst);
+  else
+    hipLaunchKernelGGL((k_dense_copy<T, false, AOS>), dense_grid(g), dense_block(g), 0, s, g, (const T*)src, (T*)dst);
+}
+
 hipError_t launch_dense_copy(const DenseGeom& g, bool fp32, bool pull, const void* src, void* dst, hipStream_t s) {
   if (fp32) {
-    if (pull)
-      hipLaunchKernelGGL((k_dense_copy<float, true>), dense_grid(g), dense_block(g), 0, s, g, (const float*)src, (float*)dst);
-    else
-      hipLaunchKernelGGL((k_dense_copy<float, false>), dense_grid(g), dense_block(g), 0, s, g, (const float*)src, (float*)dst);
+    if (g.aos) copy_t<float, true>(g, pull, src, dst, s);
+    else copy_t<float, false>(g, pull, src, dst, s);
   } else {
-    if (pull)
-      hipLaunchKernelGGL((k_dense_copy<double, true>), dense_grid(g), dense_block(g), 0, s, g, (const double*)src,
-                         (double*)dst);
-    else
-      hipLaunchKernelGGL((k_dense_copy<double, false>), dense_grid(g), dense_block(g), 0, s, g, (const double*)src,
-                         (double*)dst);
+    if (g.aos) copy_t<double, true>(g, pull, src, dst, s);
+    else copy_t<double, false>(g, pull, src, dst, s);
   }
   return hipGetLastError();
 }

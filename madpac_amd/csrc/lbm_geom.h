@@ -146,22 +146,31 @@ __device__ __forceinline__ int64_t dst_off(int q, const NbrOff& o) {
 }
 
 // index of the stored value that becomes population q of cell i in the reference state
+template <bool AOS>
 __device__ __forceinline__ int64_t pull_index(int q, uint32_t w, int64_t i, int64_t N, const NbrOff& o) {
   const bool bounce = (w >> (Q - 1 - q)) & 1u;  // x - c_q = x + c_{18-q} is not fluid
-  return bounce ? (int64_t)(Q - 1 - q) * N + i : (int64_t)q * N + i + src_off(q, o);
+  return bounce ? pop_at<AOS>(N, Q - 1 - q, i) : pop_at<AOS>(N, q, i + src_off(q, o));
+}
+__device__ __forceinline__ int64_t pull_index(const DenseGeom& g, int q, uint32_t w, int64_t i, const NbrOff& o) {
+  return g.aos ? pull_index<true>(q, w, i, g.N, o) : pull_index<false>(q, w, i, g.N, o);
 }
 
 // the same for AA's SWAPPED layout (post-collision values reflected in the own slots)
+template <bool AOS>
 __device__ __forceinline__ int64_t swapped_index(int q, uint32_t w, int64_t i, int64_t N, const NbrOff& o) {
   const bool bounce = (w >> (Q - 1 - q)) & 1u;
-  return bounce ? (int64_t)q * N + i : (int64_t)(Q - 1 - q) * N + i + src_off(q, o);
+  return bounce ? pop_at<AOS>(N, q, i) : pop_at<AOS>(N, Q - 1 - q, i + src_off(q, o));
 }
 
 // index of population q of the reference state at cell i in the layout g.layout
+template <bool AOS>
 __device__ __forceinline__ int64_t state_index(const DenseGeom& g, int q, uint32_t w, int64_t i, const NbrOff& o) {
-  if (g.layout == LAYOUT_AA_NATURAL) return (int64_t)q * g.N + i;
-  if (g.layout == LAYOUT_AA_SWAPPED) return swapped_index(q, w, i, g.N, o);
-  return pull_index(q, w, i, g.N, o);
+  if (g.layout == LAYOUT_AA_NATURAL) return pop_at<AOS>(g.N, q, i);
+  if (g.layout == LAYOUT_AA_SWAPPED) return swapped_index<AOS>(q, w, i, g.N, o);
+  return pull_index<AOS>(q, w, i, g.N, o);
+}
+__device__ __forceinline__ int64_t state_index(const DenseGeom& g, int q, uint32_t w, int64_t i, const NbrOff& o) {
+  return g.aos ? state_index<true>(g, q, w, i, o) : state_index<false>(g, q, w, i, o);
 }
 
 }  // namespace lbm

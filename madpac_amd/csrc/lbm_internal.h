@@ -45,7 +45,23 @@ struct DenseGeom {
   int64_t nblocks;           // workgroups holding a cell
   int64_t nblocks_padded;    // rounded up to a multiple of NXCD
   int layout;                // how the stored state maps to the reference state (LAYOUT_*)
+  int aos;                   // populations interleaved per group of 64 stored cells (pop_at)
 };
+
+// Address of population q of stored cell i in a state buffer.  Planes (aos = 0): plane q holds
+// population q of every cell, q * N + i.  Interleaved (aos = 1): the 19 populations of each group
+// of 64 consecutive stored cells (one wavefront's cells: a row segment or a brick) lie together,
+// 19 x 64 values per group, so the loads and stores of a wavefront stay within a few kilobytes of
+// memory instead of spreading over 19 planes N cells apart.  A neighbour's address is the same
+// function of the neighbour's stored index.  Both fit in Q * N values (N >= whole groups).
+template <bool AOS>
+__host__ __device__ __forceinline__ int64_t pop_at(int64_t N, int q, int64_t i) {
+  if constexpr (AOS) return ((((i >> 6) * Q) + q) << 6) + (i & 63);
+  else return (int64_t)q * N + i;
+}
+__host__ __device__ __forceinline__ int64_t pop_at(const DenseGeom& g, int q, int64_t i) {
+  return g.aos ? pop_at<true>(g.N, q, i) : pop_at<false>(g.N, q, i);
+}
 
 // hipMalloc that leaves no error behind when it fails: HIP keeps the failure as the thread's last
 // error, and the next launch check (hipGetLastError) would report it against an unrelated kernel.
@@ -79,7 +95,7 @@ enum : int { LAYOUT_PULL = 0, LAYOUT_AA_NATURAL = 1, LAYOUT_AA_SWAPPED = 2 };
 // lattice (bricks): FP64 +6 % with 2112 cells (16.5 KiB), FP32 stable and +1 % with 64 cells
 // (256 B); pads of 1M cells were 20 % slower.
 int64_t plane_pad(size_t esize);
-DenseGeom make_geom(int nx, int ny, int nz, const int periodic[3], bool bricked, size_t esize);
+DenseGeom make_geom(int nx, int ny, int nz, const int periodic[3], bool bricked, size_t esize, bool aos);
 // Workgroup shape: `by` rows of BX cells (rows order) or `by` bricks (bricks order), by in {2, 4, 8};
 // it fixes the grid (rny, nblocks) but not the stored order.  The untuned default, BY_DEFAULT = 4,
 // was the best shape or within 1 % of it in a sweep of all tuned variants on the porous 256^3

@@ -222,20 +222,36 @@ __device__ __forceinline__ void patch_cell(const PatchDev* __restrict__ pp, cons
 }
 
 // Box tiles of TX x TY x TZ cells (cells outside the patch idle).
-template <typename T, int TX, int TY, int TZ, bool M32, bool FL>
+// The tiles of workgroup blockIdx.x: strided (t = w, w + G, ...) or, with CHUNK, the contiguous
+// range [w T / G, (w + 1) T / G).
+template <bool CHUNK>
+__device__ __forceinline__ void tile_range(int ntiles, int& first, int& end, int& stride) {
+  if (CHUNK) {
+    first = (int)((int64_t)blockIdx.x * ntiles / gridDim.x);
+    end = (int)(((int64_t)blockIdx.x + 1) * ntiles / gridDim.x);
+    stride = 1;
+  } else {
+    first = blockIdx.x;
+    end = ntiles;
+    stride = gridDim.x;
+  }
+}
+
+template <typename T, int TX, int TY, int TZ, bool M32, bool FL, bool CHUNK>
 __global__ __launch_bounds__(256) void k_patch_step(const PatchDev* __restrict__ patches,
                                                     const TileDev* __restrict__ tiles, int ntiles,
                                                     const uint32_t* __restrict__ words,
                                                     const T* __restrict__ src, T* __restrict__ dst) {
   constexpr int TC = TX * TY * TZ;
-  int t = blockIdx.x;
-  if (t >= ntiles) return;
+  int t, end, stride;
+  tile_range<CHUNK>(ntiles, t, end, stride);
+  if (t >= end) return;
   TileDev tl = tiles[t];
   for (;;) {
     // prefetch the next tile's record; it is waited for only at the end of this iteration
-    const int tn = t + gridDim.x;
+    const int tn = t + stride;
     TileDev next{};
-    if (tn < ntiles) next = tiles[tn];
+    if (tn < end) next = tiles[tn];
     // per-lane lookups go through the global table (a private copy would live in scratch memory)
     const PatchDev* __restrict__ pp = patches + tl.patch;
     const int n = tl.n;
@@ -244,7 +260,7 @@ __global__ __launch_bounds__(256) void k_patch_step(const PatchDev* __restrict__
       if (x >= n || y >= n || z >= n) continue;
       patch_cell<T, M32, FL>(pp, words, src + tl.base, dst + tl.base, dst, n, tl.omega, x, y, z);
     }
-    if (tn >= ntiles) break;
+    if (tn >= end) break;
     t = tn;
     tl = next;
   }
@@ -267,18 +283,19 @@ __device__ __forceinline__ void split_cell(int c, int n_rt, int& x, int& y, int&
 template <typename T>
 constexpr int patch_waves_per_simd() { return sizeof(T) == 4 ? 8 : 4; }
 
-template <typename T, bool M32, bool FL>
+template <typename T, bool M32, bool FL, bool CHUNK>
 __global__ __launch_bounds__(256, patch_waves_per_simd<T>()) void k_patch_step_lin(const PatchDev* __restrict__ patches,
                                                         const TileDev* __restrict__ tiles, int ntiles,
                                                         const uint32_t* __restrict__ words,
                                                         const T* __restrict__ src, T* __restrict__ dst) {
-  int t = blockIdx.x;
-  if (t >= ntiles) return;
+  int t, end, stride;
+  tile_range<CHUNK>(ntiles, t, end, stride);
+  if (t >= end) return;
   TileDev tl = tiles[t];
   for (;;) {
-    const int tn = t + gridDim.x;
+    const int tn = t + stride;
     TileDev next{};
-    if (tn < ntiles) next = tiles[tn];
+    if (tn < end) next = tiles[tn];
     const PatchDev* __restrict__ pp = patches + tl.patch;
     const int n = tl.n;
     const int c = tl.x0 + (int)threadIdx.x;
@@ -294,7 +311,7 @@ __global__ __launch_bounds__(256, patch_waves_per_simd<T>()) void k_patch_step_l
       }
       patch_cell<T, M32, FL>(pp, words, src + tl.base, dst + tl.base, dst, n, tl.omega, x, y, z);
     }
-    if (tn >= ntiles) break;
+    if (tn >= end) break;
     t = tn;
     tl = next;
   }
@@ -538,7 +555,7 @@ struct lbm_patchset {
   int cur = 0;
   TileDev* d_tiles = nullptr;
   int ntiles = 0;
-  lbm_tiling tiling{16, 16, 1, 4};
+  lbm_tiling tiling{16, 16, 1, 4, 0};
   int cus = 256;
   double* partials = nullptr;
   int64_t npartials = 0;
@@ -603,8 +620,8 @@ int build_tiles(lbm_patchset* ps, const lbm_tiling& t) {
   return LBM_OK;
 }
 
-template <typename T, bool M32, bool FL>
-hipError_t launch_step_t(lbm_patchset* ps, const void* src, void* dst) {
+template <typename T, bool M32, bool FL, bool CHUNK>
+hipError_t launch_step_c(lbm_patchset* ps, const void* src, void* dst) {
   const lbm_tiling& t = ps->tiling;
   // blocks_per_cu = 0: one workgroup per tile (the loop body runs once)
   const int wgs = t.blocks_per_cu == 0 ? ps->ntiles : std::min(ps->ntiles, ps->cus * t.blocks_per_cu);
@@ -613,7 +630,7 @@ hipError_t launch_step_t(lbm_patchset* ps, const void* src, void* dst) {
   T* d = (T*)dst;
 #define LBM_PATCH(X, Y, Z)                                                                                     \
   if (t.tile_x == X && t.tile_y == Y && t.tile_z == Z) {                                                       \
-    hipLaunchKernelGGL((k_patch_step<T, X, Y, Z, M32, FL>), grid, block, 0, ps->stream, ps->d_patches,          \
+    hipLaunchKernelGGL((k_patch_step<T, X, Y, Z, M32, FL, CHUNK>), grid, block, 0, ps->stream, ps->d_patches,   \
                        ps->d_tiles, ps->ntiles, ps->d_words, s, d);                                            \
     return hipGetLastError();                                                                                  \
   }
@@ -624,11 +641,18 @@ hipError_t launch_step_t(lbm_patchset* ps, const void* src, void* dst) {
   LBM_PATCH(16, 4, 4)
 #undef LBM_PATCH
   if (linear_tiles(t)) {
-    hipLaunchKernelGGL((k_patch_step_lin<T, M32, FL>), grid, block, 0, ps->stream, ps->d_patches, ps->d_tiles,
-                       ps->ntiles, ps->d_words, s, d);
+    hipLaunchKernelGGL((k_patch_step_lin<T, M32, FL, CHUNK>), grid, block, 0, ps->stream, ps->d_patches,
+                       ps->d_tiles, ps->ntiles, ps->d_words, s, d);
     return hipGetLastError();
   }
   return hipErrorInvalidValue;
+}
+
+// one workgroup per tile runs the strided kernel (each workgroup's loop body runs once)
+template <typename T, bool M32, bool FL>
+hipError_t launch_step_t(lbm_patchset* ps, const void* src, void* dst) {
+  return (ps->tiling.chunked && ps->tiling.blocks_per_cu) ? launch_step_c<T, M32, FL, true>(ps, src, dst)
+                                                           : launch_step_c<T, M32, FL, false>(ps, src, dst);
 }
 
 template <bool FL>
@@ -881,6 +905,7 @@ int lbm_patches_set_tiling(lbm_patchset* ps, const lbm_tiling* t) {
   if (!ps) return LBM_ERR_ARG;
   if (!t || !shape_supported(t)) return pfail(ps, LBM_ERR_ARG, "unsupported tile shape");
   if (t->blocks_per_cu < 0 || t->blocks_per_cu > 64) return pfail(ps, LBM_ERR_ARG, "blocks_per_cu must be in 0..64");
+  if (t->chunked != 0 && t->chunked != 1) return pfail(ps, LBM_ERR_ARG, "chunked must be 0 or 1");
   PCHK(ps, hipSetDevice(ps->device));
   return build_tiles(ps, *t);
 }
@@ -936,17 +961,20 @@ int lbm_patches_autotune(lbm_patchset* ps, lbm_tiling* chosen) {
   int rc = LBM_OK;
   for (const Shape& sh : kShapes) {
     for (int bpc : bpcs) {
-      lbm_tiling t{sh.x, sh.y, sh.z, bpc};
-      rc = build_tiles(ps, t);
-      if (rc) break;
-      float ms = 0.f;
-      rc = lbm_patches_step(ps, 2, 0, nullptr);
-      if (!rc) rc = lbm_patches_step(ps, 6, 0, &ms);
-      if (rc) break;
-      if (ms < best_ms) {
-        best_ms = ms;
-        best = t;
+      for (int chunked = 0; chunked <= (bpc ? 1 : 0) && !rc; chunked++) {
+        lbm_tiling t{sh.x, sh.y, sh.z, bpc, chunked};
+        rc = build_tiles(ps, t);
+        if (rc) break;
+        float ms = 0.f;
+        rc = lbm_patches_step(ps, 2, 0, nullptr);
+        if (!rc) rc = lbm_patches_step(ps, 6, 0, &ms);
+        if (rc) break;
+        if (ms < best_ms) {
+          best_ms = ms;
+          best = t;
+        }
       }
+      if (rc) break;
     }
     if (rc) break;
   }

@@ -20,6 +20,7 @@ FP32_SHIFTED = 1       # FP32 state; FP64 moments, equilibrium and relaxation pr
 FP32_SHIFTED_F32 = 2   # FP32 state, FP32 collision arithmetic
 LAYOUT_TWO_BUFFER, LAYOUT_AA = 0, 1
 CELL_ORDER_AUTO, CELL_ORDER_ROWS, CELL_ORDER_BRICKS = 0, 1, 2
+POPULATIONS_AUTO, POPULATIONS_PLANES, POPULATIONS_INTERLEAVED = 0, 1, 2
 # boundary-condition types of a class (include/lbm_capi.h)
 BC_MOVING_WALL, BC_PRESSURE_OUTLET, BC_PRESSURE_INLET, BC_VELOCITY_INLET = 0, 1, 2, 3
 BC_CELL_VELOCITY = 16
@@ -60,7 +61,7 @@ class Config(ctypes.Structure):
     _fields_ = [("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int),
                 ("periodic", ctypes.c_int * 3), ("precision", ctypes.c_int), ("tau", ctypes.c_double),
                 ("force", ctypes.c_double * 3), ("device", ctypes.c_int), ("layout", ctypes.c_int),
-                ("cell_order", ctypes.c_int)]
+                ("cell_order", ctypes.c_int), ("populations", ctypes.c_int)]
 
 
 class AdjointStats(ctypes.Structure):
@@ -81,7 +82,7 @@ class AmrLink(ctypes.Structure):
 
 class Tiling(ctypes.Structure):
     _fields_ = [("tile_x", ctypes.c_int), ("tile_y", ctypes.c_int), ("tile_z", ctypes.c_int),
-                ("blocks_per_cu", ctypes.c_int)]
+                ("blocks_per_cu", ctypes.c_int), ("chunked", ctypes.c_int)]
 
 
 _lib = None
@@ -187,10 +188,11 @@ class Lattice:
     """A dense D3Q19 lattice on one GPU (an lbm_ctx)."""
 
     def __init__(self, nx, ny, nz, tau, precision=FP64, periodic=(False, False, False), force=(0.0, 0.0, 0.0),
-                 device=0, layout=LAYOUT_TWO_BUFFER, cell_order=CELL_ORDER_AUTO):
+                 device=0, layout=LAYOUT_TWO_BUFFER, cell_order=CELL_ORDER_AUTO, populations=POPULATIONS_AUTO):
         self._L = load_library()
         cfg = Config(nx, ny, nz, (ctypes.c_int * 3)(*[int(p) for p in periodic]), int(precision), float(tau),
-                     (ctypes.c_double * 3)(*[float(v) for v in force]), int(device), int(layout), int(cell_order))
+                     (ctypes.c_double * 3)(*[float(v) for v in force]), int(device), int(layout), int(cell_order),
+                     int(populations))
         h = ctypes.c_void_p()
         rc = self._L.lbm_create(ctypes.byref(h), ctypes.byref(cfg))
         if rc != 0:
@@ -354,6 +356,12 @@ class Lattice:
         self._check(self._L.lbm_reduce_moments(self._h, _dp(out)), "lbm_reduce_moments")
         return out
 
+    def moments_noalloc(self):
+        """moments() through lbm_reduce_moments_noalloc (no device allocation; same bits)."""
+        out = np.empty(4)
+        self._check(self._L.lbm_reduce_moments_noalloc(self._h, _dp(out)), "lbm_reduce_moments_noalloc")
+        return out
+
     def measure_copy(self, steps, pull=False):
         """Device ms of `steps` launches of the step's bandwidth reference (19-plane copy, or the
         pull pattern with pull=True) over every stored cell; the state is unchanged."""
@@ -361,12 +369,6 @@ class Lattice:
         self._check(self._L.lbm_measure_copy(self._h, 1 if pull else 0, int(steps), ctypes.byref(ms)),
                     "lbm_measure_copy")
         return ms.value
-
-    def moments_noalloc(self):
-        """moments() through lbm_reduce_moments_noalloc (no device allocation; same bits)."""
-        out = np.empty(4)
-        self._check(self._L.lbm_reduce_moments_noalloc(self._h, _dp(out)), "lbm_reduce_moments_noalloc")
-        return out
 
     def kinetic_energy(self):
         ke = ctypes.c_double()
@@ -468,10 +470,10 @@ class PatchSet:
     def autotune(self):
         t = Tiling()
         self._check(self._L.lbm_patches_autotune(self._h, ctypes.byref(t)), "lbm_patches_autotune")
-        return (t.tile_x, t.tile_y, t.tile_z, t.blocks_per_cu)
+        return (t.tile_x, t.tile_y, t.tile_z, t.blocks_per_cu, t.chunked)
 
-    def set_tiling(self, tx, ty, tz, blocks_per_cu):
-        t = Tiling(int(tx), int(ty), int(tz), int(blocks_per_cu))
+    def set_tiling(self, tx, ty, tz, blocks_per_cu, chunked=0):
+        t = Tiling(int(tx), int(ty), int(tz), int(blocks_per_cu), int(chunked))
         self._check(self._L.lbm_patches_set_tiling(self._h, ctypes.byref(t)), "lbm_patches_set_tiling")
 
     def step(self, n=1, graph=False, timed=False):

@@ -92,7 +92,7 @@ def test_graph_replay_and_autotune_are_bit_identical():
             ps.init(u=u)
             if mode == "tuned":
                 t = ps.autotune()
-                assert len(t) == 4
+                assert len(t) == 5
             g = mode != "plain"
             ps.step(4, graph=g)
             mid = ps.get_populations()  # readout between replays
@@ -114,7 +114,8 @@ def test_every_tiling_is_bit_identical(prec):
     # linear runs and box tiles, persistent grids and one workgroup per tile: same bits
     table, info, u = _setup([((2, 1, 1), 24), ((2, 2, 2), 8), ((1, 1, 1), 32), ((2, 1, 1), 16)], seed=9)
     out = []
-    for tiling in ((32, 8, 1, 4), (256, 0, 0, 0), (256, 0, 0, 8), (8, 8, 4, 0), (16, 16, 1, 2), (16, 4, 4, 64)):
+    for tiling in ((32, 8, 1, 4), (256, 0, 0, 0), (256, 0, 0, 8), (8, 8, 4, 0), (16, 16, 1, 2), (16, 4, 4, 64),
+                   (256, 0, 0, 8, 1), (256, 0, 0, 64, 1), (16, 8, 2, 4, 1)):
         with lbm.PatchSet(table, precision=prec) as ps:
             ps.init(u=u)
             ps.set_tiling(*tiling)
