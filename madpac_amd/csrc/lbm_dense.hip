@@ -38,7 +38,8 @@ DenseGeom make_geom(int nx, int ny, int nz, const int periodic[3], bool bricked,
   g.N = (bricked ? bricks * 64 : g.Nu) + plane_pad(esize);
   set_workgroup_rows(g, BY_DEFAULT);
   g.layout = LAYOUT_PULL;
-  g.aos = aos ? 1 : 0;
+  // interleaving needs whole 64-cell groups per brick or per row segment (addresses separable per axis)
+  g.aos = (aos && (bricked || nx % BX == 0)) ? 1 : 0;
   return g;
 }
 
@@ -146,13 +147,12 @@ __global__ __launch_bounds__(BX * BY_MAX) void k_dense_step(DenseGeom g, const T
                                                         CollideParams32 p32, BoundaryArgs bc) {
   const CellPos c = map_cell(g);
   if (!c.valid) return;
-  const NbrOff o = nbr_offsets(g, c);
-  const int64_t N = g.N;
+  const CellAddr<AOS> A = cell_addr<AOS>(g, c);
   T f[Q];
   uint32_t w;
   if (SPEC) {
 #pragma unroll
-    for (int q = 0; q < Q; q++) f[q] = src[pop_at<AOS>(N, q, c.i + src_off(q, o))];
+    for (int q = 0; q < Q; q++) f[q] = src[A.pull(q)];
     w = words[c.i];
     const bool fl = (w & FLUID) != 0;
     const unsigned long long fluid = __ballot(fl);  // every lane votes before any returns
@@ -161,7 +161,7 @@ __global__ __launch_bounds__(BX * BY_MAX) void k_dense_step(DenseGeom g, const T
       if (w & 0x7FFFFu) {
 #pragma unroll
         for (int q = 0; q < Q; q++)
-          if ((w >> (Q - 1 - q)) & 1u) f[q] = src[pop_at<AOS>(N, Q - 1 - q, c.i)];
+          if ((w >> (Q - 1 - q)) & 1u) f[q] = src[A.own(Q - 1 - q)];
       }
       collide_cell_bc<T, FORCE, ALPHA, WALLS, M32>(f, w, c.i, alpha, p64, p32, bc);
     } else {
@@ -175,7 +175,7 @@ __global__ __launch_bounds__(BX * BY_MAX) void k_dense_step(DenseGeom g, const T
     if (!fl && !line_has_fluid<T>(fluid)) return;
     if (fl) {
 #pragma unroll
-      for (int q = 0; q < Q; q++) f[q] = src[pull_index<AOS>(q, w, c.i, N, o)];
+      for (int q = 0; q < Q; q++) f[q] = src[((w >> (Q - 1 - q)) & 1u) ? A.own(Q - 1 - q) : A.pull(q)];
       collide_cell_bc<T, FORCE, ALPHA, WALLS, M32>(f, w, c.i, alpha, p64, p32, bc);
     } else {
 #pragma unroll
@@ -183,7 +183,7 @@ __global__ __launch_bounds__(BX * BY_MAX) void k_dense_step(DenseGeom g, const T
     }
   }
 #pragma unroll
-  for (int q = 0; q < Q; q++) __builtin_nontemporal_store(f[q], &dst[pop_at<AOS>(N, q, c.i)]);
+  for (int q = 0; q < Q; q++) __builtin_nontemporal_store(f[q], &dst[A.own(q)]);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -202,18 +202,19 @@ __global__ __launch_bounds__(BX * BY_MAX) void k_aa_step(DenseGeom g, T* __restr
   const CellPos c = map_cell(g);
   if (!c.valid) return;
   const uint32_t w = words[c.i];
-  const int64_t N = g.N;
   const bool fl = (w & FLUID) != 0;
   const unsigned long long fluid = __ballot(fl);  // every lane votes before any returns
   // Solid lanes complete the lines of fluid neighbours with zeros, as in k_dense_step: a solid
   // cell's slots are dead in both steps (no fluid cell reads them, and an odd-step push toward a
   // solid cell goes to the pusher's own reflected slot instead), so writing them races with nothing.
   if (!fl && !line_has_fluid<T>(fluid)) return;
-  const NbrOff o = nbr_offsets(g, c);
+  CellAddr<AOS> A = cell_addr<AOS>(g, c);
   T f[Q];
   if (fl) {
+    // odd: f_q(x) = a_{18-q}(x - c_q), or a_q(x) across a bounce-back link
 #pragma unroll
-    for (int q = 0; q < Q; q++) f[q] = ODD ? a[swapped_index<AOS>(q, w, c.i, N, o)] : a[pop_at<AOS>(N, q, c.i)];
+    for (int q = 0; q < Q; q++)
+      f[q] = ODD ? a[((w >> (Q - 1 - q)) & 1u) ? A.own(q) : A.own(Q - 1 - q) + src_off(q, A.o)] : a[A.own(q)];
     collide_cell_bc<T, FORCE, ALPHA, WALLS, M32>(f, w, c.i, alpha, p64, p32, bc);
   } else {
 #pragma unroll
@@ -221,15 +222,13 @@ __global__ __launch_bounds__(BX * BY_MAX) void k_aa_step(DenseGeom g, T* __restr
   }
   // The stores go to the slots just loaded.  Left to itself the compiler keeps the 19 load
   // addresses live through the collision (38 more VGPRs, one wave less per SIMD); an opaque copy
-  // of the cell index makes it recompute them here instead.
-  int64_t i = c.i;
-  asm volatile("" : "+v"(i));
+  // of the cell's base address makes it recompute them here instead.
+  asm volatile("" : "+v"(A.base));
 #pragma unroll
   for (int q = 0; q < Q; q++) {
     // x + c_q is not fluid (a solid lane stores its zeros to its own slots)
     const bool bounce = ((fl ? w : 0x7FFFFu) >> q) & 1u;
-    const int64_t di = !ODD ? pop_at<AOS>(N, Q - 1 - q, i)
-                            : (bounce ? pop_at<AOS>(N, Q - 1 - q, i) : pop_at<AOS>(N, q, i + dst_off(q, o)));
+    const int64_t di = (!ODD || bounce) ? A.own(Q - 1 - q) : A.push(q);
     // An even step writes whole lines of its own slots.  An odd step's pushes fill each line from
     // up to four wavefronts: ordinary stores let L2 merge the parts before write-back (measured
     // 5-13 % faster on a porous 256^3 channel than streaming stores, which write each part out).
@@ -287,18 +286,12 @@ template <typename T, bool PULL, bool AOS>
 __global__ __launch_bounds__(BX * BY_MAX) void k_dense_copy(DenseGeom g, const T* __restrict__ src, T* __restrict__ dst) {
   const CellPos c = map_cell(g);
   if (!c.valid) return;
-  const int64_t N = g.N;
+  const CellAddr<AOS> A = cell_addr<AOS>(g, c);
   T f[Q];
-  if (PULL) {
-    const NbrOff o = nbr_offsets(g, c);
 #pragma unroll
-    for (int q = 0; q < Q; q++) f[q] = src[pop_at<AOS>(N, q, c.i + src_off(q, o))];
-  } else {
+  for (int q = 0; q < Q; q++) f[q] = src[PULL ? A.pull(q) : A.own(q)];
 #pragma unroll
-    for (int q = 0; q < Q; q++) f[q] = src[pop_at<AOS>(N, q, c.i)];
-  }
-#pragma unroll
-  for (int q = 0; q < Q; q++) __builtin_nontemporal_store(f[q], &dst[pop_at<AOS>(N, q, c.i)]);
+  for (int q = 0; q < Q; q++) __builtin_nontemporal_store(f[q], &dst[A.own(q)]);
 }
 
 // The same state in another layout of the same cell order (e.g. pull form <-> AA NATURAL): each

@@ -122,6 +122,35 @@ __device__ __forceinline__ NbrOff nbr_offsets(const DenseGeom& g, const CellPos&
   return o;
 }
 
+// Neighbour offsets in the address space of interleaved populations (aos = 1).  There the address
+// of population q of the cell at (x, y, z) is a sum of one term per coordinate plus 64 q whenever a
+// 64-cell group is a brick (bricks) or a row segment of whole groups (rows with nx a multiple of 64,
+// the only interleaved rows make_geom allows): bricks: x (x >> 2) 1216 + (x & 3), y (y >> 2) 1216 bnx
+// + 4 (y & 3), z (z >> 2) 1216 bnx bny + 16 (z & 3); rows: x (x >> 6) 1216 + (x & 63), y 1216 y nx/64,
+// z 1216 z ny nx/64 (1216 = 19 x 64).  So a neighbour's address is the cell's plus per-axis deltas,
+// exactly as a neighbour's index is in the plane layout, and the kernels pay no more per access.
+__device__ __forceinline__ int64_t aos_term_x(const DenseGeom& g, int x) {
+  return g.bricked ? (int64_t)(x >> 2) * (Q * 64) + (x & 3) : (int64_t)(x >> 6) * (Q * 64) + (x & 63);
+}
+__device__ __forceinline__ int64_t aos_term_y(const DenseGeom& g, int y) {
+  return g.bricked ? (int64_t)(y >> 2) * (Q * 64) * g.bnx + ((y & 3) << 2) : (int64_t)y * (Q * 64) * (g.nx >> 6);
+}
+__device__ __forceinline__ int64_t aos_term_z(const DenseGeom& g, int z) {
+  return g.bricked ? (int64_t)(z >> 2) * (Q * 64) * g.bnx * g.bny + ((z & 3) << 4)
+                   : (int64_t)z * (Q * 64) * (g.nx >> 6) * g.ny;
+}
+__device__ __forceinline__ NbrOff aos_offsets(const DenseGeom& g, const CellPos& c) {
+  NbrOff o;
+  const int64_t tx = aos_term_x(g, c.x), ty = aos_term_y(g, c.y), tz = aos_term_z(g, c.z);
+  o.xm = c.x == 0 ? (g.px ? aos_term_x(g, g.nx - 1) - tx : 0) : aos_term_x(g, c.x - 1) - tx;
+  o.xp = c.x == g.nx - 1 ? (g.px ? -tx + aos_term_x(g, 0) : 0) : aos_term_x(g, c.x + 1) - tx;
+  o.ym = c.y == 0 ? (g.py ? aos_term_y(g, g.ny - 1) - ty : 0) : aos_term_y(g, c.y - 1) - ty;
+  o.yp = c.y == g.ny - 1 ? (g.py ? -ty + aos_term_y(g, 0) : 0) : aos_term_y(g, c.y + 1) - ty;
+  o.zm = c.z == 0 ? (g.pz ? aos_term_z(g, g.nz - 1) - tz : 0) : aos_term_z(g, c.z - 1) - tz;
+  o.zp = c.z == g.nz - 1 ? (g.pz ? -tz + aos_term_z(g, 0) : 0) : aos_term_z(g, c.z + 1) - tz;
+  return o;
+}
+
 // offset of the cell x - c_q (the source of population q in a pull)
 __device__ __forceinline__ int64_t src_off(int q, const NbrOff& o) {
   int64_t d = 0;
@@ -143,6 +172,32 @@ __device__ __forceinline__ int64_t dst_off(int q, const NbrOff& o) {
   if (CZ(q) == 1) d += o.zp;
   if (CZ(q) == -1) d += o.zm;
   return d;
+}
+
+// Addresses of one cell's populations and of its neighbours' in the hot kernels: own(q) is the
+// cell's population q, own(q) + src_off(q, o) / dst_off(q, o) that of the neighbour x - c_q / x + c_q.
+// Planes: base = i, o = index offsets, own(q) = q N + i.  Interleaved: base = the address of the
+// cell's population 0, o = aos_offsets, own(q) = base + 64 q.
+template <bool AOS>
+struct CellAddr {
+  int64_t base, N;
+  NbrOff o;
+  __device__ __forceinline__ int64_t own(int q) const { return AOS ? base + 64 * q : (int64_t)q * N + base; }
+  __device__ __forceinline__ int64_t pull(int q) const { return own(q) + src_off(q, o); }
+  __device__ __forceinline__ int64_t push(int q) const { return own(q) + dst_off(q, o); }
+};
+template <bool AOS>
+__device__ __forceinline__ CellAddr<AOS> cell_addr(const DenseGeom& g, const CellPos& c) {
+  CellAddr<AOS> a;
+  a.N = g.N;
+  if (AOS) {
+    a.base = pop_at<true>(g.N, 0, c.i);
+    a.o = aos_offsets(g, c);
+  } else {
+    a.base = c.i;
+    a.o = nbr_offsets(g, c);
+  }
+  return a;
 }
 
 // index of the stored value that becomes population q of cell i in the reference state
