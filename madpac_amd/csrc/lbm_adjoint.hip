@@ -42,8 +42,10 @@ struct AdjointState {
 };
 
 // ---------------------------------------------------------------------------------------------
-// dJ/df of J = cs^2 (<rho>_xin - <rho>_xout): uniform per population on the two planes.
-template <typename L>
+// dJ/df of J = cs^2 (<rho>_xin - <rho>_xout): uniform per population on the two planes.  The adjoint
+// states lambda use the state's arrangement of populations (planes or interleaved), so one set of
+// addresses serves the state and the adjoint in k_adj_step.
+template <typename L, bool AOS>
 __global__ __launch_bounds__(BX * BY_MAX) void k_adj_terminal(DenseGeom g, const uint32_t* __restrict__ words, int xin,
                                                           int xout, double vin, double vout, L* __restrict__ lam) {
   const CellPos c = map_cell(g);
@@ -52,12 +54,13 @@ __global__ __launch_bounds__(BX * BY_MAX) void k_adj_terminal(DenseGeom g, const
   double v = 0.0;
   if (fl && c.x == xin) v += vin;
   if (fl && c.x == xout) v += vout;
+  const int64_t b = AOS ? pop_at<true>(g.N, 0, c.i) : c.i;
 #pragma unroll
-  for (int q = 0; q < Q; q++) lam[(int64_t)q * g.N + c.i] = (L)v;
+  for (int q = 0; q < Q; q++) lam[AOS ? b + 64 * q : (int64_t)q * g.N + b] = (L)v;
 }
 
 // The adjoint step.  T: stored-state type; L: adjoint type (double or float); gradient in double.
-template <typename T, bool FORCE, bool ALPHA, bool WALLS>
+template <typename T, bool FORCE, bool ALPHA, bool WALLS, bool AOS>
 __global__ __launch_bounds__(BX * BY_MAX) void k_adj_step(DenseGeom g, const T* __restrict__ s,
                                                       const T* __restrict__ lam_out, T* __restrict__ lam_in,
                                                       const uint32_t* __restrict__ words, const T* __restrict__ alpha,
@@ -66,19 +69,18 @@ __global__ __launch_bounds__(BX * BY_MAX) void k_adj_step(DenseGeom g, const T* 
   const CellPos c = map_cell(g);
   if (!c.valid) return;
   const uint32_t w = words[c.i];
-  const int64_t N = g.N;
+  const CellAddr<AOS> Ad = cell_addr<AOS>(g, c);
   if (!(w & FLUID)) {
 #pragma unroll
-    for (int q = 0; q < Q; q++) lam_in[(int64_t)q * N + c.i] = T(0);
+    for (int q = 0; q < Q; q++) lam_in[Ad.own(q)] = T(0);
     return;
   }
-  const NbrOff o = nbr_offsets(g, c);
   T f[Q], mu[Q];
 #pragma unroll
   for (int q = 0; q < Q; q++) {
-    f[q] = s[pull_index(g, q, w, c.i, o)];
+    f[q] = s[((w >> (Q - 1 - q)) & 1u) ? Ad.own(Q - 1 - q) : Ad.pull(q)];
     const bool bounce = (w >> q) & 1u;  // x + c_q is not fluid
-    mu[q] = bounce ? lam_out[(int64_t)(Q - 1 - q) * N + c.i] : lam_out[(int64_t)q * N + c.i + dst_off(q, o)];
+    mu[q] = bounce ? lam_out[Ad.own(Q - 1 - q)] : lam_out[Ad.push(q)];
   }
   constexpr bool D = sizeof(T) == 8;
   using R = T;  // arithmetic type
@@ -170,7 +172,7 @@ __global__ __launch_bounds__(BX * BY_MAX) void k_adj_step(DenseGeom g, const T* 
     if (CY(q) == -1) l -= B1;
     if (CZ(q) == 1) l += B2;
     if (CZ(q) == -1) l -= B2;
-    lam_in[(int64_t)q * N + c.i] = l;
+    lam_in[Ad.own(q)] = l;
   }
   if (ALPHA) {
     R ga = R(0);
@@ -181,7 +183,7 @@ __global__ __launch_bounds__(BX * BY_MAX) void k_adj_step(DenseGeom g, const T* 
   }
 }
 
-template <typename T>
+template <typename T, bool AOS>
 static hipError_t adj_step_t(const DenseGeom& g, const void* s, const void* lo, void* li, const uint32_t* words,
                              const void* alpha, double* galpha, bool force, bool walls, const CollideParams& p,
                              const BoundaryClass* bct, hipStream_t st) {
@@ -191,7 +193,7 @@ static hipError_t adj_step_t(const DenseGeom& g, const void* s, const void* lo, 
   T* lip = (T*)li;
   const T* ap = (const T*)alpha;
 #define LBM_ADJ(F, A, W) \
-  hipLaunchKernelGGL((k_adj_step<T, F, A, W>), grid, block, 0, st, g, sp, lop, lip, words, ap, galpha, p, bct)
+  hipLaunchKernelGGL((k_adj_step<T, F, A, W, AOS>), grid, block, 0, st, g, sp, lop, lip, words, ap, galpha, p, bct)
   const int key = (force ? 4 : 0) | (alpha ? 2 : 0) | (walls ? 1 : 0);
   switch (key) {
     case 0: LBM_ADJ(false, false, false); break;
@@ -468,12 +470,21 @@ static int adjoint_two_buffer(lbm_ctx* c, int nsteps, int snapshots, int xin, in
   const double vin = (1.0 / 3.0) / hsum[1], vout = -(1.0 / 3.0) / hsum[3];
   int lc = 0;
   const dim3 grid((unsigned)c->g.nblocks_padded), block(BX, c->g.by);
-  if (c->fp32)
-    hipLaunchKernelGGL(k_adj_terminal<float>, grid, block, 0, st, c->g, c->words, xin, xout, vin, vout,
-                       (float*)a->lam[lc]);
-  else
-    hipLaunchKernelGGL(k_adj_terminal<double>, grid, block, 0, st, c->g, c->words, xin, xout, vin, vout,
-                       (double*)a->lam[lc]);
+  if (c->fp32) {
+    if (c->g.aos)
+      hipLaunchKernelGGL((k_adj_terminal<float, true>), grid, block, 0, st, c->g, c->words, xin, xout, vin, vout,
+                         (float*)a->lam[lc]);
+    else
+      hipLaunchKernelGGL((k_adj_terminal<float, false>), grid, block, 0, st, c->g, c->words, xin, xout, vin, vout,
+                         (float*)a->lam[lc]);
+  } else {
+    if (c->g.aos)
+      hipLaunchKernelGGL((k_adj_terminal<double, true>), grid, block, 0, st, c->g, c->words, xin, xout, vin, vout,
+                         (double*)a->lam[lc]);
+    else
+      hipLaunchKernelGGL((k_adj_terminal<double, false>), grid, block, 0, st, c->g, c->words, xin, xout, vin, vout,
+                         (double*)a->lam[lc]);
+  }
   ACHK(hipGetLastError());
   ACHK(hipMemsetAsync(a->galpha, 0, (size_t)N * sizeof(double), st));
 
@@ -506,10 +517,14 @@ static int adjoint_two_buffer(lbm_ctx* c, int nsteps, int snapshots, int xin, in
   auto adj_step = [&](const void* sp) -> int {
     hipError_t e = timer.begin(st, PhaseTimer::ADJOINT);
     if (e == hipSuccess)
-      e = c->fp32 ? adj_step_t<float>(c->g, sp, a->lam[lc], a->lam[lc ^ 1], c->words, c->alpha, a->galpha, c->force,
-                                       c->walls, c->p64, c->d_bct, st)
-                  : adj_step_t<double>(c->g, sp, a->lam[lc], a->lam[lc ^ 1], c->words, c->alpha, a->galpha, c->force,
-                                        c->walls, c->p64, c->d_bct, st);
+      e = c->fp32 ? (c->g.aos ? adj_step_t<float, true>(c->g, sp, a->lam[lc], a->lam[lc ^ 1], c->words, c->alpha,
+                                                         a->galpha, c->force, c->walls, c->p64, c->d_bct, st)
+                               : adj_step_t<float, false>(c->g, sp, a->lam[lc], a->lam[lc ^ 1], c->words, c->alpha,
+                                                          a->galpha, c->force, c->walls, c->p64, c->d_bct, st))
+                  : (c->g.aos ? adj_step_t<double, true>(c->g, sp, a->lam[lc], a->lam[lc ^ 1], c->words, c->alpha,
+                                                          a->galpha, c->force, c->walls, c->p64, c->d_bct, st)
+                               : adj_step_t<double, false>(c->g, sp, a->lam[lc], a->lam[lc ^ 1], c->words, c->alpha,
+                                                           a->galpha, c->force, c->walls, c->p64, c->d_bct, st));
     if (e == hipSuccess) e = timer.end(st);
     if (e != hipSuccess) return adj_fail(c, LBM_ERR_HIP, std::string("adjoint step: ") + hipGetErrorString(e));
     lc ^= 1;
