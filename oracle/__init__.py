@@ -120,6 +120,16 @@ def lib():
         L.lbo_amr_find_overlap.argtypes = [dp, u8]
         L.lbo_amr_prolongate.argtypes = [ip, dp, dp, dp, dp, u8]
         L.lbo_amr_restrict.argtypes = [ip, dp, dp]
+        L.lbo_block_cell_values.argtypes = [u8, u8, dp, dp, dp]
+        L.lbo_block_collide.argtypes = [u8, u8, dp, dp, ctypes.c_double, dp]
+        L.lbo_block_boundary.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, dp, ctypes.c_double, dp, dp, dp]
+        L.lbo_block_local_index.restype = ctypes.c_int
+        L.lbo_block_local_index.argtypes = [ip, ctypes.c_int]
+        L.lbo_block_stream_local.argtypes = [u8, dp]
+        L.lbo_block_swap_inside.argtypes = [dp]
+        L.lbo_block_nonlocal_index.restype = ctypes.c_int
+        L.lbo_block_nonlocal_index.argtypes = [ip, ctypes.c_int]
+        L.lbo_block_stream_nonlocal.argtypes = [ctypes.POINTER(u8), ctypes.POINTER(dp)]
         _lib = L
     return _lib
 
@@ -352,3 +362,88 @@ def amr_restrict(pos, coarse, fine):
     assert coarse.size == fine.size == BLOCK_CELLS * Q
     lib().lbo_amr_restrict(p, _dp(coarse), _dp(fine))
     return coarse
+
+
+# ---- the block solver's per-block and per-Peano-cell operators (6^3 blocks, AoS pdf[19 * cell + q]) ----
+BLOCK_PDFS = BLOCK_CELLS * Q
+
+
+def _block(a, n, dtype=np.float64):
+    a = np.ascontiguousarray(a, dtype=dtype).reshape(-1)
+    assert a.size == n
+    return a
+
+
+def _flags(a):
+    return np.ascontiguousarray(np.asarray(a).reshape(-1) != 0, dtype=np.uint8)
+
+
+def block_cell_values(inner, pdf, density=None, minus_one=None):
+    """computeDensity + computeVelocity; returns (density, velocity).  density holds the values of
+    the cells that are skipped (zeros if None)."""
+    inn = _flags(inner)
+    mo = _flags(minus_one) if minus_one is not None else None
+    pdf = _block(pdf, BLOCK_PDFS)
+    rho = np.zeros(BLOCK_CELLS) if density is None else np.array(density, dtype=np.float64).reshape(-1)
+    u = np.zeros(3 * BLOCK_CELLS)
+    lib().lbo_block_cell_values(_u8p(inn), _u8p(mo) if mo is not None else None, _dp(pdf), _dp(rho), _dp(u))
+    return rho, u
+
+
+def block_collide(inner, pdf, density, velocity, tau, minus_one=None):
+    """BGK collision of the inner cells; returns the new pdf (input not modified)."""
+    inn = _flags(inner)
+    mo = _flags(minus_one) if minus_one is not None else None
+    out = np.array(pdf, dtype=np.float64).reshape(-1)
+    lib().lbo_block_collide(_u8p(inn), _u8p(mo) if mo is not None else None, _dp(_block(density, BLOCK_CELLS)),
+                            _dp(_block(velocity, 3 * BLOCK_CELLS)), float(tau), _dp(out))
+    return out
+
+
+def block_boundary(pdf, density, velocity, links):
+    """Boundary links after the collision; links: iterable of (cell, q_in, type, (u0, u1, u2), rho_l).
+    Returns the new pdf (input not modified)."""
+    out = np.array(pdf, dtype=np.float64).reshape(-1)
+    rho = _block(density, BLOCK_CELLS)
+    vel = _block(velocity, 3 * BLOCK_CELLS)
+    for cell, q_in, ty, ul, rl in links:
+        uu = np.ascontiguousarray(ul, dtype=np.float64)
+        lib().lbo_block_boundary(int(cell), int(q_in), int(ty), _dp(uu), float(rl), _dp(rho), _dp(vel), _dp(out))
+    return out
+
+
+def block_local_index():
+    buf = (ctypes.c_int * (4 * 2048))()
+    n = lib().lbo_block_local_index(ctypes.cast(buf, ctypes.POINTER(ctypes.c_int)), 2048)
+    return np.frombuffer(buf, np.int32)[:4 * n].reshape(n, 4).copy()
+
+
+def block_nonlocal_index():
+    buf = (ctypes.c_int * (5 * 1024))()
+    n = lib().lbo_block_nonlocal_index(ctypes.cast(buf, ctypes.POINTER(ctypes.c_int)), 1024)
+    return np.frombuffer(buf, np.int32)[:5 * n].reshape(n, 5).copy()
+
+
+def block_stream_local(inner, pdf):
+    out = np.array(pdf, dtype=np.float64).reshape(-1)
+    assert out.size == BLOCK_PDFS
+    lib().lbo_block_stream_local(_u8p(_flags(inner)), _dp(out))
+    return out
+
+
+def block_swap_inside(pdf):
+    out = np.array(pdf, dtype=np.float64).reshape(-1)
+    assert out.size == BLOCK_PDFS
+    lib().lbo_block_swap_inside(_dp(out))
+    return out
+
+
+def block_stream_nonlocal(inner, pdfs):
+    """streamingNonLocal on the 8 blocks of one Peano cell; inner: (8, 216), pdfs: (8, 4104) arrays.
+    Returns the 8 new blocks (inputs not modified)."""
+    inn = [_flags(inner[v]) for v in range(8)]
+    out = [np.array(pdfs[v], dtype=np.float64).reshape(-1) for v in range(8)]
+    ip = (ctypes.POINTER(ctypes.c_uint8) * 8)(*[_u8p(a) for a in inn])
+    pp = (ctypes.POINTER(ctypes.c_double) * 8)(*[_dp(a) for a in out])
+    lib().lbo_block_stream_nonlocal(ip, pp)
+    return np.stack(out)

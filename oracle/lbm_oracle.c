@@ -755,3 +755,171 @@ void lbo_amr_restrict(const int pos[3], double* coarse, const double* fine) {
         }
       }
 }
+
+/* ---- the block solver's per-block and per-Peano-cell operators on 6^3 blocks ----------------
+ * Blocks as the reference's GridManagementService keeps them: pdf[19 * cell + q], density[216],
+ * velocity[648], inner[216], cell = x + 6 y + 36 z. */
+
+/* computeDensity (BLB/BlockComputeCellValues.cpp:26-43) and computeVelocity (:47-68): inner cells
+ * without a -1.0 entry (minus_one NULL = none) take rho = sum of f in q order and
+ * u = (1.0/rho) * sum f c; every other cell keeps its density and gets u = 0. */
+void lbo_block_cell_values(const uint8_t* inner, const uint8_t* minus_one, const double* pdf, double* density,
+                           double* velocity) {
+  for (int c = 0; c < LBO_BC; c++) {
+    const int act = inner[c] && !(minus_one && minus_one[c]);
+    if (act) density[c] = lbo_density(pdf + LBO_Q * c);
+    double u[3] = {0.0, 0.0, 0.0};
+    if (act) lbo_velocity(pdf + LBO_Q * c, density[c], u);
+    for (int d = 0; d < 3; d++) velocity[3 * c + d] = u[d];
+  }
+}
+
+/* CollisionModelWrapper::collide (collisionmodels/CollisionModelWrapper.cpp:28-39) with BlockBGK
+ * (BlockBGK.cpp:18-45): inner cells without a -1.0 entry relax toward the equilibrium of the stored
+ * density and velocity, f -= 1.0/tau (f - feq); tau is the value the BGK model holds. */
+void lbo_block_collide(const uint8_t* inner, const uint8_t* minus_one, const double* density,
+                       const double* velocity, double tau, double* pdf) {
+  for (int c = 0; c < LBO_BC; c++) {
+    if (!inner[c] || (minus_one && minus_one[c])) continue;
+    double feq[LBO_Q];
+    lbo_feq(density[c], velocity + 3 * c, feq);
+    lbo_bgk(pdf + LBO_Q * c, tau, feq);
+  }
+}
+
+/* One boundary link after the collision, as BlockBoundaryConditionManager::treatBoundaryAfterCollision
+ * (boundaryconditions/BlockBoundaryConditionManager.cpp:47-62) dispatches it by BoundaryData type
+ * (LB/BoundaryData.h:44-61): 0 halfway bounce-back (no-op), 1 moving wall and 5 moving obstacle
+ * (BlockMovingWall.cpp:13-33), 2 pressure outlet, 3 pressure inlet, 4 velocity inlet; q_in is the
+ * incoming direction (boundaryDataIndex(1)) and the outgoing slot 18 - q_in is written, with the
+ * rules of lbo_boundary_value and the cell's stored density and velocity. */
+void lbo_block_boundary(int cell, int q_in, int type, const double* u_l, double rho_l, const double* density,
+                        const double* velocity, double* pdf) {
+  const int q = LBO_Q - 1 - q_in;
+  double* f = pdf + LBO_Q * cell + q;
+  const double rho = density[cell];
+  if (type == 1 || type == 5) {
+    *f -= lbo_moving_wall_term(u_l, q, rho);
+    return;
+  }
+  if (type < 2 || type > 4) return;
+  double feq[LBO_Q];
+  if (type == 4) {
+    lbo_feq(rho_l, u_l, feq);
+    *f = feq[q_in];
+    return;
+  }
+  if (type == 2) lbo_feq(rho_l, velocity + 3 * cell, feq);
+  else lbo_feq(rho, u_l, feq);
+  *f = -*f + feq[q_in] + feq[q];
+}
+
+/* createIndexForLocalStreaming (nonlocaltreatment/AABlockStreaming.cpp:111-186): cells in z, y, x
+ * order and q in order; (cell, q) whose neighbour cell + c_q lies in the block is listed unless q is
+ * the rest population or its partner (neighbour, 18 - q) is listed already.  Entry k is
+ * {19 cell + q, 19 nb + 18 - q, cell, nb}.  Returns the number of entries (1440) and writes at most
+ * max of them. */
+int lbo_block_local_index(int* out, int max) {
+  static uint8_t listed[LBO_BC * LBO_Q];
+  memset(listed, 0, sizeof listed);
+  int n = 0;
+  for (int z = 0; z < LBO_BS; z++)
+    for (int y = 0; y < LBO_BS; y++)
+      for (int x = 0; x < LBO_BS; x++) {
+        const int c = x + LBO_BS * (y + LBO_BS * z);
+        for (int q = 0; q < LBO_Q; q++) {
+          const int nx = x + LBO_C[q][0], ny = y + LBO_C[q][1], nz = z + LBO_C[q][2];
+          if (nx < 0 || nx >= LBO_BS || ny < 0 || ny >= LBO_BS || nz < 0 || nz >= LBO_BS) continue;
+          const int nb = nx + LBO_BS * (ny + LBO_BS * nz);
+          if (q == LBO_REST || listed[LBO_Q * nb + LBO_Q - 1 - q]) continue;
+          listed[LBO_Q * c + q] = 1;
+          if (n < max) {
+            int* e = out + 4 * n;
+            e[0] = LBO_Q * c + q;
+            e[1] = LBO_Q * nb + LBO_Q - 1 - q;
+            e[2] = c;
+            e[3] = nb;
+          }
+          n++;
+        }
+      }
+  return n;
+}
+
+/* streamingLocal (:19-32): each listed pair is swapped when both cells are inner. */
+void lbo_block_stream_local(const uint8_t* inner, double* pdf) {
+  static int idx[4 * 2048];
+  const int n = lbo_block_local_index(idx, 2048);
+  for (int k = 0; k < n; k++) {
+    const int* e = idx + 4 * k;
+    if (!(inner[e[2]] && inner[e[3]])) continue;
+    const double t = pdf[e[0]];
+    pdf[e[0]] = pdf[e[1]];
+    pdf[e[1]] = t;
+  }
+}
+
+/* swapPdfsInsideCells (:80-92): in every cell, inner or not, f_q <-> f_{18-q} for q < 9. */
+void lbo_block_swap_inside(double* pdf) {
+  for (int c = 0; c < LBO_BC; c++)
+    for (int q = 0; q < LBO_Q / 2; q++) {
+      const double t = pdf[LBO_Q * c + q];
+      pdf[LBO_Q * c + q] = pdf[LBO_Q * c + LBO_Q - 1 - q];
+      pdf[LBO_Q * c + LBO_Q - 1 - q] = t;
+    }
+}
+
+/* getNonLocalStreamingIndex (nonlocaltreatment/CellTopology.cpp:435-465) with the even-block-size
+ * search (:317-418), stated on the Peano cell's own 6^3 sub-cells: vertex v (lexicographic, bit d =
+ * side along axis d) owns the octant [3 v_d, 3 v_d + 3) of the cell, and sub-cell s of that octant is
+ * cell s_d + 3 - 6 v_d of v's block.  For vertex pairs i < j, sub-cells s of i's octant in z, y, x
+ * order and q in order, the entry {i, j, cell of s in block i, cell of s + c_q in block j, q} is
+ * listed when s + c_q falls on j's side of the cell's centre planes along every axis (a target
+ * outside the cell keeps the side of s).  Returns the number of entries (504). */
+int lbo_block_nonlocal_index(int* out, int max) {
+  int n = 0;
+  for (int i = 0; i < 8; i++)
+    for (int j = i + 1; j < 8; j++) {
+      const int vi[3] = {i & 1, (i >> 1) & 1, i >> 2}, vj[3] = {j & 1, (j >> 1) & 1, j >> 2};
+      for (int sz = 3 * vi[2]; sz < 3 * vi[2] + 3; sz++)
+        for (int sy = 3 * vi[1]; sy < 3 * vi[1] + 3; sy++)
+          for (int sx = 3 * vi[0]; sx < 3 * vi[0] + 3; sx++) {
+            const int s[3] = {sx, sy, sz};
+            for (int q = 0; q < LBO_Q; q++) {
+              int t[3], side = 1;
+              for (int d = 0; d < 3; d++) {
+                t[d] = s[d] + LBO_C[q][d];
+                side = side && ((t[d] >= 3) == vj[d]);
+              }
+              if (!side) continue;
+              if (n < max) {
+                int* e = out + 5 * n;
+                e[0] = i;
+                e[1] = j;
+                e[2] = (s[0] + 3 - 6 * vi[0]) + LBO_BS * ((s[1] + 3 - 6 * vi[1]) + LBO_BS * (s[2] + 3 - 6 * vi[2]));
+                e[3] = (t[0] + 3 - 6 * vj[0]) + LBO_BS * ((t[1] + 3 - 6 * vj[1]) + LBO_BS * (t[2] + 3 - 6 * vj[2]));
+                e[4] = q;
+              }
+              n++;
+            }
+          }
+    }
+  return n;
+}
+
+/* streamingNonLocal (AABlockStreaming.cpp:37-53) on the 8 blocks of one Peano cell (lexicographic
+ * vertex order): entry {i, j, c1, c2, q} swaps pdf_i[19 c1 + q] with pdf_j[19 c2 + 18 - q] when
+ * c1 is inner in block i and c2 in block j. */
+void lbo_block_stream_nonlocal(const uint8_t* const* inner, double* const* pdf) {
+  static int idx[5 * 1024];
+  const int n = lbo_block_nonlocal_index(idx, 1024);
+  for (int k = 0; k < n; k++) {
+    const int* e = idx + 5 * k;
+    if (!(inner[e[0]][e[2]] && inner[e[1]][e[3]])) continue;
+    double* a = pdf[e[0]] + LBO_Q * e[2] + e[4];
+    double* b = pdf[e[1]] + LBO_Q * e[3] + LBO_Q - 1 - e[4];
+    const double t = *a;
+    *a = *b;
+    *b = t;
+  }
+}

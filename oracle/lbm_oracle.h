@@ -14,7 +14,9 @@
  * Pinning (see tests/test_oracle.py): the lattice tables, moments, equilibrium and
  * BGK relaxation are pinned by closed-form identities; the full step (collide,
  * swap streaming with implicit halfway bounce-back, moving wall) and the restriction /
- * prolongation between grid levels are pinned bit-for-bit against fixtures produced by
+ * prolongation between grid levels and the block solver's per-block operators (cell values,
+ * collision, boundary links, local / cross-block / in-cell swap streaming and their index tables)
+ * are pinned bit-for-bit against fixtures produced by
  * the compiled reference (tests/golden/), the overlap map of the latter also against a
  * brute-force run of the reference's own three-step streaming procedure; Guo forcing, Brinkman porosity, periodic wrap, the pressure-drop
  * objective and the adjoint step go beyond the reference and are pinned by analytic
@@ -155,6 +157,20 @@ void lbo_amr_find_overlap(const double* fine, uint8_t* overlap);
 void lbo_amr_prolongate(const int pos[3], const double* coarse, double* fine, double* density, double* velocity,
                         const uint8_t* overlap);
 void lbo_amr_restrict(const int pos[3], double* coarse, const double* fine);
+
+/* The block solver's operators on 6^3 blocks (BLB/BlockComputeCellValues.cpp, BLB/collisionmodels/,
+ * BLB/boundaryconditions/, BLB/nonlocaltreatment/AABlockStreaming.cpp and CellTopology.cpp). */
+void lbo_block_cell_values(const uint8_t* inner, const uint8_t* minus_one, const double* pdf, double* density,
+                           double* velocity);
+void lbo_block_collide(const uint8_t* inner, const uint8_t* minus_one, const double* density,
+                       const double* velocity, double tau, double* pdf);
+void lbo_block_boundary(int cell, int q_in, int type, const double* u_l, double rho_l, const double* density,
+                        const double* velocity, double* pdf);
+int lbo_block_local_index(int* out, int max);
+void lbo_block_stream_local(const uint8_t* inner, double* pdf);
+void lbo_block_swap_inside(double* pdf);
+int lbo_block_nonlocal_index(int* out, int max);
+void lbo_block_stream_nonlocal(const uint8_t* const* inner, double* const* pdf);
 
 #ifdef __cplusplus
 }
