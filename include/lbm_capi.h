@@ -334,9 +334,76 @@ int lbm_blocks_set_overlap(lbm_blockstore* bs, int64_t first, int64_t count, con
 int lbm_blocks_prolongate(lbm_blockstore* bs, const lbm_amr_link* links, int64_t n);
 /* restrictToCoarseLevel (:164-259) */
 int lbm_blocks_restrict(lbm_blockstore* bs, const lbm_amr_link* links, int64_t n);
-/* device time of the last find/prolongate/restrict launch in ms (HIP events); no reference
+/* device time of the last block-store operator (find/prolongate/restrict, or any of the block
+ * solver operators below; lbm_blocks_iterate: the whole call) in ms (HIP events); no reference
  * counterpart (measurement) */
 float lbm_blocks_last_ms(const lbm_blockstore* bs);
+
+/* ---- block store: the block solver's per-block and per-Peano-cell operators ----------------
+ * The kernels that the regular-grid solver mapping (BLB/mappings/RegularGrid2RegularBlockSolver.cpp:
+ * touchVertexFirstTime :83-177, handleCell :180-220, touchVertexLastTime :223-248) and the
+ * spacetree mapping (SpacetreeGrid2RegularBlockSolver.cpp:215-277, :573-577, :349) call on one block
+ * or one Peano cell at a time, batched over lists: one launch works on every listed block (or cell).
+ * Lists name blocks by id; a block appears at most once per list, and a block sits at one vertex
+ * position at most once per cell list (the 8 blocks of a Peano cell are distinct, lexicographic:
+ * vertex v = x + 2 y + 4 z), so no two workgroups touch the same values; violations return
+ * LBM_ERR_ARG before anything runs.  The blocks of one Peano cell must be the geometric neighbours
+ * the list says they are (as Peano's vertex enumerator gives them). */
+
+/* the inner bitset of blocks first .. first + count - 1 (216 flags per block, nonzero = inner;
+ * GridManagementService::getInner, BLB/services/GridManagementService.cpp:58-153); a new store has
+ * every cell inner */
+int lbm_blocks_set_inner(lbm_blockstore* bs, int64_t first, int64_t count, const uint8_t* inner);
+/* BlockComputeCellValues::computeDensity and computeVelocity (BLB/BlockComputeCellValues.cpp:26-68) on
+ * each listed block: inner cells (with use_overlap != 0: inner cells that are not overlap cells of
+ * the block's last overlap search, the reference's hasMinusOneEntries) get rho = sum f and
+ * u = (1.0/rho) sum f c; every other cell keeps its density and gets u = 0. */
+int lbm_blocks_compute_cell_values(lbm_blockstore* bs, const int64_t* blocks, int64_t n, int use_overlap);
+/* BlockCollisionModelManager::collide with BlockBGK(tau) (BLB/collisionmodels/CollisionModelWrapper.cpp:
+ * 28-39, BlockBGK.cpp:18-45): the same cells relax toward the equilibrium of the stored density and
+ * velocity, f -= 1.0/tau (f - feq); tau is the value the BGK model holds (the level's tau_eff). */
+int lbm_blocks_collide(lbm_blockstore* bs, const int64_t* blocks, int64_t n, double tau, int use_overlap);
+/* One boundary link: a BoundaryData record (LB/BoundaryData.h:44-61) with its boundaryDataIndex
+ * (cell, incoming direction), as ReceiveBoundaryDataService::getBoundaryData lists them
+ * (SLB/services/ReceiveBoundaryDataService.cpp:107-185). */
+typedef struct {
+  int64_t block;
+  int32_t cell;   /* 0..215 */
+  int32_t q_in;   /* incoming direction; the condition writes the outgoing slot 18 - q_in */
+  int32_t type;   /* BoundaryData::BoundaryType: 0 halfway bounce-back (nothing to do), 1 moving wall,
+                     2 pressure outlet, 3 pressure inlet, 4 velocity inlet (equilibrium), 5 moving
+                     obstacle (as the moving wall) */
+  int32_t pad;    /* 0 */
+  double u[3];    /* velocity_L */
+  double rho;     /* density_L */
+} lbm_block_boundary;
+/* BlockBoundaryConditionManager::treatBoundaryAfterCollision (BLB/boundaryconditions/
+ * BlockBoundaryConditionManager.cpp:47-62): each link's condition (BlockMovingWall.cpp:13-33,
+ * BlockAbstractPressureCondition.cpp:24-37, BlockVelocityInletEquilibrium.cpp:19-23) with the cell's
+ * stored density and velocity.  A (block, cell, q_in) appears at most once.  (The reference's
+ * treatBoundaryBeforeCollision is a no-op for every condition.) */
+int lbm_blocks_treat_boundary(lbm_blockstore* bs, const lbm_block_boundary* links, int64_t n);
+/* AABlockStreaming::streamingLocal (BLB/nonlocaltreatment/AABlockStreaming.cpp:19-32): inside each
+ * listed block, f[cell][q] <-> f[cell + c_q][18 - q] for every link whose two cells are inner. */
+int lbm_blocks_stream_local(lbm_blockstore* bs, const int64_t* blocks, int64_t n);
+/* AABlockStreaming::streamingNonLocal (:37-53) with the index of CellTopology::getNonLocalStreamingIndex
+ * (BLB/nonlocaltreatment/CellTopology.cpp:317-465): the same swap across the blocks of each listed
+ * Peano cell (cells: n x 8 block ids, lexicographic vertex order), the reference's ghost exchange. */
+int lbm_blocks_stream_nonlocal(lbm_blockstore* bs, const int64_t* cells, int64_t n);
+/* AABlockStreaming::swapPdfsInsideCells (:80-92): f_q <-> f_{18-q} in every cell of each listed block. */
+int lbm_blocks_swap_inside(lbm_blockstore* bs, const int64_t* blocks, int64_t n);
+/* touchVertexFirstTime's kernels on each listed block in one launch (cell values, collision,
+ * boundary links after the collision, local streaming), each block staged once through LDS; same
+ * results as the four calls above in that order.  Links may name only listed blocks. */
+int lbm_blocks_vertex_first(lbm_blockstore* bs, const int64_t* blocks, int64_t n, double tau, int use_overlap,
+                            const lbm_block_boundary* links, int64_t nlinks);
+/* nsteps iterations of the regular-grid block solver (no MD coupling, no dynamic geometry):
+ * touchVertexFirstTime on every listed block, handleCell on every listed Peano cell,
+ * touchVertexLastTime on every listed block.  The in-cell swap of one iteration runs in the same
+ * launch as the next iteration's first touch (two launches per iteration); the results equal the
+ * separate operators in the reference's order. */
+int lbm_blocks_iterate(lbm_blockstore* bs, const int64_t* blocks, int64_t nblocks_list, const int64_t* cells,
+                       int64_t ncells, double tau, const lbm_block_boundary* links, int64_t nlinks, int nsteps);
 
 #ifdef __cplusplus
 }

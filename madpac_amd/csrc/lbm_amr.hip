@@ -26,22 +26,9 @@
 #include <utility>
 #include <vector>
 
-#include "lbm_capi.h"
-#include "lbm_cell.h"
-#include "lbm_internal.h"
+#include "lbm_blockstore.h"
 
 namespace lbm {
-
-constexpr int BS = 6;                  // block edge
-constexpr int BC = BS * BS * BS;       // cells per block
-constexpr int BP = BC * Q;             // populations per block
-constexpr int BV = BC * 3;             // velocity entries per block
-
-struct AmrLinkDev {
-  int64_t coarse, fine;
-  int sx, sy, sz;    // sub-block of the coarse block (0..2 per axis)
-  int new_vertex;
-};
 
 // neighbour block (0..26, 13 = the block itself) whose populations fill entry (cell, q) after three
 // streaming steps: the one holding position 6 + x - 3 c_q per axis
@@ -151,34 +138,8 @@ __global__ __launch_bounds__(256) void k_amr_restrict(const AmrLinkDev* __restri
 
 using namespace lbm;
 
-struct lbm_blockstore {
-  int device = 0;
-  int64_t nblocks = 0;
-  hipStream_t stream = nullptr;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  std::string err;
-  double* pdf = nullptr;       // nblocks * 4104
-  double* density = nullptr;   // nblocks * 216
-  double* velocity = nullptr;  // nblocks * 648
-  uint8_t* mask = nullptr;     // nblocks * 216 overlap cells of the last search
-  AmrLinkDev* d_links = nullptr;
-  int64_t cap_links = 0;
-  float last_ms = 0.f;
-};
-
 namespace {
 thread_local std::string g_blocks_create_error;
-
-int bfail(lbm_blockstore* bs, int code, const std::string& m) {
-  if (bs) bs->err = m;
-  return code;
-}
-
-#define BCHK(bs, call)                                                                                         \
-  do {                                                                                                         \
-    hipError_t e_ = (call);                                                                                    \
-    if (e_ != hipSuccess) return bfail(bs, LBM_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
-  } while (0)
 
 enum Op { OVERLAP, PROLONGATE, RESTRICT };
 
@@ -216,14 +177,7 @@ int upload_links(lbm_blockstore* bs, const lbm_amr_link* links, int64_t n, Op op
     return bfail(bs, LBM_ERR_ARG, "a fine block appears in two links");
   for (int64_t b : fines)
     if (coarses.count(b)) return bfail(bs, LBM_ERR_ARG, "a block is fine in one link and coarse in another (one level pair per call)");
-  if (n > bs->cap_links) {
-    if (bs->d_links) (void)hipFree(bs->d_links);
-    bs->d_links = nullptr;
-    bs->cap_links = 0;
-    if (lbm::device_alloc(&bs->d_links, (size_t)n * sizeof(AmrLinkDev)) != hipSuccess)
-      return bfail(bs, LBM_ERR_NOMEM, "link table allocation failed");
-    bs->cap_links = n;
-  }
+  if (int rc = ensure_capacity(bs, &bs->d_links, &bs->cap_links, n, "link table")) return rc;
   if (n) BCHK(bs, hipMemcpyAsync(bs->d_links, h.data(), (size_t)n * sizeof(AmrLinkDev), hipMemcpyHostToDevice, bs->stream));
   return LBM_OK;
 }
@@ -244,12 +198,6 @@ int run_links(lbm_blockstore* bs, int64_t n, Op op) {
   BCHK(bs, hipEventRecord(bs->ev1, bs->stream));
   BCHK(bs, hipEventSynchronize(bs->ev1));
   BCHK(bs, hipEventElapsedTime(&bs->last_ms, bs->ev0, bs->ev1));
-  return LBM_OK;
-}
-
-int check_range(lbm_blockstore* bs, int64_t first, int64_t count) {
-  if (first < 0 || count < 0 || first > bs->nblocks || count > bs->nblocks - first)
-    return bfail(bs, LBM_ERR_ARG, "block range out of bounds");
   return LBM_OK;
 }
 }  // namespace
@@ -276,12 +224,13 @@ int lbm_blocks_create(lbm_blockstore** out, int64_t nblocks, int device) {
   if (lbm::device_alloc(&bs->pdf, nb * BP * sizeof(double)) != hipSuccess ||
       lbm::device_alloc(&bs->density, nb * BC * sizeof(double)) != hipSuccess ||
       lbm::device_alloc(&bs->velocity, nb * BV * sizeof(double)) != hipSuccess ||
-      lbm::device_alloc(&bs->mask, nb * BC) != hipSuccess)
+      lbm::device_alloc(&bs->mask, nb * BC) != hipSuccess || lbm::device_alloc(&bs->inner, nb * BC) != hipSuccess)
     return bail(LBM_ERR_NOMEM, "block store allocation failed");
   if (hipMemsetAsync(bs->pdf, 0, nb * BP * sizeof(double), bs->stream) != hipSuccess ||
       hipMemsetAsync(bs->density, 0, nb * BC * sizeof(double), bs->stream) != hipSuccess ||
       hipMemsetAsync(bs->velocity, 0, nb * BV * sizeof(double), bs->stream) != hipSuccess ||
-      hipMemsetAsync(bs->mask, 0, nb * BC, bs->stream) != hipSuccess || hipStreamSynchronize(bs->stream) != hipSuccess)
+      hipMemsetAsync(bs->mask, 0, nb * BC, bs->stream) != hipSuccess ||
+      hipMemsetAsync(bs->inner, 1, nb * BC, bs->stream) != hipSuccess || hipStreamSynchronize(bs->stream) != hipSuccess)
     return bail(LBM_ERR_HIP, "block store clear failed");
   *out = bs;
   return LBM_OK;
@@ -290,7 +239,8 @@ int lbm_blocks_create(lbm_blockstore** out, int64_t nblocks, int device) {
 int lbm_blocks_destroy(lbm_blockstore* bs) {
   if (!bs) return LBM_OK;
   if (bs->stream) (void)hipStreamSynchronize(bs->stream);
-  void* ptrs[] = {bs->pdf, bs->density, bs->velocity, bs->mask, bs->d_links};
+  void* ptrs[] = {bs->pdf, bs->density, bs->velocity, bs->mask, bs->inner, bs->d_links, bs->d_list, bs->d_cells,
+                  bs->d_bnd, bs->d_off};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (bs->ev0) (void)hipEventDestroy(bs->ev0);

@@ -1,0 +1,495 @@
+// The block solver's per-block and per-Peano-cell operators on the device block store.
+//
+// Reference counterpart: the kernels that RegularGrid2RegularBlockSolver
+// (BLB/mappings/RegularGrid2RegularBlockSolver.cpp) calls from its events, one block or one Peano
+// cell at a time: computeDensity / computeVelocity (BLB/BlockComputeCellValues.cpp:26-68), the BGK
+// collision (BLB/collisionmodels/CollisionModelWrapper.cpp:28-39, BlockBGK.cpp:18-45), the boundary
+// conditions after the collision (BLB/boundaryconditions/BlockBoundaryConditionManager.cpp:47-62),
+// and the swap streaming of BLB/nonlocaltreatment/AABlockStreaming.cpp (streamingLocal :19-32,
+// streamingNonLocal :37-53, swapPdfsInsideCells :80-92).  Here each operator is one launch over a
+// list of blocks (or Peano cells), one workgroup per block (or cell).
+//
+// Per-block operators stage the block's 4,104 populations (32.8 KB, AoS as the store keeps them)
+// through LDS with coalesced loads and stores, run the chosen operators there in the reference's
+// order, and write the block back once.  Fused, touchVertexFirstTime's four kernels move each
+// block across HBM once instead of four times; with the next iteration's in-cell swap folded in,
+// an iteration of the regular grid is two launches.  The cross-block swap touches only the
+// boundary-crossing populations and works in place in HBM: its swaps are disjoint (each
+// cross-block link belongs to exactly one Peano cell and one entry of its table), so all cells of a
+// call run concurrently.  All of it is HBM-bound byte movement plus the FP64 collision, computed in
+// the reference's operation order (bit-identical; the build uses -ffp-contract=off).
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "lbm_blockstore.h"
+
+namespace lbm {
+
+// ---------------------------------------------------------------------------------------------
+// CellTopology::getNonLocalStreamingIndex (BLB/nonlocaltreatment/CellTopology.cpp:435-465, even
+// block size :317-418), built at compile time.  Stated on the Peano cell's 6^3 sub-cells: vertex v
+// owns the octant [3 v_d, 3 v_d + 3) and sub-cell s of it is cell s_d + 3 - 6 v_d of v's block; for
+// vertex pairs i < j, sub-cells s of i's octant (z, y, x order) and directions q, the entry
+// (i, j, cell of s, cell of s + c_q in block j, q) exists when s + c_q lies on j's side of the
+// cell's centre planes along every axis.  Packed i | j << 3 | c1 << 6 | c2 << 14 | q << 22.
+constexpr int NL_ENTRIES = 504;
+struct NonLocalTable {
+  uint32_t e[NL_ENTRIES];
+  int n;
+};
+
+constexpr NonLocalTable make_nonlocal_table() {
+  NonLocalTable t{};
+  int n = 0;
+  for (int i = 0; i < 8; i++)
+    for (int j = i + 1; j < 8; j++) {
+      const int vi[3] = {i & 1, (i >> 1) & 1, i >> 2}, vj[3] = {j & 1, (j >> 1) & 1, j >> 2};
+      for (int sz = 3 * vi[2]; sz < 3 * vi[2] + 3; sz++)
+        for (int sy = 3 * vi[1]; sy < 3 * vi[1] + 3; sy++)
+          for (int sx = 3 * vi[0]; sx < 3 * vi[0] + 3; sx++)
+            for (int q = 0; q < Q; q++) {
+              const int t3[3] = {sx + CX(q), sy + CY(q), sz + CZ(q)};
+              bool side = true;
+              for (int d = 0; d < 3; d++) side = side && ((t3[d] >= 3 ? 1 : 0) == vj[d]);
+              if (!side) continue;
+              const int c1 = (sx + 3 - 6 * vi[0]) + BS * ((sy + 3 - 6 * vi[1]) + BS * (sz + 3 - 6 * vi[2]));
+              const int c2 = (t3[0] + 3 - 6 * vj[0]) + BS * ((t3[1] + 3 - 6 * vj[1]) + BS * (t3[2] + 3 - 6 * vj[2]));
+              if (n < NL_ENTRIES)
+                t.e[n] = (uint32_t)i | (uint32_t)j << 3 | (uint32_t)c1 << 6 | (uint32_t)c2 << 14 | (uint32_t)q << 22;
+              n++;
+            }
+    }
+  t.n = n;
+  return t;
+}
+static_assert(make_nonlocal_table().n == NL_ENTRIES, "D3Q19 on 6^3 blocks has 504 cross-block links per Peano cell");
+
+__constant__ NonLocalTable kNonLocal = make_nonlocal_table();
+
+// ---------------------------------------------------------------------------------------------
+enum : unsigned { OP_SWAP_IN = 1u, OP_VALUES = 2u, OP_COLLIDE = 4u, OP_BOUNDARY = 8u, OP_LOCAL = 16u };
+
+struct BlockOpArgs {
+  const int64_t* list;  // block ids, one workgroup each
+  int64_t n;
+  double* pdf;
+  double* density;
+  double* velocity;
+  const uint8_t* inner;
+  const uint8_t* mask;       // overlap cells (hasMinusOneEntries), or null
+  const BlockLinkDev* links;  // grouped by list position
+  const int32_t* off;         // links of list position p: off[p] .. off[p + 1] - 1
+  double omega;               // 1.0 / tau
+};
+
+// one boundary link after the collision on the outgoing slot f (see lbm_blocks_treat_boundary); rho
+// and (u0, u1, u2) are the cell's stored density and velocity
+__device__ __forceinline__ void block_boundary_f64(double& f, const BlockLinkDev& L, double rho, double u0,
+                                                   double u1, double u2) {
+  const int q = Q - 1 - L.q_in;
+  if (L.type == 1 || L.type == 5) {
+    f -= moving_wall_f64(q, L.u, rho);
+    return;
+  }
+  if (L.type < 2 || L.type > 4) return;
+  double rt, t0, t1, t2;
+  if (L.type == 2) {
+    rt = L.rho; t0 = u0; t1 = u1; t2 = u2;
+  } else {
+    rt = L.type == 3 ? rho : L.rho;
+    t0 = L.u[0]; t1 = L.u[1]; t2 = L.u[2];
+  }
+  double uu = t0 * t0;
+  uu += t1 * t1;
+  uu += t2 * t2;
+  const double fin = feq_f64(L.q_in, rt, t0, t1, t2, uu);
+  f = L.type == 4 ? fin : -f + fin + feq_f64(q, rt, t0, t1, t2, uu);
+}
+
+template <unsigned OPS>
+__global__ __launch_bounds__(256) void k_block_ops(BlockOpArgs a) {
+  constexpr bool WRITES = (OPS & (OP_SWAP_IN | OP_COLLIDE | OP_BOUNDARY | OP_LOCAL)) != 0;
+  constexpr bool MOMENTS = (OPS & (OP_VALUES | OP_COLLIDE | OP_BOUNDARY)) != 0;
+  __shared__ double s[BP];
+  __shared__ double srho[MOMENTS ? BC : 1];
+  __shared__ double su[MOMENTS ? BV : 1];
+  __shared__ uint8_t sin[BC];
+  const int64_t p = blockIdx.x;
+  if (p >= a.n) return;
+  const int64_t b = a.list[p];
+  const int t = threadIdx.x;
+  double* g = a.pdf + b * BP;
+  for (int k = t; k < BP; k += 256) s[k] = g[k];
+  if (t < BC) sin[t] = a.inner[b * BC + t];
+  __syncthreads();
+  const int c = t;
+  if (c < BC) {
+    double* f = s + c * Q;
+    if (OPS & OP_SWAP_IN) {
+      // the previous iteration's swapPdfsInsideCells (every cell, inner or not)
+#pragma unroll
+      for (int q = 0; q < Q / 2; q++) {
+        const double v = f[q];
+        f[q] = f[Q - 1 - q];
+        f[Q - 1 - q] = v;
+      }
+    }
+    if (MOMENTS) {
+      const bool act = sin[c] && !(a.mask && a.mask[b * BC + c]);
+      double rho, u0, u1, u2;
+      if (OPS & OP_VALUES) {
+        // computeDensity (:36-39) and computeVelocity (:57-61); skipped cells keep their density
+        // and get u = 0
+        rho = a.density[b * BC + c];
+        u0 = u1 = u2 = 0.0;
+        if (act) {
+          rho = 0.0;
+#pragma unroll
+          for (int q = 0; q < Q; q++) rho += f[q];
+          double b0 = 0.0, b1 = 0.0, b2 = 0.0;
+#pragma unroll
+          for (int q = 0; q < Q; q++) {
+            if (CX(q) == 1) b0 += f[q];
+            if (CX(q) == -1) b0 -= f[q];
+            if (CY(q) == 1) b1 += f[q];
+            if (CY(q) == -1) b1 -= f[q];
+            if (CZ(q) == 1) b2 += f[q];
+            if (CZ(q) == -1) b2 -= f[q];
+          }
+          const double inv = 1.0 / rho;
+          u0 = b0 * inv;
+          u1 = b1 * inv;
+          u2 = b2 * inv;
+          a.density[b * BC + c] = rho;
+        }
+        double* uo = a.velocity + b * BV + 3 * c;
+        uo[0] = u0;
+        uo[1] = u1;
+        uo[2] = u2;
+      } else {
+        rho = a.density[b * BC + c];
+        const double* ui = a.velocity + b * BV + 3 * c;
+        u0 = ui[0];
+        u1 = ui[1];
+        u2 = ui[2];
+      }
+      srho[c] = rho;
+      su[3 * c] = u0;
+      su[3 * c + 1] = u1;
+      su[3 * c + 2] = u2;
+      if ((OPS & OP_COLLIDE) && act) {
+        // computeFeq (BLB/BlockComputeCellValues.cpp:14-20) and BlockBGK (BlockBGK.cpp:43)
+        double uu = u0 * u0;
+        uu += u1 * u1;
+        uu += u2 * u2;
+#pragma unroll
+        for (int q = 0; q < Q; q++) f[q] -= a.omega * (f[q] - feq_f64(q, rho, u0, u1, u2, uu));
+      }
+    }
+  }
+  if (OPS & OP_BOUNDARY) {
+    __syncthreads();
+    const int32_t l1 = a.off[p + 1];
+    for (int32_t k = a.off[p] + t; k < l1; k += 256) {
+      const BlockLinkDev L = a.links[k];
+      block_boundary_f64(s[L.cell * Q + Q - 1 - L.q_in], L, srho[L.cell], su[3 * L.cell], su[3 * L.cell + 1],
+                         su[3 * L.cell + 2]);
+    }
+  }
+  if (OPS & OP_LOCAL) {
+    // streamingLocal: the link (c, q < 9) <-> (c + c_q, 18 - q) is swapped by thread c when both
+    // cells are inner; every slot belongs to one link, so the swaps are disjoint
+    __syncthreads();
+    if (c < BC && sin[c]) {
+      const int x = c % BS, y = (c / BS) % BS, z = c / (BS * BS);
+#pragma unroll
+      for (int q = 0; q < Q / 2; q++) {
+        const int nx = x + CX(q), ny = y + CY(q), nz = z + CZ(q);
+        if (nx < 0 || nx >= BS || ny < 0 || ny >= BS || nz < 0 || nz >= BS) continue;
+        const int nb = nx + BS * (ny + BS * nz);
+        if (!sin[nb]) continue;
+        const double v = s[c * Q + q];
+        s[c * Q + q] = s[nb * Q + Q - 1 - q];
+        s[nb * Q + Q - 1 - q] = v;
+      }
+    }
+  }
+  if (WRITES) {
+    __syncthreads();
+    for (int k = t; k < BP; k += 256) g[k] = s[k];
+  }
+}
+
+// treatBoundaryAfterCollision for a flat list of links, one thread each, in place in HBM
+__global__ __launch_bounds__(256) void k_block_boundary(const BlockLinkDev* __restrict__ links, int64_t n,
+                                                        double* __restrict__ pdf, const double* __restrict__ density,
+                                                        const double* __restrict__ velocity) {
+  const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (k >= n) return;
+  const BlockLinkDev L = links[k];
+  const double* u = velocity + L.block * BV + 3 * L.cell;
+  block_boundary_f64(pdf[L.block * BP + L.cell * Q + Q - 1 - L.q_in], L, density[L.block * BC + L.cell], u[0], u[1],
+                     u[2]);
+}
+
+// streamingNonLocal: one workgroup per Peano cell, one thread per table entry
+__global__ __launch_bounds__(512) void k_block_nonlocal(const int64_t* __restrict__ cells, int64_t n,
+                                                        double* __restrict__ pdf, const uint8_t* __restrict__ inner) {
+  const int64_t p = blockIdx.x;
+  const int e = threadIdx.x;
+  if (p >= n || e >= NL_ENTRIES) return;
+  const uint32_t E = kNonLocal.e[e];
+  const int i = E & 7u, j = (E >> 3) & 7u, c1 = (E >> 6) & 255u, c2 = (E >> 14) & 255u, q = (E >> 22) & 31u;
+  const int64_t bi = cells[8 * p + i], bj = cells[8 * p + j];
+  if (!inner[bi * BC + c1] || !inner[bj * BC + c2]) return;
+  double* x = pdf + bi * BP + c1 * Q + q;
+  double* y = pdf + bj * BP + c2 * Q + (Q - 1 - q);
+  const double v = *x;
+  *x = *y;
+  *y = v;
+}
+
+}  // namespace lbm
+
+using namespace lbm;
+
+namespace {
+
+// a block list: ids in range, each block once
+int upload_blocks(lbm_blockstore* bs, const int64_t* blocks, int64_t n, int64_t** dst, int64_t* cap) {
+  if (n < 0 || (n > 0 && !blocks)) return bfail(bs, LBM_ERR_ARG, "block list is null");
+  if (n > INT32_MAX) return bfail(bs, LBM_ERR_ARG, "too many blocks in one call");
+  std::vector<uint8_t> seen((size_t)bs->nblocks, 0);
+  for (int64_t k = 0; k < n; k++) {
+    const int64_t b = blocks[k];
+    if (b < 0 || b >= bs->nblocks) return bfail(bs, LBM_ERR_ARG, "block " + std::to_string(k) + ": id out of range");
+    if (seen[(size_t)b]) return bfail(bs, LBM_ERR_ARG, "block " + std::to_string(b) + " appears twice in the list");
+    seen[(size_t)b] = 1;
+  }
+  if (int rc = ensure_capacity(bs, dst, cap, std::max<int64_t>(n, 1), "block list")) return rc;
+  if (n) BCHK(bs, hipMemcpyAsync(*dst, blocks, (size_t)n * sizeof(int64_t), hipMemcpyHostToDevice, bs->stream));
+  return LBM_OK;
+}
+
+// a Peano cell list: 8 distinct blocks per cell, each (block, vertex position) once per call
+int upload_cells(lbm_blockstore* bs, const int64_t* cells, int64_t n, int64_t** dst, int64_t* cap) {
+  if (n < 0 || (n > 0 && !cells)) return bfail(bs, LBM_ERR_ARG, "cell list is null");
+  if (n > INT32_MAX / 8) return bfail(bs, LBM_ERR_ARG, "too many cells in one call");
+  std::vector<uint8_t> used((size_t)bs->nblocks, 0);
+  for (int64_t k = 0; k < n; k++) {
+    const int64_t* v = cells + 8 * k;
+    for (int i = 0; i < 8; i++) {
+      if (v[i] < 0 || v[i] >= bs->nblocks) return bfail(bs, LBM_ERR_ARG, "cell " + std::to_string(k) + ": block id out of range");
+      for (int j = 0; j < i; j++)
+        if (v[j] == v[i]) return bfail(bs, LBM_ERR_ARG, "cell " + std::to_string(k) + ": a block appears at two vertices");
+      uint8_t& u = used[(size_t)v[i]];
+      if (u & (1u << i))
+        return bfail(bs, LBM_ERR_ARG, "block " + std::to_string(v[i]) + " is vertex " + std::to_string(i) + " of two cells");
+      u |= (uint8_t)(1u << i);
+    }
+  }
+  if (int rc = ensure_capacity(bs, dst, cap, std::max<int64_t>(8 * n, 1), "cell list")) return rc;
+  if (n) BCHK(bs, hipMemcpyAsync(*dst, cells, (size_t)n * 8 * sizeof(int64_t), hipMemcpyHostToDevice, bs->stream));
+  return LBM_OK;
+}
+
+// boundary links: fields in range, each (block, cell, q_in) once; with a block list (n_list >= 0)
+// every link's block is listed and the links are grouped by list position (offsets uploaded)
+int upload_boundary(lbm_blockstore* bs, const lbm_block_boundary* links, int64_t n, const int64_t* list,
+                    int64_t n_list) {
+  if (n < 0 || (n > 0 && !links)) return bfail(bs, LBM_ERR_ARG, "link list is null");
+  if (n > INT32_MAX) return bfail(bs, LBM_ERR_ARG, "too many links in one call");
+  std::vector<int64_t> keys((size_t)n);
+  for (int64_t k = 0; k < n; k++) {
+    const lbm_block_boundary& L = links[k];
+    const std::string at = "link " + std::to_string(k) + ": ";
+    if (L.block < 0 || L.block >= bs->nblocks) return bfail(bs, LBM_ERR_ARG, at + "block id out of range");
+    if (L.cell < 0 || L.cell >= BC) return bfail(bs, LBM_ERR_ARG, at + "cell must be in 0..215");
+    if (L.q_in < 0 || L.q_in >= Q) return bfail(bs, LBM_ERR_ARG, at + "direction must be in 0..18");
+    if (L.type < 0 || L.type > 5) return bfail(bs, LBM_ERR_ARG, at + "type must be a BoundaryType in 0..5");
+    keys[(size_t)k] = (L.block * BC + L.cell) * Q + L.q_in;
+  }
+  std::sort(keys.begin(), keys.end());
+  if (std::adjacent_find(keys.begin(), keys.end()) != keys.end())
+    return bfail(bs, LBM_ERR_ARG, "two links on the same (block, cell, direction)");
+  std::vector<lbm_block_boundary> grouped;
+  const lbm_block_boundary* src = links;
+  if (n_list >= 0) {
+    std::vector<int32_t> pos((size_t)bs->nblocks, -1);
+    for (int64_t k = 0; k < n_list; k++) pos[(size_t)list[k]] = (int32_t)k;
+    std::vector<int32_t> off((size_t)n_list + 1, 0);
+    for (int64_t k = 0; k < n; k++) {
+      const int32_t p = pos[(size_t)links[k].block];
+      if (p < 0) return bfail(bs, LBM_ERR_ARG, "link " + std::to_string(k) + ": its block is not in the block list");
+      off[(size_t)p + 1]++;
+    }
+    for (int64_t k = 0; k < n_list; k++) off[(size_t)k + 1] += off[(size_t)k];
+    grouped.resize((size_t)n);
+    std::vector<int32_t> fill(off.begin(), off.end() - 1);
+    for (int64_t k = 0; k < n; k++) grouped[(size_t)fill[(size_t)pos[(size_t)links[k].block]]++] = links[k];
+    src = grouped.data();
+    if (int rc = ensure_capacity(bs, &bs->d_off, &bs->cap_off, n_list + 1, "link offsets")) return rc;
+    BCHK(bs, hipMemcpyAsync(bs->d_off, off.data(), off.size() * sizeof(int32_t), hipMemcpyHostToDevice, bs->stream));
+    // the copy reads off after this function returns: wait before it goes out of scope
+    BCHK(bs, hipStreamSynchronize(bs->stream));
+  }
+  if (int rc = ensure_capacity(bs, &bs->d_bnd, &bs->cap_bnd, std::max<int64_t>(n, 1), "link table")) return rc;
+  if (n) {
+    BCHK(bs, hipMemcpyAsync(bs->d_bnd, src, (size_t)n * sizeof(BlockLinkDev), hipMemcpyHostToDevice, bs->stream));
+    BCHK(bs, hipStreamSynchronize(bs->stream));
+  }
+  return LBM_OK;
+}
+
+BlockOpArgs op_args(lbm_blockstore* bs, int64_t n, double tau, int use_overlap) {
+  BlockOpArgs a{};
+  a.list = bs->d_list;
+  a.n = n;
+  a.pdf = bs->pdf;
+  a.density = bs->density;
+  a.velocity = bs->velocity;
+  a.inner = bs->inner;
+  a.mask = use_overlap ? bs->mask : nullptr;
+  a.links = bs->d_bnd;
+  a.off = bs->d_off;
+  a.omega = 1.0 / tau;
+  return a;
+}
+
+template <unsigned OPS>
+int launch_ops(lbm_blockstore* bs, const BlockOpArgs& a) {
+  if (a.n) {
+    hipLaunchKernelGGL(k_block_ops<OPS>, dim3((unsigned)a.n), dim3(256), 0, bs->stream, a);
+    BCHK(bs, hipGetLastError());
+  }
+  return LBM_OK;
+}
+
+int launch_nonlocal(lbm_blockstore* bs, const int64_t* d_cells, int64_t n) {
+  if (n) {
+    hipLaunchKernelGGL(k_block_nonlocal, dim3((unsigned)n), dim3(512), 0, bs->stream, d_cells, n, bs->pdf, bs->inner);
+    BCHK(bs, hipGetLastError());
+  }
+  return LBM_OK;
+}
+
+int begin(lbm_blockstore* bs) {
+  BCHK(bs, hipEventRecord(bs->ev0, bs->stream));
+  return LBM_OK;
+}
+
+int finish(lbm_blockstore* bs) {
+  BCHK(bs, hipEventRecord(bs->ev1, bs->stream));
+  BCHK(bs, hipEventSynchronize(bs->ev1));
+  BCHK(bs, hipEventElapsedTime(&bs->last_ms, bs->ev0, bs->ev1));
+  return LBM_OK;
+}
+
+int check_tau(lbm_blockstore* bs, double tau) {
+  if (!(tau > 0.0) || !(tau < 1e300)) return bfail(bs, LBM_ERR_ARG, "tau must be positive and finite");
+  return LBM_OK;
+}
+
+template <unsigned OPS>
+int run_block_op(lbm_blockstore* bs, const int64_t* blocks, int64_t n, double tau, int use_overlap) {
+  if (!bs) return LBM_ERR_ARG;
+  BCHK(bs, hipSetDevice(bs->device));
+  if (OPS & OP_COLLIDE)
+    if (int rc = check_tau(bs, tau)) return rc;
+  if (int rc = upload_blocks(bs, blocks, n, &bs->d_list, &bs->cap_list)) return rc;
+  if (int rc = begin(bs)) return rc;
+  if (int rc = launch_ops<OPS>(bs, op_args(bs, n, tau, use_overlap))) return rc;
+  return finish(bs);
+}
+
+}  // namespace
+
+extern "C" {
+
+int lbm_blocks_set_inner(lbm_blockstore* bs, int64_t first, int64_t count, const uint8_t* inner) {
+  if (!bs) return LBM_ERR_ARG;
+  if (!inner) return bfail(bs, LBM_ERR_ARG, "inner is null");
+  if (int rc = check_range(bs, first, count)) return rc;
+  BCHK(bs, hipSetDevice(bs->device));
+  std::vector<uint8_t> m((size_t)count * BC);
+  for (size_t k = 0; k < m.size(); k++) m[k] = inner[k] ? 1 : 0;
+  BCHK(bs, hipMemcpyAsync(bs->inner + first * BC, m.data(), m.size(), hipMemcpyHostToDevice, bs->stream));
+  BCHK(bs, hipStreamSynchronize(bs->stream));
+  return LBM_OK;
+}
+
+int lbm_blocks_compute_cell_values(lbm_blockstore* bs, const int64_t* blocks, int64_t n, int use_overlap) {
+  return run_block_op<OP_VALUES>(bs, blocks, n, 1.0, use_overlap);
+}
+
+int lbm_blocks_collide(lbm_blockstore* bs, const int64_t* blocks, int64_t n, double tau, int use_overlap) {
+  return run_block_op<OP_COLLIDE>(bs, blocks, n, tau, use_overlap);
+}
+
+int lbm_blocks_stream_local(lbm_blockstore* bs, const int64_t* blocks, int64_t n) {
+  return run_block_op<OP_LOCAL>(bs, blocks, n, 1.0, 0);
+}
+
+int lbm_blocks_swap_inside(lbm_blockstore* bs, const int64_t* blocks, int64_t n) {
+  return run_block_op<OP_SWAP_IN>(bs, blocks, n, 1.0, 0);
+}
+
+int lbm_blocks_treat_boundary(lbm_blockstore* bs, const lbm_block_boundary* links, int64_t n) {
+  if (!bs) return LBM_ERR_ARG;
+  BCHK(bs, hipSetDevice(bs->device));
+  if (int rc = upload_boundary(bs, links, n, nullptr, -1)) return rc;
+  if (int rc = begin(bs)) return rc;
+  if (n) {
+    hipLaunchKernelGGL(k_block_boundary, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, bs->stream, bs->d_bnd, n,
+                       bs->pdf, bs->density, bs->velocity);
+    BCHK(bs, hipGetLastError());
+  }
+  return finish(bs);
+}
+
+int lbm_blocks_stream_nonlocal(lbm_blockstore* bs, const int64_t* cells, int64_t n) {
+  if (!bs) return LBM_ERR_ARG;
+  BCHK(bs, hipSetDevice(bs->device));
+  if (int rc = upload_cells(bs, cells, n, &bs->d_list, &bs->cap_list)) return rc;
+  if (int rc = begin(bs)) return rc;
+  if (int rc = launch_nonlocal(bs, bs->d_list, n)) return rc;
+  return finish(bs);
+}
+
+int lbm_blocks_vertex_first(lbm_blockstore* bs, const int64_t* blocks, int64_t n, double tau, int use_overlap,
+                            const lbm_block_boundary* links, int64_t nlinks) {
+  if (!bs) return LBM_ERR_ARG;
+  BCHK(bs, hipSetDevice(bs->device));
+  if (int rc = check_tau(bs, tau)) return rc;
+  if (int rc = upload_blocks(bs, blocks, n, &bs->d_list, &bs->cap_list)) return rc;
+  if (int rc = upload_boundary(bs, links, nlinks, blocks, n)) return rc;
+  if (int rc = begin(bs)) return rc;
+  if (int rc = launch_ops<OP_VALUES | OP_COLLIDE | OP_BOUNDARY | OP_LOCAL>(bs, op_args(bs, n, tau, use_overlap))) return rc;
+  return finish(bs);
+}
+
+int lbm_blocks_iterate(lbm_blockstore* bs, const int64_t* blocks, int64_t nblocks_list, const int64_t* cells,
+                       int64_t ncells, double tau, const lbm_block_boundary* links, int64_t nlinks, int nsteps) {
+  if (!bs) return LBM_ERR_ARG;
+  if (nsteps < 0) return bfail(bs, LBM_ERR_ARG, "nsteps must be >= 0");
+  BCHK(bs, hipSetDevice(bs->device));
+  if (int rc = check_tau(bs, tau)) return rc;
+  if (int rc = upload_blocks(bs, blocks, nblocks_list, &bs->d_list, &bs->cap_list)) return rc;
+  if (int rc = upload_cells(bs, cells, ncells, &bs->d_cells, &bs->cap_cells)) return rc;
+  if (int rc = upload_boundary(bs, links, nlinks, blocks, nblocks_list)) return rc;
+  const BlockOpArgs a = op_args(bs, nblocks_list, tau, 0);
+  if (int rc = begin(bs)) return rc;
+  for (int s = 0; s < nsteps; s++) {
+    // touchVertexFirstTime (with the previous iteration's touchVertexLastTime folded in), handleCell
+    const int rc = s == 0 ? launch_ops<OP_VALUES | OP_COLLIDE | OP_BOUNDARY | OP_LOCAL>(bs, a)
+                          : launch_ops<OP_SWAP_IN | OP_VALUES | OP_COLLIDE | OP_BOUNDARY | OP_LOCAL>(bs, a);
+    if (rc) return rc;
+    if (int rc2 = launch_nonlocal(bs, bs->d_cells, ncells)) return rc2;
+  }
+  if (nsteps > 0)
+    if (int rc = launch_ops<OP_SWAP_IN>(bs, a)) return rc;  // the last iteration's touchVertexLastTime
+  return finish(bs);
+}
+
+}  // extern "C"

@@ -1,0 +1,96 @@
+// The device-resident store of 6^3 vertex blocks shared by the grid-level transfer operators
+// (lbm_amr.hip) and the block solver's per-block and per-Peano-cell operators (lbm_blockops.hip).
+//
+// A block is one Peano vertex's 6^3 cells in the layout of GridManagementService::VertexData
+// (BLB/services/GridManagementService.h:74-86): populations AoS pdf[19 * cell + q], density[216],
+// velocity[648] (u[3 * cell + d]), cell = x + 6 y + 36 z; the inner bitset is one byte per cell.
+#pragma once
+
+#include <cstdint>
+#include <string>
+
+#include "lbm_capi.h"
+#include "lbm_cell.h"
+#include "lbm_internal.h"
+
+namespace lbm {
+
+constexpr int BS = 6;                  // block edge
+constexpr int BC = BS * BS * BS;       // cells per block
+constexpr int BP = BC * Q;             // populations per block
+constexpr int BV = BC * 3;             // velocity entries per block
+
+struct AmrLinkDev {
+  int64_t coarse, fine;
+  int sx, sy, sz;    // sub-block of the coarse block (0..2 per axis)
+  int new_vertex;
+};
+
+// one boundary link of a block, as lbm_block_boundary with the block's position in the call's list
+struct BlockLinkDev {
+  int64_t block;
+  int32_t cell, q_in, type, pad;
+  double u[3];
+  double rho;
+};
+static_assert(sizeof(BlockLinkDev) == sizeof(lbm_block_boundary), "link layout");
+
+}  // namespace lbm
+
+struct lbm_blockstore {
+  int device = 0;
+  int64_t nblocks = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  std::string err;
+  double* pdf = nullptr;       // nblocks * 4104
+  double* density = nullptr;   // nblocks * 216
+  double* velocity = nullptr;  // nblocks * 648
+  uint8_t* mask = nullptr;     // nblocks * 216 overlap cells of the last search (hasMinusOneEntries)
+  uint8_t* inner = nullptr;    // nblocks * 216 inner flags (1 = inner)
+  lbm::AmrLinkDev* d_links = nullptr;
+  int64_t cap_links = 0;
+  int64_t* d_list = nullptr;   // block list, or 8 blocks per Peano cell
+  int64_t cap_list = 0;
+  int64_t* d_cells = nullptr;  // Peano cells of lbm_blocks_iterate
+  int64_t cap_cells = 0;
+  lbm::BlockLinkDev* d_bnd = nullptr;  // boundary links grouped by list position
+  int64_t cap_bnd = 0;
+  int32_t* d_off = nullptr;    // per list position: first link (n + 1 entries)
+  int64_t cap_off = 0;
+  float last_ms = 0.f;
+};
+
+namespace lbm {
+
+inline int bfail(lbm_blockstore* bs, int code, const std::string& m) {
+  if (bs) bs->err = m;
+  return code;
+}
+
+#define BCHK(bs, call)                                                                                         \
+  do {                                                                                                         \
+    hipError_t e_ = (call);                                                                                    \
+    if (e_ != hipSuccess) return lbm::bfail(bs, LBM_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+inline int check_range(lbm_blockstore* bs, int64_t first, int64_t count) {
+  if (first < 0 || count < 0 || first > bs->nblocks || count > bs->nblocks - first)
+    return bfail(bs, LBM_ERR_ARG, "block range out of bounds");
+  return LBM_OK;
+}
+
+// grows a device array to at least n elements (contents are not kept)
+template <class T>
+int ensure_capacity(lbm_blockstore* bs, T** p, int64_t* cap, int64_t n, const char* what) {
+  if (n <= *cap) return LBM_OK;
+  if (*p) (void)hipFree(*p);
+  *p = nullptr;
+  *cap = 0;
+  if (device_alloc(p, (size_t)n * sizeof(T)) != hipSuccess)
+    return bfail(bs, LBM_ERR_NOMEM, std::string(what) + " allocation failed");
+  *cap = n;
+  return LBM_OK;
+}
+
+}  // namespace lbm

@@ -50,6 +50,9 @@ EXPORTS = (
     "lbm_patches_set_tiling", "lbm_patches_step", "lbm_patches_reduce_moments", "lbm_patches_set_flags",
     "lbm_blocks_create", "lbm_blocks_destroy", "lbm_blocks_last_error", "lbm_blocks_set", "lbm_blocks_get",
     "lbm_blocks_find_overlap", "lbm_blocks_set_overlap", "lbm_blocks_prolongate", "lbm_blocks_restrict", "lbm_blocks_last_ms",
+    "lbm_blocks_set_inner", "lbm_blocks_compute_cell_values", "lbm_blocks_collide", "lbm_blocks_treat_boundary",
+    "lbm_blocks_stream_local", "lbm_blocks_stream_nonlocal", "lbm_blocks_swap_inside", "lbm_blocks_vertex_first",
+    "lbm_blocks_iterate",
 )
 
 
@@ -78,6 +81,17 @@ class Patch(ctypes.Structure):
 class AmrLink(ctypes.Structure):
     _fields_ = [("coarse", ctypes.c_int64), ("fine", ctypes.c_int64), ("pos", ctypes.c_int * 3),
                 ("new_vertex", ctypes.c_int)]
+
+
+class BlockBoundary(ctypes.Structure):
+    """lbm_block_boundary: a BoundaryData record with its (cell, incoming direction) index."""
+    _fields_ = [("block", ctypes.c_int64), ("cell", ctypes.c_int32), ("q_in", ctypes.c_int32),
+                ("type", ctypes.c_int32), ("pad", ctypes.c_int32), ("u", ctypes.c_double * 3), ("rho", ctypes.c_double)]
+
+
+# BoundaryData::BoundaryType
+BOUNDARY_HW_BB, BOUNDARY_MOVING_WALL, BOUNDARY_PRESSURE_OUTLET, BOUNDARY_PRESSURE_INLET, BOUNDARY_VELOCITY_INLET, \
+    BOUNDARY_MOVING_OBSTACLE = range(6)
 
 
 class Tiling(ctypes.Structure):
@@ -163,6 +177,18 @@ def load_library(path: str = LIB_PATH):
     L.lbm_blocks_restrict.argtypes = [vp, lp, ctypes.c_int64]
     L.lbm_blocks_last_ms.argtypes = [vp]
     L.lbm_blocks_last_ms.restype = ctypes.c_float
+    i64p = ctypes.POINTER(ctypes.c_int64)
+    bp = ctypes.POINTER(BlockBoundary)
+    L.lbm_blocks_set_inner.argtypes = [vp, ctypes.c_int64, ctypes.c_int64, u8p]
+    L.lbm_blocks_compute_cell_values.argtypes = [vp, i64p, ctypes.c_int64, ctypes.c_int]
+    L.lbm_blocks_collide.argtypes = [vp, i64p, ctypes.c_int64, ctypes.c_double, ctypes.c_int]
+    L.lbm_blocks_treat_boundary.argtypes = [vp, bp, ctypes.c_int64]
+    L.lbm_blocks_stream_local.argtypes = [vp, i64p, ctypes.c_int64]
+    L.lbm_blocks_stream_nonlocal.argtypes = [vp, i64p, ctypes.c_int64]
+    L.lbm_blocks_swap_inside.argtypes = [vp, i64p, ctypes.c_int64]
+    L.lbm_blocks_vertex_first.argtypes = [vp, i64p, ctypes.c_int64, ctypes.c_double, ctypes.c_int, bp, ctypes.c_int64]
+    L.lbm_blocks_iterate.argtypes = [vp, i64p, ctypes.c_int64, i64p, ctypes.c_int64, ctypes.c_double, bp,
+                                     ctypes.c_int64, ctypes.c_int]
     _lib = L
     return L
 
@@ -604,3 +630,66 @@ class BlockStore:
 
     def last_ms(self):
         return float(self._L.lbm_blocks_last_ms(self._h))
+
+    # ----- the block solver's operators -----
+    @staticmethod
+    def _ids(ids, per=1):
+        a = np.ascontiguousarray(np.asarray(ids, dtype=np.int64).reshape(-1))
+        if a.size % per:
+            raise ValueError(f"id list: size {a.size} is not a multiple of {per}")
+        return a, a.size // per, a.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))
+
+    @staticmethod
+    def boundary_links(links):
+        """links: iterable of (block, cell, q_in, type, (u0, u1, u2), rho_L) -> lbm_block_boundary array."""
+        links = list(links)
+        arr = (BlockBoundary * max(1, len(links)))()
+        for k, (b, cell, qin, ty, u, rho) in enumerate(links):
+            arr[k].block, arr[k].cell, arr[k].q_in, arr[k].type = int(b), int(cell), int(qin), int(ty)
+            for d in range(3):
+                arr[k].u[d] = float(u[d])
+            arr[k].rho = float(rho)
+        return arr, len(links)
+
+    def set_inner(self, first, inner):
+        m = np.ascontiguousarray(np.asarray(inner).reshape(-1) != 0, dtype=np.uint8)
+        if m.size % BLOCK_CELLS:
+            raise ValueError("inner: not a whole number of blocks")
+        self._check(self._L.lbm_blocks_set_inner(self._h, int(first), m.size // BLOCK_CELLS, _u8(m)), "lbm_blocks_set_inner")
+
+    def compute_cell_values(self, blocks, use_overlap=False):
+        a, n, p = self._ids(blocks)
+        self._check(self._L.lbm_blocks_compute_cell_values(self._h, p, n, int(bool(use_overlap))),
+                    "lbm_blocks_compute_cell_values")
+
+    def collide(self, blocks, tau, use_overlap=False):
+        a, n, p = self._ids(blocks)
+        self._check(self._L.lbm_blocks_collide(self._h, p, n, float(tau), int(bool(use_overlap))), "lbm_blocks_collide")
+
+    def treat_boundary(self, links):
+        arr, n = self.boundary_links(links)
+        self._check(self._L.lbm_blocks_treat_boundary(self._h, arr, n), "lbm_blocks_treat_boundary")
+
+    def stream_local(self, blocks):
+        a, n, p = self._ids(blocks)
+        self._check(self._L.lbm_blocks_stream_local(self._h, p, n), "lbm_blocks_stream_local")
+
+    def stream_nonlocal(self, cells):
+        a, n, p = self._ids(cells, 8)
+        self._check(self._L.lbm_blocks_stream_nonlocal(self._h, p, n), "lbm_blocks_stream_nonlocal")
+
+    def swap_inside(self, blocks):
+        a, n, p = self._ids(blocks)
+        self._check(self._L.lbm_blocks_swap_inside(self._h, p, n), "lbm_blocks_swap_inside")
+
+    def vertex_first(self, blocks, tau, links=(), use_overlap=False):
+        a, n, p = self._ids(blocks)
+        arr, nl = self.boundary_links(links)
+        self._check(self._L.lbm_blocks_vertex_first(self._h, p, n, float(tau), int(bool(use_overlap)), arr, nl),
+                    "lbm_blocks_vertex_first")
+
+    def iterate(self, blocks, cells, tau, links=(), steps=1):
+        a, n, p = self._ids(blocks)
+        c, nc, pc = self._ids(cells, 8)
+        arr, nl = self.boundary_links(links)
+        self._check(self._L.lbm_blocks_iterate(self._h, p, n, pc, nc, float(tau), arr, nl, int(steps)), "lbm_blocks_iterate")

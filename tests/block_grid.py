@@ -82,16 +82,15 @@ class BlockGrid:
         Returns a list of (block, cell, q_in, type, u_L, rho_L)."""
         out = []
         dims = np.array(self.dims)
-        for blk in range(self.nblocks):
-            for cell in np.nonzero(self.inner[blk])[0]:
-                xyz = self.cell_xyz[blk, cell]
-                for q in range(Q):
-                    t = xyz + C[q]
-                    if ((t >= 0) & (t < dims)).all():
-                        continue
-                    r = classify(*xyz, q)
-                    if r is not None:
-                        out.append((blk, int(cell), Q - 1 - q, r[0], r[1], r[2]))
+        blk, cell = np.nonzero(self.inner)
+        xyz = self.cell_xyz[blk, cell]
+        for q in range(Q):
+            t = xyz + C[q]
+            leaves = ~((t >= 0) & (t < dims)).all(axis=1)
+            for b, c, (x, y, z) in zip(blk[leaves], cell[leaves], xyz[leaves]):
+                r = classify(int(x), int(y), int(z), q)
+                if r is not None:
+                    out.append((int(b), int(c), Q - 1 - q, r[0], r[1], r[2]))
         return out
 
     def cavity_links(self, ulid):
