@@ -261,6 +261,60 @@ def run_block_store(lbm, ncoarse=4096, reps=5):
     return out
 
 
+def regular_block_grid(n, ulid):
+    """The n^3 lid-driven cavity as the reference's regular-grid block solver lays it out: n // 6 + 1
+    (+ 1 if 6 does not divide n) blocks per axis, slot s of block b holding cell 6 b + s - 3, inner
+    cells inside the box, Peano cells between neighbouring vertices (8 blocks, lexicographic), and
+    the lid's moving-wall links (top layer, c_z > 0)."""
+    nb = n // 6 + 1 + (1 if n % 6 else 0)
+    b = np.arange(nb ** 3)
+    s = np.arange(216)
+    bxyz = np.stack([b % nb, (b // nb) % nb, b // (nb * nb)], axis=1)
+    sxyz = np.stack([s % 6, (s // 6) % 6, s // 36], axis=1)
+    inner = np.ones((nb ** 3, 216), bool)
+    for d in range(3):
+        g = 6 * bxyz[:, None, d] + sxyz[None, :, d] - 3
+        inner &= (g >= 0) & (g < n)
+        if d == 2:
+            top = inner & (g == n - 1)
+    c0 = np.arange(nb - 1)
+    cz, cy, cx = np.meshgrid(c0, c0, c0, indexing="ij")
+    base = (cx + nb * (cy + nb * cz)).reshape(-1)
+    cells = np.stack([base + (v & 1) + nb * ((v >> 1) & 1) + nb * nb * (v >> 2) for v in range(8)], axis=1)
+    blk, cell = np.nonzero(top)
+    links = [(int(bb), int(cc), 18 - q, 1, (ulid, 0.0, 0.0), 1.0) for bb, cc in zip(blk, cell) for q in range(14, 19)]
+    return nb ** 3, inner, cells.astype(np.int64), links
+
+
+def run_block_solver(lbm, sizes=(64, 256), steps=20, tau=0.6, ulid=0.05):
+    """The reference's own three-sweep algorithm on the device block store (lbm_blocks_iterate): per
+    iteration one launch over all blocks (the in-cell swap of the previous iteration, cell values,
+    BGK, lid links, local streaming, each block staged once through LDS) and one over all Peano cells
+    (cross-block swaps in place).  Bytes per iteration: 2 x 4104 x 8 B of populations + 864 x 8 B of
+    density and velocity + 216 inner bytes per block, and 504 x 32 B per Peano cell."""
+    out = {}
+    w = np.array([1 / 36, 1 / 36, 1 / 18, 1 / 36, 1 / 36, 1 / 36, 1 / 18, 1 / 36, 1 / 18, 1 / 3,
+                  1 / 18, 1 / 36, 1 / 18, 1 / 36, 1 / 36, 1 / 36, 1 / 18, 1 / 36, 1 / 36])
+    tau_eff = 3.0 * (1.0 / 3.0 * (tau - 0.5)) + 0.5
+    for n in sizes:
+        nblocks, inner, cells, links = regular_block_grid(n, ulid)
+        with lbm.BlockStore(nblocks) as bs:
+            bs.set(0, pdf=np.broadcast_to(np.tile(w, 216), (nblocks, 216 * 19)), density=np.ones((nblocks, 216)))
+            bs.set_inner(0, inner)
+            blocks = np.arange(nblocks)
+            bs.iterate(blocks, cells, tau_eff, links, 2)
+            bs.iterate(blocks, cells, tau_eff, links, steps)
+            ms = bs.last_ms() / steps
+        nbytes = nblocks * (2 * 4104 * 8 + 864 * 8 + 216) + len(cells) * 504 * 32
+        gbs = nbytes / (ms * 1e-3) / 1e9
+        out[f"cavity {n}^3"] = {"blocks": nblocks, "peano_cells": len(cells), "links": len(links), "steps": steps,
+                                "ms_per_step": round(ms, 4), "mlups": round(n ** 3 / (ms * 1e-3) / 1e6, 1),
+                                "bytes_per_step": nbytes, "GBs": round(gbs, 1), "roofline_frac": round(gbs / HBM_PEAK_GBS, 4)}
+        log(f"[bench] block solver {n}^3 cavity: {out[f'cavity {n}^3']['mlups']:.0f} MLUPS, {ms:.3f} ms/step, "
+            f"{gbs:.0f} GB/s")
+    return out
+
+
 def cpu_baseline(wl, n=128, seed=None, tau=0.6, g=1e-6, budget_s=12.0):
     import oracle
 
@@ -307,6 +361,11 @@ def main():
     ap.add_argument("--all-fluid-sizes", default="512,256",
                     help="edges of the all-fluid Taylor-Green legs (the porous channel's size runs --steps steps)")
     ap.add_argument("--patch-steps", type=int, default=500)
+    ap.add_argument("--block-solver-sizes", default="64,256",
+                    help="edges of the lid-driven cavities run through the block solver's operators")
+    ap.add_argument("--block-steps", type=int, default=20)
+    ap.add_argument("--only-block-solver", action="store_true",
+                    help="run only the block solver leg and print its results as one JSON line")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: madpac_amd has no multi-GPU path (the domain would shard "
@@ -315,6 +374,11 @@ def main():
         sys.exit(2)
 
     from madpac_amd import lbm, workloads as wl
+
+    block_sizes = tuple(int(v) for v in args.block_solver_sizes.split(",") if v)
+    if args.only_block_solver:
+        print(json.dumps(run_block_solver(lbm, sizes=block_sizes, steps=args.block_steps)), flush=True)
+        return
 
     n = args.size
     t0 = time.perf_counter()
@@ -355,6 +419,7 @@ def main():
     amr = run_block_store(lbm)
     log(f"[bench] block store {amr['workload']}: " + ", ".join(
         f"{k} {amr[k]['ms']:.3f} ms ({amr[k]['GBs']:.0f} GB/s)" for k in ("find_overlap", "prolongate", "restrict")))
+    blk = run_block_solver(lbm, sizes=block_sizes, steps=args.block_steps)
     adj = None
     if not args.no_adjoint:
         adj = run_adjoint(lbm, wl)
@@ -400,6 +465,7 @@ def main():
         "slab_share": shard,
         "adjoint": adj,
         "restriction_prolongation": amr,
+        "block_solver": blk,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -59,12 +59,13 @@ def build(verbose: bool = False) -> str:
     return LIB
 
 
-CPP_TESTS = (("test_peano_api", True), ("test_peano_events", True), ("test_peano_amr", True), ("test_geometry", False))
+CPP_TESTS = (("test_peano_api", True), ("test_peano_events", True), ("test_peano_amr", True), ("test_peano_blockops", True),
+             ("test_geometry", False))
 
 
 def build_cpp_test(verbose: bool = False) -> str:
-    """Compile the C++ programs of tests/cpp: test_peano_api, test_peano_events and test_peano_amr (the
-    reference-shaped API, linked against the library) and test_geometry (the block store's geometry
+    """Compile the C++ programs of tests/cpp: test_peano_api, test_peano_events, test_peano_amr and
+    test_peano_blockops (the reference-shaped API, linked against the library) and test_geometry (the block store's geometry
     flags, host only)."""
     exe = None
     for name, links in CPP_TESTS:

@@ -10,8 +10,9 @@
 // list of blocks (or Peano cells), one workgroup per block (or cell).
 //
 // Per-block operators stage the block's 4,104 populations (32.8 KB, AoS as the store keeps them)
-// through LDS with coalesced loads and stores, run the chosen operators there in the reference's
-// order, and write the block back once.  Fused, touchVertexFirstTime's four kernels move each
+// through LDS with coalesced 16-byte loads and stores, run the chosen operators there in the
+// reference's order, and write the block back once.  (Persistent workgroups that load the next
+// block into registers while one is in LDS measured 7 % slower at 256^3: they need 128 VGPRs.)  Fused, touchVertexFirstTime's four kernels move each
 // block across HBM once instead of four times; with the next iteration's in-cell swap folded in,
 // an iteration of the regular grid is two launches.  The cross-block swap touches only the
 // boundary-crossing populations and works in place in HBM: its swaps are disjoint (each
@@ -107,117 +108,138 @@ __device__ __forceinline__ void block_boundary_f64(double& f, const BlockLinkDev
   f = L.type == 4 ? fin : -f + fin + feq_f64(q, rt, t0, t1, t2, uu);
 }
 
+// The block in 16-byte pieces: a block is 2,052 of them (8 rounds of 256 and 4) and starts on a
+// 16-byte boundary.
+constexpr int BP2 = BP / 2, NFULL = BP2 / 256, TAIL = BP2 - 256 * NFULL;
+static_assert(NFULL == 8 && TAIL == 4, "the staging below is written for 8 full rounds and 4 pieces");
+
 template <unsigned OPS>
 __global__ __launch_bounds__(256) void k_block_ops(BlockOpArgs a) {
   constexpr bool WRITES = (OPS & (OP_SWAP_IN | OP_COLLIDE | OP_BOUNDARY | OP_LOCAL)) != 0;
   constexpr bool MOMENTS = (OPS & (OP_VALUES | OP_COLLIDE | OP_BOUNDARY)) != 0;
-  __shared__ double s[BP];
+  __shared__ __align__(16) double s[BP];
   __shared__ double srho[MOMENTS ? BC : 1];
   __shared__ double su[MOMENTS ? BV : 1];
   __shared__ uint8_t sin[BC];
+  double2* s2 = reinterpret_cast<double2*>(s);
+  const int t = threadIdx.x;
   const int64_t p = blockIdx.x;
   if (p >= a.n) return;
   const int64_t b = a.list[p];
-  const int t = threadIdx.x;
-  double* g = a.pdf + b * BP;
-  for (int k = t; k < BP; k += 256) s[k] = g[k];
+  double2* g = reinterpret_cast<double2*>(a.pdf + b * BP);
+  {
+    // all loads of a thread are issued before the first LDS write, so a block's 32.8 KB are in
+    // flight at once
+    const double2 r0 = g[t], r1 = g[t + 256], r2 = g[t + 512], r3 = g[t + 768];
+    const double2 r4 = g[t + 1024], r5 = g[t + 1280], r6 = g[t + 1536], r7 = g[t + 1792];
+    const double2 r8 = t < TAIL ? g[t + 2048] : double2{0.0, 0.0};
+    s2[t] = r0; s2[t + 256] = r1; s2[t + 512] = r2; s2[t + 768] = r3;
+    s2[t + 1024] = r4; s2[t + 1280] = r5; s2[t + 1536] = r6; s2[t + 1792] = r7;
+    if (t < TAIL) s2[t + 2048] = r8;
+  }
   if (t < BC) sin[t] = a.inner[b * BC + t];
   __syncthreads();
-  const int c = t;
-  if (c < BC) {
-    double* f = s + c * Q;
-    if (OPS & OP_SWAP_IN) {
-      // the previous iteration's swapPdfsInsideCells (every cell, inner or not)
+  {
+    const int c = t;
+    if (c < BC) {
+      double* f = s + c * Q;
+      if (OPS & OP_SWAP_IN) {
+        // the previous iteration's swapPdfsInsideCells (every cell, inner or not)
 #pragma unroll
-      for (int q = 0; q < Q / 2; q++) {
-        const double v = f[q];
-        f[q] = f[Q - 1 - q];
-        f[Q - 1 - q] = v;
-      }
-    }
-    if (MOMENTS) {
-      const bool act = sin[c] && !(a.mask && a.mask[b * BC + c]);
-      double rho, u0, u1, u2;
-      if (OPS & OP_VALUES) {
-        // computeDensity (:36-39) and computeVelocity (:57-61); skipped cells keep their density
-        // and get u = 0
-        rho = a.density[b * BC + c];
-        u0 = u1 = u2 = 0.0;
-        if (act) {
-          rho = 0.0;
-#pragma unroll
-          for (int q = 0; q < Q; q++) rho += f[q];
-          double b0 = 0.0, b1 = 0.0, b2 = 0.0;
-#pragma unroll
-          for (int q = 0; q < Q; q++) {
-            if (CX(q) == 1) b0 += f[q];
-            if (CX(q) == -1) b0 -= f[q];
-            if (CY(q) == 1) b1 += f[q];
-            if (CY(q) == -1) b1 -= f[q];
-            if (CZ(q) == 1) b2 += f[q];
-            if (CZ(q) == -1) b2 -= f[q];
-          }
-          const double inv = 1.0 / rho;
-          u0 = b0 * inv;
-          u1 = b1 * inv;
-          u2 = b2 * inv;
-          a.density[b * BC + c] = rho;
+        for (int q = 0; q < Q / 2; q++) {
+          const double v = f[q];
+          f[q] = f[Q - 1 - q];
+          f[Q - 1 - q] = v;
         }
-        double* uo = a.velocity + b * BV + 3 * c;
-        uo[0] = u0;
-        uo[1] = u1;
-        uo[2] = u2;
-      } else {
-        rho = a.density[b * BC + c];
-        const double* ui = a.velocity + b * BV + 3 * c;
-        u0 = ui[0];
-        u1 = ui[1];
-        u2 = ui[2];
       }
-      srho[c] = rho;
-      su[3 * c] = u0;
-      su[3 * c + 1] = u1;
-      su[3 * c + 2] = u2;
-      if ((OPS & OP_COLLIDE) && act) {
-        // computeFeq (BLB/BlockComputeCellValues.cpp:14-20) and BlockBGK (BlockBGK.cpp:43)
-        double uu = u0 * u0;
-        uu += u1 * u1;
-        uu += u2 * u2;
+      if (MOMENTS) {
+        const bool act = sin[c] && !(a.mask && a.mask[b * BC + c]);
+        double rho, u0, u1, u2;
+        if (OPS & OP_VALUES) {
+          // computeDensity (:36-39) and computeVelocity (:57-61); skipped cells keep their density
+          // and get u = 0
+          rho = a.density[b * BC + c];
+          u0 = u1 = u2 = 0.0;
+          if (act) {
+            rho = 0.0;
 #pragma unroll
-        for (int q = 0; q < Q; q++) f[q] -= a.omega * (f[q] - feq_f64(q, rho, u0, u1, u2, uu));
-      }
-    }
-  }
-  if (OPS & OP_BOUNDARY) {
-    __syncthreads();
-    const int32_t l1 = a.off[p + 1];
-    for (int32_t k = a.off[p] + t; k < l1; k += 256) {
-      const BlockLinkDev L = a.links[k];
-      block_boundary_f64(s[L.cell * Q + Q - 1 - L.q_in], L, srho[L.cell], su[3 * L.cell], su[3 * L.cell + 1],
-                         su[3 * L.cell + 2]);
-    }
-  }
-  if (OPS & OP_LOCAL) {
-    // streamingLocal: the link (c, q < 9) <-> (c + c_q, 18 - q) is swapped by thread c when both
-    // cells are inner; every slot belongs to one link, so the swaps are disjoint
-    __syncthreads();
-    if (c < BC && sin[c]) {
-      const int x = c % BS, y = (c / BS) % BS, z = c / (BS * BS);
+            for (int q = 0; q < Q; q++) rho += f[q];
+            double b0 = 0.0, b1 = 0.0, b2 = 0.0;
 #pragma unroll
-      for (int q = 0; q < Q / 2; q++) {
-        const int nx = x + CX(q), ny = y + CY(q), nz = z + CZ(q);
-        if (nx < 0 || nx >= BS || ny < 0 || ny >= BS || nz < 0 || nz >= BS) continue;
-        const int nb = nx + BS * (ny + BS * nz);
-        if (!sin[nb]) continue;
-        const double v = s[c * Q + q];
-        s[c * Q + q] = s[nb * Q + Q - 1 - q];
-        s[nb * Q + Q - 1 - q] = v;
+            for (int q = 0; q < Q; q++) {
+              if (CX(q) == 1) b0 += f[q];
+              if (CX(q) == -1) b0 -= f[q];
+              if (CY(q) == 1) b1 += f[q];
+              if (CY(q) == -1) b1 -= f[q];
+              if (CZ(q) == 1) b2 += f[q];
+              if (CZ(q) == -1) b2 -= f[q];
+            }
+            const double inv = 1.0 / rho;
+            u0 = b0 * inv;
+            u1 = b1 * inv;
+            u2 = b2 * inv;
+            a.density[b * BC + c] = rho;
+          }
+          double* uo = a.velocity + b * BV + 3 * c;
+          uo[0] = u0;
+          uo[1] = u1;
+          uo[2] = u2;
+        } else {
+          rho = a.density[b * BC + c];
+          const double* ui = a.velocity + b * BV + 3 * c;
+          u0 = ui[0];
+          u1 = ui[1];
+          u2 = ui[2];
+        }
+        srho[c] = rho;
+        su[3 * c] = u0;
+        su[3 * c + 1] = u1;
+        su[3 * c + 2] = u2;
+        if ((OPS & OP_COLLIDE) && act) {
+          // computeFeq (BLB/BlockComputeCellValues.cpp:14-20) and BlockBGK (BlockBGK.cpp:43)
+          double uu = u0 * u0;
+          uu += u1 * u1;
+          uu += u2 * u2;
+#pragma unroll
+          for (int q = 0; q < Q; q++) f[q] -= a.omega * (f[q] - feq_f64(q, rho, u0, u1, u2, uu));
+        }
       }
     }
-  }
-  if (WRITES) {
-    __syncthreads();
-    for (int k = t; k < BP; k += 256) g[k] = s[k];
+    if (OPS & OP_BOUNDARY) {
+      __syncthreads();
+      const int32_t l1 = a.off[p + 1];
+      for (int32_t k = a.off[p] + t; k < l1; k += 256) {
+        const BlockLinkDev L = a.links[k];
+        block_boundary_f64(s[L.cell * Q + Q - 1 - L.q_in], L, srho[L.cell], su[3 * L.cell], su[3 * L.cell + 1],
+                           su[3 * L.cell + 2]);
+      }
+    }
+    if (OPS & OP_LOCAL) {
+      // streamingLocal: the link (c, q < 9) <-> (c + c_q, 18 - q) is swapped by thread c when both
+      // cells are inner; every slot belongs to one link, so the swaps are disjoint
+      __syncthreads();
+      if (c < BC && sin[c]) {
+        const int x = c % BS, y = (c / BS) % BS, z = c / (BS * BS);
+#pragma unroll
+        for (int q = 0; q < Q / 2; q++) {
+          const int nx = x + CX(q), ny = y + CY(q), nz = z + CZ(q);
+          if (nx < 0 || nx >= BS || ny < 0 || ny >= BS || nz < 0 || nz >= BS) continue;
+          const int nb = nx + BS * (ny + BS * nz);
+          if (!sin[nb]) continue;
+          const double v = s[c * Q + q];
+          s[c * Q + q] = s[nb * Q + Q - 1 - q];
+          s[nb * Q + Q - 1 - q] = v;
+        }
+      }
+    }
+    if (WRITES) {
+      __syncthreads();
+      const double2 w0 = s2[t], w1 = s2[t + 256], w2 = s2[t + 512], w3 = s2[t + 768];
+      const double2 w4 = s2[t + 1024], w5 = s2[t + 1280], w6 = s2[t + 1536], w7 = s2[t + 1792];
+      g[t] = w0; g[t + 256] = w1; g[t + 512] = w2; g[t + 768] = w3;
+      g[t + 1024] = w4; g[t + 1280] = w5; g[t + 1536] = w6; g[t + 1792] = w7;
+      if (t < TAIL) g[t + 2048] = s2[t + 2048];
+    }
   }
 }
 

@@ -14,6 +14,10 @@
 //        getDensity/getVelocity (6^3 blocks, vertex numbers from 8, written back)     -> GridManagementService
 //   peano/geometry/builtin Hexahedron, Sphere, intersected (builtin-geometry tag)     -> Configuration::geometry
 //   BLB/BlockRestrictionProlongation (restriction / prolongation between levels)      -> BlockRestrictionProlongation
+//   the kernels the mappings call per block / per Peano cell: BLB/BlockComputeCellValues,
+//        collisionmodels/BlockBGK, boundaryconditions/BlockBoundaryConditionManager,
+//        nonlocaltreatment/AABlockStreaming (batched over vertex lists, device block store)
+//        -> DeviceBlocks, BlockComputeCellValues, BlockBGK, BlockBoundaryConditionManager, AABlockStreaming
 //   LB/PerformanceMeasurement (MLUPS per iteration; wall clock here, not CPU time)
 // The three sweeps of one reference iteration (per-block collide + local streaming, the
 // cross-block streaming of handleCell, the in-cell swap) are one fused device step here.  A mapping
@@ -357,6 +361,99 @@ class BlockRestrictionProlongation {
     if (rc != LBM_OK) throw std::runtime_error(std::string(what) + ": " + lbm_blocks_last_error(bs_));
   }
   lbm_blockstore* bs_ = nullptr;
+};
+
+// The block solver's kernels for code that drives them itself, as faxen/lbf's solver mapping reuses
+// them (peano/applications/faxen/lbf/mappings/RegularGrid2RegularBlockSolver.cpp:111,184,205): the
+// reference's classes and method names, with the blocks in a device store addressed by vertex
+// number instead of one host block per call.  Each method takes the list of vertices (or Peano
+// cells) that the caller's traversal would visit one at a time and runs them in one launch, so a
+// mapping collects its touchVertexFirstTime vertices, its handleCell cells and its
+// touchVertexLastTime vertices and makes one call per kernel.  A vertex appears once per list and
+// at one vertex position once per cell list (lbm_blocks_* return LBM_ERR_ARG otherwise, thrown
+// here as std::runtime_error).
+class DeviceBlocks {
+ public:
+  explicit DeviceBlocks(int64_t nblocks, int device = 0) {
+    if (lbm_blocks_create(&bs_, nblocks, device) != LBM_OK)
+      throw std::runtime_error(std::string("lbm_blocks_create: ") + lbm_blocks_last_error(nullptr));
+  }
+  ~DeviceBlocks() { lbm_blocks_destroy(bs_); }
+  DeviceBlocks(const DeviceBlocks&) = delete;
+  DeviceBlocks& operator=(const DeviceBlocks&) = delete;
+  lbm_blockstore* handle() const { return bs_; }
+  // one vertex's block in or out (GridManagementService::VertexData: pdf, density, velocity, inner)
+  void setBlock(int64_t vertex, const BlockView& b) {
+    check(lbm_blocks_set(bs_, vertex, 1, b.pdf.empty() ? nullptr : b.pdf.data(),
+                         b.density.empty() ? nullptr : b.density.data(), b.velocity.empty() ? nullptr : b.velocity.data()),
+          "lbm_blocks_set");
+    if (!b.inner.empty()) check(lbm_blocks_set_inner(bs_, vertex, 1, b.inner.data()), "lbm_blocks_set_inner");
+  }
+  void getBlock(int64_t vertex, BlockView& b) const {
+    b.vertexNumber = (unsigned)vertex;
+    b.pdf.resize(LB_BLOCK_NUMBER_OF_CELLS * LB_CURRENT_DIR);
+    b.density.resize(LB_BLOCK_NUMBER_OF_CELLS);
+    b.velocity.resize(3 * LB_BLOCK_NUMBER_OF_CELLS);
+    check(lbm_blocks_get(bs_, vertex, 1, b.pdf.data(), b.density.data(), b.velocity.data()), "lbm_blocks_get");
+  }
+  void check(int rc, const char* what) const {
+    if (rc != LBM_OK) throw std::runtime_error(std::string(what) + ": " + lbm_blocks_last_error(bs_));
+  }
+
+ private:
+  lbm_blockstore* bs_ = nullptr;
+};
+
+// BLB/BlockComputeCellValues.cpp:26-68: computeDensity then computeVelocity on each listed block;
+// hasMinusOneEntries = the blocks' overlap cells of their last overlap search
+class BlockComputeCellValues {
+ public:
+  void computeDensityAndVelocity(DeviceBlocks& s, const std::vector<int64_t>& vertices,
+                                 bool hasMinusOneEntries = false) const {
+    s.check(lbm_blocks_compute_cell_values(s.handle(), vertices.data(), (int64_t)vertices.size(), hasMinusOneEntries),
+            "lbm_blocks_compute_cell_values");
+  }
+};
+
+// BLB/collisionmodels/BlockBGK.cpp:18-45 through CollisionModelWrapper::collide (:28-39)
+class BlockBGK {
+ public:
+  explicit BlockBGK(double tau) : tau_(tau) {}
+  double getTau() const { return tau_; }
+  void collide(DeviceBlocks& s, const std::vector<int64_t>& vertices, bool hasMinusOneEntries = false) const {
+    s.check(lbm_blocks_collide(s.handle(), vertices.data(), (int64_t)vertices.size(), tau_, hasMinusOneEntries),
+            "lbm_blocks_collide");
+  }
+
+ private:
+  double tau_;
+};
+
+// BLB/boundaryconditions/BlockBoundaryConditionManager.cpp:28-62; the links are the
+// (BoundaryData, boundaryDataIndex) pairs of ReceiveBoundaryDataService::getBoundaryData with the
+// block they belong to
+class BlockBoundaryConditionManager {
+ public:
+  void treatBoundaryBeforeCollision(DeviceBlocks&, const std::vector<lbm_block_boundary>&) const {}  // no-op for every condition
+  void treatBoundaryAfterCollision(DeviceBlocks& s, const std::vector<lbm_block_boundary>& links) const {
+    s.check(lbm_blocks_treat_boundary(s.handle(), links.data(), (int64_t)links.size()), "lbm_blocks_treat_boundary");
+  }
+};
+
+// BLB/nonlocaltreatment/AABlockStreaming.cpp:19-92; a Peano cell is its 8 vertices in lexicographic
+// order (vertices[enumerator(i)], i = 0..7)
+class AABlockStreaming {
+ public:
+  void streamingLocal(DeviceBlocks& s, const std::vector<int64_t>& vertices) const {
+    s.check(lbm_blocks_stream_local(s.handle(), vertices.data(), (int64_t)vertices.size()), "lbm_blocks_stream_local");
+  }
+  void streamingNonLocal(DeviceBlocks& s, const std::vector<std::array<int64_t, 8>>& cells) const {
+    s.check(lbm_blocks_stream_nonlocal(s.handle(), cells.empty() ? nullptr : cells[0].data(), (int64_t)cells.size()),
+            "lbm_blocks_stream_nonlocal");
+  }
+  void swapPdfsInsideCells(DeviceBlocks& s, const std::vector<int64_t>& vertices) const {
+    s.check(lbm_blocks_swap_inside(s.handle(), vertices.data(), (int64_t)vertices.size()), "lbm_blocks_swap_inside");
+  }
 };
 
 class Repository;

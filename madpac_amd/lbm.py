@@ -639,17 +639,21 @@ class BlockStore:
             raise ValueError(f"id list: size {a.size} is not a multiple of {per}")
         return a, a.size // per, a.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))
 
-    @staticmethod
-    def boundary_links(links):
-        """links: iterable of (block, cell, q_in, type, (u0, u1, u2), rho_L) -> lbm_block_boundary array."""
-        links = list(links)
-        arr = (BlockBoundary * max(1, len(links)))()
-        for k, (b, cell, qin, ty, u, rho) in enumerate(links):
-            arr[k].block, arr[k].cell, arr[k].q_in, arr[k].type = int(b), int(cell), int(qin), int(ty)
-            for d in range(3):
-                arr[k].u[d] = float(u[d])
-            arr[k].rho = float(rho)
-        return arr, len(links)
+    LINK_DTYPE = np.dtype([("block", "<i8"), ("cell", "<i4"), ("q_in", "<i4"), ("type", "<i4"), ("pad", "<i4"),
+                           ("u", "<f8", (3,)), ("rho", "<f8")])
+
+    @classmethod
+    def boundary_links(cls, links):
+        """links: iterable of (block, cell, q_in, type, (u0, u1, u2), rho_L), or an array of LINK_DTYPE
+        -> (pointer to lbm_block_boundary records, count, owner)."""
+        if isinstance(links, np.ndarray) and links.dtype == cls.LINK_DTYPE:
+            arr = np.ascontiguousarray(links)
+        else:
+            arr = np.array([(int(b), int(c), int(q), int(t), 0, tuple(float(x) for x in u), float(r))
+                            for b, c, q, t, u, r in links], dtype=cls.LINK_DTYPE)
+        assert cls.LINK_DTYPE.itemsize == ctypes.sizeof(BlockBoundary)
+        keep = arr if arr.size else np.zeros(1, cls.LINK_DTYPE)
+        return keep.ctypes.data_as(ctypes.POINTER(BlockBoundary)), int(arr.size), keep
 
     def set_inner(self, first, inner):
         m = np.ascontiguousarray(np.asarray(inner).reshape(-1) != 0, dtype=np.uint8)
@@ -667,7 +671,7 @@ class BlockStore:
         self._check(self._L.lbm_blocks_collide(self._h, p, n, float(tau), int(bool(use_overlap))), "lbm_blocks_collide")
 
     def treat_boundary(self, links):
-        arr, n = self.boundary_links(links)
+        arr, n, _keep = self.boundary_links(links)
         self._check(self._L.lbm_blocks_treat_boundary(self._h, arr, n), "lbm_blocks_treat_boundary")
 
     def stream_local(self, blocks):
@@ -684,12 +688,12 @@ class BlockStore:
 
     def vertex_first(self, blocks, tau, links=(), use_overlap=False):
         a, n, p = self._ids(blocks)
-        arr, nl = self.boundary_links(links)
+        arr, nl, _keep = self.boundary_links(links)
         self._check(self._L.lbm_blocks_vertex_first(self._h, p, n, float(tau), int(bool(use_overlap)), arr, nl),
                     "lbm_blocks_vertex_first")
 
     def iterate(self, blocks, cells, tau, links=(), steps=1):
         a, n, p = self._ids(blocks)
         c, nc, pc = self._ids(cells, 8)
-        arr, nl = self.boundary_links(links)
+        arr, nl, _keep = self.boundary_links(links)
         self._check(self._L.lbm_blocks_iterate(self._h, p, n, pc, nc, float(tau), arr, nl, int(steps)), "lbm_blocks_iterate")

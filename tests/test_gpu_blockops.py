@@ -3,12 +3,16 @@ _treat_boundary, _stream_local, _stream_nonlocal, _swap_inside, _vertex_first, _
 C-ABI, bit-exact against the compiled reference's fixtures (cell kernels, streaming tables and
 permutations, cavity and channel trajectories of the regular-grid block solver) and against the
 oracle's restatement on seeded blocks."""
+import hashlib
+import os
+import subprocess
+
 import numpy as np
 import pytest
 
 import oracle as O
 from block_grid import NC, Q, BlockGrid
-from conftest import golden
+from conftest import ROOT, golden
 from test_oracle_blocks import OracleOps, initial_state
 
 pytestmark = pytest.mark.gpu
@@ -184,7 +188,6 @@ def test_regular_grid_channel_trajectories(name, mode):
 
 
 def test_regular_grid_cavity64_1000_steps():
-    import hashlib
     d = golden("cavity_n64_t1000.npz")
     print("[gpu] block solver iterate on the 64^3 cavity, 1000 steps", flush=True)
     grid = BlockGrid(64, 64, 64)
@@ -215,3 +218,25 @@ def test_invalid_lists_are_rejected():
             bs.treat_boundary([(0, 5, 3, 7, (0, 0, 0), 1.0)])
         with pytest.raises(lbm.LbmError, match="tau"):
             bs.collide([0], 0.0)
+
+
+BLOCKOPS_EXE = os.path.join(ROOT, "tests", "cpp", "test_peano_blockops")
+
+
+@pytest.mark.parametrize("case", ["fixture", "oracle"])
+def test_cpp_block_kernels_drive_the_cavity(tmp_path, case):
+    """The C++ layer's kernel classes (BlockComputeCellValues, BlockBGK, BlockBoundaryConditionManager,
+    AABlockStreaming on DeviceBlocks) composed as the regular-grid mapping calls them."""
+    if case == "fixture":
+        d = golden("cavity_n13_t5.npz")
+        n, tau, ulid, steps = int(d["n"]), float(d["tau"]), float(d["ulid"]), int(d["steps"])
+        ref = d["f"].reshape(-1)
+    else:
+        n, tau, ulid, steps = 12, 0.7, 0.08, 40
+        lat = O.Lattice.cavity(n, tau, ulid)
+        ref = lat.step(lat.equilibrium(), steps)
+    print(f"[gpu] C++ block kernels on the {n}^3 cavity, {steps} steps", flush=True)
+    out = tmp_path / "f.bin"
+    subprocess.run([BLOCKOPS_EXE, str(n), repr(tau), repr(ulid), str(steps), str(out)], check=True, timeout=120)
+    f = np.fromfile(out, dtype=np.float64)
+    assert np.array_equal(f, ref)
