@@ -247,6 +247,7 @@ def main():
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--precisions", default="fp32s,fp64")
     ap.add_argument("--layout", choices=["two-buffer", "aa"], default="two-buffer")
+    ap.add_argument("--every", type=int, default=250, help="tgv: steps between kinetic-energy readings")
     ap.add_argument("--out", default=None)
     a = ap.parse_args()
     if a.what == "precision":
@@ -257,7 +258,7 @@ def main():
         r = patches_precision(steps=a.steps or 500)
     else:
         r = taylor_green(n=a.size or 512, steps=a.steps or 5000, precisions=tuple(a.precisions.split(",")),
-                         layout=lbm.LAYOUT_AA if a.layout == "aa" else lbm.LAYOUT_TWO_BUFFER)
+                         layout=lbm.LAYOUT_AA if a.layout == "aa" else lbm.LAYOUT_TWO_BUFFER, every=a.every)
     s = json.dumps(r)
     if a.out:
         with open(a.out, "w") as fh:
