@@ -399,9 +399,11 @@ int lbm_blocks_vertex_first(lbm_blockstore* bs, const int64_t* blocks, int64_t n
                             const lbm_block_boundary* links, int64_t nlinks);
 /* nsteps iterations of the regular-grid block solver (no MD coupling, no dynamic geometry):
  * touchVertexFirstTime on every listed block, handleCell on every listed Peano cell,
- * touchVertexLastTime on every listed block.  The in-cell swap of one iteration runs in the same
- * launch as the next iteration's first touch (two launches per iteration); the results equal the
- * separate operators in the reference's order. */
+ * touchVertexLastTime on every listed block.  The cross-block swaps and the in-cell swap of one
+ * iteration run in the same launch as the next iteration's first touch (one launch per iteration);
+ * the results equal the separate operators in the reference's order.  Every block of a listed cell
+ * must be in the block list, and the cells must agree on which block neighbours which (LBM_ERR_ARG
+ * otherwise). */
 int lbm_blocks_iterate(lbm_blockstore* bs, const int64_t* blocks, int64_t nblocks_list, const int64_t* cells,
                        int64_t ncells, double tau, const lbm_block_boundary* links, int64_t nlinks, int nsteps);
 

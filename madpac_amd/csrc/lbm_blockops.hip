@@ -12,13 +12,16 @@
 // Per-block operators stage the block's 4,104 populations (32.8 KB, AoS as the store keeps them)
 // through LDS with coalesced 16-byte loads and stores, run the chosen operators there in the
 // reference's order, and write the block back once.  (Persistent workgroups that load the next
-// block into registers while one is in LDS measured 7 % slower at 256^3: they need 128 VGPRs.)  Fused, touchVertexFirstTime's four kernels move each
-// block across HBM once instead of four times; with the next iteration's in-cell swap folded in,
-// an iteration of the regular grid is two launches.  The cross-block swap touches only the
-// boundary-crossing populations and works in place in HBM: its swaps are disjoint (each
-// cross-block link belongs to exactly one Peano cell and one entry of its table), so all cells of a
-// call run concurrently.  All of it is HBM-bound byte movement plus the FP64 collision, computed in
-// the reference's operation order (bit-identical; the build uses -ffp-contract=off).
+// block into registers while one is in LDS measured 7 % slower at 256^3: they need 128 VGPRs.)
+// Fused, touchVertexFirstTime's four kernels move each block across HBM once instead of four
+// times.  The cross-block swap (lbm_blocks_stream_nonlocal) touches only the boundary-crossing
+// populations and works in place in HBM: its swaps are disjoint (each cross-block link belongs to
+// exactly one Peano cell and one entry of its table), so all cells of a call run concurrently.
+// Whole iterations (lbm_blocks_iterate) take one launch each: the block pass writes its outgoing
+// crossing values into an exchange array and the next pass applies the swaps from there, before
+// the in-cell swap and the collision.  All of it is HBM-bound byte movement plus the FP64
+// collision, computed in the reference's operation order (bit-identical; the build uses
+// -ffp-contract=off).
 #include <algorithm>
 #include <string>
 #include <vector>
@@ -69,7 +72,110 @@ static_assert(make_nonlocal_table().n == NL_ENTRIES, "D3Q19 on 6^3 blocks has 50
 __constant__ NonLocalTable kNonLocal = make_nonlocal_table();
 
 // ---------------------------------------------------------------------------------------------
-enum : unsigned { OP_SWAP_IN = 1u, OP_VALUES = 2u, OP_COLLIDE = 4u, OP_BOUNDARY = 8u, OP_LOCAL = 16u };
+// A block's boundary-crossing populations: the 1,008 entries (cell, q) whose target cell + c_q lies
+// in a neighbour block, across one of 18 faces and edges (a D3Q19 link never crosses a corner).
+// Entries are grouped by direction; the segment of a direction lists its entries in (cell, q)
+// order, and the segment of the opposite direction lists their partners (cell + c_q wrapped into
+// the neighbour, 18 - q) in the same order, so entry k of a segment pairs with entry k of the
+// neighbour's opposite segment.  Packed cell | q << 8 | direction << 13.
+constexpr int NX_ENTRIES = 1008, NDIR = 18;
+struct CrossTable {
+  uint32_t e[NX_ENTRIES];
+  int start[NDIR], len[NDIR], opp[NDIR];
+  int16_t entry_of[BP];  // crossing entry of (cell, q), or -1
+  int n;
+};
+
+// direction index of each neighbour offset (ox + 1) + 3 (oy + 1) + 9 (oz + 1) with one or two
+// nonzero components, numbered in z, y, x order; -1 for the centre and the corners
+struct DirMap {
+  int d[27];
+};
+constexpr DirMap make_dir_map() {
+  DirMap m{};
+  int n = 0;
+  for (int k = 0; k < 27; k++) {
+    const int x = k % 3 - 1, y = (k / 3) % 3 - 1, z = k / 9 - 1;
+    const int nz = (x != 0) + (y != 0) + (z != 0);
+    m.d[k] = (nz >= 1 && nz <= 2) ? n++ : -1;
+  }
+  return m;
+}
+
+constexpr CrossTable make_cross_table() {
+  CrossTable t{};
+  const DirMap dm = make_dir_map();
+  int off27[NDIR] = {};
+  for (int k = 0; k < 27; k++)
+    if (dm.d[k] >= 0) off27[dm.d[k]] = k;
+  for (int d = 0; d < NDIR; d++) t.opp[d] = dm.d[26 - off27[d]];
+  // direction of every (cell, q), and the partner slot across it
+  int16_t dir_of[BP] = {}, partner_of[BP] = {};
+  for (int c = 0; c < BC; c++)
+    for (int q = 0; q < Q; q++) {
+      const int x = c % BS + CX(q), y = (c / BS) % BS + CY(q), z = c / (BS * BS) + CZ(q);
+      const int ox = x < 0 ? -1 : (x >= BS ? 1 : 0), oy = y < 0 ? -1 : (y >= BS ? 1 : 0),
+                oz = z < 0 ? -1 : (z >= BS ? 1 : 0);
+      dir_of[c * Q + q] = (int16_t)dm.d[(ox + 1) + 3 * (oy + 1) + 9 * (oz + 1)];
+      partner_of[c * Q + q] =
+          (int16_t)(((x - BS * ox) + BS * ((y - BS * oy) + BS * (z - BS * oz))) * Q + (Q - 1 - q));
+    }
+  for (int k = 0; k < BP; k++) t.entry_of[k] = -1;
+  int n = 0;
+  for (int d = 0; d < NDIR; d++) {
+    t.start[d] = n;
+    const bool first = d < t.opp[d];  // the lower index of a pair lists (cell, q) in order
+    const int src = first ? d : t.opp[d];
+    for (int k = 0; k < BP; k++) {
+      if (dir_of[k] != src) continue;
+      const int slot = first ? k : partner_of[k];  // the partner in the neighbour across src
+      if (n < NX_ENTRIES) {
+        t.e[n] = (uint32_t)(slot / Q) | (uint32_t)(slot % Q) << 8 | (uint32_t)d << 13;
+        t.entry_of[slot] = (int16_t)n;
+      }
+      n++;
+    }
+    t.len[d] = n - t.start[d];
+  }
+  t.n = n;
+  return t;
+}
+static_assert(make_cross_table().n == NX_ENTRIES, "a 6^3 block has 1,008 boundary-crossing populations");
+
+// The two tables describe the same links: every cross-block entry (i, j, c1, c2, q) of a Peano cell
+// pairs crossing entry (c1, q) of block i, in the direction from vertex i to vertex j, with its
+// partner (c2, 18 - q) in the opposite segment of block j; and partnering is an involution.
+constexpr bool cross_tables_agree() {
+  const CrossTable t = make_cross_table();
+  const NonLocalTable nl = make_nonlocal_table();
+  const DirMap dm = make_dir_map();
+  for (int d = 0; d < NDIR; d++)
+    if (t.len[d] != t.len[t.opp[d]] || t.opp[t.opp[d]] != d) return false;
+  for (int e = 0; e < NX_ENTRIES; e++) {
+    const int d = (t.e[e] >> 13) & 31, p = t.start[t.opp[d]] + (e - t.start[d]);
+    if (t.entry_of[(t.e[e] & 255) * Q + ((t.e[e] >> 8) & 31)] != e) return false;
+    if (t.start[t.opp[t.opp[d]]] + (p - t.start[t.opp[d]]) != e) return false;
+  }
+  for (int k = 0; k < NL_ENTRIES; k++) {
+    const uint32_t E = nl.e[k];
+    const int i = E & 7, j = (E >> 3) & 7, c1 = (E >> 6) & 255, c2 = (E >> 14) & 255, q = (E >> 22) & 31;
+    const int o = ((j & 1) - (i & 1) + 1) + 3 * (((j >> 1) & 1) - ((i >> 1) & 1) + 1) + 9 * ((j >> 2) - (i >> 2) + 1);
+    const int e1 = t.entry_of[c1 * Q + q], e2 = t.entry_of[c2 * Q + (Q - 1 - q)];
+    if (e1 < 0 || e2 < 0) return false;
+    const int d = (t.e[e1] >> 13) & 31;
+    if (d != dm.d[o] || t.start[t.opp[d]] + (e1 - t.start[d]) != e2) return false;
+  }
+  return true;
+}
+static_assert(cross_tables_agree(), "crossing entries and the Peano cell's cross-block table disagree");
+
+__constant__ CrossTable kCross = make_cross_table();
+
+// ---------------------------------------------------------------------------------------------
+// OP_XIN: the previous iteration's cross-block swaps, each valid entry taking its partner's
+// outgoing value from the exchange array; OP_XOUT: this iteration's outgoing values into it.
+enum : unsigned { OP_SWAP_IN = 1u, OP_VALUES = 2u, OP_COLLIDE = 4u, OP_BOUNDARY = 8u, OP_LOCAL = 16u,
+                  OP_XIN = 32u, OP_XOUT = 64u };
 
 struct BlockOpArgs {
   const int64_t* list;  // block ids, one workgroup each
@@ -82,6 +188,10 @@ struct BlockOpArgs {
   const BlockLinkDev* links;  // grouped by list position
   const int32_t* off;         // links of list position p: off[p] .. off[p + 1] - 1
   double omega;               // 1.0 / tau
+  const double* xin;          // exchange array read by OP_XIN (1,008 values per block)
+  double* xout;               // exchange array written by OP_XOUT
+  const uint32_t* valid;      // per block 32 words: bit e = entry e is swapped (listed, both cells inner)
+  const int64_t* nbr;         // per block 18 neighbour blocks by direction (-1: none)
 };
 
 // one boundary link after the collision on the outgoing slot f (see lbm_blocks_treat_boundary); rho
@@ -115,7 +225,7 @@ static_assert(NFULL == 8 && TAIL == 4, "the staging below is written for 8 full 
 
 template <unsigned OPS>
 __global__ __launch_bounds__(256) void k_block_ops(BlockOpArgs a) {
-  constexpr bool WRITES = (OPS & (OP_SWAP_IN | OP_COLLIDE | OP_BOUNDARY | OP_LOCAL)) != 0;
+  constexpr bool WRITES = (OPS & (OP_SWAP_IN | OP_COLLIDE | OP_BOUNDARY | OP_LOCAL | OP_XIN | OP_XOUT)) != 0;
   constexpr bool MOMENTS = (OPS & (OP_VALUES | OP_COLLIDE | OP_BOUNDARY)) != 0;
   __shared__ __align__(16) double s[BP];
   __shared__ double srho[MOMENTS ? BC : 1];
@@ -127,6 +237,28 @@ __global__ __launch_bounds__(256) void k_block_ops(BlockOpArgs a) {
   if (p >= a.n) return;
   const int64_t b = a.list[p];
   double2* g = reinterpret_cast<double2*>(a.pdf + b * BP);
+  // the previous iteration's cross-block swaps: each thread fetches the partner values of its (at
+  // most 4) swapped crossing entries from the exchange array while the block's own loads are in
+  // flight, and puts them into the block once it is in LDS
+  constexpr int NXT = (NX_ENTRIES + 255) / 256;
+  double xv[NXT];
+  int xs[NXT];
+  if (OPS & OP_XIN) {
+    const uint32_t* vb = a.valid + b * 32;
+#pragma unroll
+    for (int i = 0; i < NXT; i++) {
+      const int e = t + 256 * i;
+      xs[i] = -1;
+      xv[i] = 0.0;
+      if (e < NX_ENTRIES && ((vb[e >> 5] >> (e & 31)) & 1u)) {
+        const uint32_t E = kCross.e[e];
+        const int d = (E >> 13) & 31u;
+        const int64_t nb = a.nbr[b * NDIR + d];
+        xv[i] = a.xin[nb * NX_ENTRIES + kCross.start[kCross.opp[d]] + (e - kCross.start[d])];
+        xs[i] = (int)(E & 255u) * Q + (int)((E >> 8) & 31u);
+      }
+    }
+  }
   {
     // all loads of a thread are issued before the first LDS write, so a block's 32.8 KB are in
     // flight at once
@@ -139,6 +271,12 @@ __global__ __launch_bounds__(256) void k_block_ops(BlockOpArgs a) {
   }
   if (t < BC) sin[t] = a.inner[b * BC + t];
   __syncthreads();
+  if (OPS & OP_XIN) {
+#pragma unroll
+    for (int i = 0; i < NXT; i++)
+      if (xs[i] >= 0) s[xs[i]] = xv[i];
+    __syncthreads();
+  }
   {
     const int c = t;
     if (c < BC) {
@@ -234,6 +372,13 @@ __global__ __launch_bounds__(256) void k_block_ops(BlockOpArgs a) {
     }
     if (WRITES) {
       __syncthreads();
+      if (OPS & OP_XOUT) {
+        double* xo = a.xout + b * NX_ENTRIES;
+        for (int e = t; e < NX_ENTRIES; e += 256) {
+          const uint32_t E = kCross.e[e];
+          xo[e] = s[(E & 255u) * Q + ((E >> 8) & 31u)];
+        }
+      }
       const double2 w0 = s2[t], w1 = s2[t + 256], w2 = s2[t + 512], w3 = s2[t + 768];
       const double2 w4 = s2[t + 1024], w5 = s2[t + 1280], w6 = s2[t + 1536], w7 = s2[t + 1792];
       g[t] = w0; g[t + 256] = w1; g[t + 512] = w2; g[t + 768] = w3;
@@ -270,6 +415,22 @@ __global__ __launch_bounds__(512) void k_block_nonlocal(const int64_t* __restric
   const double v = *x;
   *x = *y;
   *y = v;
+}
+
+// the swap mask of lbm_blocks_iterate: bit e of block b is set when crossing entry e is swapped by
+// one of the listed Peano cells (both of its cells inner); one thread per (cell, table entry)
+__global__ __launch_bounds__(512) void k_block_valid(const int64_t* __restrict__ cells, int64_t n,
+                                                     const uint8_t* __restrict__ inner, uint32_t* __restrict__ valid) {
+  const int64_t p = blockIdx.x;
+  const int e = threadIdx.x;
+  if (p >= n || e >= NL_ENTRIES) return;
+  const uint32_t E = kNonLocal.e[e];
+  const int i = E & 7u, j = (E >> 3) & 7u, c1 = (E >> 6) & 255u, c2 = (E >> 14) & 255u, q = (E >> 22) & 31u;
+  const int64_t bi = cells[8 * p + i], bj = cells[8 * p + j];
+  if (!inner[bi * BC + c1] || !inner[bj * BC + c2]) return;
+  const int e1 = kCross.entry_of[c1 * Q + q], e2 = kCross.entry_of[c2 * Q + (Q - 1 - q)];
+  atomicOr(&valid[bi * 32 + (e1 >> 5)], 1u << (e1 & 31));
+  atomicOr(&valid[bj * 32 + (e2 >> 5)], 1u << (e2 & 31));
 }
 
 }  // namespace lbm
@@ -438,6 +599,7 @@ int lbm_blocks_set_inner(lbm_blockstore* bs, int64_t first, int64_t count, const
   for (size_t k = 0; k < m.size(); k++) m[k] = inner[k] ? 1 : 0;
   BCHK(bs, hipMemcpyAsync(bs->inner + first * BC, m.data(), m.size(), hipMemcpyHostToDevice, bs->stream));
   BCHK(bs, hipStreamSynchronize(bs->stream));
+  bs->inner_gen++;
   return LBM_OK;
 }
 
@@ -491,6 +653,12 @@ int lbm_blocks_vertex_first(lbm_blockstore* bs, const int64_t* blocks, int64_t n
   return finish(bs);
 }
 
+// One launch per iteration: iteration t's handleCell swaps and touchVertexLastTime swap run at the
+// start of iteration t + 1's block pass.  Each block pass writes its blocks' outgoing crossing
+// values (after the local streaming) into an exchange array, and the next pass takes each swapped
+// entry's value from the partner's outgoing values there (two arrays, alternating), which is what
+// the cross-block swap of the reference leaves in that slot; the blocks' own slots keep their
+// values where no swap happens.  A last pass applies the final iteration's swaps.
 int lbm_blocks_iterate(lbm_blockstore* bs, const int64_t* blocks, int64_t nblocks_list, const int64_t* cells,
                        int64_t ncells, double tau, const lbm_block_boundary* links, int64_t nlinks, int nsteps) {
   if (!bs) return LBM_ERR_ARG;
@@ -500,17 +668,64 @@ int lbm_blocks_iterate(lbm_blockstore* bs, const int64_t* blocks, int64_t nblock
   if (int rc = upload_blocks(bs, blocks, nblocks_list, &bs->d_list, &bs->cap_list)) return rc;
   if (int rc = upload_cells(bs, cells, ncells, &bs->d_cells, &bs->cap_cells)) return rc;
   if (int rc = upload_boundary(bs, links, nlinks, blocks, nblocks_list)) return rc;
-  const BlockOpArgs a = op_args(bs, nblocks_list, tau, 0);
+  // the blocks of a cell must be listed blocks
+  std::vector<uint8_t> listed((size_t)bs->nblocks, 0);
+  for (int64_t k = 0; k < nblocks_list; k++) listed[(size_t)blocks[k]] = 1;
+  for (int64_t k = 0; k < 8 * ncells; k++)
+    if (!listed[(size_t)cells[k]])
+      return bfail(bs, LBM_ERR_ARG, "cell " + std::to_string(k / 8) + ": block " + std::to_string(cells[k]) +
+                                        " is not in the block list");
+  if (int rc = ensure_capacity(bs, &bs->d_xch, &bs->cap_xch, 2 * bs->nblocks * NX_ENTRIES, "exchange arrays")) return rc;
+  const bool same = bs->iter_inner_gen == bs->inner_gen && bs->iter_cells.size() == (size_t)(8 * ncells) &&
+                    std::equal(bs->iter_cells.begin(), bs->iter_cells.end(), cells);
+  if (!same) {
+    // the neighbour of each block across each face and edge, as the cells give it (they must agree),
+    // and the swap mask; kept for the next call with the same cells and inner flags
+    bs->iter_cells.clear();
+    bs->iter_inner_gen = ~0ull;
+    constexpr DirMap dm = make_dir_map();
+    std::vector<int64_t> nbr((size_t)bs->nblocks * NDIR, -1);
+    for (int64_t p = 0; p < ncells; p++) {
+      const int64_t* v = cells + 8 * p;
+      for (int i = 0; i < 8; i++)
+        for (int j = 0; j < 8; j++) {
+          const int o = ((j & 1) - (i & 1) + 1) + 3 * (((j >> 1) & 1) - ((i >> 1) & 1) + 1) + 9 * ((j >> 2) - (i >> 2) + 1);
+          if (dm.d[o] < 0) continue;
+          int64_t& slot = nbr[(size_t)v[i] * NDIR + dm.d[o]];
+          if (slot >= 0 && slot != v[j])
+            return bfail(bs, LBM_ERR_ARG, "cells disagree on the neighbour of block " + std::to_string(v[i]));
+          slot = v[j];
+        }
+    }
+    if (int rc = ensure_capacity(bs, &bs->d_nbr, &bs->cap_nbr, bs->nblocks * NDIR, "neighbour table")) return rc;
+    if (int rc = ensure_capacity(bs, &bs->d_valid, &bs->cap_valid, bs->nblocks * 32, "swap mask")) return rc;
+    BCHK(bs, hipMemcpyAsync(bs->d_nbr, nbr.data(), nbr.size() * sizeof(int64_t), hipMemcpyHostToDevice, bs->stream));
+    BCHK(bs, hipMemsetAsync(bs->d_valid, 0, (size_t)bs->nblocks * 32 * sizeof(uint32_t), bs->stream));
+    if (ncells) {
+      hipLaunchKernelGGL(k_block_valid, dim3((unsigned)ncells), dim3(512), 0, bs->stream, bs->d_cells, ncells, bs->inner,
+                         bs->d_valid);
+      BCHK(bs, hipGetLastError());
+    }
+    BCHK(bs, hipStreamSynchronize(bs->stream));  // nbr is host memory of this call
+    bs->iter_cells.assign(cells, cells + 8 * ncells);
+    bs->iter_inner_gen = bs->inner_gen;
+  }
+  BlockOpArgs a = op_args(bs, nblocks_list, tau, 0);
+  a.valid = bs->d_valid;
+  a.nbr = bs->d_nbr;
+  double* x[2] = {bs->d_xch, bs->d_xch + bs->nblocks * NX_ENTRIES};
   if (int rc = begin(bs)) return rc;
   for (int s = 0; s < nsteps; s++) {
-    // touchVertexFirstTime (with the previous iteration's touchVertexLastTime folded in), handleCell
-    const int rc = s == 0 ? launch_ops<OP_VALUES | OP_COLLIDE | OP_BOUNDARY | OP_LOCAL>(bs, a)
-                          : launch_ops<OP_SWAP_IN | OP_VALUES | OP_COLLIDE | OP_BOUNDARY | OP_LOCAL>(bs, a);
+    a.xin = x[(s + 1) & 1];
+    a.xout = x[s & 1];
+    const int rc = s == 0 ? launch_ops<OP_VALUES | OP_COLLIDE | OP_BOUNDARY | OP_LOCAL | OP_XOUT>(bs, a)
+                          : launch_ops<OP_XIN | OP_SWAP_IN | OP_VALUES | OP_COLLIDE | OP_BOUNDARY | OP_LOCAL | OP_XOUT>(bs, a);
     if (rc) return rc;
-    if (int rc2 = launch_nonlocal(bs, bs->d_cells, ncells)) return rc2;
   }
-  if (nsteps > 0)
-    if (int rc = launch_ops<OP_SWAP_IN>(bs, a)) return rc;  // the last iteration's touchVertexLastTime
+  if (nsteps > 0) {  // the last iteration's handleCell and touchVertexLastTime
+    a.xin = x[(nsteps - 1) & 1];
+    if (int rc = launch_ops<OP_XIN | OP_SWAP_IN>(bs, a)) return rc;
+  }
   return finish(bs);
 }
 

@@ -8,6 +8,7 @@
 
 #include <cstdint>
 #include <string>
+#include <vector>
 
 #include "lbm_capi.h"
 #include "lbm_cell.h"
@@ -58,6 +59,15 @@ struct lbm_blockstore {
   int64_t cap_bnd = 0;
   int32_t* d_off = nullptr;    // per list position: first link (n + 1 entries)
   int64_t cap_off = 0;
+  int64_t* d_nbr = nullptr;    // lbm_blocks_iterate: 18 neighbours per block
+  int64_t cap_nbr = 0;
+  uint32_t* d_valid = nullptr; // lbm_blocks_iterate: 32 swap-mask words per block
+  int64_t cap_valid = 0;
+  double* d_xch = nullptr;     // lbm_blocks_iterate: two exchange arrays of 1,008 values per block
+  int64_t cap_xch = 0;
+  uint64_t inner_gen = 0;      // bumped by every lbm_blocks_set_inner
+  std::vector<int64_t> iter_cells;  // the cell list d_nbr and d_valid were built for ...
+  uint64_t iter_inner_gen = ~0ull;  // ... and the inner flags
   float last_ms = 0.f;
 };
 

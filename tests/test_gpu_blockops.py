@@ -240,3 +240,53 @@ def test_cpp_block_kernels_drive_the_cavity(tmp_path, case):
     subprocess.run([BLOCKOPS_EXE, str(n), repr(tau), repr(ulid), str(steps), str(out)], check=True, timeout=120)
     f = np.fromfile(out, dtype=np.float64)
     assert np.array_equal(f, ref)
+
+
+def test_iterate_with_a_partial_cell_list_equals_the_separate_operators():
+    """lbm_blocks_iterate carries the cross-block swaps of one iteration into the next block pass; with a
+    cell list that leaves cells out (their links are not swapped), seeded inner flags and lid links it
+    must still give the state of the separate operators, iteration by iteration."""
+    grid = BlockGrid(14, 11, 13)
+    rng = np.random.default_rng(21)
+    inner = grid.inner.copy()
+    inner[rng.random(inner.shape) < 0.1] = 0
+    cells = grid.peano_cells[rng.random(len(grid.peano_cells)) < 0.7]
+    links = [lk for lk in grid.cavity_links(0.07) if inner[lk[0], lk[1]]]
+    tau = O.tau_eff(0.57)
+    pdf0 = initial_state(grid)[0] * (1.0 + 0.01 * rng.standard_normal((grid.nblocks, BP)))
+    blocks = np.arange(grid.nblocks)
+    print(f"[gpu] iterate with {len(cells)} of {len(grid.peano_cells)} cells", flush=True)
+    inner2 = inner.copy()
+    inner2[rng.random(inner.shape) < 0.05] = 0
+    links2 = [lk for lk in links if inner2[lk[0], lk[1]]]
+    order = rng.permutation(blocks)
+    out = []
+    for mode in ("separate", "iterate"):
+        with _store(pdf0, np.ones((grid.nblocks, NC)), None, inner) as bs:
+            # one step, three more (the same cells and flags: the swap mask is reused), then changed flags
+            for steps, blk, lk in ((1, blocks, links), (3, order, links), (2, blocks, links2)):
+                if steps == 2:
+                    bs.set_inner(0, inner2)
+                if mode == "iterate":
+                    bs.iterate(blk, cells, tau, lk, steps)
+                else:
+                    for _ in range(steps):
+                        bs.vertex_first(blocks, tau, lk)
+                        bs.stream_nonlocal(cells)
+                        bs.swap_inside(blocks)
+                out.append(bs.get())
+    for a, b in zip(out[:3], out[3:]):
+        for x, y in zip(a, b):
+            assert np.array_equal(x, y)
+
+
+def test_iterate_rejects_inconsistent_cells():
+    grid = BlockGrid(8, 8, 8)
+    blocks = np.arange(grid.nblocks)
+    with lbm.BlockStore(grid.nblocks) as bs:
+        with pytest.raises(lbm.LbmError, match="not in the block list"):
+            bs.iterate(blocks[:-1], grid.peano_cells, 0.6)
+        bad = grid.peano_cells.copy()
+        bad[0, [1, 2]] = bad[0, [2, 1]]  # two vertices exchanged: the neighbours disagree with the other cells
+        with pytest.raises(lbm.LbmError, match="disagree"):
+            bs.iterate(blocks, bad, 0.6)
