@@ -290,3 +290,60 @@ def test_iterate_rejects_inconsistent_cells():
         bad[0, [1, 2]] = bad[0, [2, 1]]  # two vertices exchanged: the neighbours disagree with the other cells
         with pytest.raises(lbm.LbmError, match="disagree"):
             bs.iterate(blocks, bad, 0.6)
+
+
+def test_two_level_vertex_sequence_equals_the_oracle():
+    """The spacetree mapping's per-vertex sequence on one refined coarse cell, batched over all 64 fine
+    vertices (SpacetreeGrid2RegularBlockSolver.cpp:186-273, :349, :573-577): overlap search from the
+    -1.0 markers, cell values and collision skipping the overlap cells (hasMinusOneEntries),
+    prolongation from the father blocks, local streaming, cross-block streaming over the 27 fine Peano
+    cells, in-cell swap, restriction into the fathers.  Every stage against the oracle, bit for bit."""
+    rng = np.random.default_rng(31)
+    ncoarse, nfine = 8, 64
+    coarse = rng.uniform(0.5, 1.5, (ncoarse, BP))
+    fine = rng.uniform(0.5, 1.5, (nfine, BP))
+    fine[rng.random(fine.shape) < 0.003] = -1.0
+    inner = np.ones((ncoarse + nfine, NC), np.uint8)
+    inner[ncoarse:][rng.random((nfine, NC)) < 0.08] = 0
+    pos = [(x, y, z) for z in range(4) for y in range(4) for x in range(4)]
+    links = [((p[0] // 2) + 2 * (p[1] // 2) + 4 * (p[2] // 2), ncoarse + k, p) for k, p in enumerate(pos)]
+    cells = np.array([[ncoarse + (cx + (v & 1)) + 4 * (cy + ((v >> 1) & 1)) + 16 * (cz + (v >> 2)) for v in range(8)]
+                      for cz in range(3) for cy in range(3) for cx in range(3)], dtype=np.int64)
+    fine_ids = np.arange(ncoarse, ncoarse + nfine)
+    tau = O.tau_eff(0.7)
+    # oracle, vertex by vertex
+    oc, of = coarse.copy(), fine.copy()
+    orho = np.ones((nfine, NC))
+    ou = np.zeros((nfine, 3 * NC))
+    ov = np.zeros((nfine, NC), bool)
+    for k, p in enumerate(pos):
+        ov[k] = O.amr_find_overlap(of[k])
+        orho[k], ou[k] = O.block_cell_values(inner[ncoarse + k], of[k], density=orho[k], minus_one=ov[k])
+        of[k] = O.block_collide(inner[ncoarse + k], of[k], orho[k], ou[k], tau, minus_one=ov[k])
+    stage1 = (of.copy(), orho.copy(), ou.copy())
+    for k, p in enumerate(pos):
+        of[k], orho[k], ou[k] = O.amr_prolongate(p, oc[links[k][0]], of[k], orho[k], ou[k], ov[k])
+        of[k] = O.block_stream_local(inner[ncoarse + k], of[k])
+    for cell in cells:
+        blk = cell - ncoarse
+        of[blk] = O.block_stream_nonlocal(inner[cell], of[blk])
+    for k in range(nfine):
+        of[k] = O.block_swap_inside(of[k])
+    for k, p in enumerate(pos):
+        oc[links[k][0]] = O.amr_restrict(p, oc[links[k][0]], of[k])
+    print("[gpu] two-level vertex sequence, 8 coarse and 64 fine blocks", flush=True)
+    with _store(np.concatenate([coarse, fine]), np.ones((ncoarse + nfine, NC)),
+                np.zeros((ncoarse + nfine, 3 * NC)), inner) as bs:
+        assert np.array_equal(bs.find_overlap(links), ov)
+        bs.compute_cell_values(fine_ids, use_overlap=True)
+        bs.collide(fine_ids, tau, use_overlap=True)
+        f, rho, u = bs.get(ncoarse)
+        assert np.array_equal(f, stage1[0]) and np.array_equal(rho, stage1[1]) and np.array_equal(u, stage1[2])
+        bs.prolongate(links)
+        bs.stream_local(fine_ids)
+        bs.stream_nonlocal(cells)
+        bs.swap_inside(fine_ids)
+        bs.restrict(links)
+        f, rho, u = bs.get()
+    assert np.array_equal(f[:ncoarse], oc) and np.array_equal(f[ncoarse:], of)
+    assert np.array_equal(rho[ncoarse:], orho) and np.array_equal(u[ncoarse:], ou)
