@@ -218,11 +218,6 @@ __device__ __forceinline__ void block_boundary_f64(double& f, const BlockLinkDev
   f = L.type == 4 ? fin : -f + fin + feq_f64(q, rt, t0, t1, t2, uu);
 }
 
-// The block in 16-byte pieces: a block is 2,052 of them (8 rounds of 256 and 4) and starts on a
-// 16-byte boundary.
-constexpr int BP2 = BP / 2, NFULL = BP2 / 256, TAIL = BP2 - 256 * NFULL;
-static_assert(NFULL == 8 && TAIL == 4, "the staging below is written for 8 full rounds and 4 pieces");
-
 template <unsigned OPS>
 __global__ __launch_bounds__(256) void k_block_ops(BlockOpArgs a) {
   constexpr bool WRITES = (OPS & (OP_SWAP_IN | OP_COLLIDE | OP_BOUNDARY | OP_LOCAL | OP_XIN | OP_XOUT)) != 0;
@@ -231,12 +226,10 @@ __global__ __launch_bounds__(256) void k_block_ops(BlockOpArgs a) {
   __shared__ double srho[MOMENTS ? BC : 1];
   __shared__ double su[MOMENTS ? BV : 1];
   __shared__ uint8_t sin[BC];
-  double2* s2 = reinterpret_cast<double2*>(s);
   const int t = threadIdx.x;
   const int64_t p = blockIdx.x;
   if (p >= a.n) return;
   const int64_t b = a.list[p];
-  double2* g = reinterpret_cast<double2*>(a.pdf + b * BP);
   // the previous iteration's cross-block swaps: each thread fetches the partner values of its (at
   // most 4) swapped crossing entries from the exchange array while the block's own loads are in
   // flight, and puts them into the block once it is in LDS
@@ -259,16 +252,7 @@ __global__ __launch_bounds__(256) void k_block_ops(BlockOpArgs a) {
       }
     }
   }
-  {
-    // all loads of a thread are issued before the first LDS write, so a block's 32.8 KB are in
-    // flight at once
-    const double2 r0 = g[t], r1 = g[t + 256], r2 = g[t + 512], r3 = g[t + 768];
-    const double2 r4 = g[t + 1024], r5 = g[t + 1280], r6 = g[t + 1536], r7 = g[t + 1792];
-    const double2 r8 = t < TAIL ? g[t + 2048] : double2{0.0, 0.0};
-    s2[t] = r0; s2[t + 256] = r1; s2[t + 512] = r2; s2[t + 768] = r3;
-    s2[t + 1024] = r4; s2[t + 1280] = r5; s2[t + 1536] = r6; s2[t + 1792] = r7;
-    if (t < TAIL) s2[t + 2048] = r8;
-  }
+  stage_block(a.pdf + b * BP, s, t);
   if (t < BC) sin[t] = a.inner[b * BC + t];
   __syncthreads();
   if (OPS & OP_XIN) {
@@ -379,11 +363,7 @@ __global__ __launch_bounds__(256) void k_block_ops(BlockOpArgs a) {
           xo[e] = s[(E & 255u) * Q + ((E >> 8) & 31u)];
         }
       }
-      const double2 w0 = s2[t], w1 = s2[t + 256], w2 = s2[t + 512], w3 = s2[t + 768];
-      const double2 w4 = s2[t + 1024], w5 = s2[t + 1280], w6 = s2[t + 1536], w7 = s2[t + 1792];
-      g[t] = w0; g[t + 256] = w1; g[t + 512] = w2; g[t + 768] = w3;
-      g[t + 1024] = w4; g[t + 1280] = w5; g[t + 1536] = w6; g[t + 1792] = w7;
-      if (t < TAIL) g[t + 2048] = s2[t + 2048];
+      store_block(s, a.pdf + b * BP, t);
     }
   }
 }

@@ -27,6 +27,33 @@ struct AmrLinkDev {
   int new_vertex;
 };
 
+// A block's 4,104 populations in 2,052 16-byte pieces (a block starts on a 16-byte boundary): 8 rounds
+// of 256 threads and 4 more.  All loads of a thread are issued before its first LDS write, so a
+// block's 32.8 KB are in flight at once; stores go out the same way.  For 256-thread workgroups.
+constexpr int BP2 = BP / 2, NFULL = BP2 / 256, TAIL = BP2 - 256 * NFULL;
+static_assert(NFULL == 8 && TAIL == 4, "the staging below is written for 8 full rounds and 4 pieces");
+
+__device__ __forceinline__ void stage_block(const double* __restrict__ src, double* s, int t) {
+  const double2* g = reinterpret_cast<const double2*>(src);
+  double2* s2 = reinterpret_cast<double2*>(s);
+  const double2 r0 = g[t], r1 = g[t + 256], r2 = g[t + 512], r3 = g[t + 768];
+  const double2 r4 = g[t + 1024], r5 = g[t + 1280], r6 = g[t + 1536], r7 = g[t + 1792];
+  const double2 r8 = t < TAIL ? g[t + 2048] : double2{0.0, 0.0};
+  s2[t] = r0; s2[t + 256] = r1; s2[t + 512] = r2; s2[t + 768] = r3;
+  s2[t + 1024] = r4; s2[t + 1280] = r5; s2[t + 1536] = r6; s2[t + 1792] = r7;
+  if (t < TAIL) s2[t + 2048] = r8;
+}
+
+__device__ __forceinline__ void store_block(const double* s, double* __restrict__ dst, int t) {
+  const double2* s2 = reinterpret_cast<const double2*>(s);
+  double2* g = reinterpret_cast<double2*>(dst);
+  const double2 w0 = s2[t], w1 = s2[t + 256], w2 = s2[t + 512], w3 = s2[t + 768];
+  const double2 w4 = s2[t + 1024], w5 = s2[t + 1280], w6 = s2[t + 1536], w7 = s2[t + 1792];
+  g[t] = w0; g[t + 256] = w1; g[t + 512] = w2; g[t + 768] = w3;
+  g[t + 1024] = w4; g[t + 1280] = w5; g[t + 1536] = w6; g[t + 1792] = w7;
+  if (t < TAIL) g[t + 2048] = s2[t + 2048];
+}
+
 // one boundary link of a block, as lbm_block_boundary with the block's position in the call's list
 struct BlockLinkDev {
   int64_t block;
