@@ -140,19 +140,22 @@ __global__ __launch_bounds__(256) void k_amr_prolongate(const AmrLinkDev* __rest
   }
 }
 
-// restrictToCoarseLevel (:164-259): one thread per (coarse cell of the sub-block, population)
+// restrictToCoarseLevel (:164-259): the fine block staged through LDS, then one thread per (coarse
+// cell of the sub-block, population)
 __global__ __launch_bounds__(256) void k_amr_restrict(const AmrLinkDev* __restrict__ links, int64_t nlinks,
                                                       double* __restrict__ pdf) {
   const int64_t l = blockIdx.x;
   if (l >= nlinks) return;
   const AmrLinkDev L = links[l];
   const int t = threadIdx.x;
+  __shared__ __align__(16) double ff[BP];
+  stage_block(pdf + L.fine * BP, ff, t);
+  __syncthreads();
   if (t >= 8 * Q) return;
   const int q = t % Q, k = t / Q;
   const int x = k & 1, y = (k >> 1) & 1, z = k >> 2;
   const int h = BS / 3;
   const int cc = (L.sx * h + x) + (L.sy * h + y) * BS + (L.sz * h + z) * BS * BS;
-  const double* ff = pdf + L.fine * BP;
   double r = 0.0;
   bool marked = false;
   for (int zz = 0; zz < 3; zz++)
