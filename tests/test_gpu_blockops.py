@@ -347,3 +347,19 @@ def test_two_level_vertex_sequence_equals_the_oracle():
         f, rho, u = bs.get()
     assert np.array_equal(f[:ncoarse], oc) and np.array_equal(f[ncoarse:], of)
     assert np.array_equal(rho[ncoarse:], orho) and np.array_equal(u[ncoarse:], ou)
+
+
+def test_block_solver_equals_the_fused_step_at_128():
+    """Size-independent property at a large lattice: the reference's three sweeps on its block layout
+    (lbm_blocks_iterate, 12,167 blocks) and the fused pull step on a dense lattice give the same state
+    bit for bit after 300 steps of the 128^3 cavity."""
+    n, tau, ulid, steps = 128, 0.6, 0.05, 300
+    print(f"[gpu] block solver against the fused step, {n}^3 cavity, {steps} steps", flush=True)
+    grid = BlockGrid(n, n, n)
+    with _store(initial_state(grid)[0], np.ones((grid.nblocks, NC)), None, grid.inner) as bs:
+        bs.iterate(np.arange(grid.nblocks), grid.peano_cells, O.tau_eff(tau), grid.cavity_links(ulid), steps)
+        fb = grid.gather(bs.get()[0])
+    with lbm.cavity(n, tau, ulid) as lat:
+        lat.step(steps)
+        fd = lat.get_populations()
+    assert np.array_equal(fb, fd)
