@@ -188,3 +188,17 @@ def test_composed_block_solver_reproduces_channel_trajectories(name):
     st = initial_state(grid)
     iterate(grid, OracleOps(grid), st, O.tau_eff(float(d["tau"])), grid.channel_links(inlet), int(d["steps"]))
     assert np.array_equal(grid.gather(st[0]), d["f"].reshape(-1))
+
+
+@pytest.mark.parametrize("n", [8, 13])
+def test_bench_block_grid_is_the_reference_layout(n):
+    """bench.py's block-solver workload builds the same blocks, inner flags, Peano cells and lid links
+    as the layout the composed trajectories above pin against the reference."""
+    import bench
+    nblocks, inner, cells, links = bench.regular_block_grid(n, 0.05)
+    grid = BlockGrid(n, n, n)
+    assert nblocks == grid.nblocks
+    assert np.array_equal(inner, grid.inner != 0)
+    assert np.array_equal(cells, grid.peano_cells)
+    key = lambda lk: (lk[0], lk[1], lk[2], lk[3])
+    assert sorted(map(key, links)) == sorted(map(key, grid.cavity_links(0.05)))
