@@ -94,12 +94,15 @@ __global__ __launch_bounds__(256) void k_amr_prolongate(const AmrLinkDev* __rest
   __shared__ double sc[8 * Q];
   __shared__ double srho[8], su[8 * 3];
   __shared__ uint8_t sm[BC];
-  if (t < 8 * Q) {
-    const int k = t / Q, q = t % Q;  // coarse cell k of the sub-block: (k & 1, (k >> 1) & 1, k >> 2)
-    const int cc = (L.sx * h + (k & 1)) + (L.sy * h + ((k >> 1) & 1)) * BS + (L.sz * h + (k >> 2)) * BS * BS;
-    sc[t] = pdf[L.coarse * BP + cc * Q + q];
-  }
-  if (t < BC) sm[t] = mask[L.fine * BC + t];
+  // both loads are issued before either is waited for (clamped lanes load an in-range value and
+  // drop it)
+  const int tc = min(t, 8 * Q - 1);
+  const int k = tc / Q, q = tc % Q;  // coarse cell k of the sub-block: (k & 1, (k >> 1) & 1, k >> 2)
+  const int cc = (L.sx * h + (k & 1)) + (L.sy * h + ((k >> 1) & 1)) * BS + (L.sz * h + (k >> 2)) * BS * BS;
+  const double cv = pdf[L.coarse * BP + cc * Q + q];
+  const uint8_t mv = mask[L.fine * BC + min(t, BC - 1)];
+  if (t < 8 * Q) sc[t] = cv;
+  if (t < BC) sm[t] = mv;
   __syncthreads();
   if (t < 8) {
     double rho = 0.0, m0 = 0.0, m1 = 0.0, m2 = 0.0;

@@ -171,6 +171,35 @@ static_assert(cross_tables_agree(), "crossing entries and the Peano cell's cross
 
 __constant__ CrossTable kCross = make_cross_table();
 
+// The segment bounds as compile-time constants: the direction of crossing entry e and its partner
+// entry in the neighbour (e + start[opp d] - start[d]) follow from comparisons alone, with no table
+// load in front of the neighbour lookup.
+struct CrossSegments {
+  int start[NDIR], delta[NDIR];
+};
+constexpr CrossSegments make_cross_segments() {
+  const CrossTable t = make_cross_table();
+  CrossSegments s{};
+  for (int d = 0; d < NDIR; d++) {
+    s.start[d] = t.start[d];
+    s.delta[d] = t.start[t.opp[d]] - t.start[d];
+  }
+  return s;
+}
+constexpr CrossSegments kSeg = make_cross_segments();
+
+__device__ __forceinline__ void cross_segment(int e, int& d, int& partner) {
+  d = 0;
+  int delta = kSeg.delta[0];
+#pragma unroll
+  for (int k = 1; k < NDIR; k++)
+    if (e >= kSeg.start[k]) {
+      d = k;
+      delta = kSeg.delta[k];
+    }
+  partner = e + delta;
+}
+
 // ---------------------------------------------------------------------------------------------
 // OP_XIN: the previous iteration's cross-block swaps, each valid entry taking its partner's
 // outgoing value from the exchange array; OP_XOUT: this iteration's outgoing values into it.
@@ -232,28 +261,45 @@ __global__ __launch_bounds__(256) void k_block_ops(BlockOpArgs a) {
   const int64_t b = a.list[p];
   // the previous iteration's cross-block swaps: each thread fetches the partner values of its (at
   // most 4) swapped crossing entries from the exchange array while the block's own loads are in
-  // flight, and puts them into the block once it is in LDS
+  // flight, and puts them into the block once it is in LDS.  The swap-mask words and neighbour ids
+  // are loaded first and together (an entry's direction needs no load), then the block's pieces,
+  // then the partner values: two memory round trips in front of the block's LDS writes, with the
+  // block's loads in flight during both.
   constexpr int NXT = (NX_ENTRIES + 255) / 256;
   double xv[NXT];
-  int xs[NXT];
+  int xs[NXT];    // OP_XIN: the block slot a partner value goes to, -1 when not swapped
+  int slot[NXT];  // the block slot of each entry (cell * 19 + q)
+  uint32_t vw[NXT];
+  int64_t nb[NXT];
+  int pe[NXT];
+  if (OPS & (OP_XIN | OP_XOUT)) {
+#pragma unroll
+    for (int i = 0; i < NXT; i++) {
+      const int e = min(t + 256 * i, NX_ENTRIES - 1);
+      if (OPS & OP_XIN) {
+        int d;
+        cross_segment(e, d, pe[i]);
+        vw[i] = a.valid[b * 32 + (e >> 5)];
+        nb[i] = a.nbr[b * NDIR + d];
+      }
+      const uint32_t E = kCross.e[e];
+      slot[i] = (int)(E & 255u) * Q + (int)((E >> 8) & 31u);
+    }
+  }
+  const BlockPieces own = load_block(a.pdf + b * BP, t);
+  const uint8_t in_t = t < BC ? a.inner[b * BC + t] : 0;
   if (OPS & OP_XIN) {
-    const uint32_t* vb = a.valid + b * 32;
 #pragma unroll
     for (int i = 0; i < NXT; i++) {
       const int e = t + 256 * i;
-      xs[i] = -1;
-      xv[i] = 0.0;
-      if (e < NX_ENTRIES && ((vb[e >> 5] >> (e & 31)) & 1u)) {
-        const uint32_t E = kCross.e[e];
-        const int d = (E >> 13) & 31u;
-        const int64_t nb = a.nbr[b * NDIR + d];
-        xv[i] = a.xin[nb * NX_ENTRIES + kCross.start[kCross.opp[d]] + (e - kCross.start[d])];
-        xs[i] = (int)(E & 255u) * Q + (int)((E >> 8) & 31u);
-      }
+      // a set bit means a listed Peano cell swaps the entry, so the neighbour exists
+      const bool swapped = e < NX_ENTRIES && ((vw[i] >> (e & 31)) & 1u);
+      xv[i] = swapped ? a.xin[nb[i] * NX_ENTRIES + pe[i]] : 0.0;
+      xs[i] = swapped ? slot[i] : -1;
     }
   }
-  stage_block(a.pdf + b * BP, s, t);
-  if (t < BC) sin[t] = a.inner[b * BC + t];
+  put_block(own, s, t);
+  if (t < BC) sin[t] = in_t;
   __syncthreads();
   if (OPS & OP_XIN) {
 #pragma unroll
@@ -356,14 +402,13 @@ __global__ __launch_bounds__(256) void k_block_ops(BlockOpArgs a) {
     }
     if (WRITES) {
       __syncthreads();
+      store_block(s, a.pdf + b * BP, t);
       if (OPS & OP_XOUT) {
         double* xo = a.xout + b * NX_ENTRIES;
-        for (int e = t; e < NX_ENTRIES; e += 256) {
-          const uint32_t E = kCross.e[e];
-          xo[e] = s[(E & 255u) * Q + ((E >> 8) & 31u)];
-        }
+#pragma unroll
+        for (int i = 0; i < NXT; i++)
+          if (t + 256 * i < NX_ENTRIES) xo[t + 256 * i] = s[slot[i]];
       }
-      store_block(s, a.pdf + b * BP, t);
     }
   }
 }

@@ -33,15 +33,26 @@ struct AmrLinkDev {
 constexpr int BP2 = BP / 2, NFULL = BP2 / 256, TAIL = BP2 - 256 * NFULL;
 static_assert(NFULL == 8 && TAIL == 4, "the staging below is written for 8 full rounds and 4 pieces");
 
-__device__ __forceinline__ void stage_block(const double* __restrict__ src, double* s, int t) {
+// The staging in two halves, for kernels that issue other loads while a block's are in flight.
+struct BlockPieces {
+  double2 r0, r1, r2, r3, r4, r5, r6, r7, r8;
+};
+__device__ __forceinline__ BlockPieces load_block(const double* __restrict__ src, int t) {
   const double2* g = reinterpret_cast<const double2*>(src);
+  BlockPieces p;
+  p.r0 = g[t]; p.r1 = g[t + 256]; p.r2 = g[t + 512]; p.r3 = g[t + 768];
+  p.r4 = g[t + 1024]; p.r5 = g[t + 1280]; p.r6 = g[t + 1536]; p.r7 = g[t + 1792];
+  p.r8 = t < TAIL ? g[t + 2048] : double2{0.0, 0.0};
+  return p;
+}
+__device__ __forceinline__ void put_block(const BlockPieces& p, double* s, int t) {
   double2* s2 = reinterpret_cast<double2*>(s);
-  const double2 r0 = g[t], r1 = g[t + 256], r2 = g[t + 512], r3 = g[t + 768];
-  const double2 r4 = g[t + 1024], r5 = g[t + 1280], r6 = g[t + 1536], r7 = g[t + 1792];
-  const double2 r8 = t < TAIL ? g[t + 2048] : double2{0.0, 0.0};
-  s2[t] = r0; s2[t + 256] = r1; s2[t + 512] = r2; s2[t + 768] = r3;
-  s2[t + 1024] = r4; s2[t + 1280] = r5; s2[t + 1536] = r6; s2[t + 1792] = r7;
-  if (t < TAIL) s2[t + 2048] = r8;
+  s2[t] = p.r0; s2[t + 256] = p.r1; s2[t + 512] = p.r2; s2[t + 768] = p.r3;
+  s2[t + 1024] = p.r4; s2[t + 1280] = p.r5; s2[t + 1536] = p.r6; s2[t + 1792] = p.r7;
+  if (t < TAIL) s2[t + 2048] = p.r8;
+}
+__device__ __forceinline__ void stage_block(const double* __restrict__ src, double* s, int t) {
+  put_block(load_block(src, t), s, t);
 }
 
 __device__ __forceinline__ void store_block(const double* s, double* __restrict__ dst, int t) {
