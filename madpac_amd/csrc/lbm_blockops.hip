@@ -259,6 +259,8 @@ __global__ __launch_bounds__(256) void k_block_ops(BlockOpArgs a) {
   const int64_t p = blockIdx.x;
   if (p >= a.n) return;
   const int64_t b = a.list[p];
+  // this block's boundary links (offsets read at the start, not between the collision's phases)
+  const int32_t link0 = (OPS & OP_BOUNDARY) ? a.off[p] : 0, link1 = (OPS & OP_BOUNDARY) ? a.off[p + 1] : 0;
   // the previous iteration's cross-block swaps: each thread fetches the partner values of its (at
   // most 4) swapped crossing entries from the exchange array while the block's own loads are in
   // flight, and puts them into the block once it is in LDS.  The swap-mask words and neighbour ids
@@ -288,6 +290,22 @@ __global__ __launch_bounds__(256) void k_block_ops(BlockOpArgs a) {
   }
   const BlockPieces own = load_block(a.pdf + b * BP, t);
   const uint8_t in_t = t < BC ? a.inner[b * BC + t] : 0;
+  // the cell's stored density (kept by skipped cells), its velocity when the values are not
+  // recomputed, and its overlap flag, loaded here so that no load waits between the collision's
+  // phases
+  const int tb = min(t, BC - 1);
+  double rho_in = 0.0, u_in0 = 0.0, u_in1 = 0.0, u_in2 = 0.0;
+  uint8_t skip_in = 0;
+  if (MOMENTS) {
+    rho_in = a.density[b * BC + tb];
+    if (!(OPS & OP_VALUES)) {
+      const double* ui = a.velocity + b * BV + 3 * tb;
+      u_in0 = ui[0];
+      u_in1 = ui[1];
+      u_in2 = ui[2];
+    }
+    if (a.mask) skip_in = a.mask[b * BC + tb];
+  }
   if (OPS & OP_XIN) {
 #pragma unroll
     for (int i = 0; i < NXT; i++) {
@@ -321,12 +339,12 @@ __global__ __launch_bounds__(256) void k_block_ops(BlockOpArgs a) {
         }
       }
       if (MOMENTS) {
-        const bool act = sin[c] && !(a.mask && a.mask[b * BC + c]);
+        const bool act = sin[c] && !skip_in;
         double rho, u0, u1, u2;
         if (OPS & OP_VALUES) {
           // computeDensity (:36-39) and computeVelocity (:57-61); skipped cells keep their density
           // and get u = 0
-          rho = a.density[b * BC + c];
+          rho = rho_in;
           u0 = u1 = u2 = 0.0;
           if (act) {
             rho = 0.0;
@@ -353,11 +371,10 @@ __global__ __launch_bounds__(256) void k_block_ops(BlockOpArgs a) {
           uo[1] = u1;
           uo[2] = u2;
         } else {
-          rho = a.density[b * BC + c];
-          const double* ui = a.velocity + b * BV + 3 * c;
-          u0 = ui[0];
-          u1 = ui[1];
-          u2 = ui[2];
+          rho = rho_in;
+          u0 = u_in0;
+          u1 = u_in1;
+          u2 = u_in2;
         }
         srho[c] = rho;
         su[3 * c] = u0;
@@ -375,8 +392,7 @@ __global__ __launch_bounds__(256) void k_block_ops(BlockOpArgs a) {
     }
     if (OPS & OP_BOUNDARY) {
       __syncthreads();
-      const int32_t l1 = a.off[p + 1];
-      for (int32_t k = a.off[p] + t; k < l1; k += 256) {
+      for (int32_t k = link0 + t; k < link1; k += 256) {
         const BlockLinkDev L = a.links[k];
         block_boundary_f64(s[L.cell * Q + Q - 1 - L.q_in], L, srho[L.cell], su[3 * L.cell], su[3 * L.cell + 1],
                            su[3 * L.cell + 2]);
