@@ -188,7 +188,7 @@ constexpr CrossSegments make_cross_segments() {
 }
 constexpr CrossSegments kSeg = make_cross_segments();
 
-__device__ __forceinline__ void cross_segment(int e, int& d, int& partner) {
+__host__ __device__ constexpr void cross_segment(int e, int& d, int& partner) {
   d = 0;
   int delta = kSeg.delta[0];
 #pragma unroll
@@ -199,6 +199,19 @@ __device__ __forceinline__ void cross_segment(int e, int& d, int& partner) {
     }
   partner = e + delta;
 }
+
+// the comparisons give every entry the direction the table packs into it, and the partner entry
+// that the table's segments pair with it
+constexpr bool cross_segments_agree() {
+  const CrossTable t = make_cross_table();
+  for (int e = 0; e < NX_ENTRIES; e++) {
+    int d = -1, pe = -1;
+    cross_segment(e, d, pe);
+    if (d != (int)((t.e[e] >> 13) & 31u) || pe != t.start[t.opp[d]] + (e - t.start[d])) return false;
+  }
+  return true;
+}
+static_assert(cross_segments_agree(), "segment bounds and the crossing table disagree");
 
 // ---------------------------------------------------------------------------------------------
 // OP_XIN: the previous iteration's cross-block swaps, each valid entry taking its partner's
