@@ -407,6 +407,47 @@ int lbm_blocks_vertex_first(lbm_blockstore* bs, const int64_t* blocks, int64_t n
 int lbm_blocks_iterate(lbm_blockstore* bs, const int64_t* blocks, int64_t nblocks_list, const int64_t* cells,
                        int64_t ncells, double tau, const lbm_block_boundary* links, int64_t nlinks, int nsteps);
 
+/* ---- block store: the vertex-overlap exchange between ranks --------------------------------
+ * The reference's Parallel build keeps a copy of a vertex's block on every rank whose Peano cells
+ * touch it, runs the operators above on each copy, and merges the copies after the iteration
+ * (BLB/RegularGridBlockVertex.cpp:51-160, called from RegularGrid2RegularBlockSolver.cpp:158-159
+ * before the local streaming, :243-245 after the in-cell swap, and mergeWithNeighbour :349-364 per
+ * neighbour copy).  A copy records the populations that leave the block before streaming and adds
+ * those that arrive after it, so its difference is nonzero exactly where one of its own rank's
+ * Peano cells swapped the link; merging adds a neighbour copy's difference where the own one is
+ * 0.0.  Each block keeps its 1,008 differences (in the order of parallel::BlockBoundaryIndex4Pdfs,
+ * BLB/parallel/BlockBoundaryIndex4Pdfs.cpp:30-77) and 208 boundary densities (BlockBoundaryIndex,
+ * BLB/BlockBoundaryIndex.cpp:15-88) in the store, as the vertex record holds _lbPdfDiff and
+ * _lbDensityOnBoundary (BLB/dastgen/BlockVertex.def:11-15); a new store has them all 0.0.  The
+ * transport between GPUs is the caller's: the get and merge calls take host or device memory. */
+/* the geometry-boundary bitset of blocks first .. first + count - 1 (216 flags per block, nonzero =
+ * the cell is neither completely inside nor completely outside the domain;
+ * GridManagementService::getBoundary, BLB/services/GridManagementService.h:74-80); a new store has
+ * none set.  Only the boundary-density operators read it. */
+int lbm_blocks_set_boundary(lbm_blockstore* bs, int64_t first, int64_t count, const uint8_t* boundary);
+/* initPdfDiff (RegularGridBlockVertex.cpp:96-105) on each listed block: diff_i = -pdf[outgoing_i] */
+int lbm_blocks_init_pdf_diff(lbm_blockstore* bs, const int64_t* blocks, int64_t n);
+/* computePdfDiff (:108-120): diff_i = diff_i + pdf[incoming_i] */
+int lbm_blocks_compute_pdf_diff(lbm_blockstore* bs, const int64_t* blocks, int64_t n);
+/* the listed blocks' differences, 1,008 per block in list order, into out (host or device memory) */
+int lbm_blocks_get_pdf_diff(lbm_blockstore* bs, const int64_t* blocks, int64_t n, double* out);
+/* mergePdfDiff (:123-141) with another rank's copy of each listed block: pdf[incoming_i] =
+ * pdf[incoming_i] + neighbour_i wherever the block's own diff_i is 0.0 (neighbour: 1,008 values per
+ * listed block in list order, host or device memory) */
+int lbm_blocks_merge_pdf_diff(lbm_blockstore* bs, const int64_t* blocks, int64_t n, const double* neighbour);
+/* resetDensityOnBlockBoundary (:51-53): the listed blocks' boundary densities become 0.0 */
+int lbm_blocks_reset_density_on_boundary(lbm_blockstore* bs, const int64_t* blocks, int64_t n);
+/* setDensityOnBlockBoundary (:56-77): with dynamic_geometry != 0, boundary density i of each listed
+ * block is the density of block-boundary cell i when that cell is not inner and on the geometry
+ * boundary, else 0.0; with dynamic_geometry == 0 nothing changes */
+int lbm_blocks_set_density_on_boundary(lbm_blockstore* bs, const int64_t* blocks, int64_t n, int dynamic_geometry);
+/* the listed blocks' boundary densities, 208 per block in list order, into out (host or device) */
+int lbm_blocks_get_density_on_boundary(lbm_blockstore* bs, const int64_t* blocks, int64_t n, double* out);
+/* mergeDensityOnBoundary (:144-160): density[c_i] += neighbour_i for each block-boundary cell c_i
+ * that is not inner and on the geometry boundary (neighbour: 208 values per listed block) */
+int lbm_blocks_merge_density_on_boundary(lbm_blockstore* bs, const int64_t* blocks, int64_t n,
+                                         const double* neighbour);
+
 #ifdef __cplusplus
 }
 #endif

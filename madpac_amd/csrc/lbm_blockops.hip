@@ -487,6 +487,136 @@ __global__ __launch_bounds__(512) void k_block_valid(const int64_t* __restrict__
   atomicOr(&valid[bj * 32 + (e2 >> 5)], 1u << (e2 & 31));
 }
 
+// ---------------------------------------------------------------------------------------------
+// The vertex-overlap exchange between ranks.  parallel::BlockBoundaryIndex4Pdfs::initBoundaryIndex
+// (BLB/parallel/BlockBoundaryIndex4Pdfs.cpp:30-77): cells in z, y, x order, q in order, every
+// population whose target cell + c_q lies outside the block, with its incoming slot 19 cell + 18 - q
+// and outgoing slot 19 cell + q.  BlockBoundaryIndex (BLB/BlockBoundaryIndex.cpp:15-88): the cells
+// with a coordinate below 2 or above 3, in increasing index.
+constexpr int PB_ENTRIES = 1008, BB_CELLS = 208;
+struct PdfBoundaryTable {
+  int16_t in[PB_ENTRIES], out[PB_ENTRIES];
+  int n;
+};
+constexpr PdfBoundaryTable make_pdf_boundary_table() {
+  PdfBoundaryTable t{};
+  int n = 0;
+  for (int c = 0; c < BC; c++)
+    for (int q = 0; q < Q; q++) {
+      const int x = c % BS + CX(q), y = (c / BS) % BS + CY(q), z = c / (BS * BS) + CZ(q);
+      if (x >= 0 && x < BS && y >= 0 && y < BS && z >= 0 && z < BS) continue;
+      if (n < PB_ENTRIES) {
+        t.in[n] = (int16_t)(c * Q + Q - 1 - q);
+        t.out[n] = (int16_t)(c * Q + q);
+      }
+      n++;
+    }
+  t.n = n;
+  return t;
+}
+static_assert(make_pdf_boundary_table().n == PB_ENTRIES, "a 6^3 block has 1,008 populations leaving it");
+
+// the populations leaving a block are its crossing entries, each listed once
+constexpr bool pdf_boundary_is_crossing_set() {
+  const PdfBoundaryTable p = make_pdf_boundary_table();
+  const CrossTable t = make_cross_table();
+  bool seen[NX_ENTRIES] = {};
+  for (int i = 0; i < PB_ENTRIES; i++) {
+    const int e = t.entry_of[p.out[i]];
+    if (e < 0 || seen[e]) return false;
+    seen[e] = true;
+  }
+  return true;
+}
+static_assert(pdf_boundary_is_crossing_set(), "leaving populations and crossing entries disagree");
+
+struct BoundaryCellTable {
+  int16_t c[BB_CELLS];
+  int n;
+};
+constexpr BoundaryCellTable make_boundary_cell_table() {
+  BoundaryCellTable t{};
+  int n = 0;
+  for (int c = 0; c < BC; c++) {
+    const int x = c % BS, y = (c / BS) % BS, z = c / (BS * BS);
+    if (!(x < 2 || x > BS - 3 || y < 2 || y > BS - 3 || z < 2 || z > BS - 3)) continue;
+    if (n < BB_CELLS) t.c[n] = (int16_t)c;
+    n++;
+  }
+  t.n = n;
+  return t;
+}
+static_assert(make_boundary_cell_table().n == BB_CELLS, "a 6^3 block has 208 cells near its boundary");
+
+__constant__ PdfBoundaryTable kPdfBoundary = make_pdf_boundary_table();
+__constant__ BoundaryCellTable kBoundaryCells = make_boundary_cell_table();
+
+enum : int { PD_INIT = 0, PD_COMPUTE = 1, PD_MERGE = 2 };
+
+// initPdfDiff / computePdfDiff / mergePdfDiff (BLB/RegularGridBlockVertex.cpp:96-141) on the listed
+// blocks, one workgroup per block and up to four entries per thread (their loads issued together)
+template <int MODE>
+__global__ __launch_bounds__(256) void k_block_pdf_diff(const int64_t* __restrict__ list, int64_t n,
+                                                        double* __restrict__ pdf, double* __restrict__ diff,
+                                                        const double* __restrict__ neighbour) {
+  const int64_t p = blockIdx.x;
+  if (p >= n) return;
+  const int64_t b = list[p];
+  const int t = threadIdx.x;
+  constexpr int NT = (PB_ENTRIES + 255) / 256;
+  double* f = pdf + b * BP;
+  double* dv = diff + b * PB_ENTRIES;
+#pragma unroll
+  for (int k = 0; k < NT; k++) {
+    const int i = t + 256 * k;
+    if (i >= PB_ENTRIES) break;
+    if (MODE == PD_INIT) {
+      dv[i] = -f[kPdfBoundary.out[i]];
+    } else if (MODE == PD_COMPUTE) {
+      dv[i] = dv[i] + f[kPdfBoundary.in[i]];
+    } else if (dv[i] == 0.0) {
+      const int s = kPdfBoundary.in[i];
+      f[s] = f[s] + neighbour[p * PB_ENTRIES + i];
+    }
+  }
+}
+
+enum : int { BD_RESET = 0, BD_SET = 1, BD_MERGE = 2 };
+
+// resetDensityOnBlockBoundary / setDensityOnBlockBoundary / mergeDensityOnBoundary (:51-77,
+// :144-160): block-boundary cell i of each listed block, where it is not inner but on the geometry
+// boundary
+template <int MODE>
+__global__ __launch_bounds__(256) void k_block_boundary_density(const int64_t* __restrict__ list, int64_t n,
+                                                                double* __restrict__ density,
+                                                                const uint8_t* __restrict__ inner,
+                                                                const uint8_t* __restrict__ boundary,
+                                                                double* __restrict__ bdens,
+                                                                const double* __restrict__ neighbour) {
+  const int64_t p = blockIdx.x;
+  const int i = threadIdx.x;
+  if (p >= n || i >= BB_CELLS) return;
+  const int64_t b = list[p];
+  const int c = kBoundaryCells.c[i];
+  const bool on = !inner[b * BC + c] && boundary[b * BC + c];
+  if (MODE == BD_RESET)
+    bdens[b * BB_CELLS + i] = 0.0;
+  else if (MODE == BD_SET)
+    bdens[b * BB_CELLS + i] = on ? density[b * BC + c] : 0.0;
+  else if (on)
+    density[b * BC + c] += neighbour[p * BB_CELLS + i];
+}
+
+// rows of a per-block array for the listed blocks, in list order (width values per block)
+__global__ __launch_bounds__(256) void k_block_rows(const int64_t* __restrict__ list, int64_t n,
+                                                    const double* __restrict__ src, int width,
+                                                    double* __restrict__ dst) {
+  const int64_t p = blockIdx.x;
+  if (p >= n) return;
+  const int64_t b = list[p];
+  for (int i = threadIdx.x; i < width; i += 256) dst[p * width + i] = src[b * width + i];
+}
+
 }  // namespace lbm
 
 using namespace lbm;
@@ -640,6 +770,91 @@ int run_block_op(lbm_blockstore* bs, const int64_t* blocks, int64_t n, double ta
   return finish(bs);
 }
 
+// The exchange buffers, allocated and cleared on first use; their contents persist between calls.
+int ensure_exchange(lbm_blockstore* bs) {
+  if (bs->d_pdfdiff) return LBM_OK;
+  const size_t nb = (size_t)bs->nblocks;
+  double* d = nullptr;
+  double* r = nullptr;
+  uint8_t* f = nullptr;
+  if (device_alloc(&d, nb * PB_ENTRIES * sizeof(double)) != hipSuccess ||
+      device_alloc(&r, nb * BB_CELLS * sizeof(double)) != hipSuccess || device_alloc(&f, nb * BC) != hipSuccess) {
+    if (d) (void)hipFree(d);
+    if (r) (void)hipFree(r);
+    return bfail(bs, LBM_ERR_NOMEM, "exchange buffer allocation failed");
+  }
+  bs->d_pdfdiff = d;
+  bs->d_bdens = r;
+  bs->boundary = f;
+  BCHK(bs, hipMemsetAsync(d, 0, nb * PB_ENTRIES * sizeof(double), bs->stream));
+  BCHK(bs, hipMemsetAsync(r, 0, nb * BB_CELLS * sizeof(double), bs->stream));
+  BCHK(bs, hipMemsetAsync(f, 0, nb * BC, bs->stream));
+  BCHK(bs, hipStreamSynchronize(bs->stream));
+  return LBM_OK;
+}
+
+// Validates the list, prepares the buffers and, for a merge, stages the neighbour's values (width
+// per listed block; host or device memory).
+int exchange_prologue(lbm_blockstore* bs, const int64_t* blocks, int64_t n, const double* in, int width,
+                      const char* what) {
+  BCHK(bs, hipSetDevice(bs->device));
+  if (int rc = ensure_exchange(bs)) return rc;
+  if (int rc = upload_blocks(bs, blocks, n, &bs->d_list, &bs->cap_list)) return rc;
+  if (width) {
+    if (n && !in) return bfail(bs, LBM_ERR_ARG, std::string(what) + " are null");
+    if (int rc = ensure_capacity(bs, &bs->d_xfer, &bs->cap_xfer, std::max<int64_t>(n * width, 1), "exchange staging"))
+      return rc;
+    if (n) BCHK(bs, hipMemcpyAsync(bs->d_xfer, in, (size_t)(n * width) * sizeof(double), hipMemcpyDefault, bs->stream));
+  }
+  return LBM_OK;
+}
+
+template <int MODE>
+int run_pdf_diff(lbm_blockstore* bs, const int64_t* blocks, int64_t n, const double* neighbour) {
+  if (!bs) return LBM_ERR_ARG;
+  const int width = MODE == PD_MERGE ? PB_ENTRIES : 0;
+  if (int rc = exchange_prologue(bs, blocks, n, neighbour, width, "neighbour differences")) return rc;
+  if (int rc = begin(bs)) return rc;
+  if (n) {
+    hipLaunchKernelGGL(k_block_pdf_diff<MODE>, dim3((unsigned)n), dim3(256), 0, bs->stream, bs->d_list, n, bs->pdf,
+                       bs->d_pdfdiff, bs->d_xfer);
+    BCHK(bs, hipGetLastError());
+  }
+  return finish(bs);
+}
+
+template <int MODE>
+int run_boundary_density(lbm_blockstore* bs, const int64_t* blocks, int64_t n, const double* neighbour) {
+  if (!bs) return LBM_ERR_ARG;
+  const int width = MODE == BD_MERGE ? BB_CELLS : 0;
+  if (int rc = exchange_prologue(bs, blocks, n, neighbour, width, "neighbour boundary densities")) return rc;
+  if (int rc = begin(bs)) return rc;
+  if (n) {
+    hipLaunchKernelGGL(k_block_boundary_density<MODE>, dim3((unsigned)n), dim3(256), 0, bs->stream, bs->d_list, n,
+                       bs->density, bs->inner, bs->boundary, bs->d_bdens, bs->d_xfer);
+    BCHK(bs, hipGetLastError());
+  }
+  return finish(bs);
+}
+
+// rows of an exchange buffer for the listed blocks into out (host or device memory)
+int get_rows(lbm_blockstore* bs, const int64_t* blocks, int64_t n, double* out, bool pdf_diff) {
+  if (!bs) return LBM_ERR_ARG;
+  if (n && !out) return bfail(bs, LBM_ERR_ARG, "output is null");
+  if (int rc = exchange_prologue(bs, blocks, n, nullptr, 0, "")) return rc;
+  const int width = pdf_diff ? PB_ENTRIES : BB_CELLS;
+  if (int rc = ensure_capacity(bs, &bs->d_xfer, &bs->cap_xfer, std::max<int64_t>(n * width, 1), "exchange staging"))
+    return rc;
+  if (n) {
+    hipLaunchKernelGGL(k_block_rows, dim3((unsigned)n), dim3(256), 0, bs->stream, bs->d_list, n,
+                       pdf_diff ? bs->d_pdfdiff : bs->d_bdens, width, bs->d_xfer);
+    BCHK(bs, hipGetLastError());
+    BCHK(bs, hipMemcpyAsync(out, bs->d_xfer, (size_t)(n * width) * sizeof(double), hipMemcpyDefault, bs->stream));
+  }
+  BCHK(bs, hipStreamSynchronize(bs->stream));
+  return LBM_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -781,6 +996,59 @@ int lbm_blocks_iterate(lbm_blockstore* bs, const int64_t* blocks, int64_t nblock
     if (int rc = launch_ops<OP_XIN | OP_SWAP_IN>(bs, a)) return rc;
   }
   return finish(bs);
+}
+
+int lbm_blocks_set_boundary(lbm_blockstore* bs, int64_t first, int64_t count, const uint8_t* boundary) {
+  if (!bs) return LBM_ERR_ARG;
+  if (!boundary) return bfail(bs, LBM_ERR_ARG, "boundary flags are null");
+  if (int rc = check_range(bs, first, count)) return rc;
+  BCHK(bs, hipSetDevice(bs->device));
+  if (int rc = ensure_exchange(bs)) return rc;
+  std::vector<uint8_t> m((size_t)count * BC);
+  for (size_t k = 0; k < m.size(); k++) m[k] = boundary[k] ? 1 : 0;
+  BCHK(bs, hipMemcpyAsync(bs->boundary + first * BC, m.data(), m.size(), hipMemcpyHostToDevice, bs->stream));
+  BCHK(bs, hipStreamSynchronize(bs->stream));
+  return LBM_OK;
+}
+
+int lbm_blocks_init_pdf_diff(lbm_blockstore* bs, const int64_t* blocks, int64_t n) {
+  return run_pdf_diff<PD_INIT>(bs, blocks, n, nullptr);
+}
+
+int lbm_blocks_compute_pdf_diff(lbm_blockstore* bs, const int64_t* blocks, int64_t n) {
+  return run_pdf_diff<PD_COMPUTE>(bs, blocks, n, nullptr);
+}
+
+int lbm_blocks_get_pdf_diff(lbm_blockstore* bs, const int64_t* blocks, int64_t n, double* out) {
+  return get_rows(bs, blocks, n, out, true);
+}
+
+int lbm_blocks_merge_pdf_diff(lbm_blockstore* bs, const int64_t* blocks, int64_t n, const double* neighbour) {
+  return run_pdf_diff<PD_MERGE>(bs, blocks, n, neighbour);
+}
+
+int lbm_blocks_reset_density_on_boundary(lbm_blockstore* bs, const int64_t* blocks, int64_t n) {
+  return run_boundary_density<BD_RESET>(bs, blocks, n, nullptr);
+}
+
+int lbm_blocks_set_density_on_boundary(lbm_blockstore* bs, const int64_t* blocks, int64_t n, int dynamic_geometry) {
+  // without a dynamic geometry the reference returns before touching the buffer (:57-59); the list
+  // is still checked
+  if (!dynamic_geometry) {
+    if (!bs) return LBM_ERR_ARG;
+    BCHK(bs, hipSetDevice(bs->device));
+    return upload_blocks(bs, blocks, n, &bs->d_list, &bs->cap_list);
+  }
+  return run_boundary_density<BD_SET>(bs, blocks, n, nullptr);
+}
+
+int lbm_blocks_get_density_on_boundary(lbm_blockstore* bs, const int64_t* blocks, int64_t n, double* out) {
+  return get_rows(bs, blocks, n, out, false);
+}
+
+int lbm_blocks_merge_density_on_boundary(lbm_blockstore* bs, const int64_t* blocks, int64_t n,
+                                         const double* neighbour) {
+  return run_boundary_density<BD_MERGE>(bs, blocks, n, neighbour);
 }
 
 }  // extern "C"

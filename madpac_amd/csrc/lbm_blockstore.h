@@ -103,6 +103,11 @@ struct lbm_blockstore {
   int64_t cap_valid = 0;
   double* d_xch = nullptr;     // lbm_blocks_iterate: two exchange arrays of 1,008 values per block
   int64_t cap_xch = 0;
+  double* d_pdfdiff = nullptr; // exchange between ranks: 1,008 differences per block (allocated on first use)
+  double* d_bdens = nullptr;   // exchange between ranks: 208 boundary densities per block
+  uint8_t* boundary = nullptr; // exchange between ranks: nblocks * 216 geometry-boundary flags
+  double* d_xfer = nullptr;    // exchange between ranks: staging of the values a call copies in or out
+  int64_t cap_xfer = 0;
   uint64_t inner_gen = 0;      // bumped by every lbm_blocks_set_inner
   std::vector<int64_t> iter_cells;  // the cell list d_nbr and d_valid were built for ...
   uint64_t iter_inner_gen = ~0ull;  // ... and the inner flags

@@ -456,6 +456,52 @@ class AABlockStreaming {
   }
 };
 
+// The Parallel build's exchange of a vertex's copies between ranks (BLB/RegularGridBlockVertex.cpp:
+// 51-160, called from RegularGrid2RegularBlockSolver.cpp:158-159, :243-245 and mergeWithNeighbour
+// :349-364): the vertex methods, batched over the vertices a rank shares.  The differences
+// (LB_PDFS_ON_BLOCKBOUNDARY = 1,008 per vertex) and boundary densities (208 per vertex) stay in the
+// device store between calls; getPdfDiff / getDensityOnBoundary copy them, in list order, into the
+// buffer a transport sends, and the merges take the neighbour rank's buffer (host or device memory).
+class BlockVertexExchange {
+ public:
+  static constexpr int PdfsOnBlockBoundary = 1008;
+  static constexpr int CellsOnBlockBoundary = 208;
+  // the geometry-boundary bitset of one vertex (GridManagementService::getBoundary)
+  void setBoundary(DeviceBlocks& s, int64_t vertex, const std::vector<uint8_t>& boundary) const {
+    s.check(lbm_blocks_set_boundary(s.handle(), vertex, 1, boundary.data()), "lbm_blocks_set_boundary");
+  }
+  void initPdfDiff(DeviceBlocks& s, const std::vector<int64_t>& vertices) const {
+    s.check(lbm_blocks_init_pdf_diff(s.handle(), vertices.data(), (int64_t)vertices.size()), "lbm_blocks_init_pdf_diff");
+  }
+  void computePdfDiff(DeviceBlocks& s, const std::vector<int64_t>& vertices) const {
+    s.check(lbm_blocks_compute_pdf_diff(s.handle(), vertices.data(), (int64_t)vertices.size()),
+            "lbm_blocks_compute_pdf_diff");
+  }
+  void getPdfDiff(DeviceBlocks& s, const std::vector<int64_t>& vertices, double* out) const {
+    s.check(lbm_blocks_get_pdf_diff(s.handle(), vertices.data(), (int64_t)vertices.size(), out), "lbm_blocks_get_pdf_diff");
+  }
+  void mergePdfDiff(DeviceBlocks& s, const std::vector<int64_t>& vertices, const double* neighbour) const {
+    s.check(lbm_blocks_merge_pdf_diff(s.handle(), vertices.data(), (int64_t)vertices.size(), neighbour),
+            "lbm_blocks_merge_pdf_diff");
+  }
+  void resetDensityOnBlockBoundary(DeviceBlocks& s, const std::vector<int64_t>& vertices) const {
+    s.check(lbm_blocks_reset_density_on_boundary(s.handle(), vertices.data(), (int64_t)vertices.size()),
+            "lbm_blocks_reset_density_on_boundary");
+  }
+  void setDensityOnBlockBoundary(DeviceBlocks& s, const std::vector<int64_t>& vertices, bool useDynamicGeometry) const {
+    s.check(lbm_blocks_set_density_on_boundary(s.handle(), vertices.data(), (int64_t)vertices.size(), useDynamicGeometry),
+            "lbm_blocks_set_density_on_boundary");
+  }
+  void getDensityOnBoundary(DeviceBlocks& s, const std::vector<int64_t>& vertices, double* out) const {
+    s.check(lbm_blocks_get_density_on_boundary(s.handle(), vertices.data(), (int64_t)vertices.size(), out),
+            "lbm_blocks_get_density_on_boundary");
+  }
+  void mergeDensityOnBoundary(DeviceBlocks& s, const std::vector<int64_t>& vertices, const double* neighbour) const {
+    s.check(lbm_blocks_merge_density_on_boundary(s.handle(), vertices.data(), (int64_t)vertices.size(), neighbour),
+            "lbm_blocks_merge_density_on_boundary");
+  }
+};
+
 class Repository;
 
 // The block store (BLB/services/GridManagementService.h:40-86, .cpp:295-327): loadVertex selects a

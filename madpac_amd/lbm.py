@@ -52,7 +52,9 @@ EXPORTS = (
     "lbm_blocks_find_overlap", "lbm_blocks_set_overlap", "lbm_blocks_prolongate", "lbm_blocks_restrict", "lbm_blocks_last_ms",
     "lbm_blocks_set_inner", "lbm_blocks_compute_cell_values", "lbm_blocks_collide", "lbm_blocks_treat_boundary",
     "lbm_blocks_stream_local", "lbm_blocks_stream_nonlocal", "lbm_blocks_swap_inside", "lbm_blocks_vertex_first",
-    "lbm_blocks_iterate",
+    "lbm_blocks_iterate", "lbm_blocks_set_boundary", "lbm_blocks_init_pdf_diff", "lbm_blocks_compute_pdf_diff",
+    "lbm_blocks_get_pdf_diff", "lbm_blocks_merge_pdf_diff", "lbm_blocks_reset_density_on_boundary",
+    "lbm_blocks_set_density_on_boundary", "lbm_blocks_get_density_on_boundary", "lbm_blocks_merge_density_on_boundary",
 )
 
 
@@ -189,6 +191,13 @@ def load_library(path: str = LIB_PATH):
     L.lbm_blocks_vertex_first.argtypes = [vp, i64p, ctypes.c_int64, ctypes.c_double, ctypes.c_int, bp, ctypes.c_int64]
     L.lbm_blocks_iterate.argtypes = [vp, i64p, ctypes.c_int64, i64p, ctypes.c_int64, ctypes.c_double, bp,
                                      ctypes.c_int64, ctypes.c_int]
+    dpp = ctypes.POINTER(ctypes.c_double)
+    L.lbm_blocks_set_boundary.argtypes = [vp, ctypes.c_int64, ctypes.c_int64, u8p]
+    for name in ("init_pdf_diff", "compute_pdf_diff", "reset_density_on_boundary"):
+        getattr(L, "lbm_blocks_" + name).argtypes = [vp, i64p, ctypes.c_int64]
+    for name in ("get_pdf_diff", "merge_pdf_diff", "get_density_on_boundary", "merge_density_on_boundary"):
+        getattr(L, "lbm_blocks_" + name).argtypes = [vp, i64p, ctypes.c_int64, dpp]
+    L.lbm_blocks_set_density_on_boundary.argtypes = [vp, i64p, ctypes.c_int64, ctypes.c_int]
     _lib = L
     return L
 
@@ -534,6 +543,8 @@ def cavity(n, tau, ulid, precision=FP64, device=0, layout=LAYOUT_TWO_BUFFER):
 
 
 BLOCK_CELLS = 216
+BLOCK_BOUNDARY_PDFS = 1008   # populations leaving a block (BLB/parallel/BlockBoundaryIndex4Pdfs.cpp)
+BLOCK_BOUNDARY_CELLS = 208   # cells near the block boundary (BLB/BlockBoundaryIndex.cpp)
 
 
 class BlockStore:
@@ -697,3 +708,55 @@ class BlockStore:
         c, nc, pc = self._ids(cells, 8)
         arr, nl, _keep = self.boundary_links(links)
         self._check(self._L.lbm_blocks_iterate(self._h, p, n, pc, nc, float(tau), arr, nl, int(steps)), "lbm_blocks_iterate")
+
+    # ----- the vertex-overlap exchange between ranks (BLB/RegularGridBlockVertex.cpp:51-160) -----
+    def set_boundary(self, first, boundary):
+        m = np.ascontiguousarray(np.asarray(boundary).reshape(-1) != 0, dtype=np.uint8)
+        if m.size % BLOCK_CELLS:
+            raise ValueError("boundary: not a whole number of blocks")
+        self._check(self._L.lbm_blocks_set_boundary(self._h, int(first), m.size // BLOCK_CELLS, _u8(m)),
+                    "lbm_blocks_set_boundary")
+
+    def init_pdf_diff(self, blocks):
+        a, n, p = self._ids(blocks)
+        self._check(self._L.lbm_blocks_init_pdf_diff(self._h, p, n), "lbm_blocks_init_pdf_diff")
+
+    def compute_pdf_diff(self, blocks):
+        a, n, p = self._ids(blocks)
+        self._check(self._L.lbm_blocks_compute_pdf_diff(self._h, p, n), "lbm_blocks_compute_pdf_diff")
+
+    def _rows_out(self, fn, blocks, width):
+        a, n, p = self._ids(blocks)
+        out = np.empty((n, width))
+        self._check(getattr(self._L, fn)(self._h, p, n, _dp(out)), fn)
+        return out
+
+    def _rows_in(self, fn, blocks, values, width):
+        a, n, p = self._ids(blocks)
+        v = np.ascontiguousarray(np.asarray(values, dtype=np.float64).reshape(-1))
+        if v.size != n * width:
+            raise ValueError(f"{fn}: expected {n * width} values, got {v.size}")
+        self._check(getattr(self._L, fn)(self._h, p, n, _dp(v)), fn)
+
+    def get_pdf_diff(self, blocks):
+        """(n, 1008) differences of the listed blocks"""
+        return self._rows_out("lbm_blocks_get_pdf_diff", blocks, BLOCK_BOUNDARY_PDFS)
+
+    def merge_pdf_diff(self, blocks, neighbour):
+        self._rows_in("lbm_blocks_merge_pdf_diff", blocks, neighbour, BLOCK_BOUNDARY_PDFS)
+
+    def reset_density_on_boundary(self, blocks):
+        a, n, p = self._ids(blocks)
+        self._check(self._L.lbm_blocks_reset_density_on_boundary(self._h, p, n), "lbm_blocks_reset_density_on_boundary")
+
+    def set_density_on_boundary(self, blocks, dynamic_geometry=True):
+        a, n, p = self._ids(blocks)
+        self._check(self._L.lbm_blocks_set_density_on_boundary(self._h, p, n, int(bool(dynamic_geometry))),
+                    "lbm_blocks_set_density_on_boundary")
+
+    def get_density_on_boundary(self, blocks):
+        """(n, 208) boundary densities of the listed blocks"""
+        return self._rows_out("lbm_blocks_get_density_on_boundary", blocks, BLOCK_BOUNDARY_CELLS)
+
+    def merge_density_on_boundary(self, blocks, neighbour):
+        self._rows_in("lbm_blocks_merge_density_on_boundary", blocks, neighbour, BLOCK_BOUNDARY_CELLS)

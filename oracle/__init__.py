@@ -130,6 +130,15 @@ def lib():
         L.lbo_block_nonlocal_index.restype = ctypes.c_int
         L.lbo_block_nonlocal_index.argtypes = [ip, ctypes.c_int]
         L.lbo_block_stream_nonlocal.argtypes = [ctypes.POINTER(u8), ctypes.POINTER(dp)]
+        L.lbo_block_pdf_boundary.restype = ctypes.c_int
+        L.lbo_block_pdf_boundary.argtypes = [ip, ip, ctypes.c_int]
+        L.lbo_block_boundary_cells.restype = ctypes.c_int
+        L.lbo_block_boundary_cells.argtypes = [ip, ctypes.c_int]
+        L.lbo_block_init_pdf_diff.argtypes = [dp, dp]
+        L.lbo_block_compute_pdf_diff.argtypes = [dp, dp]
+        L.lbo_block_merge_pdf_diff.argtypes = [dp, dp, dp]
+        L.lbo_block_set_density_on_boundary.argtypes = [dp, u8, u8, ctypes.c_int, dp]
+        L.lbo_block_merge_density_on_boundary.argtypes = [dp, u8, u8, dp]
         _lib = L
     return _lib
 
@@ -447,3 +456,60 @@ def block_stream_nonlocal(inner, pdfs):
     pp = (ctypes.POINTER(ctypes.c_double) * 8)(*[_dp(a) for a in out])
     lib().lbo_block_stream_nonlocal(ip, pp)
     return np.stack(out)
+
+
+# the vertex-overlap exchange between ranks (BLB/RegularGridBlockVertex.cpp:51-160)
+BLOCK_BOUNDARY_PDFS = 1008
+BLOCK_BOUNDARY_CELLS = 208
+
+
+def block_pdf_boundary():
+    """(incoming, outgoing) slots of the populations that leave a block, in the reference's order."""
+    a = (ctypes.c_int * 2048)()
+    b = (ctypes.c_int * 2048)()
+    n = lib().lbo_block_pdf_boundary(ctypes.cast(a, ctypes.POINTER(ctypes.c_int)),
+                                     ctypes.cast(b, ctypes.POINTER(ctypes.c_int)), 2048)
+    return np.frombuffer(a, np.int32)[:n].copy(), np.frombuffer(b, np.int32)[:n].copy()
+
+
+def block_boundary_cells():
+    a = (ctypes.c_int * 256)()
+    n = lib().lbo_block_boundary_cells(ctypes.cast(a, ctypes.POINTER(ctypes.c_int)), 256)
+    return np.frombuffer(a, np.int32)[:n].copy()
+
+
+def block_init_pdf_diff(pdf):
+    diff = np.zeros(BLOCK_BOUNDARY_PDFS)
+    lib().lbo_block_init_pdf_diff(_dp(_block(pdf, BLOCK_PDFS)), _dp(diff))
+    return diff
+
+
+def block_compute_pdf_diff(pdf, diff):
+    out = np.array(diff, dtype=np.float64).reshape(-1)
+    assert out.size == BLOCK_BOUNDARY_PDFS
+    lib().lbo_block_compute_pdf_diff(_dp(_block(pdf, BLOCK_PDFS)), _dp(out))
+    return out
+
+
+def block_merge_pdf_diff(pdf, own, neighbour):
+    """Returns the merged block (input not modified)."""
+    out = np.array(pdf, dtype=np.float64).reshape(-1)
+    assert out.size == BLOCK_PDFS
+    lib().lbo_block_merge_pdf_diff(_dp(out), _dp(_block(own, BLOCK_BOUNDARY_PDFS)),
+                                   _dp(_block(neighbour, BLOCK_BOUNDARY_PDFS)))
+    return out
+
+
+def block_set_density_on_boundary(density, inner, boundary, dynamic_geometry, previous=None):
+    out = np.zeros(BLOCK_BOUNDARY_CELLS) if previous is None else np.array(previous, dtype=np.float64).reshape(-1)
+    lib().lbo_block_set_density_on_boundary(_dp(_block(density, BLOCK_CELLS)), _u8p(_flags(inner)),
+                                            _u8p(_flags(boundary)), 1 if dynamic_geometry else 0, _dp(out))
+    return out
+
+
+def block_merge_density_on_boundary(density, inner, boundary, neighbour):
+    out = np.array(density, dtype=np.float64).reshape(-1)
+    assert out.size == BLOCK_CELLS
+    lib().lbo_block_merge_density_on_boundary(_dp(out), _u8p(_flags(inner)), _u8p(_flags(boundary)),
+                                              _dp(_block(neighbour, BLOCK_BOUNDARY_CELLS)))
+    return out

@@ -923,3 +923,94 @@ void lbo_block_stream_nonlocal(const uint8_t* const* inner, double* const* pdf) 
     *b = t;
   }
 }
+
+/* The vertex-overlap exchange between ranks (compiled in the reference's Parallel build only).
+ * parallel::BlockBoundaryIndex4Pdfs::initBoundaryIndex (parallel/BlockBoundaryIndex4Pdfs.cpp:30-77):
+ * for cells in z, y, x order and q in order, every population whose target cell + c_q lies outside
+ * the 6^3 block is listed with its incoming slot 19 cell + 18 - q and its outgoing slot 19 cell + q.
+ * Returns the number of entries (1,008). */
+int lbo_block_pdf_boundary(int* incoming, int* outgoing, int max) {
+  int n = 0;
+  for (int z = 0; z < LBO_BS; z++)
+    for (int y = 0; y < LBO_BS; y++)
+      for (int x = 0; x < LBO_BS; x++)
+        for (int q = 0; q < LBO_Q; q++) {
+          const int t[3] = {x + LBO_C[q][0], y + LBO_C[q][1], z + LBO_C[q][2]};
+          int out = 0;
+          for (int d = 0; d < 3; d++) out = out || t[d] < 0 || t[d] > LBO_BS - 1;
+          if (!out) continue;
+          const int c = x + LBO_BS * (y + LBO_BS * z);
+          if (n < max) {
+            incoming[n] = LBO_Q * c + LBO_Q - 1 - q;
+            outgoing[n] = LBO_Q * c + q;
+          }
+          n++;
+        }
+  return n;
+}
+
+/* BlockBoundaryIndex (BlockBoundaryIndex.cpp:15-88): the cells with a coordinate below 2 or above
+ * 3, in increasing block index.  Returns the number of cells (208). */
+int lbo_block_boundary_cells(int* cells, int max) {
+  int n = 0;
+  for (int z = 0; z < LBO_BS; z++)
+    for (int y = 0; y < LBO_BS; y++)
+      for (int x = 0; x < LBO_BS; x++) {
+        if (!(x < 2 || x > LBO_BS - 3 || y < 2 || y > LBO_BS - 3 || z < 2 || z > LBO_BS - 3)) continue;
+        if (n < max) cells[n] = x + LBO_BS * (y + LBO_BS * z);
+        n++;
+      }
+  return n;
+}
+
+/* RegularGridBlockVertex::initPdfDiff (BLB/RegularGridBlockVertex.cpp:96-105): after the collision
+ * and the boundary conditions, before the local streaming, diff_i = -pdf[outgoing_i]. */
+void lbo_block_init_pdf_diff(const double* pdf, double* diff) {
+  static int in[1024], out[1024];
+  const int n = lbo_block_pdf_boundary(in, out, 1024);
+  for (int i = 0; i < n; i++) diff[i] = -pdf[out[i]];
+}
+
+/* computePdfDiff (:108-120): after the in-cell swap, diff_i += pdf[incoming_i].  Where no local
+ * Peano cell swapped the link, the in-cell swap has put the outgoing value back into the incoming
+ * slot and the difference is zero. */
+void lbo_block_compute_pdf_diff(const double* pdf, double* diff) {
+  static int in[1024], out[1024];
+  const int n = lbo_block_pdf_boundary(in, out, 1024);
+  for (int i = 0; i < n; i++) diff[i] = diff[i] + pdf[in[i]];
+}
+
+/* mergePdfDiff (:123-141): with the difference a neighbour rank computed for its copy of the same
+ * vertex, pdf[incoming_i] += neighbour_i wherever the own difference is exactly 0.0. */
+void lbo_block_merge_pdf_diff(double* pdf, const double* own, const double* neighbour) {
+  static int in[1024], out[1024];
+  const int n = lbo_block_pdf_boundary(in, out, 1024);
+  for (int i = 0; i < n; i++)
+    if (own[i] == 0.0) pdf[in[i]] = pdf[in[i]] + neighbour[i];
+}
+
+/* setDensityOnBlockBoundary (:56-77): with a dynamic geometry, the density of each block-boundary
+ * cell that is not inner but on the geometry boundary, 0.0 for the others; without one the buffer
+ * keeps its values (resetDensityOnBlockBoundary, :51-53, zeroes it before). */
+void lbo_block_set_density_on_boundary(const double* density, const uint8_t* inner, const uint8_t* boundary,
+                                       int dynamic_geometry, double* out) {
+  if (!dynamic_geometry) return;
+  static int cells[256];
+  const int n = lbo_block_boundary_cells(cells, 256);
+  for (int i = 0; i < n; i++) {
+    const int c = cells[i];
+    out[i] = (!inner[c] && boundary[c]) ? density[c] : 0.0;
+  }
+}
+
+/* mergeDensityOnBoundary (:144-160): density[c] += neighbour_i for each block-boundary cell that
+ * is not inner but on the geometry boundary. */
+void lbo_block_merge_density_on_boundary(double* density, const uint8_t* inner, const uint8_t* boundary,
+                                         const double* neighbour) {
+  static int cells[256];
+  const int n = lbo_block_boundary_cells(cells, 256);
+  for (int i = 0; i < n; i++) {
+    const int c = cells[i];
+    if (!inner[c] && boundary[c]) density[c] += neighbour[i];
+  }
+}

@@ -172,6 +172,19 @@ void lbo_block_swap_inside(double* pdf);
 int lbo_block_nonlocal_index(int* out, int max);
 void lbo_block_stream_nonlocal(const uint8_t* const* inner, double* const* pdf);
 
+/* The vertex-overlap exchange between ranks (BLB/RegularGridBlockVertex.cpp:51-160,
+ * BLB/parallel/BlockBoundaryIndex4Pdfs.cpp, BLB/BlockBoundaryIndex.cpp): 1,008 boundary
+ * populations and 208 block-boundary cells per block. */
+int lbo_block_pdf_boundary(int* incoming, int* outgoing, int max);
+int lbo_block_boundary_cells(int* cells, int max);
+void lbo_block_init_pdf_diff(const double* pdf, double* diff);
+void lbo_block_compute_pdf_diff(const double* pdf, double* diff);
+void lbo_block_merge_pdf_diff(double* pdf, const double* own, const double* neighbour);
+void lbo_block_set_density_on_boundary(const double* density, const uint8_t* inner, const uint8_t* boundary,
+                                       int dynamic_geometry, double* out);
+void lbo_block_merge_density_on_boundary(double* density, const uint8_t* inner, const uint8_t* boundary,
+                                         const double* neighbour);
+
 #ifdef __cplusplus
 }
 #endif

@@ -135,3 +135,62 @@ def iterate(grid, ops, state, tau, links, steps=1):
             ops.stream_nonlocal(cell, state)
         for b in range(grid.nblocks):
             ops.swap_inside(b, state)
+
+
+def partition_cells(grid, ranks, overlap=False):
+    """The Peano cells of each rank: 2 ranks split them into a lower and an upper half in z (with
+    overlap, the middle layer of cells belongs to both); 8 ranks colour them by the parity of their
+    coordinates (a vertex shared by up to 8 ranks).  Returns one array of cell rows per rank."""
+    bx, by, bz = grid.nb
+    k = np.arange(len(grid.peano_cells))
+    cx, cy, cz = k % (bx - 1), (k // (bx - 1)) % (by - 1), k // ((bx - 1) * (by - 1))
+    if ranks == 1:
+        return [grid.peano_cells]
+    if ranks == 2:
+        mid = (bz - 1) // 2
+        lo = cz <= mid if overlap else 2 * cz < bz - 1
+        hi = cz >= mid if overlap else ~lo
+        return [grid.peano_cells[lo], grid.peano_cells[hi]]
+    if ranks == 8 and not overlap:
+        owner = (cx & 1) + 2 * (cy & 1) + 4 * (cz & 1)
+        return [grid.peano_cells[owner == r] for r in range(8)]
+    raise ValueError("ranks must be 1, 2 or 8 (overlap with 2 only)")
+
+
+def iterate_partitioned(grid, ops, states, cells_of, tau, links, steps=1):
+    """The reference's Parallel iteration (RegularGrid2RegularBlockSolver.cpp:83-248 with the
+    vertex exchange of :158-159, :243-245 and mergeWithNeighbour :349-364): rank r traverses the
+    Peano cells cells_of[r] and keeps its own copy states[r] of every block those cells touch.  Per
+    step each rank runs touchVertexFirstTime on its blocks with ops[r].init_pdf_diff(b, st) before
+    the local streaming, handleCell on its cells, touchVertexLastTime with
+    ops[r].compute_pdf_diff(b, st) after the in-cell swap; then every rank merges the difference of
+    every other rank's copy into each block they share (ops[r].merge_pdf_diff(b, st, neighbour)).
+    Returns the blocks of each rank."""
+    ranks = len(states)
+    blocks_of = [np.unique(np.asarray(c).reshape(-1)) for c in cells_of]
+    by_block = {}
+    for lk in links:
+        by_block.setdefault(lk[0], []).append(lk[1:])
+    for _ in range(steps):
+        for r in range(ranks):
+            st, op = states[r], ops[r]
+            for b in blocks_of[r]:
+                op.cell_values(b, st)
+                op.collide(b, st, tau)
+                if b in by_block:
+                    op.boundary(b, st, by_block[b])
+                op.init_pdf_diff(b, st)
+                op.stream_local(b, st)
+            for cell in cells_of[r]:
+                op.stream_nonlocal(cell, st)
+            for b in blocks_of[r]:
+                op.swap_inside(b, st)
+                op.compute_pdf_diff(b, st)
+        diffs = [{int(b): ops[r].pdf_diff(b) for b in blocks_of[r]} for r in range(ranks)]
+        for r in range(ranks):
+            for o in range(ranks):
+                if o == r:
+                    continue
+                for b in np.intersect1d(blocks_of[r], blocks_of[o]):
+                    ops[r].merge_pdf_diff(b, states[r], diffs[o][int(b)])
+    return blocks_of
