@@ -315,6 +315,64 @@ def run_block_solver(lbm, sizes=(64, 256), steps=20, tau=0.6, ulid=0.05):
     return out
 
 
+def run_vertex_exchange(lbm, n=256, reps=20):
+    """The per-GPU kernels of the vertex exchange between ranks (the reference's Parallel build) at
+    their real shape: the 256^3 cavity's Peano cells cut into two z halves share one plane of
+    44 x 44 = 1,936 vertices.  Per iteration a GPU records the pdf differences of those vertices
+    before the streaming (init: 1,008 populations read, 1,008 differences written per vertex) and
+    after it (compute: 1,008 populations and differences read, 1,008 differences written), copies
+    them out for the neighbour, and merges the neighbour's (1,008 own and 1,008 neighbour
+    differences read, and here half the incoming populations read and written: the other half
+    were swapped locally)."""
+    nb = n // 6 + 1 + (1 if n % 6 else 0)
+    nv = nb * nb
+    rng = np.random.default_rng(7)
+    w = np.array([1 / 36, 1 / 36, 1 / 18, 1 / 36, 1 / 36, 1 / 36, 1 / 18, 1 / 36, 1 / 18, 1 / 3,
+                  1 / 18, 1 / 36, 1 / 18, 1 / 36, 1 / 36, 1 / 36, 1 / 18, 1 / 36, 1 / 36])
+    pdf = np.tile(w, 216) * (1.0 + 0.01 * rng.standard_normal((nv, 216 * 19)))
+    # incoming / outgoing slots of the populations leaving a block (cells in z, y, x order, q in order)
+    c = np.array([[0, -1, -1], [-1, 0, -1], [0, 0, -1], [1, 0, -1], [0, 1, -1], [-1, -1, 0], [0, -1, 0],
+                  [1, -1, 0], [-1, 0, 0], [0, 0, 0], [1, 0, 0], [-1, 1, 0], [0, 1, 0], [1, 1, 0],
+                  [0, -1, 1], [-1, 0, 1], [0, 0, 1], [1, 0, 1], [0, 1, 1]])
+    cell = np.repeat(np.arange(216), 19)
+    q = np.tile(np.arange(19), 216)
+    t = np.stack([cell % 6, (cell // 6) % 6, cell // 36], axis=1) + c[q]
+    leave = ~((t >= 0) & (t < 6)).all(axis=1)
+    out_slot, in_slot = (cell * 19 + q)[leave], (cell * 19 + 18 - q)[leave]
+    pdf[:, in_slot[::2]] = pdf[:, out_slot[::2]]
+    neighbour = 1e-3 * rng.standard_normal((nv, 1008))
+    ids = np.arange(nv)
+    res = {"init": [], "compute": [], "merge": [], "get": []}
+    with lbm.BlockStore(nv) as bs:
+        for rep in range(reps + 2):
+            bs.set(0, pdf=pdf)
+            bs.init_pdf_diff(ids)
+            t_init = bs.last_ms()
+            bs.compute_pdf_diff(ids)
+            t_comp = bs.last_ms()
+            t0 = time.perf_counter()
+            bs.get_pdf_diff(ids)
+            t_get = (time.perf_counter() - t0) * 1e3
+            bs.merge_pdf_diff(ids, neighbour)
+            t_merge = bs.last_ms()
+            if rep >= 2:
+                for k, v in (("init", t_init), ("compute", t_comp), ("merge", t_merge), ("get", t_get)):
+                    res[k].append(v)
+    per = {"init": 2 * 1008 * 8, "compute": 3 * 1008 * 8, "merge": 2 * 1008 * 8 + 2 * 504 * 8}
+    out = {"workload": f"{n}^3 cavity cut into two z halves: {nv} shared 6^3 vertices, FP64", "shared_vertices": nv,
+           "differences_bytes": nv * 1008 * 8}
+    for k in ("init", "compute", "merge"):
+        ms = float(np.mean(res[k]))
+        gbs = nv * per[k] / (ms * 1e-3) / 1e9
+        out[k] = {"ms": round(ms, 4), "algorithmic_bytes_per_vertex": per[k], "GBs": round(gbs, 1)}
+    ms = float(np.mean(res["get"]))
+    out["get_to_host"] = {"ms": round(ms, 3), "GBs": round(nv * 1008 * 8 / (ms * 1e-3) / 1e9, 1),
+                          "what": "device gather + PCIe copy to pageable host memory, wall clock"}
+    log(f"[bench] vertex exchange, {nv} shared vertices: init {out['init']['ms']:.3f} ms, compute "
+        f"{out['compute']['ms']:.3f} ms, merge {out['merge']['ms']:.3f} ms, differences to host {ms:.2f} ms")
+    return out
+
+
 def cpu_baseline(wl, n=128, seed=None, tau=0.6, g=1e-6, budget_s=12.0):
     import oracle
 
@@ -377,7 +435,9 @@ def main():
 
     block_sizes = tuple(int(v) for v in args.block_solver_sizes.split(",") if v)
     if args.only_block_solver:
-        print(json.dumps(run_block_solver(lbm, sizes=block_sizes, steps=args.block_steps)), flush=True)
+        out = run_block_solver(lbm, sizes=block_sizes, steps=args.block_steps)
+        out["vertex_exchange"] = run_vertex_exchange(lbm)
+        print(json.dumps(out), flush=True)
         return
 
     n = args.size
@@ -420,6 +480,7 @@ def main():
     log(f"[bench] block store {amr['workload']}: " + ", ".join(
         f"{k} {amr[k]['ms']:.3f} ms ({amr[k]['GBs']:.0f} GB/s)" for k in ("find_overlap", "prolongate", "restrict")))
     blk = run_block_solver(lbm, sizes=block_sizes, steps=args.block_steps)
+    vex = run_vertex_exchange(lbm)
     adj = None
     if not args.no_adjoint:
         adj = run_adjoint(lbm, wl)
@@ -466,6 +527,7 @@ def main():
         "adjoint": adj,
         "restriction_prolongation": amr,
         "block_solver": blk,
+        "vertex_exchange": vex,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
