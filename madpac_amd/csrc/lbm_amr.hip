@@ -283,7 +283,7 @@ int lbm_blocks_destroy(lbm_blockstore* bs) {
   if (bs->stream) (void)hipStreamSynchronize(bs->stream);
   void* ptrs[] = {bs->pdf, bs->density, bs->velocity, bs->mask, bs->inner, bs->d_links, bs->d_list, bs->d_cells,
                   bs->d_bnd, bs->d_off, bs->d_nbr, bs->d_valid, bs->d_xch, bs->d_pdfdiff, bs->d_bdens,
-                  bs->boundary, bs->d_xfer};
+                  bs->boundary, bs->d_xfer, bs->d_vpos};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (bs->ev0) (void)hipEventDestroy(bs->ev0);

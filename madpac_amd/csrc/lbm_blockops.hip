@@ -471,20 +471,64 @@ __global__ __launch_bounds__(512) void k_block_nonlocal(const int64_t* __restric
   *y = v;
 }
 
-// the swap mask of lbm_blocks_iterate: bit e of block b is set when crossing entry e is swapped by
-// one of the listed Peano cells (both of its cells inner); one thread per (cell, table entry)
-__global__ __launch_bounds__(512) void k_block_valid(const int64_t* __restrict__ cells, int64_t n,
+// Which Peano cell swaps each crossing entry: every entry belongs to exactly one entry
+// (i, j, c1, c2, q) of a Peano cell's cross-block table, from block i's side as (c1, q) or from
+// block j's side as (c2, 18 - q).  owner[e] is the vertex position the block has in that cell, and
+// other[e] the partner's cell (c2 or c1), both fixed by the table.
+struct EntryOwner {
+  uint8_t owner[NX_ENTRIES];
+  uint8_t other[NX_ENTRIES];
+  bool ok;
+};
+constexpr EntryOwner make_entry_owner() {
+  EntryOwner t{};
+  const NonLocalTable nl = make_nonlocal_table();
+  const CrossTable x = make_cross_table();
+  bool seen[NX_ENTRIES] = {};
+  t.ok = true;
+  for (int k = 0; k < NL_ENTRIES; k++) {
+    const uint32_t E = nl.e[k];
+    const int i = E & 7, j = (E >> 3) & 7, c1 = (E >> 6) & 255, c2 = (E >> 14) & 255, q = (E >> 22) & 31;
+    const int e1 = x.entry_of[c1 * Q + q], e2 = x.entry_of[c2 * Q + (Q - 1 - q)];
+    if (e1 < 0 || e2 < 0 || seen[e1] || seen[e2]) t.ok = false;
+    if (e1 >= 0 && e2 >= 0) {
+      seen[e1] = seen[e2] = true;
+      t.owner[e1] = (uint8_t)i;
+      t.other[e1] = (uint8_t)c2;
+      t.owner[e2] = (uint8_t)j;
+      t.other[e2] = (uint8_t)c1;
+    }
+  }
+  for (int e = 0; e < NX_ENTRIES; e++) t.ok = t.ok && seen[e];
+  return t;
+}
+static_assert(make_entry_owner().ok, "every crossing entry belongs to exactly one cross-block link of a Peano cell");
+__constant__ EntryOwner kOwner = make_entry_owner();
+
+// The swap mask of lbm_blocks_iterate: bit e of block b is set when crossing entry e is swapped by a
+// listed Peano cell, i.e. b sits at vertex position owner[e] of a listed cell (bit owner[e] of
+// vpos[b]) and both cells of the link are inner.  One workgroup per block writes all 32 words of
+// its mask (blocks in no listed cell get zeros), so no clearing pass and no atomics are needed.
+__global__ __launch_bounds__(256) void k_block_valid(int64_t nblocks, const uint8_t* __restrict__ vpos,
+                                                     const int64_t* __restrict__ nbr,
                                                      const uint8_t* __restrict__ inner, uint32_t* __restrict__ valid) {
-  const int64_t p = blockIdx.x;
-  const int e = threadIdx.x;
-  if (p >= n || e >= NL_ENTRIES) return;
-  const uint32_t E = kNonLocal.e[e];
-  const int i = E & 7u, j = (E >> 3) & 7u, c1 = (E >> 6) & 255u, c2 = (E >> 14) & 255u, q = (E >> 22) & 31u;
-  const int64_t bi = cells[8 * p + i], bj = cells[8 * p + j];
-  if (!inner[bi * BC + c1] || !inner[bj * BC + c2]) return;
-  const int e1 = kCross.entry_of[c1 * Q + q], e2 = kCross.entry_of[c2 * Q + (Q - 1 - q)];
-  atomicOr(&valid[bi * 32 + (e1 >> 5)], 1u << (e1 & 31));
-  atomicOr(&valid[bj * 32 + (e2 >> 5)], 1u << (e2 & 31));
+  const int64_t b = blockIdx.x;
+  if (b >= nblocks) return;
+  const int t = threadIdx.x;
+  const uint32_t vp = vpos[b];
+#pragma unroll
+  for (int k = 0; k < (NX_ENTRIES + 255) / 256; k++) {
+    const int e = t + 256 * k;
+    bool on = false;
+    if (e < NX_ENTRIES && ((vp >> kOwner.owner[e]) & 1u)) {
+      const uint32_t E = kCross.e[e];
+      const int64_t nb = nbr[b * NDIR + ((E >> 13) & 31u)];
+      on = nb >= 0 && inner[b * BC + (E & 255u)] && inner[nb * BC + kOwner.other[e]];
+    }
+    const unsigned long long m = __ballot(on);
+    // lanes 0-31 of a wavefront hold word e >> 5 for even (t >> 5), lanes 32-63 the next word
+    if ((t & 31) == 0) valid[b * 32 + (e >> 5)] = (uint32_t)(m >> (t & 32));
+  }
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -954,8 +998,10 @@ int lbm_blocks_iterate(lbm_blockstore* bs, const int64_t* blocks, int64_t nblock
     bs->iter_inner_gen = ~0ull;
     constexpr DirMap dm = make_dir_map();
     std::vector<int64_t> nbr((size_t)bs->nblocks * NDIR, -1);
+    std::vector<uint8_t> vpos((size_t)bs->nblocks, 0);  // bit i: the block is vertex i of a listed cell
     for (int64_t p = 0; p < ncells; p++) {
       const int64_t* v = cells + 8 * p;
+      for (int i = 0; i < 8; i++) vpos[(size_t)v[i]] |= (uint8_t)(1u << i);
       for (int i = 0; i < 8; i++)
         for (int j = 0; j < 8; j++) {
           const int o = ((j & 1) - (i & 1) + 1) + 3 * (((j >> 1) & 1) - ((i >> 1) & 1) + 1) + 9 * ((j >> 2) - (i >> 2) + 1);
@@ -968,14 +1014,13 @@ int lbm_blocks_iterate(lbm_blockstore* bs, const int64_t* blocks, int64_t nblock
     }
     if (int rc = ensure_capacity(bs, &bs->d_nbr, &bs->cap_nbr, bs->nblocks * NDIR, "neighbour table")) return rc;
     if (int rc = ensure_capacity(bs, &bs->d_valid, &bs->cap_valid, bs->nblocks * 32, "swap mask")) return rc;
+    if (int rc = ensure_capacity(bs, &bs->d_vpos, &bs->cap_vpos, bs->nblocks, "vertex positions")) return rc;
     BCHK(bs, hipMemcpyAsync(bs->d_nbr, nbr.data(), nbr.size() * sizeof(int64_t), hipMemcpyHostToDevice, bs->stream));
-    BCHK(bs, hipMemsetAsync(bs->d_valid, 0, (size_t)bs->nblocks * 32 * sizeof(uint32_t), bs->stream));
-    if (ncells) {
-      hipLaunchKernelGGL(k_block_valid, dim3((unsigned)ncells), dim3(512), 0, bs->stream, bs->d_cells, ncells, bs->inner,
-                         bs->d_valid);
-      BCHK(bs, hipGetLastError());
-    }
-    BCHK(bs, hipStreamSynchronize(bs->stream));  // nbr is host memory of this call
+    BCHK(bs, hipMemcpyAsync(bs->d_vpos, vpos.data(), vpos.size(), hipMemcpyHostToDevice, bs->stream));
+    hipLaunchKernelGGL(k_block_valid, dim3((unsigned)bs->nblocks), dim3(256), 0, bs->stream, bs->nblocks, bs->d_vpos,
+                       bs->d_nbr, bs->inner, bs->d_valid);
+    BCHK(bs, hipGetLastError());
+    BCHK(bs, hipStreamSynchronize(bs->stream));  // nbr and vpos are host memory of this call
     bs->iter_cells.assign(cells, cells + 8 * ncells);
     bs->iter_inner_gen = bs->inner_gen;
   }

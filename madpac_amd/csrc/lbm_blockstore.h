@@ -101,6 +101,8 @@ struct lbm_blockstore {
   int64_t cap_nbr = 0;
   uint32_t* d_valid = nullptr; // lbm_blocks_iterate: 32 swap-mask words per block
   int64_t cap_valid = 0;
+  uint8_t* d_vpos = nullptr;   // lbm_blocks_iterate: per block, the vertex positions it has in listed cells
+  int64_t cap_vpos = 0;
   double* d_xch = nullptr;     // lbm_blocks_iterate: two exchange arrays of 1,008 values per block
   int64_t cap_xch = 0;
   double* d_pdfdiff = nullptr; // exchange between ranks: 1,008 differences per block (allocated on first use)
