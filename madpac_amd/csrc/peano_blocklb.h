@@ -18,6 +18,8 @@
 //        collisionmodels/BlockBGK, boundaryconditions/BlockBoundaryConditionManager,
 //        nonlocaltreatment/AABlockStreaming (batched over vertex lists, device block store)
 //        -> DeviceBlocks, BlockComputeCellValues, BlockBGK, BlockBoundaryConditionManager, AABlockStreaming
+//   BLB/RegularGridBlockVertex's exchange of a vertex's copies between ranks (Parallel build:
+//        init/compute/mergePdfDiff, set/reset/mergeDensityOnBoundary)                -> BlockVertexExchange
 //   LB/PerformanceMeasurement (MLUPS per iteration; wall clock here, not CPU time)
 // The three sweeps of one reference iteration (per-block collide + local streaming, the
 // cross-block streaming of handleCell, the in-cell swap) are one fused device step here.  A mapping
@@ -468,6 +470,8 @@ class BlockVertexExchange {
   static constexpr int CellsOnBlockBoundary = 208;
   // the geometry-boundary bitset of one vertex (GridManagementService::getBoundary)
   void setBoundary(DeviceBlocks& s, int64_t vertex, const std::vector<uint8_t>& boundary) const {
+    if (boundary.size() != (size_t)LB_BLOCK_NUMBER_OF_CELLS)
+      throw std::runtime_error("setBoundary: one flag per cell of the block (216) expected");
     s.check(lbm_blocks_set_boundary(s.handle(), vertex, 1, boundary.data()), "lbm_blocks_set_boundary");
   }
   void initPdfDiff(DeviceBlocks& s, const std::vector<int64_t>& vertices) const {
