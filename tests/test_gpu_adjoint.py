@@ -110,7 +110,10 @@ def test_fp32_adjoint_tracks_fp64():
     g64, g32 = out[lbm.FP64][1], out[lbm.FP32_SHIFTED][1]
     rel = np.linalg.norm(g32 - g64) / np.linalg.norm(g64)
     print(f"[gpu] FP32 adjoint vs FP64: relative L2 {rel:.3e}", flush=True)
-    assert rel <= 1e-2
+    # Bar: the FP32 run stores populations and the adjoint in FP32 (unit round-off 6e-8) over 40
+    # forward and 40 reverse steps; it measures 9.3e-8 here, and the bar leaves 10x for other seeds
+    # and schedules
+    assert rel <= 1e-6
 
 
 def test_adjoint_gradient_is_the_same_for_every_workgroup_shape():
