@@ -18,7 +18,12 @@
  * collision, boundary links, local / cross-block / in-cell swap streaming and their index tables)
  * are pinned bit-for-bit against fixtures produced by
  * the compiled reference (tests/golden/), the overlap map of the latter also against a
- * brute-force run of the reference's own three-step streaming procedure; Guo forcing, Brinkman porosity, periodic wrap, the pressure-drop
+ * brute-force run of the reference's own three-step streaming procedure.  The vertex exchange
+ * between ranks (compiled by the reference only in its MPI build) has its two index tables pinned
+ * to the compiled reference's tables (tests/golden/parallel_index.npz) and the difference / merge
+ * rules pinned by the partition invariant: Peano cells split among ranks, each rank's copies of the
+ * blocks, differences exchanged, every copy equal to the single-domain run bit for bit, with
+ * planted mistakes in the rule failing it (tests/test_oracle_parallel.py).  Guo forcing, Brinkman porosity, periodic wrap, the pressure-drop
  * objective and the adjoint step go beyond the reference and are pinned by analytic
  * solutions (Poiseuille flow, Taylor-Green decay, the uniform-porosity fixed point
  * u + g/2 = g tau_eff / alpha, the objective of a linear density ramp), conservation
