@@ -280,6 +280,36 @@ def test_iterate_with_a_partial_cell_list_equals_the_separate_operators():
             assert np.array_equal(x, y)
 
 
+@pytest.mark.parametrize("mode", ["vertex_first", "iterate"])
+def test_fused_passes_take_links_on_every_direction(mode):
+    """Boundary links of every type on every direction, the rest population's (q_in = 9) included, on
+    seeded inner flags: the fused passes (links applied to the block in LDS, with the cell values the
+    pass computed) equal the separate operators (links applied in global memory) bit for bit."""
+    grid = BlockGrid(12, 12, 12)
+    rng = np.random.default_rng(41)
+    pdf0 = initial_state(grid)[0] * (1.0 + 0.01 * rng.standard_normal((grid.nblocks, BP)))
+    inner = grid.inner.copy()
+    inner[rng.random(inner.shape) < 0.1] = 0
+    links, seen = [], set()
+    for _ in range(4000):
+        b, c, qi = int(rng.integers(grid.nblocks)), int(rng.integers(NC)), int(rng.integers(Q))
+        if (b, c, qi) in seen:
+            continue
+        seen.add((b, c, qi))
+        links.append((b, c, qi, int(rng.integers(6)), tuple(0.03 * rng.standard_normal(3)), 1.0 + 0.02 * rng.random()))
+    assert sum(lk[2] == Q // 2 for lk in links) > 100
+    tau = O.tau_eff(0.6)
+    print(f"[gpu] {mode} with {len(links)} links on every direction", flush=True)
+    out = []
+    for m in ("separate", mode):
+        with _store(pdf0, np.ones((grid.nblocks, NC)), None, inner) as bs:
+            _regular(grid, bs, tau, links, 3, m)
+            out.append(bs.get())
+    assert np.isfinite(out[0][0]).all()
+    for x, y in zip(*out):
+        assert np.array_equal(x, y)
+
+
 def test_iterate_rejects_inconsistent_cells():
     grid = BlockGrid(8, 8, 8)
     blocks = np.arange(grid.nblocks)
