@@ -197,6 +197,34 @@ def test_exchange_argument_errors():
         assert np.array_equal(bs.get_pdf_diff([0, 1, 2]), np.zeros((3, PB)))
 
 
+def test_exchange_with_empty_lists_changes_nothing():
+    """A rank that shares no vertex with another calls the exchange with empty lists (the reference's
+    loops over an empty vertex set): every call succeeds and leaves the store as it was, and a null
+    buffer is accepted when nothing is copied."""
+    rng = np.random.default_rng(35)
+    pdf = 0.05 + 0.01 * rng.random((2, BP))
+    rho = 1.0 + 0.01 * rng.random((2, NC))
+    print("[gpu] exchange calls with empty lists", flush=True)
+    with _store(pdf, rho=rho) as bs:
+        bs.init_pdf_diff([0, 1])
+        before = bs.get_pdf_diff([0, 1])
+        none = np.zeros(0, dtype=np.int64)
+        bs.init_pdf_diff(none)
+        bs.compute_pdf_diff(none)
+        bs.merge_pdf_diff(none, np.zeros(0))
+        bs.reset_density_on_boundary(none)
+        bs.set_density_on_boundary(none)
+        bs.merge_density_on_boundary(none, np.zeros(0))
+        assert bs.get_pdf_diff(none).shape == (0, PB) and bs.get_density_on_boundary(none).shape == (0, BB)
+        p = none.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))
+        for fn in ("lbm_blocks_get_pdf_diff", "lbm_blocks_merge_pdf_diff", "lbm_blocks_get_density_on_boundary",
+                   "lbm_blocks_merge_density_on_boundary"):
+            assert getattr(bs._L, fn)(bs._h, p, 0, None) == 0, bs._L.lbm_blocks_last_error(bs._h).decode()
+        f, r, _ = bs.get()
+        assert np.array_equal(f, pdf) and np.array_equal(r, rho)
+        assert np.array_equal(bs.get_pdf_diff([0, 1]), before)
+
+
 BLOCKOPS_EXE = os.path.join(ROOT, "tests", "cpp", "test_peano_blockops")
 
 
