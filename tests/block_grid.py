@@ -137,13 +137,18 @@ def iterate(grid, ops, state, tau, links, steps=1):
             ops.swap_inside(b, state)
 
 
-def partition_cells(grid, ranks, overlap=False):
+def partition_cells(grid, ranks, overlap=False, seed=None):
     """The Peano cells of each rank: 2 ranks split them into a lower and an upper half in z (with
     overlap, the middle layer of cells belongs to both); 8 ranks colour them by the parity of their
-    coordinates (a vertex shared by up to 8 ranks).  Returns one array of cell rows per rank."""
+    coordinates (a vertex shared by up to 8 ranks); with a seed, each cell goes to a random rank (a
+    ragged partition of any rank count, as a space-filling-curve split can leave).  Returns one
+    array of cell rows per rank."""
     bx, by, bz = grid.nb
     k = np.arange(len(grid.peano_cells))
     cx, cy, cz = k % (bx - 1), (k // (bx - 1)) % (by - 1), k // ((bx - 1) * (by - 1))
+    if seed is not None and not overlap:
+        owner = np.random.default_rng(seed).integers(ranks, size=len(k))
+        return [grid.peano_cells[owner == r] for r in range(ranks)]
     if ranks == 1:
         return [grid.peano_cells]
     if ranks == 2:
@@ -154,7 +159,7 @@ def partition_cells(grid, ranks, overlap=False):
     if ranks == 8 and not overlap:
         owner = (cx & 1) + 2 * (cy & 1) + 4 * (cz & 1)
         return [grid.peano_cells[owner == r] for r in range(8)]
-    raise ValueError("ranks must be 1, 2 or 8 (overlap with 2 only)")
+    raise ValueError("ranks must be 1, 2 or 8 without a seed (overlap with 2 only)")
 
 
 def iterate_partitioned(grid, ops, states, cells_of, tau, links, steps=1):

@@ -135,9 +135,10 @@ def _single_domain(grid, pdf0, tau, links, steps):
         return bs.get()
 
 
-@pytest.mark.parametrize("n,ranks,overlap,steps", [(12, 2, False, 6), (12, 8, False, 4), (18, 2, True, 4),
-                                                  (25, 8, False, 3)])
-def test_partitioned_block_solver_equals_the_single_domain(n, ranks, overlap, steps):
+@pytest.mark.parametrize("n,ranks,overlap,steps,seed", [(12, 2, False, 6, None), (12, 8, False, 4, None),
+                                                       (18, 2, True, 4, None), (25, 8, False, 3, None),
+                                                       (19, 3, False, 3, 17)])
+def test_partitioned_block_solver_equals_the_single_domain(n, ranks, overlap, steps, seed):
     """One store per rank, each running the block operators on the blocks of its own Peano cells with
     the pdf differences recorded around the streaming, then every rank merging every other rank's
     differences into the blocks they share: every copy of every block equals the single-domain run."""
@@ -145,9 +146,9 @@ def test_partitioned_block_solver_equals_the_single_domain(n, ranks, overlap, st
     tau = O.tau_eff(0.6)
     links = grid.cavity_links(0.06)
     pdf0 = perturbed_state(grid)[0]
-    print(f"[gpu] {ranks} ranks on the {n}^3 cavity (overlap {overlap}), {steps} steps", flush=True)
+    print(f"[gpu] {ranks} ranks on the {n}^3 cavity (overlap {overlap}, seed {seed}), {steps} steps", flush=True)
     ref = _single_domain(grid, pdf0, tau, links, steps)
-    cells_of = partition_cells(grid, ranks, overlap)
+    cells_of = partition_cells(grid, ranks, overlap, seed)
     blocks_of = [np.unique(c.reshape(-1)) for c in cells_of]
     links_of = [[lk for lk in links if lk[0] in set(b.tolist())] for b in blocks_of]
     stores = [_store(pdf0) for _ in range(ranks)]

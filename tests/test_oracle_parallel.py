@@ -103,25 +103,27 @@ def perturbed_state(grid, seed=3):
     return pdf, rho, u
 
 
-def _run(n, ranks, steps, merge=None, overlap=False, tau=0.6, ulid=0.05):
+def _run(n, ranks, steps, merge=None, overlap=False, tau=0.6, ulid=0.05, seed=None):
     grid = BlockGrid(n, n, n)
     links = grid.cavity_links(ulid)
     mono = perturbed_state(grid)
     iterate(grid, OracleOps(grid), mono, O.tau_eff(tau), links, steps)
-    cells_of = partition_cells(grid, ranks, overlap)
+    cells_of = partition_cells(grid, ranks, overlap, seed)
     states = [perturbed_state(grid) for _ in range(ranks)]
     ops = [OracleExchangeOps(grid, merge) for _ in range(ranks)]
     blocks_of = iterate_partitioned(grid, ops, states, cells_of, O.tau_eff(tau), links, steps)
     return grid, mono, states, blocks_of
 
 
-@pytest.mark.parametrize("n,ranks,steps,overlap", [(12, 2, 4, False), (12, 8, 3, False), (13, 2, 3, False),
-                                                  (18, 2, 3, True)])
-def test_partitioned_iterations_equal_the_single_domain(n, ranks, steps, overlap):
+@pytest.mark.parametrize("n,ranks,steps,overlap,seed", [(12, 2, 4, False, None), (12, 8, 3, False, None),
+                                                       (13, 2, 3, False, None), (18, 2, 3, True, None),
+                                                       (12, 3, 3, False, 7), (13, 5, 2, False, 8)])
+def test_partitioned_iterations_equal_the_single_domain(n, ranks, steps, overlap, seed):
     """With overlap the middle layer of Peano cells is traversed by both ranks: their copies then
     both carry those swaps, and only the merge rule's guard (add where the own difference is 0.0)
-    keeps them from being applied twice."""
-    grid, mono, states, blocks_of = _run(n, ranks, steps, overlap=overlap)
+    keeps them from being applied twice.  With a seed the cells go to random ranks: ragged
+    boundaries, blocks shared by up to 8 ranks."""
+    grid, mono, states, blocks_of = _run(n, ranks, steps, overlap=overlap, seed=seed)
     shared = set()
     for r in range(ranks):
         b = blocks_of[r]
