@@ -110,6 +110,42 @@ def run_dense(lbm, solid, n, tau, prec, steps, warmup, g, layout=0, readout=Fals
     }
 
 
+def run_md_coupling(lbm, solid, n, tau, steps=100, box=40):
+    """The MD coupling hooks (PeanoDataExchangeService.cpp:69-218) on the porous channel's geometry
+    without its body force (coupled cells take the plain BGK collision, which the library enforces),
+    FP64: the fluid cells of a box^3 MD region in the middle of the lattice are coupled in Dirichlet mode (after
+    the step kernel, k_couple re-collides them with the MD velocity); the step with coupling against
+    the same step without it, both timed with HIP events on the context stream, and the export of the
+    region's density and velocity (k_couple_gather plus the copy to host memory, wall clock)."""
+    lo = (n - box) // 2
+    x, y, z = np.meshgrid(*(np.arange(lo, lo + box),) * 3, indexing="ij")
+    cells = (x + n * (y + n * z)).reshape(-1)
+    cells = np.sort(cells[~solid.reshape(-1)[cells]])
+    with lbm.Lattice(n, n, n, tau, precision=lbm.FP64, periodic=(True, True, True)) as lat:
+        lat.set_flags(~solid)
+        lat.init_equilibrium()
+        lat.tune(20)
+        lat.step(10)
+        plain = lat.step_timed(steps) / steps
+        lat.set_coupling_cells(cells)
+        u = np.tile([0.01, 0.0, 0.0], (len(cells), 1))
+        lat.set_coupling_data(lbm.COUPLE_DIRICHLET, u, dt=1.0)
+        lat.step(10)
+        coupled = lat.step_timed(steps) / steps
+        lat.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(10):
+            lat.get_coupling_data()
+        export = (time.perf_counter() - t0) / 10 * 1e3
+    return {"workload": f"porous channel geometry {n}^3 without body force, FP64, the fluid cells of a {box}^3 "
+                        f"MD region, Dirichlet mode",
+            "coupled_cells": int(len(cells)), "step_ms": round(plain, 4), "coupled_step_ms": round(coupled, 4),
+            "overhead_ms": round(coupled - plain, 4),
+            "overhead_ns_per_coupled_cell": round((coupled - plain) * 1e6 / len(cells), 2),
+            "export_ms": round(export, 4), "export_what": "density and velocity of the coupled cells to host "
+                                                         "memory, wall clock"}
+
+
 def run_taylor_green(lbm, wl, n=512, steps=200, tau=0.52, u0=0.04):
     """All-fluid periodic Taylor-Green vortex (the large-scale validation workload), timed like the
     porous channel: the same kernels without solid surfaces."""
@@ -479,6 +515,9 @@ def main():
     amr = run_block_store(lbm)
     log(f"[bench] block store {amr['workload']}: " + ", ".join(
         f"{k} {amr[k]['ms']:.3f} ms ({amr[k]['GBs']:.0f} GB/s)" for k in ("find_overlap", "prolongate", "restrict")))
+    md = run_md_coupling(lbm, solid, n, args.tau)
+    log(f"[bench] MD coupling, {md['coupled_cells']} cells: step {md['step_ms']:.3f} -> {md['coupled_step_ms']:.3f} ms, "
+        f"export {md['export_ms']:.3f} ms")
     blk = run_block_solver(lbm, sizes=block_sizes, steps=args.block_steps)
     vex = run_vertex_exchange(lbm)
     adj = None
@@ -518,6 +557,7 @@ def main():
         "wall_mlups": {"fp64": round(r64["wall_mlups"], 1), "fp32s": round(r32["wall_mlups"], 1)},
         "host_readout_fp64": r64["host_readout"],
         "batched_patches": patches,
+        "md_coupling": md,
         "all_fluid": tgv,
         "all_fluid_same_size": tgv_same,
         "copy_reference": {"fp64": r64["copy_reference"], "fp32s": r32["copy_reference"],
